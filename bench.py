@@ -1,0 +1,174 @@
+"""Benchmark of the plain-MPPI control step on one MI355X.
+
+Workload (default): 4 drones racing on a 12-gate lemniscate (half-width
+30 m), one controller each, K=16384 samples, T=100 steps, FP32, closed loop
+(each control step ends with every drone integrated with its action).  A
+"step" is one control step of the whole batch; `value` counts control steps
+of every controller (4 per batch step).
+
+Output: one JSON line (driver contract) with a roofline object for the
+dominant kernel (the fused rollout) and the oracle's OpenMP CPU run of the
+same workload as the baseline.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "control steps/s, batched+adjoint vs plain MPPI at equal mean trajectory cost"
+
+# FP32 flops per rollout sample-step (dynamics, gates and costs counted by the
+# oracle's op-counting scalar on the first control step of this workload, plus
+# 96 for the Philox Box-Muller noise).  tests/test_bench_constants.py keeps it
+# equal to oracle/count_ops.py.
+FLOPS_PER_SAMPLE_STEP = 275.0
+# HBM bytes per rollout sample-step the rollout kernel must move: the float4
+# noise row it writes (read back once by the weighted average).
+ROLLOUT_BYTES_PER_SAMPLE_STEP = 16.0
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def package():
+    name = "drone_race_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    d = os.path.join(ROOT, "drone-race_amd")
+    spec = importlib.util.spec_from_file_location(name, os.path.join(d, "__init__.py"), submodule_search_locations=[d])
+    m = importlib.util.module_from_spec(spec)
+    sys.modules[name] = m
+    spec.loader.exec_module(m)
+    return m
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(pkg, scn, budget_s):
+    """The oracle's FP32 restatement, OpenMP over samples, on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+
+    oracle = pyoracle.Oracle(pkg.abi)
+    threads = min(16, os.cpu_count() or 1)
+    drones = (pkg.abi.DroneState * len(scn.drone_world))(*scn.drones)
+    nominal = np.concatenate([np.tile(np.array(i.shift_fill[:], np.float32), (i.n_timesteps, 1)).reshape(-1)
+                              for i in scn.instances]).astype(np.float32)
+    steps = [0] * len(scn.instances)
+    lams = [i.lambda_ for i in scn.instances]
+    t0 = time.perf_counter()
+    oracle.world_run(scn, drones, nominal, steps, lams, 1, threads)
+    one = time.perf_counter() - t0
+    n = max(1, int(budget_s / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    oracle.world_run(scn, drones, nominal, steps, lams, n, threads)
+    dt = time.perf_counter() - t0
+    n_inst = len(scn.instances)
+    return {"value": n_inst * n / dt, "unit": "control steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} closed-loop batch steps ({n_inst} controllers each) of the same workload after 1 warm-up "
+                      f"step, oracle FP32 restatement, OpenMP {threads} threads, {dt:.1f} s",
+            "sample_steps_per_s": n * scn.sample_steps_per_step / dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2"])
+    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (each controller batch runs on a "
+              f"single MI355X; see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    pkg = package()
+    sc = pkg.scenarios
+    scn = {"c0": sc.c0_single, "c1": sc.c1_four_drones, "c2": sc.c2_batch64}[args.config]()
+    n_inst = len(scn.instances)
+    log(f"bench: workload {scn.name}: {scn.description}")
+    torch.cuda.set_device(0)
+    eng = pkg.Engine(scn)
+    tuned = None
+    if args.autotune:
+        tuned, ms = eng.autotune(trials=5)
+        log(f"bench: autotuned launch {tuned} ({ms:.3f} ms/step)")
+    launch = {f: getattr(eng.get_launch(), f) for f, _ in pkg.abi.LaunchCfg._fields_}
+
+    eng.step(args.warmup, advance=True)
+    log(f"bench: warm-up {args.warmup} steps done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.step_async(args.steps, advance=True)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    log(f"bench: timed {args.steps} steps in {elapsed:.3f} s")
+
+    # Per-kernel device time from HIP events recorded on the engine's stream
+    # between the kernels (stream launches), over the same number of steps.
+    n_timed = min(args.steps, 200)
+    eng.set_timing(True)
+    eng.step(n_timed, advance=True)
+    eng.set_timing(False)
+    kt = eng.last_timing()
+    rollout_s = kt["rollout_ms"] / n_timed * 1e-3
+    log(f"bench: per-step device time rollout {kt['rollout_ms'] / n_timed:.4f} ms, reduce {kt['reduce_ms'] / n_timed:.4f} ms, "
+        f"update+advance {kt['update_ms'] / n_timed:.4f} ms")
+    diag = eng.diag()
+    eng.close()
+
+    ss = scn.sample_steps_per_step
+    flops = FLOPS_PER_SAMPLE_STEP * ss
+    achieved = flops / rollout_s / 1e12
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{scn.name}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("rollout_hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": n_inst * args.steps / elapsed,
+        "unit": "control steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": scn.name, "controllers": n_inst,
+                   "samples": [i.n_samples for i in scn.instances][:8],
+                   "horizon": [i.n_timesteps for i in scn.instances][:8],
+                   "closed_loop": True, "launch": launch},
+        "sample_steps_per_s": ss * args.steps / elapsed,
+        "kernel_ms_per_step": {k: v / n_timed for k, v in kt.items()},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "kernel": "k_rollout", "flops_per_launch": flops,
+                     "note": "FP32 vector (VALU) work, no MFMA; peak is the published FP32 figure"},
+        "mean_traj_cost": sum(d["traj_cost"] for d in diag) / n_inst,
+    }
+    if tuned:
+        out["config"]["autotuned"] = True
+    if not args.no_cpu:
+        log("bench: CPU baseline ...")
+        out["cpu_baseline"] = cpu_baseline(pkg, scn, args.cpu_budget)
+        log(f"bench: CPU baseline {out['cpu_baseline']['value']:.2f} control steps/s")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,11 @@
+"""drone-race MPPI / PR-MPPI control engine for AMD MI355X (gfx950).
+
+The compute path is the HIP library libdr_amd.so (csrc/, C ABI in
+include/dr_api.h); this package is its Python host side: ctypes bindings,
+scenario builders and the engine handles.  There is no CPU fallback.
+"""
+from . import abi, philox, scenarios
+from .abi import EngineError, load
+from .engine import Engine, MPPI
+
+__all__ = ["abi", "philox", "scenarios", "Engine", "MPPI", "EngineError", "load"]

@@ -1,0 +1,135 @@
+"""ctypes mirror of include/dr_api.h and loader for the HIP engine library.
+
+The library is built in-tree (drone-race_amd/libdr_amd.so) by build.py.  No
+Python fallback exists: if the library is missing, load() raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdr_amd.so")
+
+ACTION_DIM = 4
+MAX_GATES = 16
+MAX_T = 256
+MAX_WORLD_DRONES = 8
+
+DR_OK, DR_EINVAL, DR_EHIP, DR_ENOMEM, DR_ESTATE = 0, 1, 2, 3, 4
+
+
+class QuadParams(C.Structure):
+    _fields_ = [(n, C.c_float) for n in (
+        "mass", "gravity", "drag", "rate_gain", "thrust_min", "thrust_max", "rate_max", "drone_radius", "dt")]
+
+
+class Gate(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("normal", C.c_float * 3), ("radius", C.c_float), ("_pad", C.c_float)]
+
+
+class Track(C.Structure):
+    _fields_ = [("n_gates", C.c_int32), ("n_win_laps", C.c_int32), ("ground_z", C.c_float), ("_pad", C.c_float),
+                ("gates", Gate * MAX_GATES)]
+
+
+class CostCfg(C.Structure):
+    _fields_ = [(n, C.c_float) for n in (
+        "gate_traversal_margin", "coll_dist_factor", "boundary_cost", "boundary_threshold_factor", "outside_cost",
+        "win_cost", "opp_win_cost", "final_adv_weight", "final_opp_adv_weight", "opp_prox_radius", "opp_prox_cost",
+        "_pad")]
+
+
+class DroneState(C.Structure):
+    _fields_ = [("p", C.c_float * 3), ("q", C.c_float * 4), ("v", C.c_float * 3), ("w", C.c_float * 3),
+                ("gate", C.c_int32), ("laps", C.c_int32), ("_pad", C.c_int32)]
+
+
+class Opponent(C.Structure):
+    _fields_ = [("p", C.c_float * 3), ("v", C.c_float * 3)]
+
+
+class Instance(C.Structure):
+    _fields_ = [("n_samples", C.c_int32), ("n_timesteps", C.c_int32), ("track", C.c_int32), ("drone", C.c_int32),
+                ("seed", C.c_uint64), ("lambda_", C.c_float), ("sigma", C.c_float * 4), ("shift_fill", C.c_float * 4),
+                ("adapt_lambda", C.c_int32)]
+
+
+class Diag(C.Structure):
+    _fields_ = [("min_cost", C.c_float), ("nu", C.c_float), ("lambda_", C.c_float), ("traj_cost", C.c_float),
+                ("step", C.c_uint32), ("_pad", C.c_int32)]
+
+
+class LaunchCfg(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "rollout_block", "rollout_grid", "reduce_spl", "reduce_tgroup", "steps_per_graph", "use_graph")]
+
+
+class MPPICfg(C.Structure):
+    _fields_ = [("n_samples", C.c_int32), ("n_timesteps", C.c_int32), ("lambda_", C.c_float),
+                ("sigma", C.c_float * 4), ("shift_fill", C.c_float * 4), ("adapt_lambda", C.c_int32),
+                ("max_opponents", C.c_int32)]
+
+
+# Every symbol include/dr_api.h declares, with its ctypes signature.
+_P = C.c_void_p
+_I = C.c_int
+_F = C.POINTER(C.c_float)
+SIGNATURES = {
+    "dr_last_error": (C.c_char_p, []),
+    "dr_version": (_I, []),
+    "dr_engine_create": (_I, [C.POINTER(QuadParams), C.POINTER(CostCfg), C.POINTER(Track), _I, C.POINTER(Instance), _I,
+                              C.POINTER(C.c_int32), _I, C.POINTER(_P)]),
+    "dr_engine_destroy": (None, [_P]),
+    "dr_engine_set_drones": (_I, [_P, C.POINTER(DroneState), _I]),
+    "dr_engine_get_drones": (_I, [_P, C.POINTER(DroneState), _I]),
+    "dr_engine_set_nominal": (_I, [_P, _I, _F, _I]),
+    "dr_engine_get_nominal": (_I, [_P, _I, _F, _I]),
+    "dr_engine_set_step": (_I, [_P, _I, C.c_uint32]),
+    "dr_engine_step": (_I, [_P, _I, _I, _F]),
+    "dr_engine_step_async": (_I, [_P, _I, _I]),
+    "dr_engine_synchronize": (_I, [_P]),
+    "dr_engine_get_costs": (_I, [_P, _I, _F, _I]),
+    "dr_engine_get_noise": (_I, [_P, _I, _F, C.c_size_t]),
+    "dr_engine_get_actions": (_I, [_P, _F, _I]),
+    "dr_engine_get_diag": (_I, [_P, C.POINTER(Diag), _I]),
+    "dr_engine_set_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
+    "dr_engine_get_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
+    "dr_engine_autotune": (_I, [_P, _I, C.POINTER(LaunchCfg), _F]),
+    "dr_engine_set_timing": (_I, [_P, _I]),
+    "dr_engine_last_timing": (_I, [_P, _F, _F, _F, _F]),
+    "dr_mppi_create": (_I, [C.POINTER(QuadParams), C.POINTER(CostCfg), C.POINTER(Track), C.POINTER(MPPICfg), C.c_uint64,
+                            C.POINTER(_P)]),
+    "dr_mppi_destroy": (None, [_P]),
+    "dr_mppi_step": (_I, [_P, C.POINTER(DroneState), C.POINTER(Opponent), _I, _F]),
+    "dr_mppi_get_nominal": (_I, [_P, _F, _I]),
+    "dr_mppi_set_nominal": (_I, [_P, _F, _I]),
+    "dr_mppi_get_diag": (_I, [_P, C.POINTER(Diag)]),
+    "dr_mppi_engine": (_P, [_P]),
+}
+
+_lib = None
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load the HIP engine library; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EngineError(f"HIP engine library not built: {path} is missing (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != DR_OK:
+        msg = _lib.dr_last_error().decode() if _lib is not None else ""
+        raise EngineError(f"dr status {rc}: {msg}")

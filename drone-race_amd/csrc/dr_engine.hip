@@ -1,0 +1,886 @@
+// Host side of the control engine: descriptor tables, device buffers, the
+// per-step launch sequence, hipGraph capture, the launch-shape autotuner and
+// the C ABI of include/dr_api.h.
+//
+// Reference host orchestration this replaces:
+//   MPPIController::allocDevice / getControl  src/mppi_controller.cu:143-210, 250-466
+// The reference runs ~11 launches and ~10 blocking copies per controller and
+// step on the default stream; here one graph replay runs the step for every
+// instance of the batch, and nothing is copied unless the caller asks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dr_kernels.h"
+
+namespace dr {
+
+hipError_t launch_rollout(int block, int grid, hipStream_t st, const InstDev* insts, InstDyn* dyn, const RTile* tiles, int n_tiles,
+                          const dr_track* tracks, const dr_drone_state* drones, const float* nominal, float* costs, float4* noise,
+                          float* tmin, const QuadConst& C, const dr_cost_cfg& KC);
+hipError_t launch_wavg(int spl, int n_wtiles, hipStream_t st, const InstDev* insts, InstDyn* dyn, const WTile* wtiles,
+                       const float* costs, const float4* noise, const float* tmin, double* part, double* den);
+hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
+                         float* nominal, float* actions, const QuadConst& C);
+hipError_t launch_advance(int n_drones, hipStream_t st, dr_drone_state* drones, const int* drone_inst, const InstDev* insts,
+                          const float* actions, const dr_track* tracks, const QuadConst& C, float margin);
+
+} // namespace dr
+
+using namespace dr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                                                                           \
+    do                                                                                                                                         \
+    {                                                                                                                                          \
+        hipError_t _e = (call);                                                                                                                \
+        if (_e != hipSuccess)                                                                                                                  \
+            return fail(DR_EHIP, std::string(#call) + ": " + hipGetErrorString(_e));                                                          \
+    } while (0)
+
+template <class T>
+struct DevBuf
+{
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count)
+    {
+        release();
+        n = count;
+        if (count == 0)
+            return hipSuccess;
+        return hipMalloc((void**)&p, count * sizeof(T));
+    }
+};
+
+bool valid_block(int b) { return b == 64 || b == 128 || b == 256; }
+bool valid_spl(int s) { return s == 1 || s == 2 || s == 4 || s == 8; }
+
+} // namespace
+
+struct dr_engine
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    QuadConst C{};
+    dr_cost_cfg cost{};
+    std::vector<dr_track> tracks;
+    std::vector<dr_instance> inst;
+    std::vector<int> drone_world;
+    std::vector<int> drone_inst;
+    int n_drones = 0;
+
+    dr_launch_cfg launch{};
+    std::vector<InstDev> idev;
+    std::vector<RTile> rtiles;
+    std::vector<WTile> wtiles;
+    long long noise_total = 0;
+    long long part_total = 0;
+    int den_total = 0;
+    int cost_total = 0;
+    int nom_total = 0;
+
+    DevBuf<InstDev> d_inst;
+    DevBuf<InstDyn> d_dyn;
+    DevBuf<RTile> d_rtiles;
+    DevBuf<WTile> d_wtiles;
+    DevBuf<dr_track> d_tracks;
+    DevBuf<dr_drone_state> d_drones;
+    DevBuf<int> d_drone_inst;
+    DevBuf<float> d_nominal;
+    DevBuf<float> d_costs;
+    DevBuf<float4> d_noise;
+    DevBuf<float> d_tmin;
+    DevBuf<double> d_part;
+    DevBuf<double> d_den;
+    DevBuf<float> d_actions;
+
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    int graph_steps[2] = {0, 0};
+
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    int timed_steps = 0;
+    float last_ms[4] = {0, 0, 0, 0};
+
+    ~dr_engine()
+    {
+        drop_graphs();
+        for (auto ev : events)
+            (void)hipEventDestroy(ev);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+
+    void drop_graphs()
+    {
+        for (int i = 0; i < 2; ++i)
+        {
+            if (graph_exec[i])
+                (void)hipGraphExecDestroy(graph_exec[i]);
+            graph_exec[i] = nullptr;
+            graph_steps[i] = 0;
+        }
+    }
+
+    int build_layout();
+    int enqueue_step(bool advance, int timed_index);
+    int run(int n_steps, bool advance);
+};
+
+int dr_engine::build_layout()
+{
+    const int rb = launch.rollout_block;
+    const int spl = launch.reduce_spl;
+    const int tg = launch.reduce_tgroup;
+    const int n = (int)inst.size();
+
+    // Rollout tiles: instances in decreasing horizon so the longest tiles
+    // are dispatched first (longest-processing-time order), each instance's
+    // tiles contiguous so that its minimum is a contiguous range.
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i)
+        order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst[a].n_timesteps > inst[b].n_timesteps; });
+
+    idev.assign(n, InstDev{});
+    rtiles.clear();
+    wtiles.clear();
+    noise_total = 0;
+    part_total = 0;
+    den_total = 0;
+    cost_total = 0;
+    nom_total = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        const dr_instance& s = inst[i];
+        InstDev& I = idev[i];
+        I.K = s.n_samples;
+        I.T = s.n_timesteps;
+        I.track = s.track;
+        I.drone = s.drone;
+        I.key0 = (uint32_t)(s.seed & 0xffffffffu);
+        I.key1 = (uint32_t)(s.seed >> 32);
+        for (int d = 0; d < 4; ++d)
+        {
+            I.sigma[d] = s.sigma[d];
+            I.shift_fill[d] = s.shift_fill[d];
+        }
+        I.adapt = s.adapt_lambda;
+        I.n_opp = 0;
+        for (int d = 0; d < n_drones; ++d)
+            if (d != s.drone && drone_world[d] == drone_world[s.drone])
+                I.opp[I.n_opp++] = d;
+        I.noise_off = noise_total;
+        noise_total += (long long)I.T * I.K;
+        I.cost_off = cost_total;
+        cost_total += I.K;
+        I.nom_off = nom_total;
+        nom_total += I.T * 4;
+        I.n_chunks = (I.K + WAVG_BLOCK * spl - 1) / (WAVG_BLOCK * spl);
+        I.part_off = part_total;
+        part_total += (long long)I.n_chunks * I.T * 4;
+        I.den_off = den_total;
+        den_total += I.n_chunks;
+    }
+    for (int oi = 0; oi < n; ++oi)
+    {
+        const int i = order[oi];
+        InstDev& I = idev[i];
+        I.rtile_first = (int)rtiles.size();
+        for (int k0 = 0; k0 <= I.K; k0 += rb)
+            rtiles.push_back(RTile{i, k0});
+        I.rtile_count = (int)rtiles.size() - I.rtile_first;
+    }
+    for (int oi = 0; oi < n; ++oi)
+    {
+        const int i = order[oi];
+        const InstDev& I = idev[i];
+        for (int c = 0; c < I.n_chunks; ++c)
+            for (int t0 = 0; t0 < I.T; t0 += tg)
+                wtiles.push_back(WTile{i, c, t0, std::min(I.T, t0 + tg)});
+    }
+
+    HIPCHK(d_inst.alloc(n));
+    HIPCHK(hipMemcpy(d_inst.p, idev.data(), n * sizeof(InstDev), hipMemcpyHostToDevice));
+    HIPCHK(d_rtiles.alloc(rtiles.size()));
+    HIPCHK(hipMemcpy(d_rtiles.p, rtiles.data(), rtiles.size() * sizeof(RTile), hipMemcpyHostToDevice));
+    HIPCHK(d_wtiles.alloc(wtiles.size()));
+    HIPCHK(hipMemcpy(d_wtiles.p, wtiles.data(), wtiles.size() * sizeof(WTile), hipMemcpyHostToDevice));
+    HIPCHK(d_tmin.alloc(rtiles.size()));
+    HIPCHK(d_part.alloc(part_total));
+    HIPCHK(d_den.alloc(den_total));
+    drop_graphs();
+    return DR_OK;
+}
+
+int dr_engine::enqueue_step(bool advance, int timed_index)
+{
+    const int n = (int)inst.size();
+    hipEvent_t* ev = (timed_index >= 0) ? &events[(size_t)timed_index * 4] : nullptr;
+    if (ev)
+        HIPCHK(hipEventRecord(ev[0], stream));
+    HIPCHK(launch_rollout(launch.rollout_block, launch.rollout_grid, stream, d_inst.p, d_dyn.p, d_rtiles.p, (int)rtiles.size(),
+                          d_tracks.p, d_drones.p, d_nominal.p, d_costs.p, d_noise.p, d_tmin.p, C, cost));
+    if (ev)
+        HIPCHK(hipEventRecord(ev[1], stream));
+    HIPCHK(launch_wavg(launch.reduce_spl, (int)wtiles.size(), stream, d_inst.p, d_dyn.p, d_wtiles.p, d_costs.p, d_noise.p, d_tmin.p,
+                       d_part.p, d_den.p));
+    if (ev)
+        HIPCHK(hipEventRecord(ev[2], stream));
+    HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C));
+    if (advance)
+        HIPCHK(launch_advance(n_drones, stream, d_drones.p, d_drone_inst.p, d_inst.p, d_actions.p, d_tracks.p, C,
+                              cost.gate_traversal_margin));
+    if (ev)
+        HIPCHK(hipEventRecord(ev[3], stream));
+    return DR_OK;
+}
+
+int dr_engine::run(int n_steps, bool advance)
+{
+    if (n_steps < 0)
+        return fail(DR_EINVAL, "n_steps < 0");
+    timed_steps = 0;
+    if (timing)
+    {
+        const size_t need = (size_t)n_steps * 4;
+        while (events.size() < need)
+        {
+            hipEvent_t ev;
+            HIPCHK(hipEventCreate(&ev));
+            events.push_back(ev);
+        }
+        for (int s = 0; s < n_steps; ++s)
+        {
+            int rc = enqueue_step(advance, s);
+            if (rc)
+                return rc;
+        }
+        timed_steps = n_steps;
+        return DR_OK;
+    }
+    if (!launch.use_graph)
+    {
+        for (int s = 0; s < n_steps; ++s)
+        {
+            int rc = enqueue_step(advance, -1);
+            if (rc)
+                return rc;
+        }
+        return DR_OK;
+    }
+    const int gi = advance ? 1 : 0;
+    const int spg = std::max(1, launch.steps_per_graph);
+    if (!graph_exec[gi] || graph_steps[gi] != spg)
+    {
+        if (graph_exec[gi])
+            (void)hipGraphExecDestroy(graph_exec[gi]);
+        graph_exec[gi] = nullptr;
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = DR_OK;
+        for (int s = 0; s < spg && rc == DR_OK; ++s)
+            rc = enqueue_step(advance, -1);
+        hipError_t ce = hipStreamEndCapture(stream, &g);
+        if (rc)
+        {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ce != hipSuccess)
+            return fail(DR_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        hipError_t ie = hipGraphInstantiate(&graph_exec[gi], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess)
+            return fail(DR_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        graph_steps[gi] = spg;
+    }
+    int done = 0;
+    while (done + spg <= n_steps)
+    {
+        HIPCHK(hipGraphLaunch(graph_exec[gi], stream));
+        done += spg;
+    }
+    for (; done < n_steps; ++done)
+    {
+        int rc = enqueue_step(advance, -1);
+        if (rc)
+            return rc;
+    }
+    return DR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+const char* dr_last_error(void) { return g_err.c_str(); }
+
+int dr_version(void) { return DR_ABI_VERSION; }
+
+static int check_track(const dr_track& t, int idx)
+{
+    if (t.n_gates < 2 || t.n_gates > DR_MAX_GATES)
+        return fail(DR_EINVAL, "track " + std::to_string(idx) + ": n_gates must be in [2, 16]");
+    for (int g = 0; g < t.n_gates; ++g)
+        if (!(t.gates[g].radius > 0.0f))
+            return fail(DR_EINVAL, "track " + std::to_string(idx) + ": gate radius must be > 0");
+    return DR_OK;
+}
+
+int dr_engine_create(const dr_quad_params* params, const dr_cost_cfg* cost, const dr_track* tracks, int n_tracks,
+                     const dr_instance* instances, int n_instances, const int32_t* drone_world, int n_drones, dr_engine** out)
+{
+    if (!out)
+        return fail(DR_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!params || !cost || !tracks || !instances || !drone_world)
+        return fail(DR_EINVAL, "NULL argument");
+    if (n_tracks < 1 || n_instances < 1 || n_drones < 1)
+        return fail(DR_EINVAL, "need at least one track, instance and drone");
+    if (!(params->mass > 0.0f) || !(params->dt > 0.0f) || !(params->drone_radius > 0.0f))
+        return fail(DR_EINVAL, "mass, dt and drone_radius must be > 0");
+    if (!(params->thrust_max >= params->thrust_min) || !(params->rate_max >= 0.0f))
+        return fail(DR_EINVAL, "control limits are inconsistent");
+    if (!(cost->opp_prox_radius > 0.0f) || !(cost->boundary_threshold_factor > 0.0f))
+        return fail(DR_EINVAL, "opp_prox_radius and boundary_threshold_factor must be > 0");
+    for (int i = 0; i < n_tracks; ++i)
+    {
+        int rc = check_track(tracks[i], i);
+        if (rc)
+            return rc;
+    }
+    std::vector<int> inst_of_drone(n_drones, -1);
+    for (int i = 0; i < n_instances; ++i)
+    {
+        const dr_instance& s = instances[i];
+        const std::string w = "instance " + std::to_string(i) + ": ";
+        if (s.n_samples < 1 || s.n_samples > (1 << 24))
+            return fail(DR_EINVAL, w + "n_samples must be in [1, 2^24]");
+        if (s.n_timesteps < 1 || s.n_timesteps > DR_MAX_T)
+            return fail(DR_EINVAL, w + "n_timesteps must be in [1, 256]");
+        if (s.track < 0 || s.track >= n_tracks)
+            return fail(DR_EINVAL, w + "track index out of range");
+        if (s.drone < 0 || s.drone >= n_drones)
+            return fail(DR_EINVAL, w + "drone index out of range");
+        if (inst_of_drone[s.drone] >= 0)
+            return fail(DR_EINVAL, w + "drone already controlled by another instance");
+        if (!(s.lambda > 0.0f))
+            return fail(DR_EINVAL, w + "lambda must be > 0");
+        inst_of_drone[s.drone] = i;
+    }
+    for (int d = 0; d < n_drones; ++d)
+    {
+        int cnt = 0;
+        for (int e = 0; e < n_drones; ++e)
+            cnt += drone_world[e] == drone_world[d];
+        if (cnt > DR_MAX_WORLD_DRONES)
+            return fail(DR_EINVAL, "a world holds more than 8 drones");
+    }
+
+    dr_engine* e = new (std::nothrow) dr_engine();
+    if (!e)
+        return fail(DR_ENOMEM, "host allocation failed");
+    auto bail = [&](int rc) {
+        delete e;
+        return rc;
+    };
+    if (hipGetDevice(&e->device) != hipSuccess)
+        return bail(fail(DR_EHIP, "no HIP device"));
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(DR_EHIP, "hipStreamCreate failed"));
+    e->C = make_quad_const(*params);
+    e->cost = *cost;
+    e->tracks.assign(tracks, tracks + n_tracks);
+    e->inst.assign(instances, instances + n_instances);
+    e->drone_world.assign(drone_world, drone_world + n_drones);
+    e->drone_inst = inst_of_drone;
+    e->n_drones = n_drones;
+    e->launch = dr_launch_cfg{128, 0, 2, 50, 1, 1};
+
+    int rc = e->build_layout();
+    if (rc)
+        return bail(rc);
+    auto chk = [&](hipError_t x, const char* what) {
+        if (x != hipSuccess)
+        {
+            fail(x == hipErrorOutOfMemory ? DR_ENOMEM : DR_EHIP, std::string(what) + ": " + hipGetErrorString(x));
+            return false;
+        }
+        return true;
+    };
+    if (!chk(e->d_dyn.alloc(n_instances), "alloc dyn") || !chk(e->d_tracks.alloc(n_tracks), "alloc tracks") ||
+        !chk(e->d_drones.alloc(n_drones), "alloc drones") || !chk(e->d_drone_inst.alloc(n_drones), "alloc drone_inst") ||
+        !chk(e->d_nominal.alloc(e->nom_total), "alloc nominal") || !chk(e->d_costs.alloc(e->cost_total), "alloc costs") ||
+        !chk(e->d_noise.alloc(e->noise_total), "alloc noise") || !chk(e->d_actions.alloc((size_t)n_instances * 4), "alloc actions"))
+        return bail(DR_ENOMEM);
+    std::vector<InstDyn> dyn(n_instances);
+    for (int i = 0; i < n_instances; ++i)
+    {
+        dyn[i] = InstDyn{};
+        dyn[i].lambda = instances[i].lambda;
+    }
+    std::vector<float> nom(e->nom_total, 0.0f);
+    std::vector<dr_drone_state> dr(n_drones);
+    for (auto& s : dr)
+    {
+        std::memset(&s, 0, sizeof(s));
+        s.q[0] = 1.0f;
+    }
+    if (!chk(hipMemcpy(e->d_dyn.p, dyn.data(), dyn.size() * sizeof(InstDyn), hipMemcpyHostToDevice), "copy dyn") ||
+        !chk(hipMemcpy(e->d_tracks.p, tracks, n_tracks * sizeof(dr_track), hipMemcpyHostToDevice), "copy tracks") ||
+        !chk(hipMemcpy(e->d_drones.p, dr.data(), n_drones * sizeof(dr_drone_state), hipMemcpyHostToDevice), "copy drones") ||
+        !chk(hipMemcpy(e->d_drone_inst.p, inst_of_drone.data(), n_drones * sizeof(int), hipMemcpyHostToDevice), "copy drone_inst") ||
+        !chk(hipMemcpy(e->d_nominal.p, nom.data(), nom.size() * sizeof(float), hipMemcpyHostToDevice), "copy nominal"))
+        return bail(DR_EHIP);
+    *out = e;
+    return DR_OK;
+}
+
+void dr_engine_destroy(dr_engine* e)
+{
+    if (e)
+    {
+        (void)hipStreamSynchronize(e->stream);
+        delete e;
+    }
+}
+
+int dr_engine_set_drones(dr_engine* e, const dr_drone_state* states, int n_drones)
+{
+    if (!e || !states || n_drones != e->n_drones)
+        return fail(DR_EINVAL, "set_drones: bad handle, pointer or count");
+    HIPCHK(hipMemcpyAsync(e->d_drones.p, states, n_drones * sizeof(dr_drone_state), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DR_OK;
+}
+
+int dr_engine_get_drones(dr_engine* e, dr_drone_state* states, int n_drones)
+{
+    if (!e || !states || n_drones != e->n_drones)
+        return fail(DR_EINVAL, "get_drones: bad handle, pointer or count");
+    HIPCHK(hipMemcpyAsync(states, e->d_drones.p, n_drones * sizeof(dr_drone_state), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DR_OK;
+}
+
+static int check_inst(dr_engine* e, int i)
+{
+    if (!e)
+        return fail(DR_EINVAL, "NULL handle");
+    if (i < 0 || i >= (int)e->inst.size())
+        return fail(DR_EINVAL, "instance index out of range");
+    return DR_OK;
+}
+
+int dr_engine_set_nominal(dr_engine* e, int instance, const float* nominal, int n_floats)
+{
+    if (int rc = check_inst(e, instance))
+        return rc;
+    const InstDev& I = e->idev[instance];
+    if (!nominal || n_floats != I.T * 4)
+        return fail(DR_EINVAL, "set_nominal: expected T*4 floats");
+    HIPCHK(hipMemcpyAsync(e->d_nominal.p + I.nom_off, nominal, n_floats * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DR_OK;
+}
+
+int dr_engine_get_nominal(dr_engine* e, int instance, float* nominal, int n_floats)
+{
+    if (int rc = check_inst(e, instance))
+        return rc;
+    const InstDev& I = e->idev[instance];
+    if (!nominal || n_floats != I.T * 4)
+        return fail(DR_EINVAL, "get_nominal: expected T*4 floats");
+    HIPCHK(hipMemcpyAsync(nominal, e->d_nominal.p + I.nom_off, n_floats * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DR_OK;
+}
+
+int dr_engine_set_step(dr_engine* e, int instance, uint32_t step)
+{
+    if (int rc = check_inst(e, instance))
+        return rc;
+    InstDyn y;
+    HIPCHK(hipMemcpyAsync(&y, e->d_dyn.p + instance, sizeof(InstDyn), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    y.step = step;
+    HIPCHK(hipMemcpyAsync(e->d_dyn.p + instance, &y, sizeof(InstDyn), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DR_OK;
+}
+
+int dr_engine_step_async(dr_engine* e, int n_steps, int advance)
+{
+    if (!e)
+        return fail(DR_EINVAL, "NULL handle");
+    return e->run(n_steps, advance != 0);
+}
+
+int dr_engine_synchronize(dr_engine* e)
+{
+    if (!e)
+        return fail(DR_EINVAL, "NULL handle");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (e->timed_steps > 0)
+    {
+        float acc[4] = {0, 0, 0, 0};
+        for (int s = 0; s < e->timed_steps; ++s)
+        {
+            hipEvent_t* ev = &e->events[(size_t)s * 4];
+            float a, b, c, t;
+            HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
+            HIPCHK(hipEventElapsedTime(&b, ev[1], ev[2]));
+            HIPCHK(hipEventElapsedTime(&c, ev[2], ev[3]));
+            HIPCHK(hipEventElapsedTime(&t, ev[0], ev[3]));
+            acc[0] += a;
+            acc[1] += b;
+            acc[2] += c;
+            acc[3] += t;
+        }
+        std::memcpy(e->last_ms, acc, sizeof(acc));
+        e->timed_steps = 0;
+    }
+    return DR_OK;
+}
+
+int dr_engine_step(dr_engine* e, int n_steps, int advance, float* actions)
+{
+    int rc = dr_engine_step_async(e, n_steps, advance);
+    if (rc)
+        return rc;
+    rc = dr_engine_synchronize(e);
+    if (rc)
+        return rc;
+    if (actions)
+        return dr_engine_get_actions(e, actions, (int)e->inst.size() * 4);
+    return DR_OK;
+}
+
+int dr_engine_get_costs(dr_engine* e, int instance, float* costs, int n)
+{
+    if (int rc = check_inst(e, instance))
+        return rc;
+    const InstDev& I = e->idev[instance];
+    if (!costs || n != I.K)
+        return fail(DR_EINVAL, "get_costs: expected K floats");
+    HIPCHK(hipMemcpyAsync(costs, e->d_costs.p + I.cost_off, n * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DR_OK;
+}
+
+int dr_engine_get_noise(dr_engine* e, int instance, float* noise, size_t n)
+{
+    if (int rc = check_inst(e, instance))
+        return rc;
+    const InstDev& I = e->idev[instance];
+    if (!noise || n != (size_t)I.T * I.K * 4)
+        return fail(DR_EINVAL, "get_noise: expected T*K*4 floats");
+    HIPCHK(hipMemcpyAsync(noise, e->d_noise.p + I.noise_off, n * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DR_OK;
+}
+
+int dr_engine_get_actions(dr_engine* e, float* actions, int n_floats)
+{
+    if (!e || !actions || n_floats != (int)e->inst.size() * 4)
+        return fail(DR_EINVAL, "get_actions: expected n_instances*4 floats");
+    HIPCHK(hipMemcpyAsync(actions, e->d_actions.p, n_floats * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DR_OK;
+}
+
+int dr_engine_get_diag(dr_engine* e, dr_diag* diag, int n_instances)
+{
+    if (!e || !diag || n_instances != (int)e->inst.size())
+        return fail(DR_EINVAL, "get_diag: expected n_instances entries");
+    std::vector<InstDyn> y(n_instances);
+    HIPCHK(hipMemcpyAsync(y.data(), e->d_dyn.p, n_instances * sizeof(InstDyn), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int i = 0; i < n_instances; ++i)
+    {
+        diag[i].min_cost = y[i].min_cost;
+        diag[i].nu = y[i].nu;
+        diag[i].lambda = y[i].lambda;
+        diag[i].traj_cost = y[i].traj_cost;
+        diag[i].step = y[i].step;
+        diag[i]._pad = 0;
+    }
+    return DR_OK;
+}
+
+int dr_engine_set_launch(dr_engine* e, const dr_launch_cfg* cfg)
+{
+    if (!e || !cfg)
+        return fail(DR_EINVAL, "NULL argument");
+    dr_launch_cfg c = *cfg;
+    if (c.rollout_block == 0)
+        c.rollout_block = 128;
+    if (c.reduce_spl == 0)
+        c.reduce_spl = 2;
+    if (c.reduce_tgroup == 0)
+        c.reduce_tgroup = 50;
+    if (c.steps_per_graph == 0)
+        c.steps_per_graph = 1;
+    if (!valid_block(c.rollout_block) || !valid_spl(c.reduce_spl) || c.reduce_tgroup < 1 || c.reduce_tgroup > WAVG_TG ||
+        c.rollout_grid < 0 || c.steps_per_graph < 1 || c.steps_per_graph > 64)
+        return fail(DR_EINVAL, "set_launch: rollout_block in {64,128,256}, reduce_spl in {1,2,4,8}, reduce_tgroup in [1,64], "
+                               "steps_per_graph in [1,64]");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const bool relayout = c.rollout_block != e->launch.rollout_block || c.reduce_spl != e->launch.reduce_spl ||
+                          c.reduce_tgroup != e->launch.reduce_tgroup;
+    e->launch = c;
+    e->drop_graphs();
+    if (relayout)
+        return e->build_layout();
+    return DR_OK;
+}
+
+int dr_engine_get_launch(dr_engine* e, dr_launch_cfg* cfg)
+{
+    if (!e || !cfg)
+        return fail(DR_EINVAL, "NULL argument");
+    *cfg = e->launch;
+    return DR_OK;
+}
+
+int dr_engine_set_timing(dr_engine* e, int on)
+{
+    if (!e)
+        return fail(DR_EINVAL, "NULL handle");
+    e->timing = on != 0;
+    return DR_OK;
+}
+
+int dr_engine_last_timing(dr_engine* e, float* rollout_ms, float* reduce_ms, float* update_ms, float* total_ms)
+{
+    if (!e)
+        return fail(DR_EINVAL, "NULL handle");
+    if (rollout_ms)
+        *rollout_ms = e->last_ms[0];
+    if (reduce_ms)
+        *reduce_ms = e->last_ms[1];
+    if (update_ms)
+        *update_ms = e->last_ms[2];
+    if (total_ms)
+        *total_ms = e->last_ms[3];
+    return DR_OK;
+}
+
+int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* best_ms)
+{
+    if (!e || trials < 1)
+        return fail(DR_EINVAL, "autotune: bad handle or trials");
+    // Save the dynamic state the timing runs will overwrite.
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const int n = (int)e->inst.size();
+    std::vector<float> nom(e->nom_total);
+    std::vector<InstDyn> dyn(n);
+    HIPCHK(hipMemcpy(nom.data(), e->d_nominal.p, nom.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dyn.data(), e->d_dyn.p, dyn.size() * sizeof(InstDyn), hipMemcpyDeviceToHost));
+
+    const dr_launch_cfg saved = e->launch;
+    dr_launch_cfg best_cfg = saved;
+    float best_t = 1e30f;
+    int n_cu = 256;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, e->device) == hipSuccess)
+            n_cu = prop.multiProcessorCount;
+    }
+    const int blocks[] = {64, 128, 256};
+    const int spls[] = {1, 2, 4};
+    const int tgs[] = {25, 50};
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    int rc = DR_OK;
+    for (int rb : blocks)
+        for (int grid_mult : {0, 4, 8})
+            for (int spl : spls)
+                for (int tg : tgs)
+                {
+                    if (rc)
+                        break;
+                    dr_launch_cfg c = saved;
+                    c.rollout_block = rb;
+                    c.rollout_grid = grid_mult ? grid_mult * n_cu * (256 / rb) : 0;
+                    c.reduce_spl = spl;
+                    c.reduce_tgroup = tg;
+                    c.use_graph = 1;
+                    c.steps_per_graph = 1;
+                    rc = dr_engine_set_launch(e, &c);
+                    if (rc)
+                        break;
+                    rc = e->run(1, false); // warm-up and graph build
+                    if (rc)
+                        break;
+                    float ms = 0;
+                    if (hipEventRecord(a, e->stream) != hipSuccess)
+                        rc = fail(DR_EHIP, "autotune: hipEventRecord failed");
+                    if (!rc)
+                        rc = e->run(trials, false);
+                    if (!rc && (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                                hipEventElapsedTime(&ms, a, b) != hipSuccess))
+                        rc = fail(DR_EHIP, "autotune: event timing failed");
+                    if (rc)
+                        break;
+                    ms /= (float)trials;
+                    if (ms < best_t)
+                    {
+                        best_t = ms;
+                        best_cfg = c;
+                    }
+                }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc)
+    {
+        dr_engine_set_launch(e, &saved);
+        return rc;
+    }
+    best_cfg.steps_per_graph = saved.steps_per_graph;
+    rc = dr_engine_set_launch(e, &best_cfg);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemcpy(e->d_nominal.p, nom.data(), nom.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_dyn.p, dyn.data(), dyn.size() * sizeof(InstDyn), hipMemcpyHostToDevice));
+    if (best)
+        *best = best_cfg;
+    if (best_ms)
+        *best_ms = best_t;
+    return DR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// single controller
+
+struct dr_mppi
+{
+    dr_engine* e = nullptr;
+    int max_opp = 0;
+    int T = 0;
+};
+
+int dr_mppi_create(const dr_quad_params* params, const dr_cost_cfg* cost, const dr_track* track, const dr_mppi_cfg* cfg, uint64_t seed,
+                   dr_mppi** out)
+{
+    if (!out || !cfg)
+        return fail(DR_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (cfg->max_opponents < 0 || cfg->max_opponents > DR_MAX_WORLD_DRONES - 1)
+        return fail(DR_EINVAL, "max_opponents must be in [0, 7]");
+    dr_instance in{};
+    in.n_samples = cfg->n_samples;
+    in.n_timesteps = cfg->n_timesteps;
+    in.track = 0;
+    in.drone = 0;
+    in.seed = seed;
+    in.lambda = cfg->lambda;
+    for (int d = 0; d < 4; ++d)
+    {
+        in.sigma[d] = cfg->sigma[d];
+        in.shift_fill[d] = cfg->shift_fill[d];
+    }
+    in.adapt_lambda = cfg->adapt_lambda;
+    const int nd = 1 + cfg->max_opponents;
+    std::vector<int32_t> world(nd, 0);
+    dr_engine* e = nullptr;
+    int rc = dr_engine_create(params, cost, track, 1, &in, 1, world.data(), nd, &e);
+    if (rc)
+        return rc;
+    dr_mppi* h = new (std::nothrow) dr_mppi();
+    if (!h)
+    {
+        dr_engine_destroy(e);
+        return fail(DR_ENOMEM, "host allocation failed");
+    }
+    h->e = e;
+    h->max_opp = cfg->max_opponents;
+    h->T = cfg->n_timesteps;
+    *out = h;
+    return DR_OK;
+}
+
+void dr_mppi_destroy(dr_mppi* h)
+{
+    if (h)
+    {
+        dr_engine_destroy(h->e);
+        delete h;
+    }
+}
+
+int dr_mppi_step(dr_mppi* h, const dr_drone_state* state, const dr_opponent* opponents, int n_opponents, float* out_action)
+{
+    if (!h || !state || !out_action)
+        return fail(DR_EINVAL, "NULL argument");
+    if (n_opponents != h->max_opp || (n_opponents > 0 && !opponents))
+        return fail(DR_EINVAL, "mppi_step: n_opponents must equal max_opponents given at creation");
+    std::vector<dr_drone_state> w(1 + h->max_opp);
+    w[0] = *state;
+    for (int j = 0; j < h->max_opp; ++j)
+    {
+        std::memset(&w[1 + j], 0, sizeof(dr_drone_state));
+        w[1 + j].q[0] = 1.0f;
+        for (int d = 0; d < 3; ++d)
+        {
+            w[1 + j].p[d] = opponents[j].p[d];
+            w[1 + j].v[d] = opponents[j].v[d];
+        }
+    }
+    int rc = dr_engine_set_drones(h->e, w.data(), (int)w.size());
+    if (rc)
+        return rc;
+    return dr_engine_step(h->e, 1, 0, out_action);
+}
+
+int dr_mppi_get_nominal(dr_mppi* h, float* nominal, int n_floats)
+{
+    if (!h)
+        return fail(DR_EINVAL, "NULL handle");
+    return dr_engine_get_nominal(h->e, 0, nominal, n_floats);
+}
+
+int dr_mppi_set_nominal(dr_mppi* h, const float* nominal, int n_floats)
+{
+    if (!h)
+        return fail(DR_EINVAL, "NULL handle");
+    return dr_engine_set_nominal(h->e, 0, nominal, n_floats);
+}
+
+int dr_mppi_get_diag(dr_mppi* h, dr_diag* diag)
+{
+    if (!h)
+        return fail(DR_EINVAL, "NULL handle");
+    return dr_engine_get_diag(h->e, diag, 1);
+}
+
+dr_engine* dr_mppi_engine(dr_mppi* h) { return h ? h->e : nullptr; }
+
+} // extern "C"

@@ -1,0 +1,67 @@
+// Device-side data layout shared by the kernels and the host engine.
+//
+// HBM layout of one batch (all instances of all worlds):
+//   nominal  float[sum_i T_i*4]            per instance, [t][d]
+//   costs    float[sum_i K_i]              per instance, [k]
+//   noise    float4[sum_i T_i*K_i]         per instance, [t][k] (one float4 = the 4 control dims)
+//   tmin     float[n_rollout_tiles]        per-rollout-workgroup minimum cost
+//   part     double[sum_i chunks_i*T_i*4]  weighted-average partial numerators, [chunk][t][d]
+//   den      double[sum_i chunks_i]        weighted-average partial denominators
+//   drones   dr_drone_state[n_drones]      world states (ego and opponents)
+// The noise is written once by the rollout in coalesced [t][k] rows and read
+// once by the weighted average; no per-sample RNG state exists.
+#pragma once
+
+#include <stdint.h>
+
+#include "dr_quad.h"
+
+namespace dr {
+
+constexpr int MAX_OPP = DR_MAX_WORLD_DRONES - 1;
+constexpr int WAVE = 64;
+constexpr int WAVG_BLOCK = 256;
+constexpr int WAVG_TG = 64; // max timesteps per weighted-average workgroup
+
+struct InstDev
+{
+    int K, T, track, drone;
+    uint32_t key0, key1;
+    float sigma[4];
+    float shift_fill[4];
+    int adapt;
+    int n_opp;
+    int opp[MAX_OPP];
+    long long noise_off; // float4 units
+    int cost_off;
+    int nom_off;
+    int rtile_first, rtile_count; // rollout tiles, contiguous in tile order
+    long long part_off;           // double units
+    int den_off;
+    int n_chunks; // weighted-average sample chunks (WAVG_BLOCK * spl samples each)
+};
+
+struct InstDyn
+{
+    uint32_t step;
+    float lambda;
+    float min_cost;
+    float nu;
+    float traj_cost;
+    float _pad[3];
+};
+
+struct RTile
+{
+    int inst;
+    int k0;
+};
+
+struct WTile
+{
+    int inst;
+    int chunk;
+    int t0, t1;
+};
+
+} // namespace dr

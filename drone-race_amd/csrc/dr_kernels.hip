@@ -1,0 +1,404 @@
+// Control-step kernels for gfx950 (wave64).  One control step of a batch is
+//   k_rollout  -> k_wavg -> k_update [-> k_advance]
+// covering every instance of the batch in each launch (tile tables built on
+// the host from the per-instance descriptors).
+//
+// Reference kernels these replace (src/kernels.cu):
+//   generateNoiseKernel :18-31 + fullRolloutKernel :34-171   -> k_rollout
+//   computeMaskedMinCostsKernel :221-301                     -> k_rollout (tile minima) + k_wavg
+//   weightedAverageKernelUnified :307-408                    -> k_wavg + k_update
+//   clampNominalKernel :434-456 + host shift mppi_controller.cu:430-436 -> k_update
+// and the closed-loop environment step of SimulationEngine::dynStep
+// (src/engine.cpp:286-312)                                 -> k_advance
+#include <hip/hip_runtime.h>
+
+#include "dr_kernels.h"
+
+namespace dr {
+
+__device__ __forceinline__ float wave_min(float v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1)
+        v = fminf(v, __shfl_xor(v, m, WAVE));
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1)
+        v += __shfl_xor(v, m, WAVE);
+    return v;
+}
+
+// One lane = one sample k of one instance; lane k == K rolls the nominal out
+// without noise (the plan's own cost, reported as traj_cost).  Per step:
+// Philox noise (4 normals) -> float4 store of sigma*z into noise[t][k] ->
+// clamp(nominal[t] + eps) -> Euler step -> gate update -> running cost.
+// The nominal, the track and the opponents' start states sit in LDS.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_rollout(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
+                                                   const RTile* __restrict__ tiles, int n_tiles, const dr_track* __restrict__ tracks,
+                                                   const dr_drone_state* __restrict__ drones, const float* __restrict__ nominal,
+                                                   float* __restrict__ costs, float4* __restrict__ noise, float* __restrict__ tmin,
+                                                   QuadConst C, dr_cost_cfg KC)
+{
+    __shared__ float s_nom[DR_MAX_T * 4];
+    __shared__ dr_track s_track;
+    __shared__ dr_opponent s_opp[MAX_OPP];
+    __shared__ float s_min[BLOCK / WAVE];
+
+    const int tid = threadIdx.x;
+    for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
+    {
+        const RTile rt = tiles[tile];
+        const InstDev& I = insts[rt.inst];
+        const int K = I.K, T = I.T;
+        const uint32_t step = dyn[rt.inst].step;
+        const int n_opp = I.n_opp;
+
+        __syncthreads(); // LDS of the previous tile is no longer read
+        for (int i = tid; i < T * 4; i += BLOCK)
+            s_nom[i] = nominal[I.nom_off + i];
+        {
+            const float* src = reinterpret_cast<const float*>(tracks + I.track);
+            float* dst = reinterpret_cast<float*>(&s_track);
+            for (int i = tid; i < (int)(sizeof(dr_track) / 4); i += BLOCK)
+                dst[i] = src[i];
+        }
+        if (tid < n_opp)
+        {
+            const dr_drone_state& o = drones[I.opp[tid]];
+            for (int d = 0; d < 3; ++d)
+            {
+                s_opp[tid].p[d] = o.p[d];
+                s_opp[tid].v[d] = o.v[d];
+            }
+        }
+        __syncthreads();
+
+        const int k = rt.k0 + tid;
+        float cost = __builtin_inff();
+        if (k <= K)
+        {
+            const bool eval = (k == K);
+            QState s = state_from_abi(drones[I.drone]);
+            float acc = 0.0f, decay = 1.0f;
+            bool stop = false;
+            float4* nrow = noise + I.noise_off + k;
+            for (int t = 0; t < T; ++t)
+            {
+                float e[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                if (!eval)
+                {
+                    float z[4];
+                    normal4(I.key0, I.key1, (uint32_t)k, (uint32_t)t, step, STREAM_CONTROL, z);
+                    e[0] = I.sigma[0] * z[0];
+                    e[1] = I.sigma[1] * z[1];
+                    e[2] = I.sigma[2] * z[2];
+                    e[3] = I.sigma[3] * z[3];
+                    nrow[(long long)t * K] = make_float4(e[0], e[1], e[2], e[3]);
+                }
+                if (!stop)
+                {
+                    float uin[4], u[4];
+                    uin[0] = s_nom[t * 4 + 0] + e[0];
+                    uin[1] = s_nom[t * 4 + 1] + e[1];
+                    uin[2] = s_nom[t * 4 + 2] + e[2];
+                    uin[3] = s_nom[t * 4 + 3] + e[3];
+                    clamp_control(C, uin, u);
+                    const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+                    quad_step(C, s, u);
+                    update_gate(s_track.gates, s_track.n_gates, KC.gate_traversal_margin, p0, s);
+                    const float tau = C.dt * (float)(t + 1);
+                    int term;
+                    const float c = stage_cost(C, KC, s_track, s, s_opp, n_opp, tau, &term);
+                    acc = acc + c * decay;
+                    decay = decay * DECAY;
+                    stop = (term != TERM_NONE);
+                }
+            }
+            acc = acc + final_cost(KC, s_track, s);
+            if (eval)
+                dyn[rt.inst].traj_cost = acc;
+            else
+            {
+                costs[I.cost_off + k] = acc;
+                cost = acc;
+            }
+        }
+        const float wm = wave_min(cost);
+        if ((tid & (WAVE - 1)) == 0)
+            s_min[tid / WAVE] = wm;
+        __syncthreads();
+        if (tid == 0)
+        {
+            float m = s_min[0];
+            for (int w = 1; w < BLOCK / WAVE; ++w)
+                m = fminf(m, s_min[w]);
+            tmin[tile] = m;
+        }
+    }
+}
+
+// Weighted average over one chunk of SPL*256 samples for timesteps [t0, t1):
+// w_k = exp(-(c_k - min)/lambda) (min over the instance's rollout tiles),
+// numerator[t][d] = sum_k w_k eps_k[t][d] and denominator sum_k w_k, both in
+// FP64 so that the summation order does not move the FP32 result.  Lanes run
+// along k: each [t][k] row is one coalesced float4 read per lane.
+template <int SPL>
+__global__ __launch_bounds__(WAVG_BLOCK) void k_wavg(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
+                                                     const WTile* __restrict__ wtiles, const float* __restrict__ costs,
+                                                     const float4* __restrict__ noise, const float* __restrict__ tmin,
+                                                     double* __restrict__ part, double* __restrict__ den)
+{
+    __shared__ double s_part[WAVG_BLOCK / WAVE][WAVG_TG][4];
+    __shared__ float s_min[WAVG_BLOCK / WAVE];
+    __shared__ double s_den[WAVG_BLOCK / WAVE];
+
+    const int tid = threadIdx.x;
+    const int wave = tid / WAVE, lane = tid & (WAVE - 1);
+    const WTile wt = wtiles[blockIdx.x];
+    const InstDev& I = insts[wt.inst];
+    const int K = I.K, T = I.T;
+
+    float m = __builtin_inff();
+    for (int i = tid; i < I.rtile_count; i += WAVG_BLOCK)
+        m = fminf(m, tmin[I.rtile_first + i]);
+    m = wave_min(m);
+    if (lane == 0)
+        s_min[wave] = m;
+    __syncthreads();
+    float minC = s_min[0];
+    for (int w = 1; w < WAVG_BLOCK / WAVE; ++w)
+        minC = fminf(minC, s_min[w]);
+
+    const float lam = dyn[wt.inst].lambda;
+    float wk[SPL];
+    int kk[SPL];
+    double dsum = 0.0;
+#pragma unroll
+    for (int r = 0; r < SPL; ++r)
+    {
+        kk[r] = wt.chunk * (WAVG_BLOCK * SPL) + r * WAVG_BLOCK + tid;
+        wk[r] = 0.0f;
+        if (kk[r] < K && __builtin_isfinite(minC))
+            wk[r] = expf_ieee(-(costs[I.cost_off + kk[r]] - minC) / lam);
+        dsum += (double)wk[r];
+    }
+
+    if (wt.t0 == 0)
+    {
+        dsum = wave_sum(dsum);
+        if (lane == 0)
+            s_den[wave] = dsum;
+        __syncthreads();
+        if (tid == 0)
+        {
+            double d = s_den[0];
+            for (int w = 1; w < WAVG_BLOCK / WAVE; ++w)
+                d += s_den[w];
+            den[I.den_off + wt.chunk] = d;
+            if (wt.chunk == 0)
+                dyn[wt.inst].min_cost = minC;
+        }
+    }
+
+    const float4* base = noise + I.noise_off;
+    for (int t = wt.t0; t < wt.t1; ++t)
+    {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+        for (int r = 0; r < SPL; ++r)
+        {
+            if (kk[r] < K)
+            {
+                const float4 e = base[(long long)t * K + kk[r]];
+                const double w = (double)wk[r];
+                a0 += w * (double)e.x;
+                a1 += w * (double)e.y;
+                a2 += w * (double)e.z;
+                a3 += w * (double)e.w;
+            }
+        }
+        a0 = wave_sum(a0);
+        a1 = wave_sum(a1);
+        a2 = wave_sum(a2);
+        a3 = wave_sum(a3);
+        if (lane == 0)
+        {
+            s_part[wave][t - wt.t0][0] = a0;
+            s_part[wave][t - wt.t0][1] = a1;
+            s_part[wave][t - wt.t0][2] = a2;
+            s_part[wave][t - wt.t0][3] = a3;
+        }
+    }
+    __syncthreads();
+    const int n = (wt.t1 - wt.t0) * 4;
+    double* out = part + I.part_off + ((long long)wt.chunk * T + wt.t0) * 4;
+    for (int i = tid; i < n; i += WAVG_BLOCK)
+    {
+        double v = s_part[0][i / 4][i & 3];
+        for (int w = 1; w < WAVG_BLOCK / WAVE; ++w)
+            v += s_part[w][i / 4][i & 3];
+        out[i] = v;
+    }
+}
+
+// Per instance: numerator and denominator over chunks in fixed order,
+// nominal += num/den when den > 1e-30 (kernels.cu:402-403), box clamp of the
+// whole nominal (kernels.cu:434-456), action = first control, shift left by
+// one step with shift_fill at the end (mppi_controller.cu:430-436), nu and
+// the optional temperature update (mppi_controller.cu:413-424).
+__global__ __launch_bounds__(256) void k_update(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
+                                                const double* __restrict__ part, const double* __restrict__ den,
+                                                float* __restrict__ nominal, float* __restrict__ actions, QuadConst C)
+{
+    __shared__ float s_u[DR_MAX_T * 4];
+    const int inst = blockIdx.x;
+    const InstDev& I = insts[inst];
+    const int T = I.T, tid = threadIdx.x;
+
+    double dn = 0.0;
+    for (int c = 0; c < I.n_chunks; ++c)
+        dn += den[I.den_off + c];
+
+    for (int i = tid; i < T * 4; i += blockDim.x)
+    {
+        double num = 0.0;
+        for (int c = 0; c < I.n_chunks; ++c)
+            num += part[I.part_off + (long long)c * T * 4 + i];
+        float u = nominal[I.nom_off + i];
+        if (dn > 1e-30)
+            u = u + (float)(num / dn);
+        s_u[i] = u;
+    }
+    __syncthreads();
+    for (int t = tid; t < T; t += blockDim.x)
+    {
+        float in[4] = {s_u[t * 4 + 0], s_u[t * 4 + 1], s_u[t * 4 + 2], s_u[t * 4 + 3]};
+        float out[4];
+        clamp_control(C, in, out);
+        for (int d = 0; d < 4; ++d)
+            s_u[t * 4 + d] = out[d];
+    }
+    __syncthreads();
+    if (tid < 4)
+        actions[inst * 4 + tid] = s_u[tid];
+    for (int i = tid; i < T * 4; i += blockDim.x)
+    {
+        const int t = i / 4, d = i & 3;
+        nominal[I.nom_off + i] = (t + 1 < T) ? s_u[i + 4] : I.shift_fill[d];
+    }
+    if (tid == 0)
+    {
+        InstDyn& Y = dyn[inst];
+        const float nu = (float)dn;
+        Y.nu = nu;
+        if (I.adapt)
+        {
+            if (nu > 0.001f * (float)I.K)
+                Y.lambda = Y.lambda * 0.9f;
+            else if (nu < 0.0005f * (float)I.K)
+                Y.lambda = Y.lambda * 1.2f;
+        }
+        Y.step = Y.step + 1u;
+    }
+}
+
+// Closed loop: every drone driven by an instance applies that instance's
+// action through the same dynamics and gate logic, noise-free; drones with no
+// controller keep their velocity.
+__global__ void k_advance(dr_drone_state* __restrict__ drones, int n_drones, const int* __restrict__ drone_inst,
+                          const InstDev* __restrict__ insts, const float* __restrict__ actions, const dr_track* __restrict__ tracks,
+                          QuadConst C, float margin)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n_drones)
+        return;
+    const int inst = drone_inst[d];
+    dr_drone_state st = drones[d];
+    if (inst < 0)
+    {
+        for (int i = 0; i < 3; ++i)
+            st.p[i] = st.p[i] + C.dt * st.v[i];
+    }
+    else
+    {
+        QState s = state_from_abi(st);
+        float in[4] = {actions[inst * 4 + 0], actions[inst * 4 + 1], actions[inst * 4 + 2], actions[inst * 4 + 3]};
+        float u[4];
+        clamp_control(C, in, u);
+        const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+        quad_step(C, s, u);
+        const dr_track& tr = tracks[insts[inst].track];
+        update_gate(tr.gates, tr.n_gates, margin, p0, s);
+        state_to_abi(s, st);
+    }
+    drones[d] = st;
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers (host)
+
+hipError_t launch_rollout(int block, int grid, hipStream_t st, const InstDev* insts, InstDyn* dyn, const RTile* tiles, int n_tiles,
+                          const dr_track* tracks, const dr_drone_state* drones, const float* nominal, float* costs, float4* noise,
+                          float* tmin, const QuadConst& C, const dr_cost_cfg& KC)
+{
+    const int g = grid > 0 ? (grid < n_tiles ? grid : n_tiles) : n_tiles;
+    switch (block)
+    {
+    case 64:
+        k_rollout<64><<<g, 64, 0, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, KC);
+        break;
+    case 128:
+        k_rollout<128><<<g, 128, 0, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, KC);
+        break;
+    case 256:
+        k_rollout<256><<<g, 256, 0, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, KC);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_wavg(int spl, int n_wtiles, hipStream_t st, const InstDev* insts, InstDyn* dyn, const WTile* wtiles,
+                       const float* costs, const float4* noise, const float* tmin, double* part, double* den)
+{
+    switch (spl)
+    {
+    case 1:
+        k_wavg<1><<<n_wtiles, WAVG_BLOCK, 0, st>>>(insts, dyn, wtiles, costs, noise, tmin, part, den);
+        break;
+    case 2:
+        k_wavg<2><<<n_wtiles, WAVG_BLOCK, 0, st>>>(insts, dyn, wtiles, costs, noise, tmin, part, den);
+        break;
+    case 4:
+        k_wavg<4><<<n_wtiles, WAVG_BLOCK, 0, st>>>(insts, dyn, wtiles, costs, noise, tmin, part, den);
+        break;
+    case 8:
+        k_wavg<8><<<n_wtiles, WAVG_BLOCK, 0, st>>>(insts, dyn, wtiles, costs, noise, tmin, part, den);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
+                         float* nominal, float* actions, const QuadConst& C)
+{
+    k_update<<<n_inst, 256, 0, st>>>(insts, dyn, part, den, nominal, actions, C);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance(int n_drones, hipStream_t st, dr_drone_state* drones, const int* drone_inst, const InstDev* insts,
+                          const float* actions, const dr_track* tracks, const QuadConst& C, float margin)
+{
+    k_advance<<<(n_drones + 63) / 64, 64, 0, st>>>(drones, n_drones, drone_inst, insts, actions, tracks, C, margin);
+    return hipGetLastError();
+}
+
+} // namespace dr

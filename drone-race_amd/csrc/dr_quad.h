@@ -1,0 +1,256 @@
+// Quad13 environment policy: 13-state quaternion quadrotor, gate race,
+// constant-velocity opponents.  Every expression is written in the order in
+// which it must be evaluated: the engine is compiled with -ffp-contract=off
+// and only the explicit fmaf calls fuse, so host and device agree bit for bit.
+//
+// The reference compiles one 2-D point-mass environment (EnvStratRace,
+// include/environments/env_stratrace.h:711-894).  The quadrotor below is the
+// environment the benchmark configurations call for; its gate bookkeeping
+// follows the reference's gate logic written for any dimension:
+//   gate crossing   include/environments/env_dronerace.h:412-470 (updateGates)
+//   race progress   include/environments/env_dronerace.h:374-394 (getAnyAdvance)
+// and its cost terms follow the structure of include/costs.cuh:7-39
+// (stateCost / finalCost): a running cost evaluated on the post-step state,
+// discounted by DECAY per step, plus a terminal progress cost.
+#pragma once
+
+#include <math.h>
+
+#include "dr_rng.h"
+#include "../../include/dr_api.h"
+
+namespace dr {
+
+// include/config.h:30
+constexpr float DECAY = 0.97f;
+
+enum Terminal : int
+{
+    TERM_NONE = 0,
+    TERM_LOSE = 1,
+    TERM_WIN = 2,
+};
+
+struct QState
+{
+    float p[3], q[4], v[3], w[3];
+    int gate, laps;
+};
+
+// Constants derived once on the host from dr_quad_params (same float ops in
+// the oracle): 1/m, dt/2 and dt*rate_gain.
+struct QuadConst
+{
+    float dt, half_dt, rate_k, inv_mass, gravity, drag;
+    float thrust_min, thrust_max, rate_max, drone_radius;
+};
+
+inline QuadConst make_quad_const(const dr_quad_params& P)
+{
+    QuadConst c;
+    c.dt = P.dt;
+    c.half_dt = 0.5f * P.dt;
+    c.rate_k = P.dt * P.rate_gain;
+    c.inv_mass = 1.0f / P.mass;
+    c.gravity = P.gravity;
+    c.drag = P.drag;
+    c.thrust_min = P.thrust_min;
+    c.thrust_max = P.thrust_max;
+    c.rate_max = P.rate_max;
+    c.drone_radius = P.drone_radius;
+    return c;
+}
+
+DR_HD float clampf(float x, float lo, float hi)
+{
+    return fminf(fmaxf(x, lo), hi);
+}
+
+// Box clamp of a control (collective thrust, three body rates).  Plays the
+// role of the acceleration clamp of env_stratrace.h:753-772 for this model.
+DR_HD void clamp_control(const QuadConst& C, const float in[4], float out[4])
+{
+    out[0] = clampf(in[0], C.thrust_min, C.thrust_max);
+    out[1] = clampf(in[1], -C.rate_max, C.rate_max);
+    out[2] = clampf(in[2], -C.rate_max, C.rate_max);
+    out[3] = clampf(in[3], -C.rate_max, C.rate_max);
+}
+
+// One explicit Euler step.  Position uses the old velocity, velocity the old
+// attitude, attitude the old body rates; the quaternion is renormalised.
+DR_HD void quad_step(const QuadConst& C, QState& s, const float u[4])
+{
+    const float qw = s.q[0], qx = s.q[1], qy = s.q[2], qz = s.q[3];
+    const float wx = s.w[0], wy = s.w[1], wz = s.w[2];
+
+    // thrust direction R(q) e3
+    const float b3x = 2.0f * (qx * qz + qw * qy);
+    const float b3y = 2.0f * (qy * qz - qw * qx);
+    const float b3z = 1.0f - 2.0f * (qx * qx + qy * qy);
+    const float at = u[0] * C.inv_mass;
+    const float ax = at * b3x - C.drag * s.v[0];
+    const float ay = at * b3y - C.drag * s.v[1];
+    const float az = at * b3z - C.drag * s.v[2] - C.gravity;
+
+    s.p[0] = s.p[0] + C.dt * s.v[0];
+    s.p[1] = s.p[1] + C.dt * s.v[1];
+    s.p[2] = s.p[2] + C.dt * s.v[2];
+
+    s.v[0] = s.v[0] + C.dt * ax;
+    s.v[1] = s.v[1] + C.dt * ay;
+    s.v[2] = s.v[2] + C.dt * az;
+
+    // q <- q + dt/2 * q (x) (0, w)
+    const float dqw = -(qx * wx + qy * wy + qz * wz);
+    const float dqx = qw * wx + qy * wz - qz * wy;
+    const float dqy = qw * wy - qx * wz + qz * wx;
+    const float dqz = qw * wz + qx * wy - qy * wx;
+    float nw = qw + C.half_dt * dqw;
+    float nx = qx + C.half_dt * dqx;
+    float ny = qy + C.half_dt * dqy;
+    float nz = qz + C.half_dt * dqz;
+    const float inv = 1.0f / __builtin_sqrtf(nw * nw + nx * nx + ny * ny + nz * nz);
+    s.q[0] = nw * inv;
+    s.q[1] = nx * inv;
+    s.q[2] = ny * inv;
+    s.q[3] = nz * inv;
+
+    // first-order body-rate tracking
+    s.w[0] = wx + C.rate_k * (u[1] - wx);
+    s.w[1] = wy + C.rate_k * (u[2] - wy);
+    s.w[2] = wz + C.rate_k * (u[3] - wz);
+}
+
+// Gate crossing test on the segment p0 -> s.p (env_dronerace.h:412-470):
+// lambda = n.(c - p0) / n.(p1 - p0); crossing if lambda in [0, 1] and the
+// crossing point lies within radius*margin of the centre; either direction.
+DR_HD void update_gate(const dr_gate* gates, int n_gates, float margin, const float p0[3], QState& s)
+{
+    int next = s.gate + 1;
+    if (next == n_gates)
+        next = 0;
+    const dr_gate& g = gates[next];
+    const float num = g.normal[0] * (g.center[0] - p0[0]) + g.normal[1] * (g.center[1] - p0[1]) + g.normal[2] * (g.center[2] - p0[2]);
+    const float den = g.normal[0] * (s.p[0] - p0[0]) + g.normal[1] * (s.p[1] - p0[1]) + g.normal[2] * (s.p[2] - p0[2]);
+    if (fabsf(den) < 1e-10f)
+        return;
+    const float lam = num / den;
+    if (lam < 0.0f || lam > 1.0f)
+        return;
+    const float om = 1.0f - lam;
+    const float dx = om * p0[0] + lam * s.p[0] - g.center[0];
+    const float dy = om * p0[1] + lam * s.p[1] - g.center[1];
+    const float dz = om * p0[2] + lam * s.p[2] - g.center[2];
+    const float rad = g.radius * g.radius * (margin * margin);
+    if (dx * dx + dy * dy + dz * dz <= rad)
+    {
+        s.gate = next;
+        if (next == 0)
+            s.laps = s.laps + 1;
+    }
+}
+
+// Race progress (env_dronerace.h:374-394): gate + n_gates*laps +
+// 0.8 * (1 - |p - c_next| / |c_next - c_gate|).
+DR_HD float advance(const dr_gate* gates, int n_gates, const QState& s)
+{
+    int next = s.gate + 1;
+    if (next == n_gates)
+        next = 0;
+    const dr_gate& a = gates[s.gate];
+    const dr_gate& b = gates[next];
+    const float gx = b.center[0] - s.p[0], gy = b.center[1] - s.p[1], gz = b.center[2] - s.p[2];
+    const float cx = b.center[0] - a.center[0], cy = b.center[1] - a.center[1], cz = b.center[2] - a.center[2];
+    const float sq_gate = gx * gx + gy * gy + gz * gz;
+    const float sq_cons = cx * cx + cy * cy + cz * cz;
+    return (float)s.gate + (float)(n_gates * s.laps) + 0.8f * (1.0f - __builtin_sqrtf(sq_gate / sq_cons));
+}
+
+// Running cost on the post-step state (costs.cuh:7-28 structure):
+//   opponent proximity  opp_prox_cost * (1 - d/r_prox)^2 for each d < r_prox
+//   boundary distance   bd = min(min_j d_j - drone_radius, z - ground_z)
+//   near-boundary       boundary_cost * (1 - bd/danger)^2 for bd < danger
+//   collision           outside_cost, terminal (TERM_LOSE), for bd < r*coll
+//   winner              -win_cost, terminal (TERM_WIN), for laps >= win laps
+// Opponent j is predicted at constant velocity: o_j = p_j + v_j * tau with
+// tau = dt * (t + 1) the time of the post-step state.
+DR_HD float stage_cost(const QuadConst& C, const dr_cost_cfg& K, const dr_track& tr, const QState& s, const dr_opponent* opp, int n_opp,
+                       float tau, int* term)
+{
+    float cost = 0.0f;
+    float min_sq = __builtin_inff();
+    const float r2 = K.opp_prox_radius * K.opp_prox_radius;
+    for (int j = 0; j < n_opp; ++j)
+    {
+        const float dx = s.p[0] - (opp[j].p[0] + opp[j].v[0] * tau);
+        const float dy = s.p[1] - (opp[j].p[1] + opp[j].v[1] * tau);
+        const float dz = s.p[2] - (opp[j].p[2] + opp[j].v[2] * tau);
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        min_sq = fminf(min_sq, d2);
+        if (d2 < r2)
+        {
+            const float x = 1.0f - __builtin_sqrtf(d2) / K.opp_prox_radius;
+            cost = cost + K.opp_prox_cost * (x * x);
+        }
+    }
+    const float d_opp = __builtin_sqrtf(min_sq) - C.drone_radius;
+    const float d_ground = s.p[2] - tr.ground_z;
+    const float bd = fminf(d_opp, d_ground);
+    const float danger = K.boundary_threshold_factor * C.drone_radius;
+    if (bd < danger)
+    {
+        const float x = 1.0f - bd / danger;
+        cost = cost + K.boundary_cost * (x * x);
+    }
+    int t = TERM_NONE;
+    if (bd < C.drone_radius * K.coll_dist_factor)
+    {
+        cost = cost + K.outside_cost;
+        t = TERM_LOSE;
+    }
+    else if (s.laps >= tr.n_win_laps)
+    {
+        cost = cost - K.win_cost;
+        t = TERM_WIN;
+    }
+    *term = t;
+    return cost;
+}
+
+// Terminal cost (costs.cuh:31-39): -final_adv_weight * advance.
+DR_HD float final_cost(const dr_cost_cfg& K, const dr_track& tr, const QState& s)
+{
+    return -K.final_adv_weight * advance(tr.gates, tr.n_gates, s);
+}
+
+DR_HD QState state_from_abi(const dr_drone_state& d)
+{
+    QState s;
+    for (int i = 0; i < 3; ++i)
+    {
+        s.p[i] = d.p[i];
+        s.v[i] = d.v[i];
+        s.w[i] = d.w[i];
+    }
+    for (int i = 0; i < 4; ++i)
+        s.q[i] = d.q[i];
+    s.gate = d.gate;
+    s.laps = d.laps;
+    return s;
+}
+
+DR_HD void state_to_abi(const QState& s, dr_drone_state& d)
+{
+    for (int i = 0; i < 3; ++i)
+    {
+        d.p[i] = s.p[i];
+        d.v[i] = s.v[i];
+        d.w[i] = s.w[i];
+    }
+    for (int i = 0; i < 4; ++i)
+        d.q[i] = s.q[i];
+    d.gate = s.gate;
+    d.laps = s.laps;
+}
+
+} // namespace dr

@@ -1,0 +1,177 @@
+"""Python handle over the C ABI (include/dr_api.h).
+
+Mirrors the reference controller interface (include/controllers.h:12-111):
+MPPI.step(state, opponents) plays Controller::getControl(state, outAction);
+Engine drives a whole batch of controllers and their closed-loop worlds.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import check
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Engine:
+    def __init__(self, scenario):
+        lib = abi.load()
+        self.lib = lib
+        self.sc = scenario
+        n_t = len(scenario.tracks)
+        self._tracks = (abi.Track * n_t)(*scenario.tracks)
+        n_i = len(scenario.instances)
+        self._inst = (abi.Instance * n_i)(*scenario.instances)
+        n_d = len(scenario.drone_world)
+        self._world = (C.c_int32 * n_d)(*scenario.drone_world)
+        self.n_inst, self.n_drones = n_i, n_d
+        h = C.c_void_p()
+        check(lib.dr_engine_create(C.byref(scenario.params), C.byref(scenario.cost), self._tracks, n_t, self._inst, n_i,
+                                   self._world, n_d, C.byref(h)))
+        self.h = h
+        self.set_drones(scenario.drones)
+        for i, ins in enumerate(scenario.instances):
+            nom = np.tile(np.array(ins.shift_fill[:], dtype=np.float32), (ins.n_timesteps, 1))
+            self.set_nominal(i, nom)
+
+    def close(self):
+        if self.h:
+            self.lib.dr_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_drones(self, states):
+        arr = (abi.DroneState * self.n_drones)(*states)
+        check(self.lib.dr_engine_set_drones(self.h, arr, self.n_drones))
+
+    def get_drones(self):
+        arr = (abi.DroneState * self.n_drones)()
+        check(self.lib.dr_engine_get_drones(self.h, arr, self.n_drones))
+        return arr
+
+    def set_nominal(self, i, nom):
+        a = np.ascontiguousarray(nom, dtype=np.float32).reshape(-1)
+        check(self.lib.dr_engine_set_nominal(self.h, i, _fptr(a), a.size))
+
+    def get_nominal(self, i):
+        T = self.sc.instances[i].n_timesteps
+        a = np.empty(T * 4, dtype=np.float32)
+        check(self.lib.dr_engine_get_nominal(self.h, i, _fptr(a), a.size))
+        return a.reshape(T, 4)
+
+    def set_step(self, i, step):
+        check(self.lib.dr_engine_set_step(self.h, i, step))
+
+    def step(self, n_steps=1, advance=False):
+        a = np.empty(self.n_inst * 4, dtype=np.float32)
+        check(self.lib.dr_engine_step(self.h, n_steps, 1 if advance else 0, _fptr(a)))
+        return a.reshape(self.n_inst, 4)
+
+    def step_async(self, n_steps=1, advance=False):
+        check(self.lib.dr_engine_step_async(self.h, n_steps, 1 if advance else 0))
+
+    def synchronize(self):
+        check(self.lib.dr_engine_synchronize(self.h))
+
+    def costs(self, i):
+        K = self.sc.instances[i].n_samples
+        a = np.empty(K, dtype=np.float32)
+        check(self.lib.dr_engine_get_costs(self.h, i, _fptr(a), K))
+        return a
+
+    def noise(self, i):
+        ins = self.sc.instances[i]
+        a = np.empty(ins.n_timesteps * ins.n_samples * 4, dtype=np.float32)
+        check(self.lib.dr_engine_get_noise(self.h, i, _fptr(a), a.size))
+        return a.reshape(ins.n_timesteps, ins.n_samples, 4)
+
+    def actions(self):
+        a = np.empty(self.n_inst * 4, dtype=np.float32)
+        check(self.lib.dr_engine_get_actions(self.h, _fptr(a), a.size))
+        return a.reshape(self.n_inst, 4)
+
+    def diag(self):
+        d = (abi.Diag * self.n_inst)()
+        check(self.lib.dr_engine_get_diag(self.h, d, self.n_inst))
+        return [dict(min_cost=x.min_cost, nu=x.nu, lam=x.lambda_, traj_cost=x.traj_cost, step=x.step) for x in d]
+
+    def set_launch(self, **kw):
+        cfg = self.get_launch()
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        check(self.lib.dr_engine_set_launch(self.h, C.byref(cfg)))
+
+    def get_launch(self):
+        cfg = abi.LaunchCfg()
+        check(self.lib.dr_engine_get_launch(self.h, C.byref(cfg)))
+        return cfg
+
+    def autotune(self, trials=5):
+        cfg = abi.LaunchCfg()
+        ms = C.c_float()
+        check(self.lib.dr_engine_autotune(self.h, trials, C.byref(cfg), C.byref(ms)))
+        return {f: getattr(cfg, f) for f, _ in abi.LaunchCfg._fields_}, ms.value
+
+    def set_timing(self, on):
+        check(self.lib.dr_engine_set_timing(self.h, 1 if on else 0))
+
+    def last_timing(self):
+        v = [C.c_float() for _ in range(4)]
+        check(self.lib.dr_engine_last_timing(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(("rollout_ms", "reduce_ms", "update_ms", "total_ms"), [x.value for x in v]))
+
+
+class MPPI:
+    """Single controller with the reference's getControl shape (controllers.h:23-111)."""
+
+    def __init__(self, params, cost, track, n_samples, n_timesteps, seed, lam=1.0, sigma=(2.0, 0.5, 0.5, 0.5),
+                 fill=(0.0, 0.0, 0.0, 0.0), adapt=0, max_opponents=0):
+        lib = abi.load()
+        self.lib = lib
+        self.T = n_timesteps
+        self.max_opp = max_opponents
+        cfg = abi.MPPICfg(n_samples=n_samples, n_timesteps=n_timesteps, lambda_=lam, sigma=(C.c_float * 4)(*sigma),
+                          shift_fill=(C.c_float * 4)(*fill), adapt_lambda=adapt, max_opponents=max_opponents)
+        self._track = track
+        h = C.c_void_p()
+        check(lib.dr_mppi_create(C.byref(params), C.byref(cost), C.byref(track), C.byref(cfg), seed, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.dr_mppi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, state, opponents=()):
+        opp = (abi.Opponent * max(1, len(opponents)))(*opponents)
+        out = np.empty(4, dtype=np.float32)
+        check(self.lib.dr_mppi_step(self.h, C.byref(state), opp, len(opponents), _fptr(out)))
+        return out
+
+    def get_nominal(self):
+        a = np.empty(self.T * 4, dtype=np.float32)
+        check(self.lib.dr_mppi_get_nominal(self.h, _fptr(a), a.size))
+        return a.reshape(self.T, 4)
+
+    def set_nominal(self, nom):
+        a = np.ascontiguousarray(nom, dtype=np.float32).reshape(-1)
+        check(self.lib.dr_mppi_set_nominal(self.h, _fptr(a), a.size))
+
+    def diag(self):
+        d = abi.Diag()
+        check(self.lib.dr_mppi_get_diag(self.h, C.byref(d)))
+        return dict(min_cost=d.min_cost, nu=d.nu, lam=d.lambda_, traj_cost=d.traj_cost, step=d.step)

@@ -1,0 +1,187 @@
+"""Workloads of the benchmark configurations, built from seeded Philox draws.
+
+Choices the configurations leave open, fixed here:
+  quadrotor    mass 1 kg, g 9.81, linear drag 0.1/s, body-rate gain 20/s
+               (tau 50 ms), thrust in [0, 30] N, rates in [-6, 6] rad/s,
+               radius 0.15 m, Euler dt 0.02 s
+  gates        altitude 4 m before jitter; the jitter applies to x, y and z;
+               normal = horizontal track tangent rotated by the yaw jitter
+  lemniscate   Bernoulli, x = a cos s / (1 + sin^2 s), y = a sin s cos s /
+               (1 + sin^2 s), a = 30 m, gates at s = 2 pi (i + 1/2) / n
+  start        drones hover 35% of the way from the last gate to gate 0,
+               spread 2.5 m apart across the approach, gate index n - 1
+  nominal      hover thrust (m g) and zero rates; the shift fills the freed
+               last step with the hover control (the reference fills zero,
+               mppi_controller.cu:430-436; shift_fill = 0 restores that)
+  costs        MPPIConfig defaults where a field exists (config.h:116-154);
+               opponent proximity radius 2 m, weight 50; final progress
+               weight 10
+"""
+import math
+
+import numpy as np
+
+from . import abi
+from .philox import uniforms, STREAM_TRACK
+
+HOVER_FILL = None  # set from the quad params below
+
+
+def quad_params():
+    return abi.QuadParams(mass=1.0, gravity=9.81, drag=0.1, rate_gain=20.0, thrust_min=0.0, thrust_max=30.0,
+                          rate_max=6.0, drone_radius=0.15, dt=0.02)
+
+
+def hover_control(p=None):
+    p = p or quad_params()
+    return [p.mass * p.gravity, 0.0, 0.0, 0.0]
+
+
+def cost_cfg(opp_prox_radius=2.0):
+    return abi.CostCfg(gate_traversal_margin=0.95, coll_dist_factor=1.0, boundary_cost=10.0,
+                       boundary_threshold_factor=1.5, outside_cost=1000.0, win_cost=1000.0, opp_win_cost=1000.0,
+                       final_adv_weight=10.0, final_opp_adv_weight=0.0, opp_prox_radius=opp_prox_radius,
+                       opp_prox_cost=50.0, _pad=0.0)
+
+
+def _make_track(centers, tangents, radius, n_win_laps=100, ground_z=0.0):
+    tr = abi.Track()
+    n = len(centers)
+    assert 2 <= n <= abi.MAX_GATES
+    tr.n_gates = n
+    tr.n_win_laps = n_win_laps
+    tr.ground_z = ground_z
+    for i in range(n):
+        g = tr.gates[i]
+        for d in range(3):
+            g.center[d] = float(centers[i][d])
+            g.normal[d] = float(tangents[i][d])
+        g.radius = float(radius)
+    return tr
+
+
+def track_circle(seed, n_gates=8, radius=20.0, jitter=1.0, gate_radius=1.0, altitude=4.0, salt=0):
+    """Gates at angles 2 pi i / n on a circle, centres + U(-jitter, jitter) per axis."""
+    u = uniforms(seed, 3 * n_gates, STREAM_TRACK, salt)
+    centers, tangents = [], []
+    for i in range(n_gates):
+        a = 2.0 * math.pi * i / n_gates
+        c = np.array([radius * math.cos(a), radius * math.sin(a), altitude])
+        c = c + jitter * (2.0 * u[3 * i:3 * i + 3] - 1.0)
+        centers.append(c)
+        tangents.append([-math.sin(a), math.cos(a), 0.0])
+    return _make_track(centers, tangents, gate_radius)
+
+
+def track_lemniscate(seed, n_gates=12, half_width=30.0, jitter=1.5, yaw_jitter=0.3, gate_radius=1.0, altitude=4.0, salt=0):
+    """Bernoulli lemniscate figure-eight; position jitter U(-j, j) per axis, yaw jitter U(-y, y)."""
+    u = uniforms(seed, 4 * n_gates, STREAM_TRACK, salt)
+    a = half_width
+    centers, tangents = [], []
+    for i in range(n_gates):
+        s = 2.0 * math.pi * (i + 0.5) / n_gates
+        den = 1.0 + math.sin(s) ** 2
+        x = a * math.cos(s) / den
+        y = a * math.sin(s) * math.cos(s) / den
+        h = 1e-6
+        den2 = 1.0 + math.sin(s + h) ** 2
+        dx = (a * math.cos(s + h) / den2 - x) / h
+        dy = (a * math.sin(s + h) * math.cos(s + h) / den2 - y) / h
+        yaw = math.atan2(dy, dx) + yaw_jitter * (2.0 * u[4 * i + 3] - 1.0)
+        c = np.array([x, y, altitude]) + jitter * (2.0 * u[4 * i:4 * i + 3] - 1.0)
+        centers.append(c)
+        tangents.append([math.cos(yaw), math.sin(yaw), 0.0])
+    return _make_track(centers, tangents, gate_radius)
+
+
+def start_states(track, n_drones, spacing=2.5, frac=0.35):
+    """Hovering drones on the approach to gate 0, spread across it."""
+    n = track.n_gates
+    g_prev = np.array(track.gates[n - 1].center[:])
+    g0 = np.array(track.gates[0].center[:])
+    base = g_prev + frac * (g0 - g_prev)
+    d = g0 - g_prev
+    lat = np.array([-d[1], d[0], 0.0])
+    lat /= max(np.linalg.norm(lat), 1e-9)
+    out = (abi.DroneState * n_drones)()
+    for j in range(n_drones):
+        off = (j - 0.5 * (n_drones - 1)) * spacing
+        p = base + off * lat
+        s = out[j]
+        for i in range(3):
+            s.p[i] = float(p[i])
+            s.v[i] = 0.0
+            s.w[i] = 0.0
+        s.q[0], s.q[1], s.q[2], s.q[3] = 1.0, 0.0, 0.0, 0.0
+        s.gate = n - 1
+        s.laps = 0
+    return out
+
+
+def instance(n_samples, n_timesteps, track, drone, seed, lam=1.0, sigma=(2.0, 0.5, 0.5, 0.5), fill=None, adapt=0):
+    fill = hover_control() if fill is None else fill
+    return abi.Instance(n_samples=n_samples, n_timesteps=n_timesteps, track=track, drone=drone, seed=seed,
+                        lambda_=lam, sigma=(abi.C.c_float * 4)(*sigma), shift_fill=(abi.C.c_float * 4)(*fill),
+                        adapt_lambda=adapt)
+
+
+class Scenario:
+    """Everything dr_engine_create needs, plus the run length."""
+
+    def __init__(self, name, tracks, instances, drone_world, drones, steps, description):
+        self.name = name
+        self.tracks = tracks
+        self.instances = instances
+        self.drone_world = drone_world
+        self.drones = drones
+        self.steps = steps
+        self.description = description
+        self.params = quad_params()
+        self.cost = cost_cfg()
+
+    @property
+    def sample_steps_per_step(self):
+        return sum(i.n_samples * i.n_timesteps for i in self.instances)
+
+
+def c0_single(seed=0, n_samples=1024, n_timesteps=50, steps=500):
+    """Plain MPPI, 1 quadrotor, K=1024, T=50, 8 gates on r=20 m, U(-1,1) jitter, seed 0."""
+    tr = track_circle(seed, 8, 20.0, 1.0)
+    drones = start_states(tr, 1)
+    inst = [instance(n_samples, n_timesteps, 0, 0, seed_mix(seed, 0))]
+    return Scenario("c0_single_quad_k1024_t50", [tr], inst, [0], drones, steps,
+                    "1 quadrotor, K=1024, T=50, 8-gate circle r=20 m, seed 0")
+
+
+def c1_four_drones(seed=1, n_samples=16384, n_timesteps=100, steps=1000):
+    """Plain MPPI, 4 drones racing, K=16384, T=100, 12-gate lemniscate a=30 m, seed 1."""
+    tr = track_lemniscate(seed, 12, 30.0, 1.5, 0.3)
+    drones = start_states(tr, 4)
+    inst = [instance(n_samples, n_timesteps, 0, j, seed_mix(seed, j)) for j in range(4)]
+    return Scenario("c1_4drones_k16384_t100", [tr], inst, [0, 0, 0, 0], drones, steps,
+                    "4 drones, one controller each, K=16384, T=100, 12-gate lemniscate a=30 m, seed 1")
+
+
+def c2_batch64(seed=3, n_inst=64, steps=200):
+    """64 single-drone controllers: K log-uniform [1024, 32768], T uniform {30..150},
+    each on its own seeded 8-12 gate circle of radius U(15, 40) m."""
+    u = uniforms(seed, 4 * n_inst, STREAM_TRACK, salt=7)
+    tracks, inst, worlds, drones_all = [], [], [], []
+    for i in range(n_inst):
+        k = int(round(math.exp(math.log(1024) + u[4 * i] * (math.log(32768) - math.log(1024)))))
+        t = 30 + int(u[4 * i + 1] * 121)
+        ng = 8 + int(u[4 * i + 2] * 5)
+        r = 15.0 + 25.0 * u[4 * i + 3]
+        tr = track_circle(seed_mix(seed, 1000 + i), ng, r, 1.0)
+        tracks.append(tr)
+        inst.append(instance(k, t, i, i, seed_mix(seed, i)))
+        worlds.append(i)
+        drones_all.append(start_states(tr, 1)[0])
+    drones = (abi.DroneState * n_inst)(*drones_all)
+    return Scenario("c2_batch64_uneven", tracks, inst, worlds, drones, steps,
+                    "64 single-drone controllers, K~logU[1024,32768], T~U{30..150}, own 8-12 gate circle r~U(15,40)")
+
+
+def seed_mix(seed, i):
+    """64-bit Philox key for instance i of a scenario seeded `seed`."""
+    return ((int(seed) * 0x9E3779B97F4A7C15) ^ (0xA5A5A5A5 + 0x632BE59BD9B4E019 * (i + 1))) & 0xFFFFFFFFFFFFFFFF

@@ -1,0 +1,234 @@
+/*
+ * drone-race MI355X control engine: C ABI.
+ *
+ * Plain pointers and sizes only; every entry point returns a dr_status and
+ * never throws.  dr_last_error() returns a thread-local description of the
+ * last failure.  Handles are not thread-safe; distinct handles are.  Each
+ * handle owns its device memory (a private copy of the tracks included) and
+ * one HIP stream; callers own every host buffer passed in.
+ *
+ * Reference interface being replaced (pablo-gr-rey/drone-race):
+ *   Controller::getControl(const SimState&, float* outAction)
+ *       include/controllers.h:12-21  -> dr_mppi_step / dr_engine_step
+ *   MPPIController(const EnvironmentConfig&, const MPPIConfig&, int seed,
+ *       std::optional<std::vector<float>> nominal)
+ *       include/controllers.h:23-111 -> dr_mppi_create (+ dr_mppi_set_nominal)
+ *   MPPIController::~MPPIController  src/mppi_controller.cu:51-54
+ *                                     -> dr_mppi_destroy
+ *   MPPIController::h_nominal         include/controllers.h:37
+ *                                     -> dr_mppi_get_nominal
+ *   MPPIConfig                        include/config.h:116-154 -> dr_mppi_cfg
+ *   EnvironmentConfig                 include/environments/env_stratrace_defs.h:53-86
+ *                                     -> dr_quad_params + dr_track (+ dr_cost_cfg)
+ *   SimState                          include/environments/env_stratrace_defs.h:88-96
+ *                                     -> dr_drone_state
+ */
+#ifndef DR_API_H
+#define DR_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DR_ABI_VERSION 1
+
+#define DR_ACTION_DIM 4
+#define DR_STATE_DIM 13
+#define DR_MAX_GATES 16
+#define DR_MAX_T 256
+#define DR_MAX_WORLD_DRONES 8
+
+typedef enum dr_status
+{
+    DR_OK = 0,
+    DR_EINVAL = 1,   /* bad argument or size */
+    DR_EHIP = 2,     /* HIP runtime error */
+    DR_ENOMEM = 3,   /* device or host allocation failed */
+    DR_ESTATE = 4,   /* call not valid in the handle's current state */
+} dr_status;
+
+/* Quadrotor model.  SI units; dt is the Euler step and the control period. */
+typedef struct dr_quad_params
+{
+    float mass;         /* kg */
+    float gravity;      /* m/s^2 */
+    float drag;         /* 1/s, linear drag */
+    float rate_gain;    /* 1/s, first-order body-rate tracking gain */
+    float thrust_min;   /* N */
+    float thrust_max;   /* N */
+    float rate_max;     /* rad/s */
+    float drone_radius; /* m */
+    float dt;           /* s */
+} dr_quad_params;
+
+typedef struct dr_gate
+{
+    float center[3];
+    float normal[3]; /* unit */
+    float radius;
+    float _pad;
+} dr_gate;
+
+typedef struct dr_track
+{
+    int32_t n_gates;    /* 2..DR_MAX_GATES */
+    int32_t n_win_laps; /* laps counted at each passage of gate 0 */
+    float ground_z;     /* m */
+    float _pad;
+    dr_gate gates[DR_MAX_GATES];
+} dr_track;
+
+/* Cost weights.  Field names follow MPPIConfig (include/config.h:116-154). */
+typedef struct dr_cost_cfg
+{
+    float gate_traversal_margin;     /* gate counts if crossed within radius*margin */
+    float coll_dist_factor;          /* collision if boundary distance < radius*factor */
+    float boundary_cost;             /* near-boundary penalty weight */
+    float boundary_threshold_factor; /* penalty starts at radius*factor */
+    float outside_cost;              /* collision cost (terminal) */
+    float win_cost;                  /* bonus when laps reach n_win_laps (terminal) */
+    float opp_win_cost;              /* reserved: opponents' laps are not simulated */
+    float final_adv_weight;          /* terminal -weight * race progress */
+    float final_opp_adv_weight;      /* reserved: opponents' progress is not simulated */
+    float opp_prox_radius;           /* opponent-proximity radius, m */
+    float opp_prox_cost;             /* weight of (1 - d/r)^2 inside the radius */
+    float _pad;
+} dr_cost_cfg;
+
+/* Drone state: position, unit quaternion (w, x, y, z), velocity, body
+ * rates, index of the last gate passed, laps. */
+typedef struct dr_drone_state
+{
+    float p[3];
+    float q[4];
+    float v[3];
+    float w[3];
+    int32_t gate;
+    int32_t laps;
+    int32_t _pad;
+} dr_drone_state;
+
+/* Constant-velocity opponent prediction seen by one controller. */
+typedef struct dr_opponent
+{
+    float p[3];
+    float v[3];
+} dr_opponent;
+
+/* One controller (one drone) in a batch. */
+typedef struct dr_instance
+{
+    int32_t n_samples;    /* K >= 1 */
+    int32_t n_timesteps;  /* T in 1..DR_MAX_T */
+    int32_t track;        /* index into the engine's track table */
+    int32_t drone;        /* ego drone slot; opponents = other drones of its world */
+    uint64_t seed;        /* Philox key */
+    float lambda;         /* temperature: w = exp(-(c - min)/lambda) */
+    float sigma[DR_ACTION_DIM];
+    float shift_fill[DR_ACTION_DIM]; /* written at t = T-1 by the shift (reference: 0) */
+    int32_t adapt_lambda; /* 1: the reference's nu-driven temperature update */
+} dr_instance;
+
+typedef struct dr_diag
+{
+    float min_cost;   /* min over samples of the last rollout */
+    float nu;         /* sum of weights of the last update */
+    float lambda;     /* temperature used by the next step */
+    float traj_cost;  /* noise-free cost of the nominal the last rollout started from */
+    uint32_t step;    /* control steps taken */
+    int32_t _pad;
+} dr_diag;
+
+/* Launch shape of the control step; chosen by dr_engine_autotune or set by
+ * hand.  Zero fields select the defaults. */
+typedef struct dr_launch_cfg
+{
+    int32_t rollout_block;    /* samples per rollout workgroup: 64, 128, 256 */
+    int32_t rollout_grid;     /* 0: one workgroup per tile; >0 persistent grid */
+    int32_t reduce_spl;       /* samples per lane in the weighted average: 1..8 */
+    int32_t reduce_tgroup;    /* timesteps per weighted-average workgroup */
+    int32_t steps_per_graph;  /* control steps captured per hipGraph */
+    int32_t use_graph;        /* 0: plain stream launches */
+} dr_launch_cfg;
+
+typedef struct dr_engine dr_engine;
+
+const char* dr_last_error(void);
+int dr_version(void);
+
+/* Batch engine.  drone_world[d] groups drones into worlds; an instance's
+ * opponents are the other drones of its ego drone's world, predicted at
+ * constant velocity from their state at the start of each control step. */
+int dr_engine_create(const dr_quad_params* params, const dr_cost_cfg* cost, const dr_track* tracks, int n_tracks,
+                     const dr_instance* instances, int n_instances, const int32_t* drone_world, int n_drones,
+                     dr_engine** out);
+void dr_engine_destroy(dr_engine* e);
+
+int dr_engine_set_drones(dr_engine* e, const dr_drone_state* states, int n_drones);
+int dr_engine_get_drones(dr_engine* e, dr_drone_state* states, int n_drones);
+int dr_engine_set_nominal(dr_engine* e, int instance, const float* nominal, int n_floats);
+int dr_engine_get_nominal(dr_engine* e, int instance, float* nominal, int n_floats);
+int dr_engine_set_step(dr_engine* e, int instance, uint32_t step);
+
+/* Run n_steps control steps (rollout, min, weighted average, update, clamp,
+ * shift).  advance != 0 integrates every controlled drone with its action
+ * after each step (closed loop); actions (n_instances*4, may be NULL)
+ * receives the last step's actions.  Synchronous. */
+int dr_engine_step(dr_engine* e, int n_steps, int advance, float* actions);
+/* Same work, enqueued only; dr_engine_synchronize waits for it. */
+int dr_engine_step_async(dr_engine* e, int n_steps, int advance);
+int dr_engine_synchronize(dr_engine* e);
+
+/* Results of the last control step, for tests and telemetry. */
+int dr_engine_get_costs(dr_engine* e, int instance, float* costs, int n);
+int dr_engine_get_noise(dr_engine* e, int instance, float* noise, size_t n); /* layout [t][k][4] */
+int dr_engine_get_actions(dr_engine* e, float* actions, int n_floats);
+int dr_engine_get_diag(dr_engine* e, dr_diag* diag, int n_instances);
+
+int dr_engine_set_launch(dr_engine* e, const dr_launch_cfg* cfg);
+int dr_engine_get_launch(dr_engine* e, dr_launch_cfg* cfg);
+/* Time candidate launch shapes on the engine's own workload (state is
+ * saved and restored) and keep the fastest.  trials = replays per shape. */
+int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* best_ms);
+/* on != 0: later steps run as plain stream launches with HIP events
+ * between the kernels (no graph), so that dr_engine_last_timing can report
+ * per-kernel device time. */
+int dr_engine_set_timing(dr_engine* e, int on);
+/* Device time of the last timed dr_engine_step / step_async batch, per
+ * kernel kind, from HIP events on the engine stream (ms, summed over steps;
+ * update_ms includes the closed-loop advance when it ran). */
+int dr_engine_last_timing(dr_engine* e, float* rollout_ms, float* reduce_ms, float* update_ms, float* total_ms);
+
+/* Single-controller entry points with the reference's shape. */
+typedef struct dr_mppi_cfg
+{
+    int32_t n_samples;
+    int32_t n_timesteps;
+    float lambda;
+    float sigma[DR_ACTION_DIM];
+    float shift_fill[DR_ACTION_DIM];
+    int32_t adapt_lambda;
+    int32_t max_opponents; /* <= DR_MAX_WORLD_DRONES - 1 */
+} dr_mppi_cfg;
+
+typedef struct dr_mppi dr_mppi;
+
+int dr_mppi_create(const dr_quad_params* params, const dr_cost_cfg* cost, const dr_track* track, const dr_mppi_cfg* cfg,
+                   uint64_t seed, dr_mppi** out);
+void dr_mppi_destroy(dr_mppi* h);
+/* One control step from `state` with the given opponents; writes the first
+ * action of the updated nominal, then shifts the nominal. */
+int dr_mppi_step(dr_mppi* h, const dr_drone_state* state, const dr_opponent* opponents, int n_opponents, float* out_action);
+int dr_mppi_get_nominal(dr_mppi* h, float* nominal, int n_floats);
+int dr_mppi_set_nominal(dr_mppi* h, const float* nominal, int n_floats);
+int dr_mppi_get_diag(dr_mppi* h, dr_diag* diag);
+dr_engine* dr_mppi_engine(dr_mppi* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DR_API_H */

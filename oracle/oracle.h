@@ -1,0 +1,415 @@
+// ORACLE — test infrastructure only.
+//
+// CPU restatement of the MPPI control step, used by tests/, by the smoke
+// check and by bench.py's CPU-baseline leg, and never by the product path.
+// It is compiled with g++ (-ffp-contract=off) into oracle/liboracle.so.
+//
+// Templated on the scalar type: S = float reproduces the GPU engine's FP32
+// arithmetic operation by operation (same noise bits, same dynamics bits);
+// S = double is the FP64 oracle whose distance to the FP32 results is
+// reported.  The noise itself (Philox4x32-10 + Box-Muller) is always
+// generated in FP32, so both oracles consume the same samples.
+//
+// Reference logic restated (pablo-gr-rey/drone-race):
+//   rollout loop, discount, terminal cost    src/kernels.cu:100-155
+//   running / terminal cost structure        include/costs.cuh:7-39
+//   weights, masked min (plain case)         src/kernels.cu:221-301, 307-408
+//   weighted update nom += sum(w eps)/sum(w) src/kernels.cu:376-403
+//   L2/box clamp of the nominal              src/kernels.cu:434-456
+//   action pick and shift with zero fill     src/mppi_controller.cu:392-397, 430-436
+//   temperature adaptation                   src/mppi_controller.cu:413-424
+//   gate crossing and race progress          include/environments/env_dronerace.h:374-394, 412-470
+// The quadrotor dynamics have no reference counterpart; they are pinned by
+// the physical invariants tested in tests/test_oracle_quad.py.
+//
+// Parity pins for each part (tests/):
+//   Philox4x32-10        Random123 known-answer vectors (tests/golden/philox_kat.json)
+//   log/exp/sincos       libm in FP64 over dense sweeps, <= 2 ulp
+//   Box-Muller normals   closed form from libm; moments and KS test
+//   Quad13 step          hover equilibrium, free fall, quaternion norm, zero-rate attitude
+//   gate crossing        hand-derived crossing / miss / margin cases
+//   running cost         closed-form values at chosen distances
+//   MPPI update          brute-force FP64 softmax on tiny K; sum of weights; min sample w = 1
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#include "../include/dr_api.h"
+
+namespace orc {
+
+// ---------------------------------------------------------------- noise (FP32)
+
+struct W4
+{
+    uint32_t v[4];
+};
+
+inline W4 philox(W4 c, uint32_t k0, uint32_t k1)
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    for (int round = 0; round < 10; ++round)
+    {
+        if (round)
+        {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t a = (uint64_t)M0 * c.v[0];
+        const uint64_t b = (uint64_t)M1 * c.v[2];
+        W4 n;
+        n.v[0] = (uint32_t)(b >> 32) ^ c.v[1] ^ k0;
+        n.v[1] = (uint32_t)b;
+        n.v[2] = (uint32_t)(a >> 32) ^ c.v[3] ^ k1;
+        n.v[3] = (uint32_t)a;
+        c = n;
+    }
+    return c;
+}
+
+inline uint32_t bits(float f)
+{
+    union { float f; uint32_t u; } x;
+    x.f = f;
+    return x.u;
+}
+
+inline float from_bits(uint32_t u)
+{
+    union { float f; uint32_t u; } x;
+    x.u = u;
+    return x.f;
+}
+
+// ln x for normal x > 0 via ln m = 2 atanh((m-1)/(m+1)), m in [sqrt(1/2), sqrt(2)).
+inline float log32(float x)
+{
+    const uint32_t b = bits(x);
+    int ex = (int)(b >> 23) - 127;
+    float m = from_bits((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421356f)
+    {
+        m = m * 0.5f;
+        ex = ex + 1;
+    }
+    const float f = m - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    float q = fmaf(0.0909090936f, z, 0.111111112f);
+    q = fmaf(q, z, 0.142857149f);
+    q = fmaf(q, z, 0.200000003f);
+    q = fmaf(q, z, 0.333333343f);
+    const float r = (s * z) * q;
+    const float lnm = 2.0f * (s + r);
+    const float fe = (float)ex;
+    return fmaf(fe, 0.693145752f, fmaf(fe, 1.42860677e-06f, lnm));
+}
+
+// sin, cos of 2 pi k / 2^24.
+inline void sincos_turns(uint32_t k, float* sn, float* cs)
+{
+    const uint32_t quarter = (k + 0x200000u) >> 22;
+    const int32_t rem = (int32_t)k - (int32_t)(quarter << 22);
+    const float a = (float)rem * 3.74507028e-07f;
+    const float a2 = a * a;
+    float ps = fmaf(2.75573188e-06f, a2, -1.98412701e-04f);
+    ps = fmaf(ps, a2, 8.33333377e-03f);
+    ps = fmaf(ps, a2, -0.166666672f);
+    const float s = fmaf(a * a2, ps, a);
+    float pc = fmaf(-2.75573188e-07f, a2, 2.48015876e-05f);
+    pc = fmaf(pc, a2, -1.38888892e-03f);
+    pc = fmaf(pc, a2, 4.16666679e-02f);
+    pc = fmaf(pc, a2, -0.5f);
+    const float c = fmaf(a2, pc, 1.0f);
+    const uint32_t qd = quarter & 3u;
+    *sn = qd == 0u ? s : (qd == 1u ? c : (qd == 2u ? -s : -c));
+    *cs = qd == 0u ? c : (qd == 1u ? -s : (qd == 2u ? -c : s));
+}
+
+inline void normal_pair(uint32_t a, uint32_t b, float* z0, float* z1)
+{
+    const float u1 = (float)((a >> 8) + 1u) * 5.9604644775390625e-08f;
+    const float r = sqrtf(-2.0f * log32(u1));
+    float s, c;
+    sincos_turns(b >> 8, &s, &c);
+    *z0 = r * c;
+    *z1 = r * s;
+}
+
+inline void normals4(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, float z[4])
+{
+    W4 c = {{c0, c1, c2, c3}};
+    const W4 o = philox(c, k0, k1);
+    normal_pair(o.v[0], o.v[1], &z[0], &z[1]);
+    normal_pair(o.v[2], o.v[3], &z[2], &z[3]);
+}
+
+// exp for x <= 88: Cody-Waite reduction, degree-7 Taylor, flush below e^-86.
+inline float exp32(float x)
+{
+    if (x < -86.0f)
+        return 0.0f;
+    if (x > 88.0f)
+        return INFINITY;
+    const float n = rintf(x * 1.44269502f);
+    const float r = fmaf(n, -1.42860677e-06f, fmaf(n, -0.693145752f, x));
+    float p = fmaf(1.98412701e-04f, r, 1.38888892e-03f);
+    p = fmaf(p, r, 8.33333377e-03f);
+    p = fmaf(p, r, 4.16666679e-02f);
+    p = fmaf(p, r, 0.166666672f);
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    return p * from_bits((uint32_t)((int)n + 127) << 23);
+}
+
+// ------------------------------------------------------------- Quad13 model
+
+template <class S>
+inline S sq_root(S x)
+{
+    return sqrt(x);
+}
+template <>
+inline float sq_root<float>(float x)
+{
+    return sqrtf(x);
+}
+
+template <class S>
+inline S mn(S a, S b)
+{
+    return b < a ? b : a;
+}
+template <class S>
+inline S mx(S a, S b)
+{
+    return a < b ? b : a;
+}
+
+template <class S>
+struct Model
+{
+    S dt, half_dt, rate_k, inv_mass, gravity, drag, tmin, tmax, rmax, radius;
+
+    // Constants are formed in FP32 first (as the engine does on the host),
+    // then widened.
+    explicit Model(const dr_quad_params& P)
+    {
+        const float h = 0.5f * P.dt, rk = P.dt * P.rate_gain, im = 1.0f / P.mass;
+        dt = (S)P.dt;
+        half_dt = (S)h;
+        rate_k = (S)rk;
+        inv_mass = (S)im;
+        gravity = (S)P.gravity;
+        drag = (S)P.drag;
+        tmin = (S)P.thrust_min;
+        tmax = (S)P.thrust_max;
+        rmax = (S)P.rate_max;
+        radius = (S)P.drone_radius;
+    }
+};
+
+template <class S>
+struct St
+{
+    S p[3], q[4], v[3], w[3];
+    int gate, laps;
+};
+
+template <class S>
+inline St<S> load_state(const dr_drone_state& d)
+{
+    St<S> s;
+    for (int i = 0; i < 3; ++i)
+    {
+        s.p[i] = (S)d.p[i];
+        s.v[i] = (S)d.v[i];
+        s.w[i] = (S)d.w[i];
+    }
+    for (int i = 0; i < 4; ++i)
+        s.q[i] = (S)d.q[i];
+    s.gate = d.gate;
+    s.laps = d.laps;
+    return s;
+}
+
+template <class S>
+inline S clamp(S x, S lo, S hi)
+{
+    return mn(mx(x, lo), hi);
+}
+
+template <class S>
+inline void clamp_u(const Model<S>& M, const S in[4], S out[4])
+{
+    out[0] = clamp(in[0], M.tmin, M.tmax);
+    for (int d = 1; d < 4; ++d)
+        out[d] = clamp(in[d], -M.rmax, M.rmax);
+}
+
+template <class S>
+inline void euler(const Model<S>& M, St<S>& s, const S u[4])
+{
+    const S qw = s.q[0], qx = s.q[1], qy = s.q[2], qz = s.q[3];
+    const S wx = s.w[0], wy = s.w[1], wz = s.w[2];
+    const S two = (S)2, one = (S)1;
+    // R(q) e3
+    const S bx = two * (qx * qz + qw * qy);
+    const S by = two * (qy * qz - qw * qx);
+    const S bz = one - two * (qx * qx + qy * qy);
+    const S at = u[0] * M.inv_mass;
+    const S acc[3] = {at * bx - M.drag * s.v[0], at * by - M.drag * s.v[1], at * bz - M.drag * s.v[2] - M.gravity};
+    for (int i = 0; i < 3; ++i)
+        s.p[i] = s.p[i] + M.dt * s.v[i];
+    for (int i = 0; i < 3; ++i)
+        s.v[i] = s.v[i] + M.dt * acc[i];
+    const S dq[4] = {-(qx * wx + qy * wy + qz * wz), qw * wx + qy * wz - qz * wy, qw * wy - qx * wz + qz * wx,
+                     qw * wz + qx * wy - qy * wx};
+    S n[4];
+    for (int i = 0; i < 4; ++i)
+        n[i] = s.q[i] + M.half_dt * dq[i];
+    const S inv = one / sq_root(n[0] * n[0] + n[1] * n[1] + n[2] * n[2] + n[3] * n[3]);
+    for (int i = 0; i < 4; ++i)
+        s.q[i] = n[i] * inv;
+    s.w[0] = wx + M.rate_k * (u[1] - wx);
+    s.w[1] = wy + M.rate_k * (u[2] - wy);
+    s.w[2] = wz + M.rate_k * (u[3] - wz);
+}
+
+inline bool tiny(float x) { return fabsf(x) < 1e-10f; }
+inline bool tiny(double x) { return fabs(x) < 1e-10; }
+
+// env_dronerace.h:412-470 in 3-D.
+template <class S>
+inline void gate_update(const dr_track& tr, S margin, const S p0[3], St<S>& s)
+{
+    const int nxt = (s.gate + 1) % tr.n_gates;
+    const dr_gate& g = tr.gates[nxt];
+    S c[3], n[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        c[i] = (S)g.center[i];
+        n[i] = (S)g.normal[i];
+    }
+    const S num = n[0] * (c[0] - p0[0]) + n[1] * (c[1] - p0[1]) + n[2] * (c[2] - p0[2]);
+    const S den = n[0] * (s.p[0] - p0[0]) + n[1] * (s.p[1] - p0[1]) + n[2] * (s.p[2] - p0[2]);
+    if (tiny(den))
+        return;
+    const S lam = num / den;
+    if (lam < (S)0 || lam > (S)1)
+        return;
+    const S om = (S)1 - lam;
+    S d2 = 0;
+    S d[3];
+    for (int i = 0; i < 3; ++i)
+        d[i] = om * p0[i] + lam * s.p[i] - c[i];
+    d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const S r = (S)g.radius;
+    if (d2 <= r * r * (margin * margin))
+    {
+        s.gate = nxt;
+        if (nxt == 0)
+            s.laps = s.laps + 1;
+    }
+}
+
+// env_dronerace.h:374-394
+template <class S>
+inline S progress(const dr_track& tr, const St<S>& s)
+{
+    const int nxt = (s.gate + 1) % tr.n_gates;
+    const dr_gate& a = tr.gates[s.gate];
+    const dr_gate& b = tr.gates[nxt];
+    S g2 = 0, c2 = 0;
+    S gd[3], cd[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        gd[i] = (S)b.center[i] - s.p[i];
+        cd[i] = (S)b.center[i] - (S)a.center[i];
+    }
+    g2 = gd[0] * gd[0] + gd[1] * gd[1] + gd[2] * gd[2];
+    c2 = cd[0] * cd[0] + cd[1] * cd[1] + cd[2] * cd[2];
+    return (S)s.gate + (S)(tr.n_gates * s.laps) + (S)0.8f * ((S)1 - sq_root(g2 / c2));
+}
+
+// costs.cuh:7-28 structure with the quadrotor's boundary distance.
+template <class S>
+inline S running_cost(const Model<S>& M, const dr_cost_cfg& K, const dr_track& tr, const St<S>& s, const dr_opponent* opp, int n_opp,
+                      S tau, int* term)
+{
+    S cost = 0;
+    S min_sq = (S)INFINITY;
+    const S rp = (S)K.opp_prox_radius;
+    const S r2 = rp * rp;
+    for (int j = 0; j < n_opp; ++j)
+    {
+        S d[3];
+        for (int i = 0; i < 3; ++i)
+            d[i] = s.p[i] - ((S)opp[j].p[i] + (S)opp[j].v[i] * tau);
+        const S d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        min_sq = mn(min_sq, d2);
+        if (d2 < r2)
+        {
+            const S x = (S)1 - sq_root(d2) / rp;
+            cost = cost + (S)K.opp_prox_cost * (x * x);
+        }
+    }
+    const S d_opp = sq_root(min_sq) - M.radius;
+    const S d_ground = s.p[2] - (S)tr.ground_z;
+    const S bd = mn(d_opp, d_ground);
+    const S danger = (S)(K.boundary_threshold_factor * M.radius);
+    if (bd < danger)
+    {
+        const S x = (S)1 - bd / danger;
+        cost = cost + (S)K.boundary_cost * (x * x);
+    }
+    *term = 0;
+    if (bd < (S)(M.radius * K.coll_dist_factor))
+    {
+        cost = cost + (S)K.outside_cost;
+        *term = 1;
+    }
+    else if (s.laps >= tr.n_win_laps)
+    {
+        cost = cost - (S)K.win_cost;
+        *term = 2;
+    }
+    return cost;
+}
+
+} // namespace orc
+
+// ------------------------------------------------------------ op counting
+// Scalar that counts FP operations (add, sub, mul, div, sqrt, min/max and
+// comparisons each count 1) for the algorithmic work figure of the rollout.
+namespace orc {
+
+struct OpTally
+{
+    long long flops = 0;
+};
+extern thread_local OpTally g_tally;
+
+struct Cn
+{
+    double v;
+    Cn() : v(0) {}
+    Cn(double x) : v(x) {}
+    explicit operator double() const { return v; }
+    explicit operator float() const { return (float)v; }
+};
+inline Cn operator+(Cn a, Cn b) { ++g_tally.flops; return Cn(a.v + b.v); }
+inline Cn operator-(Cn a, Cn b) { ++g_tally.flops; return Cn(a.v - b.v); }
+inline Cn operator*(Cn a, Cn b) { ++g_tally.flops; return Cn(a.v * b.v); }
+inline Cn operator/(Cn a, Cn b) { ++g_tally.flops; return Cn(a.v / b.v); }
+inline Cn operator-(Cn a) { return Cn(-a.v); }
+inline bool operator<(Cn a, Cn b) { ++g_tally.flops; return a.v < b.v; }
+inline bool operator>(Cn a, Cn b) { ++g_tally.flops; return a.v > b.v; }
+inline bool operator<=(Cn a, Cn b) { ++g_tally.flops; return a.v <= b.v; }
+template <>
+inline Cn sq_root<Cn>(Cn x) { ++g_tally.flops; return Cn(sqrt(x.v)); }
+inline bool tiny(Cn x) { ++g_tally.flops; return fabs(x.v) < 1e-10; }
+
+} // namespace orc

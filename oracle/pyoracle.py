@@ -1,0 +1,159 @@
+"""ORACLE — test infrastructure only.
+
+ctypes binding of oracle/liboracle.so (the CPU restatement in oracle.h /
+oracle.cpp).  Imported by tests/, __graft_entry__.smoke() and bench.py's
+CPU-baseline leg as the checker; never by the product.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib(abi):
+    """Load liboracle.so; `abi` is the engine's ctypes struct module."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    F, D, I, U = C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_int, C.c_uint32
+    L.or_philox4x32_10.argtypes = [C.POINTER(U), C.POINTER(U), C.POINTER(U)]
+    L.or_normals4.argtypes = [U, U, U, U, U, U, F]
+    L.or_log32.restype = C.c_float
+    L.or_log32.argtypes = [C.c_float]
+    L.or_exp32.restype = C.c_float
+    L.or_exp32.argtypes = [C.c_float]
+    L.or_sincos_turns.argtypes = [U, F, F]
+    L.or_quad_step.argtypes = [I, C.POINTER(abi.QuadParams), C.POINTER(abi.Track), C.c_float, D, C.POINTER(I),
+                               C.POINTER(I), D]
+    L.or_progress.restype = C.c_double
+    L.or_progress.argtypes = [I, C.POINTER(abi.Track), D, I, I]
+    L.or_running_cost.restype = C.c_double
+    L.or_running_cost.argtypes = [I, C.POINTER(abi.QuadParams), C.POINTER(abi.CostCfg), C.POINTER(abi.Track), D, I, I,
+                                  C.POINTER(abi.Opponent), I, C.c_double, C.POINTER(I)]
+    L.or_rollout.argtypes = [I, C.POINTER(abi.QuadParams), C.POINTER(abi.CostCfg), C.POINTER(abi.Track),
+                             C.POINTER(abi.DroneState), C.POINTER(abi.Opponent), I, F, I, I, U, U, U, F, D, F, D, I]
+    L.or_mppi_update.argtypes = [I, C.POINTER(abi.QuadParams), D, F, F, I, I, C.c_float, F, D, D, D, D, D, D]
+    L.or_world_run.argtypes = [C.POINTER(abi.QuadParams), C.POINTER(abi.CostCfg), C.POINTER(abi.Track),
+                               C.POINTER(abi.Instance), I, C.POINTER(C.c_int32), C.POINTER(abi.DroneState), I, F,
+                               C.POINTER(U), F, I, I, F, F]
+    _lib = L
+    return L
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    def __init__(self, abi):
+        self.abi = abi
+        self.L = lib(abi)
+
+    def philox(self, ctr, key):
+        c = (C.c_uint32 * 4)(*ctr)
+        k = (C.c_uint32 * 2)(*key)
+        o = (C.c_uint32 * 4)()
+        self.L.or_philox4x32_10(c, k, o)
+        return list(o)
+
+    def normals4(self, key, ctr):
+        o = np.empty(4, dtype=np.float32)
+        self.L.or_normals4(key[0], key[1], ctr[0], ctr[1], ctr[2], ctr[3], _f(o))
+        return o
+
+    def log32(self, x):
+        return self.L.or_log32(x)
+
+    def exp32(self, x):
+        return self.L.or_exp32(x)
+
+    def sincos(self, k):
+        s, c = C.c_float(), C.c_float()
+        self.L.or_sincos_turns(k, C.byref(s), C.byref(c))
+        return s.value, c.value
+
+    def quad_step(self, f64, params, x, u, track=None, margin=0.95, gate=0, laps=0):
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        g, l = C.c_int(gate), C.c_int(laps)
+        self.L.or_quad_step(int(f64), C.byref(params), C.byref(track) if track is not None else None, margin, _d(x),
+                            C.byref(g), C.byref(l), _d(u))
+        return x, g.value, l.value
+
+    def progress(self, f64, track, x, gate, laps):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        return self.L.or_progress(int(f64), C.byref(track), _d(x), gate, laps)
+
+    def running_cost(self, f64, params, cost, track, x, gate, laps, opponents, tau):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        opp = (self.abi.Opponent * max(1, len(opponents)))(*opponents)
+        term = C.c_int()
+        c = self.L.or_running_cost(int(f64), C.byref(params), C.byref(cost), C.byref(track), _d(x), gate, laps, opp,
+                                   len(opponents), tau, C.byref(term))
+        return c, term.value
+
+    def rollout(self, f64, params, cost, track, ego, opponents, nominal, K, T, seed, step, sigma, want_noise=True,
+                threads=8):
+        nominal = np.ascontiguousarray(nominal, dtype=np.float32).reshape(-1)
+        sig = np.ascontiguousarray(sigma, dtype=np.float32)
+        costs = np.empty(K, dtype=np.float64)
+        noise = np.empty(T * K * 4, dtype=np.float32) if want_noise else None
+        traj = C.c_double()
+        opp = (self.abi.Opponent * max(1, len(opponents)))(*opponents)
+        rc = self.L.or_rollout(int(f64), C.byref(params), C.byref(cost), C.byref(track), C.byref(ego), opp,
+                               len(opponents), _f(nominal), K, T, seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF, step,
+                               _f(sig), _d(costs), _f(noise) if noise is not None else None, C.byref(traj), threads)
+        assert rc == 0
+        return costs, (noise.reshape(T, K, 4) if noise is not None else None), traj.value
+
+    def update(self, f64, params, costs, noise, nominal, lam, fill):
+        K = costs.size
+        T = nominal.size // 4
+        costs = np.ascontiguousarray(costs, dtype=np.float64)
+        noise = np.ascontiguousarray(noise, dtype=np.float32).reshape(-1)
+        nominal = np.ascontiguousarray(nominal, dtype=np.float32).reshape(-1)
+        fill = np.ascontiguousarray(fill, dtype=np.float32)
+        upd = np.empty(T * 4)
+        act = np.empty(4)
+        sh = np.empty(T * 4)
+        w = np.empty(K)
+        mn, nu = C.c_double(), C.c_double()
+        rc = self.L.or_mppi_update(int(f64), C.byref(params), _d(costs), _f(noise), _f(nominal), K, T, lam, _f(fill),
+                                   _d(upd), _d(act), _d(sh), C.byref(mn), C.byref(nu), _d(w))
+        assert rc == 0
+        return dict(updated=upd.reshape(T, 4), action=act, shifted=sh.reshape(T, 4), min_cost=mn.value, nu=nu.value,
+                    weights=w)
+
+    def world_run(self, sc, drones, nominal, steps, lambdas, n_steps, threads=1):
+        """Closed-loop FP32 run; drones/nominal/steps/lambdas are updated in place."""
+        abi = self.abi
+        n_t, n_i, n_d = len(sc.tracks), len(sc.instances), len(sc.drone_world)
+        tracks = (abi.Track * n_t)(*sc.tracks)
+        inst = (abi.Instance * n_i)(*sc.instances)
+        world = (C.c_int32 * n_d)(*sc.drone_world)
+        st = (C.c_uint32 * n_i)(*steps)
+        lam = np.ascontiguousarray(lambdas, dtype=np.float32)
+        acts = np.empty(n_i * 4, dtype=np.float32)
+        traj = np.empty(n_i, dtype=np.float32)
+        rc = self.L.or_world_run(C.byref(sc.params), C.byref(sc.cost), tracks, inst, n_i, world, drones, n_d,
+                                 _f(nominal), st, _f(lam), n_steps, threads, _f(acts), _f(traj))
+        assert rc == 0
+        steps[:] = list(st)
+        lambdas[:] = lam
+        return acts.reshape(n_i, 4), traj

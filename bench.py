@@ -24,7 +24,7 @@ METRIC = "control steps/s, batched+adjoint vs plain MPPI at equal mean trajector
 # oracle's op-counting scalar on the first control step of this workload, plus
 # 96 for the Philox Box-Muller noise).  tests/test_bench_constants.py keeps it
 # equal to oracle/count_ops.py.
-FLOPS_PER_SAMPLE_STEP = 275.0
+FLOPS_PER_SAMPLE_STEP = 270.07
 # HBM bytes per rollout sample-step the rollout kernel must move: the float4
 # noise row it writes (read back once by the weighted average).
 ROLLOUT_BYTES_PER_SAMPLE_STEP = 16.0

@@ -60,7 +60,7 @@ class Diag(C.Structure):
 
 class LaunchCfg(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
-        "rollout_block", "rollout_grid", "reduce_spl", "reduce_tgroup", "steps_per_graph", "use_graph")]
+        "rollout_block", "rollout_grid", "reduce_chunk", "noise_batch", "steps_per_graph", "use_graph")]
 
 
 class MPPICfg(C.Structure):

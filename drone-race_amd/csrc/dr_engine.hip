@@ -20,10 +20,11 @@
 
 namespace dr {
 
-hipError_t launch_rollout(int block, int grid, hipStream_t st, const InstDev* insts, InstDyn* dyn, const RTile* tiles, int n_tiles,
-                          const dr_track* tracks, const dr_drone_state* drones, const float* nominal, float* costs, float4* noise,
-                          float* tmin, const QuadConst& C, const dr_cost_cfg& KC);
-hipError_t launch_wavg(int spl, int n_wtiles, hipStream_t st, const InstDev* insts, InstDyn* dyn, const WTile* wtiles,
+hipError_t launch_rollout(int block, int nb, int grid, hipStream_t st, const InstDev* insts, InstDyn* dyn, const RTile* tiles,
+                          int n_tiles, const dr_track* tracks, const dr_drone_state* drones, const float* nominal, float* costs,
+                          float4* noise, float* tmin, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, int max_T,
+                          int opp_stride);
+hipError_t launch_wavg(int chunk, int n_wtiles, hipStream_t st, const InstDev* insts, InstDyn* dyn, const WTile* wtiles,
                        const float* costs, const float4* noise, const float* tmin, double* part, double* den);
 hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
                          float* nominal, float* actions, const QuadConst& C);
@@ -76,7 +77,8 @@ struct DevBuf
 };
 
 bool valid_block(int b) { return b == 64 || b == 128 || b == 256; }
-bool valid_spl(int s) { return s == 1 || s == 2 || s == 4 || s == 8; }
+bool valid_chunk(int c) { return c >= 64 && c <= WAVG_MAX_CHUNK && c % 64 == 0; }
+bool valid_nb(int b) { return b == 1 || b == 2 || b == 4; }
 
 } // namespace
 
@@ -85,7 +87,10 @@ struct dr_engine
     int device = 0;
     hipStream_t stream = nullptr;
     QuadConst C{};
+    CostConst Q{};
     dr_cost_cfg cost{};
+    int max_T = 1;
+    int opp_stride = 1;
     std::vector<dr_track> tracks;
     std::vector<dr_instance> inst;
     std::vector<int> drone_world;
@@ -153,8 +158,7 @@ struct dr_engine
 int dr_engine::build_layout()
 {
     const int rb = launch.rollout_block;
-    const int spl = launch.reduce_spl;
-    const int tg = launch.reduce_tgroup;
+    const int chunk = launch.reduce_chunk;
     const int n = (int)inst.size();
 
     // Rollout tiles: instances in decreasing horizon so the longest tiles
@@ -199,7 +203,7 @@ int dr_engine::build_layout()
         cost_total += I.K;
         I.nom_off = nom_total;
         nom_total += I.T * 4;
-        I.n_chunks = (I.K + WAVG_BLOCK * spl - 1) / (WAVG_BLOCK * spl);
+        I.n_chunks = (I.K + chunk - 1) / chunk;
         I.part_off = part_total;
         part_total += (long long)I.n_chunks * I.T * 4;
         I.den_off = den_total;
@@ -219,8 +223,14 @@ int dr_engine::build_layout()
         const int i = order[oi];
         const InstDev& I = idev[i];
         for (int c = 0; c < I.n_chunks; ++c)
-            for (int t0 = 0; t0 < I.T; t0 += tg)
-                wtiles.push_back(WTile{i, c, t0, std::min(I.T, t0 + tg)});
+            wtiles.push_back(WTile{i, c});
+    }
+    max_T = 1;
+    opp_stride = 1;
+    for (const InstDev& I : idev)
+    {
+        max_T = std::max(max_T, I.T);
+        opp_stride = std::max(opp_stride, I.n_opp);
     }
 
     HIPCHK(d_inst.alloc(n));
@@ -242,11 +252,12 @@ int dr_engine::enqueue_step(bool advance, int timed_index)
     hipEvent_t* ev = (timed_index >= 0) ? &events[(size_t)timed_index * 4] : nullptr;
     if (ev)
         HIPCHK(hipEventRecord(ev[0], stream));
-    HIPCHK(launch_rollout(launch.rollout_block, launch.rollout_grid, stream, d_inst.p, d_dyn.p, d_rtiles.p, (int)rtiles.size(),
-                          d_tracks.p, d_drones.p, d_nominal.p, d_costs.p, d_noise.p, d_tmin.p, C, cost));
+    HIPCHK(launch_rollout(launch.rollout_block, launch.noise_batch, launch.rollout_grid, stream, d_inst.p, d_dyn.p, d_rtiles.p,
+                          (int)rtiles.size(), d_tracks.p, d_drones.p, d_nominal.p, d_costs.p, d_noise.p, d_tmin.p, C, Q, cost, max_T,
+                          opp_stride));
     if (ev)
         HIPCHK(hipEventRecord(ev[1], stream));
-    HIPCHK(launch_wavg(launch.reduce_spl, (int)wtiles.size(), stream, d_inst.p, d_dyn.p, d_wtiles.p, d_costs.p, d_noise.p, d_tmin.p,
+    HIPCHK(launch_wavg(launch.reduce_chunk, (int)wtiles.size(), stream, d_inst.p, d_dyn.p, d_wtiles.p, d_costs.p, d_noise.p, d_tmin.p,
                        d_part.p, d_den.p));
     if (ev)
         HIPCHK(hipEventRecord(ev[2], stream));
@@ -415,13 +426,14 @@ int dr_engine_create(const dr_quad_params* params, const dr_cost_cfg* cost, cons
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(DR_EHIP, "hipStreamCreate failed"));
     e->C = make_quad_const(*params);
+    e->Q = make_cost_const(*params, *cost);
     e->cost = *cost;
     e->tracks.assign(tracks, tracks + n_tracks);
     e->inst.assign(instances, instances + n_instances);
     e->drone_world.assign(drone_world, drone_world + n_drones);
     e->drone_inst = inst_of_drone;
     e->n_drones = n_drones;
-    e->launch = dr_launch_cfg{128, 0, 2, 50, 1, 1};
+    e->launch = dr_launch_cfg{128, 0, 256, 4, 1, 1};
 
     int rc = e->build_layout();
     if (rc)
@@ -641,19 +653,18 @@ int dr_engine_set_launch(dr_engine* e, const dr_launch_cfg* cfg)
     dr_launch_cfg c = *cfg;
     if (c.rollout_block == 0)
         c.rollout_block = 128;
-    if (c.reduce_spl == 0)
-        c.reduce_spl = 2;
-    if (c.reduce_tgroup == 0)
-        c.reduce_tgroup = 50;
+    if (c.reduce_chunk == 0)
+        c.reduce_chunk = 256;
+    if (c.noise_batch == 0)
+        c.noise_batch = 4;
     if (c.steps_per_graph == 0)
         c.steps_per_graph = 1;
-    if (!valid_block(c.rollout_block) || !valid_spl(c.reduce_spl) || c.reduce_tgroup < 1 || c.reduce_tgroup > WAVG_TG ||
-        c.rollout_grid < 0 || c.steps_per_graph < 1 || c.steps_per_graph > 64)
-        return fail(DR_EINVAL, "set_launch: rollout_block in {64,128,256}, reduce_spl in {1,2,4,8}, reduce_tgroup in [1,64], "
-                               "steps_per_graph in [1,64]");
+    if (!valid_block(c.rollout_block) || !valid_chunk(c.reduce_chunk) || !valid_nb(c.noise_batch) || c.rollout_grid < 0 ||
+        c.steps_per_graph < 1 || c.steps_per_graph > 64)
+        return fail(DR_EINVAL, "set_launch: rollout_block in {64,128,256}, reduce_chunk a multiple of 64 in [64,4096], "
+                               "noise_batch in {1,2,4}, steps_per_graph in [1,64]");
     HIPCHK(hipStreamSynchronize(e->stream));
-    const bool relayout = c.rollout_block != e->launch.rollout_block || c.reduce_spl != e->launch.reduce_spl ||
-                          c.reduce_tgroup != e->launch.reduce_tgroup;
+    const bool relayout = c.rollout_block != e->launch.rollout_block || c.reduce_chunk != e->launch.reduce_chunk;
     e->launch = c;
     e->drop_graphs();
     if (relayout)
@@ -714,24 +725,24 @@ int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* bes
             n_cu = prop.multiProcessorCount;
     }
     const int blocks[] = {64, 128, 256};
-    const int spls[] = {1, 2, 4};
-    const int tgs[] = {25, 50};
+    const int chunks[] = {128, 256, 512, 1024};
+    const int nbs[] = {2, 4};
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
     int rc = DR_OK;
     for (int rb : blocks)
         for (int grid_mult : {0, 4, 8})
-            for (int spl : spls)
-                for (int tg : tgs)
+            for (int ch : chunks)
+                for (int nb : nbs)
                 {
                     if (rc)
                         break;
                     dr_launch_cfg c = saved;
                     c.rollout_block = rb;
                     c.rollout_grid = grid_mult ? grid_mult * n_cu * (256 / rb) : 0;
-                    c.reduce_spl = spl;
-                    c.reduce_tgroup = tg;
+                    c.reduce_chunk = ch;
+                    c.noise_batch = nb;
                     c.use_graph = 1;
                     c.steps_per_graph = 1;
                     rc = dr_engine_set_launch(e, &c);

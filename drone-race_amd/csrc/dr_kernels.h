@@ -3,13 +3,14 @@
 // HBM layout of one batch (all instances of all worlds):
 //   nominal  float[sum_i T_i*4]            per instance, [t][d]
 //   costs    float[sum_i K_i]              per instance, [k]
-//   noise    float4[sum_i T_i*K_i]         per instance, [t][k] (one float4 = the 4 control dims)
+//   noise    float4[sum_i K_i*T_i]         per instance, [k][t] (one float4 = the 4 control dims)
 //   tmin     float[n_rollout_tiles]        per-rollout-workgroup minimum cost
 //   part     double[sum_i chunks_i*T_i*4]  weighted-average partial numerators, [chunk][t][d]
 //   den      double[sum_i chunks_i]        weighted-average partial denominators
 //   drones   dr_drone_state[n_drones]      world states (ego and opponents)
-// The noise is written once by the rollout in coalesced [t][k] rows and read
-// once by the weighted average; no per-sample RNG state exists.
+// The noise is written once by the rollout (each lane its own sample row) and
+// read once by the weighted average (lanes along t: each row is one coalesced
+// read); no per-sample RNG state exists.
 #pragma once
 
 #include <stdint.h>
@@ -21,7 +22,7 @@ namespace dr {
 constexpr int MAX_OPP = DR_MAX_WORLD_DRONES - 1;
 constexpr int WAVE = 64;
 constexpr int WAVG_BLOCK = 256;
-constexpr int WAVG_TG = 64; // max timesteps per weighted-average workgroup
+constexpr int WAVG_MAX_CHUNK = 4096; // max samples per weighted-average workgroup
 
 struct InstDev
 {
@@ -38,7 +39,7 @@ struct InstDev
     int rtile_first, rtile_count; // rollout tiles, contiguous in tile order
     long long part_off;           // double units
     int den_off;
-    int n_chunks; // weighted-average sample chunks (WAVG_BLOCK * spl samples each)
+    int n_chunks; // weighted-average sample chunks
 };
 
 struct InstDyn
@@ -61,7 +62,6 @@ struct WTile
 {
     int inst;
     int chunk;
-    int t0, t1;
 };
 
 } // namespace dr

@@ -34,19 +34,23 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // One lane = one sample k of one instance; lane k == K rolls the nominal out
 // without noise (the plan's own cost, reported as traj_cost).  Per step:
-// Philox noise (4 normals) -> float4 store of sigma*z into noise[t][k] ->
-// clamp(nominal[t] + eps) -> Euler step -> gate update -> running cost.
-// The nominal, the track and the opponents' start states sit in LDS.
-template <int BLOCK>
+// Philox noise (4 normals) -> float4 store of sigma*z into the sample's noise
+// row -> clamp(nominal[t] + eps) -> Euler step -> gate update -> running
+// cost.  Noise is generated NB steps at a time: the NB Philox/Box-Muller
+// chains are independent, so the scheduler interleaves them, which hides the
+// quarter-rate multiplies and transcendental latencies at the one or two
+// waves per SIMD that a single-batch rollout gives.  The nominal, the track
+// and the opponents' predicted positions (one row per step) sit in LDS.
+template <int BLOCK, int NB>
 __global__ __launch_bounds__(BLOCK) void k_rollout(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
                                                    const RTile* __restrict__ tiles, int n_tiles, const dr_track* __restrict__ tracks,
                                                    const dr_drone_state* __restrict__ drones, const float* __restrict__ nominal,
                                                    float* __restrict__ costs, float4* __restrict__ noise, float* __restrict__ tmin,
-                                                   QuadConst C, dr_cost_cfg KC)
+                                                   QuadConst C, CostConst Q, dr_cost_cfg KC, int opp_stride)
 {
+    extern __shared__ float4 s_opp[]; // [T][opp_stride]
     __shared__ float s_nom[DR_MAX_T * 4];
     __shared__ dr_track s_track;
-    __shared__ dr_opponent s_opp[MAX_OPP];
     __shared__ float s_min[BLOCK / WAVE];
 
     const int tid = threadIdx.x;
@@ -67,14 +71,10 @@ __global__ __launch_bounds__(BLOCK) void k_rollout(const InstDev* __restrict__ i
             for (int i = tid; i < (int)(sizeof(dr_track) / 4); i += BLOCK)
                 dst[i] = src[i];
         }
-        if (tid < n_opp)
+        for (int i = tid; i < T * n_opp; i += BLOCK)
         {
-            const dr_drone_state& o = drones[I.opp[tid]];
-            for (int d = 0; d < 3; ++d)
-            {
-                s_opp[tid].p[d] = o.p[d];
-                s_opp[tid].v[d] = o.v[d];
-            }
+            const int t = i / n_opp, j = i - t * n_opp;
+            s_opp[t * opp_stride + j] = opp_predict(drones[I.opp[j]], C.dt * (float)(t + 1));
         }
         __syncthreads();
 
@@ -86,37 +86,47 @@ __global__ __launch_bounds__(BLOCK) void k_rollout(const InstDev* __restrict__ i
             QState s = state_from_abi(drones[I.drone]);
             float acc = 0.0f, decay = 1.0f;
             bool stop = false;
-            float4* nrow = noise + I.noise_off + k;
-            for (int t = 0; t < T; ++t)
+            float4* nrow = noise + I.noise_off + (long long)k * T;
+            for (int t0 = 0; t0 < T; t0 += NB)
             {
-                float e[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-                if (!eval)
+                float e[NB][4];
+#pragma unroll
+                for (int u = 0; u < NB; ++u)
                 {
                     float z[4];
-                    normal4(I.key0, I.key1, (uint32_t)k, (uint32_t)t, step, STREAM_CONTROL, z);
-                    e[0] = I.sigma[0] * z[0];
-                    e[1] = I.sigma[1] * z[1];
-                    e[2] = I.sigma[2] * z[2];
-                    e[3] = I.sigma[3] * z[3];
-                    nrow[(long long)t * K] = make_float4(e[0], e[1], e[2], e[3]);
+                    normal4(I.key0, I.key1, (uint32_t)k, (uint32_t)(t0 + u), step, STREAM_CONTROL, z);
+#pragma unroll
+                    for (int d = 0; d < 4; ++d)
+                        e[u][d] = eval ? 0.0f : I.sigma[d] * z[d];
                 }
-                if (!stop)
+                if (!eval)
                 {
-                    float uin[4], u[4];
-                    uin[0] = s_nom[t * 4 + 0] + e[0];
-                    uin[1] = s_nom[t * 4 + 1] + e[1];
-                    uin[2] = s_nom[t * 4 + 2] + e[2];
-                    uin[3] = s_nom[t * 4 + 3] + e[3];
-                    clamp_control(C, uin, u);
-                    const float p0[3] = {s.p[0], s.p[1], s.p[2]};
-                    quad_step(C, s, u);
-                    update_gate(s_track.gates, s_track.n_gates, KC.gate_traversal_margin, p0, s);
-                    const float tau = C.dt * (float)(t + 1);
-                    int term;
-                    const float c = stage_cost(C, KC, s_track, s, s_opp, n_opp, tau, &term);
-                    acc = acc + c * decay;
-                    decay = decay * DECAY;
-                    stop = (term != TERM_NONE);
+#pragma unroll
+                    for (int u = 0; u < NB; ++u)
+                        if (t0 + u < T)
+                            nrow[t0 + u] = make_float4(e[u][0], e[u][1], e[u][2], e[u][3]);
+                }
+#pragma unroll
+                for (int u = 0; u < NB; ++u)
+                {
+                    const int t = t0 + u;
+                    if (t < T && !stop)
+                    {
+                        float uin[4], uc[4];
+                        uin[0] = s_nom[t * 4 + 0] + e[u][0];
+                        uin[1] = s_nom[t * 4 + 1] + e[u][1];
+                        uin[2] = s_nom[t * 4 + 2] + e[u][2];
+                        uin[3] = s_nom[t * 4 + 3] + e[u][3];
+                        clamp_control(C, uin, uc);
+                        const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+                        quad_step(C, s, uc);
+                        update_gate(s_track.gates, s_track.n_gates, KC.gate_traversal_margin, p0, s);
+                        int term;
+                        const float c = stage_cost(C, Q, KC, s_track, s, s_opp + t * opp_stride, n_opp, &term);
+                        acc = acc + c * decay;
+                        decay = decay * DECAY;
+                        stop = (term != TERM_NONE);
+                    }
                 }
             }
             acc = acc + final_cost(KC, s_track, s);
@@ -142,18 +152,22 @@ __global__ __launch_bounds__(BLOCK) void k_rollout(const InstDev* __restrict__ i
     }
 }
 
-// Weighted average over one chunk of SPL*256 samples for timesteps [t0, t1):
-// w_k = exp(-(c_k - min)/lambda) (min over the instance's rollout tiles),
-// numerator[t][d] = sum_k w_k eps_k[t][d] and denominator sum_k w_k, both in
-// FP64 so that the summation order does not move the FP32 result.  Lanes run
-// along k: each [t][k] row is one coalesced float4 read per lane.
-template <int SPL>
+// Weighted average over one chunk of samples [k0, k0 + n):
+//   w_k = exp(-(c_k - min)/lambda), min over the instance's rollout tiles,
+//   numerator[t][d] = sum_k w_k eps_k[t][d], denominator = sum_k w_k,
+// both accumulated in FP64 so the summation order cannot move the FP32
+// result.  The chunk's weights are computed once into LDS; lane (t, sub)
+// then streams the float4 noise of step t of samples sub, sub+S, ... where
+// S = floor(256/T) lanes share each step: the lanes of one sample read one
+// contiguous T*16-byte row (coalesced) and no cross-lane reduction is needed
+// until the final S-way sum.
 __global__ __launch_bounds__(WAVG_BLOCK) void k_wavg(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
                                                      const WTile* __restrict__ wtiles, const float* __restrict__ costs,
                                                      const float4* __restrict__ noise, const float* __restrict__ tmin,
-                                                     double* __restrict__ part, double* __restrict__ den)
+                                                     double* __restrict__ part, double* __restrict__ den, int chunk)
 {
-    __shared__ double s_part[WAVG_BLOCK / WAVE][WAVG_TG][4];
+    __shared__ float s_w[WAVG_MAX_CHUNK];
+    __shared__ double s_acc[WAVG_BLOCK][4];
     __shared__ float s_min[WAVG_BLOCK / WAVE];
     __shared__ double s_den[WAVG_BLOCK / WAVE];
 
@@ -175,73 +189,60 @@ __global__ __launch_bounds__(WAVG_BLOCK) void k_wavg(const InstDev* __restrict__
         minC = fminf(minC, s_min[w]);
 
     const float lam = dyn[wt.inst].lambda;
-    float wk[SPL];
-    int kk[SPL];
+    const int k0 = wt.chunk * chunk;
+    const int n = min(chunk, K - k0);
     double dsum = 0.0;
-#pragma unroll
-    for (int r = 0; r < SPL; ++r)
-    {
-        kk[r] = wt.chunk * (WAVG_BLOCK * SPL) + r * WAVG_BLOCK + tid;
-        wk[r] = 0.0f;
-        if (kk[r] < K && __builtin_isfinite(minC))
-            wk[r] = expf_ieee(-(costs[I.cost_off + kk[r]] - minC) / lam);
-        dsum += (double)wk[r];
-    }
-
-    if (wt.t0 == 0)
-    {
-        dsum = wave_sum(dsum);
-        if (lane == 0)
-            s_den[wave] = dsum;
-        __syncthreads();
-        if (tid == 0)
-        {
-            double d = s_den[0];
-            for (int w = 1; w < WAVG_BLOCK / WAVE; ++w)
-                d += s_den[w];
-            den[I.den_off + wt.chunk] = d;
-            if (wt.chunk == 0)
-                dyn[wt.inst].min_cost = minC;
-        }
-    }
-
-    const float4* base = noise + I.noise_off;
-    for (int t = wt.t0; t < wt.t1; ++t)
-    {
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-#pragma unroll
-        for (int r = 0; r < SPL; ++r)
-        {
-            if (kk[r] < K)
-            {
-                const float4 e = base[(long long)t * K + kk[r]];
-                const double w = (double)wk[r];
-                a0 += w * (double)e.x;
-                a1 += w * (double)e.y;
-                a2 += w * (double)e.z;
-                a3 += w * (double)e.w;
-            }
-        }
-        a0 = wave_sum(a0);
-        a1 = wave_sum(a1);
-        a2 = wave_sum(a2);
-        a3 = wave_sum(a3);
-        if (lane == 0)
-        {
-            s_part[wave][t - wt.t0][0] = a0;
-            s_part[wave][t - wt.t0][1] = a1;
-            s_part[wave][t - wt.t0][2] = a2;
-            s_part[wave][t - wt.t0][3] = a3;
-        }
-    }
-    __syncthreads();
-    const int n = (wt.t1 - wt.t0) * 4;
-    double* out = part + I.part_off + ((long long)wt.chunk * T + wt.t0) * 4;
     for (int i = tid; i < n; i += WAVG_BLOCK)
     {
-        double v = s_part[0][i / 4][i & 3];
+        float w = 0.0f;
+        if (__builtin_isfinite(minC))
+            w = expf_ieee(-(costs[I.cost_off + k0 + i] - minC) / lam);
+        s_w[i] = w;
+        dsum += (double)w;
+    }
+    dsum = wave_sum(dsum);
+    if (lane == 0)
+        s_den[wave] = dsum;
+    __syncthreads();
+    if (tid == 0)
+    {
+        double d = s_den[0];
         for (int w = 1; w < WAVG_BLOCK / WAVE; ++w)
-            v += s_part[w][i / 4][i & 3];
+            d += s_den[w];
+        den[I.den_off + wt.chunk] = d;
+        if (wt.chunk == 0)
+            dyn[wt.inst].min_cost = minC;
+    }
+
+    const int S = WAVG_BLOCK / T;
+    const int sub = tid / T, t = tid - sub * T;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    if (sub < S)
+    {
+        const float4* base = noise + I.noise_off + (long long)k0 * T + t;
+#pragma unroll 8
+        for (int i = sub; i < n; i += S)
+        {
+            const float4 e = base[(long long)i * T];
+            const double w = (double)s_w[i];
+            a0 += w * (double)e.x;
+            a1 += w * (double)e.y;
+            a2 += w * (double)e.z;
+            a3 += w * (double)e.w;
+        }
+    }
+    s_acc[tid][0] = a0;
+    s_acc[tid][1] = a1;
+    s_acc[tid][2] = a2;
+    s_acc[tid][3] = a3;
+    __syncthreads();
+    double* out = part + I.part_off + (long long)wt.chunk * T * 4;
+    for (int i = tid; i < T * 4; i += WAVG_BLOCK)
+    {
+        const int tt = i >> 2, d = i & 3;
+        double v = s_acc[tt][d];
+        for (int q = 1; q < S; ++q)
+            v += s_acc[q * T + tt][d];
         out[i] = v;
     }
 }
@@ -342,21 +343,47 @@ __global__ void k_advance(dr_drone_state* __restrict__ drones, int n_drones, con
 // ---------------------------------------------------------------------------
 // launch wrappers (host)
 
-hipError_t launch_rollout(int block, int grid, hipStream_t st, const InstDev* insts, InstDyn* dyn, const RTile* tiles, int n_tiles,
-                          const dr_track* tracks, const dr_drone_state* drones, const float* nominal, float* costs, float4* noise,
-                          float* tmin, const QuadConst& C, const dr_cost_cfg& KC)
+template <int BLOCK>
+static void rollout_nb(int nb, int g, size_t shm, hipStream_t st, const InstDev* insts, InstDyn* dyn, const RTile* tiles, int n_tiles,
+                       const dr_track* tracks, const dr_drone_state* drones, const float* nominal, float* costs, float4* noise,
+                       float* tmin, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, int opp_stride)
 {
+    switch (nb)
+    {
+    case 1:
+        k_rollout<BLOCK, 1><<<g, BLOCK, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                   opp_stride);
+        break;
+    case 2:
+        k_rollout<BLOCK, 2><<<g, BLOCK, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                   opp_stride);
+        break;
+    default:
+        k_rollout<BLOCK, 4><<<g, BLOCK, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                   opp_stride);
+        break;
+    }
+}
+
+hipError_t launch_rollout(int block, int nb, int grid, hipStream_t st, const InstDev* insts, InstDyn* dyn, const RTile* tiles,
+                          int n_tiles, const dr_track* tracks, const dr_drone_state* drones, const float* nominal, float* costs,
+                          float4* noise, float* tmin, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, int max_T,
+                          int opp_stride)
+{
+    if (nb != 1 && nb != 2 && nb != 4)
+        return hipErrorInvalidValue;
     const int g = grid > 0 ? (grid < n_tiles ? grid : n_tiles) : n_tiles;
+    const size_t shm = (size_t)max_T * opp_stride * sizeof(float4);
     switch (block)
     {
     case 64:
-        k_rollout<64><<<g, 64, 0, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, KC);
+        rollout_nb<64>(nb, g, shm, st, insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC, opp_stride);
         break;
     case 128:
-        k_rollout<128><<<g, 128, 0, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, KC);
+        rollout_nb<128>(nb, g, shm, st, insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC, opp_stride);
         break;
     case 256:
-        k_rollout<256><<<g, 256, 0, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, KC);
+        rollout_nb<256>(nb, g, shm, st, insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC, opp_stride);
         break;
     default:
         return hipErrorInvalidValue;
@@ -364,26 +391,12 @@ hipError_t launch_rollout(int block, int grid, hipStream_t st, const InstDev* in
     return hipGetLastError();
 }
 
-hipError_t launch_wavg(int spl, int n_wtiles, hipStream_t st, const InstDev* insts, InstDyn* dyn, const WTile* wtiles,
+hipError_t launch_wavg(int chunk, int n_wtiles, hipStream_t st, const InstDev* insts, InstDyn* dyn, const WTile* wtiles,
                        const float* costs, const float4* noise, const float* tmin, double* part, double* den)
 {
-    switch (spl)
-    {
-    case 1:
-        k_wavg<1><<<n_wtiles, WAVG_BLOCK, 0, st>>>(insts, dyn, wtiles, costs, noise, tmin, part, den);
-        break;
-    case 2:
-        k_wavg<2><<<n_wtiles, WAVG_BLOCK, 0, st>>>(insts, dyn, wtiles, costs, noise, tmin, part, den);
-        break;
-    case 4:
-        k_wavg<4><<<n_wtiles, WAVG_BLOCK, 0, st>>>(insts, dyn, wtiles, costs, noise, tmin, part, den);
-        break;
-    case 8:
-        k_wavg<8><<<n_wtiles, WAVG_BLOCK, 0, st>>>(insts, dyn, wtiles, costs, noise, tmin, part, den);
-        break;
-    default:
+    if (chunk < 1 || chunk > WAVG_MAX_CHUNK)
         return hipErrorInvalidValue;
-    }
+    k_wavg<<<n_wtiles, WAVG_BLOCK, 0, st>>>(insts, dyn, wtiles, costs, noise, tmin, part, den, chunk);
     return hipGetLastError();
 }
 

@@ -61,6 +61,28 @@ inline QuadConst make_quad_const(const dr_quad_params& P)
     return c;
 }
 
+// Cost constants derived once on the host (same float ops in the oracle):
+// reciprocals replace the divisions of the running cost, far2 is the squared
+// opponent distance beyond which the boundary distance cannot reach the
+// penalty band (no square root needed there).
+struct CostConst
+{
+    float r2_prox, inv_prox, danger, inv_danger, coll, far2;
+};
+
+inline CostConst make_cost_const(const dr_quad_params& P, const dr_cost_cfg& K)
+{
+    CostConst c;
+    c.r2_prox = K.opp_prox_radius * K.opp_prox_radius;
+    c.inv_prox = 1.0f / K.opp_prox_radius;
+    c.danger = K.boundary_threshold_factor * P.drone_radius;
+    c.inv_danger = 1.0f / c.danger;
+    c.coll = P.drone_radius * K.coll_dist_factor;
+    const float reach = (c.danger > c.coll ? c.danger : c.coll) + P.drone_radius;
+    c.far2 = reach * reach;
+    return c;
+}
+
 DR_HD float clampf(float x, float lo, float hi)
 {
     return fminf(fmaxf(x, lo), hi);
@@ -121,32 +143,36 @@ DR_HD void quad_step(const QuadConst& C, QState& s, const float u[4])
     s.w[2] = wz + C.rate_k * (u[3] - wz);
 }
 
-// Gate crossing test on the segment p0 -> s.p (env_dronerace.h:412-470):
-// lambda = n.(c - p0) / n.(p1 - p0); crossing if lambda in [0, 1] and the
-// crossing point lies within radius*margin of the centre; either direction.
+// Gate crossing test on the segment p0 -> p1 = s.p (env_dronerace.h:412-470),
+// written without the division: with num = n.(c - p0) and den = n.(p1 - p0),
+// lambda = num/den lies in [0, 1] iff num and den have the same sign and
+// |num| <= |den|, and den * (x - c) = den (p0 - c) + num (p1 - p0) for the
+// crossing point x, so the radius test is |.|^2 <= (radius*margin)^2 den^2.
+// Either crossing direction counts; |den| < 1e-10 (motion in the gate plane)
+// never counts.
 DR_HD void update_gate(const dr_gate* gates, int n_gates, float margin, const float p0[3], QState& s)
 {
     int next = s.gate + 1;
     if (next == n_gates)
         next = 0;
     const dr_gate& g = gates[next];
-    const float num = g.normal[0] * (g.center[0] - p0[0]) + g.normal[1] * (g.center[1] - p0[1]) + g.normal[2] * (g.center[2] - p0[2]);
-    const float den = g.normal[0] * (s.p[0] - p0[0]) + g.normal[1] * (s.p[1] - p0[1]) + g.normal[2] * (s.p[2] - p0[2]);
-    if (fabsf(den) < 1e-10f)
-        return;
-    const float lam = num / den;
-    if (lam < 0.0f || lam > 1.0f)
-        return;
-    const float om = 1.0f - lam;
-    const float dx = om * p0[0] + lam * s.p[0] - g.center[0];
-    const float dy = om * p0[1] + lam * s.p[1] - g.center[1];
-    const float dz = om * p0[2] + lam * s.p[2] - g.center[2];
-    const float rad = g.radius * g.radius * (margin * margin);
-    if (dx * dx + dy * dy + dz * dz <= rad)
+    const float ax = s.p[0] - p0[0], ay = s.p[1] - p0[1], az = s.p[2] - p0[2];
+    const float cx = g.center[0] - p0[0], cy = g.center[1] - p0[1], cz = g.center[2] - p0[2];
+    const float num = g.normal[0] * cx + g.normal[1] * cy + g.normal[2] * cz;
+    const float den = g.normal[0] * ax + g.normal[1] * ay + g.normal[2] * az;
+    const bool cand = (den > 0.0f) ? (num >= 0.0f && num <= den) : (num <= 0.0f && num >= den);
+    if (cand && fabsf(den) >= 1e-10f)
     {
-        s.gate = next;
-        if (next == 0)
-            s.laps = s.laps + 1;
+        const float wx = num * ax - den * cx;
+        const float wy = num * ay - den * cy;
+        const float wz = num * az - den * cz;
+        const float rm = g.radius * margin;
+        if (wx * wx + wy * wy + wz * wz <= (rm * rm) * (den * den))
+        {
+            s.gate = next;
+            if (next == 0)
+                s.laps = s.laps + 1;
+        }
     }
 }
 
@@ -172,38 +198,40 @@ DR_HD float advance(const dr_gate* gates, int n_gates, const QState& s)
 //   near-boundary       boundary_cost * (1 - bd/danger)^2 for bd < danger
 //   collision           outside_cost, terminal (TERM_LOSE), for bd < r*coll
 //   winner              -win_cost, terminal (TERM_WIN), for laps >= win laps
-// Opponent j is predicted at constant velocity: o_j = p_j + v_j * tau with
-// tau = dt * (t + 1) the time of the post-step state.
-DR_HD float stage_cost(const QuadConst& C, const dr_cost_cfg& K, const dr_track& tr, const QState& s, const dr_opponent* opp, int n_opp,
-                       float tau, int* term)
+// opp holds the opponents' predicted positions at this step (x, y, z, -).
+// An opponent farther than sqrt(far2) cannot bring bd below the penalty band,
+// so its distance is taken as +inf without a square root.
+DR_HD float stage_cost(const QuadConst& C, const CostConst& Q, const dr_cost_cfg& K, const dr_track& tr, const QState& s,
+                       const float4* opp, int n_opp, int* term)
 {
     float cost = 0.0f;
     float min_sq = __builtin_inff();
-    const float r2 = K.opp_prox_radius * K.opp_prox_radius;
     for (int j = 0; j < n_opp; ++j)
     {
-        const float dx = s.p[0] - (opp[j].p[0] + opp[j].v[0] * tau);
-        const float dy = s.p[1] - (opp[j].p[1] + opp[j].v[1] * tau);
-        const float dz = s.p[2] - (opp[j].p[2] + opp[j].v[2] * tau);
+        const float4 o = opp[j];
+        const float dx = s.p[0] - o.x;
+        const float dy = s.p[1] - o.y;
+        const float dz = s.p[2] - o.z;
         const float d2 = dx * dx + dy * dy + dz * dz;
         min_sq = fminf(min_sq, d2);
-        if (d2 < r2)
+        if (d2 < Q.r2_prox)
         {
-            const float x = 1.0f - __builtin_sqrtf(d2) / K.opp_prox_radius;
+            const float x = 1.0f - __builtin_sqrtf(d2) * Q.inv_prox;
             cost = cost + K.opp_prox_cost * (x * x);
         }
     }
-    const float d_opp = __builtin_sqrtf(min_sq) - C.drone_radius;
+    float d_opp = __builtin_inff();
+    if (min_sq < Q.far2)
+        d_opp = __builtin_sqrtf(min_sq) - C.drone_radius;
     const float d_ground = s.p[2] - tr.ground_z;
     const float bd = fminf(d_opp, d_ground);
-    const float danger = K.boundary_threshold_factor * C.drone_radius;
-    if (bd < danger)
+    if (bd < Q.danger)
     {
-        const float x = 1.0f - bd / danger;
+        const float x = 1.0f - bd * Q.inv_danger;
         cost = cost + K.boundary_cost * (x * x);
     }
     int t = TERM_NONE;
-    if (bd < C.drone_radius * K.coll_dist_factor)
+    if (bd < Q.coll)
     {
         cost = cost + K.outside_cost;
         t = TERM_LOSE;
@@ -215,6 +243,17 @@ DR_HD float stage_cost(const QuadConst& C, const dr_cost_cfg& K, const dr_track&
     }
     *term = t;
     return cost;
+}
+
+// Predicted opponent position after t+1 control steps: p + v * (dt * (t+1)).
+DR_HD float4 opp_predict(const dr_drone_state& o, float tau)
+{
+    float4 r;
+    r.x = o.p[0] + o.v[0] * tau;
+    r.y = o.p[1] + o.v[1] * tau;
+    r.z = o.p[2] + o.v[2] * tau;
+    r.w = 0.0f;
+    return r;
 }
 
 // Terminal cost (costs.cuh:31-39): -final_adv_weight * advance.

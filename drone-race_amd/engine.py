@@ -91,7 +91,7 @@ class Engine:
         ins = self.sc.instances[i]
         a = np.empty(ins.n_timesteps * ins.n_samples * 4, dtype=np.float32)
         check(self.lib.dr_engine_get_noise(self.h, i, _fptr(a), a.size))
-        return a.reshape(ins.n_timesteps, ins.n_samples, 4)
+        return a.reshape(ins.n_samples, ins.n_timesteps, 4)  # [k][t][d]
 
     def actions(self):
         a = np.empty(self.n_inst * 4, dtype=np.float32)

@@ -148,8 +148,8 @@ typedef struct dr_launch_cfg
 {
     int32_t rollout_block;    /* samples per rollout workgroup: 64, 128, 256 */
     int32_t rollout_grid;     /* 0: one workgroup per tile; >0 persistent grid */
-    int32_t reduce_spl;       /* samples per lane in the weighted average: 1..8 */
-    int32_t reduce_tgroup;    /* timesteps per weighted-average workgroup */
+    int32_t reduce_chunk;     /* samples per weighted-average workgroup: 64..4096, multiple of 64 */
+    int32_t noise_batch;      /* rollout steps whose noise is generated together: 1, 2, 4 */
     int32_t steps_per_graph;  /* control steps captured per hipGraph */
     int32_t use_graph;        /* 0: plain stream launches */
 } dr_launch_cfg;
@@ -184,7 +184,7 @@ int dr_engine_synchronize(dr_engine* e);
 
 /* Results of the last control step, for tests and telemetry. */
 int dr_engine_get_costs(dr_engine* e, int instance, float* costs, int n);
-int dr_engine_get_noise(dr_engine* e, int instance, float* noise, size_t n); /* layout [t][k][4] */
+int dr_engine_get_noise(dr_engine* e, int instance, float* noise, size_t n); /* layout [k][t][4] */
 int dr_engine_get_actions(dr_engine* e, float* actions, int n_floats);
 int dr_engine_get_diag(dr_engine* e, dr_diag* diag, int n_instances);
 

@@ -49,12 +49,13 @@ St<S> arr_to_state(const double* a, int gate, int laps)
 // update, running cost * decay (decay *= 0.97), then the terminal cost.  eps
 // is NULL for the noise-free rollout of the nominal.
 template <class S>
-S rollout_one(const Model<S>& M, const dr_cost_cfg& K, const dr_track& tr, const dr_drone_state& ego, const dr_opponent* opp, int n_opp,
+S rollout_one(const dr_quad_params& P, const Model<S>& M, const dr_cost_cfg& K, const dr_track& tr, const dr_drone_state& ego, const dr_opponent* opp, int n_opp,
               const float* nominal, int T, const float* eps, long long eps_stride)
 {
     St<S> s = load_state<S>(ego);
     S acc = 0, decay = 1;
     const S margin = (S)K.gate_traversal_margin;
+    const CostK<S> Q(P, K);
     for (int t = 0; t < T; ++t)
     {
         S uin[4], u[4];
@@ -72,7 +73,7 @@ S rollout_one(const Model<S>& M, const dr_cost_cfg& K, const dr_track& tr, const
         gate_update(tr, margin, p0, s);
         int term = 0;
         const S tau = (S)(M.dt * (S)(t + 1));
-        const S c = running_cost(M, K, tr, s, opp, n_opp, tau, &term);
+        const S c = running_cost(M, Q, K, tr, s, opp, n_opp, tau, &term);
         acc = acc + c * decay;
         decay = decay * (S)0.97f;
         if (term)
@@ -88,7 +89,7 @@ void rollout_all(const dr_quad_params& P, const dr_cost_cfg& K, const dr_track& 
 {
     const Model<S> M(P);
     if (traj_cost)
-        *traj_cost = (double)rollout_one<S>(M, K, tr, ego, opp, n_opp, nominal, T, nullptr, 0);
+        *traj_cost = (double)rollout_one<S>(P, M, K, tr, ego, opp, n_opp, nominal, T, nullptr, 0);
 #pragma omp parallel num_threads(n_threads > 0 ? n_threads : 1)
     {
         std::vector<float> eps((size_t)T * 4);
@@ -103,9 +104,9 @@ void rollout_all(const dr_quad_params& P, const dr_cost_cfg& K, const dr_track& 
                     eps[(size_t)t * 4 + d] = sigma[d] * z[d];
                 if (noise_out)
                     for (int d = 0; d < 4; ++d)
-                        noise_out[((size_t)t * n_samples + k) * 4 + d] = eps[(size_t)t * 4 + d];
+                        noise_out[((size_t)k * T + t) * 4 + d] = eps[(size_t)t * 4 + d];
             }
-            costs[k] = (double)rollout_one<S>(M, K, tr, ego, opp, n_opp, nominal, T, eps.data(), 4);
+            costs[k] = (double)rollout_one<S>(P, M, K, tr, ego, opp, n_opp, nominal, T, eps.data(), 4);
         }
     }
 }
@@ -140,7 +141,7 @@ void update(const dr_quad_params& P, const double* costs, const float* noise, co
         {
             double num = 0.0;
             for (int k = 0; k < n_samples; ++k)
-                num += w[k] * (double)noise[((size_t)t * n_samples + k) * 4 + d];
+                num += w[k] * (double)noise[((size_t)k * T + t) * 4 + d];
             S v = (S)nominal[t * 4 + d];
             if (den > 1e-30)
             {
@@ -233,12 +234,14 @@ double or_running_cost(int f64, const dr_quad_params* P, const dr_cost_cfg* K, c
                        const dr_opponent* opp, int n_opp, double tau, int* term)
 {
     if (f64)
-        return running_cost(Model<double>(*P), *K, *tr, arr_to_state<double>(x, gate, laps), opp, n_opp, tau, term);
-    return (double)running_cost(Model<float>(*P), *K, *tr, arr_to_state<float>(x, gate, laps), opp, n_opp, (float)tau, term);
+        return running_cost(Model<double>(*P), CostK<double>(*P, *K), *K, *tr, arr_to_state<double>(x, gate, laps), opp, n_opp, tau,
+                            term);
+    return (double)running_cost(Model<float>(*P), CostK<float>(*P, *K), *K, *tr, arr_to_state<float>(x, gate, laps), opp, n_opp,
+                                (float)tau, term);
 }
 
 // K sampled rollouts of one controller.  costs[K] (FP32 values widened in
-// FP32 mode); noise_out [t][k][4] (may be NULL); traj_cost = noise-free
+// FP32 mode); noise_out [k][t][4] (may be NULL); traj_cost = noise-free
 // rollout of the nominal (may be NULL).
 int or_rollout(int f64, const dr_quad_params* P, const dr_cost_cfg* K, const dr_track* tr, const dr_drone_state* ego, const dr_opponent* opp,
                int n_opp, const float* nominal, int n_samples, int T, uint32_t key0, uint32_t key1, uint32_t step, const float* sigma,
@@ -393,7 +396,7 @@ int or_count_rollout_ops(const dr_quad_params* P, const dr_cost_cfg* K, const dr
             for (int d = 0; d < 4; ++d)
                 eps[(size_t)t * 4 + d] = sigma[d] * z[d];
         }
-        (void)rollout_one<Cn>(M, *K, *tr, *ego, opp, n_opp, nominal, T, eps.data(), 4);
+        (void)rollout_one<Cn>(*P, M, *K, *tr, *ego, opp, n_opp, nominal, T, eps.data(), 4);
     }
     *flops_out = g_tally.flops;
     return 0;

@@ -278,36 +278,36 @@ inline void euler(const Model<S>& M, St<S>& s, const S u[4])
     s.w[2] = wz + M.rate_k * (u[3] - wz);
 }
 
-inline bool tiny(float x) { return fabsf(x) < 1e-10f; }
-inline bool tiny(double x) { return fabs(x) < 1e-10; }
+// env_dronerace.h:412-470 in 3-D, division-free: lambda = num/den in [0, 1]
+// iff num, den share a sign and |num| <= |den|; the crossing point x obeys
+// den (x - c) = num (p1 - p0) - den (c - p0), compared against
+// (radius*margin)^2 den^2.  |den| < 1e-10 never counts.
+inline bool tiny(float x) { return !(fabsf(x) >= 1e-10f); }
+inline bool tiny(double x) { return !(fabs(x) >= 1e-10); }
 
-// env_dronerace.h:412-470 in 3-D.
 template <class S>
 inline void gate_update(const dr_track& tr, S margin, const S p0[3], St<S>& s)
 {
     const int nxt = (s.gate + 1) % tr.n_gates;
     const dr_gate& g = tr.gates[nxt];
-    S c[3], n[3];
+    S a[3], c[3], n[3];
     for (int i = 0; i < 3; ++i)
     {
-        c[i] = (S)g.center[i];
+        a[i] = s.p[i] - p0[i];
+        c[i] = (S)g.center[i] - p0[i];
         n[i] = (S)g.normal[i];
     }
-    const S num = n[0] * (c[0] - p0[0]) + n[1] * (c[1] - p0[1]) + n[2] * (c[2] - p0[2]);
-    const S den = n[0] * (s.p[0] - p0[0]) + n[1] * (s.p[1] - p0[1]) + n[2] * (s.p[2] - p0[2]);
-    if (tiny(den))
+    const S num = n[0] * c[0] + n[1] * c[1] + n[2] * c[2];
+    const S den = n[0] * a[0] + n[1] * a[1] + n[2] * a[2];
+    const S zero = (S)0;
+    const bool cand = (den > zero) ? (!(num < zero) && num <= den) : (num <= zero && !(num < den));
+    if (!cand || tiny(den))
         return;
-    const S lam = num / den;
-    if (lam < (S)0 || lam > (S)1)
-        return;
-    const S om = (S)1 - lam;
-    S d2 = 0;
-    S d[3];
+    S w[3];
     for (int i = 0; i < 3; ++i)
-        d[i] = om * p0[i] + lam * s.p[i] - c[i];
-    d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    const S r = (S)g.radius;
-    if (d2 <= r * r * (margin * margin))
+        w[i] = num * a[i] - den * c[i];
+    const S rm = (S)g.radius * margin;
+    if (w[0] * w[0] + w[1] * w[1] + w[2] * w[2] <= (rm * rm) * (den * den))
     {
         s.gate = nxt;
         if (nxt == 0)
@@ -334,15 +334,37 @@ inline S progress(const dr_track& tr, const St<S>& s)
     return (S)s.gate + (S)(tr.n_gates * s.laps) + (S)0.8f * ((S)1 - sq_root(g2 / c2));
 }
 
-// costs.cuh:7-28 structure with the quadrotor's boundary distance.
+// Cost constants formed in FP32 exactly as the engine does on the host.
 template <class S>
-inline S running_cost(const Model<S>& M, const dr_cost_cfg& K, const dr_track& tr, const St<S>& s, const dr_opponent* opp, int n_opp,
-                      S tau, int* term)
+struct CostK
+{
+    S r2_prox, inv_prox, danger, inv_danger, coll, far2;
+    CostK(const dr_quad_params& P, const dr_cost_cfg& K)
+    {
+        const float r2 = K.opp_prox_radius * K.opp_prox_radius;
+        const float ip = 1.0f / K.opp_prox_radius;
+        const float dg = K.boundary_threshold_factor * P.drone_radius;
+        const float idg = 1.0f / dg;
+        const float cl = P.drone_radius * K.coll_dist_factor;
+        const float reach = (dg > cl ? dg : cl) + P.drone_radius;
+        r2_prox = (S)r2;
+        inv_prox = (S)ip;
+        danger = (S)dg;
+        inv_danger = (S)idg;
+        coll = (S)cl;
+        far2 = (S)(reach * reach);
+    }
+};
+
+// costs.cuh:7-28 structure with the quadrotor's boundary distance (see the
+// engine's stage_cost for the terms).  Opponents beyond sqrt(far2) cannot
+// reach the penalty band; their distance is +inf.
+template <class S>
+inline S running_cost(const Model<S>& M, const CostK<S>& Q, const dr_cost_cfg& K, const dr_track& tr, const St<S>& s,
+                      const dr_opponent* opp, int n_opp, S tau, int* term)
 {
     S cost = 0;
     S min_sq = (S)INFINITY;
-    const S rp = (S)K.opp_prox_radius;
-    const S r2 = rp * rp;
     for (int j = 0; j < n_opp; ++j)
     {
         S d[3];
@@ -350,23 +372,24 @@ inline S running_cost(const Model<S>& M, const dr_cost_cfg& K, const dr_track& t
             d[i] = s.p[i] - ((S)opp[j].p[i] + (S)opp[j].v[i] * tau);
         const S d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         min_sq = mn(min_sq, d2);
-        if (d2 < r2)
+        if (d2 < Q.r2_prox)
         {
-            const S x = (S)1 - sq_root(d2) / rp;
+            const S x = (S)1 - sq_root(d2) * Q.inv_prox;
             cost = cost + (S)K.opp_prox_cost * (x * x);
         }
     }
-    const S d_opp = sq_root(min_sq) - M.radius;
+    S d_opp = (S)INFINITY;
+    if (min_sq < Q.far2)
+        d_opp = sq_root(min_sq) - M.radius;
     const S d_ground = s.p[2] - (S)tr.ground_z;
     const S bd = mn(d_opp, d_ground);
-    const S danger = (S)(K.boundary_threshold_factor * M.radius);
-    if (bd < danger)
+    if (bd < Q.danger)
     {
-        const S x = (S)1 - bd / danger;
+        const S x = (S)1 - bd * Q.inv_danger;
         cost = cost + (S)K.boundary_cost * (x * x);
     }
     *term = 0;
-    if (bd < (S)(M.radius * K.coll_dist_factor))
+    if (bd < Q.coll)
     {
         cost = cost + (S)K.outside_cost;
         *term = 1;

@@ -120,11 +120,11 @@ class Oracle:
                                len(opponents), _f(nominal), K, T, seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF, step,
                                _f(sig), _d(costs), _f(noise) if noise is not None else None, C.byref(traj), threads)
         assert rc == 0
-        return costs, (noise.reshape(T, K, 4) if noise is not None else None), traj.value
+        return costs, (noise.reshape(K, T, 4) if noise is not None else None), traj.value
 
     def update(self, f64, params, costs, noise, nominal, lam, fill):
         K = costs.size
-        T = nominal.size // 4
+        T = nominal.size // 4  # noise layout [k][t][4]
         costs = np.ascontiguousarray(costs, dtype=np.float64)
         noise = np.ascontiguousarray(noise, dtype=np.float32).reshape(-1)
         nominal = np.ascontiguousarray(nominal, dtype=np.float32).reshape(-1)

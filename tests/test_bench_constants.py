@@ -1,0 +1,20 @@
+"""bench.py's algorithmic work figure equals what the oracle's op counter
+measures on the benchmark workload (so the roofline cannot drift from the code)."""
+import importlib.util
+import os
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _load(path, name):
+    spec = importlib.util.spec_from_file_location(name, path)
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def test_flops_per_sample_step_matches_counter(pkg):
+    bench = _load(os.path.join(ROOT, "bench.py"), "bench_mod")
+    count_ops = _load(os.path.join(ROOT, "oracle", "count_ops.py"), "count_ops_mod")
+    measured = count_ops.count(pkg.scenarios.c1_four_drones())
+    assert abs(bench.FLOPS_PER_SAMPLE_STEP - measured) < 0.01, measured

@@ -21,7 +21,7 @@ def brute_update(P, costs, noise, nominal, lam, fill):
     c = np.asarray(costs, dtype=np.float64)
     lam = float(np.float32(lam))  # the temperature is an FP32 field of the ABI
     w = np.exp(-(c - c.min()) / lam)
-    delta = np.einsum("k,tkd->td", w, noise.astype(np.float64)) / w.sum()
+    delta = np.einsum("k,ktd->td", w, noise.astype(np.float64)) / w.sum()
     u = nominal.astype(np.float64) + delta
     u[:, 0] = np.clip(u[:, 0], P.thrust_min, P.thrust_max)
     u[:, 1:] = np.clip(u[:, 1:], -P.rate_max, P.rate_max)
@@ -34,7 +34,7 @@ def test_update_matches_brute_force(oracle, setup):
     rng = np.random.default_rng(3)
     for Kn, T, lam in [(1, 1, 1.0), (7, 3, 1.0), (64, 10, 0.3), (200, 25, 5.0)]:
         costs = rng.normal(0, 20, Kn)
-        noise = rng.normal(0, 1, (T, Kn, 4)).astype(np.float32)
+        noise = rng.normal(0, 1, (Kn, T, 4)).astype(np.float32)
         nominal = (np.array(sc.hover_control(P)) + rng.normal(0, 0.5, (T, 4))).astype(np.float32)
         fill = np.array(sc.hover_control(P), dtype=np.float32)
         u, sh, w = brute_update(P, costs, noise, nominal, lam, fill)
@@ -54,7 +54,7 @@ def test_weight_invariants(oracle, setup):
     rng = np.random.default_rng(5)
     Kn, T = 500, 8
     costs = rng.normal(100, 30, Kn)
-    noise = rng.normal(0, 1, (T, Kn, 4)).astype(np.float32)
+    noise = rng.normal(0, 1, (Kn, T, 4)).astype(np.float32)
     nominal = np.tile(np.array(sc.hover_control(P), dtype=np.float32), (T, 1))
     r = oracle.update(1, P, costs, noise, nominal, 2.0, nominal[0])
     w = r["weights"]
@@ -69,18 +69,18 @@ def test_low_temperature_selects_best_sample(oracle, setup):
     rng = np.random.default_rng(9)
     Kn, T = 50, 6
     costs = rng.normal(0, 10, Kn)
-    noise = rng.normal(0, 0.1, (T, Kn, 4)).astype(np.float32)
+    noise = rng.normal(0, 0.1, (Kn, T, 4)).astype(np.float32)
     nominal = np.tile(np.array(sc.hover_control(P), dtype=np.float32), (T, 1))
     r = oracle.update(1, P, costs, noise, nominal, 1e-3, nominal[0])
     best = np.argmin(costs)
-    np.testing.assert_allclose(r["updated"], nominal + noise[:, best, :], atol=1e-9)
+    np.testing.assert_allclose(r["updated"], nominal + noise[best], atol=1e-9)
 
 
 def test_zero_fill_shift_as_reference(oracle, setup):
     sc, P, K, tr, ego = setup
     T, Kn = 5, 3
     nominal = np.arange(T * 4, dtype=np.float32).reshape(T, 4) * 0.1
-    noise = np.zeros((T, Kn, 4), dtype=np.float32)
+    noise = np.zeros((Kn, T, 4), dtype=np.float32)
     r = oracle.update(1, P, np.zeros(Kn), noise, nominal, 1.0, np.zeros(4, dtype=np.float32))
     np.testing.assert_allclose(r["shifted"][:-1], r["updated"][1:])
     assert np.all(r["shifted"][-1] == 0.0)
@@ -101,7 +101,7 @@ def test_rollout_loop_structure(oracle, setup):
         xs, g, l = x.copy(), ego.gate, ego.laps
         acc, decay = 0.0, 1.0
         for t in range(T):
-            u = nominal[t].astype(np.float64) + noise[t, k].astype(np.float64)
+            u = nominal[t].astype(np.float64) + noise[k, t].astype(np.float64)
             xs, g, l = oracle.quad_step(1, P, xs, u, track=tr, margin=K.gate_traversal_margin, gate=g, laps=l)
             c, term = oracle.running_cost(1, P, K, tr, xs, g, l, [], float(np.float32(P.dt)) * (t + 1))
             acc = acc + c * decay

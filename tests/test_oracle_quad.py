@@ -175,20 +175,20 @@ def test_running_cost_closed_forms(oracle, pkg, P):
     assert (c, t) == (0.0, 0)
     # one opponent at distance 1 (predicted there at tau = 0.5 from p - v tau)
     c, t = oracle.running_cost(1, P, K, tr, x, 0, 0, [opp((1.0, -0.5, 4), (0, 1.0, 0))], 0.5)
-    assert t == 0 and c == pytest.approx(K.opp_prox_cost * (1 - 1.0 / rp) ** 2, rel=1e-12)
+    assert t == 0 and c == pytest.approx(K.opp_prox_cost * (1 - 1.0 / rp) ** 2, rel=1e-6)
     # opponent inside the collision distance: proximity + near-boundary + outside, terminal
     d = 0.1
     c, t = oracle.running_cost(1, P, K, tr, x, 0, 0, [opp((d, 0, 4))], 0.0)
     bd = d - r
     expect = K.opp_prox_cost * (1 - d / rp) ** 2 + K.boundary_cost * (1 - bd / danger) ** 2 + K.outside_cost
-    assert t == 1 and c == pytest.approx(expect, rel=1e-9)
+    assert t == 1 and c == pytest.approx(expect, rel=1e-6)
     # near the ground, outside the collision distance: near-boundary only
     z = 0.2
     c, t = oracle.running_cost(1, P, K, tr, state(p=(0, 0, z)), 0, 0, [], 0.0)
-    assert t == 0 and c == pytest.approx(K.boundary_cost * (1 - z / danger) ** 2, rel=1e-9)
+    assert t == 0 and c == pytest.approx(K.boundary_cost * (1 - z / danger) ** 2, rel=1e-6)
     # below the collision distance from the ground
     c, t = oracle.running_cost(1, P, K, tr, state(p=(0, 0, 0.1)), 0, 0, [], 0.0)
-    assert t == 1 and c == pytest.approx(K.boundary_cost * (1 - 0.1 / danger) ** 2 + K.outside_cost, rel=1e-9)
+    assert t == 1 and c == pytest.approx(K.boundary_cost * (1 - 0.1 / danger) ** 2 + K.outside_cost, rel=1e-6)
     # winner
     c, t = oracle.running_cost(1, P, K, tr, x, 0, tr.n_win_laps, [], 0.0)
     assert t == 2 and c == -K.win_cost

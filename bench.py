@@ -75,6 +75,42 @@ def cpu_baseline(pkg, scn, budget_s):
             "sample_steps_per_s": n * scn.sample_steps_per_step / dt}
 
 
+def batch_vs_per_instance(pkg, steps=50, warmup=5):
+    """64 uneven controllers (K ~ logU[1024, 32768], T ~ U{30..150}): one
+    batched engine (one graph replay per control step of all 64) against 64
+    single-controller engines stepped one after the other with plain stream
+    launches and a synchronisation per controller, the way the reference's
+    getControl runs (mppi_controller.cu:250-466)."""
+    sc = pkg.scenarios
+    scn = sc.c2_batch64()
+    n = len(scn.instances)
+    eng = pkg.Engine(scn)
+    eng.step(warmup, advance=True)
+    t0 = time.perf_counter()
+    eng.step(steps, advance=True)
+    batched = n * steps / (time.perf_counter() - t0)
+    eng.close()
+    singles = []
+    for i, ins in enumerate(scn.instances):
+        ins1 = sc.instance(ins.n_samples, ins.n_timesteps, 0, 0, ins.seed)
+        one = sc.Scenario(f"c2_{i}", [scn.tracks[i]], [ins1], [0], (pkg.abi.DroneState * 1)(scn.drones[i]), 1, "")
+        e = pkg.Engine(one)
+        e.set_launch(use_graph=0)
+        singles.append(e)
+    for e in singles:
+        e.step(warmup, advance=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for e in singles:
+            e.step(1, advance=True)
+    per_instance = n * steps / (time.perf_counter() - t0)
+    for e in singles:
+        e.close()
+    return {"workload": scn.name, "batched_control_steps_per_s": batched,
+            "per_instance_control_steps_per_s": per_instance, "speedup": batched / per_instance,
+            "sample_steps_per_batch_step": scn.sample_steps_per_step, "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,7 +120,19 @@ def main():
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-batch64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 512 MiB buffer 3 times (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.empty(512 << 20, dtype=torch.uint8, device="cuda:0").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel(), "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (each controller batch runs on a "
               f"single MI355X; see DESIGN.md for how the path would shard)", file=sys.stderr)
@@ -163,6 +211,10 @@ def main():
     }
     if tuned:
         out["config"]["autotuned"] = True
+    if not args.no_batch64:
+        log("bench: 64 uneven controllers, batched vs per-instance ...")
+        out["batch64"] = batch_vs_per_instance(pkg)
+        log(f"bench: batch64 speedup {out['batch64']['speedup']:.2f}x")
     if not args.no_cpu:
         log("bench: CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(pkg, scn, args.cpu_budget)

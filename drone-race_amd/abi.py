@@ -60,7 +60,8 @@ class Diag(C.Structure):
 
 class LaunchCfg(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
-        "rollout_block", "rollout_grid", "reduce_chunk", "noise_batch", "steps_per_graph", "use_graph")]
+        "rollout_block", "rollout_grid", "reduce_chunk", "noise_batch", "steps_per_graph", "use_graph", "reduce_tgroup",
+        "rollout_split")]
 
 
 class MPPICfg(C.Structure):

@@ -50,3 +50,14 @@ def build(verbose=False):
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+
+
+def build_demo(out=None):
+    """Build tests/cpp/controller_demo against libdr_amd.so (C++ shim check)."""
+    root = os.path.dirname(HERE)
+    out = out or os.path.join(root, "tests", "cpp", "controller_demo")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "include"),
+           os.path.join(root, "tests", "cpp", "controller_demo.cpp"), "-o", out,
+           "-L", HERE, "-ldr_amd", "-Wl,-rpath," + HERE]
+    subprocess.run(cmd, check=True)
+    return out

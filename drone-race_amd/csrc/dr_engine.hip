@@ -20,16 +20,15 @@
 
 namespace dr {
 
-hipError_t launch_rollout(int block, int nb, int grid, hipStream_t st, const InstDev* insts, InstDyn* dyn, const RTile* tiles,
-                          int n_tiles, const dr_track* tracks, const dr_drone_state* drones, const float* nominal, float* costs,
-                          float4* noise, float* tmin, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, int max_T,
-                          int opp_stride);
+hipError_t launch_rollout(int block, int nb, int split, int grid, hipStream_t st, const InstDev* insts, InstDyn* dyn,
+                          const RTile* tiles, int n_tiles, const dr_track* tracks, const dr_drone_state* drones, const float* nominal,
+                          float* costs, float4* noise, float* tmin, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC,
+                          int max_T, int opp_stride);
 hipError_t launch_wavg(int chunk, int n_wtiles, hipStream_t st, const InstDev* insts, InstDyn* dyn, const WTile* wtiles,
                        const float* costs, const float4* noise, const float* tmin, double* part, double* den);
 hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
-                         float* nominal, float* actions, const QuadConst& C);
-hipError_t launch_advance(int n_drones, hipStream_t st, dr_drone_state* drones, const int* drone_inst, const InstDev* insts,
-                          const float* actions, const dr_track* tracks, const QuadConst& C, float margin);
+                         float* nominal, float* actions, const QuadConst& C, int advance, dr_drone_state* drones, const int* drone_inst,
+                         int n_drones, const dr_track* tracks, float margin);
 
 } // namespace dr
 
@@ -157,7 +156,7 @@ struct dr_engine
 
 int dr_engine::build_layout()
 {
-    const int rb = launch.rollout_block;
+    const int rb = launch.rollout_split ? WAVE : launch.rollout_block;
     const int chunk = launch.reduce_chunk;
     const int n = (int)inst.size();
 
@@ -222,8 +221,10 @@ int dr_engine::build_layout()
     {
         const int i = order[oi];
         const InstDev& I = idev[i];
+        const int tg = launch.reduce_tgroup;
         for (int c = 0; c < I.n_chunks; ++c)
-            wtiles.push_back(WTile{i, c});
+            for (int t0 = 0; t0 < I.T; t0 += tg)
+                wtiles.push_back(WTile{i, c, t0, std::min(I.T, t0 + tg)});
     }
     max_T = 1;
     opp_stride = 1;
@@ -252,19 +253,17 @@ int dr_engine::enqueue_step(bool advance, int timed_index)
     hipEvent_t* ev = (timed_index >= 0) ? &events[(size_t)timed_index * 4] : nullptr;
     if (ev)
         HIPCHK(hipEventRecord(ev[0], stream));
-    HIPCHK(launch_rollout(launch.rollout_block, launch.noise_batch, launch.rollout_grid, stream, d_inst.p, d_dyn.p, d_rtiles.p,
-                          (int)rtiles.size(), d_tracks.p, d_drones.p, d_nominal.p, d_costs.p, d_noise.p, d_tmin.p, C, Q, cost, max_T,
-                          opp_stride));
+    HIPCHK(launch_rollout(launch.rollout_block, launch.noise_batch, launch.rollout_split, launch.rollout_grid, stream, d_inst.p,
+                          d_dyn.p, d_rtiles.p, (int)rtiles.size(), d_tracks.p, d_drones.p, d_nominal.p, d_costs.p, d_noise.p, d_tmin.p, C, Q,
+                          cost, max_T, opp_stride));
     if (ev)
         HIPCHK(hipEventRecord(ev[1], stream));
     HIPCHK(launch_wavg(launch.reduce_chunk, (int)wtiles.size(), stream, d_inst.p, d_dyn.p, d_wtiles.p, d_costs.p, d_noise.p, d_tmin.p,
                        d_part.p, d_den.p));
     if (ev)
         HIPCHK(hipEventRecord(ev[2], stream));
-    HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C));
-    if (advance)
-        HIPCHK(launch_advance(n_drones, stream, d_drones.p, d_drone_inst.p, d_inst.p, d_actions.p, d_tracks.p, C,
-                              cost.gate_traversal_margin));
+    HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, advance ? 1 : 0, d_drones.p,
+                         d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin));
     if (ev)
         HIPCHK(hipEventRecord(ev[3], stream));
     return DR_OK;
@@ -433,7 +432,7 @@ int dr_engine_create(const dr_quad_params* params, const dr_cost_cfg* cost, cons
     e->drone_world.assign(drone_world, drone_world + n_drones);
     e->drone_inst = inst_of_drone;
     e->n_drones = n_drones;
-    e->launch = dr_launch_cfg{128, 0, 256, 4, 1, 1};
+    e->launch = dr_launch_cfg{128, 0, 1024, 1, 1, 1, 16, 1};
 
     int rc = e->build_layout();
     if (rc)
@@ -654,17 +653,21 @@ int dr_engine_set_launch(dr_engine* e, const dr_launch_cfg* cfg)
     if (c.rollout_block == 0)
         c.rollout_block = 128;
     if (c.reduce_chunk == 0)
-        c.reduce_chunk = 256;
+        c.reduce_chunk = 1024;
+    if (c.reduce_tgroup == 0)
+        c.reduce_tgroup = 16;
     if (c.noise_batch == 0)
-        c.noise_batch = 4;
+        c.noise_batch = 1;
     if (c.steps_per_graph == 0)
         c.steps_per_graph = 1;
     if (!valid_block(c.rollout_block) || !valid_chunk(c.reduce_chunk) || !valid_nb(c.noise_batch) || c.rollout_grid < 0 ||
-        c.steps_per_graph < 1 || c.steps_per_graph > 64)
+        c.steps_per_graph < 1 || c.steps_per_graph > 64 || c.reduce_tgroup < 1 || c.reduce_tgroup > DR_MAX_T ||
+        (c.rollout_split != 0 && c.rollout_split != 1))
         return fail(DR_EINVAL, "set_launch: rollout_block in {64,128,256}, reduce_chunk a multiple of 64 in [64,4096], "
-                               "noise_batch in {1,2,4}, steps_per_graph in [1,64]");
+                               "noise_batch in {1,2,4}, steps_per_graph in [1,64], reduce_tgroup in [1,256], rollout_split in {0,1}");
     HIPCHK(hipStreamSynchronize(e->stream));
-    const bool relayout = c.rollout_block != e->launch.rollout_block || c.reduce_chunk != e->launch.reduce_chunk;
+    const bool relayout = c.rollout_block != e->launch.rollout_block || c.reduce_chunk != e->launch.reduce_chunk ||
+                          c.reduce_tgroup != e->launch.reduce_tgroup || c.rollout_split != e->launch.rollout_split;
     e->launch = c;
     e->drop_graphs();
     if (relayout)
@@ -725,8 +728,8 @@ int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* bes
             n_cu = prop.multiProcessorCount;
     }
     const int blocks[] = {64, 128, 256};
-    const int chunks[] = {128, 256, 512, 1024};
-    const int nbs[] = {2, 4};
+    const int chunks[] = {512, 1024, 2048};
+    const int tgs[] = {16, 32, 64};
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
@@ -734,7 +737,7 @@ int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* bes
     for (int rb : blocks)
         for (int grid_mult : {0, 4, 8})
             for (int ch : chunks)
-                for (int nb : nbs)
+                for (int tg : tgs)
                 {
                     if (rc)
                         break;
@@ -742,7 +745,7 @@ int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* bes
                     c.rollout_block = rb;
                     c.rollout_grid = grid_mult ? grid_mult * n_cu * (256 / rb) : 0;
                     c.reduce_chunk = ch;
-                    c.noise_batch = nb;
+                    c.reduce_tgroup = tg;
                     c.use_graph = 1;
                     c.steps_per_graph = 1;
                     rc = dr_engine_set_launch(e, &c);

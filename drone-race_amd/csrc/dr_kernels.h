@@ -62,6 +62,7 @@ struct WTile
 {
     int inst;
     int chunk;
+    int t0, t1;
 };
 
 } // namespace dr

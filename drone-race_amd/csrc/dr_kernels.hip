@@ -1,5 +1,5 @@
 // Control-step kernels for gfx950 (wave64).  One control step of a batch is
-//   k_rollout  -> k_wavg -> k_update [-> k_advance]
+//   k_rollout (or k_rollout_ws) -> k_wavg -> k_update
 // covering every instance of the batch in each launch (tile tables built on
 // the host from the per-instance descriptors).
 //
@@ -9,7 +9,7 @@
 //   weightedAverageKernelUnified :307-408                    -> k_wavg + k_update
 //   clampNominalKernel :434-456 + host shift mppi_controller.cu:430-436 -> k_update
 // and the closed-loop environment step of SimulationEngine::dynStep
-// (src/engine.cpp:286-312)                                 -> k_advance
+// (src/engine.cpp:286-312)                                 -> k_update (advance)
 #include <hip/hip_runtime.h>
 
 #include "dr_kernels.h"
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(BLOCK) void k_rollout(const InstDev* __restrict__ i
                         update_gate(s_track.gates, s_track.n_gates, KC.gate_traversal_margin, p0, s);
                         int term;
                         const float c = stage_cost(C, Q, KC, s_track, s, s_opp + t * opp_stride, n_opp, &term);
-                        acc = acc + c * decay;
+                        acc = fma_(c, decay, acc);
                         decay = decay * DECAY;
                         stop = (term != TERM_NONE);
                     }
@@ -152,15 +152,153 @@ __global__ __launch_bounds__(BLOCK) void k_rollout(const InstDev* __restrict__ i
     }
 }
 
-// Weighted average over one chunk of samples [k0, k0 + n):
+// Warp-specialised rollout: a workgroup of two waves covers 64 samples.
+// Wave 0 (producer) generates the Philox/Box-Muller noise of a chunk of CH
+// steps into a double-buffered LDS ring and into the samples' noise rows in
+// HBM; wave 1 (consumer) integrates the dynamics and costs of the previous
+// chunk from the ring.  One barrier per chunk.  This gives every SIMD two
+// independent instruction streams when the batch has only one sample per
+// lane per SIMD (65536 samples over 1024 SIMDs), so dependent VALU chains
+// no longer stall the issue slot.  Arithmetic is identical to k_rollout.
+template <int CH>
+__global__ __launch_bounds__(128) void k_rollout_ws(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
+                                                    const RTile* __restrict__ tiles, int n_tiles, const dr_track* __restrict__ tracks,
+                                                    const dr_drone_state* __restrict__ drones, const float* __restrict__ nominal,
+                                                    float* __restrict__ costs, float4* __restrict__ noise, float* __restrict__ tmin,
+                                                    QuadConst C, CostConst Q, dr_cost_cfg KC, int opp_stride)
+{
+    extern __shared__ float4 s_opp[]; // [T][opp_stride]
+    __shared__ float4 s_eps[2][CH][WAVE];
+    __shared__ float s_nom[DR_MAX_T * 4];
+    __shared__ dr_track s_track;
+    __shared__ float s_min[2];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const bool producer = tid < WAVE;
+    for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
+    {
+        const RTile rt = tiles[tile];
+        const InstDev& I = insts[rt.inst];
+        const int K = I.K, T = I.T;
+        const uint32_t step = dyn[rt.inst].step;
+        const int n_opp = I.n_opp;
+        const uint32_t key0 = I.key0, key1 = I.key1;
+        const float sg0 = I.sigma[0], sg1 = I.sigma[1], sg2 = I.sigma[2], sg3 = I.sigma[3];
+
+        __syncthreads(); // LDS of the previous tile is no longer read
+        for (int i = tid; i < T * 4; i += 128)
+            s_nom[i] = nominal[I.nom_off + i];
+        {
+            const float* src = reinterpret_cast<const float*>(tracks + I.track);
+            float* dst = reinterpret_cast<float*>(&s_track);
+            for (int i = tid; i < (int)(sizeof(dr_track) / 4); i += 128)
+                dst[i] = src[i];
+        }
+        for (int i = tid; i < T * n_opp; i += 128)
+        {
+            const int t = i / n_opp, j = i - t * n_opp;
+            s_opp[t * opp_stride + j] = opp_predict(drones[I.opp[j]], C.dt * (float)(t + 1));
+        }
+        __syncthreads();
+
+        const int k = rt.k0 + lane;
+        const bool live = k <= K;
+        const bool eval = (k == K);
+        const int n_chunks = (T + CH - 1) / CH;
+        float4* nrow = noise + I.noise_off + (long long)k * T;
+
+        QState s;
+        float acc = 0.0f, decay = 1.0f;
+        bool stop = false;
+        if (!producer && live)
+            s = state_from_abi(drones[I.drone]);
+
+        for (int i = 0; i <= n_chunks; ++i)
+        {
+            if (producer)
+            {
+                if (i < n_chunks && live)
+                {
+                    float4* ring = &s_eps[i & 1][0][0];
+#pragma unroll
+                    for (int u = 0; u < CH; ++u)
+                    {
+                        const int t = i * CH + u;
+                        float z[4];
+                        normal4(key0, key1, (uint32_t)k, (uint32_t)t, step, STREAM_CONTROL, z);
+                        float4 e = make_float4(sg0 * z[0], sg1 * z[1], sg2 * z[2], sg3 * z[3]);
+                        if (eval)
+                            e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        ring[u * WAVE + lane] = e;
+                        if (!eval && t < T)
+                            nrow[t] = e;
+                    }
+                }
+            }
+            else if (i > 0 && live)
+            {
+                const int j = i - 1;
+                const float4* ring = &s_eps[j & 1][0][0];
+#pragma unroll
+                for (int u = 0; u < CH; ++u)
+                {
+                    const int t = j * CH + u;
+                    if (t < T && !stop)
+                    {
+                        const float4 e = ring[u * WAVE + lane];
+                        float uin[4], uc[4];
+                        uin[0] = s_nom[t * 4 + 0] + e.x;
+                        uin[1] = s_nom[t * 4 + 1] + e.y;
+                        uin[2] = s_nom[t * 4 + 2] + e.z;
+                        uin[3] = s_nom[t * 4 + 3] + e.w;
+                        clamp_control(C, uin, uc);
+                        const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+                        quad_step(C, s, uc);
+                        update_gate(s_track.gates, s_track.n_gates, KC.gate_traversal_margin, p0, s);
+                        int term;
+                        const float c = stage_cost(C, Q, KC, s_track, s, s_opp + t * opp_stride, n_opp, &term);
+                        acc = fma_(c, decay, acc);
+                        decay = decay * DECAY;
+                        stop = (term != TERM_NONE);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+        float cost = __builtin_inff();
+        if (!producer && live)
+        {
+            acc = acc + final_cost(KC, s_track, s);
+            if (eval)
+                dyn[rt.inst].traj_cost = acc;
+            else
+            {
+                costs[I.cost_off + k] = acc;
+                cost = acc;
+            }
+        }
+        const float wm = wave_min(cost);
+        if (lane == 0)
+            s_min[tid / WAVE] = wm;
+        __syncthreads();
+        if (tid == 0)
+            tmin[tile] = fminf(s_min[0], s_min[1]);
+    }
+}
+
+// Weighted average over one chunk of samples [k0, k0 + n) and timesteps
+// [t0, t1):
 //   w_k = exp(-(c_k - min)/lambda), min over the instance's rollout tiles,
 //   numerator[t][d] = sum_k w_k eps_k[t][d], denominator = sum_k w_k,
 // both accumulated in FP64 so the summation order cannot move the FP32
 // result.  The chunk's weights are computed once into LDS; lane (t, sub)
 // then streams the float4 noise of step t of samples sub, sub+S, ... where
-// S = floor(256/T) lanes share each step: the lanes of one sample read one
-// contiguous T*16-byte row (coalesced) and no cross-lane reduction is needed
-// until the final S-way sum.
+// S = floor(256/(t1-t0)) lanes share each step: the lanes of one sample read
+// one contiguous (t1-t0)*16-byte piece of its row (coalesced) and no
+// cross-lane reduction is needed until the final S-way sum.  Only the
+// workgroups with t0 == 0 write the chunk's denominator.
 __global__ __launch_bounds__(WAVG_BLOCK) void k_wavg(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
                                                      const WTile* __restrict__ wtiles, const float* __restrict__ costs,
                                                      const float4* __restrict__ noise, const float* __restrict__ tmin,
@@ -204,7 +342,7 @@ __global__ __launch_bounds__(WAVG_BLOCK) void k_wavg(const InstDev* __restrict__
     if (lane == 0)
         s_den[wave] = dsum;
     __syncthreads();
-    if (tid == 0)
+    if (tid == 0 && wt.t0 == 0)
     {
         double d = s_den[0];
         for (int w = 1; w < WAVG_BLOCK / WAVE; ++w)
@@ -214,12 +352,13 @@ __global__ __launch_bounds__(WAVG_BLOCK) void k_wavg(const InstDev* __restrict__
             dyn[wt.inst].min_cost = minC;
     }
 
-    const int S = WAVG_BLOCK / T;
-    const int sub = tid / T, t = tid - sub * T;
+    const int TG = wt.t1 - wt.t0;
+    const int S = WAVG_BLOCK / TG;
+    const int sub = tid / TG, tt = tid - sub * TG;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     if (sub < S)
     {
-        const float4* base = noise + I.noise_off + (long long)k0 * T + t;
+        const float4* base = noise + I.noise_off + (long long)k0 * T + wt.t0 + tt;
 #pragma unroll 8
         for (int i = sub; i < n; i += S)
         {
@@ -236,40 +375,70 @@ __global__ __launch_bounds__(WAVG_BLOCK) void k_wavg(const InstDev* __restrict__
     s_acc[tid][2] = a2;
     s_acc[tid][3] = a3;
     __syncthreads();
-    double* out = part + I.part_off + (long long)wt.chunk * T * 4;
-    for (int i = tid; i < T * 4; i += WAVG_BLOCK)
+    double* out = part + I.part_off + ((long long)wt.chunk * T + wt.t0) * 4;
+    for (int i = tid; i < TG * 4; i += WAVG_BLOCK)
     {
-        const int tt = i >> 2, d = i & 3;
-        double v = s_acc[tt][d];
+        const int r = i >> 2, d = i & 3;
+        double v = s_acc[r][d];
         for (int q = 1; q < S; ++q)
-            v += s_acc[q * T + tt][d];
+            v += s_acc[q * TG + r][d];
         out[i] = v;
     }
+}
+
+// Sum of n values p[0], p[stride], ... in index order; loads are issued in
+// batches of 8 so that their latencies overlap (same order as a plain loop).
+__device__ __forceinline__ double sum_in_order(const double* __restrict__ p, int n, long long stride)
+{
+    double acc = 0.0;
+    for (int c0 = 0; c0 < n; c0 += 8)
+    {
+        double v[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            v[b] = (c0 + b < n) ? p[(long long)(c0 + b) * stride] : 0.0;
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            acc += v[b];
+    }
+    return acc;
+}
+
+__device__ __forceinline__ void advance_drone(dr_drone_state& st, const float* act, const dr_track& tr, const QuadConst& C, float margin)
+{
+    QState s = state_from_abi(st);
+    float u[4];
+    clamp_control(C, act, u);
+    const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+    quad_step(C, s, u);
+    update_gate(tr.gates, tr.n_gates, margin, p0, s);
+    state_to_abi(s, st);
 }
 
 // Per instance: numerator and denominator over chunks in fixed order,
 // nominal += num/den when den > 1e-30 (kernels.cu:402-403), box clamp of the
 // whole nominal (kernels.cu:434-456), action = first control, shift left by
 // one step with shift_fill at the end (mppi_controller.cu:430-436), nu and
-// the optional temperature update (mppi_controller.cu:413-424).
+// the optional temperature update (mppi_controller.cu:413-424).  With
+// advance != 0 the closed loop follows (SimulationEngine::dynStep,
+// engine.cpp:286-312): the instance's drone integrates its action with the
+// same dynamics and gate logic, noise-free, and drones without a controller
+// (distributed over the workgroups) keep their velocity.
 __global__ __launch_bounds__(256) void k_update(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
                                                 const double* __restrict__ part, const double* __restrict__ den,
-                                                float* __restrict__ nominal, float* __restrict__ actions, QuadConst C)
+                                                float* __restrict__ nominal, float* __restrict__ actions, QuadConst C, int advance,
+                                                dr_drone_state* __restrict__ drones, const int* __restrict__ drone_inst, int n_drones,
+                                                const dr_track* __restrict__ tracks, float margin)
 {
     __shared__ float s_u[DR_MAX_T * 4];
     const int inst = blockIdx.x;
     const InstDev& I = insts[inst];
     const int T = I.T, tid = threadIdx.x;
 
-    double dn = 0.0;
-    for (int c = 0; c < I.n_chunks; ++c)
-        dn += den[I.den_off + c];
-
+    const double dn = sum_in_order(den + I.den_off, I.n_chunks, 1);
     for (int i = tid; i < T * 4; i += blockDim.x)
     {
-        double num = 0.0;
-        for (int c = 0; c < I.n_chunks; ++c)
-            num += part[I.part_off + (long long)c * T * 4 + i];
+        const double num = sum_in_order(part + I.part_off + i, I.n_chunks, (long long)T * 4);
         float u = nominal[I.nom_off + i];
         if (dn > 1e-30)
             u = u + (float)(num / dn);
@@ -305,39 +474,16 @@ __global__ __launch_bounds__(256) void k_update(const InstDev* __restrict__ inst
                 Y.lambda = Y.lambda * 1.2f;
         }
         Y.step = Y.step + 1u;
+        if (advance)
+        {
+            const float act[4] = {s_u[0], s_u[1], s_u[2], s_u[3]};
+            advance_drone(drones[I.drone], act, tracks[I.track], C, margin);
+            for (int d = inst; d < n_drones; d += gridDim.x)
+                if (drone_inst[d] < 0)
+                    for (int c = 0; c < 3; ++c)
+                        drones[d].p[c] = fma_(C.dt, drones[d].v[c], drones[d].p[c]);
+        }
     }
-}
-
-// Closed loop: every drone driven by an instance applies that instance's
-// action through the same dynamics and gate logic, noise-free; drones with no
-// controller keep their velocity.
-__global__ void k_advance(dr_drone_state* __restrict__ drones, int n_drones, const int* __restrict__ drone_inst,
-                          const InstDev* __restrict__ insts, const float* __restrict__ actions, const dr_track* __restrict__ tracks,
-                          QuadConst C, float margin)
-{
-    const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= n_drones)
-        return;
-    const int inst = drone_inst[d];
-    dr_drone_state st = drones[d];
-    if (inst < 0)
-    {
-        for (int i = 0; i < 3; ++i)
-            st.p[i] = st.p[i] + C.dt * st.v[i];
-    }
-    else
-    {
-        QState s = state_from_abi(st);
-        float in[4] = {actions[inst * 4 + 0], actions[inst * 4 + 1], actions[inst * 4 + 2], actions[inst * 4 + 3]};
-        float u[4];
-        clamp_control(C, in, u);
-        const float p0[3] = {s.p[0], s.p[1], s.p[2]};
-        quad_step(C, s, u);
-        const dr_track& tr = tracks[insts[inst].track];
-        update_gate(tr.gates, tr.n_gates, margin, p0, s);
-        state_to_abi(s, st);
-    }
-    drones[d] = st;
 }
 
 // ---------------------------------------------------------------------------
@@ -365,15 +511,35 @@ static void rollout_nb(int nb, int g, size_t shm, hipStream_t st, const InstDev*
     }
 }
 
-hipError_t launch_rollout(int block, int nb, int grid, hipStream_t st, const InstDev* insts, InstDyn* dyn, const RTile* tiles,
-                          int n_tiles, const dr_track* tracks, const dr_drone_state* drones, const float* nominal, float* costs,
-                          float4* noise, float* tmin, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, int max_T,
-                          int opp_stride)
+hipError_t launch_rollout(int block, int nb, int split, int grid, hipStream_t st, const InstDev* insts, InstDyn* dyn,
+                          const RTile* tiles, int n_tiles, const dr_track* tracks, const dr_drone_state* drones, const float* nominal,
+                          float* costs, float4* noise, float* tmin, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC,
+                          int max_T, int opp_stride)
 {
     if (nb != 1 && nb != 2 && nb != 4)
         return hipErrorInvalidValue;
     const int g = grid > 0 ? (grid < n_tiles ? grid : n_tiles) : n_tiles;
     const size_t shm = (size_t)max_T * opp_stride * sizeof(float4);
+    if (split)
+    {
+        // tiles of 64 samples; chunk length = 4 * noise_batch steps
+        switch (nb)
+        {
+        case 1:
+            k_rollout_ws<4><<<g, 128, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                 opp_stride);
+            break;
+        case 2:
+            k_rollout_ws<8><<<g, 128, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                 opp_stride);
+            break;
+        default:
+            k_rollout_ws<16><<<g, 128, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                  opp_stride);
+            break;
+        }
+        return hipGetLastError();
+    }
     switch (block)
     {
     case 64:
@@ -401,16 +567,10 @@ hipError_t launch_wavg(int chunk, int n_wtiles, hipStream_t st, const InstDev* i
 }
 
 hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
-                         float* nominal, float* actions, const QuadConst& C)
+                         float* nominal, float* actions, const QuadConst& C, int advance, dr_drone_state* drones, const int* drone_inst,
+                         int n_drones, const dr_track* tracks, float margin)
 {
-    k_update<<<n_inst, 256, 0, st>>>(insts, dyn, part, den, nominal, actions, C);
-    return hipGetLastError();
-}
-
-hipError_t launch_advance(int n_drones, hipStream_t st, dr_drone_state* drones, const int* drone_inst, const InstDev* insts,
-                          const float* actions, const dr_track* tracks, const QuadConst& C, float margin)
-{
-    k_advance<<<(n_drones + 63) / 64, 64, 0, st>>>(drones, n_drones, drone_inst, insts, actions, tracks, C, margin);
+    k_update<<<n_inst, 256, 0, st>>>(insts, dyn, part, den, nominal, actions, C, advance, drones, drone_inst, n_drones, tracks, margin);
     return hipGetLastError();
 }
 

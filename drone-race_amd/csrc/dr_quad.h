@@ -83,6 +83,11 @@ inline CostConst make_cost_const(const dr_quad_params& P, const dr_cost_cfg& K)
     return c;
 }
 
+DR_HD float fma_(float a, float b, float c)
+{
+    return __builtin_fmaf(a, b, c);
+}
+
 DR_HD float clampf(float x, float lo, float hi)
 {
     return fminf(fmaxf(x, lo), hi);
@@ -100,47 +105,48 @@ DR_HD void clamp_control(const QuadConst& C, const float in[4], float out[4])
 
 // One explicit Euler step.  Position uses the old velocity, velocity the old
 // attitude, attitude the old body rates; the quaternion is renormalised.
+// Products that feed a sum are fused (fmaf) in the order written.
 DR_HD void quad_step(const QuadConst& C, QState& s, const float u[4])
 {
     const float qw = s.q[0], qx = s.q[1], qy = s.q[2], qz = s.q[3];
     const float wx = s.w[0], wy = s.w[1], wz = s.w[2];
 
     // thrust direction R(q) e3
-    const float b3x = 2.0f * (qx * qz + qw * qy);
-    const float b3y = 2.0f * (qy * qz - qw * qx);
-    const float b3z = 1.0f - 2.0f * (qx * qx + qy * qy);
+    const float b3x = 2.0f * fma_(qx, qz, qw * qy);
+    const float b3y = 2.0f * fma_(qy, qz, -(qw * qx));
+    const float b3z = fma_(-2.0f, fma_(qx, qx, qy * qy), 1.0f);
     const float at = u[0] * C.inv_mass;
-    const float ax = at * b3x - C.drag * s.v[0];
-    const float ay = at * b3y - C.drag * s.v[1];
-    const float az = at * b3z - C.drag * s.v[2] - C.gravity;
+    const float ax = fma_(at, b3x, -(C.drag * s.v[0]));
+    const float ay = fma_(at, b3y, -(C.drag * s.v[1]));
+    const float az = fma_(at, b3z, -(C.drag * s.v[2])) - C.gravity;
 
-    s.p[0] = s.p[0] + C.dt * s.v[0];
-    s.p[1] = s.p[1] + C.dt * s.v[1];
-    s.p[2] = s.p[2] + C.dt * s.v[2];
+    s.p[0] = fma_(C.dt, s.v[0], s.p[0]);
+    s.p[1] = fma_(C.dt, s.v[1], s.p[1]);
+    s.p[2] = fma_(C.dt, s.v[2], s.p[2]);
 
-    s.v[0] = s.v[0] + C.dt * ax;
-    s.v[1] = s.v[1] + C.dt * ay;
-    s.v[2] = s.v[2] + C.dt * az;
+    s.v[0] = fma_(C.dt, ax, s.v[0]);
+    s.v[1] = fma_(C.dt, ay, s.v[1]);
+    s.v[2] = fma_(C.dt, az, s.v[2]);
 
     // q <- q + dt/2 * q (x) (0, w)
-    const float dqw = -(qx * wx + qy * wy + qz * wz);
-    const float dqx = qw * wx + qy * wz - qz * wy;
-    const float dqy = qw * wy - qx * wz + qz * wx;
-    const float dqz = qw * wz + qx * wy - qy * wx;
-    float nw = qw + C.half_dt * dqw;
-    float nx = qx + C.half_dt * dqx;
-    float ny = qy + C.half_dt * dqy;
-    float nz = qz + C.half_dt * dqz;
-    const float inv = 1.0f / __builtin_sqrtf(nw * nw + nx * nx + ny * ny + nz * nz);
+    const float dqw = -fma_(qx, wx, fma_(qy, wy, qz * wz));
+    const float dqx = fma_(qw, wx, fma_(qy, wz, -(qz * wy)));
+    const float dqy = fma_(qw, wy, fma_(qz, wx, -(qx * wz)));
+    const float dqz = fma_(qw, wz, fma_(qx, wy, -(qy * wx)));
+    const float nw = fma_(C.half_dt, dqw, qw);
+    const float nx = fma_(C.half_dt, dqx, qx);
+    const float ny = fma_(C.half_dt, dqy, qy);
+    const float nz = fma_(C.half_dt, dqz, qz);
+    const float inv = 1.0f / __builtin_sqrtf(fma_(nw, nw, fma_(nx, nx, fma_(ny, ny, nz * nz))));
     s.q[0] = nw * inv;
     s.q[1] = nx * inv;
     s.q[2] = ny * inv;
     s.q[3] = nz * inv;
 
     // first-order body-rate tracking
-    s.w[0] = wx + C.rate_k * (u[1] - wx);
-    s.w[1] = wy + C.rate_k * (u[2] - wy);
-    s.w[2] = wz + C.rate_k * (u[3] - wz);
+    s.w[0] = fma_(C.rate_k, u[1] - wx, wx);
+    s.w[1] = fma_(C.rate_k, u[2] - wy, wy);
+    s.w[2] = fma_(C.rate_k, u[3] - wz, wz);
 }
 
 // Gate crossing test on the segment p0 -> p1 = s.p (env_dronerace.h:412-470),
@@ -158,16 +164,16 @@ DR_HD void update_gate(const dr_gate* gates, int n_gates, float margin, const fl
     const dr_gate& g = gates[next];
     const float ax = s.p[0] - p0[0], ay = s.p[1] - p0[1], az = s.p[2] - p0[2];
     const float cx = g.center[0] - p0[0], cy = g.center[1] - p0[1], cz = g.center[2] - p0[2];
-    const float num = g.normal[0] * cx + g.normal[1] * cy + g.normal[2] * cz;
-    const float den = g.normal[0] * ax + g.normal[1] * ay + g.normal[2] * az;
+    const float num = fma_(g.normal[0], cx, fma_(g.normal[1], cy, g.normal[2] * cz));
+    const float den = fma_(g.normal[0], ax, fma_(g.normal[1], ay, g.normal[2] * az));
     const bool cand = (den > 0.0f) ? (num >= 0.0f && num <= den) : (num <= 0.0f && num >= den);
     if (cand && fabsf(den) >= 1e-10f)
     {
-        const float wx = num * ax - den * cx;
-        const float wy = num * ay - den * cy;
-        const float wz = num * az - den * cz;
+        const float wx = fma_(num, ax, -(den * cx));
+        const float wy = fma_(num, ay, -(den * cy));
+        const float wz = fma_(num, az, -(den * cz));
         const float rm = g.radius * margin;
-        if (wx * wx + wy * wy + wz * wz <= (rm * rm) * (den * den))
+        if (fma_(wx, wx, fma_(wy, wy, wz * wz)) <= (rm * rm) * (den * den))
         {
             s.gate = next;
             if (next == 0)
@@ -212,12 +218,12 @@ DR_HD float stage_cost(const QuadConst& C, const CostConst& Q, const dr_cost_cfg
         const float dx = s.p[0] - o.x;
         const float dy = s.p[1] - o.y;
         const float dz = s.p[2] - o.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
+        const float d2 = fma_(dx, dx, fma_(dy, dy, dz * dz));
         min_sq = fminf(min_sq, d2);
         if (d2 < Q.r2_prox)
         {
-            const float x = 1.0f - __builtin_sqrtf(d2) * Q.inv_prox;
-            cost = cost + K.opp_prox_cost * (x * x);
+            const float x = fma_(-__builtin_sqrtf(d2), Q.inv_prox, 1.0f);
+            cost = fma_(K.opp_prox_cost, x * x, cost);
         }
     }
     float d_opp = __builtin_inff();
@@ -227,8 +233,8 @@ DR_HD float stage_cost(const QuadConst& C, const CostConst& Q, const dr_cost_cfg
     const float bd = fminf(d_opp, d_ground);
     if (bd < Q.danger)
     {
-        const float x = 1.0f - bd * Q.inv_danger;
-        cost = cost + K.boundary_cost * (x * x);
+        const float x = fma_(-bd, Q.inv_danger, 1.0f);
+        cost = fma_(K.boundary_cost, x * x, cost);
     }
     int t = TERM_NONE;
     if (bd < Q.coll)
@@ -249,9 +255,9 @@ DR_HD float stage_cost(const QuadConst& C, const CostConst& Q, const dr_cost_cfg
 DR_HD float4 opp_predict(const dr_drone_state& o, float tau)
 {
     float4 r;
-    r.x = o.p[0] + o.v[0] * tau;
-    r.y = o.p[1] + o.v[1] * tau;
-    r.z = o.p[2] + o.v[2] * tau;
+    r.x = fma_(o.v[0], tau, o.p[0]);
+    r.y = fma_(o.v[1], tau, o.p[1]);
+    r.z = fma_(o.v[2], tau, o.p[2]);
     r.w = 0.0f;
     return r;
 }

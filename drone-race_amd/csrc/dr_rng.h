@@ -138,13 +138,13 @@ DR_HD void sincos_turns24(uint32_t k, float* s_out, float* c_out)
     pc = __builtin_fmaf(pc, a2, 4.16666679e-02f);                 // 1/4!
     pc = __builtin_fmaf(pc, a2, -0.5f);
     const float cs = __builtin_fmaf(a2, pc, 1.0f);
-    switch (qu & 3u)
-    {
-    case 0u: *s_out = sn; *c_out = cs; break;
-    case 1u: *s_out = cs; *c_out = -sn; break;
-    case 2u: *s_out = -sn; *c_out = -cs; break;
-    default: *s_out = -cs; *c_out = sn; break;
-    }
+    // rotate by the quarter turns: odd quarters swap (sin, cos) -> (cos, -sin),
+    // quarters 2 and 3 negate both (selects, no branches)
+    const bool odd = (qu & 1u) != 0u, half = (qu & 2u) != 0u;
+    const float s1 = odd ? cs : sn;
+    const float c1 = odd ? -sn : cs;
+    *s_out = half ? -s1 : s1;
+    *c_out = half ? -c1 : c1;
 }
 
 // Two standard normals from two 32-bit words (Box-Muller).

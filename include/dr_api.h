@@ -152,6 +152,8 @@ typedef struct dr_launch_cfg
     int32_t noise_batch;      /* rollout steps whose noise is generated together: 1, 2, 4 */
     int32_t steps_per_graph;  /* control steps captured per hipGraph */
     int32_t use_graph;        /* 0: plain stream launches */
+    int32_t reduce_tgroup;    /* timesteps per weighted-average workgroup: 1..256 */
+    int32_t rollout_split;    /* 1: producer/consumer waves (noise wave + dynamics wave per 64 samples) */
 } dr_launch_cfg;
 
 typedef struct dr_engine dr_engine;

@@ -74,7 +74,7 @@ S rollout_one(const dr_quad_params& P, const Model<S>& M, const dr_cost_cfg& K, 
         int term = 0;
         const S tau = (S)(M.dt * (S)(t + 1));
         const S c = running_cost(M, Q, K, tr, s, opp, n_opp, tau, &term);
-        acc = acc + c * decay;
+        acc = fm(c, decay, acc);
         decay = decay * (S)0.97f;
         if (term)
             break;
@@ -347,7 +347,7 @@ int or_world_run(const dr_quad_params* P, const dr_cost_cfg* K, const dr_track* 
             if (ctrl < 0)
             {
                 for (int c = 0; c < 3; ++c)
-                    drones[d].p[c] = drones[d].p[c] + M.dt * drones[d].v[c];
+                    drones[d].p[c] = fmaf(M.dt, drones[d].v[c], drones[d].p[c]);
                 continue;
             }
             St<float> s = load_state<float>(drones[d]);

@@ -177,6 +177,10 @@ inline float sq_root<float>(float x)
     return sqrtf(x);
 }
 
+// fused multiply-add, one rounding (the engine's fmaf)
+inline float fm(float a, float b, float c) { return fmaf(a, b, c); }
+inline double fm(double a, double b, double c) { return fma(a, b, c); }
+
 template <class S>
 inline S mn(S a, S b)
 {
@@ -256,26 +260,26 @@ inline void euler(const Model<S>& M, St<S>& s, const S u[4])
     const S wx = s.w[0], wy = s.w[1], wz = s.w[2];
     const S two = (S)2, one = (S)1;
     // R(q) e3
-    const S bx = two * (qx * qz + qw * qy);
-    const S by = two * (qy * qz - qw * qx);
-    const S bz = one - two * (qx * qx + qy * qy);
+    const S bx = two * fm(qx, qz, qw * qy);
+    const S by = two * fm(qy, qz, -(qw * qx));
+    const S bz = fm(-two, fm(qx, qx, qy * qy), one);
     const S at = u[0] * M.inv_mass;
-    const S acc[3] = {at * bx - M.drag * s.v[0], at * by - M.drag * s.v[1], at * bz - M.drag * s.v[2] - M.gravity};
+    const S acc[3] = {fm(at, bx, -(M.drag * s.v[0])), fm(at, by, -(M.drag * s.v[1])), fm(at, bz, -(M.drag * s.v[2])) - M.gravity};
     for (int i = 0; i < 3; ++i)
-        s.p[i] = s.p[i] + M.dt * s.v[i];
+        s.p[i] = fm(M.dt, s.v[i], s.p[i]);
     for (int i = 0; i < 3; ++i)
-        s.v[i] = s.v[i] + M.dt * acc[i];
-    const S dq[4] = {-(qx * wx + qy * wy + qz * wz), qw * wx + qy * wz - qz * wy, qw * wy - qx * wz + qz * wx,
-                     qw * wz + qx * wy - qy * wx};
+        s.v[i] = fm(M.dt, acc[i], s.v[i]);
+    const S dq[4] = {-fm(qx, wx, fm(qy, wy, qz * wz)), fm(qw, wx, fm(qy, wz, -(qz * wy))), fm(qw, wy, fm(qz, wx, -(qx * wz))),
+                     fm(qw, wz, fm(qx, wy, -(qy * wx)))};
     S n[4];
     for (int i = 0; i < 4; ++i)
-        n[i] = s.q[i] + M.half_dt * dq[i];
-    const S inv = one / sq_root(n[0] * n[0] + n[1] * n[1] + n[2] * n[2] + n[3] * n[3]);
+        n[i] = fm(M.half_dt, dq[i], s.q[i]);
+    const S inv = one / sq_root(fm(n[0], n[0], fm(n[1], n[1], fm(n[2], n[2], n[3] * n[3]))));
     for (int i = 0; i < 4; ++i)
         s.q[i] = n[i] * inv;
-    s.w[0] = wx + M.rate_k * (u[1] - wx);
-    s.w[1] = wy + M.rate_k * (u[2] - wy);
-    s.w[2] = wz + M.rate_k * (u[3] - wz);
+    s.w[0] = fm(M.rate_k, u[1] - wx, wx);
+    s.w[1] = fm(M.rate_k, u[2] - wy, wy);
+    s.w[2] = fm(M.rate_k, u[3] - wz, wz);
 }
 
 // env_dronerace.h:412-470 in 3-D, division-free: lambda = num/den in [0, 1]
@@ -297,17 +301,17 @@ inline void gate_update(const dr_track& tr, S margin, const S p0[3], St<S>& s)
         c[i] = (S)g.center[i] - p0[i];
         n[i] = (S)g.normal[i];
     }
-    const S num = n[0] * c[0] + n[1] * c[1] + n[2] * c[2];
-    const S den = n[0] * a[0] + n[1] * a[1] + n[2] * a[2];
+    const S num = fm(n[0], c[0], fm(n[1], c[1], n[2] * c[2]));
+    const S den = fm(n[0], a[0], fm(n[1], a[1], n[2] * a[2]));
     const S zero = (S)0;
     const bool cand = (den > zero) ? (!(num < zero) && num <= den) : (num <= zero && !(num < den));
     if (!cand || tiny(den))
         return;
     S w[3];
     for (int i = 0; i < 3; ++i)
-        w[i] = num * a[i] - den * c[i];
+        w[i] = fm(num, a[i], -(den * c[i]));
     const S rm = (S)g.radius * margin;
-    if (w[0] * w[0] + w[1] * w[1] + w[2] * w[2] <= (rm * rm) * (den * den))
+    if (fm(w[0], w[0], fm(w[1], w[1], w[2] * w[2])) <= (rm * rm) * (den * den))
     {
         s.gate = nxt;
         if (nxt == 0)
@@ -369,13 +373,13 @@ inline S running_cost(const Model<S>& M, const CostK<S>& Q, const dr_cost_cfg& K
     {
         S d[3];
         for (int i = 0; i < 3; ++i)
-            d[i] = s.p[i] - ((S)opp[j].p[i] + (S)opp[j].v[i] * tau);
-        const S d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            d[i] = s.p[i] - fm((S)opp[j].v[i], tau, (S)opp[j].p[i]);
+        const S d2 = fm(d[0], d[0], fm(d[1], d[1], d[2] * d[2]));
         min_sq = mn(min_sq, d2);
         if (d2 < Q.r2_prox)
         {
-            const S x = (S)1 - sq_root(d2) * Q.inv_prox;
-            cost = cost + (S)K.opp_prox_cost * (x * x);
+            const S x = fm(-sq_root(d2), Q.inv_prox, (S)1);
+            cost = fm((S)K.opp_prox_cost, x * x, cost);
         }
     }
     S d_opp = (S)INFINITY;
@@ -385,8 +389,8 @@ inline S running_cost(const Model<S>& M, const CostK<S>& Q, const dr_cost_cfg& K
     const S bd = mn(d_opp, d_ground);
     if (bd < Q.danger)
     {
-        const S x = (S)1 - bd * Q.inv_danger;
-        cost = cost + (S)K.boundary_cost * (x * x);
+        const S x = fm(-bd, Q.inv_danger, (S)1);
+        cost = fm((S)K.boundary_cost, x * x, cost);
     }
     *term = 0;
     if (bd < Q.coll)
@@ -433,6 +437,7 @@ inline bool operator>(Cn a, Cn b) { ++g_tally.flops; return a.v > b.v; }
 inline bool operator<=(Cn a, Cn b) { ++g_tally.flops; return a.v <= b.v; }
 template <>
 inline Cn sq_root<Cn>(Cn x) { ++g_tally.flops; return Cn(sqrt(x.v)); }
-inline bool tiny(Cn x) { ++g_tally.flops; return fabs(x.v) < 1e-10; }
+inline bool tiny(Cn x) { ++g_tally.flops; return !(fabs(x.v) >= 1e-10); }
+inline Cn fm(Cn a, Cn b, Cn c) { g_tally.flops += 2; return Cn(fma(a.v, b.v, c.v)); }
 
 } // namespace orc

@@ -175,9 +175,12 @@ def test_launch_shapes_agree(pkg):
     scn.instances[1] = sc.instance(1111, 70, 0, 1, 99)
     scn.instances[2] = sc.instance(257, 12, 0, 2, 98)
     ref = None
-    shapes = [dict(rollout_block=rb, rollout_grid=g, reduce_chunk=ch, noise_batch=nb, use_graph=ug, steps_per_graph=spg)
-              for rb, g, ch, nb, ug, spg in [(128, 0, 256, 4, 1, 1), (64, 0, 64, 1, 0, 1), (256, 7, 1024, 2, 1, 2),
-                                            (128, 3, 4096, 4, 1, 3), (64, 1, 192, 2, 0, 1)]]
+    shapes = [dict(rollout_block=rb, rollout_grid=g, reduce_chunk=ch, noise_batch=nb, use_graph=ug, steps_per_graph=spg,
+                   reduce_tgroup=tg, rollout_split=sp)
+              for rb, g, ch, nb, ug, spg, tg, sp in [(128, 0, 256, 4, 1, 1, 25, 0), (64, 0, 64, 1, 0, 1, 256, 0),
+                                                    (256, 7, 1024, 2, 1, 2, 7, 0), (128, 3, 4096, 4, 1, 3, 64, 0),
+                                                    (64, 1, 192, 2, 0, 1, 1, 0), (128, 0, 1024, 1, 1, 1, 25, 1),
+                                                    (128, 0, 512, 2, 0, 1, 16, 1), (128, 5, 1024, 4, 1, 2, 32, 1)]]
     for shape in shapes:
         eng = pkg.Engine(scn)
         eng.set_launch(**shape)

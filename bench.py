@@ -117,7 +117,8 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2"])
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--launch", default="", help="launch shape as JSON (skips the autotuner)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-batch64", action="store_true")
@@ -148,7 +149,9 @@ def main():
     torch.cuda.set_device(0)
     eng = pkg.Engine(scn)
     tuned = None
-    if args.autotune:
+    if args.launch:
+        eng.set_launch(**json.loads(args.launch))
+    elif not args.no_autotune:
         tuned, ms = eng.autotune(trials=5)
         log(f"bench: autotuned launch {tuned} ({ms:.3f} ms/step)")
     launch = {f: getattr(eng.get_launch(), f) for f, _ in pkg.abi.LaunchCfg._fields_}
@@ -173,7 +176,13 @@ def main():
     rollout_s = kt["rollout_ms"] / n_timed * 1e-3
     log(f"bench: per-step device time rollout {kt['rollout_ms'] / n_timed:.4f} ms, reduce {kt['reduce_ms'] / n_timed:.4f} ms, "
         f"update+advance {kt['update_ms'] / n_timed:.4f} ms")
-    diag = eng.diag()
+    # plan quality: noise-free cost of each controller's nominal, mean over 20 steps
+    eng.set_eval(True)
+    traj = []
+    for _ in range(20):
+        eng.step(1, advance=True)
+        traj.append(sum(d["traj_cost"] for d in eng.diag()) / n_inst)
+    eng.set_eval(False)
     eng.close()
 
     ss = scn.sample_steps_per_step
@@ -207,7 +216,7 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": "k_rollout", "flops_per_launch": flops,
                      "note": "FP32 vector (VALU) work, no MFMA; peak is the published FP32 figure"},
-        "mean_traj_cost": sum(d["traj_cost"] for d in diag) / n_inst,
+        "mean_traj_cost": sum(traj) / len(traj),
     }
     if tuned:
         out["config"]["autotuned"] = True

@@ -96,6 +96,7 @@ SIGNATURES = {
     "dr_engine_get_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
     "dr_engine_autotune": (_I, [_P, _I, C.POINTER(LaunchCfg), _F]),
     "dr_engine_set_timing": (_I, [_P, _I]),
+    "dr_engine_set_eval": (_I, [_P, _I]),
     "dr_engine_last_timing": (_I, [_P, _F, _F, _F, _F]),
     "dr_mppi_create": (_I, [C.POINTER(QuadParams), C.POINTER(CostCfg), C.POINTER(Track), C.POINTER(MPPICfg), C.c_uint64,
                             C.POINTER(_P)]),

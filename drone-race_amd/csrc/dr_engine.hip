@@ -24,6 +24,9 @@ hipError_t launch_rollout(int block, int nb, int split, int grid, hipStream_t st
                           const RTile* tiles, int n_tiles, const dr_track* tracks, const dr_drone_state* drones, const float* nominal,
                           float* costs, float4* noise, float* tmin, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC,
                           int max_T, int opp_stride);
+hipError_t launch_eval(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const dr_track* tracks,
+                       const dr_drone_state* drones, const float* nominal, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC,
+                       int max_T, int opp_stride);
 hipError_t launch_wavg(int chunk, int n_wtiles, hipStream_t st, const InstDev* insts, InstDyn* dyn, const WTile* wtiles,
                        const float* costs, const float4* noise, const float* tmin, double* part, double* den);
 hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
@@ -85,6 +88,9 @@ struct dr_engine
 {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr; // forked stream of the nominal's noise-free rollout
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool eval = false;
     QuadConst C{};
     CostConst Q{};
     dr_cost_cfg cost{};
@@ -134,6 +140,12 @@ struct dr_engine
         drop_graphs();
         for (auto ev : events)
             (void)hipEventDestroy(ev);
+        if (ev_fork)
+            (void)hipEventDestroy(ev_fork);
+        if (ev_join)
+            (void)hipEventDestroy(ev_join);
+        if (side)
+            (void)hipStreamDestroy(side);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
@@ -213,7 +225,7 @@ int dr_engine::build_layout()
         const int i = order[oi];
         InstDev& I = idev[i];
         I.rtile_first = (int)rtiles.size();
-        for (int k0 = 0; k0 <= I.K; k0 += rb)
+        for (int k0 = 0; k0 < I.K; k0 += rb)
             rtiles.push_back(RTile{i, k0});
         I.rtile_count = (int)rtiles.size() - I.rtile_first;
     }
@@ -253,6 +265,13 @@ int dr_engine::enqueue_step(bool advance, int timed_index)
     hipEvent_t* ev = (timed_index >= 0) ? &events[(size_t)timed_index * 4] : nullptr;
     if (ev)
         HIPCHK(hipEventRecord(ev[0], stream));
+    if (eval)
+    {
+        HIPCHK(hipEventRecord(ev_fork, stream));
+        HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
+        HIPCHK(launch_eval(n, side, d_inst.p, d_dyn.p, d_tracks.p, d_drones.p, d_nominal.p, C, Q, cost, max_T, opp_stride));
+        HIPCHK(hipEventRecord(ev_join, side));
+    }
     HIPCHK(launch_rollout(launch.rollout_block, launch.noise_batch, launch.rollout_split, launch.rollout_grid, stream, d_inst.p,
                           d_dyn.p, d_rtiles.p, (int)rtiles.size(), d_tracks.p, d_drones.p, d_nominal.p, d_costs.p, d_noise.p, d_tmin.p, C, Q,
                           cost, max_T, opp_stride));
@@ -262,6 +281,8 @@ int dr_engine::enqueue_step(bool advance, int timed_index)
                        d_part.p, d_den.p));
     if (ev)
         HIPCHK(hipEventRecord(ev[2], stream));
+    if (eval)
+        HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
     HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, advance ? 1 : 0, d_drones.p,
                          d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin));
     if (ev)
@@ -422,8 +443,11 @@ int dr_engine_create(const dr_quad_params* params, const dr_cost_cfg* cost, cons
     };
     if (hipGetDevice(&e->device) != hipSuccess)
         return bail(fail(DR_EHIP, "no HIP device"));
-    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(DR_EHIP, "hipStreamCreate failed"));
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(DR_EHIP, "stream or event creation failed"));
     e->C = make_quad_const(*params);
     e->Q = make_cost_const(*params, *cost);
     e->cost = *cost;
@@ -662,9 +686,9 @@ int dr_engine_set_launch(dr_engine* e, const dr_launch_cfg* cfg)
         c.steps_per_graph = 1;
     if (!valid_block(c.rollout_block) || !valid_chunk(c.reduce_chunk) || !valid_nb(c.noise_batch) || c.rollout_grid < 0 ||
         c.steps_per_graph < 1 || c.steps_per_graph > 64 || c.reduce_tgroup < 1 || c.reduce_tgroup > DR_MAX_T ||
-        (c.rollout_split != 0 && c.rollout_split != 1))
+        (c.rollout_split < 0 || c.rollout_split > 2))
         return fail(DR_EINVAL, "set_launch: rollout_block in {64,128,256}, reduce_chunk a multiple of 64 in [64,4096], "
-                               "noise_batch in {1,2,4}, steps_per_graph in [1,64], reduce_tgroup in [1,256], rollout_split in {0,1}");
+                               "noise_batch in {1,2,4}, steps_per_graph in [1,64], reduce_tgroup in [1,256], rollout_split in {0,1,2}");
     HIPCHK(hipStreamSynchronize(e->stream));
     const bool relayout = c.rollout_block != e->launch.rollout_block || c.reduce_chunk != e->launch.reduce_chunk ||
                           c.reduce_tgroup != e->launch.reduce_tgroup || c.rollout_split != e->launch.rollout_split;
@@ -688,6 +712,16 @@ int dr_engine_set_timing(dr_engine* e, int on)
     if (!e)
         return fail(DR_EINVAL, "NULL handle");
     e->timing = on != 0;
+    return DR_OK;
+}
+
+int dr_engine_set_eval(dr_engine* e, int on)
+{
+    if (!e)
+        return fail(DR_EINVAL, "NULL handle");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->eval = on != 0;
+    e->drop_graphs();
     return DR_OK;
 }
 
@@ -727,50 +761,59 @@ int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* bes
         if (hipGetDeviceProperties(&prop, e->device) == hipSuccess)
             n_cu = prop.multiProcessorCount;
     }
-    const int blocks[] = {64, 128, 256};
-    const int chunks[] = {512, 1024, 2048};
-    const int tgs[] = {16, 32, 64};
+    // Candidate shapes: the three rollout structures (one lane per sample,
+    // noise wave + dynamics wave, four-stage pipeline) x noise batch x
+    // weighted-average chunk x timestep group.
+    std::vector<dr_launch_cfg> cands;
+    for (int split : {0, 1, 2})
+        for (int nb : {1, 2, 4})
+            for (int ch : {512, 1024, 2048})
+                for (int tg : {16, 32})
+                    for (int rb : {128, 256})
+                    {
+                        if (split != 0 && rb != 128)
+                            continue;
+                        dr_launch_cfg c = saved;
+                        c.rollout_split = split;
+                        c.noise_batch = nb;
+                        c.reduce_chunk = ch;
+                        c.reduce_tgroup = tg;
+                        c.rollout_block = rb;
+                        c.rollout_grid = 0;
+                        c.use_graph = 1;
+                        c.steps_per_graph = 1;
+                        cands.push_back(c);
+                    }
+    (void)n_cu;
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
     int rc = DR_OK;
-    for (int rb : blocks)
-        for (int grid_mult : {0, 4, 8})
-            for (int ch : chunks)
-                for (int tg : tgs)
-                {
-                    if (rc)
-                        break;
-                    dr_launch_cfg c = saved;
-                    c.rollout_block = rb;
-                    c.rollout_grid = grid_mult ? grid_mult * n_cu * (256 / rb) : 0;
-                    c.reduce_chunk = ch;
-                    c.reduce_tgroup = tg;
-                    c.use_graph = 1;
-                    c.steps_per_graph = 1;
-                    rc = dr_engine_set_launch(e, &c);
-                    if (rc)
-                        break;
-                    rc = e->run(1, false); // warm-up and graph build
-                    if (rc)
-                        break;
-                    float ms = 0;
-                    if (hipEventRecord(a, e->stream) != hipSuccess)
-                        rc = fail(DR_EHIP, "autotune: hipEventRecord failed");
-                    if (!rc)
-                        rc = e->run(trials, false);
-                    if (!rc && (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
-                                hipEventElapsedTime(&ms, a, b) != hipSuccess))
-                        rc = fail(DR_EHIP, "autotune: event timing failed");
-                    if (rc)
-                        break;
-                    ms /= (float)trials;
-                    if (ms < best_t)
-                    {
-                        best_t = ms;
-                        best_cfg = c;
-                    }
-                }
+    for (const dr_launch_cfg& c : cands)
+    {
+        rc = dr_engine_set_launch(e, &c);
+        if (rc)
+            break;
+        rc = e->run(1, false); // warm-up and graph build
+        if (rc)
+            break;
+        float ms = 0;
+        if (hipEventRecord(a, e->stream) != hipSuccess)
+            rc = fail(DR_EHIP, "autotune: hipEventRecord failed");
+        if (!rc)
+            rc = e->run(trials, false);
+        if (!rc && (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                    hipEventElapsedTime(&ms, a, b) != hipSuccess))
+            rc = fail(DR_EHIP, "autotune: event timing failed");
+        if (rc)
+            break;
+        ms /= (float)trials;
+        if (ms < best_t)
+        {
+            best_t = ms;
+            best_cfg = c;
+        }
+    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (rc)

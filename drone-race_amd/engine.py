@@ -120,6 +120,9 @@ class Engine:
         check(self.lib.dr_engine_autotune(self.h, trials, C.byref(cfg), C.byref(ms)))
         return {f: getattr(cfg, f) for f, _ in abi.LaunchCfg._fields_}, ms.value
 
+    def set_eval(self, on):
+        check(self.lib.dr_engine_set_eval(self.h, 1 if on else 0))
+
     def set_timing(self, on):
         check(self.lib.dr_engine_set_timing(self.h, 1 if on else 0))
 

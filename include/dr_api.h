@@ -153,7 +153,7 @@ typedef struct dr_launch_cfg
     int32_t steps_per_graph;  /* control steps captured per hipGraph */
     int32_t use_graph;        /* 0: plain stream launches */
     int32_t reduce_tgroup;    /* timesteps per weighted-average workgroup: 1..256 */
-    int32_t rollout_split;    /* 1: producer/consumer waves (noise wave + dynamics wave per 64 samples) */
+    int32_t rollout_split;    /* 0: one lane per sample; 1: noise wave + dynamics wave; 2: four-stage wave pipeline */
 } dr_launch_cfg;
 
 typedef struct dr_engine dr_engine;
@@ -195,6 +195,10 @@ int dr_engine_get_launch(dr_engine* e, dr_launch_cfg* cfg);
 /* Time candidate launch shapes on the engine's own workload (state is
  * saved and restored) and keep the fastest.  trials = replays per shape. */
 int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* best_ms);
+/* on != 0: each step also rolls the instance's nominal out without noise
+ * (dr_diag.traj_cost), on a second stream concurrently with the sampled
+ * rollout.  Off by default; traj_cost is not updated while off. */
+int dr_engine_set_eval(dr_engine* e, int on);
 /* on != 0: later steps run as plain stream launches with HIP events
  * between the kernels (no graph), so that dr_engine_last_timing can report
  * per-kernel device time. */

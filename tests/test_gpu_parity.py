@@ -31,6 +31,12 @@ def small_scenario(sc, n_samples=1000, n_timesteps=37, n_drones=2, controlled=1,
     return sc.Scenario("test", [tr], inst, [0] * n_drones, drones, 1, "test")
 
 
+def make_engine(pkg, scn):
+    eng = pkg.Engine(scn)
+    eng.set_eval(True)
+    return eng
+
+
 def opponents_of(pkg, scn, drone):
     out = []
     for d in range(len(scn.drone_world)):
@@ -87,7 +93,7 @@ def test_single_step_parity(pkg, oracle):
     sc = pkg.scenarios
     say("case: single instance K=1000 T=37 with one constant-velocity opponent")
     scn = small_scenario(sc)
-    eng = pkg.Engine(scn)
+    eng = make_engine(pkg, scn)
     nom0 = eng.get_nominal(0).copy()
     nom0[:, 0] += 0.7
     nom0[:, 1] = 0.2
@@ -103,7 +109,7 @@ def test_ragged_and_extreme_sizes(pkg, oracle, K, T):
     sc = pkg.scenarios
     say(f"case: K={K} T={T}")
     scn = small_scenario(sc, n_samples=K, n_timesteps=T, n_drones=1)
-    eng = pkg.Engine(scn)
+    eng = make_engine(pkg, scn)
     nom0 = eng.get_nominal(0).copy()
     eng.step(1)
     check_instance(pkg, oracle, eng, scn, 0, nom0)
@@ -119,7 +125,7 @@ def test_immediate_collision(pkg, oracle):
         scn.drones[1].p[c] = scn.drones[0].p[c]
         scn.drones[1].v[c] = 0.0
     scn.drones[1].p[0] += 0.05
-    eng = pkg.Engine(scn)
+    eng = make_engine(pkg, scn)
     nom0 = eng.get_nominal(0).copy()
     eng.step(1)
     upd = check_instance(pkg, oracle, eng, scn, 0, nom0)
@@ -134,7 +140,7 @@ def test_multi_instance_world_one_step(pkg, oracle):
     scn = small_scenario(sc, n_samples=777, n_timesteps=25, n_drones=4, controlled=3, track="lemniscate")
     scn.instances[1] = sc.instance(2048, 40, 0, 1, sc.seed_mix(5, 1))
     scn.instances[2] = sc.instance(1500, 33, 0, 2, sc.seed_mix(5, 2))
-    eng = pkg.Engine(scn)
+    eng = make_engine(pkg, scn)
     noms = [eng.get_nominal(i).copy() for i in range(3)]
     eng.step(1)
     for i in range(3):
@@ -146,7 +152,7 @@ def test_closed_loop_matches_oracle(pkg, oracle):
     sc = pkg.scenarios
     say("case: closed loop, 4 drones all controlled, K=1024 T=30, 6 steps")
     scn = small_scenario(sc, n_samples=1024, n_timesteps=30, n_drones=4, controlled=4, track="lemniscate")
-    eng = pkg.Engine(scn)
+    eng = make_engine(pkg, scn)
     n_steps = 6
     acts = eng.step(n_steps, advance=True)
     g_drones = eng.get_drones()
@@ -180,9 +186,11 @@ def test_launch_shapes_agree(pkg):
               for rb, g, ch, nb, ug, spg, tg, sp in [(128, 0, 256, 4, 1, 1, 25, 0), (64, 0, 64, 1, 0, 1, 256, 0),
                                                     (256, 7, 1024, 2, 1, 2, 7, 0), (128, 3, 4096, 4, 1, 3, 64, 0),
                                                     (64, 1, 192, 2, 0, 1, 1, 0), (128, 0, 1024, 1, 1, 1, 25, 1),
-                                                    (128, 0, 512, 2, 0, 1, 16, 1), (128, 5, 1024, 4, 1, 2, 32, 1)]]
+                                                    (128, 0, 512, 2, 0, 1, 16, 1), (128, 5, 1024, 4, 1, 2, 32, 1),
+                                                    (128, 0, 1024, 1, 1, 1, 16, 2), (128, 0, 1024, 2, 0, 1, 16, 2),
+                                                    (128, 9, 256, 4, 1, 2, 25, 2)]]
     for shape in shapes:
-        eng = pkg.Engine(scn)
+        eng = make_engine(pkg, scn)
         eng.set_launch(**shape)
         eng.step(3, advance=True)
         res = ([eng.costs(i) for i in range(3)], [eng.noise(i) for i in range(3)],
@@ -205,7 +213,7 @@ def test_deterministic_across_runs(pkg):
     scn = small_scenario(sc, n_samples=2000, n_timesteps=40, n_drones=2, controlled=2)
     out = []
     for _ in range(2):
-        eng = pkg.Engine(scn)
+        eng = make_engine(pkg, scn)
         eng.step(4, advance=True)
         out.append(([eng.get_nominal(i) for i in range(2)], eng.actions(), [eng.costs(i) for i in range(2)]))
         eng.close()
@@ -219,7 +227,7 @@ def test_full_size_four_drone_step(pkg, oracle):
     sc = pkg.scenarios
     say("case: full size 4 x K=16384 x T=100, one step")
     scn = sc.c1_four_drones()
-    eng = pkg.Engine(scn)
+    eng = make_engine(pkg, scn)
     noms = [eng.get_nominal(i).copy() for i in range(4)]
     eng.step(1)
     for i in range(4):

@@ -488,6 +488,295 @@ __global__ __launch_bounds__(64) void k_eval(const InstDev* __restrict__ insts, 
     dyn[inst].traj_cost = acc + final_cost(KC, s_track, s);
 }
 
+// Weighted average over one chunk of samples [k0, k0 + n) and timesteps
+// [t0, t1):
+//   w_k = exp(-(c_k - min)/lambda), min over the instance's rollout tiles,
+//   numerator[t][d] = sum_k w_k eps_k[t][d], denominator = sum_k w_k,
+// both accumulated in FP64 so the summation order cannot move the FP32
+// result.  The chunk's weights are computed once into LDS; lane (t, sub)
+// then streams the float4 noise of step t of samples sub, sub+S, ... where
+// S = floor(256/(t1-t0)) lanes share each step: the lanes of one sample read
+// one contiguous (t1-t0)*16-byte piece of its row (coalesced) and no
+// cross-lane reduction is needed until the final S-way sum.  Only the
+// workgroups with t0 == 0 write the chunk's denominator.
+__global__ __launch_bounds__(WAVG_BLOCK) void k_wavg(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
+                                                     const WTile* __restrict__ wtiles, const float* __restrict__ costs,
+                                                     const float4* __restrict__ noise, const float* __restrict__ tmin,
+                                                     double* __restrict__ part, double* __restrict__ den, int chunk)
+{
+    __shared__ float s_w[WAVG_MAX_CHUNK];
+    __shared__ double s_acc[WAVG_BLOCK][4];
+    __shared__ float s_min[WAVG_BLOCK / WAVE];
+    __shared__ double s_den[WAVG_BLOCK / WAVE];
+
+    const int tid = threadIdx.x;
+    const int wave = tid / WAVE, lane = tid & (WAVE - 1);
+    const WTile wt = wtiles[blockIdx.x];
+    const InstDev& I = insts[wt.inst];
+    const int K = I.K, T = I.T;
+
+    float m = __builtin_inff();
+    for (int i = tid; i < I.rtile_count; i += WAVG_BLOCK)
+        m = fminf(m, tmin[I.rtile_first + i]);
+    m = wave_min(m);
+    if (lane == 0)
+        s_min[wave] = m;
+    __syncthreads();
+    float minC = s_min[0];
+    for (int w = 1; w < WAVG_BLOCK / WAVE; ++w)
+        minC = fminf(minC, s_min[w]);
+
+    const float lam = dyn[wt.inst].lambda;
+    const int k0 = wt.chunk * chunk;
+    const int n = min(chunk, K - k0);
+    double dsum = 0.0;
+    for (int i = tid; i < n; i += WAVG_BLOCK)
+    {
+        float w = 0.0f;
+        if (__builtin_isfinite(minC))
+            w = expf_ieee(-(costs[I.cost_off + k0 + i] - minC) / lam);
+        s_w[i] = w;
+        dsum += (double)w;
+    }
+    dsum = wave_sum(dsum);
+    if (lane == 0)
+        s_den[wave] = dsum;
+    __syncthreads();
+    if (tid == 0 && wt.t0 == 0)
+    {
+        double d = s_den[0];
+        for (int w = 1; w < WAVG_BLOCK / WAVE; ++w)
+            d += s_den[w];
+        den[I.den_off + wt.chunk] = d;
+        if (wt.chunk == 0)
+            dyn[wt.inst].min_cost = minC;
+    }
+
+    const int TG = wt.t1 - wt.t0;
+    const int S = WAVG_BLOCK / TG;
+    const int sub = tid / TG, tt = tid - sub * TG;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    if (sub < S)
+    {
+        const float4* base = noise + I.noise_off + (long long)k0 * T + wt.t0 + tt;
+#pragma unroll 8
+        for (int i = sub; i < n; i += S)
+        {
+            const float4 e = base[(long long)i * T];
+            const double w = (double)s_w[i];
+            a0 += w * (double)e.x;
+            a1 += w * (double)e.y;
+            a2 += w * (double)e.z;
+            a3 += w * (double)e.w;
+        }
+    }
+    s_acc[tid][0] = a0;
+    s_acc[tid][1] = a1;
+    s_acc[tid][2] = a2;
+    s_acc[tid][3] = a3;
+    __syncthreads();
+    double* out = part + I.part_off + ((long long)wt.chunk * T + wt.t0) * 4;
+    for (int i = tid; i < TG * 4; i += WAVG_BLOCK)
+    {
+        const int r = i >> 2, d = i & 3;
+        double v = s_acc[r][d];
+        for (int q = 1; q < S; ++q)
+            v += s_acc[q * TG + r][d];
+        out[i] = v;
+    }
+}
+
+// Sum of n values p[0], p[stride], ... in index order; loads are issued in
+// batches of 8 so that their latencies overlap (same order as a plain loop).
+__device__ __forceinline__ double sum_in_order(const double* __restrict__ p, int n, long long stride)
+{
+    double acc = 0.0;
+    for (int c0 = 0; c0 < n; c0 += 8)
+    {
+        double v[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            v[b] = (c0 + b < n) ? p[(long long)(c0 + b) * stride] : 0.0;
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            acc += v[b];
+    }
+    return acc;
+}
+
+__device__ __forceinline__ void advance_drone(dr_drone_state& st, const float* act, const dr_track& tr, const QuadConst& C, float margin)
+{
+    QState s = state_from_abi(st);
+    float u[4];
+    clamp_control(C, act, u);
+    const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+    quad_step(C, s, u);
+    update_gate(tr.gates, tr.n_gates, margin, p0, s);
+    state_to_abi(s, st);
+}
+
+// Per instance: numerator and denominator over chunks in fixed order,
+// nominal += num/den when den > 1e-30 (kernels.cu:402-403), box clamp of the
+// whole nominal (kernels.cu:434-456), action = first control, shift left by
+// one step with shift_fill at the end (mppi_controller.cu:430-436), nu and
+// the optional temperature update (mppi_controller.cu:413-424).  With
+// advance != 0 the closed loop follows (SimulationEngine::dynStep,
+// engine.cpp:286-312): the instance's drone integrates its action with the
+// same dynamics and gate logic, noise-free, and drones without a controller
+// (distributed over the workgroups) keep their velocity.
+__global__ __launch_bounds__(256) void k_update(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
+                                                const double* __restrict__ part, const double* __restrict__ den,
+                                                float* __restrict__ nominal, float* __restrict__ actions, QuadConst C, int advance,
+                                                dr_drone_state* __restrict__ drones, const int* __restrict__ drone_inst, int n_drones,
+                                                const dr_track* __restrict__ tracks, float margin)
+{
+    __shared__ float s_u[DR_MAX_T * 4];
+    const int inst = blockIdx.x;
+    const InstDev& I = insts[inst];
+    const int T = I.T, tid = threadIdx.x;
+
+    const double dn = sum_in_order(den + I.den_off, I.n_chunks, 1);
+    for (int i = tid; i < T * 4; i += blockDim.x)
+    {
+        const double num = sum_in_order(part + I.part_off + i, I.n_chunks, (long long)T * 4);
+        float u = nominal[I.nom_off + i];
+        if (dn > 1e-30)
+            u = u + (float)(num / dn);
+        s_u[i] = u;
+    }
+    __syncthreads();
+    for (int t = tid; t < T; t += blockDim.x)
+    {
+        float in[4] = {s_u[t * 4 + 0], s_u[t * 4 + 1], s_u[t * 4 + 2], s_u[t * 4 + 3]};
+        float out[4];
+        clamp_control(C, in, out);
+        for (int d = 0; d < 4; ++d)
+            s_u[t * 4 + d] = out[d];
+    }
+    __syncthreads();
+    if (tid < 4)
+        actions[inst * 4 + tid] = s_u[tid];
+    for (int i = tid; i < T * 4; i += blockDim.x)
+    {
+        const int t = i / 4, d = i & 3;
+        nominal[I.nom_off + i] = (t + 1 < T) ? s_u[i + 4] : I.shift_fill[d];
+    }
+    if (tid == 0)
+    {
+        InstDyn& Y = dyn[inst];
+        const float nu = (float)dn;
+        Y.nu = nu;
+        if (I.adapt)
+        {
+            if (nu > 0.001f * (float)I.K)
+                Y.lambda = Y.lambda * 0.9f;
+            else if (nu < 0.0005f * (float)I.K)
+                Y.lambda = Y.lambda * 1.2f;
+        }
+        Y.step = Y.step + 1u;
+        if (advance)
+        {
+            const float act[4] = {s_u[0], s_u[1], s_u[2], s_u[3]};
+            advance_drone(drones[I.drone], act, tracks[I.track], C, margin);
+            for (int d = inst; d < n_drones; d += gridDim.x)
+                if (drone_inst[d] < 0)
+                    for (int c = 0; c < 3; ++c)
+                        drones[d].p[c] = fma_(C.dt, drones[d].v[c], drones[d].p[c]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers (host)
+
+template <int BLOCK>
+static void rollout_nb(int nb, int g, size_t shm, hipStream_t st, const InstDev* insts, InstDyn* dyn, const RTile* tiles, int n_tiles,
+                       const dr_track* tracks, const dr_drone_state* drones, const float* nominal, float* costs, float4* noise,
+                       float* tmin, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, int opp_stride)
+{
+    switch (nb)
+    {
+    case 1:
+        k_rollout<BLOCK, 1><<<g, BLOCK, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                   opp_stride);
+        break;
+    case 2:
+        k_rollout<BLOCK, 2><<<g, BLOCK, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                   opp_stride);
+        break;
+    default:
+        k_rollout<BLOCK, 4><<<g, BLOCK, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                   opp_stride);
+        break;
+    }
+}
+
+hipError_t launch_rollout(int block, int nb, int split, int grid, hipStream_t st, const InstDev* insts, InstDyn* dyn,
+                          const RTile* tiles, int n_tiles, const dr_track* tracks, const dr_drone_state* drones, const float* nominal,
+                          float* costs, float4* noise, float* tmin, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC,
+                          int max_T, int opp_stride)
+{
+    if (nb != 1 && nb != 2 && nb != 4)
+        return hipErrorInvalidValue;
+    const int g = grid > 0 ? (grid < n_tiles ? grid : n_tiles) : n_tiles;
+    const size_t shm = (size_t)max_T * opp_stride * sizeof(float4);
+    if (split == 2)
+    {
+        // tiles of 64 samples, four stage waves; chunk length = 2 * noise_batch steps
+        switch (nb)
+        {
+        case 1:
+            k_rollout_p4<2><<<g, 256, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                 opp_stride);
+            break;
+        case 2:
+            k_rollout_p4<4><<<g, 256, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                 opp_stride);
+            break;
+        default:
+            k_rollout_p4<8><<<g, 256, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                 opp_stride);
+            break;
+        }
+        return hipGetLastError();
+    }
+    if (split)
+    {
+        // tiles of 64 samples; chunk length = 4 * noise_batch steps
+        switch (nb)
+        {
+        case 1:
+            k_rollout_ws<4><<<g, 128, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                 opp_stride);
+            break;
+        case 2:
+            k_rollout_ws<8><<<g, 128, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                 opp_stride);
+            break;
+        default:
+            k_rollout_ws<16><<<g, 128, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                  opp_stride);
+            break;
+        }
+        return hipGetLastError();
+    }
+    switch (block)
+    {
+    case 64:
+        rollout_nb<64>(nb, g, shm, st, insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC, opp_stride);
+        break;
+    case 128:
+        rollout_nb<128>(nb, g, shm, st, insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC, opp_stride);
+        break;
+    case 256:
+        rollout_nb<256>(nb, g, shm, st, insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC, opp_stride);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_eval(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const dr_track* tracks,
                        const dr_drone_state* drones, const float* nominal, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC,
                        int max_T, int opp_stride)

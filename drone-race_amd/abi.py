@@ -97,6 +97,9 @@ SIGNATURES = {
     "dr_engine_autotune": (_I, [_P, _I, C.POINTER(LaunchCfg), _F]),
     "dr_engine_set_timing": (_I, [_P, _I]),
     "dr_engine_set_eval": (_I, [_P, _I]),
+    "dr_engine_cost_grad": (_I, [_P, _I, _F, _I, _F, _F, _I]),
+    "dr_engine_cost_grad_device": (_I, [_P, _I, _P, _I, _P, _P, _I, _P]),
+    "dr_engine_set_refine": (_I, [_P, _I, C.c_float, _I]),
     "dr_engine_last_timing": (_I, [_P, _F, _F, _F, _F]),
     "dr_mppi_create": (_I, [C.POINTER(QuadParams), C.POINTER(CostCfg), C.POINTER(Track), C.POINTER(MPPICfg), C.c_uint64,
                             C.POINTER(_P)]),
@@ -122,6 +125,13 @@ def load(path=LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise EngineError(f"HIP engine library not built: {path} is missing (run __graft_entry__.build())")
+    # PyTorch ships its own libamdhip64.so.7 (same soname as ROCm's).  Loading
+    # torch first makes the engine bind to that runtime, so that torch and the
+    # engine share one HIP runtime in the process (tensors, streams, autograd).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

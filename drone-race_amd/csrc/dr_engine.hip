@@ -31,7 +31,13 @@ hipError_t launch_wavg(int chunk, int n_wtiles, hipStream_t st, const InstDev* i
                        const float* costs, const float4* noise, const float* tmin, double* part, double* den);
 hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
                          float* nominal, float* actions, const QuadConst& C, int advance, dr_drone_state* drones, const int* drone_inst,
-                         int n_drones, const dr_track* tracks, float margin);
+                         int n_drones, const dr_track* tracks, float margin, int mode);
+hipError_t launch_costgrad(int lanes, hipStream_t st, const InstDev* insts, int inst, int T, int n_ck, int ck,
+                           const dr_track* tracks, const dr_drone_state* drones, const float* U, int n_seq, float* J, float* G,
+                           const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, int opp_stride);
+hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, int max_T, int max_slots, const dr_track* tracks,
+                         const dr_drone_state* drones, float* nominal, int n_steps, float step, int ck, const QuadConst& C,
+                         const CostConst& Q, const dr_cost_cfg& KC, int opp_stride);
 
 } // namespace dr
 
@@ -91,6 +97,10 @@ struct dr_engine
     hipStream_t side = nullptr; // forked stream of the nominal's noise-free rollout
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool eval = false;
+    int refine_steps = 0; // adjoint gradient steps per control step (0: plain MPPI)
+    float refine_step = 0.05f;
+    int refine_ck = 10;
+    DevBuf<float> d_gU, d_gJ, d_gG; // scratch of dr_engine_cost_grad
     QuadConst C{};
     CostConst Q{};
     dr_cost_cfg cost{};
@@ -162,6 +172,15 @@ struct dr_engine
     }
 
     int build_layout();
+    // state slots of one adjoint lane: checkpoints + one recomputed segment
+    int slots_for(int T, int ck) const { return (T + ck - 1) / ck + ck; }
+    int max_refine_slots() const
+    {
+        int m = 1;
+        for (const InstDev& I : idev)
+            m = std::max(m, slots_for(I.T, refine_ck));
+        return m;
+    }
     int enqueue_step(bool advance, int timed_index);
     int run(int n_steps, bool advance);
 };
@@ -283,8 +302,18 @@ int dr_engine::enqueue_step(bool advance, int timed_index)
         HIPCHK(hipEventRecord(ev[2], stream));
     if (eval)
         HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
-    HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, advance ? 1 : 0, d_drones.p,
-                         d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin));
+    if (refine_steps > 0)
+    {
+        HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, 0, d_drones.p,
+                             d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin, 1));
+        HIPCHK(launch_refine(n, stream, d_inst.p, max_T, max_refine_slots(), d_tracks.p, d_drones.p, d_nominal.p, refine_steps,
+                             refine_step, refine_ck, C, Q, cost, opp_stride));
+        HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, advance ? 1 : 0, d_drones.p,
+                             d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin, 2));
+    }
+    else
+        HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, advance ? 1 : 0, d_drones.p,
+                             d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin, 0));
     if (ev)
         HIPCHK(hipEventRecord(ev[3], stream));
     return DR_OK;
@@ -722,6 +751,75 @@ int dr_engine_set_eval(dr_engine* e, int on)
     HIPCHK(hipStreamSynchronize(e->stream));
     e->eval = on != 0;
     e->drop_graphs();
+    return DR_OK;
+}
+
+static const size_t kAdjLdsBudget = 64 * 1024;
+
+int dr_engine_set_refine(dr_engine* e, int n_grad_steps, float step, int ckpt_every)
+{
+    if (!e)
+        return fail(DR_EINVAL, "NULL handle");
+    if (n_grad_steps < 0 || n_grad_steps > 100 || !(step >= 0.0f) || ckpt_every < 1 || ckpt_every > DR_MAX_T)
+        return fail(DR_EINVAL, "set_refine: n_grad_steps in [0,100], step >= 0, ckpt_every in [1,256]");
+    size_t need = 0;
+    for (const InstDev& I : e->idev)
+        need = std::max(need, ((size_t)e->max_T * e->opp_stride + (size_t)e->slots_for(I.T, ckpt_every) * 4) * sizeof(float4));
+    if (need > kAdjLdsBudget)
+        return fail(DR_EINVAL, "set_refine: checkpoint interval needs too much LDS (use ckpt_every near sqrt(T))");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->refine_steps = n_grad_steps;
+    e->refine_step = step;
+    e->refine_ck = ckpt_every;
+    e->drop_graphs();
+    return DR_OK;
+}
+
+int dr_engine_cost_grad_device(dr_engine* e, int instance, const float* U, int n_seq, float* J, float* dJdU, int ckpt_every,
+                               void* stream)
+{
+    if (int rc = check_inst(e, instance))
+        return rc;
+    if (!U || !J || n_seq < 1 || ckpt_every < 1)
+        return fail(DR_EINVAL, "cost_grad: bad pointers or sizes");
+    const InstDev& I = e->idev[instance];
+    const int ck = std::min(ckpt_every, I.T);
+    const int n_ck = (I.T + ck - 1) / ck;
+    const size_t table = (size_t)I.T * e->opp_stride * sizeof(float4);
+    const size_t per_lane = (size_t)(n_ck + ck) * 4 * sizeof(float4);
+    int lanes = 64;
+    while (lanes > 1 && table + lanes * per_lane > kAdjLdsBudget)
+        lanes /= 2;
+    if (table + lanes * per_lane > kAdjLdsBudget)
+        return fail(DR_EINVAL, "cost_grad: horizon and checkpoint interval need too much LDS");
+    hipStream_t st = stream ? (hipStream_t)stream : e->stream;
+    HIPCHK(launch_costgrad(lanes, st, e->d_inst.p, instance, I.T, n_ck, ck, e->d_tracks.p, e->d_drones.p, U, n_seq, J, dJdU, e->C, e->Q,
+                           e->cost, e->opp_stride));
+    return DR_OK;
+}
+
+int dr_engine_cost_grad(dr_engine* e, int instance, const float* U, int n_seq, float* J, float* dJdU, int ckpt_every)
+{
+    if (int rc = check_inst(e, instance))
+        return rc;
+    if (!U || !J || n_seq < 1)
+        return fail(DR_EINVAL, "cost_grad: bad pointers or sizes");
+    const size_t n = (size_t)n_seq * e->idev[instance].T * 4;
+    if (e->d_gU.n < n)
+    {
+        HIPCHK(e->d_gU.alloc(n));
+        HIPCHK(e->d_gG.alloc(n));
+    }
+    if (e->d_gJ.n < (size_t)n_seq)
+        HIPCHK(e->d_gJ.alloc(n_seq));
+    HIPCHK(hipMemcpyAsync(e->d_gU.p, U, n * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    int rc = dr_engine_cost_grad_device(e, instance, e->d_gU.p, n_seq, e->d_gJ.p, dJdU ? e->d_gG.p : nullptr, ckpt_every, nullptr);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemcpyAsync(J, e->d_gJ.p, n_seq * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    if (dJdU)
+        HIPCHK(hipMemcpyAsync(dJdU, e->d_gG.p, n * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
     return DR_OK;
 }
 

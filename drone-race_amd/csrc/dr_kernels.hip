@@ -12,6 +12,7 @@
 // (src/engine.cpp:286-312)                                 -> k_update (advance)
 #include <hip/hip_runtime.h>
 
+#include "dr_adjoint.h"
 #include "dr_kernels.h"
 
 namespace dr {
@@ -488,6 +489,201 @@ __global__ __launch_bounds__(64) void k_eval(const InstDev* __restrict__ insts, 
     dyn[inst].traj_cost = acc + final_cost(KC, s_track, s);
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint rollout: J(U) and dJ/dU for control sequences of one instance,
+// noise-free, from the instance's current drone state.  Checkpoints every
+// ck steps (one state slot per checkpoint in LDS), each segment recomputed
+// forward into LDS, then swept backward (dr_adjoint.h).  A state slot holds
+// (p, q, v, w, gate, laps) and the discount weight of that state's running
+// cost in four float4 words.
+
+struct AdjCtx
+{
+    const dr_track* tr;
+    const float4* opp; // [T][opp_stride] predicted opponent positions
+    int opp_stride, n_opp, T, ck;
+    QuadConst C;
+    CostConst Q;
+    dr_cost_cfg K;
+};
+
+__device__ __forceinline__ void put_state(float4* buf, int slot, int stride, const QState& s, float weight)
+{
+    buf[(slot * 4 + 0) * stride] = make_float4(s.p[0], s.p[1], s.p[2], s.q[0]);
+    buf[(slot * 4 + 1) * stride] = make_float4(s.q[1], s.q[2], s.q[3], s.v[0]);
+    buf[(slot * 4 + 2) * stride] = make_float4(s.v[1], s.v[2], s.w[0], s.w[1]);
+    buf[(slot * 4 + 3) * stride] = make_float4(s.w[2], u2f((uint32_t)s.gate), u2f((uint32_t)s.laps), weight);
+}
+
+__device__ __forceinline__ QState get_state(const float4* buf, int slot, int stride, float* weight)
+{
+    QState s;
+    const float4 a = buf[(slot * 4 + 0) * stride], b = buf[(slot * 4 + 1) * stride];
+    const float4 c = buf[(slot * 4 + 2) * stride], d = buf[(slot * 4 + 3) * stride];
+    s.p[0] = a.x; s.p[1] = a.y; s.p[2] = a.z; s.q[0] = a.w;
+    s.q[1] = b.x; s.q[2] = b.y; s.q[3] = b.z; s.v[0] = b.w;
+    s.v[1] = c.x; s.v[2] = c.y; s.w[0] = c.z; s.w[1] = c.w;
+    s.w[2] = d.x; s.gate = (int)f2u(d.y); s.laps = (int)f2u(d.z);
+    *weight = d.w;
+    return s;
+}
+
+__device__ __forceinline__ void adj_step(const AdjCtx& A, QState& s, const float* U, int t, float uc[4])
+{
+    float uin[4] = {U[t * 4 + 0], U[t * 4 + 1], U[t * 4 + 2], U[t * 4 + 3]};
+    clamp_control(A.C, uin, uc);
+    const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+    quad_step(A.C, s, uc);
+    update_gate(A.tr->gates, A.tr->n_gates, A.K.gate_traversal_margin, p0, s);
+}
+
+// One control sequence; buf = this lane's state slots (stride between words).
+__device__ float cost_grad_lane(const AdjCtx& A, const QState& x0, const float* U, float* G, float4* buf, int stride)
+{
+    const int T = A.T, ck = A.ck;
+    const int n_ck = (T + ck - 1) / ck;
+    QState s = x0;
+    float acc = 0.0f, decay = 1.0f, w_prev = 0.0f;
+    int t_end = T - 1;
+    for (int t = 0; t < T; ++t)
+    {
+        if (t % ck == 0)
+            put_state(buf, t / ck, stride, s, w_prev);
+        float uc[4];
+        adj_step(A, s, U, t, uc);
+        int term;
+        const float c = stage_cost(A.C, A.Q, A.K, *A.tr, s, A.opp + t * A.opp_stride, A.n_opp, &term);
+        acc = fma_(c, decay, acc);
+        w_prev = decay;
+        decay = decay * DECAY;
+        if (term != TERM_NONE)
+        {
+            t_end = t;
+            break;
+        }
+    }
+    const float J = acc + final_cost(A.K, *A.tr, s);
+    if (!G)
+        return J;
+    for (int i = 0; i < T * 4; ++i)
+        G[i] = 0.0f;
+    QGrad g;
+    for (int i = 0; i < 3; ++i)
+        g.p[i] = g.v[i] = g.w[i] = 0.0f;
+    for (int i = 0; i < 4; ++i)
+        g.q[i] = 0.0f;
+    final_cost_grad(A.K, *A.tr, s, g.p);
+    stage_cost_grad(A.C, A.Q, A.K, *A.tr, s, A.opp + t_end * A.opp_stride, A.n_opp, w_prev, g.p);
+    for (int sg = t_end / ck; sg >= 0; --sg)
+    {
+        const int t0 = sg * ck;
+        const int t1 = min(t0 + ck, t_end + 1);
+        float w;
+        QState x = get_state(buf, sg, stride, &w);
+        for (int t = t0; t < t1; ++t)
+        {
+            put_state(buf, n_ck + (t - t0), stride, x, w);
+            float uc[4];
+            adj_step(A, x, U, t, uc);
+            w = (t == 0) ? 1.0f : w * DECAY;
+        }
+        for (int t = t1 - 1; t >= t0; --t)
+        {
+            float wt;
+            const QState xt = get_state(buf, n_ck + (t - t0), stride, &wt);
+            float uin[4] = {U[t * 4 + 0], U[t * 4 + 1], U[t * 4 + 2], U[t * 4 + 3]}, uc[4], gu[4];
+            clamp_control(A.C, uin, uc);
+            quad_step_vjp(A.C, xt, uc, g, gu);
+            G[t * 4 + 0] = gu[0] * clamp_grad(uin[0], A.C.thrust_min, A.C.thrust_max);
+            G[t * 4 + 1] = gu[1] * clamp_grad(uin[1], -A.C.rate_max, A.C.rate_max);
+            G[t * 4 + 2] = gu[2] * clamp_grad(uin[2], -A.C.rate_max, A.C.rate_max);
+            G[t * 4 + 3] = gu[3] * clamp_grad(uin[3], -A.C.rate_max, A.C.rate_max);
+            if (t >= 1)
+                stage_cost_grad(A.C, A.Q, A.K, *A.tr, xt, A.opp + (t - 1) * A.opp_stride, A.n_opp, wt, g.p);
+        }
+    }
+    return J;
+}
+
+__device__ void stage_inputs(const InstDev& I, const dr_track* tracks, const dr_drone_state* drones, const QuadConst& C, dr_track* s_track,
+                             float4* s_opp, int opp_stride)
+{
+    const int T = I.T, n_opp = I.n_opp;
+    for (int i = threadIdx.x; i < T * n_opp; i += blockDim.x)
+    {
+        const int t = i / n_opp, j = i - t * n_opp;
+        s_opp[t * opp_stride + j] = opp_predict(drones[I.opp[j]], C.dt * (float)(t + 1));
+    }
+    const float* src = reinterpret_cast<const float*>(tracks + I.track);
+    float* dst = reinterpret_cast<float*>(s_track);
+    for (int i = threadIdx.x; i < (int)(sizeof(dr_track) / 4); i += blockDim.x)
+        dst[i] = src[i];
+}
+
+// J and dJ/dU for n_seq sequences U[n_seq][T][4] of instance `inst`, one lane
+// per sequence.  Dynamic LDS: opponent table, then lanes x slots state words.
+__global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr_track* __restrict__ tracks,
+                           const dr_drone_state* __restrict__ drones, const float* __restrict__ U, int n_seq, float* __restrict__ J,
+                           float* __restrict__ G, int ck, QuadConst C, CostConst Q, dr_cost_cfg KC, int opp_stride)
+{
+    extern __shared__ float4 s_dyn[];
+    __shared__ dr_track s_track;
+    const InstDev& I = insts[inst];
+    const int T = I.T;
+    float4* s_opp = s_dyn;
+    float4* s_buf = s_dyn + T * opp_stride;
+    stage_inputs(I, tracks, drones, C, &s_track, s_opp, opp_stride);
+    __syncthreads();
+    const int seq = blockIdx.x * blockDim.x + threadIdx.x;
+    if (seq >= n_seq)
+        return;
+    AdjCtx A{&s_track, s_opp, opp_stride, I.n_opp, T, ck, C, Q, KC};
+    const QState x0 = state_from_abi(drones[I.drone]);
+    J[seq] = cost_grad_lane(A, x0, U + (long long)seq * T * 4, G ? G + (long long)seq * T * 4 : nullptr, s_buf + threadIdx.x,
+                            blockDim.x);
+}
+
+// Adjoint refinement of each instance's updated nominal (one workgroup per
+// instance, one lane): n_steps projected gradient steps
+// U <- clamp(U - step * dJ/dU), J and dJ/dU from cost_grad_lane.
+__global__ __launch_bounds__(64) void k_refine(const InstDev* __restrict__ insts, const dr_track* __restrict__ tracks,
+                                               const dr_drone_state* __restrict__ drones, float* __restrict__ nominal, int n_steps,
+                                               float step, int ck, QuadConst C, CostConst Q, dr_cost_cfg KC, int opp_stride)
+{
+    extern __shared__ float4 s_dyn[];
+    __shared__ dr_track s_track;
+    __shared__ float s_u[DR_MAX_T * 4];
+    __shared__ float s_g[DR_MAX_T * 4];
+    const InstDev& I = insts[blockIdx.x];
+    const int T = I.T;
+    float4* s_opp = s_dyn;
+    float4* s_buf = s_dyn + T * opp_stride;
+    stage_inputs(I, tracks, drones, C, &s_track, s_opp, opp_stride);
+    for (int i = threadIdx.x; i < T * 4; i += blockDim.x)
+        s_u[i] = nominal[I.nom_off + i];
+    __syncthreads();
+    AdjCtx A{&s_track, s_opp, opp_stride, I.n_opp, T, ck, C, Q, KC};
+    const QState x0 = state_from_abi(drones[I.drone]);
+    for (int it = 0; it < n_steps; ++it)
+    {
+        if (threadIdx.x == 0)
+            (void)cost_grad_lane(A, x0, s_u, s_g, s_buf, 1);
+        __syncthreads();
+        for (int t = threadIdx.x; t < T; t += blockDim.x)
+        {
+            float in[4], out[4];
+            for (int d = 0; d < 4; ++d)
+                in[d] = fma_(-step, s_g[t * 4 + d], s_u[t * 4 + d]);
+            clamp_control(C, in, out);
+            for (int d = 0; d < 4; ++d)
+                s_u[t * 4 + d] = out[d];
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < T * 4; i += blockDim.x)
+        nominal[I.nom_off + i] = s_u[i];
+}
+
 // Weighted average over one chunk of samples [k0, k0 + n) and timesteps
 // [t0, t1):
 //   w_k = exp(-(c_k - min)/lambda), min over the instance's rollout tiles,
@@ -624,36 +820,68 @@ __device__ __forceinline__ void advance_drone(dr_drone_state& st, const float* a
 // engine.cpp:286-312): the instance's drone integrates its action with the
 // same dynamics and gate logic, noise-free, and drones without a controller
 // (distributed over the workgroups) keep their velocity.
+// mode 0: update, clamp, action, shift, advance; mode 1: update and clamp
+// only (the unshifted nominal is written back for the adjoint refinement);
+// mode 2: action, shift, advance from the (refined) unshifted nominal.
 __global__ __launch_bounds__(256) void k_update(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
                                                 const double* __restrict__ part, const double* __restrict__ den,
                                                 float* __restrict__ nominal, float* __restrict__ actions, QuadConst C, int advance,
                                                 dr_drone_state* __restrict__ drones, const int* __restrict__ drone_inst, int n_drones,
-                                                const dr_track* __restrict__ tracks, float margin)
+                                                const dr_track* __restrict__ tracks, float margin, int mode)
 {
     __shared__ float s_u[DR_MAX_T * 4];
     const int inst = blockIdx.x;
     const InstDev& I = insts[inst];
     const int T = I.T, tid = threadIdx.x;
 
-    const double dn = sum_in_order(den + I.den_off, I.n_chunks, 1);
-    for (int i = tid; i < T * 4; i += blockDim.x)
+    double dn = 0.0;
+    if (mode == 2)
     {
-        const double num = sum_in_order(part + I.part_off + i, I.n_chunks, (long long)T * 4);
-        float u = nominal[I.nom_off + i];
-        if (dn > 1e-30)
-            u = u + (float)(num / dn);
-        s_u[i] = u;
+        for (int i = tid; i < T * 4; i += blockDim.x)
+            s_u[i] = nominal[I.nom_off + i];
+        __syncthreads();
     }
-    __syncthreads();
-    for (int t = tid; t < T; t += blockDim.x)
+    else
     {
-        float in[4] = {s_u[t * 4 + 0], s_u[t * 4 + 1], s_u[t * 4 + 2], s_u[t * 4 + 3]};
-        float out[4];
-        clamp_control(C, in, out);
-        for (int d = 0; d < 4; ++d)
-            s_u[t * 4 + d] = out[d];
+        dn = sum_in_order(den + I.den_off, I.n_chunks, 1);
+        for (int i = tid; i < T * 4; i += blockDim.x)
+        {
+            const double num = sum_in_order(part + I.part_off + i, I.n_chunks, (long long)T * 4);
+            float u = nominal[I.nom_off + i];
+            if (dn > 1e-30)
+                u = u + (float)(num / dn);
+            s_u[i] = u;
+        }
+        __syncthreads();
+        for (int t = tid; t < T; t += blockDim.x)
+        {
+            float in[4] = {s_u[t * 4 + 0], s_u[t * 4 + 1], s_u[t * 4 + 2], s_u[t * 4 + 3]};
+            float out[4];
+            clamp_control(C, in, out);
+            for (int d = 0; d < 4; ++d)
+                s_u[t * 4 + d] = out[d];
+        }
+        __syncthreads();
+        if (mode == 1)
+        {
+            for (int i = tid; i < T * 4; i += blockDim.x)
+                nominal[I.nom_off + i] = s_u[i];
+            if (tid == 0)
+            {
+                InstDyn& Y = dyn[inst];
+                const float nu = (float)dn;
+                Y.nu = nu;
+                if (I.adapt)
+                {
+                    if (nu > 0.001f * (float)I.K)
+                        Y.lambda = Y.lambda * 0.9f;
+                    else if (nu < 0.0005f * (float)I.K)
+                        Y.lambda = Y.lambda * 1.2f;
+                }
+            }
+            return;
+        }
     }
-    __syncthreads();
     if (tid < 4)
         actions[inst * 4 + tid] = s_u[tid];
     for (int i = tid; i < T * 4; i += blockDim.x)
@@ -664,14 +892,17 @@ __global__ __launch_bounds__(256) void k_update(const InstDev* __restrict__ inst
     if (tid == 0)
     {
         InstDyn& Y = dyn[inst];
-        const float nu = (float)dn;
-        Y.nu = nu;
-        if (I.adapt)
+        if (mode == 0)
         {
-            if (nu > 0.001f * (float)I.K)
-                Y.lambda = Y.lambda * 0.9f;
-            else if (nu < 0.0005f * (float)I.K)
-                Y.lambda = Y.lambda * 1.2f;
+            const float nu = (float)dn;
+            Y.nu = nu;
+            if (I.adapt)
+            {
+                if (nu > 0.001f * (float)I.K)
+                    Y.lambda = Y.lambda * 0.9f;
+                else if (nu < 0.0005f * (float)I.K)
+                    Y.lambda = Y.lambda * 1.2f;
+            }
         }
         Y.step = Y.step + 1u;
         if (advance)
@@ -797,9 +1028,28 @@ hipError_t launch_wavg(int chunk, int n_wtiles, hipStream_t st, const InstDev* i
 
 hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
                          float* nominal, float* actions, const QuadConst& C, int advance, dr_drone_state* drones, const int* drone_inst,
-                         int n_drones, const dr_track* tracks, float margin)
+                         int n_drones, const dr_track* tracks, float margin, int mode)
 {
-    k_update<<<n_inst, 256, 0, st>>>(insts, dyn, part, den, nominal, actions, C, advance, drones, drone_inst, n_drones, tracks, margin);
+    k_update<<<n_inst, 256, 0, st>>>(insts, dyn, part, den, nominal, actions, C, advance, drones, drone_inst, n_drones, tracks, margin,
+                                     mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_costgrad(int lanes, hipStream_t st, const InstDev* insts, int inst, int T, int n_ck, int ck,
+                           const dr_track* tracks, const dr_drone_state* drones, const float* U, int n_seq, float* J, float* G,
+                           const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, int opp_stride)
+{
+    const size_t shm = ((size_t)T * opp_stride + (size_t)lanes * (n_ck + ck) * 4) * sizeof(float4);
+    k_costgrad<<<(n_seq + lanes - 1) / lanes, lanes, shm, st>>>(insts, inst, tracks, drones, U, n_seq, J, G, ck, C, Q, KC, opp_stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, int max_T, int max_slots, const dr_track* tracks,
+                         const dr_drone_state* drones, float* nominal, int n_steps, float step, int ck, const QuadConst& C,
+                         const CostConst& Q, const dr_cost_cfg& KC, int opp_stride)
+{
+    const size_t shm = ((size_t)max_T * opp_stride + (size_t)max_slots * 4) * sizeof(float4);
+    k_refine<<<n_inst, 64, shm, st>>>(insts, tracks, drones, nominal, n_steps, step, ck, C, Q, KC, opp_stride);
     return hipGetLastError();
 }
 

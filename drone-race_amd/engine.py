@@ -123,6 +123,21 @@ class Engine:
     def set_eval(self, on):
         check(self.lib.dr_engine_set_eval(self.h, 1 if on else 0))
 
+    def cost_grad(self, i, U, ckpt_every=10, grad=True):
+        """J and dJ/dU of control sequences U[n][T][4] of instance i (noise-free adjoint rollout)."""
+        U = np.ascontiguousarray(U, dtype=np.float32)
+        n = U.shape[0]
+        J = np.empty(n, dtype=np.float32)
+        G = np.empty_like(U) if grad else None
+        check(self.lib.dr_engine_cost_grad(self.h, i, _fptr(U), n, _fptr(J), _fptr(G) if grad else None, ckpt_every))
+        return J, G
+
+    def cost_grad_device(self, i, u_ptr, n_seq, j_ptr, g_ptr, ckpt_every, stream):
+        check(self.lib.dr_engine_cost_grad_device(self.h, i, u_ptr, n_seq, j_ptr, g_ptr, ckpt_every, stream))
+
+    def set_refine(self, n_grad_steps, step=0.05, ckpt_every=10):
+        check(self.lib.dr_engine_set_refine(self.h, n_grad_steps, step, ckpt_every))
+
     def set_timing(self, on):
         check(self.lib.dr_engine_set_timing(self.h, 1 if on else 0))
 

@@ -208,6 +208,22 @@ int dr_engine_set_timing(dr_engine* e, int on);
  * update_ms includes the closed-loop advance when it ran). */
 int dr_engine_last_timing(dr_engine* e, float* rollout_ms, float* reduce_ms, float* update_ms, float* total_ms);
 
+/* Adjoint of the noise-free rollout cost.  For n_seq control sequences
+ * U[n_seq][T][4] of `instance` (from its current drone state, opponents and
+ * track) writes J[n_seq] and dJdU[n_seq][T][4] (may be NULL), checkpointing
+ * every ckpt_every steps.  Host pointers; synchronous.  No reference
+ * counterpart: the reference has no differentiable rollout. */
+int dr_engine_cost_grad(dr_engine* e, int instance, const float* U, int n_seq, float* J, float* dJdU, int ckpt_every);
+/* Same with device pointers, enqueued on `stream` (a hipStream_t; NULL = the
+ * engine's stream).  Used by the PyTorch autograd wrapper. */
+int dr_engine_cost_grad_device(dr_engine* e, int instance, const float* U, int n_seq, float* J, float* dJdU, int ckpt_every,
+                               void* stream);
+/* Adjoint refinement inside every control step: after the MPPI update,
+ * n_grad_steps projected gradient steps U <- clamp(U - step * dJ/dU) on each
+ * instance's nominal, then action and shift.  n_grad_steps = 0 (default) is
+ * plain MPPI. */
+int dr_engine_set_refine(dr_engine* e, int n_grad_steps, float step, int ckpt_every);
+
 /* Single-controller entry points with the reference's shape. */
 typedef struct dr_mppi_cfg
 {

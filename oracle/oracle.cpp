@@ -4,6 +4,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "oracle.h"
@@ -168,6 +169,284 @@ void update(const dr_quad_params& P, const double* costs, const float* noise, co
             shifted[t * 4 + d] = (t + 1 < T) ? (double)u[(size_t)(t + 1) * 4 + d] : (double)fill[d];
     *min_cost = (double)mnc;
     *nu = sizeof(S) == 4 ? (double)(float)den : den;
+}
+
+// ---------------------------------------------------------------- adjoint
+// Reverse-mode restatement of the rollout cost J(U) (noise-free) with
+// checkpoints every `ck` steps and recomputation in between.  Same
+// differentiation rules as the engine: clamp' = 1 on [lo, hi]; gate
+// crossings and the terminal stop piecewise constant; min() takes the
+// gradient of its active side.
+
+template <class S>
+struct G13
+{
+    S p[3], q[4], v[3], w[3];
+};
+
+template <class S>
+void vjp_step(const Model<S>& M, const St<S>& x, const S u[4], G13<S>& gx, S gu[4])
+{
+    const S qw = x.q[0], qx = x.q[1], qy = x.q[2], qz = x.q[3];
+    const S wx = x.w[0], wy = x.w[1], wz = x.w[2];
+    const S two = (S)2, one = (S)1;
+    const S b3[3] = {two * fm(qx, qz, qw * qy), two * fm(qy, qz, -(qw * qx)), fm(-two, fm(qx, qx, qy * qy), one)};
+    const S at = u[0] * M.inv_mass;
+    const S dq[4] = {-fm(qx, wx, fm(qy, wy, qz * wz)), fm(qw, wx, fm(qy, wz, -(qz * wy))), fm(qw, wy, fm(qz, wx, -(qx * wz))),
+                     fm(qw, wz, fm(qx, wy, -(qy * wx)))};
+    S n[4];
+    for (int i = 0; i < 4; ++i)
+        n[i] = fm(M.half_dt, dq[i], x.q[i]);
+    const S inv = one / sq_root(fm(n[0], n[0], fm(n[1], n[1], fm(n[2], n[2], n[3] * n[3]))));
+    S o[4];
+    for (int i = 0; i < 4; ++i)
+        o[i] = n[i] * inv;
+    G13<S> g;
+    const S kk = M.rate_k;
+    for (int i = 0; i < 3; ++i)
+    {
+        gu[1 + i] = kk * gx.w[i];
+        g.w[i] = (one - kk) * gx.w[i];
+    }
+    const S dot = fm(o[0], gx.q[0], fm(o[1], gx.q[1], fm(o[2], gx.q[2], o[3] * gx.q[3])));
+    S gn[4];
+    for (int i = 0; i < 4; ++i)
+        gn[i] = (gx.q[i] - o[i] * dot) * inv;
+    const S h = M.half_dt;
+    const S hw = h * gn[0], hx = h * gn[1], hy = h * gn[2], hz = h * gn[3];
+    g.q[0] = gn[0] + (wx * hx + wy * hy + wz * hz);
+    g.q[1] = gn[1] + (-wx * hw - wz * hy + wy * hz);
+    g.q[2] = gn[2] + (-wy * hw + wz * hx - wx * hz);
+    g.q[3] = gn[3] + (-wz * hw - wy * hx + wx * hy);
+    g.w[0] = g.w[0] + (-qx * hw + qw * hx + qz * hy - qy * hz);
+    g.w[1] = g.w[1] + (-qy * hw - qz * hx + qw * hy + qx * hz);
+    g.w[2] = g.w[2] + (-qz * hw + qy * hx - qx * hy + qw * hz);
+    S ga[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        ga[i] = M.dt * gx.v[i];
+        g.p[i] = gx.p[i];
+        g.v[i] = gx.v[i] + M.dt * gx.p[i] - M.drag * ga[i];
+    }
+    const S gat = ga[0] * b3[0] + ga[1] * b3[1] + ga[2] * b3[2];
+    gu[0] = gat * M.inv_mass;
+    const S gb0 = at * ga[0], gb1 = at * ga[1], gb2 = at * ga[2];
+    g.q[0] = g.q[0] + two * (qy * gb0 - qx * gb1);
+    g.q[1] = g.q[1] + two * (qz * gb0 - qw * gb1) - (S)4 * qx * gb2;
+    g.q[2] = g.q[2] + two * (qw * gb0 + qz * gb1) - (S)4 * qy * gb2;
+    g.q[3] = g.q[3] + two * (qx * gb0 + qy * gb1);
+    gx = g;
+}
+
+template <class S>
+void stage_grad(const Model<S>& M, const CostK<S>& Q, const dr_cost_cfg& K, const dr_track& tr, const St<S>& s, const dr_opponent* opp,
+                int n_opp, S tau, S weight, S g[3])
+{
+    S min_sq = (S)INFINITY;
+    int jmin = -1;
+    S om[3] = {0, 0, 0};
+    for (int j = 0; j < n_opp; ++j)
+    {
+        S o[3], d[3];
+        for (int i = 0; i < 3; ++i)
+        {
+            o[i] = fm((S)opp[j].v[i], tau, (S)opp[j].p[i]);
+            d[i] = s.p[i] - o[i];
+        }
+        const S d2 = fm(d[0], d[0], fm(d[1], d[1], d[2] * d[2]));
+        if (d2 < min_sq)
+        {
+            min_sq = d2;
+            jmin = j;
+            for (int i = 0; i < 3; ++i)
+                om[i] = o[i];
+        }
+        if (d2 < Q.r2_prox && d2 > (S)0)
+        {
+            const S dd = sq_root(d2);
+            const S x = fm(-dd, Q.inv_prox, (S)1);
+            const S f = weight * ((S)2 * (S)K.opp_prox_cost * x) * (-Q.inv_prox) / dd;
+            for (int i = 0; i < 3; ++i)
+                g[i] = fm(f, d[i], g[i]);
+        }
+    }
+    S d_opp = (S)INFINITY;
+    if (min_sq < Q.far2)
+        d_opp = sq_root(min_sq) - M.radius;
+    const S d_ground = s.p[2] - (S)tr.ground_z;
+    const bool opp_side = d_opp < d_ground;
+    const S bd = opp_side ? d_opp : d_ground;
+    if (bd < Q.danger)
+    {
+        const S x = fm(-bd, Q.inv_danger, (S)1);
+        const S dbd = weight * ((S)2 * (S)K.boundary_cost * x) * (-Q.inv_danger);
+        if (opp_side && jmin >= 0)
+        {
+            const S dd = sq_root(min_sq);
+            if (dd > (S)0)
+            {
+                const S f = dbd / dd;
+                for (int i = 0; i < 3; ++i)
+                    g[i] = fm(f, s.p[i] - om[i], g[i]);
+            }
+        }
+        else
+            g[2] = g[2] + dbd;
+    }
+}
+
+template <class S>
+void final_grad(const dr_cost_cfg& K, const dr_track& tr, const St<S>& s, S g[3])
+{
+    const int nxt = (s.gate + 1) % tr.n_gates;
+    const dr_gate& a = tr.gates[s.gate];
+    const dr_gate& b = tr.gates[nxt];
+    S gd[3], cd[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        gd[i] = (S)b.center[i] - s.p[i];
+        cd[i] = (S)b.center[i] - (S)a.center[i];
+    }
+    const S g2 = gd[0] * gd[0] + gd[1] * gd[1] + gd[2] * gd[2];
+    const S c2 = cd[0] * cd[0] + cd[1] * cd[1] + cd[2] * cd[2];
+    const S sr = sq_root(g2 / c2);
+    if (sr > (S)0)
+    {
+        const S f = -(S)K.final_adv_weight * (S)0.8f / (sr * c2);
+        for (int i = 0; i < 3; ++i)
+            g[i] = fm(f, gd[i], g[i]);
+    }
+}
+
+template <class S>
+St<S> step_state(const Model<S>& M, const dr_track& tr, S margin, const St<S>& x, const float* U, S u_out[4])
+{
+    St<S> s = x;
+    S uin[4];
+    for (int d = 0; d < 4; ++d)
+        uin[d] = (S)U[d];
+    clamp_u(M, uin, u_out);
+    const S p0[3] = {s.p[0], s.p[1], s.p[2]};
+    euler(M, s, u_out);
+    gate_update(tr, margin, p0, s);
+    return s;
+}
+
+// J(U) and dJ/dU for one control sequence (T x 4); sig (may be NULL)
+// receives a signature of the discrete decisions (clamp activity, gate
+// index sequence, stop step, active cost branches) for kink detection.
+template <class S>
+S cost_grad(const dr_quad_params& P, const dr_cost_cfg& K, const dr_track& tr, const dr_drone_state& ego, const dr_opponent* opp, int n_opp,
+            const float* U, int T, int ck, S* dJdU, uint64_t* sig)
+{
+    const Model<S> M(P);
+    const CostK<S> Q(P, K);
+    const S margin = (S)K.gate_traversal_margin;
+    const int n_ck = (T + ck - 1) / ck;
+    std::vector<St<S>> cks(n_ck);
+    std::vector<S> dec(T + 1);
+    St<S> s = load_state<S>(ego);
+    S acc = 0, decay = 1;
+    int t_end = T - 1;
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+    for (int t = 0; t < T; ++t)
+    {
+        if (t % ck == 0)
+            cks[t / ck] = s;
+        dec[t] = decay;
+        S u[4];
+        for (int d = 0; d < 4; ++d)
+        {
+            const S x = (S)U[t * 4 + d];
+            const S lo = d == 0 ? M.tmin : -M.rmax, hi = d == 0 ? M.tmax : M.rmax;
+            mix((uint64_t)((x < lo) ? 1 : (x > hi ? 2 : 0)));
+        }
+        s = step_state(M, tr, margin, s, U + t * 4, u);
+        mix((uint64_t)(s.gate * 131 + s.laps));
+        int term = 0;
+        const S tau = (S)(M.dt * (S)(t + 1));
+        const S c = running_cost(M, Q, K, tr, s, opp, n_opp, tau, &term);
+        // active-branch signature of the running cost
+        {
+            S min_sq = (S)INFINITY;
+            uint64_t bits = 0;
+            for (int j = 0; j < n_opp; ++j)
+            {
+                S d[3];
+                for (int i = 0; i < 3; ++i)
+                    d[i] = s.p[i] - fm((S)opp[j].v[i], tau, (S)opp[j].p[i]);
+                const S d2 = fm(d[0], d[0], fm(d[1], d[1], d[2] * d[2]));
+                if (d2 < min_sq)
+                {
+                    min_sq = d2;
+                    bits = (bits & ~0xffull) | (uint64_t)j;
+                }
+                if (d2 < Q.r2_prox)
+                    bits |= 1ull << (8 + j);
+            }
+            S d_opp = (S)INFINITY;
+            if (min_sq < Q.far2)
+                d_opp = sq_root(min_sq) - M.radius;
+            const S d_ground = s.p[2] - (S)tr.ground_z;
+            const S bd = d_opp < d_ground ? d_opp : d_ground;
+            bits |= (uint64_t)(min_sq < Q.far2) << 20;
+            bits |= (uint64_t)(d_opp < d_ground) << 21;
+            bits |= (uint64_t)(bd < Q.danger) << 22;
+            mix(bits);
+        }
+        acc = fm(c, decay, acc);
+        decay = decay * (S)0.97f;
+        if (term)
+        {
+            t_end = t;
+            mix(1000 + (uint64_t)t);
+            break;
+        }
+    }
+    const S J = acc + (-(S)K.final_adv_weight * progress(tr, s));
+    if (sig)
+        *sig = h;
+    if (!dJdU)
+        return J;
+    for (int i = 0; i < T * 4; ++i)
+        dJdU[i] = 0;
+    G13<S> g;
+    for (int i = 0; i < 3; ++i)
+        g.p[i] = g.v[i] = g.w[i] = 0;
+    for (int i = 0; i < 4; ++i)
+        g.q[i] = 0;
+    final_grad(K, tr, s, g.p);
+    stage_grad(M, Q, K, tr, s, opp, n_opp, (S)(M.dt * (S)(t_end + 1)), dec[t_end], g.p);
+    std::vector<St<S>> seg(ck + 1);
+    for (int sg = t_end / ck; sg >= 0; --sg)
+    {
+        const int t0 = sg * ck;
+        const int t1 = std::min(t0 + ck, t_end + 1);
+        seg[0] = cks[sg];
+        for (int t = t0; t < t1; ++t)
+        {
+            S u[4];
+            seg[t - t0 + 1] = step_state(M, tr, margin, seg[t - t0], U + t * 4, u);
+        }
+        for (int t = t1 - 1; t >= t0; --t)
+        {
+            const St<S>& x = seg[t - t0];
+            S uin[4], u[4], gu[4];
+            for (int d = 0; d < 4; ++d)
+                uin[d] = (S)U[t * 4 + d];
+            clamp_u(M, uin, u);
+            vjp_step(M, x, u, g, gu);
+            for (int d = 0; d < 4; ++d)
+            {
+                const S lo = d == 0 ? M.tmin : -M.rmax, hi = d == 0 ? M.tmax : M.rmax;
+                dJdU[t * 4 + d] = (uin[d] >= lo && uin[d] <= hi) ? gu[d] : (S)0;
+            }
+            if (t >= 1)
+                stage_grad(M, Q, K, tr, x, opp, n_opp, (S)(M.dt * (S)t), dec[t - 1], g.p);
+        }
+    }
+    return J;
 }
 
 } // namespace
@@ -399,6 +678,40 @@ int or_count_rollout_ops(const dr_quad_params* P, const dr_cost_cfg* K, const dr
         (void)rollout_one<Cn>(*P, M, *K, *tr, *ego, opp, n_opp, nominal, T, eps.data(), 4);
     }
     *flops_out = g_tally.flops;
+    return 0;
+}
+
+// J and dJ/dU for n_seq control sequences U[n_seq][T][4] (noise-free,
+// checkpoints every ck steps); sig (may be NULL) gets the discrete-decision
+// signature of each sequence.
+int or_cost_grad(int f64, const dr_quad_params* P, const dr_cost_cfg* K, const dr_track* tr, const dr_drone_state* ego,
+                 const dr_opponent* opp, int n_opp, const float* U, int n_seq, int T, int ck, double* J, double* dJdU, uint64_t* sig,
+                 int n_threads)
+{
+    if (!P || !K || !tr || !ego || !U || !J || n_seq < 1 || T < 1 || ck < 1 || (n_opp > 0 && !opp))
+        return 1;
+#pragma omp parallel for schedule(dynamic) num_threads(n_threads > 0 ? n_threads : 1)
+    for (int i = 0; i < n_seq; ++i)
+    {
+        if (f64)
+        {
+            std::vector<double> g(dJdU ? (size_t)T * 4 : 0);
+            J[i] = cost_grad<double>(*P, *K, *tr, *ego, opp, n_opp, U + (size_t)i * T * 4, T, ck, dJdU ? g.data() : nullptr,
+                                     sig ? sig + i : nullptr);
+            if (dJdU)
+                for (int j = 0; j < T * 4; ++j)
+                    dJdU[(size_t)i * T * 4 + j] = g[j];
+        }
+        else
+        {
+            std::vector<float> g(dJdU ? (size_t)T * 4 : 0);
+            J[i] = (double)cost_grad<float>(*P, *K, *tr, *ego, opp, n_opp, U + (size_t)i * T * 4, T, ck, dJdU ? g.data() : nullptr,
+                                            sig ? sig + i : nullptr);
+            if (dJdU)
+                for (int j = 0; j < T * 4; ++j)
+                    dJdU[(size_t)i * T * 4 + j] = (double)g[j];
+        }
+    }
     return 0;
 }
 

@@ -48,6 +48,9 @@ def lib(abi):
     L.or_world_run.argtypes = [C.POINTER(abi.QuadParams), C.POINTER(abi.CostCfg), C.POINTER(abi.Track),
                                C.POINTER(abi.Instance), I, C.POINTER(C.c_int32), C.POINTER(abi.DroneState), I, F,
                                C.POINTER(U), F, I, I, F, F]
+    L.or_cost_grad.argtypes = [I, C.POINTER(abi.QuadParams), C.POINTER(abi.CostCfg), C.POINTER(abi.Track),
+                               C.POINTER(abi.DroneState), C.POINTER(abi.Opponent), I, F, I, I, I, D, D,
+                               C.POINTER(C.c_uint64), I]
     _lib = L
     return L
 
@@ -139,6 +142,20 @@ class Oracle:
         assert rc == 0
         return dict(updated=upd.reshape(T, 4), action=act, shifted=sh.reshape(T, 4), min_cost=mn.value, nu=nu.value,
                     weights=w)
+
+    def cost_grad(self, f64, params, cost, track, ego, opponents, U, ck=10, grad=True, threads=8):
+        """J, dJ/dU and the discrete-decision signature of each sequence U[n][T][4]."""
+        U = np.ascontiguousarray(U, dtype=np.float32)
+        n, T = U.shape[0], U.shape[1]
+        J = np.empty(n)
+        G = np.empty((n, T, 4)) if grad else None
+        sig = np.empty(n, dtype=np.uint64)
+        opp = (self.abi.Opponent * max(1, len(opponents)))(*opponents)
+        rc = self.L.or_cost_grad(int(f64), C.byref(params), C.byref(cost), C.byref(track), C.byref(ego), opp,
+                                 len(opponents), _f(U), n, T, ck, _d(J), _d(G) if grad else None,
+                                 sig.ctypes.data_as(C.POINTER(C.c_uint64)), threads)
+        assert rc == 0
+        return J, G, sig
 
     def world_run(self, sc, drones, nominal, steps, lambdas, n_steps, threads=1):
         """Closed-loop FP32 run; drones/nominal/steps/lambdas are updated in place."""

@@ -111,6 +111,41 @@ def batch_vs_per_instance(pkg, steps=50, warmup=5):
             "sample_steps_per_batch_step": scn.sample_steps_per_step, "steps": steps}
 
 
+def adjoint_vs_plain(pkg, steps=100, warmup=10, cost_steps=100):
+    """8 drones on the 12-gate lemniscate: plain MPPI at K=16384 against MPPI at
+    K=4096 followed by 3 adjoint gradient steps (step 0.05, checkpoints every
+    10) on the weighted nominal.  Throughput: closed-loop control steps/s
+    (graph replays).  Quality: mean over controllers and the first cost_steps
+    steps of the nominal's noise-free cost (the plan each step executes),
+    from the same start, plus gates passed at the end."""
+    sc = pkg.scenarios
+    out = {}
+    for name, K, refine in (("plain_k16384", 16384, 0), ("adjoint_k4096", 4096, 3)):
+        scn = sc.c4_eight_drones(K)
+        n = len(scn.instances)
+        eng = pkg.Engine(scn)
+        eng.set_refine(refine, 0.05, 10)
+        eng.step(warmup, advance=True)
+        t0 = time.perf_counter()
+        eng.step(steps, advance=True)
+        sps = n * steps / (time.perf_counter() - t0)
+        eng.close()
+        eng = pkg.Engine(scn)
+        eng.set_refine(refine, 0.05, 10)
+        eng.set_eval(True)
+        costs = []
+        for _ in range(cost_steps):
+            eng.step(1, advance=True)
+            costs.append(sum(d["traj_cost"] for d in eng.diag()) / n)
+        prog = [d.gate + d.laps * scn.tracks[0].n_gates for d in eng.get_drones()]
+        eng.close()
+        out[name] = {"control_steps_per_s": sps, "mean_traj_cost": sum(costs) / len(costs),
+                     "gates_passed_mean": sum(prog) / len(prog) - (scn.tracks[0].n_gates - 1)}
+    out["speedup"] = out["adjoint_k4096"]["control_steps_per_s"] / out["plain_k16384"]["control_steps_per_s"]
+    out["cost_ratio"] = out["adjoint_k4096"]["mean_traj_cost"] / out["plain_k16384"]["mean_traj_cost"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +157,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-batch64", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 512 MiB buffer 3 times (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -220,6 +256,10 @@ def main():
     }
     if tuned:
         out["config"]["autotuned"] = True
+    if not args.no_adjoint:
+        log("bench: 8 drones, adjoint-refined K=4096 vs plain K=16384 ...")
+        out["adjoint"] = adjoint_vs_plain(pkg)
+        log(f"bench: adjoint {json.dumps(out['adjoint'])}")
     if not args.no_batch64:
         log("bench: 64 uneven controllers, batched vs per-instance ...")
         out["batch64"] = batch_vs_per_instance(pkg)

@@ -182,6 +182,15 @@ def c2_batch64(seed=3, n_inst=64, steps=200):
                     "64 single-drone controllers, K~logU[1024,32768], T~U{30..150}, own 8-12 gate circle r~U(15,40)")
 
 
+def c4_eight_drones(n_samples=4096, n_timesteps=100, seed=1):
+    """8 drones, one controller each, on the 12-gate lemniscate of c1 (seed 1)."""
+    tr = track_lemniscate(seed, 12, 30.0, 1.5, 0.3)
+    drones = start_states(tr, 8, spacing=2.5)
+    inst = [instance(n_samples, n_timesteps, 0, j, seed_mix(seed, 100 + j)) for j in range(8)]
+    return Scenario(f"c4_8drones_k{n_samples}_t{n_timesteps}", [tr], inst, [0] * 8, drones, 500,
+                    f"8 drones, one controller each, K={n_samples}, T={n_timesteps}, 12-gate lemniscate a=30 m, seed {seed}")
+
+
 def seed_mix(seed, i):
     """64-bit Philox key for instance i of a scenario seeded `seed`."""
     return ((int(seed) * 0x9E3779B97F4A7C15) ^ (0xA5A5A5A5 + 0x632BE59BD9B4E019 * (i + 1))) & 0xFFFFFFFFFFFFFFFF

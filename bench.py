@@ -146,6 +146,34 @@ def adjoint_vs_plain(pkg, steps=100, warmup=10, cost_steps=100):
     return out
 
 
+def pr_mppi(pkg, steps=100, warmup=10):
+    """PR-MPPI, 4 drones, B ~ U{3..6} opponent-velocity branches per controller,
+    Dirichlet(1) beliefs, K = 8192 samples evaluated under every branch for
+    both plans, T = 80, closed loop, graph replays."""
+    sc = pkg.scenarios
+    scn = sc.c3_pr_four_drones()
+    n = len(scn.instances)
+    eng = pkg.PREngine(scn)
+    eng.step(warmup, advance=True)
+    t0 = time.perf_counter()
+    eng.step(steps, advance=True)
+    elapsed = time.perf_counter() - t0
+    eng.set_timing(True)
+    eng.step(10, advance=True)
+    ms = eng.last_timing() / 10
+    eng.set_timing(False)
+    diag = eng.diag()
+    eng.close()
+    rollouts = sum(2 * i.n_branches * i.n_samples for i in scn.instances)
+    ss = sum(2 * i.n_branches * i.n_samples * i.n_timesteps for i in scn.instances)
+    return {"workload": scn.name, "branches": [i.n_branches for i in scn.instances],
+            "control_steps_per_s": n * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+            "rollouts_per_step": rollouts, "rollout_sample_steps_per_s": ss * steps / elapsed,
+            "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate": float(ms[1]), "weighted_avg": float(ms[2]),
+                                   "update_choice_advance": float(ms[3]), "step": float(ms[4])},
+            "use_nom_plan": [int(d.use_nom_plan) for d in diag], "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +186,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-batch64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-pr", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 512 MiB buffer 3 times (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -260,6 +289,10 @@ def main():
         log("bench: 8 drones, adjoint-refined K=4096 vs plain K=16384 ...")
         out["adjoint"] = adjoint_vs_plain(pkg)
         log(f"bench: adjoint {json.dumps(out['adjoint'])}")
+    if not args.no_pr:
+        log("bench: PR-MPPI, 4 drones, K=8192 per branch, T=80 ...")
+        out["pr_mppi"] = pr_mppi(pkg)
+        log(f"bench: pr_mppi {json.dumps(out['pr_mppi'])}")
     if not args.no_batch64:
         log("bench: 64 uneven controllers, batched vs per-instance ...")
         out["batch64"] = batch_vs_per_instance(pkg)

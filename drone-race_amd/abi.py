@@ -14,6 +14,9 @@ MAX_GATES = 16
 MAX_T = 256
 MAX_WORLD_DRONES = 8
 
+MAX_BRANCHES = 8
+MAX_MODELS = 16
+
 DR_OK, DR_EINVAL, DR_EHIP, DR_ENOMEM, DR_ESTATE = 0, 1, 2, 3, 4
 
 
@@ -70,6 +73,19 @@ class MPPICfg(C.Structure):
                 ("max_opponents", C.c_int32)]
 
 
+class PrInstance(C.Structure):
+    _fields_ = [("n_samples", C.c_int32), ("n_timesteps", C.c_int32), ("track", C.c_int32), ("drone", C.c_int32),
+                ("seed", C.c_uint64), ("lambda_", C.c_float), ("sigma", C.c_float * 4), ("shift_fill", C.c_float * 4),
+                ("adapt_lambda", C.c_int32), ("n_branches", C.c_int32), ("sampled", C.c_int32), ("n_models", C.c_int32),
+                ("safety_weight", C.c_float), ("belief", C.c_float * MAX_BRANCHES),
+                ("vel_shift", ((C.c_float * 3) * (MAX_WORLD_DRONES - 1)) * MAX_BRANCHES)]
+
+
+class PrDiag(C.Structure):
+    _fields_ = [("min_costs", C.c_float * 3), ("nu", C.c_float * 3), ("lambda_", C.c_float * 3), ("cand_costs", C.c_float * 2),
+                ("use_nom_plan", C.c_int32), ("reset_nom", C.c_int32), ("step", C.c_uint32), ("_pad", C.c_int32)]
+
+
 # Every symbol include/dr_api.h declares, with its ctypes signature.
 _P = C.c_void_p
 _I = C.c_int
@@ -109,6 +125,27 @@ SIGNATURES = {
     "dr_mppi_set_nominal": (_I, [_P, _F, _I]),
     "dr_mppi_get_diag": (_I, [_P, C.POINTER(Diag)]),
     "dr_mppi_engine": (_P, [_P]),
+    "dr_pr_create": (_I, [C.POINTER(QuadParams), C.POINTER(CostCfg), C.POINTER(Track), _I, C.POINTER(PrInstance), _I,
+                          C.POINTER(C.c_int32), _I, C.POINTER(_P)]),
+    "dr_pr_destroy": (None, [_P]),
+    "dr_pr_set_drones": (_I, [_P, C.POINTER(DroneState), _I]),
+    "dr_pr_get_drones": (_I, [_P, C.POINTER(DroneState), _I]),
+    "dr_pr_set_nominals": (_I, [_P, _I, _F, _F, _I]),
+    "dr_pr_get_nominals": (_I, [_P, _I, _F, _F, _I]),
+    "dr_pr_set_belief": (_I, [_P, _I, _F, _I]),
+    "dr_pr_step": (_I, [_P, _I, _I, _F]),
+    "dr_pr_get_diag": (_I, [_P, C.POINTER(PrDiag), _I]),
+    "dr_pr_get_noise": (_I, [_P, _I, _F, C.c_size_t]),
+    "dr_pr_get_aggregates": (_I, [_P, _I, _F, _I]),
+    "dr_pr_get_candidates": (_I, [_P, _I, _F, _F, _I]),
+    "dr_pr_get_models": (_I, [_P, _I, C.POINTER(C.c_int32), _I]),
+    "dr_pr_get_pair_costs": (_I, [_P, _I, _F, _I]),
+    "dr_pr_step_async": (_I, [_P, _I, _I]),
+    "dr_pr_synchronize": (_I, [_P]),
+    "dr_pr_set_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
+    "dr_pr_get_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
+    "dr_pr_set_timing": (_I, [_P, _I]),
+    "dr_pr_last_timing": (_I, [_P, _F, _I]),
 }
 
 _lib = None

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "dr_host.h"
 #include "dr_kernels.h"
 
 namespace dr {
@@ -53,42 +54,27 @@ int fail(int code, const std::string& msg)
     return code;
 }
 
-#define HIPCHK(call)                                                                                                                           \
-    do                                                                                                                                         \
-    {                                                                                                                                          \
-        hipError_t _e = (call);                                                                                                                \
-        if (_e != hipSuccess)                                                                                                                  \
-            return fail(DR_EHIP, std::string(#call) + ": " + hipGetErrorString(_e));                                                          \
-    } while (0)
-
-template <class T>
-struct DevBuf
-{
-    T* p = nullptr;
-    size_t n = 0;
-    ~DevBuf() { release(); }
-    void release()
-    {
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    hipError_t alloc(size_t count)
-    {
-        release();
-        n = count;
-        if (count == 0)
-            return hipSuccess;
-        return hipMalloc((void**)&p, count * sizeof(T));
-    }
-};
-
 bool valid_block(int b) { return b == 64 || b == 128 || b == 256; }
 bool valid_chunk(int c) { return c >= 64 && c <= WAVG_MAX_CHUNK && c % 64 == 0; }
 bool valid_nb(int b) { return b == 1 || b == 2 || b == 4; }
 
 } // namespace
+
+namespace dr {
+
+int host_fail(int code, const std::string& msg) { return fail(code, msg); }
+
+int check_track(const dr_track& t, int idx)
+{
+    if (t.n_gates < 2 || t.n_gates > DR_MAX_GATES)
+        return fail(DR_EINVAL, "track " + std::to_string(idx) + ": n_gates must be in [2, 16]");
+    for (int g = 0; g < t.n_gates; ++g)
+        if (!(t.gates[g].radius > 0.0f))
+            return fail(DR_EINVAL, "track " + std::to_string(idx) + ": gate radius must be > 0");
+    return DR_OK;
+}
+
+} // namespace dr
 
 struct dr_engine
 {
@@ -402,16 +388,6 @@ extern "C" {
 const char* dr_last_error(void) { return g_err.c_str(); }
 
 int dr_version(void) { return DR_ABI_VERSION; }
-
-static int check_track(const dr_track& t, int idx)
-{
-    if (t.n_gates < 2 || t.n_gates > DR_MAX_GATES)
-        return fail(DR_EINVAL, "track " + std::to_string(idx) + ": n_gates must be in [2, 16]");
-    for (int g = 0; g < t.n_gates; ++g)
-        if (!(t.gates[g].radius > 0.0f))
-            return fail(DR_EINVAL, "track " + std::to_string(idx) + ": gate radius must be > 0");
-    return DR_OK;
-}
 
 int dr_engine_create(const dr_quad_params* params, const dr_cost_cfg* cost, const dr_track* tracks, int n_tracks,
                      const dr_instance* instances, int n_instances, const int32_t* drone_world, int n_drones, dr_engine** out)

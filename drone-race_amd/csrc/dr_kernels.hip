@@ -14,24 +14,9 @@
 
 #include "dr_adjoint.h"
 #include "dr_kernels.h"
+#include "dr_wave.h"
 
 namespace dr {
-
-__device__ __forceinline__ float wave_min(float v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1)
-        v = fminf(v, __shfl_xor(v, m, WAVE));
-    return v;
-}
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1)
-        v += __shfl_xor(v, m, WAVE);
-    return v;
-}
 
 // One lane = one sample k of one instance.  Per step:
 // Philox noise (4 normals) -> float4 store of sigma*z into the sample's noise
@@ -780,35 +765,6 @@ __global__ __launch_bounds__(WAVG_BLOCK) void k_wavg(const InstDev* __restrict__
             v += s_acc[q * TG + r][d];
         out[i] = v;
     }
-}
-
-// Sum of n values p[0], p[stride], ... in index order; loads are issued in
-// batches of 8 so that their latencies overlap (same order as a plain loop).
-__device__ __forceinline__ double sum_in_order(const double* __restrict__ p, int n, long long stride)
-{
-    double acc = 0.0;
-    for (int c0 = 0; c0 < n; c0 += 8)
-    {
-        double v[8];
-#pragma unroll
-        for (int b = 0; b < 8; ++b)
-            v[b] = (c0 + b < n) ? p[(long long)(c0 + b) * stride] : 0.0;
-#pragma unroll
-        for (int b = 0; b < 8; ++b)
-            acc += v[b];
-    }
-    return acc;
-}
-
-__device__ __forceinline__ void advance_drone(dr_drone_state& st, const float* act, const dr_track& tr, const QuadConst& C, float margin)
-{
-    QState s = state_from_abi(st);
-    float u[4];
-    clamp_control(C, act, u);
-    const float p0[3] = {s.p[0], s.p[1], s.p[2]};
-    quad_step(C, s, u);
-    update_gate(tr.gates, tr.n_gates, margin, p0, s);
-    state_to_abi(s, st);
 }
 
 // Per instance: numerator and denominator over chunks in fixed order,

@@ -251,6 +251,54 @@ DR_HD float stage_cost(const QuadConst& C, const CostConst& Q, const dr_cost_cfg
     return cost;
 }
 
+// PR-MPPI running cost and safety on the post-step state (costs.cuh:44-63
+// and :78-86 structure): the terms of stage_cost without the collision
+// cost (a collision still ends the rollout), and
+//   safety = r*coll - bd   (> 0 iff the state is unsafe)
+// with the exact boundary distance (the square root is always taken, since
+// the safety value is used as a cost of its own).  opp_at(j) returns the
+// predicted position of opponent j at this step.
+template <class OppAt>
+DR_HD float pr_stage_cost(const QuadConst& C, const CostConst& Q, const dr_cost_cfg& K, const dr_track& tr, const QState& s, int n_opp,
+                          OppAt opp_at, float* safety, int* term)
+{
+    float cost = 0.0f;
+    float min_sq = __builtin_inff();
+    for (int j = 0; j < n_opp; ++j)
+    {
+        const float4 o = opp_at(j);
+        const float dx = s.p[0] - o.x;
+        const float dy = s.p[1] - o.y;
+        const float dz = s.p[2] - o.z;
+        const float d2 = fma_(dx, dx, fma_(dy, dy, dz * dz));
+        min_sq = fminf(min_sq, d2);
+        if (d2 < Q.r2_prox)
+        {
+            const float x = fma_(-__builtin_sqrtf(d2), Q.inv_prox, 1.0f);
+            cost = fma_(K.opp_prox_cost, x * x, cost);
+        }
+    }
+    const float d_opp = __builtin_sqrtf(min_sq) - C.drone_radius;
+    const float d_ground = s.p[2] - tr.ground_z;
+    const float bd = fminf(d_opp, d_ground);
+    if (bd < Q.danger)
+    {
+        const float x = fma_(-bd, Q.inv_danger, 1.0f);
+        cost = fma_(K.boundary_cost, x * x, cost);
+    }
+    *safety = Q.coll - bd;
+    int t = TERM_NONE;
+    if (bd < Q.coll)
+        t = TERM_LOSE;
+    else if (s.laps >= tr.n_win_laps)
+    {
+        cost = cost - K.win_cost;
+        t = TERM_WIN;
+    }
+    *term = t;
+    return cost;
+}
+
 // Predicted opponent position after t+1 control steps: p + v * (dt * (t+1)).
 DR_HD float4 opp_predict(const dr_drone_state& o, float tau)
 {
@@ -296,6 +344,19 @@ DR_HD void state_to_abi(const QState& s, dr_drone_state& d)
         d.q[i] = s.q[i];
     d.gate = s.gate;
     d.laps = s.laps;
+}
+
+// Closed-loop step of a controlled drone: clamp its action, one Euler step,
+// gate update (SimulationEngine::dynStep, engine.cpp:286-312).
+DR_HD void advance_drone(dr_drone_state& st, const float* act, const dr_track& tr, const QuadConst& C, float margin)
+{
+    QState s = state_from_abi(st);
+    float u[4];
+    clamp_control(C, act, u);
+    const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+    quad_step(C, s, u);
+    update_gate(tr.gates, tr.n_gates, margin, p0, s);
+    state_to_abi(s, st);
 }
 
 } // namespace dr

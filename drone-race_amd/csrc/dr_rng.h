@@ -32,6 +32,7 @@ enum : uint32_t
     STREAM_CONTROL = 0u, // control sampling noise, counter (k, t, step, 0)
     STREAM_TRACK = 1u,   // scenario generation (gate jitter, radii, sizes)
     STREAM_BRANCH = 2u,  // PR-MPPI branch beliefs and opponent velocity shifts
+    STREAM_MODEL = 3u,   // PR-MPPI opponent-model draws, counter (m, 0, step, 3)
 };
 
 struct U4
@@ -194,6 +195,23 @@ DR_HD float expf_ieee(float x)
     p = __builtin_fmaf(p, r, 1.0f);
     const int n = (int)nf;
     return p * u2f((uint32_t)(n + 127) << 23);
+}
+
+// Opponent model m of a sampled PR-MPPI step: inverse CDF of the belief at
+// u in (0, 1] (prmppi_kernels.cu:34-47: running FP32 sum, first i with
+// u <= sum, else the last model).
+DR_HD int sample_branch(uint32_t k0, uint32_t k1, uint32_t m, uint32_t step, const float* belief, int n_branches)
+{
+    const U4 r = philox4x32_10(U4{m, 0u, step, STREAM_MODEL}, k0, k1);
+    const float u = uniform_open0(r.x);
+    float acc = 0.0f;
+    for (int i = 0; i < n_branches; ++i)
+    {
+        acc = acc + belief[i];
+        if (u <= acc)
+            return i;
+    }
+    return n_branches - 1;
 }
 
 } // namespace dr

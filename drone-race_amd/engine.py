@@ -193,3 +193,139 @@ class MPPI:
         d = abi.Diag()
         check(self.lib.dr_mppi_get_diag(self.h, C.byref(d)))
         return dict(min_cost=d.min_cost, nu=d.nu, lam=d.lambda_, traj_cost=d.traj_cost, step=d.step)
+
+
+class PREngine:
+    """A batch of PR-MPPI controllers (PRMPPIController, include/controllers.h:114-179).
+
+    get_nominals() returns (h_nom_nominal, h_rob_nominal); diag() carries
+    use_nom_plan / reset_nom / the three minima, nu and temperatures."""
+
+    def __init__(self, scenario):
+        lib = abi.load()
+        self.lib = lib
+        self.sc = scenario
+        n_t = len(scenario.tracks)
+        self._tracks = (abi.Track * n_t)(*scenario.tracks)
+        n_i = len(scenario.instances)
+        self._inst = (abi.PrInstance * n_i)(*scenario.instances)
+        n_d = len(scenario.drone_world)
+        self._world = (C.c_int32 * n_d)(*scenario.drone_world)
+        self.n_inst, self.n_drones = n_i, n_d
+        h = C.c_void_p()
+        check(lib.dr_pr_create(C.byref(scenario.params), C.byref(scenario.cost), self._tracks, n_t, self._inst, n_i,
+                               self._world, n_d, C.byref(h)))
+        self.h = h
+        self.set_drones(scenario.drones)
+        for i, ins in enumerate(scenario.instances):
+            nom = np.tile(np.array(ins.shift_fill[:], dtype=np.float32), (ins.n_timesteps, 1))
+            self.set_nominals(i, nom, nom)
+
+    def close(self):
+        if self.h:
+            self.lib.dr_pr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def n_models(self, i):
+        ins = self.sc.instances[i]
+        return ins.n_models if ins.sampled else ins.n_branches
+
+    def set_drones(self, states):
+        arr = (abi.DroneState * self.n_drones)(*states)
+        check(self.lib.dr_pr_set_drones(self.h, arr, self.n_drones))
+
+    def get_drones(self):
+        arr = (abi.DroneState * self.n_drones)()
+        check(self.lib.dr_pr_get_drones(self.h, arr, self.n_drones))
+        return arr
+
+    def set_nominals(self, i, nom, rob):
+        a = np.ascontiguousarray(nom, dtype=np.float32).reshape(-1)
+        b = np.ascontiguousarray(rob, dtype=np.float32).reshape(-1)
+        check(self.lib.dr_pr_set_nominals(self.h, i, _fptr(a), _fptr(b), a.size))
+
+    def get_nominals(self, i):
+        T = self.sc.instances[i].n_timesteps
+        a = np.empty(T * 4, dtype=np.float32)
+        b = np.empty(T * 4, dtype=np.float32)
+        check(self.lib.dr_pr_get_nominals(self.h, i, _fptr(a), _fptr(b), a.size))
+        return a.reshape(T, 4), b.reshape(T, 4)
+
+    def set_belief(self, i, belief):
+        a = np.ascontiguousarray(belief, dtype=np.float32)
+        check(self.lib.dr_pr_set_belief(self.h, i, _fptr(a), a.size))
+
+    def step(self, n_steps=1, advance=False):
+        a = np.empty(self.n_inst * 4, dtype=np.float32)
+        check(self.lib.dr_pr_step(self.h, n_steps, 1 if advance else 0, _fptr(a)))
+        return a.reshape(self.n_inst, 4)
+
+    def step_async(self, n_steps=1, advance=False):
+        check(self.lib.dr_pr_step_async(self.h, n_steps, 1 if advance else 0))
+
+    def synchronize(self):
+        check(self.lib.dr_pr_synchronize(self.h))
+
+    def diag(self):
+        arr = (abi.PrDiag * self.n_inst)()
+        check(self.lib.dr_pr_get_diag(self.h, arr, self.n_inst))
+        return arr
+
+    def noise(self, i):
+        ins = self.sc.instances[i]
+        a = np.empty(ins.n_samples * ins.n_timesteps * 4, dtype=np.float32)
+        check(self.lib.dr_pr_get_noise(self.h, i, _fptr(a), a.size))
+        return a.reshape(ins.n_samples, ins.n_timesteps, 4)
+
+    def pair_costs(self, i):
+        """[plan][m][k][cost, max safety] of the last step."""
+        ins = self.sc.instances[i]
+        M = self.n_models(i)
+        a = np.empty(2 * M * ins.n_samples * 2, dtype=np.float32)
+        check(self.lib.dr_pr_get_pair_costs(self.h, i, _fptr(a), a.size))
+        return a.reshape(2, M, ins.n_samples, 2)
+
+    def aggregates(self, i):
+        ins = self.sc.instances[i]
+        a = np.empty(ins.n_samples * 4, dtype=np.float32)
+        check(self.lib.dr_pr_get_aggregates(self.h, i, _fptr(a), a.size))
+        return a.reshape(ins.n_samples, 4)
+
+    def candidates(self, i):
+        T = self.sc.instances[i].n_timesteps
+        a = np.empty(T * 4, dtype=np.float32)
+        b = np.empty(T * 4, dtype=np.float32)
+        check(self.lib.dr_pr_get_candidates(self.h, i, _fptr(a), _fptr(b), a.size))
+        return a.reshape(T, 4), b.reshape(T, 4)
+
+    def models(self, i):
+        M = self.n_models(i)
+        a = (C.c_int32 * M)()
+        check(self.lib.dr_pr_get_models(self.h, i, a, M))
+        return np.array(a[:], dtype=np.int32)
+
+    def set_launch(self, **kw):
+        cfg = self.get_launch()
+        for k, v in kw.items():
+            setattr(cfg, k, int(v))
+        check(self.lib.dr_pr_set_launch(self.h, C.byref(cfg)))
+
+    def get_launch(self):
+        cfg = abi.LaunchCfg()
+        check(self.lib.dr_pr_get_launch(self.h, C.byref(cfg)))
+        return cfg
+
+    def set_timing(self, on=True):
+        check(self.lib.dr_pr_set_timing(self.h, 1 if on else 0))
+
+    def last_timing(self):
+        """Device ms summed over the last timed run: rollout, aggregation, weighted averages, update, step."""
+        a = np.empty(5, dtype=np.float32)
+        check(self.lib.dr_pr_last_timing(self.h, _fptr(a), 5))
+        return a

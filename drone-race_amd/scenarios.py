@@ -22,7 +22,7 @@ import math
 import numpy as np
 
 from . import abi
-from .philox import uniforms, STREAM_TRACK
+from .philox import uniforms, normals, STREAM_TRACK, STREAM_BRANCH
 
 HOVER_FILL = None  # set from the quad params below
 
@@ -189,6 +189,56 @@ def c4_eight_drones(n_samples=4096, n_timesteps=100, seed=1):
     inst = [instance(n_samples, n_timesteps, 0, j, seed_mix(seed, 100 + j)) for j in range(8)]
     return Scenario(f"c4_8drones_k{n_samples}_t{n_timesteps}", [tr], inst, [0] * 8, drones, 500,
                     f"8 drones, one controller each, K={n_samples}, T={n_timesteps}, 12-gate lemniscate a=30 m, seed {seed}")
+
+
+def pr_instance(n_samples, n_timesteps, track, drone, seed, belief, vel_shift, lam=1.0, sigma=(2.0, 0.5, 0.5, 0.5), fill=None,
+                adapt=0, sampled=0, n_models=0, safety_weight=10000.0):
+    """One PR-MPPI controller: belief[B], vel_shift[B][opponent slot][3] (opponent slots are the
+    other drones of the world in increasing drone index)."""
+    fill = hover_control() if fill is None else fill
+    ins = abi.PrInstance(n_samples=n_samples, n_timesteps=n_timesteps, track=track, drone=drone, seed=seed, lambda_=lam,
+                         sigma=(abi.C.c_float * 4)(*sigma), shift_fill=(abi.C.c_float * 4)(*fill), adapt_lambda=adapt,
+                         n_branches=len(belief), sampled=sampled, n_models=n_models, safety_weight=safety_weight)
+    for b, p in enumerate(belief):
+        ins.belief[b] = float(p)
+    vs = np.asarray(vel_shift, dtype=np.float32)
+    for b in range(vs.shape[0]):
+        for j in range(vs.shape[1]):
+            for c in range(3):
+                ins.vel_shift[b][j][c] = float(vs[b, j, c])
+    return ins
+
+
+def branch_draws(seed, i, n_opp, b_min=3, b_max=6):
+    """B ~ U{b_min..b_max}, belief ~ Dirichlet(1) (normalised Exp(1) draws) and per-branch
+    opponent-velocity shifts ~ N(0, 1) m/s per opponent and axis, from Philox stream 2."""
+    u = uniforms(seed, 1 + abi.MAX_BRANCHES, STREAM_BRANCH, salt=2 * i)
+    B = b_min + int(u[0] * (b_max - b_min + 1))
+    e = -np.log1p(-u[1:1 + B])
+    belief = (e / e.sum()).astype(np.float32)
+    z = normals(seed, B * max(n_opp, 1) * 3, STREAM_BRANCH, salt=2 * i + 1)
+    shift = z.reshape(B, max(n_opp, 1), 3)[:, :n_opp, :].astype(np.float32)
+    return B, belief, shift
+
+
+class PrScenario(Scenario):
+    """Scenario of PR-MPPI controllers (dr_pr_create)."""
+
+
+def c3_pr_four_drones(seed=2, n_samples=8192, n_timesteps=80, steps=500, sampled=0, n_models=0):
+    """PR-MPPI, 4 drones on the 12-gate lemniscate (a = 30 m) generated with the seed; each
+    controller holds B ~ U{3..6} opponent-velocity branches with a Dirichlet(1) belief; K samples
+    evaluated under every branch, T steps."""
+    tr = track_lemniscate(seed, 12, 30.0, 1.5, 0.3)
+    drones = start_states(tr, 4)
+    inst = []
+    for j in range(4):
+        B, belief, shift = branch_draws(seed, j, 3)
+        inst.append(pr_instance(n_samples, n_timesteps, 0, j, seed_mix(seed, 200 + j), belief, shift, sampled=sampled,
+                                n_models=n_models))
+    return PrScenario(f"c3_pr_4drones_k{n_samples}_t{n_timesteps}", [tr], inst, [0, 0, 0, 0], drones, steps,
+                      f"PR-MPPI, 4 drones, B~U{{3..6}} branches, Dirichlet(1) belief, N(0,1) velocity shifts, "
+                      f"K={n_samples} per branch, T={n_timesteps}, 12-gate lemniscate a=30 m, seed {seed}")
 
 
 def seed_mix(seed, i):

@@ -224,6 +224,86 @@ int dr_engine_cost_grad_device(dr_engine* e, int instance, const float* U, int n
  * plain MPPI. */
 int dr_engine_set_refine(dr_engine* e, int n_grad_steps, float step, int ckpt_every);
 
+/* ---------------------------------------------------------------------
+ * PR-MPPI ("Parameter-Robust MPPI"), restating PRMPPIController
+ * (include/controllers.h:114-179, src/prmppi_controller.cu:142-349) for the
+ * quadrotor race.  Opponent-behaviour models: branch b shifts every
+ * opponent's constant velocity by vel_shift[b][j] and has belief[b].
+ * sampled = 0: the models are the B branches, costs averaged with the
+ * belief; sampled = 1: P = n_models models are drawn from the belief each
+ * step (prmppi_kernels.cu:23-51) and averaged uniformly (the reference). */
+#define DR_MAX_BRANCHES 8
+#define DR_MAX_MODELS 16
+
+typedef struct dr_pr_instance
+{
+    int32_t n_samples;
+    int32_t n_timesteps;
+    int32_t track;
+    int32_t drone;
+    uint64_t seed;
+    float lambda;                     /* initial temperature of the three weightings */
+    float sigma[DR_ACTION_DIM];
+    float shift_fill[DR_ACTION_DIM];
+    int32_t adapt_lambda;             /* 1: prmppi_controller.cu:328-346 */
+    int32_t n_branches;               /* B in 1..DR_MAX_BRANCHES */
+    int32_t sampled;                  /* 0: belief-weighted branches; 1: sampled models */
+    int32_t n_models;                 /* P when sampled, 1..DR_MAX_MODELS */
+    float safety_weight;              /* added to a sample's cost if any model is unsafe */
+    float belief[DR_MAX_BRANCHES];
+    float vel_shift[DR_MAX_BRANCHES][DR_MAX_WORLD_DRONES - 1][3];
+} dr_pr_instance;
+
+typedef struct dr_pr_diag
+{
+    float min_costs[3];  /* nominal/full, robust/full, robust/safety */
+    float nu[3];
+    float lambda[3];
+    float cand_costs[2]; /* model-averaged full cost of the two candidates */
+    int32_t use_nom_plan;
+    int32_t reset_nom;
+    uint32_t step;
+    int32_t _pad;
+} dr_pr_diag;
+
+typedef struct dr_prengine dr_prengine;
+
+int dr_pr_create(const dr_quad_params* params, const dr_cost_cfg* cost, const dr_track* tracks, int n_tracks,
+                 const dr_pr_instance* instances, int n_instances, const int32_t* drone_world, int n_drones, dr_prengine** out);
+void dr_pr_destroy(dr_prengine* e);
+int dr_pr_set_drones(dr_prengine* e, const dr_drone_state* states, int n_drones);
+int dr_pr_get_drones(dr_prengine* e, dr_drone_state* states, int n_drones);
+/* Both plans (T x 4 each, nom may equal rob).  As the reference does at the
+ * start of every getControl, the plans are shifted by one step before they
+ * are used (prmppi_controller.cu:163-175). */
+int dr_pr_set_nominals(dr_prengine* e, int instance, const float* nom, const float* rob, int n_floats);
+/* The plans after the last step (unshifted, as h_nom_nominal / h_rob_nominal). */
+int dr_pr_get_nominals(dr_prengine* e, int instance, float* nom, float* rob, int n_floats);
+int dr_pr_set_belief(dr_prengine* e, int instance, const float* belief, int n_branches);
+int dr_pr_step(dr_prengine* e, int n_steps, int advance, float* actions);
+int dr_pr_get_diag(dr_prengine* e, dr_pr_diag* diag, int n_instances);
+/* Results of the last step for tests: noise [k][t][4]; per-sample
+ * aggregates [k] x (nominal full cost, nominal max safety, robust full
+ * cost, robust max safety); candidates cand1, cand2 (T x 4); branch of each
+ * model this step (n_models ints). */
+int dr_pr_get_noise(dr_prengine* e, int instance, float* noise, size_t n);
+int dr_pr_get_aggregates(dr_prengine* e, int instance, float* agg, int n_floats);
+int dr_pr_get_candidates(dr_prengine* e, int instance, float* cand1, float* cand2, int n_floats);
+int dr_pr_get_models(dr_prengine* e, int instance, int32_t* branch_of_model, int n_models);
+/* Cost and max safety of every rollout of the last step, [plan][m][k][2]. */
+int dr_pr_get_pair_costs(dr_prengine* e, int instance, float* out, int n_floats);
+int dr_pr_step_async(dr_prengine* e, int n_steps, int advance);
+int dr_pr_synchronize(dr_prengine* e);
+/* rollout_block: waves*64 per rollout workgroup (0: every rollout of a
+ * sample group, at most 16 waves); reduce_chunk <= 2048; noise_batch and
+ * rollout_split are unused. */
+int dr_pr_set_launch(dr_prengine* e, const dr_launch_cfg* cfg);
+int dr_pr_get_launch(dr_prengine* e, dr_launch_cfg* cfg);
+int dr_pr_set_timing(dr_prengine* e, int enable);
+/* Device ms summed over the last timed run: rollout, aggregation, weighted
+ * averages, update (+ closed-loop advance), whole step. */
+int dr_pr_last_timing(dr_prengine* e, float* ms, int n);
+
 /* Single-controller entry points with the reference's shape. */
 typedef struct dr_mppi_cfg
 {

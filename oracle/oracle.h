@@ -19,6 +19,8 @@
 //   action pick and shift with zero fill     src/mppi_controller.cu:392-397, 430-436
 //   temperature adaptation                   src/mppi_controller.cu:413-424
 //   gate crossing and race progress          include/environments/env_dronerace.h:374-394, 412-470
+//   PR-MPPI step (oracle_pr.cpp)             src/prmppi_kernels.cu:23-393, src/prmppi_controller.cu:142-349,
+//                                            include/costs.cuh:44-86
 // The quadrotor dynamics have no reference counterpart; they are pinned by
 // the physical invariants tested in tests/test_oracle_quad.py.
 //
@@ -404,6 +406,66 @@ inline S running_cost(const Model<S>& M, const CostK<S>& Q, const dr_cost_cfg& K
         *term = 2;
     }
     return cost;
+}
+
+// PR-MPPI running cost and safety (costs.cuh:44-63, :78-86 structure; the
+// engine's pr_stage_cost): the running cost without the collision term and
+// safety = r*coll - bd with the exact boundary distance.  opp[j].v already
+// holds the model's (shifted) velocity.
+template <class S>
+inline S pr_running_cost(const Model<S>& M, const CostK<S>& Q, const dr_cost_cfg& K, const dr_track& tr, const St<S>& s,
+                         const dr_opponent* opp, int n_opp, S tau, S* safety, int* term)
+{
+    S cost = 0;
+    S min_sq = (S)INFINITY;
+    for (int j = 0; j < n_opp; ++j)
+    {
+        S d[3];
+        for (int i = 0; i < 3; ++i)
+            d[i] = s.p[i] - fm((S)opp[j].v[i], tau, (S)opp[j].p[i]);
+        const S d2 = fm(d[0], d[0], fm(d[1], d[1], d[2] * d[2]));
+        min_sq = mn(min_sq, d2);
+        if (d2 < Q.r2_prox)
+        {
+            const S x = fm(-sq_root(d2), Q.inv_prox, (S)1);
+            cost = fm((S)K.opp_prox_cost, x * x, cost);
+        }
+    }
+    const S d_opp = sq_root(min_sq) - M.radius;
+    const S d_ground = s.p[2] - (S)tr.ground_z;
+    const S bd = mn(d_opp, d_ground);
+    if (bd < Q.danger)
+    {
+        const S x = fm(-bd, Q.inv_danger, (S)1);
+        cost = fm((S)K.boundary_cost, x * x, cost);
+    }
+    *safety = Q.coll - bd;
+    *term = 0;
+    if (bd < Q.coll)
+        *term = 1;
+    else if (s.laps >= tr.n_win_laps)
+    {
+        cost = cost - (S)K.win_cost;
+        *term = 2;
+    }
+    return cost;
+}
+
+// Opponent model m of a sampled PR-MPPI step (prmppi_kernels.cu:23-51):
+// u in (0, 1] from Philox counter (m, 0, step, 3); first i whose running
+// FP32 belief sum reaches u, else the last model.
+inline int sample_branch(uint32_t k0, uint32_t k1, uint32_t m, uint32_t step, const float* belief, int n_branches)
+{
+    const W4 r = philox(W4{{m, 0u, step, 3u}}, k0, k1);
+    const float u = (float)((r.v[0] >> 8) + 1u) * 5.9604644775390625e-08f;
+    float acc = 0.0f;
+    for (int i = 0; i < n_branches; ++i)
+    {
+        acc = acc + belief[i];
+        if (u <= acc)
+            return i;
+    }
+    return n_branches - 1;
 }
 
 } // namespace orc

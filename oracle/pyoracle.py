@@ -51,6 +51,12 @@ def lib(abi):
     L.or_cost_grad.argtypes = [I, C.POINTER(abi.QuadParams), C.POINTER(abi.CostCfg), C.POINTER(abi.Track),
                                C.POINTER(abi.DroneState), C.POINTER(abi.Opponent), I, F, I, I, I, D, D,
                                C.POINTER(C.c_uint64), I]
+    L.or_pr_step.argtypes = [I, C.POINTER(abi.QuadParams), C.POINTER(abi.CostCfg), C.POINTER(abi.Track),
+                             C.POINTER(abi.DroneState), C.POINTER(abi.Opponent), I, C.POINTER(abi.PrInstance), F, U, F, F,
+                             F, I, F, D, D, C.POINTER(I), D, D]
+    L.or_pr_choose.argtypes = [I, C.POINTER(abi.QuadParams), C.POINTER(abi.CostCfg), C.POINTER(abi.Track),
+                               C.POINTER(abi.DroneState), C.POINTER(abi.Opponent), I, C.POINTER(abi.PrInstance), F,
+                               C.POINTER(I), D, F, D, D, D, D]
     _lib = L
     return L
 
@@ -174,3 +180,105 @@ class Oracle:
         steps[:] = list(st)
         lambdas[:] = lam
         return acts.reshape(n_i, 4), traj
+
+    # ------------------------------------------------------------ PR-MPPI
+
+    def pr_step(self, f64, params, cost, track, ego, opponents, ins, belief, step, lam, nom, rob, threads=8,
+                want_noise=True, want_pairs=True):
+        """One PR-MPPI step of one controller from the shifted plans nom, rob (T x 4)."""
+        K, T = ins.n_samples, ins.n_timesteps
+        M = ins.n_models if ins.sampled else ins.n_branches
+        nom = np.ascontiguousarray(nom, dtype=np.float32).reshape(-1)
+        rob = np.ascontiguousarray(rob, dtype=np.float32).reshape(-1)
+        bel = np.zeros(self.abi.MAX_BRANCHES, dtype=np.float32)
+        bel[:len(belief)] = belief
+        lam = np.ascontiguousarray(lam, dtype=np.float32)
+        noise = np.empty(K * T * 4, dtype=np.float32) if want_noise else None
+        pairs = np.empty(2 * M * K * 2) if want_pairs else None
+        agg = np.empty(K * 4)
+        models = (C.c_int * M)()
+        cand = np.empty(3 * T * 4)
+        res = np.empty(6)
+        opp = (self.abi.Opponent * max(1, len(opponents)))(*opponents)
+        rc = self.L.or_pr_step(int(f64), C.byref(params), C.byref(cost), C.byref(track), C.byref(ego), opp, len(opponents),
+                               C.byref(ins), _f(bel), step, _f(lam), _f(nom), _f(rob), threads,
+                               _f(noise) if noise is not None else None, _d(pairs) if pairs is not None else None, _d(agg),
+                               models, _d(cand), _d(res))
+        assert rc == 0
+        return dict(noise=noise.reshape(K, T, 4) if noise is not None else None,
+                    pairs=pairs.reshape(2, M, K, 2) if pairs is not None else None, agg=agg.reshape(K, 4),
+                    models=np.array(models[:], dtype=np.int32), cand=cand.reshape(3, T, 4), min=res[:3], nu=res[3:6])
+
+    def pr_choose(self, f64, params, cost, track, ego, opponents, ins, belief, models, cand, lam, nu):
+        """Candidate costs and choice for candidates cand[3][T][4] (cand1, cand2, new robust plan)."""
+        M = ins.n_models if ins.sampled else ins.n_branches
+        bel = np.zeros(self.abi.MAX_BRANCHES, dtype=np.float32)
+        bel[:len(belief)] = belief
+        mdl = (C.c_int * M)(*[int(x) for x in models])
+        cand = np.ascontiguousarray(cand, dtype=np.float64).reshape(-1)
+        lam = np.ascontiguousarray(lam, dtype=np.float32)
+        nu = np.ascontiguousarray(nu, dtype=np.float64)
+        cc = np.empty(2 * M)
+        cs = np.empty(2 * M)
+        res = np.empty(11)
+        opp = (self.abi.Opponent * max(1, len(opponents)))(*opponents)
+        rc = self.L.or_pr_choose(int(f64), C.byref(params), C.byref(cost), C.byref(track), C.byref(ego), opp,
+                                 len(opponents), C.byref(ins), _f(bel), mdl, _d(cand), _f(lam), _d(nu), _d(cc), _d(cs),
+                                 _d(res))
+        assert rc == 0
+        return dict(cand_cost=res[0:2], use_nom=int(res[2]), reset_nom=int(res[3]), action=res[4:8], lam=res[8:11],
+                    cc=cc.reshape(2, M), cs=cs.reshape(2, M))
+
+    def world_opponents(self, sc, drones, d):
+        """Opponents of drone d: the other drones of its world in increasing index."""
+        out = []
+        for e in range(len(sc.drone_world)):
+            if e != d and sc.drone_world[e] == sc.drone_world[d]:
+                o = self.abi.Opponent()
+                for c in range(3):
+                    o.p[c] = drones[e].p[c]
+                    o.v[c] = drones[e].v[c]
+                out.append(o)
+        return out
+
+    def pr_world_step(self, sc, drones, works, steps, lambdas, threads=8):
+        """One closed-loop FP32 PR-MPPI step of every controller of sc; drones, works (shifted
+        (nom, rob) plans), steps and lambdas are updated in place.  Returns the actions and the
+        per-instance step and choice records."""
+        recs = []
+        for i, ins in enumerate(sc.instances):
+            belief = np.array(ins.belief[:ins.n_branches], dtype=np.float32)
+            opps = self.world_opponents(sc, drones, ins.drone)
+            r = self.pr_step(0, sc.params, sc.cost, sc.tracks[ins.track], drones[ins.drone], opps, ins, belief, steps[i],
+                             lambdas[i], works[i][0], works[i][1], threads, want_noise=False, want_pairs=False)
+            c = self.pr_choose(0, sc.params, sc.cost, sc.tracks[ins.track], drones[ins.drone], opps, ins, belief,
+                               r["models"], r["cand"], lambdas[i], r["nu"])
+            recs.append((r, c))
+        acts = np.zeros((len(sc.instances), 4), dtype=np.float32)
+        for i, ins in enumerate(sc.instances):
+            r, c = recs[i]
+            pick = 0 if c["cand_cost"][0] <= c["cand_cost"][1] else 1
+            chosen = r["cand"][pick].astype(np.float32)
+            robp = r["cand"][2].astype(np.float32)
+            fill = np.array(ins.shift_fill[:], dtype=np.float32)
+            works[i] = (np.vstack([chosen[1:], fill[None]]), np.vstack([robp[1:], fill[None]]))
+            lambdas[i] = np.array(c["lam"], dtype=np.float32)
+            steps[i] += 1
+            acts[i] = np.array(c["action"], dtype=np.float32)
+        controlled = {ins.drone: i for i, ins in enumerate(sc.instances)}
+        for d in range(len(sc.drone_world)):
+            s = drones[d]
+            if d in controlled:
+                ins = sc.instances[controlled[d]]
+                x = np.array(list(s.p) + list(s.q) + list(s.v) + list(s.w), dtype=np.float64)
+                x, g, l = self.quad_step(0, sc.params, x, acts[controlled[d]].astype(np.float64), sc.tracks[ins.track],
+                                         sc.cost.gate_traversal_margin, s.gate, s.laps)
+                for c in range(3):
+                    s.p[c], s.v[c], s.w[c] = x[c], x[7 + c], x[10 + c]
+                for c in range(4):
+                    s.q[c] = x[3 + c]
+                s.gate, s.laps = g, l
+            else:
+                raise NotImplementedError("pr_world_step: every drone must have a controller")
+        return acts, recs
+

@@ -14,7 +14,7 @@ HEADER = os.path.join(ROOT, "include", "dr_api.h")
 STRUCTS = {
     "dr_quad_params": "QuadParams", "dr_gate": "Gate", "dr_track": "Track", "dr_cost_cfg": "CostCfg",
     "dr_drone_state": "DroneState", "dr_opponent": "Opponent", "dr_instance": "Instance", "dr_diag": "Diag",
-    "dr_launch_cfg": "LaunchCfg", "dr_mppi_cfg": "MPPICfg",
+    "dr_launch_cfg": "LaunchCfg", "dr_mppi_cfg": "MPPICfg", "dr_pr_instance": "PrInstance", "dr_pr_diag": "PrDiag",
 }
 RENAMED = {"lambda_": "lambda"}
 

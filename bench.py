@@ -174,6 +174,28 @@ def pr_mppi(pkg, steps=100, warmup=10):
             "use_nom_plan": [int(d.use_nom_plan) for d in diag], "steps": steps}
 
 
+def sr_race(pkg, steps=50, warmup=5, n_inst=1):
+    """Plain MPPI on the reference's StratRace2D race at its dense setting
+    (K = 2^16, T = 70, temperature 10, sigma 1.5, round track r = 6 m), closed
+    loop with the environment's noise, graph replays; n_inst worlds."""
+    sc = pkg.scenarios
+    scn = sc.sr_race(n_inst=n_inst)
+    eng = pkg.SREngine(scn)
+    eng.step(warmup, advance=True)
+    t0 = time.perf_counter()
+    eng.step(steps, advance=True)
+    elapsed = time.perf_counter() - t0
+    eng.set_timing(True)
+    eng.step(5, advance=True)
+    ms = eng.last_timing() / 5
+    eng.close()
+    ss = scn.sample_steps_per_step
+    return {"workload": scn.name, "control_steps_per_s": n_inst * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+            "sample_steps_per_s": ss * steps / elapsed,
+            "kernel_ms_per_step": {"rollout": float(ms[0]), "weighted_avg": float(ms[1]), "update_world": float(ms[2]),
+                                   "step": float(ms[3])}, "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +209,9 @@ def main():
     ap.add_argument("--no-batch64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-pr", action="store_true")
+    ap.add_argument("--pr-only", action="store_true", help="run only the PR-MPPI section (for profiling)")
+    ap.add_argument("--no-sr", action="store_true")
+    ap.add_argument("--sr-only", action="store_true", help="run only the StratRace2D section (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 512 MiB buffer 3 times (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -208,6 +233,12 @@ def main():
 
     pkg = package()
     sc = pkg.scenarios
+    if args.sr_only:
+        print(json.dumps({"sr_race": sr_race(pkg, steps=args.steps, warmup=args.warmup)}), flush=True)
+        return
+    if args.pr_only:
+        print(json.dumps({"pr_mppi": pr_mppi(pkg, steps=args.steps, warmup=args.warmup)}), flush=True)
+        return
     scn = {"c0": sc.c0_single, "c1": sc.c1_four_drones, "c2": sc.c2_batch64}[args.config]()
     n_inst = len(scn.instances)
     log(f"bench: workload {scn.name}: {scn.description}")
@@ -293,6 +324,10 @@ def main():
         log("bench: PR-MPPI, 4 drones, K=8192 per branch, T=80 ...")
         out["pr_mppi"] = pr_mppi(pkg)
         log(f"bench: pr_mppi {json.dumps(out['pr_mppi'])}")
+    if not args.no_sr:
+        log("bench: StratRace2D plain MPPI, K=65536, T=70 ...")
+        out["sr_race"] = sr_race(pkg)
+        log(f"bench: sr_race {json.dumps(out['sr_race'])}")
     if not args.no_batch64:
         log("bench: 64 uneven controllers, batched vs per-instance ...")
         out["batch64"] = batch_vs_per_instance(pkg)

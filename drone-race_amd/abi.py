@@ -16,6 +16,8 @@ MAX_WORLD_DRONES = 8
 
 MAX_BRANCHES = 8
 MAX_MODELS = 16
+SR_TRACK_SAMPLES = 512
+SR_MAX_MODELS = 4
 
 DR_OK, DR_EINVAL, DR_EHIP, DR_ENOMEM, DR_ESTATE = 0, 1, 2, 3, 4
 
@@ -86,6 +88,36 @@ class PrDiag(C.Structure):
                 ("use_nom_plan", C.c_int32), ("reset_nom", C.c_int32), ("step", C.c_uint32), ("_pad", C.c_int32)]
 
 
+class SrOppConfig(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("s1", "s2", "s3", "speed_scale", "action_noise")]
+
+
+class SrEnv(C.Structure):
+    _fields_ = [("dt", C.c_float), ("drone_radius", C.c_float), ("pos_noise", C.c_float), ("speed_noise", C.c_float),
+                ("action_noise", C.c_float), ("max_speed", C.c_float * 2), ("max_accel", C.c_float * 2),
+                ("track_width", C.c_float), ("n_win_laps", C.c_int32), ("kp", C.c_float), ("kv", C.c_float),
+                ("max_opp_lat_dist_fact", C.c_float), ("track_length", C.c_float), ("n_models", C.c_int32),
+                ("opp", SrOppConfig * SR_MAX_MODELS), ("p_grid", (C.c_float * 2) * SR_TRACK_SAMPLES),
+                ("t_grid", (C.c_float * 2) * SR_TRACK_SAMPLES), ("kappa_grid", C.c_float * SR_TRACK_SAMPLES)]
+
+
+class SrState(C.Structure):
+    _fields_ = [("pos", C.c_float * 4), ("vel", C.c_float * 4), ("s", C.c_float * 2), ("lat_dist", C.c_float * 2),
+                ("laps", C.c_int32 * 2)]
+
+
+class SrCost(C.Structure):
+    _fields_ = [(n, C.c_float) for n in (
+        "coll_dist_factor", "boundary_cost", "boundary_threshold_factor", "outside_cost", "win_cost", "opp_win_cost",
+        "final_adv_weight", "final_opp_adv_weight")]
+
+
+class SrInstance(C.Structure):
+    _fields_ = [("n_samples", C.c_int32), ("n_timesteps", C.c_int32), ("true_theta", C.c_int32), ("adapt_lambda", C.c_int32),
+                ("seed", C.c_uint64), ("lambda_", C.c_float), ("sigma", C.c_float), ("shift_fill", C.c_float * 2),
+                ("_pad", C.c_int32 * 2)]
+
+
 # Every symbol include/dr_api.h declares, with its ctypes signature.
 _P = C.c_void_p
 _I = C.c_int
@@ -146,6 +178,21 @@ SIGNATURES = {
     "dr_pr_get_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
     "dr_pr_set_timing": (_I, [_P, _I]),
     "dr_pr_last_timing": (_I, [_P, _F, _I]),
+    "dr_sr_create": (_I, [C.POINTER(SrEnv), C.POINTER(SrCost), C.POINTER(SrInstance), _I, C.POINTER(_P)]),
+    "dr_sr_destroy": (None, [_P]),
+    "dr_sr_set_states": (_I, [_P, C.POINTER(SrState), _I]),
+    "dr_sr_get_states": (_I, [_P, C.POINTER(SrState), _I]),
+    "dr_sr_set_nominal": (_I, [_P, _I, _F, _I]),
+    "dr_sr_get_nominal": (_I, [_P, _I, _F, _I]),
+    "dr_sr_step": (_I, [_P, _I, _I, _F]),
+    "dr_sr_get_events": (_I, [_P, C.POINTER(C.c_int32), _I]),
+    "dr_sr_get_costs": (_I, [_P, _I, _F, _I]),
+    "dr_sr_get_noise": (_I, [_P, _I, _F, C.c_size_t]),
+    "dr_sr_get_diag": (_I, [_P, C.POINTER(Diag), _I]),
+    "dr_sr_set_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
+    "dr_sr_get_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
+    "dr_sr_set_timing": (_I, [_P, _I]),
+    "dr_sr_last_timing": (_I, [_P, _F, _I]),
 }
 
 _lib = None

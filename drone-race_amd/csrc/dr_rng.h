@@ -33,6 +33,7 @@ enum : uint32_t
     STREAM_TRACK = 1u,   // scenario generation (gate jitter, radii, sizes)
     STREAM_BRANCH = 2u,  // PR-MPPI branch beliefs and opponent velocity shifts
     STREAM_MODEL = 3u,   // PR-MPPI opponent-model draws, counter (m, 0, step, 3)
+    STREAM_ENV = 4u,     // environment noise of a 2-D race rollout, counter (k, 4 t + block, step, 4)
 };
 
 struct U4

@@ -329,3 +329,94 @@ class PREngine:
         a = np.empty(5, dtype=np.float32)
         check(self.lib.dr_pr_last_timing(self.h, _fptr(a), 5))
         return a
+
+
+class SREngine:
+    """A batch of plain-MPPI controllers on the 2-D race environment, one world each."""
+
+    def __init__(self, scenario):
+        lib = abi.load()
+        self.lib = lib
+        self.sc = scenario
+        n = len(scenario.instances)
+        self._inst = (abi.SrInstance * n)(*scenario.instances)
+        self.n_inst = n
+        h = C.c_void_p()
+        check(lib.dr_sr_create(C.byref(scenario.env), C.byref(scenario.cost), self._inst, n, C.byref(h)))
+        self.h = h
+        self.set_states(scenario.states)
+        for i, ins in enumerate(scenario.instances):
+            self.set_nominal(i, np.zeros((ins.n_timesteps, 2), dtype=np.float32))
+
+    def close(self):
+        if self.h:
+            self.lib.dr_sr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_states(self, states):
+        arr = (abi.SrState * self.n_inst)(*states)
+        check(self.lib.dr_sr_set_states(self.h, arr, self.n_inst))
+
+    def get_states(self):
+        arr = (abi.SrState * self.n_inst)()
+        check(self.lib.dr_sr_get_states(self.h, arr, self.n_inst))
+        return arr
+
+    def set_nominal(self, i, nom):
+        a = np.ascontiguousarray(nom, dtype=np.float32).reshape(-1)
+        check(self.lib.dr_sr_set_nominal(self.h, i, _fptr(a), a.size))
+
+    def get_nominal(self, i):
+        T = self.sc.instances[i].n_timesteps
+        a = np.empty(T * 2, dtype=np.float32)
+        check(self.lib.dr_sr_get_nominal(self.h, i, _fptr(a), a.size))
+        return a.reshape(T, 2)
+
+    def step(self, n_steps=1, advance=False):
+        a = np.empty(self.n_inst * 2, dtype=np.float32)
+        check(self.lib.dr_sr_step(self.h, n_steps, 1 if advance else 0, _fptr(a)))
+        return a.reshape(self.n_inst, 2)
+
+    def events(self):
+        a = (C.c_int32 * self.n_inst)()
+        check(self.lib.dr_sr_get_events(self.h, a, self.n_inst))
+        return np.array(a[:], dtype=np.int32)
+
+    def costs(self, i):
+        K = self.sc.instances[i].n_samples
+        a = np.empty(K, dtype=np.float32)
+        check(self.lib.dr_sr_get_costs(self.h, i, _fptr(a), K))
+        return a
+
+    def noise(self, i):
+        ins = self.sc.instances[i]
+        a = np.empty(ins.n_samples * ins.n_timesteps * 2, dtype=np.float32)
+        check(self.lib.dr_sr_get_noise(self.h, i, _fptr(a), a.size))
+        return a.reshape(ins.n_samples, ins.n_timesteps, 2)
+
+    def diag(self):
+        arr = (abi.Diag * self.n_inst)()
+        check(self.lib.dr_sr_get_diag(self.h, arr, self.n_inst))
+        return [dict(min_cost=d.min_cost, nu=d.nu, lambda_=d.lambda_, step=d.step) for d in arr]
+
+    def set_launch(self, **kw):
+        cfg = abi.LaunchCfg()
+        check(self.lib.dr_sr_get_launch(self.h, C.byref(cfg)))
+        for k, v in kw.items():
+            setattr(cfg, k, int(v))
+        check(self.lib.dr_sr_set_launch(self.h, C.byref(cfg)))
+
+    def set_timing(self, on=True):
+        check(self.lib.dr_sr_set_timing(self.h, 1 if on else 0))
+
+    def last_timing(self):
+        """Device ms summed over the last timed run: rollout, weighted average, update + world, step."""
+        a = np.empty(4, dtype=np.float32)
+        check(self.lib.dr_sr_last_timing(self.h, _fptr(a), 4))
+        return a

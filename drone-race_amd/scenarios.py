@@ -241,6 +241,109 @@ def c3_pr_four_drones(seed=2, n_samples=8192, n_timesteps=80, steps=500, sampled
                       f"K={n_samples} per branch, T={n_timesteps}, 12-gate lemniscate a=30 m, seed {seed}")
 
 
+# ---------------------------------------------------------------------------
+# StratRace2D (the reference's compiled 2-D race)
+
+def sr_track_grids(track_fn, n=512):
+    """Track tables as the reference's StratRaceEnvironmentConfig.__post_init__ builds them
+    (python-src/utils.py:382-428): polyline of track_fn(k / n), arc length, periodic cubic
+    splines x(s), y(s) on a uniform arc-length grid; unit tangent; signed curvature."""
+    from scipy.interpolate import CubicSpline
+    q = np.array([track_fn(k / n) for k in range(n)], dtype=np.float64)
+    d = np.linalg.norm(np.diff(q, axis=0, append=q[0:1]), axis=1)
+    s_k = np.zeros(n + 1)
+    s_k[1:] = np.cumsum(d)
+    L = s_k[-1]
+    qp = np.vstack([q, q[0:1]])
+    cx = CubicSpline(s_k, qp[:, 0], bc_type="periodic")
+    cy = CubicSpline(s_k, qp[:, 1], bc_type="periodic")
+    s = np.linspace(0.0, L, n, endpoint=False)
+    p = np.column_stack([cx(s), cy(s)])
+    dx, dy = cx(s, 1), cy(s, 1)
+    dp = np.column_stack([dx, dy])
+    t = dp / np.maximum(np.linalg.norm(dp, axis=1, keepdims=True), 1e-6)
+    ddx, ddy = cx(s, 2), cy(s, 2)
+    kappa = (dx * ddy - dy * ddx) / np.maximum((dx ** 2 + dy ** 2) ** 1.5, 1e-6)
+    return L, p, t, kappa
+
+
+def sr_opp_preset(drone_radius, s1="default", s2="default", s3="default", speed_scale=1.0, action_noise=0.1):
+    """OppConfig.preset of python-src/utils.py:290-342 (one opponent)."""
+    v1 = {"insensitive": 0.0, "default": 2.5 * drone_radius, "conservative": 4 * drone_radius}
+    v2 = {"infinite": 0.0, "default": 1 / (4 * drone_radius) ** 2, "insensitive": 1 / (1e-5 * drone_radius) ** 2}
+    v3 = {"no-block": 0.0, "default": 20 / drone_radius}
+    return abi.SrOppConfig(s1=v1.get(s1, s1), s2=v2.get(s2, s2), s3=v3.get(s3, s3), speed_scale=speed_scale,
+                           action_noise=action_noise)
+
+
+def sr_env(radius=6.0, drone_radius=0.4, n_models=2, action_noise=0.1, pos_noise=0.0, speed_noise=0.0,
+           env_action_noise=0.0):
+    """The reference's stratRaceEnv (python-src/main.py:864-916): round track of radius 6 m,
+    drone radius 0.4, max speed (3, 2), max accel (2, 3), track width 2, one lap, kV = 5,
+    max opponent lateral factor 1.9, dt 0.1; opponent models = the first n_models presets."""
+    L, p, t, kappa = sr_track_grids(lambda f: radius * np.array([math.cos(2 * math.pi * f), math.sin(2 * math.pi * f)]))
+    e = abi.SrEnv()
+    e.dt, e.drone_radius = 0.1, drone_radius
+    e.pos_noise, e.speed_noise, e.action_noise = pos_noise, speed_noise, env_action_noise
+    e.max_speed[0], e.max_speed[1] = 3.0, 2.0
+    e.max_accel[0], e.max_accel[1] = 2.0, 3.0
+    e.track_width, e.n_win_laps = 2.0, 1
+    e.kp, e.kv, e.max_opp_lat_dist_fact = 10.0, 5.0, 1.9
+    e.track_length = L
+    presets = [dict(s1="insensitive", s3="default"), dict(s1="insensitive", s3="no-block"),
+               dict(s1="default", s3="default"), dict(s1="conservative", s3="no-block")]
+    e.n_models = n_models
+    for m in range(n_models):
+        e.opp[m] = sr_opp_preset(drone_radius, action_noise=action_noise, **presets[m])
+    pf, tf, kf = p.astype(np.float32), t.astype(np.float32), kappa.astype(np.float32)
+    for i in range(abi.SR_TRACK_SAMPLES):
+        e.p_grid[i][0], e.p_grid[i][1] = float(pf[i, 0]), float(pf[i, 1])
+        e.t_grid[i][0], e.t_grid[i][1] = float(tf[i, 0]), float(tf[i, 1])
+        e.kappa_grid[i] = float(kf[i])
+    return e
+
+
+def sr_cost():
+    """The dense MPPIConfig of stratRaceEnv (python-src/main.py:920-942)."""
+    return abi.SrCost(coll_dist_factor=1.3, boundary_cost=50.0, boundary_threshold_factor=1.7, outside_cost=1e6,
+                      win_cost=1e6, opp_win_cost=1e5, final_adv_weight=500.0, final_opp_adv_weight=0.0)
+
+
+def sr_start_state(env, start=(0.05, 0.2)):
+    """Ego and opponent at the grid samples round(512 * start), at rest (main.py:908-916)."""
+    idx = [int(round(f * abi.SR_TRACK_SAMPLES)) for f in start]
+    s = abi.SrState()
+    for a, i in enumerate(idx):
+        s.pos[2 * a], s.pos[2 * a + 1] = env.p_grid[i][0], env.p_grid[i][1]
+        s.s[a] = float(np.float32(i / abi.SR_TRACK_SAMPLES * env.track_length))
+    return s
+
+
+class SrScenario:
+    def __init__(self, name, env, cost, instances, states, steps, description):
+        self.name, self.env, self.cost = name, env, cost
+        self.instances, self.states, self.steps, self.description = instances, states, steps, description
+
+    @property
+    def sample_steps_per_step(self):
+        return sum(i.n_samples * i.n_timesteps for i in self.instances)
+
+
+def sr_instance(n_samples, n_timesteps, seed, theta=0, lam=10.0, sigma=1.5, adapt=0):
+    return abi.SrInstance(n_samples=n_samples, n_timesteps=n_timesteps, true_theta=theta, adapt_lambda=adapt, seed=seed,
+                          lambda_=lam, sigma=sigma)
+
+
+def sr_race(n_inst=1, n_samples=65536, n_timesteps=70, seed=42, steps=300, **env_kw):
+    """The reference's StratRace2D plain-MPPI race (K = 2^16, T = 70, temperature 10, sigma 1.5),
+    n_inst independent worlds; instance i follows opponent model i % n_models."""
+    env = sr_env(**env_kw)
+    inst = [sr_instance(n_samples, n_timesteps, seed_mix(seed, 300 + i), theta=i % env.n_models) for i in range(n_inst)]
+    st = [sr_start_state(env) for _ in range(n_inst)]
+    return SrScenario(f"sr_race_{n_inst}x_k{n_samples}_t{n_timesteps}", env, sr_cost(), inst, st, steps,
+                      f"StratRace2D, {n_inst} worlds, K={n_samples}, T={n_timesteps}, round track r=6 m")
+
+
 def seed_mix(seed, i):
     """64-bit Philox key for instance i of a scenario seeded `seed`."""
     return ((int(seed) * 0x9E3779B97F4A7C15) ^ (0xA5A5A5A5 + 0x632BE59BD9B4E019 * (i + 1))) & 0xFFFFFFFFFFFFFFFF

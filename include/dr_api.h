@@ -304,6 +304,108 @@ int dr_pr_set_timing(dr_prengine* e, int enable);
  * averages, update (+ closed-loop advance), whole step. */
 int dr_pr_last_timing(dr_prengine* e, float* ms, int n);
 
+/* ---------------------------------------------------------------------
+ * StratRace2D: the reference's compiled environment (2-D race on a closed
+ * track, one opponent driven by the reference's opponent model), restating
+ * include/environments/env_stratrace_defs.h and env_stratrace.h:33-894 and
+ * the running/terminal costs of include/costs.cuh:7-39, with plain (dense,
+ * single committed opponent model) MPPI on top (src/kernels.cu:34-171,
+ * 307-456, src/mppi_controller.cu:250-466).  Control = 2-D acceleration. */
+#define DR_SR_TRACK_SAMPLES 512
+#define DR_SR_MAX_MODELS 4
+
+typedef struct dr_sr_opp_config  /* OppConfig, env_stratrace_defs.h:42-50, one opponent */
+{
+    float s1;           /* overtaking lateral distance */
+    float s2;           /* longitudinal influence exp(-s2 ds^2) */
+    float s3;           /* blocking strength */
+    float speed_scale;
+    float action_noise;
+} dr_sr_opp_config;
+
+typedef struct dr_sr_env         /* EnvironmentConfig, env_stratrace_defs.h:53-86 */
+{
+    float dt;
+    float drone_radius;
+    float pos_noise;              /* posNoiseLevel */
+    float speed_noise;            /* speedNoiseLevel */
+    float action_noise;           /* actionNoiseLevel */
+    float max_speed[2];           /* ego, opponent */
+    float max_accel[2];
+    float track_width;
+    int32_t n_win_laps;
+    float kp;
+    float kv;
+    float max_opp_lat_dist_fact;
+    float track_length;
+    int32_t n_models;             /* N_TRUE_MODELS in 1..DR_SR_MAX_MODELS */
+    dr_sr_opp_config opp[DR_SR_MAX_MODELS];
+    float p_grid[DR_SR_TRACK_SAMPLES][2];   /* centre line at s = L i / 512 */
+    float t_grid[DR_SR_TRACK_SAMPLES][2];   /* unit tangent */
+    float kappa_grid[DR_SR_TRACK_SAMPLES];  /* curvature */
+} dr_sr_env;
+
+typedef struct dr_sr_state       /* SimState, env_stratrace_defs.h:88-96 (ego first) */
+{
+    float pos[4];
+    float vel[4];
+    float s[2];
+    float lat_dist[2];
+    int32_t laps[2];
+} dr_sr_state;
+
+typedef struct dr_sr_cost        /* cost fields of MPPIConfig, config.h:116-154 */
+{
+    float coll_dist_factor;
+    float boundary_cost;
+    float boundary_threshold_factor;
+    float outside_cost;
+    float win_cost;
+    float opp_win_cost;
+    float final_adv_weight;
+    float final_opp_adv_weight;
+} dr_sr_cost;
+
+typedef struct dr_sr_instance
+{
+    int32_t n_samples;
+    int32_t n_timesteps;
+    int32_t true_theta;   /* opponent model the rollouts and the world follow */
+    int32_t adapt_lambda; /* 1: mppi_controller.cu:413-424 */
+    uint64_t seed;
+    float lambda;         /* temperature ("invTemperature") */
+    float sigma;          /* samplingNoise, both control dimensions */
+    float shift_fill[2];  /* freed last step after the shift (the reference: 0) */
+    int32_t _pad[2];
+} dr_sr_instance;
+
+typedef struct dr_sr_engine dr_sr_engine;
+
+int dr_sr_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_sr_instance* instances, int n_instances, dr_sr_engine** out);
+void dr_sr_destroy(dr_sr_engine* e);
+/* One world state per instance. */
+int dr_sr_set_states(dr_sr_engine* e, const dr_sr_state* states, int n);
+int dr_sr_get_states(dr_sr_engine* e, dr_sr_state* states, int n);
+int dr_sr_set_nominal(dr_sr_engine* e, int instance, const float* nominal, int n_floats); /* T x 2 */
+int dr_sr_get_nominal(dr_sr_engine* e, int instance, float* nominal, int n_floats);
+/* n_steps control steps of every instance; advance != 0 steps each world
+ * with its action (SimulationEngine::dynStep, engine.cpp:286-312, with the
+ * environment's noise); a world stops at its first event. */
+int dr_sr_step(dr_sr_engine* e, int n_steps, int advance, float* actions /* n x 2, may be NULL */);
+/* Last world event per instance: 0 none, 1 ego outside, 2 ego wins, 3 opponent wins. */
+int dr_sr_get_events(dr_sr_engine* e, int32_t* events, int n);
+int dr_sr_get_costs(dr_sr_engine* e, int instance, float* costs, int n);
+/* Control noise of the last step, [k][t][2]. */
+int dr_sr_get_noise(dr_sr_engine* e, int instance, float* noise, size_t n);
+int dr_sr_get_diag(dr_sr_engine* e, dr_diag* diag, int n);
+/* rollout_block in {64, 128, 256}, reduce_chunk, reduce_tgroup, use_graph,
+ * steps_per_graph are used. */
+int dr_sr_set_launch(dr_sr_engine* e, const dr_launch_cfg* cfg);
+int dr_sr_get_launch(dr_sr_engine* e, dr_launch_cfg* cfg);
+int dr_sr_set_timing(dr_sr_engine* e, int enable);
+/* Device ms summed over the last timed run: rollout, weighted average, update + world step, whole step. */
+int dr_sr_last_timing(dr_sr_engine* e, float* ms, int n);
+
 /* Single-controller entry points with the reference's shape. */
 typedef struct dr_mppi_cfg
 {

@@ -57,6 +57,15 @@ def lib(abi):
     L.or_pr_choose.argtypes = [I, C.POINTER(abi.QuadParams), C.POINTER(abi.CostCfg), C.POINTER(abi.Track),
                                C.POINTER(abi.DroneState), C.POINTER(abi.Opponent), I, C.POINTER(abi.PrInstance), F,
                                C.POINTER(I), D, F, D, D, D, D]
+    L.or_sr_env_step.argtypes = [I, C.POINTER(abi.SrEnv), I, F, C.POINTER(abi.SrState), C.c_uint64, U, U, U, I]
+    L.or_sr_project.restype = C.c_double
+    L.or_sr_project.argtypes = [I, C.POINTER(abi.SrEnv), C.c_double, C.c_double, C.c_double, D]
+    L.or_sr_opp_action.argtypes = [I, C.POINTER(abi.SrEnv), I, C.POINTER(abi.SrState), D]
+    L.or_sr_state_cost.restype = C.c_double
+    L.or_sr_state_cost.argtypes = [I, C.POINTER(abi.SrEnv), C.POINTER(abi.SrCost), C.POINTER(abi.SrState)]
+    L.or_sr_rollout.argtypes = [I, C.POINTER(abi.SrEnv), C.POINTER(abi.SrCost), C.POINTER(abi.SrState), I, F, I, I,
+                                C.c_uint64, U, C.c_float, I, D, F]
+    L.or_sr_update.argtypes = [I, D, F, F, I, I, C.c_float, C.c_float, F, D, D, D, D, D]
     _lib = L
     return L
 
@@ -281,4 +290,50 @@ class Oracle:
             else:
                 raise NotImplementedError("pr_world_step: every drone must have a controller")
         return acts, recs
+
+    # ------------------------------------------------------------ StratRace2D
+
+    def sr_env_step(self, f64, env, theta, ego, state, seed, k, t, step, lose_on_opp_win=True):
+        """One environment step (in place on a copy); returns (state, terminal code)."""
+        st = self.abi.SrState.from_buffer_copy(state)
+        u = np.ascontiguousarray(ego, dtype=np.float32)
+        r = self.L.or_sr_env_step(int(f64), C.byref(env), theta, _f(u), C.byref(st), seed, k, t, step,
+                                  1 if lose_on_opp_win else 0)
+        return st, r
+
+    def sr_project(self, f64, env, x, y, prev_s):
+        lat = C.c_double()
+        s = self.L.or_sr_project(int(f64), C.byref(env), x, y, prev_s, C.byref(lat))
+        return s, lat.value
+
+    def sr_opp_action(self, f64, env, theta, state):
+        o = np.empty(2)
+        self.L.or_sr_opp_action(int(f64), C.byref(env), theta, C.byref(state), _d(o))
+        return o
+
+    def sr_state_cost(self, f64, env, cost, state):
+        return self.L.or_sr_state_cost(int(f64), C.byref(env), C.byref(cost), C.byref(state))
+
+    def sr_rollout(self, f64, env, cost, state, theta, nominal, K, T, seed, step, sigma, threads=8, want_noise=True):
+        nominal = np.ascontiguousarray(nominal, dtype=np.float32).reshape(-1)
+        costs = np.empty(K)
+        noise = np.empty(K * T * 2, dtype=np.float32) if want_noise else None
+        rc = self.L.or_sr_rollout(int(f64), C.byref(env), C.byref(cost), C.byref(state), theta, _f(nominal), K, T, seed, step,
+                                  sigma, threads, _d(costs), _f(noise) if noise is not None else None)
+        assert rc == 0
+        return costs, (noise.reshape(K, T, 2) if noise is not None else None)
+
+    def sr_update(self, f64, costs, noise, nominal, lam, max_accel, fill=(0.0, 0.0)):
+        K = costs.size
+        T = nominal.size // 2
+        costs = np.ascontiguousarray(costs, dtype=np.float64)
+        noise = np.ascontiguousarray(noise, dtype=np.float32).reshape(-1)
+        nominal = np.ascontiguousarray(nominal, dtype=np.float32).reshape(-1)
+        fill = np.ascontiguousarray(fill, dtype=np.float32)
+        upd, act, sh = np.empty(T * 2), np.empty(2), np.empty(T * 2)
+        mn, nu = C.c_double(), C.c_double()
+        rc = self.L.or_sr_update(int(f64), _d(costs), _f(noise), _f(nominal), K, T, lam, max_accel, _f(fill), _d(upd), _d(act),
+                                 _d(sh), C.byref(mn), C.byref(nu))
+        assert rc == 0
+        return dict(updated=upd.reshape(T, 2), action=act, shifted=sh.reshape(T, 2), min_cost=mn.value, nu=nu.value)
 

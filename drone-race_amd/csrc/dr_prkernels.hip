@@ -30,6 +30,17 @@ struct ModelOpp
     }
 };
 
+// The same prediction from registers (index constant after unrolling).
+struct RegOpp
+{
+    float4 p[MAX_OPP], v[MAX_OPP];
+    float tau;
+    __device__ float4 operator()(int j) const
+    {
+        return make_float4(fma_(v[j].x, tau, p[j].x), fma_(v[j].y, tau, p[j].y), fma_(v[j].z, tau, p[j].z), 0.0f);
+    }
+};
+
 __device__ __forceinline__ void stage_model_opp(float4 (*dst)[2], const dr_drone_state& o, const PrInstDev& I, int b, int j)
 {
     dst[j][0] = make_float4(o.p[0], o.p[1], o.p[2], 0.0f);
@@ -419,30 +430,56 @@ __global__ __launch_bounds__(256) void k_pr_update(const PrInstDev* __restrict__
 
     if (tid < 2 * M)
     {
+        // One latency-bound lane per (candidate, model): the model's
+        // opponents, the next gate and the next control are kept in
+        // registers so that no step waits on LDS.
         const int which = tid / M, m = tid - which * M;
-        const float* U = s_c[which];
+        const float4* U = reinterpret_cast<const float4*>(s_c[which]);
+        RegOpp ro;
+#pragma unroll
+        for (int j = 0; j < MAX_OPP; ++j)
+        {
+            ro.p[j] = j < n_opp ? s_pv[m][j][0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ro.v[j] = j < n_opp ? s_pv[m][j][1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
         QState s = state_from_abi(drones[I.drone]);
+        NextGate ng;
+        ng.load(s_track.gates, s_track.n_gates, s.gate);
         float acc = 0.0f, decay = 1.0f, smax = -__builtin_inff();
+        // The state integrates every step; the cost, the safety and the state
+        // the terminal cost reads are frozen at the first terminal step, so
+        // the dynamics of step t+1 do not wait for the cost of step t.
+        QState fin = s;
+        bool stop = false;
+        float4 un = U[0];
         for (int t = 0; t < T; ++t)
         {
-            float uin[4], uc[4];
-            for (int d = 0; d < 4; ++d)
-                uin[d] = U[t * 4 + d] + 0.0f;
+            const float4 u4 = un;
+            if (t + 1 < T)
+                un = U[t + 1];
+            float uin[4] = {u4.x + 0.0f, u4.y + 0.0f, u4.z + 0.0f, u4.w + 0.0f}, uc[4];
             clamp_control(C, uin, uc);
             const float p0[3] = {s.p[0], s.p[1], s.p[2]};
             quad_step(C, s, uc);
-            update_gate(s_track.gates, s_track.n_gates, KC.gate_traversal_margin, p0, s);
-            const ModelOpp opp{s_pv[m], C.dt * (float)(t + 1)};
+            ng.update(s_track.gates, s_track.n_gates, KC.gate_traversal_margin, p0, s);
+            ro.tau = C.dt * (float)(t + 1);
             float sf;
             int term;
-            const float c = pr_stage_cost(C, Q, KC, s_track, s, n_opp, opp, &sf, &term);
-            acc = fma_(c, decay, acc);
-            smax = fmaxf(smax, sf);
-            decay = decay * DECAY;
-            if (term != TERM_NONE)
-                break;
+            const float c = pr_stage_cost(C, Q, KC, s_track, s, n_opp, ro, &sf, &term);
+            if (!stop)
+            {
+                acc = fma_(c, decay, acc);
+                smax = fmaxf(smax, sf);
+                decay = decay * DECAY;
+                fin.p[0] = s.p[0];
+                fin.p[1] = s.p[1];
+                fin.p[2] = s.p[2];
+                fin.gate = s.gate;
+                fin.laps = s.laps;
+                stop = term != TERM_NONE;
+            }
         }
-        acc = acc + final_cost(KC, s_track, s);
+        acc = acc + final_cost(KC, s_track, fin);
         s_cc[tid] = smax > 0.0f ? acc + I.safety_weight : acc;
         s_cs[tid] = smax;
     }

@@ -156,12 +156,9 @@ DR_HD void quad_step(const QuadConst& C, QState& s, const float u[4])
 // crossing point x, so the radius test is |.|^2 <= (radius*margin)^2 den^2.
 // Either crossing direction counts; |den| < 1e-10 (motion in the gate plane)
 // never counts.
-DR_HD void update_gate(const dr_gate* gates, int n_gates, float margin, const float p0[3], QState& s)
+// Crossing test against gate g (index next) for the move p0 -> s.p.
+DR_HD void cross_gate(const dr_gate& g, int next, float margin, const float p0[3], QState& s)
 {
-    int next = s.gate + 1;
-    if (next == n_gates)
-        next = 0;
-    const dr_gate& g = gates[next];
     const float ax = s.p[0] - p0[0], ay = s.p[1] - p0[1], az = s.p[2] - p0[2];
     const float cx = g.center[0] - p0[0], cy = g.center[1] - p0[1], cz = g.center[2] - p0[2];
     const float num = fma_(g.normal[0], cx, fma_(g.normal[1], cy, g.normal[2] * cz));
@@ -181,6 +178,33 @@ DR_HD void update_gate(const dr_gate* gates, int n_gates, float margin, const fl
         }
     }
 }
+
+DR_HD void update_gate(const dr_gate* gates, int n_gates, float margin, const float p0[3], QState& s)
+{
+    int next = s.gate + 1;
+    if (next == n_gates)
+        next = 0;
+    cross_gate(gates[next], next, margin, p0, s);
+}
+
+// The next gate held in registers for a single latency-bound rollout: the
+// same test as update_gate, the gate reloaded only after a crossing.
+struct NextGate
+{
+    dr_gate g;
+    int next;
+    DR_HD void load(const dr_gate* gates, int n_gates, int gate)
+    {
+        next = gate + 1 == n_gates ? 0 : gate + 1;
+        g = gates[next];
+    }
+    DR_HD void update(const dr_gate* gates, int n_gates, float margin, const float p0[3], QState& s)
+    {
+        cross_gate(g, next, margin, p0, s);
+        if (s.gate == next)
+            load(gates, n_gates, s.gate);
+    }
+};
 
 // Race progress (env_dronerace.h:374-394): gate + n_gates*laps +
 // 0.8 * (1 - |p - c_next| / |c_next - c_gate|).
@@ -264,8 +288,11 @@ DR_HD float pr_stage_cost(const QuadConst& C, const CostConst& Q, const dr_cost_
 {
     float cost = 0.0f;
     float min_sq = __builtin_inff();
-    for (int j = 0; j < n_opp; ++j)
+#pragma unroll
+    for (int j = 0; j < DR_MAX_WORLD_DRONES - 1; ++j)
     {
+        if (j >= n_opp)
+            break;
         const float4 o = opp_at(j);
         const float dx = s.p[0] - o.x;
         const float dy = s.p[1] - o.y;

@@ -22,7 +22,8 @@
 //   * the opponent's nominal action is computed for the followed model only:
 //     the other models' actions feed the belief update alone, which the
 //     plain-MPPI path does not run;
-//   * the tangent sampled and never used at env_stratrace.h:367 is skipped,
+//   * the tangent and curvature sampled and never used at env_stratrace.h:368
+//     and :783-786 are skipped,
 //     and computeSdot reuses the tangent its caller has just sampled at the
 //     same s (same function of the same input).
 // Environment noise draws come from Philox (counter (k, 4 t + block, step,

@@ -21,7 +21,14 @@
  *   EnvironmentConfig                 include/environments/env_stratrace_defs.h:53-86
  *                                     -> dr_quad_params + dr_track (+ dr_cost_cfg)
  *   SimState                          include/environments/env_stratrace_defs.h:88-96
- *                                     -> dr_drone_state
+ *                                     -> dr_drone_state (quadrotor race), dr_sr_state (StratRace2D)
+ *   PRMPPIController(EnvironmentConfig, PRMPPIConfig, seed, nominal),
+ *       getControl, h_nom_nominal, h_rob_nominal, useNomPlan, resetNom
+ *       include/controllers.h:114-179 -> dr_pr_create / dr_pr_step /
+ *                                     dr_pr_get_nominals / dr_pr_get_diag
+ *   PRMPPIConfig                      include/config.h:157-192 -> dr_pr_instance
+ *   EnvironmentConfig, SimState of the StratRace2D build
+ *       include/environments/env_stratrace_defs.h:42-96 -> dr_sr_env, dr_sr_state
  */
 #ifndef DR_API_H
 #define DR_API_H

@@ -7,13 +7,20 @@
 //   MPPIController(EnvironmentConfig, MPPIConfig, seed, nominal) controllers.h:29
 //   MPPIController::mppiConfig, h_nominal                       controllers.h:35-37
 //
+//   PRMPPIController(EnvironmentConfig, PRMPPIConfig, seed, nominal) controllers.h:120
+//   PRMPPIController::h_nom_nominal, h_rob_nominal, resetNom, useNomPlan controllers.h:128-135
+//   stratrace::MPPIController: the same MPPIController interface over the
+//   reference's own StratRace2D environment (env_stratrace_defs.h:42-96).
+//
 // Errors from the C ABI are rethrown as std::runtime_error, as the
 // reference's constructors and checks throw (mppi_controller.cu:27-29).
-// The environment is the quadrotor gate race (13-state drone, constant-
-// velocity opponents); one opponent model, so the belief has one entry.
+// MPPIController and PRMPPIController run the quadrotor gate race (13-state
+// drone, constant-velocity opponents); stratrace::MPPIController runs the
+// 2-D race with the reference's opponent model.
 #pragma once
 
 #include <array>
+#include <cmath>
 #include <optional>
 #include <stdexcept>
 #include <string>
@@ -129,5 +136,192 @@ class MPPIController : public Controller
     EnvironmentConfig envConfig;
     dr_mppi* h = nullptr;
 };
+
+// PR-MPPI over opponent-velocity branches (the reference's PRMPPIConfig
+// fields, config.h:157-192, plus the branches of this build).  With
+// sampled = true, P = ceil((1 - delta) / delta) models are drawn from the
+// belief every step and averaged uniformly, as the reference does
+// (python-src/utils.py:537-538, prmppi_kernels.cu:23-51, :160); otherwise
+// the branches are averaged with the belief.
+struct PRMPPIConfig
+{
+    int nSamples = 1024;
+    int nTimesteps = 50;
+    float invTemperature = 1.0f;
+    std::array<float, ACTION_DIM> samplingNoise{2.0f, 0.5f, 0.5f, 0.5f};
+    std::array<float, ACTION_DIM> shiftFill{0.0f, 0.0f, 0.0f, 0.0f};
+    bool adaptTemperature = false;
+    float safetyWeight = 10000.0f;
+    float delta = 0.1f;
+    bool sampled = false;
+    std::vector<float> belief{1.0f};                           // one entry per branch
+    std::vector<std::array<float, 3 * MAX_OPPONENTS>> velShift{{}}; // per branch: opponent j's shift at [3 j .. 3 j + 2]
+};
+
+class PRMPPIController : public Controller
+{
+  public:
+    PRMPPIController(const EnvironmentConfig& c, const PRMPPIConfig& mc, int seed, std::optional<std::vector<float>> nominal = std::nullopt)
+        : mppiConfig(mc), envConfig(c)
+    {
+        const int B = (int)mc.belief.size();
+        if (B < 1 || B > DR_MAX_BRANCHES || (int)mc.velShift.size() != B)
+            throw std::runtime_error("Invalid PRMPPI construction: need 1..8 branches with one velocity shift each");
+        dr_pr_instance in{};
+        in.n_samples = mc.nSamples;
+        in.n_timesteps = mc.nTimesteps;
+        in.track = 0;
+        in.drone = 0;
+        in.seed = (uint64_t)(uint32_t)seed;
+        in.lambda = mc.invTemperature;
+        for (int d = 0; d < ACTION_DIM; ++d)
+        {
+            in.sigma[d] = mc.samplingNoise[d];
+            in.shift_fill[d] = mc.shiftFill[d];
+        }
+        in.adapt_lambda = mc.adaptTemperature ? 1 : 0;
+        in.n_branches = B;
+        in.sampled = mc.sampled ? 1 : 0;
+        in.n_models = mc.sampled ? (int)std::ceil((1.0f - mc.delta) / mc.delta) : 0;
+        in.safety_weight = mc.safetyWeight;
+        for (int b = 0; b < B; ++b)
+        {
+            in.belief[b] = mc.belief[b];
+            for (int j = 0; j < MAX_OPPONENTS; ++j)
+                for (int k = 0; k < 3; ++k)
+                    in.vel_shift[b][j][k] = mc.velShift[b][3 * j + k];
+        }
+        const int nd = 1 + c.maxOpponents;
+        std::vector<int32_t> world(nd, 0);
+        check(dr_pr_create(&c.quad, &c.cost, &c.track, 1, &in, 1, world.data(), nd, &h), "PRMPPIController");
+        const size_t n = (size_t)mc.nTimesteps * ACTION_DIM;
+        if (nominal && nominal->size() != n)
+        {
+            dr_pr_destroy(h);
+            throw std::runtime_error("Invalid PRMPPI construction: expected nominal size " + std::to_string(n) + ", got " +
+                                     std::to_string(nominal->size()));
+        }
+        h_nom_nominal = nominal ? *nominal : std::vector<float>(n, 0.0f);
+        h_rob_nominal = h_nom_nominal;
+        check(dr_pr_set_nominals(h, 0, h_nom_nominal.data(), h_rob_nominal.data(), (int)n), "PRMPPIController");
+    }
+
+    ~PRMPPIController() override { dr_pr_destroy(h); }
+    PRMPPIController(const PRMPPIController&) = delete;
+    PRMPPIController& operator=(const PRMPPIController&) = delete;
+
+    void getControl(const SimState& state, float* outAction) override
+    {
+        if (state.nOpponents != envConfig.maxOpponents)
+            throw std::runtime_error("getControl: nOpponents must equal maxOpponents");
+        std::vector<dr_drone_state> w(1 + envConfig.maxOpponents);
+        w[0] = state.ego;
+        for (int j = 0; j < envConfig.maxOpponents; ++j)
+        {
+            w[1 + j] = dr_drone_state{};
+            w[1 + j].q[0] = 1.0f;
+            for (int d = 0; d < 3; ++d)
+            {
+                w[1 + j].p[d] = state.opponents[j].p[d];
+                w[1 + j].v[d] = state.opponents[j].v[d];
+            }
+        }
+        check(dr_pr_set_drones(h, w.data(), (int)w.size()), "getControl");
+        check(dr_pr_step(h, 1, 0, outAction), "getControl");
+        check(dr_pr_get_nominals(h, 0, h_nom_nominal.data(), h_rob_nominal.data(), (int)h_nom_nominal.size()), "getControl");
+        dr_pr_diag d{};
+        check(dr_pr_get_diag(h, &d, 1), "getControl");
+        useNomPlan = d.use_nom_plan != 0;
+        resetNom = d.reset_nom != 0;
+        for (int j = 0; j < 3; ++j)
+            temperatures[j] = d.lambda[j];
+    }
+
+    PRMPPIConfig mppiConfig;
+    std::vector<float> h_nom_nominal; // (T, dim), unshifted
+    std::vector<float> h_rob_nominal; // (T, dim), unshifted
+    bool resetNom = false;
+    bool useNomPlan = true;
+    std::array<float, 3> temperatures{}; // nominal/full, robust/full, robust/safety
+
+  private:
+    EnvironmentConfig envConfig;
+    dr_prengine* h = nullptr;
+};
+
+namespace stratrace {
+
+inline constexpr int ACTION_DIM = 2;
+
+// The reference's SimState / EnvironmentConfig of the StratRace2D build
+// (env_stratrace_defs.h:53-96), with the track grids held by value.
+using SimState = dr_sr_state;
+using EnvironmentConfig = dr_sr_env;
+
+// MPPIConfig fields of the dense plain path (config.h:116-154).
+struct MPPIConfig
+{
+    int nSamples = 65536;
+    int nTimesteps = 70;
+    float invTemperature = 10.0f;
+    float samplingNoise = 1.5f;
+    dr_sr_cost cost{1.3f, 50.0f, 1.7f, 1e6f, 1e6f, 1e5f, 500.0f, 0.0f};
+    bool adaptTemperature = false;
+};
+
+class MPPIController
+{
+  public:
+    // trueTheta: the opponent model the rollouts assume (a committed belief).
+    MPPIController(const EnvironmentConfig& c, const MPPIConfig& mc, int seed, int trueTheta = 0,
+                   std::optional<std::vector<float>> nominal = std::nullopt)
+        : mppiConfig(mc)
+    {
+        dr_sr_instance in{};
+        in.n_samples = mc.nSamples;
+        in.n_timesteps = mc.nTimesteps;
+        in.true_theta = trueTheta;
+        in.adapt_lambda = mc.adaptTemperature ? 1 : 0;
+        in.seed = (uint64_t)(uint32_t)seed;
+        in.lambda = mc.invTemperature;
+        in.sigma = mc.samplingNoise;
+        check(dr_sr_create(&c, &mc.cost, &in, 1, &h), "MPPIController");
+        const size_t n = (size_t)mc.nTimesteps * ACTION_DIM;
+        if (nominal && nominal->size() != n)
+        {
+            dr_sr_destroy(h);
+            throw std::runtime_error("Invalid MPPI construction: expected nominal size " + std::to_string(n) + ", got " +
+                                     std::to_string(nominal->size()));
+        }
+        h_nominal = nominal ? *nominal : std::vector<float>(n, 0.0f);
+        check(dr_sr_set_nominal(h, 0, h_nominal.data(), (int)n), "MPPIController");
+    }
+
+    ~MPPIController() { dr_sr_destroy(h); }
+    MPPIController(const MPPIController&) = delete;
+    MPPIController& operator=(const MPPIController&) = delete;
+
+    void getControl(const SimState& state, float* outAction)
+    {
+        check(dr_sr_set_states(h, &state, 1), "getControl");
+        check(dr_sr_step(h, 1, 0, outAction), "getControl");
+        check(dr_sr_get_nominal(h, 0, h_nominal.data(), (int)h_nominal.size()), "getControl");
+        dr_diag d{};
+        check(dr_sr_get_diag(h, &d, 1), "getControl");
+        mppiConfig.invTemperature = d.lambda;
+        lastMinCost = d.min_cost;
+        lastNu = d.nu;
+    }
+
+    MPPIConfig mppiConfig;
+    std::vector<float> h_nominal; // (T, 2), already shifted for the next step
+    float lastMinCost = 0.0f;
+    float lastNu = 0.0f;
+
+  private:
+    dr_sr_engine* h = nullptr;
+};
+
+} // namespace stratrace
 
 } // namespace drone_race

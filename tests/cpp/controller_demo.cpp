@@ -2,6 +2,7 @@
 // reference's SimulationEngine drives its controller (engine.cpp:314-363):
 // getControl, apply the action, repeat.  Prints the actions and the final
 // nominal; the GPU test compares them with the batch path.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 
@@ -56,6 +57,70 @@ int main(int argc, char** argv)
         catch (const std::runtime_error& e)
         {
             std::printf("throws: %s\n", e.what());
+        }
+        // PR-MPPI over two opponent-velocity branches
+        PRMPPIConfig pc;
+        pc.nSamples = 1024;
+        pc.nTimesteps = 30;
+        pc.shiftFill = {9.81f, 0.0f, 0.0f, 0.0f};
+        pc.belief = {0.3f, 0.7f};
+        pc.velShift.assign(2, {});
+        pc.velShift[0][0] = 1.0f;
+        pc.velShift[1][1] = -1.0f;
+        std::vector<float> pnom((size_t)pc.nTimesteps * ACTION_DIM);
+        for (int t = 0; t < pc.nTimesteps; ++t)
+            pnom[(size_t)t * 4] = 9.81f;
+        PRMPPIController pr(env, pc, 11, pnom);
+        for (int i = 0; i < steps; ++i)
+        {
+            float a[4];
+            pr.getControl(s, a);
+            std::printf("pr_action %d %.9g %.9g %.9g %.9g use_nom %d reset %d\n", i, a[0], a[1], a[2], a[3], (int)pr.useNomPlan,
+                        (int)pr.resetNom);
+        }
+        // StratRace2D: analytic circle of radius 6 (p, unit tangent, curvature 1/6)
+        static stratrace::EnvironmentConfig se{};
+        se.dt = 0.1f;
+        se.drone_radius = 0.4f;
+        se.max_speed[0] = 3.0f;
+        se.max_speed[1] = 2.0f;
+        se.max_accel[0] = 2.0f;
+        se.max_accel[1] = 3.0f;
+        se.track_width = 2.0f;
+        se.n_win_laps = 1;
+        se.kp = 10.0f;
+        se.kv = 5.0f;
+        se.max_opp_lat_dist_fact = 1.9f;
+        se.track_length = (float)(2.0 * M_PI * 6.0);
+        se.n_models = 2;
+        se.opp[0] = dr_sr_opp_config{0.0f, 1.0f / (1.6f * 1.6f), 50.0f, 1.0f, 0.1f};
+        se.opp[1] = dr_sr_opp_config{0.0f, 1.0f / (1.6f * 1.6f), 0.0f, 1.0f, 0.1f};
+        for (int i = 0; i < DR_SR_TRACK_SAMPLES; ++i)
+        {
+            const double th = 2.0 * M_PI * i / DR_SR_TRACK_SAMPLES;
+            se.p_grid[i][0] = (float)(6.0 * std::cos(th));
+            se.p_grid[i][1] = (float)(6.0 * std::sin(th));
+            se.t_grid[i][0] = (float)(-std::sin(th));
+            se.t_grid[i][1] = (float)std::cos(th);
+            se.kappa_grid[i] = 1.0f / 6.0f;
+        }
+        stratrace::MPPIConfig sc;
+        sc.nSamples = 4096;
+        sc.nTimesteps = 40;
+        stratrace::MPPIController sr(se, sc, 5, 1);
+        stratrace::SimState ss{};
+        const int i0 = 26, i1 = 102;
+        ss.pos[0] = se.p_grid[i0][0];
+        ss.pos[1] = se.p_grid[i0][1];
+        ss.pos[2] = se.p_grid[i1][0];
+        ss.pos[3] = se.p_grid[i1][1];
+        ss.s[0] = (float)i0 / DR_SR_TRACK_SAMPLES * se.track_length;
+        ss.s[1] = (float)i1 / DR_SR_TRACK_SAMPLES * se.track_length;
+        for (int i = 0; i < steps; ++i)
+        {
+            float a[2];
+            sr.getControl(ss, a);
+            std::printf("sr_action %d %.9g %.9g\n", i, a[0], a[1]);
         }
     }
     catch (const std::exception& e)

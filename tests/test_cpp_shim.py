@@ -54,3 +54,57 @@ def test_shim_matches_python_handle(pkg):
     py = np.stack([h.step(s, [o]) for _ in range(3)])
     h.close()
     assert np.array_equal(acts, py)
+    # PR-MPPI shim: same steps through the Python PR handle (opponent as an uncontrolled drone)
+    pr_acts = np.array([[float(x) for x in l.split()[2:6]] for l in res if l.startswith("pr_action")], dtype=np.float32)
+    sc = pkg.scenarios
+    shift = np.zeros((2, 1, 3), np.float32)
+    shift[0, 0, 0] = 1.0
+    shift[1, 0, 1] = -1.0
+    ins = sc.pr_instance(1024, 30, 0, 0, 11, [0.3, 0.7], shift, lam=1.0, sigma=(2.0, 0.5, 0.5, 0.5), fill=(9.81, 0, 0, 0))
+    drones = (abi.DroneState * 2)()
+    drones[0] = s
+    drones[1].p[:] = [8.0, -6.0, 4.0]
+    drones[1].v[:] = [1.0, 1.0, 0.0]
+    drones[1].q[:] = [1.0, 0.0, 0.0, 0.0]
+    scn = sc.PrScenario("shim_pr", [tr], [ins], [0, 0], drones, 1, "")
+    scn.params, scn.cost = P, K
+    eng = pkg.PREngine(scn)
+    pnom = np.zeros((30, 4), np.float32)
+    pnom[:, 0] = 9.81
+    eng.set_nominals(0, pnom, pnom)
+    py_pr = []
+    for _ in range(3):
+        eng.set_drones(drones)
+        py_pr.append(eng.step(1, advance=False)[0])
+    eng.close()
+    assert np.array_equal(pr_acts, np.stack(py_pr))
+    # StratRace2D shim on the analytic circle
+    sr_acts = np.array([[float(x) for x in l.split()[2:4]] for l in res if l.startswith("sr_action")], dtype=np.float32)
+    env = abi.SrEnv()
+    env.dt, env.drone_radius = 0.1, 0.4
+    env.max_speed[:] = [3.0, 2.0]
+    env.max_accel[:] = [2.0, 3.0]
+    env.track_width, env.n_win_laps, env.kp, env.kv, env.max_opp_lat_dist_fact = 2.0, 1, 10.0, 5.0, 1.9
+    env.track_length = float(np.float32(2.0 * np.pi * 6.0))
+    env.n_models = 2
+    env.opp[0] = abi.SrOppConfig(0.0, float(np.float32(1.0) / (np.float32(1.6) * np.float32(1.6))), 50.0, 1.0, 0.1)
+    env.opp[1] = abi.SrOppConfig(0.0, float(np.float32(1.0) / (np.float32(1.6) * np.float32(1.6))), 0.0, 1.0, 0.1)
+    th = 2.0 * np.pi * np.arange(512) / 512
+    for i in range(512):
+        env.p_grid[i][:] = [float(np.float32(6.0 * np.cos(th[i]))), float(np.float32(6.0 * np.sin(th[i])))]
+        env.t_grid[i][:] = [float(np.float32(-np.sin(th[i]))), float(np.float32(np.cos(th[i])))]
+        env.kappa_grid[i] = float(np.float32(1.0) / np.float32(6.0))
+    st = abi.SrState()
+    st.pos[:] = [env.p_grid[26][0], env.p_grid[26][1], env.p_grid[102][0], env.p_grid[102][1]]
+    st.s[:] = [float(np.float32(np.float32(26) / np.float32(512) * np.float32(env.track_length))),
+               float(np.float32(np.float32(102) / np.float32(512) * np.float32(env.track_length)))]
+    sscn = sc.SrScenario("shim_sr", env, abi.SrCost(1.3, 50.0, 1.7, 1e6, 1e6, 1e5, 500.0, 0.0),
+                         [sc.sr_instance(4096, 40, 5, theta=1, lam=10.0, sigma=1.5)], [st], 1, "")
+    seng = pkg.SREngine(sscn)
+    py_sr = []
+    for _ in range(3):
+        seng.set_states([st])
+        py_sr.append(seng.step(1, advance=False)[0])
+    seng.close()
+    assert np.array_equal(sr_acts, np.stack(py_sr))
+

@@ -118,6 +118,13 @@ class SrInstance(C.Structure):
                 ("_pad", C.c_int32 * 2)]
 
 
+class PrsrInstance(C.Structure):
+    _fields_ = [("n_samples", C.c_int32), ("n_timesteps", C.c_int32), ("sampled", C.c_int32), ("n_draws", C.c_int32),
+                ("seed", C.c_uint64), ("lambda_", C.c_float), ("sigma", C.c_float), ("safety_weight", C.c_float),
+                ("adapt_lambda", C.c_int32), ("belief", C.c_float * SR_MAX_MODELS), ("shift_fill", C.c_float * 2),
+                ("true_theta", C.c_int32), ("_pad", C.c_int32)]
+
+
 # Every symbol include/dr_api.h declares, with its ctypes signature.
 _P = C.c_void_p
 _I = C.c_int
@@ -193,6 +200,10 @@ SIGNATURES = {
     "dr_sr_get_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
     "dr_sr_set_timing": (_I, [_P, _I]),
     "dr_sr_last_timing": (_I, [_P, _F, _I]),
+    "dr_prsr_create": (_I, [C.POINTER(SrEnv), C.POINTER(SrCost), C.POINTER(PrsrInstance), _I, C.POINTER(_P)]),
+    "dr_pr_set_sr_states": (_I, [_P, C.POINTER(SrState), _I]),
+    "dr_pr_get_sr_states": (_I, [_P, C.POINTER(SrState), _I]),
+    "dr_pr_get_events": (_I, [_P, C.POINTER(C.c_int32), _I]),
 }
 
 _lib = None

@@ -17,6 +17,9 @@ int host_fail(int code, const std::string& msg);
 // DR_OK, or DR_EINVAL with a message naming track idx.
 int check_track(const dr_track& t, int idx);
 
+// DR_OK, or DR_EINVAL naming the first invalid field of a StratRace2D environment.
+int check_sr_env(const dr_sr_env& env);
+
 template <class T>
 struct DevBuf
 {

@@ -33,6 +33,14 @@ hipError_t launch_pr_update(int n_inst, hipStream_t st, const PrInstDev* insts, 
                             const dr_drone_state* drones, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC);
 hipError_t launch_pr_advance(int n_drones, hipStream_t st, const PrInstDev* insts, const float* actions, dr_drone_state* drones,
                              const int* drone_inst, const dr_track* tracks, const QuadConst& C, float margin);
+hipError_t launch_prsr_rollout(int waves, int grid, hipStream_t st, const PrInstDev* insts, const PrDyn* dyn, const PrTile* tiles, int n_tiles,
+                               const dr_sr_env* env, const dr_sr_cost& KC, const dr_sr_state* states, const float* work, float2* pair,
+                               float4* noise);
+hipError_t launch_prsr_update(int n_inst, hipStream_t st, const PrInstDev* insts, PrDyn* dyn, const double* part, const double* den,
+                              float* work, float* plan, float* cand, int* models, float* actions, const dr_sr_env* env, const dr_sr_cost& KC,
+                              const dr_sr_state* states);
+hipError_t launch_prsr_advance(int n_inst, hipStream_t st, const PrInstDev* insts, const PrDyn* dyn, const dr_sr_env* env, dr_sr_state* states,
+                               const float* actions, int* events);
 
 } // namespace dr
 
@@ -51,6 +59,22 @@ void shift_plan(const float* src, float* dst, int T, const float* fill)
             dst[t * 4 + d] = (t + 1 < T) ? src[(t + 1) * 4 + d] : fill[d];
 }
 
+// Plans of D control dimensions <-> the engine's 4-wide layout (D = 2 for
+// StratRace2D: lanes z, w stay zero).
+void widen(const float* src, float* dst, int T, int D)
+{
+    for (int t = 0; t < T; ++t)
+        for (int d = 0; d < 4; ++d)
+            dst[t * 4 + d] = d < D ? src[t * D + d] : 0.0f;
+}
+
+void narrow(const float* src, float* dst, int T, int D)
+{
+    for (int t = 0; t < T; ++t)
+        for (int d = 0; d < D; ++d)
+            dst[t * D + d] = src[t * 4 + d];
+}
+
 } // namespace
 
 struct dr_prengine
@@ -62,6 +86,10 @@ struct dr_prengine
     dr_cost_cfg cost{};
     std::vector<dr_track> tracks;
     std::vector<dr_pr_instance> inst;
+    int env_kind = 0; // 0 quadrotor race, 1 StratRace2D
+    dr_sr_env srenv{};
+    dr_sr_cost srcost{};
+    std::vector<dr_prsr_instance> srinst;
     std::vector<int> drone_world;
     std::vector<int> drone_inst;
     int n_drones = 0;
@@ -89,6 +117,9 @@ struct dr_prengine
     DevBuf<double> d_part, d_den;
     DevBuf<int> d_models;
     DevBuf<float> d_actions;
+    DevBuf<dr_sr_env> d_srenv;
+    DevBuf<dr_sr_state> d_srstates;
+    DevBuf<int> d_events;
 
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     int graph_steps[2] = {0, 0};
@@ -129,6 +160,7 @@ struct dr_prengine
             w = std::max(w, 2 * I.M);
         return std::min(w, PR_MAX_WAVES);
     }
+    int dim() const { return env_kind ? 2 : 4; } // control dimensions of the plans the caller sees
     int build_layout();
     int enqueue_step(bool advance, int timed_index);
     int run(int n_steps, bool advance);
@@ -143,32 +175,57 @@ int dr_prengine::build_layout()
     agg_total = plan_total = model_total = den_total = 0;
     for (int i = 0; i < n; ++i)
     {
-        const dr_pr_instance& s = inst[i];
         PrInstDev& I = idev[i];
-        I.K = s.n_samples;
-        I.T = s.n_timesteps;
-        I.track = s.track;
-        I.drone = s.drone;
-        I.key0 = (uint32_t)(s.seed & 0xffffffffu);
-        I.key1 = (uint32_t)(s.seed >> 32);
-        for (int d = 0; d < 4; ++d)
+        if (env_kind == 1)
         {
-            I.sigma[d] = s.sigma[d];
-            I.shift_fill[d] = s.shift_fill[d];
+            const dr_prsr_instance& r = srinst[i];
+            I.K = r.n_samples;
+            I.T = r.n_timesteps;
+            I.track = 0;
+            I.drone = i;
+            I.key0 = (uint32_t)(r.seed & 0xffffffffu);
+            I.key1 = (uint32_t)(r.seed >> 32);
+            I.sigma[0] = I.sigma[1] = r.sigma;
+            I.sigma[2] = I.sigma[3] = 0.0f;
+            I.shift_fill[0] = r.shift_fill[0];
+            I.shift_fill[1] = r.shift_fill[1];
+            I.shift_fill[2] = I.shift_fill[3] = 0.0f;
+            I.adapt = r.adapt_lambda;
+            I.n_opp = 0;
+            I.B = srenv.n_models;
+            I.sampled = r.sampled ? 1 : 0;
+            I.M = r.sampled ? r.n_draws : srenv.n_models;
+            I.safety_weight = r.safety_weight;
+            I.world_theta = r.true_theta;
         }
-        I.adapt = s.adapt_lambda;
-        I.n_opp = 0;
-        for (int d = 0; d < n_drones; ++d)
-            if (d != s.drone && drone_world[d] == drone_world[s.drone])
-                I.opp[I.n_opp++] = d;
-        I.M = n_models_of(s);
-        I.B = s.n_branches;
-        I.sampled = s.sampled ? 1 : 0;
-        I.safety_weight = s.safety_weight;
-        for (int b = 0; b < DR_MAX_BRANCHES; ++b)
-            for (int j = 0; j < MAX_OPP; ++j)
-                for (int c = 0; c < 3; ++c)
-                    I.shift[b][j][c] = s.vel_shift[b][j][c];
+        else
+        {
+            const dr_pr_instance& s = inst[i];
+            I.K = s.n_samples;
+            I.T = s.n_timesteps;
+            I.track = s.track;
+            I.drone = s.drone;
+            I.key0 = (uint32_t)(s.seed & 0xffffffffu);
+            I.key1 = (uint32_t)(s.seed >> 32);
+            for (int d = 0; d < 4; ++d)
+            {
+                I.sigma[d] = s.sigma[d];
+                I.shift_fill[d] = s.shift_fill[d];
+            }
+            I.adapt = s.adapt_lambda;
+            I.n_opp = 0;
+            for (int d = 0; d < n_drones; ++d)
+                if (d != s.drone && drone_world[d] == drone_world[s.drone])
+                    I.opp[I.n_opp++] = d;
+            I.M = n_models_of(s);
+            I.B = s.n_branches;
+            I.sampled = s.sampled ? 1 : 0;
+            I.safety_weight = s.safety_weight;
+            for (int b = 0; b < DR_MAX_BRANCHES; ++b)
+                for (int j = 0; j < MAX_OPP; ++j)
+                    for (int c = 0; c < 3; ++c)
+                        I.shift[b][j][c] = s.vel_shift[b][j][c];
+        }
         I.noise_off = noise_total;
         noise_total += (long long)I.K * I.T;
         I.pair_off = pair_total;
@@ -245,8 +302,12 @@ int dr_prengine::enqueue_step(bool advance, int timed_index)
     hipEvent_t* ev = (timed_index >= 0) ? &events[(size_t)timed_index * 5] : nullptr;
     if (ev)
         HIPCHK(hipEventRecord(ev[0], stream));
-    HIPCHK(launch_pr_rollout(waves(), launch.rollout_grid, stream, d_inst.p, d_dyn.p, d_rtiles.p, (int)rtiles.size(), d_tracks.p, d_drones.p,
-                             d_work.p, d_pair.p, d_noise.p, C, Q, cost));
+    if (env_kind == 1)
+        HIPCHK(launch_prsr_rollout(waves(), launch.rollout_grid, stream, d_inst.p, d_dyn.p, d_rtiles.p, (int)rtiles.size(), d_srenv.p, srcost,
+                                   d_srstates.p, d_work.p, d_pair.p, d_noise.p));
+    else
+        HIPCHK(launch_pr_rollout(waves(), launch.rollout_grid, stream, d_inst.p, d_dyn.p, d_rtiles.p, (int)rtiles.size(), d_tracks.p,
+                                 d_drones.p, d_work.p, d_pair.p, d_noise.p, C, Q, cost));
     if (ev)
         HIPCHK(hipEventRecord(ev[1], stream));
     HIPCHK(launch_pr_agg((int)atiles.size(), stream, d_inst.p, d_dyn.p, d_atiles.p, d_pair.p, d_agg.p, d_ablk.p));
@@ -256,11 +317,21 @@ int dr_prengine::enqueue_step(bool advance, int timed_index)
                           d_work.p, d_part.p, d_den.p));
     if (ev)
         HIPCHK(hipEventRecord(ev[3], stream));
-    HIPCHK(launch_pr_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_work.p, d_plan.p, d_cand.p, d_models.p, d_actions.p, d_tracks.p,
-                            d_drones.p, C, Q, cost));
-    if (advance)
-        HIPCHK(launch_pr_advance(n_drones, stream, d_inst.p, d_actions.p, d_drones.p, d_drone_inst.p, d_tracks.p, C,
-                                 cost.gate_traversal_margin));
+    if (env_kind == 1)
+    {
+        HIPCHK(launch_prsr_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_work.p, d_plan.p, d_cand.p, d_models.p, d_actions.p,
+                                  d_srenv.p, srcost, d_srstates.p));
+        if (advance)
+            HIPCHK(launch_prsr_advance(n, stream, d_inst.p, d_dyn.p, d_srenv.p, d_srstates.p, d_actions.p, d_events.p));
+    }
+    else
+    {
+        HIPCHK(launch_pr_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_work.p, d_plan.p, d_cand.p, d_models.p, d_actions.p,
+                                d_tracks.p, d_drones.p, C, Q, cost));
+        if (advance)
+            HIPCHK(launch_pr_advance(n_drones, stream, d_inst.p, d_actions.p, d_drones.p, d_drone_inst.p, d_tracks.p, C,
+                                     cost.gate_traversal_margin));
+    }
     if (ev)
         HIPCHK(hipEventRecord(ev[4], stream));
     return DR_OK;
@@ -532,6 +603,8 @@ void dr_pr_destroy(dr_prengine* e)
 
 int dr_pr_set_drones(dr_prengine* e, const dr_drone_state* states, int n_drones)
 {
+    if (e && e->env_kind == 1)
+        return host_fail(DR_ESTATE, "set_drones: a StratRace2D engine takes dr_pr_set_sr_states");
     if (!e || !states || n_drones != e->n_drones)
         return host_fail(DR_EINVAL, "set_drones: bad handle, pointer or count");
     return copy_in(e, e->d_drones.p, states, (size_t)n_drones);
@@ -539,6 +612,8 @@ int dr_pr_set_drones(dr_prengine* e, const dr_drone_state* states, int n_drones)
 
 int dr_pr_get_drones(dr_prengine* e, dr_drone_state* states, int n_drones)
 {
+    if (e && e->env_kind == 1)
+        return host_fail(DR_ESTATE, "get_drones: a StratRace2D engine takes dr_pr_get_sr_states");
     if (!e || !states || n_drones != e->n_drones)
         return host_fail(DR_EINVAL, "get_drones: bad handle, pointer or count");
     return copy_out(e, states, e->d_drones.p, (size_t)n_drones);
@@ -549,11 +624,12 @@ int dr_pr_set_nominals(dr_prengine* e, int instance, const float* nom, const flo
     if (int rc = check_pr_inst(e, instance))
         return rc;
     const PrInstDev& I = e->idev[instance];
-    if (!nom || !rob || n_floats != I.T * 4)
-        return host_fail(DR_EINVAL, "set_nominals: expected two plans of T*4 floats");
+    const int D = e->dim();
+    if (!nom || !rob || n_floats != I.T * D)
+        return host_fail(DR_EINVAL, "set_nominals: expected two plans of T*" + std::to_string(D) + " floats");
     std::vector<float> plan(2 * I.T * 4), work(2 * I.T * 4);
-    std::memcpy(plan.data(), nom, n_floats * sizeof(float));
-    std::memcpy(plan.data() + I.T * 4, rob, n_floats * sizeof(float));
+    widen(nom, plan.data(), I.T, D);
+    widen(rob, plan.data() + I.T * 4, I.T, D);
     for (int p = 0; p < 2; ++p)
         shift_plan(&plan[p * I.T * 4], &work[p * I.T * 4], I.T, I.shift_fill);
     if (int rc = copy_in(e, e->d_plan.p + I.plan_off, plan.data(), plan.size()))
@@ -566,11 +642,15 @@ int dr_pr_get_nominals(dr_prengine* e, int instance, float* nom, float* rob, int
     if (int rc = check_pr_inst(e, instance))
         return rc;
     const PrInstDev& I = e->idev[instance];
-    if (!nom || !rob || n_floats != I.T * 4)
-        return host_fail(DR_EINVAL, "get_nominals: expected two plans of T*4 floats");
-    if (int rc = copy_out(e, nom, e->d_plan.p + I.plan_off, (size_t)n_floats))
+    const int D = e->dim();
+    if (!nom || !rob || n_floats != I.T * D)
+        return host_fail(DR_EINVAL, "get_nominals: expected two plans of T*" + std::to_string(D) + " floats");
+    std::vector<float> plan(2 * I.T * 4);
+    if (int rc = copy_out(e, plan.data(), e->d_plan.p + I.plan_off, plan.size()))
         return rc;
-    return copy_out(e, rob, e->d_plan.p + I.plan_off + I.T * 4, (size_t)n_floats);
+    narrow(plan.data(), nom, I.T, D);
+    narrow(plan.data() + I.T * 4, rob, I.T, D);
+    return DR_OK;
 }
 
 int dr_pr_set_belief(dr_prengine* e, int instance, const float* belief, int n_branches)
@@ -627,7 +707,13 @@ int dr_pr_step(dr_prengine* e, int n_steps, int advance, float* actions)
     if (int rc = dr_pr_synchronize(e))
         return rc;
     if (actions)
-        return copy_out(e, actions, e->d_actions.p, e->inst.size() * 4);
+    {
+        const int n = (int)e->inst.size();
+        std::vector<float> a((size_t)n * 4);
+        if (int rc = copy_out(e, a.data(), e->d_actions.p, a.size()))
+            return rc;
+        narrow(a.data(), actions, n, e->dim());
+    }
     return DR_OK;
 }
 
@@ -662,9 +748,16 @@ int dr_pr_get_noise(dr_prengine* e, int instance, float* noise, size_t n)
     if (int rc = check_pr_inst(e, instance))
         return rc;
     const PrInstDev& I = e->idev[instance];
-    if (!noise || n != (size_t)I.K * I.T * 4)
-        return host_fail(DR_EINVAL, "get_noise: expected K*T*4 floats");
-    return copy_out(e, reinterpret_cast<float4*>(noise), e->d_noise.p + I.noise_off, (size_t)I.K * I.T);
+    const int D = e->dim();
+    if (!noise || n != (size_t)I.K * I.T * D)
+        return host_fail(DR_EINVAL, "get_noise: expected K*T*" + std::to_string(D) + " floats");
+    if (D == 4)
+        return copy_out(e, reinterpret_cast<float4*>(noise), e->d_noise.p + I.noise_off, (size_t)I.K * I.T);
+    std::vector<float> rows((size_t)I.K * I.T * 4);
+    if (int rc = copy_out(e, reinterpret_cast<float4*>(rows.data()), e->d_noise.p + I.noise_off, (size_t)I.K * I.T))
+        return rc;
+    narrow(rows.data(), noise, I.K * I.T, D);
+    return DR_OK;
 }
 
 int dr_pr_get_pair_costs(dr_prengine* e, int instance, float* out, int n_floats)
@@ -692,11 +785,15 @@ int dr_pr_get_candidates(dr_prengine* e, int instance, float* cand1, float* cand
     if (int rc = check_pr_inst(e, instance))
         return rc;
     const PrInstDev& I = e->idev[instance];
-    if (!cand1 || !cand2 || n_floats != I.T * 4)
-        return host_fail(DR_EINVAL, "get_candidates: expected two plans of T*4 floats");
-    if (int rc = copy_out(e, cand1, e->d_cand.p + I.plan_off, (size_t)n_floats))
+    const int D = e->dim();
+    if (!cand1 || !cand2 || n_floats != I.T * D)
+        return host_fail(DR_EINVAL, "get_candidates: expected two plans of T*" + std::to_string(D) + " floats");
+    std::vector<float> c(2 * I.T * 4);
+    if (int rc = copy_out(e, c.data(), e->d_cand.p + I.plan_off, c.size()))
         return rc;
-    return copy_out(e, cand2, e->d_cand.p + I.plan_off + I.T * 4, (size_t)n_floats);
+    narrow(c.data(), cand1, I.T, D);
+    narrow(c.data() + I.T * 4, cand2, I.T, D);
+    return DR_OK;
 }
 
 int dr_pr_get_models(dr_prengine* e, int instance, int32_t* branch_of_model, int n_models)
@@ -742,6 +839,129 @@ int dr_pr_last_timing(dr_prengine* e, float* ms, int n)
         return host_fail(DR_EINVAL, "last_timing: expected 5 floats");
     std::memcpy(ms, e->last_ms, sizeof(e->last_ms));
     return DR_OK;
+}
+
+int dr_prsr_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_prsr_instance* instances, int n_instances, dr_prengine** out)
+{
+    if (!out)
+        return host_fail(DR_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!env || !cost || !instances)
+        return host_fail(DR_EINVAL, "NULL argument");
+    if (n_instances < 1)
+        return host_fail(DR_EINVAL, "need at least one instance");
+    if (int rc = check_sr_env(*env))
+        return rc;
+    for (int i = 0; i < n_instances; ++i)
+    {
+        const dr_prsr_instance& r = instances[i];
+        const std::string w = "instance " + std::to_string(i) + ": ";
+        if (r.n_samples < 1 || r.n_samples > (1 << 24))
+            return host_fail(DR_EINVAL, w + "n_samples must be in [1, 2^24]");
+        if (r.n_timesteps < 1 || r.n_timesteps > DR_MAX_T)
+            return host_fail(DR_EINVAL, w + "n_timesteps must be in [1, 256]");
+        if (r.sampled != 0 && r.sampled != 1)
+            return host_fail(DR_EINVAL, w + "sampled must be 0 or 1");
+        if (r.sampled && (r.n_draws < 1 || r.n_draws > DR_MAX_MODELS))
+            return host_fail(DR_EINVAL, w + "n_draws must be in [1, 16]");
+        if (!valid_belief(r.belief, env->n_models))
+            return host_fail(DR_EINVAL, w + "belief entries must be finite and >= 0");
+        if (!(r.lambda > 0.0f))
+            return host_fail(DR_EINVAL, w + "lambda must be > 0");
+        if (!(r.sigma >= 0.0f) || !std::isfinite(r.sigma) || !std::isfinite(r.safety_weight))
+            return host_fail(DR_EINVAL, w + "sigma and safety_weight must be finite (sigma >= 0)");
+        if (r.true_theta < 0 || r.true_theta >= env->n_models)
+            return host_fail(DR_EINVAL, w + "true_theta out of range");
+    }
+    dr_prengine* e = new (std::nothrow) dr_prengine();
+    if (!e)
+        return host_fail(DR_ENOMEM, "host allocation failed");
+    auto bail = [&](int rc) {
+        delete e;
+        return rc;
+    };
+    if (hipGetDevice(&e->device) != hipSuccess)
+        return bail(host_fail(DR_EHIP, "no HIP device"));
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(host_fail(DR_EHIP, "stream creation failed"));
+    e->env_kind = 1;
+    e->srenv = *env;
+    e->srcost = *cost;
+    e->srinst.assign(instances, instances + n_instances);
+    e->inst.assign(n_instances, dr_pr_instance{});
+    e->n_drones = 0;
+    e->launch = dr_launch_cfg{0, 0, 1024, 1, 1, 1, 16, 0};
+    if (int rc = e->build_layout())
+        return bail(rc);
+    auto chk = [&](hipError_t x, const char* what) {
+        if (x != hipSuccess)
+        {
+            host_fail(x == hipErrorOutOfMemory ? DR_ENOMEM : DR_EHIP, std::string(what) + ": " + hipGetErrorString(x));
+            return false;
+        }
+        return true;
+    };
+    if (!chk(e->d_dyn.alloc(n_instances), "alloc dyn") || !chk(e->d_srenv.alloc(1), "alloc env") ||
+        !chk(e->d_srstates.alloc(n_instances), "alloc states") || !chk(e->d_events.alloc(n_instances), "alloc events") ||
+        !chk(e->d_work.alloc(e->plan_total), "alloc work") || !chk(e->d_plan.alloc(e->plan_total), "alloc plan") ||
+        !chk(e->d_cand.alloc(e->plan_total), "alloc cand") || !chk(e->d_noise.alloc(e->noise_total), "alloc noise") ||
+        !chk(e->d_pair.alloc(e->pair_total), "alloc pair") || !chk(e->d_agg.alloc(e->agg_total), "alloc agg") ||
+        !chk(e->d_models.alloc(e->model_total), "alloc models") || !chk(e->d_actions.alloc((size_t)n_instances * 4), "alloc actions"))
+        return bail(DR_ENOMEM);
+    std::vector<PrDyn> dyn(n_instances);
+    for (int i = 0; i < n_instances; ++i)
+    {
+        std::memset(&dyn[i], 0, sizeof(PrDyn));
+        for (int j = 0; j < 3; ++j)
+            dyn[i].lambda[j] = instances[i].lambda;
+        for (int b = 0; b < env->n_models; ++b)
+            dyn[i].belief[b] = instances[i].belief[b];
+        dyn[i].use_nom = 1;
+    }
+    std::vector<float> plan(e->plan_total, 0.0f), work(e->plan_total, 0.0f);
+    for (int i = 0; i < n_instances; ++i)
+    {
+        const PrInstDev& I = e->idev[i];
+        for (int p = 0; p < 2; ++p)
+            shift_plan(&plan[I.plan_off + p * I.T * 4], &work[I.plan_off + p * I.T * 4], I.T, I.shift_fill);
+    }
+    std::vector<dr_sr_state> st(n_instances);
+    std::memset(st.data(), 0, st.size() * sizeof(dr_sr_state));
+    std::vector<int> ev(n_instances, 0);
+    if (!chk(hipMemcpy(e->d_dyn.p, dyn.data(), dyn.size() * sizeof(PrDyn), hipMemcpyHostToDevice), "copy dyn") ||
+        !chk(hipMemcpy(e->d_srenv.p, env, sizeof(dr_sr_env), hipMemcpyHostToDevice), "copy env") ||
+        !chk(hipMemcpy(e->d_srstates.p, st.data(), st.size() * sizeof(dr_sr_state), hipMemcpyHostToDevice), "copy states") ||
+        !chk(hipMemcpy(e->d_events.p, ev.data(), ev.size() * sizeof(int), hipMemcpyHostToDevice), "copy events") ||
+        !chk(hipMemcpy(e->d_plan.p, plan.data(), plan.size() * sizeof(float), hipMemcpyHostToDevice), "copy plan") ||
+        !chk(hipMemcpy(e->d_work.p, work.data(), work.size() * sizeof(float), hipMemcpyHostToDevice), "copy work") ||
+        !chk(hipMemcpy(e->d_cand.p, plan.data(), plan.size() * sizeof(float), hipMemcpyHostToDevice), "copy cand"))
+        return bail(DR_EHIP);
+    *out = e;
+    return DR_OK;
+}
+
+int dr_pr_set_sr_states(dr_prengine* e, const dr_sr_state* states, int n)
+{
+    if (!e || !states || e->env_kind != 1 || n != (int)e->inst.size())
+        return host_fail(DR_EINVAL, "set_sr_states: needs a StratRace2D engine and one state per instance");
+    if (int rc = copy_in(e, e->d_srstates.p, states, (size_t)n))
+        return rc;
+    std::vector<int> ev(n, 0);
+    return copy_in(e, e->d_events.p, ev.data(), ev.size());
+}
+
+int dr_pr_get_sr_states(dr_prengine* e, dr_sr_state* states, int n)
+{
+    if (!e || !states || e->env_kind != 1 || n != (int)e->inst.size())
+        return host_fail(DR_EINVAL, "get_sr_states: needs a StratRace2D engine and one state per instance");
+    return copy_out(e, states, e->d_srstates.p, (size_t)n);
+}
+
+int dr_pr_get_events(dr_prengine* e, int32_t* events, int n)
+{
+    if (!e || !events || e->env_kind != 1 || n != (int)e->inst.size())
+        return host_fail(DR_EINVAL, "get_events: needs a StratRace2D engine and one value per instance");
+    return copy_out(e, events, e->d_events.p, (size_t)n);
 }
 
 } // extern "C"

@@ -49,6 +49,7 @@ struct PrInstDev
     int den_off;
     int n_chunks;
     int rtile_first, rtile_count;
+    int world_theta; // StratRace2D: opponent model of the instance's world
 };
 
 struct PrDyn

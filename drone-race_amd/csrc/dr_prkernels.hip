@@ -12,6 +12,7 @@
 // and the closed-loop step (SimulationEngine::dynStep, engine.cpp:286-312) -> k_pr_advance.
 #include <hip/hip_runtime.h>
 
+#include "dr_prcommon.h"
 #include "dr_prkernels.h"
 #include "dr_wave.h"
 
@@ -45,11 +46,6 @@ __device__ __forceinline__ void stage_model_opp(float4 (*dst)[2], const dr_drone
 {
     dst[j][0] = make_float4(o.p[0], o.p[1], o.p[2], 0.0f);
     dst[j][1] = make_float4(o.v[0] + I.shift[b][j][0], o.v[1] + I.shift[b][j][1], o.v[2] + I.shift[b][j][2], 0.0f);
-}
-
-__device__ __forceinline__ int model_branch(const PrInstDev& I, const PrDyn& Y, uint32_t step, int m)
-{
-    return I.sampled ? sample_branch(I.key0, I.key1, (uint32_t)m, step, Y.belief, I.B) : m;
 }
 
 // A workgroup of W waves covers 64 samples of one instance and up to W of
@@ -387,13 +383,9 @@ __global__ __launch_bounds__(256) void k_pr_update(const PrInstDev* __restrict__
                                                    int* __restrict__ models, float* __restrict__ actions, const dr_track* __restrict__ tracks,
                                                    const dr_drone_state* __restrict__ drones, QuadConst C, CostConst Q, dr_cost_cfg KC)
 {
-    __shared__ float s_c[3][DR_MAX_T * 4];
-    __shared__ float s_cc[2 * DR_MAX_MODELS], s_cs[2 * DR_MAX_MODELS];
+    __shared__ PrUpdShared sh;
     __shared__ float4 s_pv[DR_MAX_MODELS][MAX_OPP][2];
-    __shared__ int s_br[DR_MAX_MODELS];
     __shared__ dr_track s_track;
-    __shared__ double s_dn[3];
-    __shared__ int s_pick, s_use;
 
     const int inst = blockIdx.x;
     const PrInstDev& I = insts[inst];
@@ -401,10 +393,7 @@ __global__ __launch_bounds__(256) void k_pr_update(const PrInstDev* __restrict__
     const int T = I.T, M = I.M, n_opp = I.n_opp, tid = threadIdx.x;
     const uint32_t step = Y.step;
 
-    if (tid < 3)
-        s_dn[tid] = sum_in_order(den + I.den_off + tid, I.n_chunks, 3);
-    if (tid < M)
-        s_br[tid] = model_branch(I, Y, step, tid);
+    pr_begin(I, Y, step, den, sh, tid);
     {
         const float* src = reinterpret_cast<const float*>(tracks + I.track);
         float* dst = reinterpret_cast<float*>(&s_track);
@@ -415,17 +404,9 @@ __global__ __launch_bounds__(256) void k_pr_update(const PrInstDev* __restrict__
     for (int i = tid; i < M * n_opp; i += blockDim.x)
     {
         const int m = i / n_opp, j = i - m * n_opp;
-        stage_model_opp(s_pv[m], drones[I.opp[j]], I, s_br[m], j);
+        stage_model_opp(s_pv[m], drones[I.opp[j]], I, sh.br[m], j);
     }
-    for (int i = tid; i < 3 * T * 4; i += blockDim.x)
-    {
-        const int j = i / (T * 4), r = i - j * T * 4;
-        const double num = sum_in_order(part + I.part_off + (long long)j * T * 4 + r, I.n_chunks, 3LL * T * 4);
-        float u = work[I.plan_off + (j == 0 ? 0 : T * 4) + r];
-        if (s_dn[j] > 1e-30)
-            u = (float)(num / s_dn[j]);
-        s_c[j][r] = u;
-    }
+    pr_candidates(I, part, work, sh, tid);
     __syncthreads();
 
     if (tid < 2 * M)
@@ -434,7 +415,7 @@ __global__ __launch_bounds__(256) void k_pr_update(const PrInstDev* __restrict__
         // opponents, the next gate and the next control are kept in
         // registers so that no step waits on LDS.
         const int which = tid / M, m = tid - which * M;
-        const float4* U = reinterpret_cast<const float4*>(s_c[which]);
+        const float4* U = reinterpret_cast<const float4*>(sh.c[which]);
         RegOpp ro;
 #pragma unroll
         for (int j = 0; j < MAX_OPP; ++j)
@@ -480,71 +461,11 @@ __global__ __launch_bounds__(256) void k_pr_update(const PrInstDev* __restrict__
             }
         }
         acc = acc + final_cost(KC, s_track, fin);
-        s_cc[tid] = smax > 0.0f ? acc + I.safety_weight : acc;
-        s_cs[tid] = smax;
+        sh.cc[tid] = smax > 0.0f ? acc + I.safety_weight : acc;
+        sh.cs[tid] = smax;
     }
     __syncthreads();
-    if (tid == 0)
-    {
-        float c[2];
-        for (int which = 0; which < 2; ++which)
-        {
-            float a = 0.0f;
-            if (I.sampled)
-            {
-                for (int m = 0; m < M; ++m)
-                    a = a + s_cc[which * M + m];
-                a = a / (float)M;
-            }
-            else
-                for (int m = 0; m < M; ++m)
-                    a = fma_(Y.belief[m], s_cc[which * M + m], a);
-            c[which] = a;
-        }
-        const int pick = c[0] <= c[1] ? 0 : 1;
-        int use = 1;
-        for (int m = 0; m < M; ++m)
-            if (s_cs[pick * M + m] > 0.0f)
-                use = 0;
-        Y.cand_cost[0] = c[0];
-        Y.cand_cost[1] = c[1];
-        Y.use_nom = use;
-        Y.reset_nom = c[1] < c[0] ? 1 : 0;
-        for (int j = 0; j < 3; ++j)
-        {
-            const float nu = (float)s_dn[j];
-            Y.nu[j] = nu;
-            if (I.adapt)
-            {
-                float l = Y.lambda[j];
-                if (nu > 0.001f * (float)I.K && l > 0.01f)
-                    l = l * 0.9f;
-                else if (nu < 0.0005f * (float)I.K && l < 100.0f)
-                    l = l * 1.2f;
-                Y.lambda[j] = l;
-            }
-        }
-        Y.step = step + 1u;
-        s_pick = pick;
-        s_use = use;
-    }
-    __syncthreads();
-    const float* chosen = s_c[s_pick];
-    if (tid < 4)
-        actions[inst * 4 + tid] = s_use ? chosen[tid] : s_c[2][tid];
-    if (tid < M)
-        models[I.model_off + tid] = s_br[tid];
-    const int off = I.plan_off;
-    for (int i = tid; i < T * 4; i += blockDim.x)
-    {
-        const int t = i >> 2, d = i & 3;
-        plan[off + i] = chosen[i];
-        plan[off + T * 4 + i] = s_c[2][i];
-        work[off + i] = (t + 1 < T) ? chosen[i + 4] : I.shift_fill[d];
-        work[off + T * 4 + i] = (t + 1 < T) ? s_c[2][i + 4] : I.shift_fill[d];
-        cand[off + i] = s_c[0][i];
-        cand[off + T * 4 + i] = s_c[1][i];
-    }
+    pr_finish(I, Y, step, sh, work, plan, cand, models, actions, inst, tid);
 }
 
 // Closed loop: each controlled drone integrates its instance's action;

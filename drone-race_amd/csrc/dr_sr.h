@@ -497,6 +497,31 @@ DR_HD float final_cost(const dr_sr_env& E, const dr_sr_cost& K, const St& s, flo
     return cost;
 }
 
+// PRMPPIstateCost (costs.cuh:44-63): boundary band with the opponent-win
+// rule of trackBoundaryDist, win and opponent-win terms; no collision cost.
+DR_HD float pr_state_cost(const dr_sr_env& E, const dr_sr_cost& K, const St& s)
+{
+    float cost = 0.0f;
+    const float bd = boundary_dist(E, s, true);
+    const float danger = K.boundary_threshold_factor * E.drone_radius;
+    if (bd < danger)
+    {
+        const float q = bd / E.drone_radius;
+        cost = cost + K.boundary_cost * (q * q);
+    }
+    if (is_winner(E, s))
+        cost = cost - K.win_cost;
+    if (is_opp_winner(E, s))
+        cost = cost + K.opp_win_cost;
+    return cost;
+}
+
+// PRMPPIsafetyCost (costs.cuh:78-86): > 0 iff the state is unsafe.
+DR_HD float pr_safety(const dr_sr_env& E, const dr_sr_cost& K, const St& s)
+{
+    return E.drone_radius * K.coll_dist_factor - boundary_dist(E, s, false);
+}
+
 DR_HD St state_from_abi(const dr_sr_state& a)
 {
     St s;

@@ -274,6 +274,26 @@ bool finite_pos(float x) { return std::isfinite(x) && x > 0.0f; }
 
 } // namespace
 
+namespace dr {
+
+int check_sr_env(const dr_sr_env& env)
+{
+    if (!finite_pos(env.dt) || !finite_pos(env.drone_radius) || !finite_pos(env.track_length) || !finite_pos(env.track_width))
+        return host_fail(DR_EINVAL, "dt, drone_radius, track_length and track_width must be finite and > 0");
+    for (int a = 0; a < 2; ++a)
+        if (!finite_pos(env.max_speed[a]) || !finite_pos(env.max_accel[a]))
+            return host_fail(DR_EINVAL, "max_speed and max_accel must be finite and > 0");
+    if (env.n_models < 1 || env.n_models > DR_SR_MAX_MODELS)
+        return host_fail(DR_EINVAL, "n_models must be in [1, 4]");
+    for (int i = 0; i < DR_SR_TRACK_SAMPLES; ++i)
+        if (!std::isfinite(env.p_grid[i][0]) || !std::isfinite(env.p_grid[i][1]) || !std::isfinite(env.kappa_grid[i]) ||
+            !(env.t_grid[i][0] * env.t_grid[i][0] + env.t_grid[i][1] * env.t_grid[i][1] > 0.0f))
+            return host_fail(DR_EINVAL, "track grids must be finite with nonzero tangents");
+    return DR_OK;
+}
+
+} // namespace dr
+
 extern "C" {
 
 int dr_sr_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_sr_instance* instances, int n_instances, dr_sr_engine** out)
@@ -285,17 +305,8 @@ int dr_sr_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_sr_insta
         return host_fail(DR_EINVAL, "NULL argument");
     if (n_instances < 1)
         return host_fail(DR_EINVAL, "need at least one instance");
-    if (!finite_pos(env->dt) || !finite_pos(env->drone_radius) || !finite_pos(env->track_length) || !finite_pos(env->track_width))
-        return host_fail(DR_EINVAL, "dt, drone_radius, track_length and track_width must be finite and > 0");
-    for (int a = 0; a < 2; ++a)
-        if (!finite_pos(env->max_speed[a]) || !finite_pos(env->max_accel[a]))
-            return host_fail(DR_EINVAL, "max_speed and max_accel must be finite and > 0");
-    if (env->n_models < 1 || env->n_models > DR_SR_MAX_MODELS)
-        return host_fail(DR_EINVAL, "n_models must be in [1, 4]");
-    for (int i = 0; i < DR_SR_TRACK_SAMPLES; ++i)
-        if (!std::isfinite(env->p_grid[i][0]) || !std::isfinite(env->p_grid[i][1]) || !std::isfinite(env->kappa_grid[i]) ||
-            !(env->t_grid[i][0] * env->t_grid[i][0] + env->t_grid[i][1] * env->t_grid[i][1] > 0.0f))
-            return host_fail(DR_EINVAL, "track grids must be finite with nonzero tangents");
+    if (int rc = check_sr_env(*env))
+        return rc;
     for (int i = 0; i < n_instances; ++i)
     {
         const dr_sr_instance& s = instances[i];

@@ -420,3 +420,65 @@ class SREngine:
         a = np.empty(4, dtype=np.float32)
         check(self.lib.dr_sr_last_timing(self.h, _fptr(a), 4))
         return a
+
+
+class PRSREngine(PREngine):
+    """PR-MPPI controllers on StratRace2D worlds (the reference's PR-MPPI on its own environment):
+    plans are T x 2, one world state (abi.SrState) per instance."""
+
+    D = 2
+
+    def __init__(self, scenario):
+        lib = abi.load()
+        self.lib = lib
+        self.sc = scenario
+        n = len(scenario.instances)
+        self._inst = (abi.PrsrInstance * n)(*scenario.instances)
+        self.n_inst = n
+        h = C.c_void_p()
+        check(lib.dr_prsr_create(C.byref(scenario.env), C.byref(scenario.cost), self._inst, n, C.byref(h)))
+        self.h = h
+        self.set_states(scenario.states)
+
+    def n_models(self, i):
+        ins = self.sc.instances[i]
+        return ins.n_draws if ins.sampled else self.sc.env.n_models
+
+    def set_states(self, states):
+        arr = (abi.SrState * self.n_inst)(*states)
+        check(self.lib.dr_pr_set_sr_states(self.h, arr, self.n_inst))
+
+    def get_states(self):
+        arr = (abi.SrState * self.n_inst)()
+        check(self.lib.dr_pr_get_sr_states(self.h, arr, self.n_inst))
+        return arr
+
+    def events(self):
+        a = (C.c_int32 * self.n_inst)()
+        check(self.lib.dr_pr_get_events(self.h, a, self.n_inst))
+        return np.array(a[:], dtype=np.int32)
+
+    def get_nominals(self, i):
+        T = self.sc.instances[i].n_timesteps
+        a = np.empty(T * 2, dtype=np.float32)
+        b = np.empty(T * 2, dtype=np.float32)
+        check(self.lib.dr_pr_get_nominals(self.h, i, _fptr(a), _fptr(b), a.size))
+        return a.reshape(T, 2), b.reshape(T, 2)
+
+    def step(self, n_steps=1, advance=False):
+        a = np.empty(self.n_inst * 2, dtype=np.float32)
+        check(self.lib.dr_pr_step(self.h, n_steps, 1 if advance else 0, _fptr(a)))
+        return a.reshape(self.n_inst, 2)
+
+    def noise(self, i):
+        ins = self.sc.instances[i]
+        a = np.empty(ins.n_samples * ins.n_timesteps * 2, dtype=np.float32)
+        check(self.lib.dr_pr_get_noise(self.h, i, _fptr(a), a.size))
+        return a.reshape(ins.n_samples, ins.n_timesteps, 2)
+
+    def candidates(self, i):
+        T = self.sc.instances[i].n_timesteps
+        a = np.empty(T * 2, dtype=np.float32)
+        b = np.empty(T * 2, dtype=np.float32)
+        check(self.lib.dr_pr_get_candidates(self.h, i, _fptr(a), _fptr(b), a.size))
+        return a.reshape(T, 2), b.reshape(T, 2)

@@ -344,6 +344,37 @@ def sr_race(n_inst=1, n_samples=65536, n_timesteps=70, seed=42, steps=300, **env
                       f"StratRace2D, {n_inst} worlds, K={n_samples}, T={n_timesteps}, round track r=6 m")
 
 
+def prsr_instance(n_samples, n_timesteps, seed, belief, lam=10.0, sigma=0.5, safety_weight=1e4, sampled=0, n_draws=0,
+                  true_theta=0, adapt=0, fill=(0.0, 0.0)):
+    ins = abi.PrsrInstance(n_samples=n_samples, n_timesteps=n_timesteps, sampled=sampled, n_draws=n_draws, seed=seed,
+                           lambda_=lam, sigma=sigma, safety_weight=safety_weight, adapt_lambda=adapt, true_theta=true_theta)
+    for b, p in enumerate(belief):
+        ins.belief[b] = float(p)
+    ins.shift_fill[0], ins.shift_fill[1] = fill
+    return ins
+
+
+def pr_sr_cost():
+    """The PRMPPIConfig of stratRaceEnv (python-src/main.py:985-1004): collision factor 1.2,
+    boundary cost 20 (factor 1.3), win 1e6, opponent win 1e5, terminal advance weight 500."""
+    return abi.SrCost(coll_dist_factor=1.2, boundary_cost=20.0, boundary_threshold_factor=1.3, outside_cost=0.0,
+                      win_cost=1e6, opp_win_cost=1e5, final_adv_weight=500.0, final_opp_adv_weight=0.0)
+
+
+def pr_sr_race(n_inst=1, n_samples=65536, n_timesteps=60, seed=42, sampled=1, delta=0.1, steps=300, **env_kw):
+    """The reference's StratRace2D PR-MPPI race: K = 2^16, T = 60, temperature 10, sigma 0.5,
+    safety weight 1e4, P = ceil((1 - delta) / delta) models drawn from the belief (uniform over the
+    environment's models); world i follows model i % n_models."""
+    env = sr_env(**env_kw)
+    P = int(math.ceil((1 - delta) / delta))
+    belief = [1.0 / env.n_models] * env.n_models
+    inst = [prsr_instance(n_samples, n_timesteps, seed_mix(seed, 400 + i), belief, sampled=sampled, n_draws=P if sampled else 0,
+                          true_theta=i % env.n_models) for i in range(n_inst)]
+    st = [sr_start_state(env) for _ in range(n_inst)]
+    return SrScenario(f"pr_sr_race_{n_inst}x_k{n_samples}_t{n_timesteps}", env, pr_sr_cost(), inst, st, steps,
+                      f"PR-MPPI on StratRace2D, {n_inst} worlds, K={n_samples}, T={n_timesteps}, P={P if sampled else env.n_models}")
+
+
 def seed_mix(seed, i):
     """64-bit Philox key for instance i of a scenario seeded `seed`."""
     return ((int(seed) * 0x9E3779B97F4A7C15) ^ (0xA5A5A5A5 + 0x632BE59BD9B4E019 * (i + 1))) & 0xFFFFFFFFFFFFFFFF

@@ -386,6 +386,30 @@ typedef struct dr_sr_instance
     int32_t _pad[2];
 } dr_sr_instance;
 
+/* PR-MPPI on StratRace2D (PRMPPIController as the reference runs it on its
+ * environment): models are the environment's true models theta; sampled =
+ * 1 draws P = n_draws models from the belief each step and averages them
+ * uniformly (prmppi_kernels.cu:23-51, :160), sampled = 0 weights all
+ * n_models with the belief.  Running cost PRMPPIstateCost, safety
+ * PRMPPIsafetyCost, terminal finalCost (costs.cuh:44-86) with the fields of
+ * dr_sr_cost.  Plans are T x 2. */
+typedef struct dr_prsr_instance
+{
+    int32_t n_samples;
+    int32_t n_timesteps;
+    int32_t sampled;
+    int32_t n_draws;       /* P when sampled, 1..DR_MAX_MODELS */
+    uint64_t seed;
+    float lambda;          /* initial temperature of the three weightings */
+    float sigma;           /* samplingNoise */
+    float safety_weight;
+    int32_t adapt_lambda;
+    float belief[DR_SR_MAX_MODELS];
+    float shift_fill[2];
+    int32_t true_theta;    /* opponent model of the instance's world (closed loop) */
+    int32_t _pad;
+} dr_prsr_instance;
+
 typedef struct dr_sr_engine dr_sr_engine;
 
 int dr_sr_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_sr_instance* instances, int n_instances, dr_sr_engine** out);
@@ -412,6 +436,14 @@ int dr_sr_get_launch(dr_sr_engine* e, dr_launch_cfg* cfg);
 int dr_sr_set_timing(dr_sr_engine* e, int enable);
 /* Device ms summed over the last timed run: rollout, weighted average, update + world step, whole step. */
 int dr_sr_last_timing(dr_sr_engine* e, float* ms, int n);
+
+/* A PR-MPPI engine over StratRace2D worlds (one per instance); the other
+ * dr_pr_* calls apply, with T x 2 plans and [k][t][2] noise. */
+int dr_prsr_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_prsr_instance* instances, int n_instances, dr_prengine** out);
+int dr_pr_set_sr_states(dr_prengine* e, const dr_sr_state* states, int n);
+int dr_pr_get_sr_states(dr_prengine* e, dr_sr_state* states, int n);
+/* Last world event per instance (0 none, 1 ego outside, 2 ego wins, 3 opponent wins). */
+int dr_pr_get_events(dr_prengine* e, int32_t* events, int n);
 
 /* Single-controller entry points with the reference's shape. */
 typedef struct dr_mppi_cfg

@@ -15,6 +15,11 @@
 //   src/kernels.cu:100-155 (dense)                  rollout loop
 //   src/kernels.cu:307-408, :434-456                weighted update, L2 clamp of the nominal
 //   src/mppi_controller.cu:392-436                  action, temperature adaptation, zero-fill shift
+//   include/costs.cuh:44-86                         PRMPPIstateCost, PRMPPIsafetyCost, PRMPPIfinalCost
+//   src/prmppi_kernels.cu:55-393                    PR rollouts, averages, minima, candidates, safety
+//   src/prmppi_controller.cu:227-346                choice, action, three temperatures
+// The PR safety check reuses the candidate rollouts (same environment
+// noise); the reference rolls the chosen plan out again with fresh draws.
 // Where the engine departs from the reference (its own exp, x*x for
 // powf(x, 2), the grid index held at 511), this file departs the same way;
 // the departures change no value where the reference is defined.
@@ -525,6 +530,234 @@ S rollout(const Env<S>& E, const dr_sr_cost& K, const dr_sr_state& x0, int theta
     return cost + final_cost(E, K, s);
 }
 
+// ------------------------------------------------------------ PR-MPPI
+
+// PRMPPIstateCost (costs.cuh:44-63): trackBoundaryDist with the opponent-win
+// rule, band cost, win / opponent-win terms; no collision cost.
+template <class S>
+S pr_state_cost(const Env<S>& E, const dr_sr_cost& K, const SState<S>& s)
+{
+    S cost = 0;
+    const S bd = bdist(E, s, true);
+    const S danger = (S)K.boundary_threshold_factor * E.r;
+    if (bd < danger)
+    {
+        const S q = bd / E.r;
+        cost = cost + (S)K.boundary_cost * (q * q);
+    }
+    if (winner(E, s))
+        cost = cost - (S)K.win_cost;
+    if (s.laps[1] >= E.e.n_win_laps)
+        cost = cost + (S)K.opp_win_cost;
+    return cost;
+}
+
+// PRMPPIsafetyCost (costs.cuh:78-86): > 0 iff unsafe.
+template <class S>
+S pr_safety(const Env<S>& E, const dr_sr_cost& K, const SState<S>& s)
+{
+    return E.r * (S)K.coll_dist_factor - bdist(E, s, false);
+}
+
+// One PR rollout (prmppi_kernels.cu:98-127): plan U (T x 2, + eps when not
+// NULL), environment noise from counter word kk.
+template <class S, class UT>
+void prsr_rollout(const Env<S>& E, const dr_sr_cost& K, const dr_sr_state& x0, int theta, const UT* U, const float* eps, int T,
+                  uint32_t key0, uint32_t key1, uint32_t kk, uint32_t step, S* cost, S* safe)
+{
+    SState<S> s = load<S>(x0);
+    S acc = 0, decay = 1, smax = -(S)INFINITY;
+    for (int t = 0; t < T; ++t)
+    {
+        S u[2];
+        for (int d = 0; d < 2; ++d)
+        {
+            const float e = eps ? eps[t * 2 + d] : 0.0f;
+            u[d] = sizeof(S) == 4 ? (S)((float)U[t * 2 + d] + e) : (S)U[t * 2 + d] + (S)e;
+        }
+        Normals rng{key0, key1, kk, 4u * (uint32_t)t, step};
+        const int term = env_step(E, theta, u, s, rng, true);
+        acc = fm(pr_state_cost(E, K, s), decay, acc);
+        smax = mx(smax, pr_safety(E, K, s));
+        decay = decay * (S)0.97f;
+        if (term)
+            break;
+    }
+    *cost = acc + final_cost(E, K, s);
+    *safe = smax;
+}
+
+int prsr_models(const dr_prsr_instance& I, const dr_sr_env& env) { return I.sampled ? I.n_draws : env.n_models; }
+
+template <class S>
+S prsr_avg(const dr_prsr_instance& I, const float* belief, const S* c, int M)
+{
+    S acc = 0;
+    if (I.sampled)
+    {
+        for (int m = 0; m < M; ++m)
+            acc = acc + c[m];
+        return acc / (S)M;
+    }
+    for (int m = 0; m < M; ++m)
+        acc = fm((S)belief[m], c[m], acc);
+    return acc;
+}
+
+// Environment-noise counter word of rollout (model m, sample k) and of
+// candidate lane l.
+inline uint32_t sample_stream(int m, int K, int k) { return (uint32_t)m * (uint32_t)K + (uint32_t)k; }
+inline uint32_t cand_stream(int lane) { return 0xF0000000u + (uint32_t)lane; }
+
+template <class S>
+void prsr_step(const dr_sr_env& env, const dr_sr_cost& K, const dr_sr_state& x0, const dr_prsr_instance& I, const float* belief,
+               uint32_t step, const float lam[3], const float* nom, const float* rob, int n_threads, float* noise_out, double* pair_out,
+               double* agg_out, int* models_out, double* cand_out, double* res_out)
+{
+    const int N = I.n_samples, T = I.n_timesteps, M = prsr_models(I, env);
+    const uint32_t key0 = (uint32_t)(I.seed & 0xffffffffu), key1 = (uint32_t)(I.seed >> 32);
+    const Env<S> E(env);
+    std::vector<int> th(M);
+    for (int m = 0; m < M; ++m)
+        th[m] = I.sampled ? sample_branch(key0, key1, (uint32_t)m, step, belief, env.n_models) : m;
+    if (models_out)
+        for (int m = 0; m < M; ++m)
+            models_out[m] = th[m];
+    std::vector<float> eps_all((size_t)N * T * 2);
+    std::vector<S> fn(N), fr(N), sr(N);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads > 0 ? n_threads : 1)
+    for (int k = 0; k < N; ++k)
+    {
+        float* eps = &eps_all[(size_t)k * T * 2];
+        for (int t = 0; t < T; ++t)
+        {
+            const W4 w = philox(W4{{(uint32_t)k, (uint32_t)t, step, 0u}}, key0, key1);
+            float z0, z1;
+            normal_pair(w.v[0], w.v[1], &z0, &z1);
+            eps[t * 2] = I.sigma * z0;
+            eps[t * 2 + 1] = I.sigma * z1;
+        }
+        S c[2][DR_MAX_MODELS], sf[2][DR_MAX_MODELS];
+        for (int plan = 0; plan < 2; ++plan)
+            for (int m = 0; m < M; ++m)
+            {
+                prsr_rollout<S, float>(E, K, x0, th[m], plan ? rob : nom, eps, T, key0, key1, sample_stream(m, N, k), step, &c[plan][m],
+                                       &sf[plan][m]);
+                if (pair_out)
+                {
+                    double* o = pair_out + (((size_t)plan * M + m) * N + k) * 2;
+                    o[0] = (double)c[plan][m];
+                    o[1] = (double)sf[plan][m];
+                }
+            }
+        S full[2], smax[2];
+        for (int plan = 0; plan < 2; ++plan)
+        {
+            const S avg = prsr_avg<S>(I, belief, c[plan], M);
+            S mxs = -(S)INFINITY;
+            for (int m = 0; m < M; ++m)
+                mxs = mx(mxs, sf[plan][m]);
+            full[plan] = mxs > (S)0 ? avg + (S)I.safety_weight : avg;
+            smax[plan] = mxs;
+        }
+        fn[k] = full[0];
+        fr[k] = full[1];
+        sr[k] = smax[1];
+        if (agg_out)
+        {
+            agg_out[(size_t)k * 4 + 0] = (double)full[0];
+            agg_out[(size_t)k * 4 + 1] = (double)smax[0];
+            agg_out[(size_t)k * 4 + 2] = (double)full[1];
+            agg_out[(size_t)k * 4 + 3] = (double)smax[1];
+        }
+    }
+    if (noise_out)
+        memcpy(noise_out, eps_all.data(), eps_all.size() * sizeof(float));
+    const std::vector<S>* cs[3] = {&fn, &fr, &sr};
+    for (int j = 0; j < 3; ++j)
+    {
+        const std::vector<S>& c = *cs[j];
+        S mnc = (S)INFINITY;
+        for (int k = 0; k < N; ++k)
+            mnc = mn(mnc, c[k]);
+        const float* base = j == 0 ? nom : rob;
+        std::vector<double> num((size_t)T * 2, 0.0);
+        double den = 0.0;
+        for (int k = 0; k < N; ++k)
+        {
+            double w;
+            if (sizeof(S) == 4)
+                w = isfinite((double)mnc) ? (double)exp32(-((float)c[k] - (float)mnc) / lam[j]) : 0.0;
+            else
+                w = isfinite((double)mnc) ? exp(-((double)c[k] - (double)mnc) / (double)lam[j]) : 0.0;
+            const float* eps = &eps_all[(size_t)k * T * 2];
+            for (int i = 0; i < T * 2; ++i)
+                num[i] += w * (sizeof(S) == 4 ? (double)(base[i] + eps[i]) : (double)base[i] + (double)eps[i]);
+            den += w;
+        }
+        for (int i = 0; i < T * 2; ++i)
+        {
+            double v = (double)base[i];
+            if (den > 1e-30)
+                v = sizeof(S) == 4 ? (double)(float)(num[i] / den) : num[i] / den;
+            cand_out[(size_t)j * T * 2 + i] = v;
+        }
+        res_out[j] = (double)mnc;
+        res_out[3 + j] = sizeof(S) == 4 ? (double)(float)den : den;
+    }
+}
+
+template <class S>
+void prsr_choose(const dr_sr_env& env, const dr_sr_cost& K, const dr_sr_state& x0, const dr_prsr_instance& I, const float* belief,
+                 const int* models, const double* cand, uint32_t step, const float lam[3], const double nu[3], double* cc_out, double* cs_out,
+                 double* res_out)
+{
+    const int T = I.n_timesteps, M = prsr_models(I, env);
+    const uint32_t key0 = (uint32_t)(I.seed & 0xffffffffu), key1 = (uint32_t)(I.seed >> 32);
+    const Env<S> E(env);
+    std::vector<S> U((size_t)3 * T * 2);
+    for (size_t i = 0; i < U.size(); ++i)
+        U[i] = sizeof(S) == 4 ? (S)(float)cand[i] : (S)cand[i];
+    S cc[2][DR_MAX_MODELS], sf[2][DR_MAX_MODELS];
+    for (int which = 0; which < 2; ++which)
+        for (int m = 0; m < M; ++m)
+        {
+            S c;
+            prsr_rollout<S, S>(E, K, x0, models[m], &U[(size_t)which * T * 2], nullptr, T, key0, key1, cand_stream(which * M + m), step, &c,
+                               &sf[which][m]);
+            cc[which][m] = sf[which][m] > (S)0 ? c + (S)I.safety_weight : c;
+            cc_out[which * M + m] = (double)cc[which][m];
+            cs_out[which * M + m] = (double)sf[which][m];
+        }
+    const S c1 = prsr_avg<S>(I, belief, cc[0], M);
+    const S c2 = prsr_avg<S>(I, belief, cc[1], M);
+    const int pick = c1 <= c2 ? 0 : 1;
+    int use_nom = 1;
+    for (int m = 0; m < M; ++m)
+        if (sf[pick][m] > (S)0)
+            use_nom = 0;
+    res_out[0] = (double)c1;
+    res_out[1] = (double)c2;
+    res_out[2] = use_nom;
+    res_out[3] = c2 < c1 ? 1 : 0;
+    const S* act = use_nom ? &U[(size_t)pick * T * 2] : &U[(size_t)2 * T * 2];
+    res_out[4] = (double)act[0];
+    res_out[5] = (double)act[1];
+    for (int j = 0; j < 3; ++j)
+    {
+        float l = lam[j];
+        if (I.adapt_lambda)
+        {
+            const float v = (float)nu[j];
+            if (v > 0.001f * (float)I.n_samples && l > 0.01f)
+                l = l * 0.9f;
+            else if (v < 0.0005f * (float)I.n_samples && l < 100.0f)
+                l = l * 1.2f;
+        }
+        res_out[6 + j] = (double)l;
+    }
+}
+
 } // namespace
 
 extern "C" {
@@ -672,6 +905,42 @@ int or_sr_update(int f64, const double* costs, const float* noise, const float* 
             shifted[t * 2 + d] = (t + 1 < T) ? updated[(t + 1) * 2 + d] : (double)fill[d];
     *min_cost = mnc;
     *nu = f64 ? den : (double)(float)den;
+    return 0;
+}
+
+// PR-MPPI step of one StratRace2D controller from the shifted plans nom, rob
+// (T x 2): noise [k][t][2], pair costs [plan][m][k][2], aggregates [k][4],
+// models [M], candidates [3][T][2], res = min[3], nu[3].
+int or_prsr_step(int f64, const dr_sr_env* env, const dr_sr_cost* K, const dr_sr_state* x0, const dr_prsr_instance* inst, const float* belief,
+                 uint32_t step, const float* lam, const float* nom, const float* rob, int n_threads, float* noise_out, double* pair_out,
+                 double* agg_out, int* models_out, double* cand_out, double* res_out)
+{
+    if (!env || !K || !x0 || !inst || !belief || !lam || !nom || !rob || !cand_out || !res_out)
+        return 1;
+    const int M = prsr_models(*inst, *env);
+    if (M < 1 || M > DR_MAX_MODELS)
+        return 1;
+    if (f64)
+        prsr_step<double>(*env, *K, *x0, *inst, belief, step, lam, nom, rob, n_threads, noise_out, pair_out, agg_out, models_out, cand_out,
+                          res_out);
+    else
+        prsr_step<float>(*env, *K, *x0, *inst, belief, step, lam, nom, rob, n_threads, noise_out, pair_out, agg_out, models_out, cand_out,
+                         res_out);
+    return 0;
+}
+
+// Candidate costs and choice for candidates [3][T][2]: cc/cs [which][m];
+// res = cand cost 1, 2, use_nom, reset_nom, action[2], lambda[3].
+int or_prsr_choose(int f64, const dr_sr_env* env, const dr_sr_cost* K, const dr_sr_state* x0, const dr_prsr_instance* inst, const float* belief,
+                   const int* models, const double* cand, uint32_t step, const float* lam, const double* nu, double* cc_out, double* cs_out,
+                   double* res_out)
+{
+    if (!env || !K || !x0 || !inst || !belief || !models || !cand || !lam || !nu || !cc_out || !cs_out || !res_out)
+        return 1;
+    if (f64)
+        prsr_choose<double>(*env, *K, *x0, *inst, belief, models, cand, step, lam, nu, cc_out, cs_out, res_out);
+    else
+        prsr_choose<float>(*env, *K, *x0, *inst, belief, models, cand, step, lam, nu, cc_out, cs_out, res_out);
     return 0;
 }
 

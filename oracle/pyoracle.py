@@ -66,6 +66,10 @@ def lib(abi):
     L.or_sr_rollout.argtypes = [I, C.POINTER(abi.SrEnv), C.POINTER(abi.SrCost), C.POINTER(abi.SrState), I, F, I, I,
                                 C.c_uint64, U, C.c_float, I, D, F]
     L.or_sr_update.argtypes = [I, D, F, F, I, I, C.c_float, C.c_float, F, D, D, D, D, D]
+    L.or_prsr_step.argtypes = [I, C.POINTER(abi.SrEnv), C.POINTER(abi.SrCost), C.POINTER(abi.SrState), C.POINTER(abi.PrsrInstance), F,
+                               U, F, F, F, I, F, D, D, C.POINTER(I), D, D]
+    L.or_prsr_choose.argtypes = [I, C.POINTER(abi.SrEnv), C.POINTER(abi.SrCost), C.POINTER(abi.SrState), C.POINTER(abi.PrsrInstance),
+                                 F, C.POINTER(I), D, U, F, D, D, D, D]
     _lib = L
     return L
 
@@ -336,4 +340,42 @@ class Oracle:
                                  _d(sh), C.byref(mn), C.byref(nu))
         assert rc == 0
         return dict(updated=upd.reshape(T, 2), action=act, shifted=sh.reshape(T, 2), min_cost=mn.value, nu=nu.value)
+
+    def prsr_step(self, f64, env, cost, state, ins, belief, step, lam, nom, rob, threads=8, want_noise=True, want_pairs=True):
+        """One PR-MPPI step of a StratRace2D controller from the shifted plans nom, rob (T x 2)."""
+        K, T = ins.n_samples, ins.n_timesteps
+        M = ins.n_draws if ins.sampled else env.n_models
+        nom = np.ascontiguousarray(nom, dtype=np.float32).reshape(-1)
+        rob = np.ascontiguousarray(rob, dtype=np.float32).reshape(-1)
+        bel = np.zeros(self.abi.SR_MAX_MODELS, dtype=np.float32)
+        bel[:len(belief)] = belief
+        lam = np.ascontiguousarray(lam, dtype=np.float32)
+        noise = np.empty(K * T * 2, dtype=np.float32) if want_noise else None
+        pairs = np.empty(2 * M * K * 2) if want_pairs else None
+        agg = np.empty(K * 4)
+        models = (C.c_int * M)()
+        cand = np.empty(3 * T * 2)
+        res = np.empty(6)
+        rc = self.L.or_prsr_step(int(f64), C.byref(env), C.byref(cost), C.byref(state), C.byref(ins), _f(bel), step, _f(lam),
+                                 _f(nom), _f(rob), threads, _f(noise) if noise is not None else None,
+                                 _d(pairs) if pairs is not None else None, _d(agg), models, _d(cand), _d(res))
+        assert rc == 0
+        return dict(noise=noise.reshape(K, T, 2) if noise is not None else None,
+                    pairs=pairs.reshape(2, M, K, 2) if pairs is not None else None, agg=agg.reshape(K, 4),
+                    models=np.array(models[:], dtype=np.int32), cand=cand.reshape(3, T, 2), min=res[:3], nu=res[3:6])
+
+    def prsr_choose(self, f64, env, cost, state, ins, belief, models, cand, step, lam, nu):
+        M = ins.n_draws if ins.sampled else env.n_models
+        bel = np.zeros(self.abi.SR_MAX_MODELS, dtype=np.float32)
+        bel[:len(belief)] = belief
+        mdl = (C.c_int * M)(*[int(x) for x in models])
+        cand = np.ascontiguousarray(cand, dtype=np.float64).reshape(-1)
+        lam = np.ascontiguousarray(lam, dtype=np.float32)
+        nu = np.ascontiguousarray(nu, dtype=np.float64)
+        cc, cs, res = np.empty(2 * M), np.empty(2 * M), np.empty(9)
+        rc = self.L.or_prsr_choose(int(f64), C.byref(env), C.byref(cost), C.byref(state), C.byref(ins), _f(bel), mdl, _d(cand), step,
+                                   _f(lam), _d(nu), _d(cc), _d(cs), _d(res))
+        assert rc == 0
+        return dict(cand_cost=res[0:2], use_nom=int(res[2]), reset_nom=int(res[3]), action=res[4:6], lam=res[6:9],
+                    cc=cc.reshape(2, M), cs=cs.reshape(2, M))
 

@@ -196,6 +196,28 @@ def sr_race(pkg, steps=50, warmup=5, n_inst=1):
                                    "step": float(ms[3])}, "steps": steps}
 
 
+def pr_sr_race(pkg, steps=30, warmup=3):
+    """PR-MPPI on the reference's StratRace2D race at its PR setting (K = 2^16, T = 60,
+    P = 9 models drawn from the belief, temperature 10, sigma 0.5), closed loop, one world."""
+    sc = pkg.scenarios
+    scn = sc.pr_sr_race(n_inst=1)
+    eng = pkg.PRSREngine(scn)
+    eng.step(warmup, advance=True)
+    t0 = time.perf_counter()
+    eng.step(steps, advance=True)
+    elapsed = time.perf_counter() - t0
+    eng.set_timing(True)
+    eng.step(5, advance=True)
+    ms = eng.last_timing() / 5
+    eng.close()
+    ins = scn.instances[0]
+    rollouts = 2 * ins.n_draws * ins.n_samples
+    return {"workload": scn.name, "control_steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+            "rollouts_per_step": rollouts, "rollout_env_steps_per_s": rollouts * ins.n_timesteps * steps / elapsed,
+            "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate": float(ms[1]), "weighted_avg": float(ms[2]),
+                                   "update_choice_world": float(ms[3]), "step": float(ms[4])}, "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,7 +256,8 @@ def main():
     pkg = package()
     sc = pkg.scenarios
     if args.sr_only:
-        print(json.dumps({"sr_race": sr_race(pkg, steps=args.steps, warmup=args.warmup)}), flush=True)
+        print(json.dumps({"sr_race": sr_race(pkg, steps=args.steps, warmup=args.warmup),
+                          "pr_sr_race": pr_sr_race(pkg, steps=min(args.steps, 30), warmup=args.warmup)}), flush=True)
         return
     if args.pr_only:
         print(json.dumps({"pr_mppi": pr_mppi(pkg, steps=args.steps, warmup=args.warmup)}), flush=True)
@@ -328,6 +351,9 @@ def main():
         log("bench: StratRace2D plain MPPI, K=65536, T=70 ...")
         out["sr_race"] = sr_race(pkg)
         log(f"bench: sr_race {json.dumps(out['sr_race'])}")
+        log("bench: PR-MPPI on StratRace2D, K=65536, T=60, P=9 ...")
+        out["pr_sr_race"] = pr_sr_race(pkg)
+        log(f"bench: pr_sr_race {json.dumps(out['pr_sr_race'])}")
     if not args.no_batch64:
         log("bench: 64 uneven controllers, batched vs per-instance ...")
         out["batch64"] = batch_vs_per_instance(pkg)

@@ -9,6 +9,6 @@ scenario builders and the engine handles.  There is no CPU fallback.
 """
 from . import abi, philox, scenarios
 from .abi import EngineError, load
-from .engine import Engine, MPPI, PREngine, PRSREngine, SREngine
+from .engine import BMEngine, Engine, MPPI, PREngine, PRSREngine, SREngine
 
-__all__ = ["abi", "philox", "scenarios", "Engine", "MPPI", "PREngine", "PRSREngine", "SREngine", "EngineError", "load"]
+__all__ = ["abi", "philox", "scenarios", "BMEngine", "Engine", "MPPI", "PREngine", "PRSREngine", "SREngine", "EngineError", "load"]

@@ -125,6 +125,21 @@ class PrsrInstance(C.Structure):
                 ("true_theta", C.c_int32), ("_pad", C.c_int32)]
 
 
+MAX_KNOTS = 60
+
+
+class BmInstance(C.Structure):
+    _fields_ = [("n_samples", C.c_int32), ("n_timesteps", C.c_int32), ("n_knots", C.c_int32), ("knots", C.c_int32 * MAX_KNOTS),
+                ("seed", C.c_uint64), ("lambda_", C.c_float), ("sigma", C.c_float), ("min_confidence", C.c_float),
+                ("true_theta", C.c_int32), ("adapt_lambda", C.c_int32), ("init_belief", C.c_float * SR_MAX_MODELS),
+                ("_pad", C.c_int32)]
+
+
+class BmDiag(C.Structure):
+    _fields_ = [("lambda_", C.c_float), ("nu", C.c_float), ("min_cost", C.c_float), ("step", C.c_uint32),
+                ("branch", C.c_int32), ("_pad", C.c_int32)]
+
+
 # Every symbol include/dr_api.h declares, with its ctypes signature.
 _P = C.c_void_p
 _I = C.c_int
@@ -204,6 +219,27 @@ SIGNATURES = {
     "dr_pr_set_sr_states": (_I, [_P, C.POINTER(SrState), _I]),
     "dr_pr_get_sr_states": (_I, [_P, C.POINTER(SrState), _I]),
     "dr_pr_get_events": (_I, [_P, C.POINTER(C.c_int32), _I]),
+    "dr_bm_create": (_I, [C.POINTER(SrEnv), C.POINTER(SrCost), C.POINTER(BmInstance), _I, C.POINTER(_P)]),
+    "dr_bm_destroy": (None, [_P]),
+    "dr_bm_set_states": (_I, [_P, C.POINTER(SrState), _I]),
+    "dr_bm_get_states": (_I, [_P, C.POINTER(SrState), _I]),
+    "dr_bm_set_belief": (_I, [_P, _I, _F, _I]),
+    "dr_bm_get_belief": (_I, [_P, _I, _F, _I]),
+    "dr_bm_set_knots": (_I, [_P, _I, _F, _I]),
+    "dr_bm_get_knots": (_I, [_P, _I, _F, _I]),
+    "dr_bm_get_nominal": (_I, [_P, _I, _F, _I]),
+    "dr_bm_get_basis": (_I, [_P, _I, _F, _I]),
+    "dr_bm_step": (_I, [_P, _I, _I, _F]),
+    "dr_bm_get_events": (_I, [_P, C.POINTER(C.c_int32), _I]),
+    "dr_bm_get_noise": (_I, [_P, _I, _F, C.c_size_t]),
+    "dr_bm_get_costs_true": (_I, [_P, _I, _F, _I]),
+    "dr_bm_get_branches": (_I, [_P, _I, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _I]),
+    "dr_bm_get_costs": (_I, [_P, _I, _F, _I]),
+    "dr_bm_get_min_costs": (_I, [_P, _I, _F, _I]),
+    "dr_bm_get_nu": (_I, [_P, _I, _F, _I]),
+    "dr_bm_get_diag": (_I, [_P, C.POINTER(BmDiag), _I]),
+    "dr_bm_set_timing": (_I, [_P, _I]),
+    "dr_bm_last_timing": (_I, [_P, _F, _I]),
 }
 
 _lib = None

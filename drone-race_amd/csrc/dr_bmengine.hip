@@ -1,0 +1,636 @@
+// Host side of the branching spline MPPI engine on StratRace2D: the natural
+// cubic spline basis, descriptor tables, device buffers, the per-step
+// launch sequence, hipGraph capture and the dr_bm_* C ABI.
+//
+// Reference host orchestration this replaces:
+//   MPPIController::buildSplineMatrix      src/mppi_controller.cu:67-141
+//   MPPIController::getControl (no verification) src/mppi_controller.cu:250-466
+//   SimulationEngine::dynStep              src/engine.cpp:286-312
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dr_bmkernels.h"
+#include "dr_host.h"
+
+namespace dr {
+
+hipError_t launch_bm_rollout(int max_m, int max_tm, int grid, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmTile* tiles,
+                             int n_tiles, const dr_sr_env* env, const dr_sr_cost& KC, const dr_sr_state* states, const float* knots,
+                             const float* basis, float2* noise, float* ctrue, int* used, int* btime);
+hipError_t launch_bm_reduce(int n_inst, int max_nbk, int n_rtiles_min, int n_rtiles_avg, hipStream_t st, const BmInstDev* insts,
+                            BmDyn* dyn, const BmRTile* rt_min, const BmRTile* rt_avg, const float* ctrue, float* costs, const int* used,
+                            const int* btime, float* mins, const float2* noise, double* part, hipEvent_t mid);
+hipError_t launch_bm_update(int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const double* part, const float* basis,
+                            float* knots, float* dense, float* nu, float* actions, float max_accel);
+hipError_t launch_bm_advance(int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const dr_sr_env* env, dr_sr_state* states,
+                             const float* actions, int* events);
+
+// Natural cubic spline basis B[t][i] = spline through e_i at the knots,
+// evaluated at t = 0..T-1 with endpoint hold (mppi_controller.cu:67-141, the
+// same FP32 operations: tridiagonal system by the Thomas algorithm, then the
+// piecewise cubic).
+std::vector<float> spline_basis(const int* tau, int M, int T)
+{
+    std::vector<float> B((size_t)T * M, 0.0f);
+    std::vector<float> h(M - 1);
+    for (int j = 0; j < M - 1; ++j)
+        h[j] = (float)(tau[j + 1] - tau[j]);
+    for (int basis = 0; basis < M; ++basis)
+    {
+        std::vector<float> y(M, 0.0f);
+        y[basis] = 1.0f;
+        std::vector<float> m(M, 0.0f);
+        if (M > 2)
+        {
+            const int N = M - 2;
+            std::vector<float> a(N), b(N), c(N), d(N);
+            for (int i = 0; i < N; ++i)
+            {
+                const int j = i + 1;
+                a[i] = h[j - 1];
+                b[i] = 2.0f * (h[j - 1] + h[j]);
+                c[i] = h[j];
+                d[i] = 6.0f * ((y[j + 1] - y[j]) / h[j] - (y[j] - y[j - 1]) / h[j - 1]);
+            }
+            for (int i = 1; i < N; ++i)
+            {
+                const float w = a[i] / b[i - 1];
+                b[i] = b[i] - w * c[i - 1];
+                d[i] = d[i] - w * d[i - 1];
+            }
+            std::vector<float> x(N);
+            x[N - 1] = d[N - 1] / b[N - 1];
+            for (int i = N - 2; i >= 0; --i)
+                x[i] = (d[i] - c[i] * x[i + 1]) / b[i];
+            for (int i = 0; i < N; ++i)
+                m[i + 1] = x[i];
+        }
+        int seg = 0;
+        for (int t = 0; t < T; ++t)
+        {
+            float val;
+            if (t <= tau[0])
+                val = y[0];
+            else if (t >= tau[M - 1])
+                val = y[M - 1];
+            else
+            {
+                while (!(tau[seg] <= t && t <= tau[seg + 1]))
+                    ++seg;
+                const float x0 = (float)tau[seg], x1 = (float)tau[seg + 1];
+                const float hh = x1 - x0;
+                const float A = (x1 - (float)t) / hh;
+                const float BB = ((float)t - x0) / hh;
+                val = m[seg] * (A * A * A - A) * (hh * hh) / 6.0f + m[seg + 1] * (BB * BB * BB - BB) * (hh * hh) / 6.0f + y[seg] * A +
+                      y[seg + 1] * BB;
+            }
+            B[(size_t)t * M + basis] = val;
+        }
+    }
+    return B;
+}
+
+} // namespace dr
+
+using namespace dr;
+
+struct dr_bm_engine
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    dr_sr_env env{};
+    dr_sr_cost cost{};
+    std::vector<dr_bm_instance> inst;
+    std::vector<BmInstDev> idev;
+    std::vector<BmTile> tiles;
+    std::vector<BmRTile> rt_min, rt_avg;
+    int max_m = 0, max_tm = 0, max_nbk = 0;
+    long long noise_total = 0, part_total = 0;
+    int ct_total = 0, cost_total = 0, min_total = 0, knot_total = 0, dense_total = 0, B_total = 0, nu_total = 0;
+
+    DevBuf<BmInstDev> d_inst;
+    DevBuf<BmDyn> d_dyn;
+    DevBuf<BmTile> d_tiles;
+    DevBuf<BmRTile> d_rt_min, d_rt_avg;
+    DevBuf<dr_sr_env> d_env;
+    DevBuf<dr_sr_state> d_states;
+    DevBuf<float> d_knots, d_dense, d_B, d_ctrue, d_costs, d_mins, d_nu, d_actions;
+    DevBuf<int> d_used, d_btime, d_events;
+    DevBuf<float2> d_noise;
+    DevBuf<double> d_part;
+
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    int timed_steps = 0;
+    float last_ms[5] = {0, 0, 0, 0, 0};
+
+    ~dr_bm_engine()
+    {
+        for (auto& g : graph_exec)
+            if (g)
+                (void)hipGraphExecDestroy(g);
+        for (auto ev : events)
+            (void)hipEventDestroy(ev);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+    int enqueue_step(bool advance, int timed_index);
+    int run(int n_steps, bool advance);
+};
+
+int dr_bm_engine::enqueue_step(bool advance, int timed_index)
+{
+    const int n = (int)inst.size();
+    hipEvent_t* ev = (timed_index >= 0) ? &events[(size_t)timed_index * 5] : nullptr;
+    if (ev)
+        HIPCHK(hipEventRecord(ev[0], stream));
+    HIPCHK(launch_bm_rollout(max_m, max_tm, 0, stream, d_inst.p, d_dyn.p, d_tiles.p, (int)tiles.size(), d_env.p, cost, d_states.p, d_knots.p, d_B.p,
+                             d_noise.p, d_ctrue.p, d_used.p, d_btime.p));
+    if (ev)
+        HIPCHK(hipEventRecord(ev[1], stream));
+    HIPCHK(launch_bm_reduce(n, max_nbk, (int)rt_min.size(), (int)rt_avg.size(), stream, d_inst.p, d_dyn.p, d_rt_min.p, d_rt_avg.p, d_ctrue.p,
+                            d_costs.p, d_used.p, d_btime.p, d_mins.p, d_noise.p, d_part.p, ev ? ev[2] : nullptr));
+    if (ev)
+        HIPCHK(hipEventRecord(ev[3], stream));
+    HIPCHK(launch_bm_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_B.p, d_knots.p, d_dense.p, d_nu.p, d_actions.p, env.max_accel[0]));
+    if (advance)
+        HIPCHK(launch_bm_advance(n, stream, d_inst.p, d_dyn.p, d_env.p, d_states.p, d_actions.p, d_events.p));
+    if (ev)
+        HIPCHK(hipEventRecord(ev[4], stream));
+    return DR_OK;
+}
+
+int dr_bm_engine::run(int n_steps, bool advance)
+{
+    if (n_steps < 0)
+        return host_fail(DR_EINVAL, "n_steps < 0");
+    timed_steps = 0;
+    if (timing)
+    {
+        while (events.size() < (size_t)n_steps * 5)
+        {
+            hipEvent_t ev;
+            HIPCHK(hipEventCreate(&ev));
+            events.push_back(ev);
+        }
+        for (int s = 0; s < n_steps; ++s)
+            if (int rc = enqueue_step(advance, s))
+                return rc;
+        timed_steps = n_steps;
+        return DR_OK;
+    }
+    const int gi = advance ? 1 : 0;
+    if (!graph_exec[gi])
+    {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_step(advance, -1);
+        hipError_t ce = hipStreamEndCapture(stream, &g);
+        if (rc)
+        {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ce != hipSuccess)
+            return host_fail(DR_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        hipError_t ie = hipGraphInstantiate(&graph_exec[gi], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess)
+            return host_fail(DR_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+    for (int s = 0; s < n_steps; ++s)
+        HIPCHK(hipGraphLaunch(graph_exec[gi], stream));
+    return DR_OK;
+}
+
+namespace {
+
+int check_bm(dr_bm_engine* e, int i)
+{
+    if (!e)
+        return host_fail(DR_EINVAL, "NULL handle");
+    if (i < 0 || i >= (int)e->inst.size())
+        return host_fail(DR_EINVAL, "instance index out of range");
+    return DR_OK;
+}
+
+template <class T>
+int bm_out(dr_bm_engine* e, T* dst, const T* src, size_t n)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DR_OK;
+}
+
+template <class T>
+int bm_in(dr_bm_engine* e, T* dst, const T* src, size_t n)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DR_OK;
+}
+
+int get_dyn(dr_bm_engine* e, std::vector<BmDyn>& dyn)
+{
+    dyn.resize(e->inst.size());
+    return bm_out(e, dyn.data(), e->d_dyn.p, dyn.size());
+}
+
+} // namespace
+
+extern "C" {
+
+int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_instance* instances, int n_instances, dr_bm_engine** out)
+{
+    if (!out)
+        return host_fail(DR_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!env || !cost || !instances)
+        return host_fail(DR_EINVAL, "NULL argument");
+    if (n_instances < 1)
+        return host_fail(DR_EINVAL, "need at least one instance");
+    if (int rc = check_sr_env(*env))
+        return rc;
+    for (int i = 0; i < n_instances; ++i)
+    {
+        const dr_bm_instance& s = instances[i];
+        const std::string w = "instance " + std::to_string(i) + ": ";
+        if (s.n_samples < 1 || s.n_samples > (1 << 22))
+            return host_fail(DR_EINVAL, w + "n_samples must be in [1, 2^22]");
+        if (s.n_timesteps < 2 || s.n_timesteps > DR_MAX_T)
+            return host_fail(DR_EINVAL, w + "n_timesteps must be in [2, 256]");
+        if (s.n_knots < 2 || s.n_knots > DR_MAX_KNOTS || s.n_knots * s.n_timesteps > BM_MAX_TM)
+            return host_fail(DR_EINVAL, w + "n_knots must be in [2, 60] with n_knots * n_timesteps <= 8192");
+        if (s.knots[0] < 0 || s.knots[s.n_knots - 1] > s.n_timesteps - 1)
+            return host_fail(DR_EINVAL, w + "knots must lie in [0, T - 1]");
+        for (int m = 1; m < s.n_knots; ++m)
+            if (s.knots[m] <= s.knots[m - 1])
+                return host_fail(DR_EINVAL, w + "knots must be strictly increasing");
+        if (!(s.lambda > 0.0f) || !(s.sigma >= 0.0f) || !std::isfinite(s.sigma) || !std::isfinite(s.min_confidence))
+            return host_fail(DR_EINVAL, w + "lambda must be > 0, sigma finite and >= 0, min_confidence finite");
+        if (s.true_theta < 0 || s.true_theta >= env->n_models)
+            return host_fail(DR_EINVAL, w + "true_theta out of range");
+        for (int th = 0; th < env->n_models; ++th)
+            if (!(s.init_belief[th] >= 0.0f) || !std::isfinite(s.init_belief[th]))
+                return host_fail(DR_EINVAL, w + "belief entries must be finite and >= 0");
+    }
+    dr_bm_engine* e = new (std::nothrow) dr_bm_engine();
+    if (!e)
+        return host_fail(DR_ENOMEM, "host allocation failed");
+    auto bail = [&](int rc) {
+        delete e;
+        return rc;
+    };
+    if (hipGetDevice(&e->device) != hipSuccess)
+        return bail(host_fail(DR_EHIP, "no HIP device"));
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(host_fail(DR_EHIP, "stream creation failed"));
+    e->env = *env;
+    e->cost = *cost;
+    e->inst.assign(instances, instances + n_instances);
+    const int n = n_instances, NT = env->n_models, NB = NT + 1;
+    e->idev.assign(n, BmInstDev{});
+    std::vector<float> Ball;
+    for (int i = 0; i < n; ++i)
+    {
+        const dr_bm_instance& s = instances[i];
+        BmInstDev& I = e->idev[i];
+        I.K = s.n_samples;
+        I.T = s.n_timesteps;
+        I.M = s.n_knots;
+        I.NB = NB;
+        I.NT = NT;
+        for (int m = 0; m < DR_MAX_KNOTS; ++m)
+            I.knots[m] = m < s.n_knots ? s.knots[m] : 0;
+        I.key0 = (uint32_t)(s.seed & 0xffffffffu);
+        I.key1 = (uint32_t)(s.seed >> 32);
+        I.sigma = s.sigma;
+        I.min_conf = s.min_confidence;
+        I.true_theta = s.true_theta;
+        I.adapt = s.adapt_lambda;
+        I.noise_off = e->noise_total;
+        e->noise_total += (long long)NB * I.M * I.K;
+        I.ct_off = e->ct_total;
+        e->ct_total += NT * I.K;
+        I.cost_off = e->cost_total;
+        e->cost_total += NB * I.K;
+        I.min_off = e->min_total;
+        e->min_total += NB * I.M;
+        I.knot_off = e->knot_total;
+        e->knot_total += NB * I.M * 2;
+        I.dense_off = e->dense_total;
+        e->dense_total += NB * I.T * 2;
+        I.B_off = e->B_total;
+        e->B_total += I.T * I.M;
+        const std::vector<float> B = spline_basis(s.knots, I.M, I.T);
+        Ball.insert(Ball.end(), B.begin(), B.end());
+        I.n_chunks = (I.K + BM_CHUNK - 1) / BM_CHUNK;
+        I.part_off = e->part_total;
+        e->part_total += (long long)NB * I.M * I.n_chunks * 3;
+        I.nu_off = e->nu_total;
+        e->nu_total += NB;
+        e->max_m = std::max(e->max_m, I.M);
+        e->max_tm = std::max(e->max_tm, I.T * I.M);
+        e->max_nbk = std::max(e->max_nbk, NB * I.K);
+        for (int k0 = 0; k0 < I.K; k0 += BM_BLOCK)
+            for (int th = 0; th < NT; ++th)
+                e->tiles.push_back(BmTile{i, k0, th, 0});
+        for (int b = 0; b < NB; ++b)
+            for (int m = 0; m < I.M; ++m)
+            {
+                e->rt_min.push_back(BmRTile{i, b, m, 0});
+                for (int c = 0; c < I.n_chunks; ++c)
+                    e->rt_avg.push_back(BmRTile{i, b, m, c});
+            }
+    }
+    for (const float v : Ball)
+        if (!std::isfinite(v))
+            return bail(host_fail(DR_EINVAL, "spline basis not finite (check the knots)"));
+    auto chk = [&](hipError_t x, const char* what) {
+        if (x != hipSuccess)
+        {
+            host_fail(x == hipErrorOutOfMemory ? DR_ENOMEM : DR_EHIP, std::string(what) + ": " + hipGetErrorString(x));
+            return false;
+        }
+        return true;
+    };
+    if (!chk(e->d_inst.alloc(n), "alloc inst") || !chk(e->d_dyn.alloc(n), "alloc dyn") || !chk(e->d_tiles.alloc(e->tiles.size()), "alloc tiles") ||
+        !chk(e->d_rt_min.alloc(e->rt_min.size()), "alloc rt") || !chk(e->d_rt_avg.alloc(e->rt_avg.size()), "alloc rt") ||
+        !chk(e->d_env.alloc(1), "alloc env") || !chk(e->d_states.alloc(n), "alloc states") || !chk(e->d_knots.alloc(e->knot_total), "alloc knots") ||
+        !chk(e->d_dense.alloc(e->dense_total), "alloc dense") || !chk(e->d_B.alloc(e->B_total), "alloc B") ||
+        !chk(e->d_ctrue.alloc(e->ct_total), "alloc ctrue") || !chk(e->d_costs.alloc(e->cost_total), "alloc costs") ||
+        !chk(e->d_mins.alloc(e->min_total), "alloc mins") || !chk(e->d_nu.alloc(e->nu_total), "alloc nu") ||
+        !chk(e->d_actions.alloc((size_t)n * 2), "alloc actions") || !chk(e->d_used.alloc(e->ct_total), "alloc used") ||
+        !chk(e->d_btime.alloc(e->ct_total), "alloc btime") || !chk(e->d_events.alloc(n), "alloc events") ||
+        !chk(e->d_noise.alloc(e->noise_total), "alloc noise") || !chk(e->d_part.alloc(e->part_total), "alloc part"))
+        return bail(DR_ENOMEM);
+    std::vector<BmDyn> dyn(n);
+    for (int i = 0; i < n; ++i)
+    {
+        std::memset(&dyn[i], 0, sizeof(BmDyn));
+        dyn[i].lambda = instances[i].lambda;
+        for (int th = 0; th < NT; ++th)
+            dyn[i].belief[th] = instances[i].init_belief[th];
+    }
+    std::vector<float> zk(e->knot_total, 0.0f), zd(e->dense_total, 0.0f);
+    std::vector<dr_sr_state> st(n);
+    std::memset(st.data(), 0, st.size() * sizeof(dr_sr_state));
+    std::vector<int> ev(n, 0);
+    if (!chk(hipMemcpy(e->d_inst.p, e->idev.data(), n * sizeof(BmInstDev), hipMemcpyHostToDevice), "copy inst") ||
+        !chk(hipMemcpy(e->d_dyn.p, dyn.data(), n * sizeof(BmDyn), hipMemcpyHostToDevice), "copy dyn") ||
+        !chk(hipMemcpy(e->d_tiles.p, e->tiles.data(), e->tiles.size() * sizeof(BmTile), hipMemcpyHostToDevice), "copy tiles") ||
+        !chk(hipMemcpy(e->d_rt_min.p, e->rt_min.data(), e->rt_min.size() * sizeof(BmRTile), hipMemcpyHostToDevice), "copy rt") ||
+        !chk(hipMemcpy(e->d_rt_avg.p, e->rt_avg.data(), e->rt_avg.size() * sizeof(BmRTile), hipMemcpyHostToDevice), "copy rt") ||
+        !chk(hipMemcpy(e->d_env.p, env, sizeof(dr_sr_env), hipMemcpyHostToDevice), "copy env") ||
+        !chk(hipMemcpy(e->d_states.p, st.data(), n * sizeof(dr_sr_state), hipMemcpyHostToDevice), "copy states") ||
+        !chk(hipMemcpy(e->d_knots.p, zk.data(), zk.size() * sizeof(float), hipMemcpyHostToDevice), "copy knots") ||
+        !chk(hipMemcpy(e->d_dense.p, zd.data(), zd.size() * sizeof(float), hipMemcpyHostToDevice), "copy dense") ||
+        !chk(hipMemcpy(e->d_B.p, Ball.data(), Ball.size() * sizeof(float), hipMemcpyHostToDevice), "copy B") ||
+        !chk(hipMemcpy(e->d_events.p, ev.data(), n * sizeof(int), hipMemcpyHostToDevice), "copy events"))
+        return bail(DR_EHIP);
+    *out = e;
+    return DR_OK;
+}
+
+void dr_bm_destroy(dr_bm_engine* e)
+{
+    if (e)
+    {
+        (void)hipStreamSynchronize(e->stream);
+        delete e;
+    }
+}
+
+int dr_bm_set_states(dr_bm_engine* e, const dr_sr_state* states, int n)
+{
+    if (!e || !states || n != (int)e->inst.size())
+        return host_fail(DR_EINVAL, "set_states: bad handle, pointer or count");
+    if (int rc = bm_in(e, e->d_states.p, states, (size_t)n))
+        return rc;
+    std::vector<int> ev(n, 0);
+    return bm_in(e, e->d_events.p, ev.data(), ev.size());
+}
+
+int dr_bm_get_states(dr_bm_engine* e, dr_sr_state* states, int n)
+{
+    if (!e || !states || n != (int)e->inst.size())
+        return host_fail(DR_EINVAL, "get_states: bad handle, pointer or count");
+    return bm_out(e, states, e->d_states.p, (size_t)n);
+}
+
+int dr_bm_set_belief(dr_bm_engine* e, int instance, const float* belief, int n_models)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    if (!belief || n_models != e->env.n_models)
+        return host_fail(DR_EINVAL, "set_belief: expected n_models values");
+    for (int th = 0; th < n_models; ++th)
+        if (!(belief[th] >= 0.0f) || !std::isfinite(belief[th]))
+            return host_fail(DR_EINVAL, "set_belief: entries must be finite and >= 0");
+    float b[DR_SR_MAX_MODELS] = {0};
+    std::memcpy(b, belief, n_models * sizeof(float));
+    return bm_in(e, e->d_dyn.p[instance].belief, b, (size_t)DR_SR_MAX_MODELS);
+}
+
+int dr_bm_get_belief(dr_bm_engine* e, int instance, float* belief, int n_models)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    if (!belief || n_models != e->env.n_models)
+        return host_fail(DR_EINVAL, "get_belief: expected n_models values");
+    std::vector<BmDyn> dyn;
+    if (int rc = get_dyn(e, dyn))
+        return rc;
+    std::memcpy(belief, dyn[instance].belief, n_models * sizeof(float));
+    return DR_OK;
+}
+
+int dr_bm_set_knots(dr_bm_engine* e, int instance, const float* knots, int n_floats)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    const BmInstDev& I = e->idev[instance];
+    if (!knots || n_floats != I.NB * I.M * 2)
+        return host_fail(DR_EINVAL, "set_knots: expected (n_models + 1) * M * 2 floats");
+    return bm_in(e, e->d_knots.p + I.knot_off, knots, (size_t)n_floats);
+}
+
+int dr_bm_get_knots(dr_bm_engine* e, int instance, float* knots, int n_floats)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    const BmInstDev& I = e->idev[instance];
+    if (!knots || n_floats != I.NB * I.M * 2)
+        return host_fail(DR_EINVAL, "get_knots: expected (n_models + 1) * M * 2 floats");
+    return bm_out(e, knots, e->d_knots.p + I.knot_off, (size_t)n_floats);
+}
+
+int dr_bm_get_nominal(dr_bm_engine* e, int instance, float* nominal, int n_floats)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    const BmInstDev& I = e->idev[instance];
+    if (!nominal || n_floats != I.NB * I.T * 2)
+        return host_fail(DR_EINVAL, "get_nominal: expected (n_models + 1) * T * 2 floats");
+    return bm_out(e, nominal, e->d_dense.p + I.dense_off, (size_t)n_floats);
+}
+
+int dr_bm_get_basis(dr_bm_engine* e, int instance, float* B, int n_floats)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    const BmInstDev& I = e->idev[instance];
+    if (!B || n_floats != I.T * I.M)
+        return host_fail(DR_EINVAL, "get_basis: expected T * M floats");
+    return bm_out(e, B, e->d_B.p + I.B_off, (size_t)n_floats);
+}
+
+int dr_bm_step(dr_bm_engine* e, int n_steps, int advance, float* actions)
+{
+    if (!e)
+        return host_fail(DR_EINVAL, "NULL handle");
+    if (int rc = e->run(n_steps, advance != 0))
+        return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (e->timed_steps > 0)
+    {
+        float acc[5] = {0, 0, 0, 0, 0};
+        for (int s = 0; s < e->timed_steps; ++s)
+        {
+            hipEvent_t* ev = &e->events[(size_t)s * 5];
+            for (int q = 0; q < 4; ++q)
+            {
+                float v;
+                HIPCHK(hipEventElapsedTime(&v, ev[q], ev[q + 1]));
+                acc[q] += v;
+            }
+            float t;
+            HIPCHK(hipEventElapsedTime(&t, ev[0], ev[4]));
+            acc[4] += t;
+        }
+        std::memcpy(e->last_ms, acc, sizeof(acc));
+        e->timed_steps = 0;
+    }
+    if (actions)
+        return bm_out(e, actions, e->d_actions.p, e->inst.size() * 2);
+    return DR_OK;
+}
+
+int dr_bm_get_events(dr_bm_engine* e, int32_t* events, int n)
+{
+    if (!e || !events || n != (int)e->inst.size())
+        return host_fail(DR_EINVAL, "get_events: bad handle, pointer or count");
+    return bm_out(e, events, e->d_events.p, (size_t)n);
+}
+
+int dr_bm_get_noise(dr_bm_engine* e, int instance, float* noise, size_t n)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    const BmInstDev& I = e->idev[instance];
+    if (!noise || n != (size_t)I.NB * I.M * I.K * 2)
+        return host_fail(DR_EINVAL, "get_noise: expected (n_models + 1) * M * K * 2 floats");
+    return bm_out(e, reinterpret_cast<float2*>(noise), e->d_noise.p + I.noise_off, (size_t)I.NB * I.M * I.K);
+}
+
+int dr_bm_get_costs_true(dr_bm_engine* e, int instance, float* costs, int n)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    const BmInstDev& I = e->idev[instance];
+    if (!costs || n != I.NT * I.K)
+        return host_fail(DR_EINVAL, "get_costs_true: expected n_models * K floats");
+    return bm_out(e, costs, e->d_ctrue.p + I.ct_off, (size_t)n);
+}
+
+int dr_bm_get_branches(dr_bm_engine* e, int instance, int32_t* used, int32_t* time, int n)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    const BmInstDev& I = e->idev[instance];
+    if (!used || !time || n != I.NT * I.K)
+        return host_fail(DR_EINVAL, "get_branches: expected n_models * K values each");
+    if (int rc = bm_out(e, used, e->d_used.p + I.ct_off, (size_t)n))
+        return rc;
+    return bm_out(e, time, e->d_btime.p + I.ct_off, (size_t)n);
+}
+
+int dr_bm_get_costs(dr_bm_engine* e, int instance, float* costs, int n)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    const BmInstDev& I = e->idev[instance];
+    if (!costs || n != I.NB * I.K)
+        return host_fail(DR_EINVAL, "get_costs: expected (n_models + 1) * K floats");
+    return bm_out(e, costs, e->d_costs.p + I.cost_off, (size_t)n);
+}
+
+int dr_bm_get_min_costs(dr_bm_engine* e, int instance, float* mins, int n)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    const BmInstDev& I = e->idev[instance];
+    if (!mins || n != I.NB * I.M)
+        return host_fail(DR_EINVAL, "get_min_costs: expected (n_models + 1) * M floats");
+    return bm_out(e, mins, e->d_mins.p + I.min_off, (size_t)n);
+}
+
+int dr_bm_get_nu(dr_bm_engine* e, int instance, float* nu, int n)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    const BmInstDev& I = e->idev[instance];
+    if (!nu || n != I.NB)
+        return host_fail(DR_EINVAL, "get_nu: expected n_models + 1 floats");
+    return bm_out(e, nu, e->d_nu.p + I.nu_off, (size_t)n);
+}
+
+int dr_bm_get_diag(dr_bm_engine* e, dr_bm_diag* diag, int n)
+{
+    if (!e || !diag || n != (int)e->inst.size())
+        return host_fail(DR_EINVAL, "get_diag: bad handle, pointer or count");
+    std::vector<BmDyn> dyn;
+    if (int rc = get_dyn(e, dyn))
+        return rc;
+    std::vector<float> nu(e->nu_total), mins(e->min_total);
+    if (int rc = bm_out(e, nu.data(), e->d_nu.p, nu.size()))
+        return rc;
+    if (int rc = bm_out(e, mins.data(), e->d_mins.p, mins.size()))
+        return rc;
+    for (int i = 0; i < n; ++i)
+    {
+        const BmInstDev& I = e->idev[i];
+        std::memset(&diag[i], 0, sizeof(dr_bm_diag));
+        diag[i].lambda = dyn[i].lambda;
+        diag[i].step = dyn[i].step;
+        diag[i].branch = dyn[i].branch;
+        diag[i].nu = nu[I.nu_off + dyn[i].branch];
+        diag[i].min_cost = mins[I.min_off + dyn[i].branch * I.M];
+    }
+    return DR_OK;
+}
+
+int dr_bm_set_timing(dr_bm_engine* e, int on)
+{
+    if (!e)
+        return host_fail(DR_EINVAL, "NULL handle");
+    e->timing = on != 0;
+    return DR_OK;
+}
+
+int dr_bm_last_timing(dr_bm_engine* e, float* ms, int n)
+{
+    if (!e || !ms || n != 5)
+        return host_fail(DR_EINVAL, "last_timing: expected 5 floats");
+    std::memcpy(ms, e->last_ms, sizeof(e->last_ms));
+    return DR_OK;
+}
+
+} // extern "C"

@@ -373,17 +373,79 @@ DR_HD bool is_opp_winner(const dr_sr_env& E, const St& s)
     return s.laps[1] >= E.n_win_laps;
 }
 
-// environmentStep<false, false, false, lose_on_opp_win> with applyNoise = true.
-DR_HD int env_step(const Tables& tb, const dr_sr_env& E, int theta, const float ego[2], St& s, NormalSeq& rng, bool lose_on_opp_win)
+// updateBelief (env_stratrace.h:166-222) for one opponent: the observed
+// opponent action against each model's nominal action (scaled to the
+// opponent's max acceleration), Gaussian likelihood with variance
+// 0.2^2 + actionNoise^2, product and normalisation (uniform below 1e-20).
+// log and exp are the build's (dr_rng.h).
+DR_HD void update_belief(const dr_sr_env& E, float* belief, const float obs[2], const float (*nom)[2])
+{
+    const float sigmaEnv = 0.2f;
+    const float log2pi = logf_ieee(2.0f * 3.14159265358979323846f);
+    float sum = 0.0f;
+    for (int th = 0; th < E.n_models; ++th)
+    {
+        const float sqA = nom[th][0] * nom[th][0] + nom[th][1] * nom[th][1];
+        float scale = 1.0f;
+        if (sqA > E.max_accel[1] * E.max_accel[1])
+            scale = E.max_accel[1] / __builtin_sqrtf(sqA);
+        const float dx = obs[0] - nom[th][0] * scale;
+        const float dy = obs[1] - nom[th][1] * scale;
+        const float sqDist = dx * dx + dy * dy;
+        const float an = E.opp[th].action_noise;
+        const float sqSigma = sigmaEnv * sigmaEnv + an * an;
+        const float ll = -0.5f * sqDist / sqSigma - log2pi - logf_ieee(sqSigma);
+        belief[th] = belief[th] * expf_ieee(ll);
+        sum = sum + belief[th];
+    }
+    if (sum < 1e-20f)
+    {
+        const float uniform = 1.0f / (float)E.n_models;
+        for (int th = 0; th < E.n_models; ++th)
+            belief[th] = uniform;
+    }
+    else
+        for (int th = 0; th < E.n_models; ++th)
+            belief[th] = belief[th] / sum;
+}
+
+// findConfident (state.h:93-115) for one model factor: theta + 1 for the
+// first model whose belief exceeds the threshold, else 0.
+DR_HD int find_confident(const float* belief, int n_models, float threshold)
+{
+    for (int th = 0; th < n_models; ++th)
+        if (belief[th] > threshold)
+            return th + 1;
+    return 0;
+}
+
+// environmentStep<BELIEF, false, false, lose_on_opp_win> with applyNoise =
+// true.  With BELIEF the nominal opponent action of every model is formed
+// and the belief is updated from the opponent's applied action.
+template <bool BELIEF>
+DR_HD int env_step_t(const Tables& tb, const dr_sr_env& E, int theta, const float ego[2], St& s, NormalSeq& rng, bool lose_on_opp_win,
+                     float* belief)
 {
     const float prevS0 = s.S[0], prevS1 = s.S[1];
     float act[4];
     act[0] = ego[0];
     act[1] = ego[1];
+    float noms[DR_SR_MAX_MODELS][2];
+    if (BELIEF)
+    {
+        for (int th = 0; th < E.n_models; ++th)
+            opp_action(tb, E, E.opp[th], s, noms[th]);
+    }
     {
         const dr_sr_opp_config& cfg = E.opp[theta];
         float nom[2];
-        opp_action(tb, E, cfg, s, nom);
+        if (BELIEF)
+        {
+            nom[0] = noms[theta][0];
+            nom[1] = noms[theta][1];
+        }
+        else
+            opp_action(tb, E, cfg, s, nom);
 #pragma unroll
         for (int d = 0; d < 2; ++d)
         {
@@ -455,6 +517,11 @@ DR_HD int env_step(const Tables& tb, const dr_sr_env& E, int theta, const float 
         if (s.S[a] > prev[a] + tb.L * 0.5f)
             s.laps[a] = s.laps[a] - 1;
     }
+    if (BELIEF)
+    {
+        const float obs[2] = {act[2], act[3]};
+        update_belief(E, belief, obs, noms);
+    }
     if (boundary_dist(E, s, lose_on_opp_win) < E.drone_radius)
         return SR_LOSE;
     if (is_winner(E, s))
@@ -462,6 +529,12 @@ DR_HD int env_step(const Tables& tb, const dr_sr_env& E, int theta, const float 
     if (!lose_on_opp_win && boundary_dist(E, s, true) < E.drone_radius)
         return SR_OPP_WIN;
     return SR_NONE;
+}
+
+
+DR_HD int env_step(const Tables& tb, const dr_sr_env& E, int theta, const float ego[2], St& s, NormalSeq& rng, bool lose_on_opp_win)
+{
+    return env_step_t<false>(tb, E, theta, ego, s, rng, lose_on_opp_win, nullptr);
 }
 
 // stateCost (costs.cuh:7-28).

@@ -482,3 +482,136 @@ class PRSREngine(PREngine):
         b = np.empty(T * 2, dtype=np.float32)
         check(self.lib.dr_pr_get_candidates(self.h, i, _fptr(a), _fptr(b), a.size))
         return a.reshape(T, 2), b.reshape(T, 2)
+
+
+class BMEngine:
+    """Branching spline MPPI controllers on StratRace2D worlds (the reference's default
+    MPPIController with splines, without plan verification), one world each."""
+
+    def __init__(self, scenario):
+        lib = abi.load()
+        self.lib = lib
+        self.sc = scenario
+        n = len(scenario.instances)
+        self._inst = (abi.BmInstance * n)(*scenario.instances)
+        self.n_inst = n
+        h = C.c_void_p()
+        check(lib.dr_bm_create(C.byref(scenario.env), C.byref(scenario.cost), self._inst, n, C.byref(h)))
+        self.h = h
+        self.set_states(scenario.states)
+
+    def close(self):
+        if self.h:
+            self.lib.dr_bm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _dims(self, i):
+        ins = self.sc.instances[i]
+        return ins.n_samples, ins.n_timesteps, ins.n_knots, self.sc.env.n_models
+
+    def set_states(self, states):
+        arr = (abi.SrState * self.n_inst)(*states)
+        check(self.lib.dr_bm_set_states(self.h, arr, self.n_inst))
+
+    def get_states(self):
+        arr = (abi.SrState * self.n_inst)()
+        check(self.lib.dr_bm_get_states(self.h, arr, self.n_inst))
+        return arr
+
+    def set_belief(self, i, belief):
+        b = np.ascontiguousarray(belief, dtype=np.float32)
+        check(self.lib.dr_bm_set_belief(self.h, i, _fptr(b), b.size))
+
+    def belief(self, i):
+        b = np.empty(self.sc.env.n_models, dtype=np.float32)
+        check(self.lib.dr_bm_get_belief(self.h, i, _fptr(b), b.size))
+        return b
+
+    def set_knots(self, i, knots):
+        a = np.ascontiguousarray(knots, dtype=np.float32).reshape(-1)
+        check(self.lib.dr_bm_set_knots(self.h, i, _fptr(a), a.size))
+
+    def knots(self, i):
+        K, T, M, NT = self._dims(i)
+        a = np.empty((NT + 1) * M * 2, dtype=np.float32)
+        check(self.lib.dr_bm_get_knots(self.h, i, _fptr(a), a.size))
+        return a.reshape(NT + 1, M, 2)
+
+    def nominal(self, i):
+        K, T, M, NT = self._dims(i)
+        a = np.empty((NT + 1) * T * 2, dtype=np.float32)
+        check(self.lib.dr_bm_get_nominal(self.h, i, _fptr(a), a.size))
+        return a.reshape(NT + 1, T, 2)
+
+    def basis(self, i):
+        K, T, M, NT = self._dims(i)
+        a = np.empty(T * M, dtype=np.float32)
+        check(self.lib.dr_bm_get_basis(self.h, i, _fptr(a), a.size))
+        return a.reshape(T, M)
+
+    def step(self, n_steps=1, advance=False):
+        a = np.empty(self.n_inst * 2, dtype=np.float32)
+        check(self.lib.dr_bm_step(self.h, n_steps, 1 if advance else 0, _fptr(a)))
+        return a.reshape(self.n_inst, 2)
+
+    def events(self):
+        a = (C.c_int32 * self.n_inst)()
+        check(self.lib.dr_bm_get_events(self.h, a, self.n_inst))
+        return np.array(a[:], dtype=np.int32)
+
+    def noise(self, i):
+        K, T, M, NT = self._dims(i)
+        a = np.empty((NT + 1) * M * K * 2, dtype=np.float32)
+        check(self.lib.dr_bm_get_noise(self.h, i, _fptr(a), a.size))
+        return a.reshape(NT + 1, M, K, 2)
+
+    def costs_true(self, i):
+        K, T, M, NT = self._dims(i)
+        a = np.empty(NT * K, dtype=np.float32)
+        check(self.lib.dr_bm_get_costs_true(self.h, i, _fptr(a), a.size))
+        return a.reshape(NT, K)
+
+    def branches(self, i):
+        K, T, M, NT = self._dims(i)
+        u = np.empty(NT * K, dtype=np.int32)
+        t = np.empty(NT * K, dtype=np.int32)
+        ip = C.POINTER(C.c_int32)
+        check(self.lib.dr_bm_get_branches(self.h, i, u.ctypes.data_as(ip), t.ctypes.data_as(ip), u.size))
+        return u.reshape(NT, K), t.reshape(NT, K)
+
+    def costs(self, i):
+        K, T, M, NT = self._dims(i)
+        a = np.empty((NT + 1) * K, dtype=np.float32)
+        check(self.lib.dr_bm_get_costs(self.h, i, _fptr(a), a.size))
+        return a.reshape(NT + 1, K)
+
+    def min_costs(self, i):
+        K, T, M, NT = self._dims(i)
+        a = np.empty((NT + 1) * M, dtype=np.float32)
+        check(self.lib.dr_bm_get_min_costs(self.h, i, _fptr(a), a.size))
+        return a.reshape(NT + 1, M)
+
+    def nu(self, i):
+        a = np.empty(self.sc.env.n_models + 1, dtype=np.float32)
+        check(self.lib.dr_bm_get_nu(self.h, i, _fptr(a), a.size))
+        return a
+
+    def diag(self):
+        arr = (abi.BmDiag * self.n_inst)()
+        check(self.lib.dr_bm_get_diag(self.h, arr, self.n_inst))
+        return arr
+
+    def set_timing(self, on=True):
+        check(self.lib.dr_bm_set_timing(self.h, 1 if on else 0))
+
+    def last_timing(self):
+        """Device ms summed over the last timed run: rollout, aggregation + minima, weighted averages, update + world, step."""
+        a = np.empty(5, dtype=np.float32)
+        check(self.lib.dr_bm_last_timing(self.h, _fptr(a), 5))
+        return a

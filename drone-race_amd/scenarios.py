@@ -375,6 +375,45 @@ def pr_sr_race(n_inst=1, n_samples=65536, n_timesteps=60, seed=42, sampled=1, de
                       f"PR-MPPI on StratRace2D, {n_inst} worlds, K={n_samples}, T={n_timesteps}, P={P if sampled else env.n_models}")
 
 
+def bm_knots(n_timesteps, n_knots):
+    """Default knot times of MPPIConfig (python-src/utils.py:496-497): tau_m = m * floor((T - 2) / (M - 1))."""
+    return [m * ((n_timesteps - 2) // (n_knots - 1)) for m in range(n_knots)]
+
+
+def bm_instance(n_samples, n_timesteps, seed, n_knots=20, knots=None, lam=10.0, sigma=0.5, min_confidence=0.95, true_theta=0,
+                adapt=1, belief=(0.5, 0.5)):
+    tau = bm_knots(n_timesteps, n_knots) if knots is None else list(knots)
+    ins = abi.BmInstance(n_samples=n_samples, n_timesteps=n_timesteps, n_knots=len(tau), seed=seed, lambda_=lam, sigma=sigma,
+                         min_confidence=min_confidence, true_theta=true_theta, adapt_lambda=adapt)
+    for m, t in enumerate(tau):
+        ins.knots[m] = int(t)
+    for b, p in enumerate(belief):
+        ins.init_belief[b] = float(p)
+    return ins
+
+
+def bm_cost():
+    """The spline MPPIConfig of stratRaceEnv (python-src/main.py:944-968): collision factor 1.4,
+    boundary cost 50 (factor 1.8), outside 1e7, win 1e5, opponent win 1e5, terminal advance weight 50."""
+    return abi.SrCost(coll_dist_factor=1.4, boundary_cost=50.0, boundary_threshold_factor=1.8, outside_cost=1e7,
+                      win_cost=1e5, opp_win_cost=1e5, final_adv_weight=50.0, final_opp_adv_weight=0.0)
+
+
+def bm_race(n_inst=1, n_samples=65536, n_timesteps=80, n_knots=20, seed=42, steps=300, n_models=2, **env_kw):
+    """The reference's default branching spline MPPI race on StratRace2D (stratRaceEnv with
+    splines, main.py:944-968): K = 2^16, T = 80, 20 knots, temperature 10 (adapted), sigma 0.5,
+    confidence 0.95, two opponent models with a uniform initial belief; world i follows model
+    i % n_models."""
+    env = sr_env(n_models=n_models, **env_kw)
+    belief = [1.0 / n_models] * n_models
+    inst = [bm_instance(n_samples, n_timesteps, seed_mix(seed, 500 + i), n_knots=n_knots, true_theta=i % n_models, belief=belief)
+            for i in range(n_inst)]
+    st = [sr_start_state(env) for _ in range(n_inst)]
+    return SrScenario(f"bm_race_{n_inst}x_k{n_samples}_t{n_timesteps}_m{n_knots}", env, bm_cost(), inst, st, steps,
+                      f"branching spline MPPI on StratRace2D, {n_inst} worlds, K={n_samples}, T={n_timesteps}, {n_knots} knots, "
+                      f"{n_models} opponent models")
+
+
 def seed_mix(seed, i):
     """64-bit Philox key for instance i of a scenario seeded `seed`."""
     return ((int(seed) * 0x9E3779B97F4A7C15) ^ (0xA5A5A5A5 + 0x632BE59BD9B4E019 * (i + 1))) & 0xFFFFFFFFFFFFFFFF

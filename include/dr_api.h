@@ -445,6 +445,77 @@ int dr_pr_get_sr_states(dr_prengine* e, dr_sr_state* states, int n);
 /* Last world event per instance (0 none, 1 ego outside, 2 ego wins, 3 opponent wins). */
 int dr_pr_get_events(dr_prengine* e, int32_t* events, int n);
 
+/* ---------------------------------------------------------------------
+ * Branching MPPI on StratRace2D: the reference's default MPPIController
+ * (USE_SPLINES, src/kernels.cu:34-486, src/mppi_controller.cu:56-141,
+ * 250-466) without plan verification.  Plans: n_models + 1 branch plans
+ * (0 = nominal, theta + 1 = specialised to model theta), each M spline
+ * knots x 2; rollouts follow every true model compatible with the
+ * committed branch, with belief updates and branching in the rollout;
+ * masked minima and belief-weighted averages per (plan, knot); natural
+ * cubic spline interpolation, L2 clamp of the knots, spline shift of the
+ * chosen branch. */
+#define DR_MAX_KNOTS 60
+
+typedef struct dr_bm_instance
+{
+    int32_t n_samples;
+    int32_t n_timesteps;
+    int32_t n_knots;                 /* M in 2..DR_MAX_KNOTS */
+    int32_t knots[DR_MAX_KNOTS];     /* tau_m: increasing, tau_0 >= 0, tau_{M-1} <= T - 1 */
+    uint64_t seed;
+    float lambda;                    /* invTemperature */
+    float sigma;                     /* samplingNoise */
+    float min_confidence;            /* belief above which a branch commits */
+    int32_t true_theta;              /* opponent model of the instance's world */
+    int32_t adapt_lambda;
+    float init_belief[DR_SR_MAX_MODELS];
+    int32_t _pad;
+} dr_bm_instance;
+
+typedef struct dr_bm_engine dr_bm_engine;
+
+int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_instance* instances, int n_instances, dr_bm_engine** out);
+void dr_bm_destroy(dr_bm_engine* e);
+int dr_bm_set_states(dr_bm_engine* e, const dr_sr_state* states, int n);
+int dr_bm_get_states(dr_bm_engine* e, dr_sr_state* states, int n);
+/* The controller's belief (h_belief), n_models values per instance. */
+int dr_bm_set_belief(dr_bm_engine* e, int instance, const float* belief, int n_models);
+int dr_bm_get_belief(dr_bm_engine* e, int instance, float* belief, int n_models);
+/* Spline knots of every branch plan, [n_models + 1][M][2]. */
+int dr_bm_set_knots(dr_bm_engine* e, int instance, const float* knots, int n_floats);
+int dr_bm_get_knots(dr_bm_engine* e, int instance, float* knots, int n_floats);
+/* Dense plans after the last step ([n_models + 1][T][2]; the chosen branch shifted). */
+int dr_bm_get_nominal(dr_bm_engine* e, int instance, float* nominal, int n_floats);
+/* The natural cubic spline basis B ([T][M]), mppi_controller.cu:67-141. */
+int dr_bm_get_basis(dr_bm_engine* e, int instance, float* B, int n_floats);
+int dr_bm_step(dr_bm_engine* e, int n_steps, int advance, float* actions /* n x 2, may be NULL */);
+int dr_bm_get_events(dr_bm_engine* e, int32_t* events, int n);
+/* Results of the last step: noise [n_models + 1][M][K][2]; per-model
+ * costs [n_models][K] (+inf where the model was skipped); branch used and
+ * branching time per model [n_models][K]; aggregated costs [n_models + 1][K];
+ * masked minima [n_models + 1][M]; nu [n_models + 1]. */
+int dr_bm_get_noise(dr_bm_engine* e, int instance, float* noise, size_t n);
+int dr_bm_get_costs_true(dr_bm_engine* e, int instance, float* costs, int n);
+int dr_bm_get_branches(dr_bm_engine* e, int instance, int32_t* used, int32_t* time, int n);
+int dr_bm_get_costs(dr_bm_engine* e, int instance, float* costs, int n);
+int dr_bm_get_min_costs(dr_bm_engine* e, int instance, float* mins, int n);
+int dr_bm_get_nu(dr_bm_engine* e, int instance, float* nu, int n);
+typedef struct dr_bm_diag
+{
+    float lambda;    /* temperature of the next step */
+    float nu;        /* nu of the chosen branch in the last step */
+    float min_cost;  /* masked minimum of the chosen branch at knot 0 */
+    uint32_t step;   /* control steps taken */
+    int32_t branch;  /* branch of the last action: 0 nominal, theta + 1 committed */
+    int32_t _pad;
+} dr_bm_diag;
+/* Per-instance diagnostics (what getControl prints, mppi_controller.cu:394-424). */
+int dr_bm_get_diag(dr_bm_engine* e, dr_bm_diag* diag, int n);
+int dr_bm_set_timing(dr_bm_engine* e, int enable);
+/* Device ms summed over the last timed run: rollout, aggregation + masked minima, weighted averages, update + world, step. */
+int dr_bm_last_timing(dr_bm_engine* e, float* ms, int n);
+
 /* Single-controller entry points with the reference's shape. */
 typedef struct dr_mppi_cfg
 {

@@ -10,7 +10,13 @@
 //   include/environments/env_stratrace.h:302-315   computeSdot
 //   include/environments/env_stratrace.h:318-461   computeOppTargetLatDist (N_OPP = 1)
 //   include/environments/env_stratrace.h:577-706   computeOpponentAction
-//   include/environments/env_stratrace.h:711-894   environmentStep (no belief update / branching)
+//   include/environments/env_stratrace.h:166-222   updateBelief
+//   include/environments/env_stratrace.h:711-894   environmentStep (belief update; branching in the caller)
+//   include/state.h:93-115, :154-228               findConfident, initBranchState, sampleContributed,
+//                                                   splineSampleContributed
+//   src/mppi_controller.cu:67-141                   buildSplineMatrix
+//   src/kernels.cu:34-486, src/mppi_controller.cu:250-466
+//                                                   branching spline MPPI step (no plan verification)
 //   include/costs.cuh:7-39                          stateCost, finalCost
 //   src/kernels.cu:100-155 (dense)                  rollout loop
 //   src/kernels.cu:307-408, :434-456                weighted update, L2 clamp of the nominal
@@ -387,14 +393,70 @@ bool winner(const Env<S>& E, const SState<S>& s)
     return ab(s.lat[1]) > E.tw - E.r;
 }
 
-// environmentStep<false, false, false, lose>(applyNoise = true); returns
-// 0 none, 1 lose, 2 win, 3 opponent wins.
+inline float lg(float x) { return log32(x); }
+inline double lg(double x) { return log(x); }
+
+// updateBelief (env_stratrace.h:166-222), one opponent: Gaussian likelihood
+// of the applied opponent action `obs` under each model's nominal action
+// (scaled to the opponent's max acceleration), variance 0.2^2 + noise^2;
+// product, then normalisation (uniform when the sum is below 1e-20).
 template <class S>
-int env_step(const Env<S>& E, int theta, const S ego[2], SState<S>& s, Normals& rng, bool lose)
+void update_belief(const Env<S>& E, S* belief, const S obs[2], const S (*nom)[2])
+{
+    const S sigmaEnv = (S)0.2f;
+    const S log2pi = lg((S)2.0f * (S)3.14159265358979323846f);
+    S sum = 0;
+    const int NT = E.e.n_models;
+    for (int th = 0; th < NT; ++th)
+    {
+        const S sqA = nom[th][0] * nom[th][0] + nom[th][1] * nom[th][1];
+        S scale = 1;
+        if (sqA > E.max_accel[1] * E.max_accel[1])
+            scale = E.max_accel[1] / sq_root(sqA);
+        const S dx = obs[0] - nom[th][0] * scale;
+        const S dy = obs[1] - nom[th][1] * scale;
+        const S sqDist = dx * dx + dy * dy;
+        const S an = (S)E.e.opp[th].action_noise;
+        const S sqSigma = sigmaEnv * sigmaEnv + an * an;
+        const S ll = (S)-0.5f * sqDist / sqSigma - log2pi - lg(sqSigma);
+        belief[th] = belief[th] * ex(ll);
+        sum = sum + belief[th];
+    }
+    if (sum < (S)1e-20f)
+    {
+        const S uniform = (S)1 / (S)NT;
+        for (int th = 0; th < NT; ++th)
+            belief[th] = uniform;
+    }
+    else
+        for (int th = 0; th < NT; ++th)
+            belief[th] = belief[th] / sum;
+}
+
+// findConfident (state.h:93-115), one model factor: theta + 1 of the first
+// model whose belief exceeds the threshold, else 0.
+template <class S>
+int find_confident(const S* belief, int NT, float threshold)
+{
+    for (int th = 0; th < NT; ++th)
+        if (belief[th] > (S)threshold)
+            return th + 1;
+    return 0;
+}
+
+// environmentStep<belief != NULL, false, false, lose>(applyNoise = true);
+// returns 0 none, 1 lose, 2 win, 3 opponent wins.  With a belief, it is
+// updated from the applied opponent action (env_stratrace.h:856-859).
+template <class S>
+int env_step(const Env<S>& E, int theta, const S ego[2], SState<S>& s, Normals& rng, bool lose, S* belief = nullptr)
 {
     const S prevS[2] = {s.Sa[0], s.Sa[1]};
     S act[4] = {ego[0], ego[1], 0, 0};
     const dr_sr_opp_config& c = E.e.opp[theta];
+    S noms[DR_SR_MAX_MODELS][2];
+    if (belief)
+        for (int th = 0; th < E.e.n_models; ++th)
+            opp_action(E, E.e.opp[th], s, noms[th]);
     S nom[2];
     opp_action(E, c, s, nom);
     for (int d = 0; d < 2; ++d)
@@ -457,6 +519,11 @@ int env_step(const Env<S>& E, int theta, const S ego[2], SState<S>& s, Normals& 
             s.laps[a] = s.laps[a] + 1;
         if (s.Sa[a] > prevS[a] + E.L * (S)0.5f)
             s.laps[a] = s.laps[a] - 1;
+    }
+    if (belief)
+    {
+        const S obs[2] = {act[2], act[3]};
+        update_belief(E, belief, obs, noms);
     }
     if (bdist(E, s, lose) < E.r)
         return 1;
@@ -758,6 +825,368 @@ void prsr_choose(const dr_sr_env& env, const dr_sr_cost& K, const dr_sr_state& x
     }
 }
 
+
+// ------------------------------------------------------------ branching spline MPPI
+
+// Natural cubic spline basis B[t][i] (buildSplineMatrix,
+// mppi_controller.cu:67-141): the spline through e_i at the knots tau
+// (second derivatives by the Thomas algorithm, zero at both ends), read at
+// t = 0..T-1 with the end values held outside [tau_0, tau_{M-1}].
+template <class S>
+void spline_basis(const int* tau, int M, int T, S* B)
+{
+    std::vector<S> h(M - 1);
+    for (int j = 0; j < M - 1; ++j)
+        h[j] = (S)(tau[j + 1] - tau[j]);
+    for (int basis = 0; basis < M; ++basis)
+    {
+        std::vector<S> y(M, (S)0), m(M, (S)0);
+        y[basis] = 1;
+        if (M > 2)
+        {
+            const int N = M - 2;
+            std::vector<S> a(N), b(N), c(N), d(N), x(N);
+            for (int i = 0; i < N; ++i)
+            {
+                const int j = i + 1;
+                a[i] = h[j - 1];
+                b[i] = (S)2 * (h[j - 1] + h[j]);
+                c[i] = h[j];
+                d[i] = (S)6 * ((y[j + 1] - y[j]) / h[j] - (y[j] - y[j - 1]) / h[j - 1]);
+            }
+            for (int i = 1; i < N; ++i)
+            {
+                const S w = a[i] / b[i - 1];
+                b[i] = b[i] - w * c[i - 1];
+                d[i] = d[i] - w * d[i - 1];
+            }
+            x[N - 1] = d[N - 1] / b[N - 1];
+            for (int i = N - 2; i >= 0; --i)
+                x[i] = (d[i] - c[i] * x[i + 1]) / b[i];
+            for (int i = 0; i < N; ++i)
+                m[i + 1] = x[i];
+        }
+        int seg = 0;
+        for (int t = 0; t < T; ++t)
+        {
+            S val;
+            if (t <= tau[0])
+                val = y[0];
+            else if (t >= tau[M - 1])
+                val = y[M - 1];
+            else
+            {
+                while (!(tau[seg] <= t && t <= tau[seg + 1]))
+                    ++seg;
+                const S x0 = (S)tau[seg], x1 = (S)tau[seg + 1];
+                const S hh = x1 - x0;
+                const S A = (x1 - (S)t) / hh;
+                const S Bq = ((S)t - x0) / hh;
+                val = m[seg] * (A * A * A - A) * (hh * hh) / (S)6 + m[seg + 1] * (Bq * Bq * Bq - Bq) * (hh * hh) / (S)6 + y[seg] * A +
+                      y[seg + 1] * Bq;
+            }
+            B[(size_t)t * M + basis] = val;
+        }
+    }
+}
+
+// Knot noise of sample k, plan b, knot m: sigma * (z0, z1), the first
+// Box-Muller pair of Philox (k, b * DR_MAX_KNOTS + m, step, 0).
+inline void bm_knot_noise(uint32_t key0, uint32_t key1, uint32_t k, int b, int m, uint32_t step, float sigma, float e[2])
+{
+    const W4 w = philox(W4{{k, (uint32_t)(b * DR_MAX_KNOTS + m), step, 0u}}, key0, key1);
+    float z0, z1;
+    normal_pair(w.v[0], w.v[1], &z0, &z1);
+    e[0] = sigma * z0;
+    e[1] = sigma * z1;
+}
+
+// sampleContributed / splineSampleContributed (state.h:203-228), one factor.
+inline bool bm_sample_contributed(int b, int tLocal, int btime, int used, int T)
+{
+    const int origin = (b != 0 && btime > 0) ? btime : 0;
+    const int tGlobal = tLocal + origin;
+    if (tGlobal >= T)
+        return false;
+    if ((b != 0 && b != used) || (b == 0 && used != 0 && tGlobal >= btime))
+        return false;
+    return true;
+}
+
+inline bool bm_spline_contributed(int b, int m, int btime, int used, int T, int M, const int* tau)
+{
+    return (m > 0 && bm_sample_contributed(b, tau[m - 1], btime, used, T)) || bm_sample_contributed(b, tau[m], btime, used, T) ||
+           (m < M - 1 && bm_sample_contributed(b, tau[m + 1], btime, used, T));
+}
+
+// Rollouts of sample k under every true model compatible with the initial
+// commitment (fullRolloutKernel, kernels.cu:34-171): action = sum_m
+// B[t - t_origin][m] (knot + noise) of the current branch; the environment
+// step updates the belief; a branch commits when the belief is confident.
+template <class S>
+void bm_rollout(const Env<S>& E, const dr_sr_cost& K, const dr_sr_state& x0, const dr_bm_instance& I, const float* belief0f, const float* knots,
+                const S* B, uint32_t step, uint32_t k, S* ctrue, int* used, int* btime)
+{
+    const int NT = E.e.n_models, T = I.n_timesteps, M = I.n_knots, NK = I.n_samples;
+    const uint32_t key0 = (uint32_t)(I.seed & 0xffffffffu), key1 = (uint32_t)(I.seed >> 32);
+    S belief0[DR_SR_MAX_MODELS];
+    for (int th = 0; th < NT; ++th)
+        belief0[th] = (S)belief0f[th];
+    const int firstPred = find_confident(belief0, NT, I.min_confidence);
+    const int firstTime = firstPred == 0 ? -1 : 0;
+    for (int th = 0; th < NT; ++th)
+    {
+        ctrue[th] = (S)INFINITY;
+        used[th] = firstPred;
+        btime[th] = firstTime;
+    }
+    for (int th = 0; th < NT; ++th)
+    {
+        if (firstPred != 0 && firstPred != th + 1)
+            continue;
+        SState<S> s = load<S>(x0);
+        S belief[DR_SR_MAX_MODELS];
+        for (int q = 0; q < NT; ++q)
+            belief[q] = belief0[q];
+        int pred = firstPred, time = firstTime, origin = 0;
+        S acc = 0, decay = 1;
+        for (int t = 0; t < T; ++t)
+        {
+            const S* Brow = B + (size_t)(t - origin) * M;
+            const float* kn = knots + (size_t)pred * M * 2;
+            S u[2] = {0, 0};
+            for (int m = 0; m < M; ++m)
+            {
+                float e[2];
+                bm_knot_noise(key0, key1, k, pred, m, step, I.sigma, e);
+                for (int d = 0; d < 2; ++d)
+                {
+                    const S v = sizeof(S) == 4 ? (S)(kn[m * 2 + d] + e[d]) : (S)kn[m * 2 + d] + (S)e[d];
+                    u[d] = fm(Brow[m], v, u[d]);
+                }
+            }
+            Normals rng{key0, key1, (uint32_t)th * (uint32_t)NK + k, 4u * (uint32_t)t, step};
+            const int term = env_step(E, th, u, s, rng, true, belief);
+            const int np = find_confident(belief, NT, I.min_confidence);
+            if (pred == 0 && np != 0)
+            {
+                pred = np;
+                time = t + 1;
+                origin = t + 1;
+            }
+            acc = fm(state_cost(E, K, s), decay, acc);
+            decay = decay * (S)0.97f;
+            if (term != 0)
+                break;
+        }
+        ctrue[th] = acc + final_cost(E, K, s);
+        used[th] = pred;
+        btime[th] = time;
+    }
+}
+
+// One control step of one branching spline MPPI controller (kernels.cu:175-486,
+// mppi_controller.cu:250-466 without verification).  In FP32 the weights
+// are FP32 and the weighted sums FP64, as in the engine.
+template <class S>
+void bm_step(const dr_sr_env& env, const dr_sr_cost& K, const dr_sr_state& x0, const dr_bm_instance& I, const float* belief, uint32_t step,
+             float lambda, const float* knots, int n_threads, float* noise_out, double* ctrue_out, int* used_out, int* btime_out,
+             double* costs_out, double* mins_out, double* nu_out, double* knots_out, double* dense_out, double* res_out)
+{
+    const Env<S> E(env);
+    const int NT = env.n_models, NB = NT + 1, T = I.n_timesteps, M = I.n_knots, NK = I.n_samples;
+    const uint32_t key0 = (uint32_t)(I.seed & 0xffffffffu), key1 = (uint32_t)(I.seed >> 32);
+    std::vector<S> B((size_t)T * M);
+    spline_basis<S>(I.knots, M, T, B.data());
+    std::vector<float> noise((size_t)NB * M * NK * 2);
+    std::vector<S> ct((size_t)NT * NK);
+    std::vector<int> used((size_t)NT * NK), bt((size_t)NT * NK);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads > 0 ? n_threads : 1)
+    for (int k = 0; k < NK; ++k)
+    {
+        for (int b = 0; b < NB; ++b)
+            for (int m = 0; m < M; ++m)
+                bm_knot_noise(key0, key1, (uint32_t)k, b, m, step, I.sigma, &noise[(((size_t)b * M + m) * NK + k) * 2]);
+        S c[DR_SR_MAX_MODELS];
+        int u[DR_SR_MAX_MODELS], t[DR_SR_MAX_MODELS];
+        bm_rollout<S>(E, K, x0, I, belief, knots, B.data(), step, (uint32_t)k, c, u, t);
+        for (int th = 0; th < NT; ++th)
+        {
+            ct[(size_t)th * NK + k] = c[th];
+            used[(size_t)th * NK + k] = u[th];
+            bt[(size_t)th * NK + k] = t[th];
+        }
+    }
+    // aggregateBranchCostsKernel (kernels.cu:175-217)
+    std::vector<S> costs((size_t)NB * NK);
+    for (int b = 0; b < NB; ++b)
+        for (int k = 0; k < NK; ++k)
+        {
+            S num = 0, den = 0;
+            for (int th = 0; th < NT; ++th)
+            {
+                if (b != 0 && b != th + 1)
+                    continue;
+                const S c = ct[(size_t)th * NK + k];
+                if (!std::isfinite((double)c))
+                    continue;
+                const S w = (S)belief[th];
+                num = fm(w, c, num);
+                den = den + w;
+            }
+            costs[(size_t)b * NK + k] = den > (S)1e-30f ? num / den : (S)INFINITY;
+        }
+    const int initPred = find_confident(belief, NT, I.min_confidence);
+    auto coeff_of = [&](int b, int m, int k) {
+        S coeff = 0;
+        for (int th = 0; th < NT; ++th)
+        {
+            if (belief[th] <= 0.0f)
+                continue;
+            if (b != 0 && b != th + 1)
+                continue;
+            if (bm_spline_contributed(b, m, bt[(size_t)th * NK + k], used[(size_t)th * NK + k], T, M, I.knots))
+                coeff = coeff + (S)belief[th];
+        }
+        return coeff;
+    };
+    // computeMaskedMinCostsKernel (kernels.cu:221-301)
+    std::vector<S> mins((size_t)NB * M, (S)INFINITY);
+    for (int b = 0; b < NB; ++b)
+    {
+        if (initPred != 0 && initPred != b)
+            continue;
+        for (int m = 0; m < M; ++m)
+        {
+            S v = (S)INFINITY;
+            for (int k = 0; k < NK; ++k)
+                if (coeff_of(b, m, k) > (S)0.01f)
+                    v = mn(v, costs[(size_t)b * NK + k]);
+            mins[(size_t)b * M + m] = v;
+        }
+    }
+    // weightedAverageKernelUnified (kernels.cu:307-408)
+    std::vector<S> kn((size_t)NB * M * 2);
+    std::vector<S> nu(NB, (S)0);
+    for (int b = 0; b < NB; ++b)
+        for (int m = 0; m < M; ++m)
+        {
+            const S minC = mins[(size_t)b * M + m];
+            double n0 = 0, n1 = 0, dn = 0;
+            if (!std::isinf((double)minC))
+                for (int k = 0; k < NK; ++k)
+                {
+                    const S coeff = coeff_of(b, m, k);
+                    if (!(coeff > (S)0.01f))
+                        continue;
+                    const float* e = &noise[(((size_t)b * M + m) * NK + k) * 2];
+                    double wc;
+                    if (sizeof(S) == 4)
+                    {
+                        const float w = exp32(-((float)costs[(size_t)b * NK + k] - (float)minC) / lambda);
+                        wc = (double)(w * (float)coeff);
+                    }
+                    else
+                        wc = exp(-((double)costs[(size_t)b * NK + k] - (double)minC) / (double)lambda) * (double)coeff;
+                    n0 += wc * (double)e[0];
+                    n1 += wc * (double)e[1];
+                    dn += wc;
+                }
+            const double nums[2] = {n0, n1};
+            for (int d = 0; d < 2; ++d)
+            {
+                S v = (S)knots[((size_t)b * M + m) * 2 + d];
+                if (dn > 1e-30)
+                    v = sizeof(S) == 4 ? (S)((float)v + (float)(nums[d] / dn)) : v + (S)(nums[d] / dn);
+                kn[((size_t)b * M + m) * 2 + d] = v;
+            }
+            if (m == 0)
+                nu[b] = (S)dn;
+        }
+    // interpolateSplineKernel (kernels.cu:411-431) from the unclamped knots
+    std::vector<S> dense((size_t)NB * T * 2);
+    for (int b = 0; b < NB; ++b)
+        for (int t = 0; t < T; ++t)
+            for (int d = 0; d < 2; ++d)
+            {
+                S sum = 0;
+                for (int m = 0; m < M; ++m)
+                    sum = fm(B[(size_t)t * M + m], kn[((size_t)b * M + m) * 2 + d], sum);
+                dense[((size_t)b * T + t) * 2 + d] = sum;
+            }
+    // clampNominalKernel (kernels.cu:434-456) on the knots
+    const S ma = (S)env.max_accel[0];
+    for (int bm = 0; bm < NB * M; ++bm)
+    {
+        const S a = kn[bm * 2], c = kn[bm * 2 + 1];
+        const S sq = a * a + c * c;
+        if (sq > ma * ma)
+        {
+            const S sc = ma / sq_root(sq);
+            kn[bm * 2] = a * sc;
+            kn[bm * 2 + 1] = c * sc;
+        }
+    }
+    // action, temperature, shifts (mppi_controller.cu:385-457)
+    const int br = initPred;
+    const S a0 = dense[(size_t)br * T * 2], a1 = dense[(size_t)br * T * 2 + 1];
+    float lam = lambda;
+    if (I.adapt_lambda)
+    {
+        const float usedNu = (float)nu[br];
+        if (usedNu > 0.001f * (float)NK)
+            lam = lam * 0.9f;
+        else if (usedNu < 0.0005f * (float)NK)
+            lam = lam * 1.2f;
+    }
+    std::vector<S> sh((size_t)M * 2);
+    for (int m = 0; m < M; ++m)
+    {
+        int tEval = I.knots[m] + 1;
+        if (tEval >= T)
+            tEval = T - 1;
+        for (int d = 0; d < 2; ++d)
+        {
+            S sum = 0;
+            for (int j = 0; j < M; ++j)
+                sum = fm(B[(size_t)tEval * M + j], kn[((size_t)br * M + j) * 2 + d], sum);
+            sh[m * 2 + d] = sum;
+        }
+    }
+    for (int m = 0; m < M * 2; ++m)
+        kn[(size_t)br * M * 2 + m] = sh[m];
+    for (int t = 0; t < T; ++t)
+        for (int d = 0; d < 2; ++d)
+            dense[((size_t)br * T + t) * 2 + d] = t + 1 < T ? dense[((size_t)br * T + t + 1) * 2 + d] : (S)0;
+
+    if (noise_out)
+        memcpy(noise_out, noise.data(), noise.size() * sizeof(float));
+    for (size_t i = 0; i < ct.size(); ++i)
+    {
+        if (ctrue_out)
+            ctrue_out[i] = (double)ct[i];
+        if (used_out)
+            used_out[i] = used[i];
+        if (btime_out)
+            btime_out[i] = bt[i];
+    }
+    if (costs_out)
+        for (size_t i = 0; i < costs.size(); ++i)
+            costs_out[i] = (double)costs[i];
+    for (size_t i = 0; i < mins.size(); ++i)
+        mins_out[i] = (double)mins[i];
+    for (int b = 0; b < NB; ++b)
+        nu_out[b] = (double)nu[b];
+    for (size_t i = 0; i < kn.size(); ++i)
+        knots_out[i] = (double)kn[i];
+    for (size_t i = 0; i < dense.size(); ++i)
+        dense_out[i] = (double)dense[i];
+    res_out[0] = (double)a0;
+    res_out[1] = (double)a1;
+    res_out[2] = (double)lam;
+    res_out[3] = (double)br;
+}
+
 } // namespace
 
 extern "C" {
@@ -941,6 +1370,106 @@ int or_prsr_choose(int f64, const dr_sr_env* env, const dr_sr_cost* K, const dr_
         prsr_choose<double>(*env, *K, *x0, *inst, belief, models, cand, step, lam, nu, cc_out, cs_out, res_out);
     else
         prsr_choose<float>(*env, *K, *x0, *inst, belief, models, cand, step, lam, nu, cc_out, cs_out, res_out);
+    return 0;
+}
+
+// Natural cubic spline basis [T][M] of the knots tau.
+int or_bm_basis(int f64, const int* tau, int M, int T, double* B)
+{
+    if (!tau || !B || M < 2 || T < 1)
+        return 1;
+    std::vector<float> bf((size_t)T * M);
+    if (f64)
+        spline_basis<double>(tau, M, T, B);
+    else
+    {
+        spline_basis<float>(tau, M, T, bf.data());
+        for (size_t i = 0; i < bf.size(); ++i)
+            B[i] = bf[i];
+    }
+    return 0;
+}
+
+// Belief update of one observed opponent action against each model's
+// nominal action noms [n_models][2] (belief updated in place).
+int or_sr_update_belief(int f64, const dr_sr_env* env, double* belief, const double* obs, const double* noms)
+{
+    if (!env || !belief || !obs || !noms)
+        return 1;
+    const int NT = env->n_models;
+    if (f64)
+    {
+        const Env<double> E(*env);
+        double nm[DR_SR_MAX_MODELS][2];
+        for (int th = 0; th < NT; ++th)
+            for (int d = 0; d < 2; ++d)
+                nm[th][d] = noms[th * 2 + d];
+        update_belief(E, belief, obs, nm);
+        return 0;
+    }
+    const Env<float> E(*env);
+    float b[DR_SR_MAX_MODELS], o[2] = {(float)obs[0], (float)obs[1]}, nm[DR_SR_MAX_MODELS][2];
+    for (int th = 0; th < NT; ++th)
+    {
+        b[th] = (float)belief[th];
+        for (int d = 0; d < 2; ++d)
+            nm[th][d] = (float)noms[th * 2 + d];
+    }
+    update_belief(E, b, o, nm);
+    for (int th = 0; th < NT; ++th)
+        belief[th] = b[th];
+    return 0;
+}
+
+// One world step with belief update (SimulationEngine::dynStep,
+// engine.cpp:286-312): the opponent follows theta; belief [n_models] in/out.
+int or_sr_env_step_belief(int f64, const dr_sr_env* env, int theta, const float* ego, dr_sr_state* st, float* belief, uint64_t seed,
+                          uint32_t k, uint32_t t, uint32_t step, int lose_on_opp_win)
+{
+    Normals rng{(uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32), k, 4u * t, step};
+    const int NT = env->n_models;
+    if (f64)
+    {
+        const Env<double> E(*env);
+        SState<double> s = load<double>(*st);
+        const double u[2] = {ego[0], ego[1]};
+        double b[DR_SR_MAX_MODELS];
+        for (int th = 0; th < NT; ++th)
+            b[th] = belief[th];
+        const int r = env_step(E, theta, u, s, rng, lose_on_opp_win != 0, b);
+        store(s, *st);
+        for (int th = 0; th < NT; ++th)
+            belief[th] = (float)b[th];
+        return r;
+    }
+    const Env<float> E(*env);
+    SState<float> s = load<float>(*st);
+    const float u[2] = {ego[0], ego[1]};
+    const int r = env_step(E, theta, u, s, rng, lose_on_opp_win != 0, belief);
+    store(s, *st);
+    return r;
+}
+
+// Branching spline MPPI step of one StratRace2D controller from its belief
+// and knots [n_models + 1][M][2]: noise [n_models + 1][M][K][2], per-model
+// costs / branch used / branching time [n_models][K], aggregated costs
+// [n_models + 1][K], masked minima [n_models + 1][M], nu [n_models + 1],
+// knots after update + clamp + shift, dense plans [n_models + 1][T][2]
+// after the shift, res = action[2], next lambda, chosen branch.
+int or_bm_step(int f64, const dr_sr_env* env, const dr_sr_cost* K, const dr_sr_state* x0, const dr_bm_instance* inst, const float* belief,
+               uint32_t step, float lambda, const float* knots, int n_threads, float* noise_out, double* ctrue_out, int* used_out,
+               int* btime_out, double* costs_out, double* mins_out, double* nu_out, double* knots_out, double* dense_out, double* res_out)
+{
+    if (!env || !K || !x0 || !inst || !belief || !knots || !mins_out || !nu_out || !knots_out || !dense_out || !res_out)
+        return 1;
+    if (inst->n_knots < 2 || inst->n_knots > DR_MAX_KNOTS || env->n_models < 1 || env->n_models > DR_SR_MAX_MODELS)
+        return 1;
+    if (f64)
+        bm_step<double>(*env, *K, *x0, *inst, belief, step, lambda, knots, n_threads, noise_out, ctrue_out, used_out, btime_out, costs_out,
+                        mins_out, nu_out, knots_out, dense_out, res_out);
+    else
+        bm_step<float>(*env, *K, *x0, *inst, belief, step, lambda, knots, n_threads, noise_out, ctrue_out, used_out, btime_out, costs_out,
+                       mins_out, nu_out, knots_out, dense_out, res_out);
     return 0;
 }
 

@@ -70,6 +70,11 @@ def lib(abi):
                                U, F, F, F, I, F, D, D, C.POINTER(I), D, D]
     L.or_prsr_choose.argtypes = [I, C.POINTER(abi.SrEnv), C.POINTER(abi.SrCost), C.POINTER(abi.SrState), C.POINTER(abi.PrsrInstance),
                                  F, C.POINTER(I), D, U, F, D, D, D, D]
+    L.or_bm_basis.argtypes = [I, C.POINTER(I), I, I, D]
+    L.or_sr_update_belief.argtypes = [I, C.POINTER(abi.SrEnv), D, D, D]
+    L.or_sr_env_step_belief.argtypes = [I, C.POINTER(abi.SrEnv), I, F, C.POINTER(abi.SrState), F, C.c_uint64, U, U, U, I]
+    L.or_bm_step.argtypes = [I, C.POINTER(abi.SrEnv), C.POINTER(abi.SrCost), C.POINTER(abi.SrState), C.POINTER(abi.BmInstance), F,
+                             U, C.c_float, F, I, F, D, C.POINTER(I), C.POINTER(I), D, D, D, D, D, D]
     _lib = L
     return L
 
@@ -379,3 +384,57 @@ class Oracle:
         return dict(cand_cost=res[0:2], use_nom=int(res[2]), reset_nom=int(res[3]), action=res[4:6], lam=res[6:9],
                     cc=cc.reshape(2, M), cs=cs.reshape(2, M))
 
+    # ---------------------------------------------------------- branching spline MPPI
+
+    def bm_basis(self, f64, knots, T):
+        """Natural cubic spline basis [T][M] of the integer knots."""
+        tau = (C.c_int * len(knots))(*[int(x) for x in knots])
+        B = np.empty(T * len(knots))
+        assert self.L.or_bm_basis(int(f64), tau, len(knots), T, _d(B)) == 0
+        return B.reshape(T, len(knots))
+
+    def sr_update_belief(self, f64, env, belief, obs, noms):
+        """Belief after one observed opponent action, given each model's nominal action noms [n_models][2]."""
+        b = np.array(belief, dtype=np.float64)
+        o = np.ascontiguousarray(obs, dtype=np.float64)
+        n = np.ascontiguousarray(noms, dtype=np.float64).reshape(-1)
+        assert self.L.or_sr_update_belief(int(f64), C.byref(env), _d(b), _d(o), _d(n)) == 0
+        return b
+
+    def sr_env_step_belief(self, f64, env, theta, ego, state, belief, seed, k, t, step, lose_on_opp_win=True):
+        """One environment step with belief update; returns (state, belief, terminal code)."""
+        st = self.abi.SrState.from_buffer_copy(state)
+        u = np.ascontiguousarray(ego, dtype=np.float32)
+        b = np.zeros(self.abi.SR_MAX_MODELS, dtype=np.float32)
+        b[:len(belief)] = belief
+        r = self.L.or_sr_env_step_belief(int(f64), C.byref(env), theta, _f(u), C.byref(st), _f(b), seed, k, t, step,
+                                         1 if lose_on_opp_win else 0)
+        return st, b[:env.n_models].copy(), r
+
+    def bm_step(self, f64, env, cost, state, ins, belief, step, lam, knots, threads=8, want_noise=True):
+        """One branching spline MPPI step of a StratRace2D controller from its belief and knots [n_models + 1][M][2]."""
+        K, T, M, NT = ins.n_samples, ins.n_timesteps, ins.n_knots, env.n_models
+        NB = NT + 1
+        kn = np.ascontiguousarray(knots, dtype=np.float32).reshape(-1)
+        assert kn.size == NB * M * 2
+        bel = np.zeros(self.abi.SR_MAX_MODELS, dtype=np.float32)
+        bel[:NT] = belief[:NT]
+        noise = np.empty(NB * M * K * 2, dtype=np.float32) if want_noise else None
+        ct = np.empty(NT * K)
+        used = np.empty(NT * K, dtype=np.int32)
+        bt = np.empty(NT * K, dtype=np.int32)
+        costs = np.empty(NB * K)
+        mins = np.empty(NB * M)
+        nu = np.empty(NB)
+        kout = np.empty(NB * M * 2)
+        dense = np.empty(NB * T * 2)
+        res = np.empty(4)
+        ip = C.POINTER(C.c_int)
+        rc = self.L.or_bm_step(int(f64), C.byref(env), C.byref(cost), C.byref(state), C.byref(ins), _f(bel), step, lam, _f(kn), threads,
+                               _f(noise) if noise is not None else None, _d(ct), used.ctypes.data_as(ip), bt.ctypes.data_as(ip),
+                               _d(costs), _d(mins), _d(nu), _d(kout), _d(dense), _d(res))
+        assert rc == 0
+        return dict(noise=noise.reshape(NB, M, K, 2) if noise is not None else None, ctrue=ct.reshape(NT, K),
+                    used=used.reshape(NT, K), btime=bt.reshape(NT, K), costs=costs.reshape(NB, K), mins=mins.reshape(NB, M),
+                    nu=nu, knots=kout.reshape(NB, M, 2), dense=dense.reshape(NB, T, 2), action=res[:2], lambda_=float(res[2]),
+                    branch=int(res[3]))

@@ -218,6 +218,31 @@ def pr_sr_race(pkg, steps=30, warmup=3):
                                    "update_choice_world": float(ms[3]), "step": float(ms[4])}, "steps": steps}
 
 
+def bm_race(pkg, steps=30, warmup=3):
+    """Branching spline MPPI (the reference's default MPPI controller) on its StratRace2D race at
+    the spline setting (K = 2^16, T = 80, 20 knots, two opponent models, uniform initial belief,
+    temperature 10 adapted, sigma 0.5), closed loop with belief updates, one world."""
+    sc = pkg.scenarios
+    scn = sc.bm_race(n_inst=1)
+    eng = pkg.BMEngine(scn)
+    eng.step(warmup, advance=True)
+    t0 = time.perf_counter()
+    eng.step(steps, advance=True)
+    elapsed = time.perf_counter() - t0
+    eng.set_timing(True)
+    eng.step(5, advance=True)
+    ms = eng.last_timing() / 5
+    d = eng.diag()[0]
+    eng.close()
+    ins = scn.instances[0]
+    rollouts = scn.env.n_models * ins.n_samples
+    return {"workload": scn.name, "control_steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+            "rollouts_per_step": rollouts, "rollout_env_steps_per_s": rollouts * ins.n_timesteps * steps / elapsed,
+            "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate_min": float(ms[1]), "weighted_avg": float(ms[2]),
+                                   "update_world": float(ms[3]), "step": float(ms[4])},
+            "final_branch": int(d.branch), "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +259,7 @@ def main():
     ap.add_argument("--pr-only", action="store_true", help="run only the PR-MPPI section (for profiling)")
     ap.add_argument("--no-sr", action="store_true")
     ap.add_argument("--sr-only", action="store_true", help="run only the StratRace2D section (for profiling)")
+    ap.add_argument("--bm-only", action="store_true", help="run only the branching spline MPPI section (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 512 MiB buffer 3 times (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -255,9 +281,13 @@ def main():
 
     pkg = package()
     sc = pkg.scenarios
+    if args.bm_only:
+        print(json.dumps({"bm_race": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup)}), flush=True)
+        return
     if args.sr_only:
         print(json.dumps({"sr_race": sr_race(pkg, steps=args.steps, warmup=args.warmup),
-                          "pr_sr_race": pr_sr_race(pkg, steps=min(args.steps, 30), warmup=args.warmup)}), flush=True)
+                          "pr_sr_race": pr_sr_race(pkg, steps=min(args.steps, 30), warmup=args.warmup),
+                          "bm_race": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup)}), flush=True)
         return
     if args.pr_only:
         print(json.dumps({"pr_mppi": pr_mppi(pkg, steps=args.steps, warmup=args.warmup)}), flush=True)
@@ -354,6 +384,9 @@ def main():
         log("bench: PR-MPPI on StratRace2D, K=65536, T=60, P=9 ...")
         out["pr_sr_race"] = pr_sr_race(pkg)
         log(f"bench: pr_sr_race {json.dumps(out['pr_sr_race'])}")
+        log("bench: branching spline MPPI on StratRace2D, K=65536, T=80, 20 knots ...")
+        out["bm_race"] = bm_race(pkg)
+        log(f"bench: bm_race {json.dumps(out['bm_race'])}")
     if not args.no_batch64:
         log("bench: 64 uneven controllers, batched vs per-instance ...")
         out["batch64"] = batch_vs_per_instance(pkg)

@@ -22,9 +22,9 @@ namespace dr {
 hipError_t launch_bm_rollout(int max_m, int max_tm, int grid, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmTile* tiles,
                              int n_tiles, const dr_sr_env* env, const dr_sr_cost& KC, const dr_sr_state* states, const float* knots,
                              const float* basis, float2* noise, float* ctrue, int* used, int* btime);
-hipError_t launch_bm_reduce(int n_inst, int max_nbk, int n_rtiles_min, int n_rtiles_avg, hipStream_t st, const BmInstDev* insts,
-                            BmDyn* dyn, const BmRTile* rt_min, const BmRTile* rt_avg, const float* ctrue, float* costs, const int* used,
-                            const int* btime, float* mins, const float2* noise, double* part, hipEvent_t mid);
+hipError_t launch_bm_reduce(int n_inst, int max_nbk, int n_rtiles, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const BmRTile* rt,
+                            const float* ctrue, float* costs, const int* used, const int* btime, float* pmin, float* mins,
+                            const float2* noise, double* part, hipEvent_t mid);
 hipError_t launch_bm_update(int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const double* part, const float* basis,
                             float* knots, float* dense, float* nu, float* actions, float max_accel);
 hipError_t launch_bm_advance(int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const dr_sr_env* env, dr_sr_state* states,
@@ -108,18 +108,18 @@ struct dr_bm_engine
     std::vector<dr_bm_instance> inst;
     std::vector<BmInstDev> idev;
     std::vector<BmTile> tiles;
-    std::vector<BmRTile> rt_min, rt_avg;
+    std::vector<BmRTile> rtiles;
     int max_m = 0, max_tm = 0, max_nbk = 0;
     long long noise_total = 0, part_total = 0;
-    int ct_total = 0, cost_total = 0, min_total = 0, knot_total = 0, dense_total = 0, B_total = 0, nu_total = 0;
+    int ct_total = 0, cost_total = 0, min_total = 0, pmin_total = 0, knot_total = 0, dense_total = 0, B_total = 0, nu_total = 0;
 
     DevBuf<BmInstDev> d_inst;
     DevBuf<BmDyn> d_dyn;
     DevBuf<BmTile> d_tiles;
-    DevBuf<BmRTile> d_rt_min, d_rt_avg;
+    DevBuf<BmRTile> d_rt;
     DevBuf<dr_sr_env> d_env;
     DevBuf<dr_sr_state> d_states;
-    DevBuf<float> d_knots, d_dense, d_B, d_ctrue, d_costs, d_mins, d_nu, d_actions;
+    DevBuf<float> d_knots, d_dense, d_B, d_ctrue, d_costs, d_mins, d_pmin, d_nu, d_actions;
     DevBuf<int> d_used, d_btime, d_events;
     DevBuf<float2> d_noise;
     DevBuf<double> d_part;
@@ -154,8 +154,8 @@ int dr_bm_engine::enqueue_step(bool advance, int timed_index)
                              d_noise.p, d_ctrue.p, d_used.p, d_btime.p));
     if (ev)
         HIPCHK(hipEventRecord(ev[1], stream));
-    HIPCHK(launch_bm_reduce(n, max_nbk, (int)rt_min.size(), (int)rt_avg.size(), stream, d_inst.p, d_dyn.p, d_rt_min.p, d_rt_avg.p, d_ctrue.p,
-                            d_costs.p, d_used.p, d_btime.p, d_mins.p, d_noise.p, d_part.p, ev ? ev[2] : nullptr));
+    HIPCHK(launch_bm_reduce(n, max_nbk, (int)rtiles.size(), stream, d_inst.p, d_dyn.p, d_rt.p, d_ctrue.p, d_costs.p, d_used.p, d_btime.p,
+                            d_pmin.p, d_mins.p, d_noise.p, d_part.p, ev ? ev[2] : nullptr));
     if (ev)
         HIPCHK(hipEventRecord(ev[3], stream));
     HIPCHK(launch_bm_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_B.p, d_knots.p, d_dense.p, d_nu.p, d_actions.p, env.max_accel[0]));
@@ -334,6 +334,8 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
         I.n_chunks = (I.K + BM_CHUNK - 1) / BM_CHUNK;
         I.part_off = e->part_total;
         e->part_total += (long long)NB * I.M * I.n_chunks * 3;
+        I.pmin_off = e->pmin_total;
+        e->pmin_total += NB * I.M * I.n_chunks;
         I.nu_off = e->nu_total;
         e->nu_total += NB;
         e->max_m = std::max(e->max_m, I.M);
@@ -344,11 +346,8 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
                 e->tiles.push_back(BmTile{i, k0, th, 0});
         for (int b = 0; b < NB; ++b)
             for (int m = 0; m < I.M; ++m)
-            {
-                e->rt_min.push_back(BmRTile{i, b, m, 0});
                 for (int c = 0; c < I.n_chunks; ++c)
-                    e->rt_avg.push_back(BmRTile{i, b, m, c});
-            }
+                    e->rtiles.push_back(BmRTile{i, b, m, c});
     }
     for (const float v : Ball)
         if (!std::isfinite(v))
@@ -362,7 +361,7 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
         return true;
     };
     if (!chk(e->d_inst.alloc(n), "alloc inst") || !chk(e->d_dyn.alloc(n), "alloc dyn") || !chk(e->d_tiles.alloc(e->tiles.size()), "alloc tiles") ||
-        !chk(e->d_rt_min.alloc(e->rt_min.size()), "alloc rt") || !chk(e->d_rt_avg.alloc(e->rt_avg.size()), "alloc rt") ||
+        !chk(e->d_rt.alloc(e->rtiles.size()), "alloc rt") || !chk(e->d_pmin.alloc(e->pmin_total), "alloc pmin") ||
         !chk(e->d_env.alloc(1), "alloc env") || !chk(e->d_states.alloc(n), "alloc states") || !chk(e->d_knots.alloc(e->knot_total), "alloc knots") ||
         !chk(e->d_dense.alloc(e->dense_total), "alloc dense") || !chk(e->d_B.alloc(e->B_total), "alloc B") ||
         !chk(e->d_ctrue.alloc(e->ct_total), "alloc ctrue") || !chk(e->d_costs.alloc(e->cost_total), "alloc costs") ||
@@ -386,8 +385,7 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
     if (!chk(hipMemcpy(e->d_inst.p, e->idev.data(), n * sizeof(BmInstDev), hipMemcpyHostToDevice), "copy inst") ||
         !chk(hipMemcpy(e->d_dyn.p, dyn.data(), n * sizeof(BmDyn), hipMemcpyHostToDevice), "copy dyn") ||
         !chk(hipMemcpy(e->d_tiles.p, e->tiles.data(), e->tiles.size() * sizeof(BmTile), hipMemcpyHostToDevice), "copy tiles") ||
-        !chk(hipMemcpy(e->d_rt_min.p, e->rt_min.data(), e->rt_min.size() * sizeof(BmRTile), hipMemcpyHostToDevice), "copy rt") ||
-        !chk(hipMemcpy(e->d_rt_avg.p, e->rt_avg.data(), e->rt_avg.size() * sizeof(BmRTile), hipMemcpyHostToDevice), "copy rt") ||
+        !chk(hipMemcpy(e->d_rt.p, e->rtiles.data(), e->rtiles.size() * sizeof(BmRTile), hipMemcpyHostToDevice), "copy rt") ||
         !chk(hipMemcpy(e->d_env.p, env, sizeof(dr_sr_env), hipMemcpyHostToDevice), "copy env") ||
         !chk(hipMemcpy(e->d_states.p, st.data(), n * sizeof(dr_sr_state), hipMemcpyHostToDevice), "copy states") ||
         !chk(hipMemcpy(e->d_knots.p, zk.data(), zk.size() * sizeof(float), hipMemcpyHostToDevice), "copy knots") ||

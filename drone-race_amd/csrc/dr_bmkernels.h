@@ -11,6 +11,7 @@
 //   btime  int[NT][K]           branching time (-1: never)
 //   costs  float[NB][K]         belief-weighted branch costs
 //   mins   float[NB][M]         masked minima per (plan, knot)
+//   pmin   float[NB][M][chunks] masked minima per sample chunk
 //   part   double[NB][M][chunks][3]  weighted-average partials (num x, num y, den)
 #pragma once
 
@@ -41,6 +42,7 @@ struct BmInstDev
     int B_off;           // T * M
     int n_chunks;
     long long part_off;  // double units, NB * M * n_chunks * 3
+    int pmin_off;        // NB * M * n_chunks
     int nu_off;          // NB
 };
 

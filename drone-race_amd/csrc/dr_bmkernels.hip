@@ -77,9 +77,12 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
     __shared__ float s_k[NS];
     extern __shared__ float s_B[];
     __shared__ float s_kn[(DR_SR_MAX_MODELS + 1) * DR_MAX_KNOTS * 2];
+    __shared__ float s_bc[2 * DR_SR_MAX_MODELS];
 
     const int tid = threadIdx.x;
     const dr_sr_env& E = *env;
+    if (tid < E.n_models)
+        belief_consts(E, tid, &s_bc[tid], &s_bc[DR_SR_MAX_MODELS + tid]);
     for (int i = tid; i < NS * 2; i += BM_BLOCK)
     {
         s_p[i] = (&E.p_grid[0][0])[i];
@@ -168,9 +171,10 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                     }
                     const float ego[2] = {u0, u1};
                     NormalSeq rng = normal_seq(key0, key1, (uint32_t)(th * K + k), (uint32_t)t, step);
-                    const int term = env_step_t<true>(tb, E, th, ego, s, rng, true, belief);
-                    const int np = find_confident(belief, NT, I.min_conf);
-                    if (pred == 0 && np != 0)
+                    // once committed, the rollout no longer reads its belief
+                    const int term = env_step_t<true>(tb, E, th, ego, s, rng, true, belief, s_bc, pred == 0);
+                    const int np = pred == 0 ? find_confident(belief, NT, I.min_conf) : 0;
+                    if (np != 0)
                     {
                         pred = np;
                         time = t + 1;
@@ -241,10 +245,11 @@ __device__ __forceinline__ float knot_coeff(const BmInstDev& I, const float* bel
     return coeff;
 }
 
-// computeMaskedMinCostsKernel: one workgroup per (instance, plan, knot).
+// computeMaskedMinCostsKernel: one workgroup per (instance, plan, knot,
+// sample chunk); the chunk minima are combined by k_bm_wavg.
 __global__ __launch_bounds__(BM_RED_BLOCK) void k_bm_min(const BmInstDev* __restrict__ insts, const BmDyn* __restrict__ dyn,
                                                          const BmRTile* __restrict__ rt, const float* __restrict__ costs,
-                                                         const int* __restrict__ used, const int* __restrict__ btime, float* __restrict__ mins)
+                                                         const int* __restrict__ used, const int* __restrict__ btime, float* __restrict__ pmin)
 {
     __shared__ float s_m[BM_RED_BLOCK / WAVE];
     const BmRTile r = rt[blockIdx.x];
@@ -255,9 +260,12 @@ __global__ __launch_bounds__(BM_RED_BLOCK) void k_bm_min(const BmInstDev* __rest
     const int initPred = find_confident(belief, I.NT, I.min_conf);
     float v = __builtin_inff();
     if (initPred == 0 || initPred == r.b)
-        for (int k = threadIdx.x; k < I.K; k += BM_RED_BLOCK)
+    {
+        const int k0 = r.chunk * BM_CHUNK, k1 = min(I.K, k0 + BM_CHUNK);
+        for (int k = k0 + threadIdx.x; k < k1; k += BM_RED_BLOCK)
             if (knot_coeff(I, belief, used, btime, r.b, r.m, k) > 0.01f)
                 v = fminf(v, costs[I.cost_off + r.b * I.K + k]);
+    }
     v = wave_min(v);
     if ((threadIdx.x & (WAVE - 1)) == 0)
         s_m[threadIdx.x / WAVE] = v;
@@ -267,18 +275,18 @@ __global__ __launch_bounds__(BM_RED_BLOCK) void k_bm_min(const BmInstDev* __rest
         float m = s_m[0];
         for (int w = 1; w < BM_RED_BLOCK / WAVE; ++w)
             m = fminf(m, s_m[w]);
-        mins[I.min_off + r.b * I.M + r.m] = m;
+        pmin[I.pmin_off + (r.b * I.M + r.m) * I.n_chunks + r.chunk] = m;
     }
 }
 
 // weightedAverageKernelUnified: per (instance, plan, knot, sample chunk),
-// FP64 partial sums of w * coeff * eps (both dimensions) and w * coeff over
+// the masked minimum over the chunk minima, then FP64 partial sums of w * coeff * eps (both dimensions) and w * coeff over
 // the eligible samples, w = exp(-(cost - min) / lambda).
 __global__ __launch_bounds__(BM_RED_BLOCK) void k_bm_wavg(const BmInstDev* __restrict__ insts, const BmDyn* __restrict__ dyn,
                                                           const BmRTile* __restrict__ rt, const float* __restrict__ costs,
                                                           const int* __restrict__ used, const int* __restrict__ btime,
-                                                          const float* __restrict__ mins, const float2* __restrict__ noise,
-                                                          double* __restrict__ part)
+                                                          const float* __restrict__ pmin, float* __restrict__ mins,
+                                                          const float2* __restrict__ noise, double* __restrict__ part)
 {
     __shared__ double s_a[3][BM_RED_BLOCK / WAVE];
     const BmRTile r = rt[blockIdx.x];
@@ -287,7 +295,12 @@ __global__ __launch_bounds__(BM_RED_BLOCK) void k_bm_wavg(const BmInstDev* __res
     float belief[DR_SR_MAX_MODELS];
     for (int th = 0; th < DR_SR_MAX_MODELS; ++th)
         belief[th] = Y.belief[th];
-    const float minC = mins[I.min_off + r.b * I.M + r.m];
+    const float* pm = pmin + I.pmin_off + (r.b * I.M + r.m) * I.n_chunks;
+    float minC = pm[0];
+    for (int c = 1; c < I.n_chunks; ++c)
+        minC = fminf(minC, pm[c]);
+    if (r.chunk == 0 && threadIdx.x == 0)
+        mins[I.min_off + r.b * I.M + r.m] = minC;
     const float lam = Y.lambda;
     double n0 = 0.0, n1 = 0.0, dn = 0.0;
     if (!__builtin_isinf(minC))
@@ -485,19 +498,19 @@ hipError_t launch_bm_rollout(int max_m, int max_tm, int grid, hipStream_t st, co
     return hipGetLastError();
 }
 
-hipError_t launch_bm_reduce(int n_inst, int max_nbk, int n_rtiles_min, int n_rtiles_avg, hipStream_t st, const BmInstDev* insts,
-                            BmDyn* dyn, const BmRTile* rt_min, const BmRTile* rt_avg, const float* ctrue, float* costs, const int* used,
-                            const int* btime, float* mins, const float2* noise, double* part, hipEvent_t mid)
+hipError_t launch_bm_reduce(int n_inst, int max_nbk, int n_rtiles, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const BmRTile* rt,
+                            const float* ctrue, float* costs, const int* used, const int* btime, float* pmin, float* mins,
+                            const float2* noise, double* part, hipEvent_t mid)
 {
     k_bm_aggregate<<<dim3((max_nbk + 255) / 256, n_inst), 256, 0, st>>>(insts, dyn, 0, ctrue, costs);
-    k_bm_min<<<n_rtiles_min, BM_RED_BLOCK, 0, st>>>(insts, dyn, rt_min, costs, used, btime, mins);
+    k_bm_min<<<n_rtiles, BM_RED_BLOCK, 0, st>>>(insts, dyn, rt, costs, used, btime, pmin);
     if (mid)
     {
         hipError_t e = hipEventRecord(mid, st);
         if (e != hipSuccess)
             return e;
     }
-    k_bm_wavg<<<n_rtiles_avg, BM_RED_BLOCK, 0, st>>>(insts, dyn, rt_avg, costs, used, btime, mins, noise, part);
+    k_bm_wavg<<<n_rtiles, BM_RED_BLOCK, 0, st>>>(insts, dyn, rt, costs, used, btime, pmin, mins, noise, part);
     return hipGetLastError();
 }
 

@@ -38,6 +38,7 @@ namespace dr {
 namespace sr {
 
 constexpr int NS = DR_SR_TRACK_SAMPLES;
+static_assert((NS & (NS - 1)) == 0, "track sample count must be a power of two");
 constexpr int WINDOW = 10; // env_stratrace_defs.h:39
 
 enum : int
@@ -86,7 +87,7 @@ DR_HD void sample2(const float* arr, float s, float L, float out[2])
     int i0;
     float t;
     wrap_index(s, L, &i0, &t);
-    const int i1 = (i0 + 1) % NS;
+    const int i1 = (i0 + 1) & (NS - 1);
     out[0] = (1.0f - t) * arr[i0 * 2] + t * arr[i1 * 2];
     out[1] = (1.0f - t) * arr[i0 * 2 + 1] + t * arr[i1 * 2 + 1];
 }
@@ -96,7 +97,7 @@ DR_HD float sample1(const float* arr, float s, float L)
     int i0;
     float t;
     wrap_index(s, L, &i0, &t);
-    const int i1 = (i0 + 1) % NS;
+    const int i1 = (i0 + 1) & (NS - 1);
     return (1.0f - t) * arr[i0] + t * arr[i1];
 }
 
@@ -124,11 +125,11 @@ DR_HD float project(const Tables& tb, float x, float y, float prevS, float* lat)
     w = w - __builtin_floorf(w);
     const float fw = w * (float)NS;
     const int idxPrev = (fw >= 0.0f && fw <= (float)NS) ? (int)fw : 0;
-    int best = (idxPrev - WINDOW + NS) % NS;
+    int best = (idxPrev - WINDOW + NS) & (NS - 1); // operands >= 0, NS a power of two
     float bestSq = sq_dist_to_sample(tb.p, best, x, y);
     for (int off = NS - WINDOW + 1; off <= NS + WINDOW; ++off)
     {
-        const int i = (idxPrev + off) % NS;
+        const int i = (idxPrev + off) & (NS - 1);
         const float d = sq_dist_to_sample(tb.p, i, x, y);
         if (d < bestSq)
         {
@@ -139,8 +140,8 @@ DR_HD float project(const Tables& tb, float x, float y, float prevS, float* lat)
     float segSq = __builtin_inff(), bestS = 0.0f, cx = 0.0f, cy = 0.0f;
     for (int seg = -1; seg <= 0; ++seg)
     {
-        const int a = (best + seg + NS) % NS;
-        const int b = (a + 1) % NS;
+        const int a = (best + seg + NS) & (NS - 1);
+        const int b = (a + 1) & (NS - 1);
         const float ax = tb.p[a * 2], ay = tb.p[a * 2 + 1];
         const float abx = tb.p[b * 2] - ax;
         const float aby = tb.p[b * 2 + 1] - ay;
@@ -186,95 +187,126 @@ DR_HD float sdot(const Tables& tb, const St& s, int i)
     return sdot_with(tb, s, i, tang);
 }
 
-// computeOppTargetLatDist for the single opponent (iOpp = 1): the only
-// other agent ahead and behind is the ego (j = 0).
-DR_HD void opp_target(const Tables& tb, const dr_sr_env& E, const dr_sr_opp_config& cfg, const St& s, float* latDis_out, float* speedGoal_out)
+// computeSdot with the unit tangent and the curvature at S[i] already sampled.
+DR_HD float sdot_tk(const St& s, int i, const float tang[2], float k)
 {
-    const float L = tb.L;
-    float dAhead = s.S[0] - s.S[1];
-    if (dAhead < 0.0f)
-        dAhead = dAhead + L;
-    float dBehind = s.S[0] - s.S[1];
-    if (dBehind > 0.0f)
-        dBehind = dBehind - L;
+    float metric = 1.0f - k * s.lat[i];
+    if (__builtin_fabsf(metric) < 1e-12f)
+        metric = __builtin_copysignf(1e-12f, metric);
+    return (tang[0] * s.vel[2 * i] + tang[1] * s.vel[2 * i + 1]) / metric;
+}
 
+// The model-independent part of computeOppTargetLatDist and
+// computeOpponentAction at one state (single opponent, iOpp = 1): track
+// samples at both agents, their progress rates, the opponent's track frame
+// and lateral limits.  Every opponent model's target and action reuse it,
+// and the environment step's progress update reuses the two rates (they read
+// S, lat and vel only, which the position update leaves unchanged).
+struct OppGeom
+{
+    float dAhead, dBehind, negQuarterL, sdot_i, sdot_j, tang_j[2], deltaE, blockTol, oppLongSpeed, maxLat;
+    float tang[2], nx, ny, kappa, e_n, v_t, v_n, e_n_max, metric, a_cent, a_lat_max, d_pos, d_neg;
+};
+
+DR_HD OppGeom opp_geom(const Tables& tb, const dr_sr_env& E, const St& s)
+{
+    OppGeom g;
+    const float L = tb.L;
+    g.dAhead = s.S[0] - s.S[1];
+    if (g.dAhead < 0.0f)
+        g.dAhead = g.dAhead + L;
+    g.dBehind = s.S[0] - s.S[1];
+    if (g.dBehind > 0.0f)
+        g.dBehind = g.dBehind - L;
+    g.negQuarterL = -L * 0.25f;
+    // opponent: tangent and curvature at S[1]
+    const float s_now = s.S[1];
+    float pc[2];
+    sample2(tb.p, s_now, tb.L, pc);
+    sample_tangent(tb, s_now, g.tang);
+    g.kappa = sample1(tb.kappa, s_now, tb.L);
+    g.sdot_i = sdot_tk(s, 1, g.tang, g.kappa);
+    // ego: tangent and curvature at S[0]
+    sample_tangent(tb, s.S[0], g.tang_j);
+    g.sdot_j = sdot_tk(s, 0, g.tang_j, sample1(tb.kappa, s.S[0], tb.L));
+    g.blockTol = E.max_speed[0] * 0.3f;
+    g.deltaE = s.lat[0] - s.lat[1];
+    g.oppLongSpeed = s.vel[0] * g.tang_j[0] + s.vel[1] * g.tang_j[1];
+    g.maxLat = E.track_width - E.drone_radius * E.max_opp_lat_dist_fact;
+
+    const float maxAccel = E.max_accel[1];
+    const float px = s.pos[2], py = s.pos[3];
+    const float vx = s.vel[2], vy = s.vel[3];
+    g.nx = -g.tang[1];
+    g.ny = g.tang[0];
+    g.e_n = (px - pc[0]) * g.nx + (py - pc[1]) * g.ny;
+    g.v_t = vx * g.tang[0] + vy * g.tang[1];
+    g.v_n = vx * g.nx + vy * g.ny;
+    g.e_n_max = E.track_width - E.drone_radius * 1.2f;
+    g.metric = 1.0f - g.kappa * g.e_n;
+    if (__builtin_fabsf(g.metric) < 1e-6f)
+        g.metric = __builtin_copysignf(1e-6f, g.metric);
+    g.a_cent = g.v_t * g.v_t * g.kappa / g.metric;
+    g.a_lat_max = fmaxf(maxAccel - __builtin_fabsf(g.a_cent), maxAccel * 0.2f);
+    g.d_pos = fmaxf(g.e_n_max - g.e_n, 0.001f);
+    g.d_neg = fmaxf(g.e_n_max + g.e_n, 0.001f);
+    return g;
+}
+
+// computeOppTargetLatDist (env_stratrace.h:318-461) of one opponent model:
+// the only other agent ahead and behind is the ego (j = 0).
+DR_HD void opp_target(const dr_sr_env& E, const dr_sr_opp_config& cfg, const St& s, const OppGeom& g, float* latDis_out,
+                      float* speedGoal_out)
+{
     float latDis = 0.0f;
-    const float sdot_i = sdot(tb, s, 1);
     const float baseSpeed = cfg.speed_scale * E.max_speed[1];
     float speedGoal = baseSpeed;
-
-    float tang_j[2];
-    sample_tangent(tb, s.S[0], tang_j);
-    const float sdot_j = sdot_with(tb, s, 0, tang_j);
-    const float blockTol = E.max_speed[0] * 0.3f;
-    const float deltaE = s.lat[0] - s.lat[1];
 #pragma unroll
     for (int k = 0; k <= 1; ++k)
     {
-        const float deltaS = k ? dAhead : dBehind;
-        const bool shouldBlock = (deltaS < 0.0f) && (deltaS > -L * 0.25f) && (sdot_j + blockTol > sdot_i);
-        const bool shouldOvertake = (sdot_i > sdot_j) && (__builtin_fabsf(deltaE) < cfg.s1);
+        const float deltaS = k ? g.dAhead : g.dBehind;
+        const bool shouldBlock = (deltaS < 0.0f) && (deltaS > g.negQuarterL) && (g.sdot_j + g.blockTol > g.sdot_i);
+        const bool shouldOvertake = (g.sdot_i > g.sdot_j) && (__builtin_fabsf(g.deltaE) < cfg.s1);
         const float factor = expf_ieee(-cfg.s2 * deltaS * deltaS);
         if (shouldBlock)
         {
-            const float blockTerm = s.lat[0] * (1.0f - expf_ieee(-cfg.s3 * (sdot_j + blockTol - sdot_i))) * factor;
+            const float blockTerm = s.lat[0] * (1.0f - expf_ieee(-cfg.s3 * (g.sdot_j + g.blockTol - g.sdot_i))) * factor;
             latDis = latDis + blockTerm;
         }
         if (shouldOvertake)
         {
-            const float overtakeTerm = (s.lat[0] - __builtin_copysignf(cfg.s1, deltaE)) * factor;
+            const float overtakeTerm = (s.lat[0] - __builtin_copysignf(cfg.s1, g.deltaE)) * factor;
             latDis = latDis + overtakeTerm;
-            const float oppLongSpeed = s.vel[0] * tang_j[0] + s.vel[1] * tang_j[1];
             if (deltaS < 3.0f * E.drone_radius)
-                speedGoal = fminf(speedGoal, 0.7f * oppLongSpeed);
+                speedGoal = fminf(speedGoal, 0.7f * g.oppLongSpeed);
             else if (deltaS < 6.0f * E.drone_radius && (double)cfg.s1 > 1e-6)
             {
-                const float latFactor = __builtin_fabsf(deltaE) / cfg.s1;
-                speedGoal = fminf(speedGoal, latFactor * oppLongSpeed + (1.0f - latFactor) * baseSpeed);
+                const float latFactor = __builtin_fabsf(g.deltaE) / cfg.s1;
+                speedGoal = fminf(speedGoal, latFactor * g.oppLongSpeed + (1.0f - latFactor) * baseSpeed);
             }
         }
     }
-    const float maxLat = E.track_width - E.drone_radius * E.max_opp_lat_dist_fact;
-    *latDis_out = fmaxf(-maxLat, fminf(maxLat, latDis));
+    *latDis_out = fmaxf(-g.maxLat, fminf(g.maxLat, latDis));
     *speedGoal_out = speedGoal;
 }
 
-// computeOpponentAction (the active "second try" controller) for the opponent.
-DR_HD void opp_action(const Tables& tb, const dr_sr_env& E, const dr_sr_opp_config& cfg, const St& s, float out[2])
+// computeOpponentAction (env_stratrace.h:577-706, the active "second try"
+// controller) of one opponent model, from the shared geometry.
+DR_HD void opp_action_g(const dr_sr_env& E, const dr_sr_opp_config& cfg, const St& s, const OppGeom& g, float out[2])
 {
     const float maxAccel = E.max_accel[1];
     const float dt = E.dt;
     float latDisRaw, speedGoal;
-    opp_target(tb, E, cfg, s, &latDisRaw, &speedGoal);
+    opp_target(E, cfg, s, g, &latDisRaw, &speedGoal);
 
-    const float s_now = s.S[1];
-    const float px = s.pos[2], py = s.pos[3];
-    const float vx = s.vel[2], vy = s.vel[3];
-    float pc[2], tang[2];
-    sample2(tb.p, s_now, tb.L, pc);
-    sample_tangent(tb, s_now, tang);
-    const float kappa = sample1(tb.kappa, s_now, tb.L);
-    const float nx = -tang[1], ny = tang[0];
-    const float e_n = (px - pc[0]) * nx + (py - pc[1]) * ny;
-    const float v_t = vx * tang[0] + vy * tang[1];
-    const float v_n = vx * nx + vy * ny;
-
-    const float e_n_max = E.track_width - E.drone_radius * 1.2f;
-    float metric = 1.0f - kappa * e_n;
-    if (__builtin_fabsf(metric) < 1e-6f)
-        metric = __builtin_copysignf(1e-6f, metric);
-    const float a_cent = v_t * v_t * kappa / metric;
-    const float a_lat_max = fmaxf(maxAccel - __builtin_fabsf(a_cent), maxAccel * 0.2f);
-
-    const float latDis = fmaxf(-e_n_max, fminf(e_n_max, latDisRaw));
-    const float e_err = latDis - e_n;
-    const float d_pos = fmaxf(e_n_max - e_n, 0.001f);
-    const float d_neg = fmaxf(e_n_max + e_n, 0.001f);
+    const float a_lat_max = g.a_lat_max, v_n = g.v_n;
+    const float latDis = fmaxf(-g.e_n_max, fminf(g.e_n_max, latDisRaw));
+    const float e_err = latDis - g.e_n;
 
     float a_n;
-    if (v_n > 0.0f && v_n * v_n > 1.8f * a_lat_max * d_pos)
+    if (v_n > 0.0f && v_n * v_n > 1.8f * a_lat_max * g.d_pos)
         a_n = -a_lat_max;
-    else if (v_n < 0.0f && v_n * v_n > 1.8f * a_lat_max * d_neg)
+    else if (v_n < 0.0f && v_n * v_n > 1.8f * a_lat_max * g.d_neg)
         a_n = a_lat_max;
     else if (__builtin_fabsf(e_err) < 0.02f)
     {
@@ -293,28 +325,35 @@ DR_HD void opp_action(const Tables& tb, const dr_sr_env& E, const dr_sr_opp_conf
         const float v_next = v_n + a_n * dt;
         if (v_next > 0.0f)
         {
-            const float v_safe = __builtin_sqrtf(2.0f * a_lat_max * d_pos);
+            const float v_safe = __builtin_sqrtf(2.0f * a_lat_max * g.d_pos);
             if (v_next > v_safe)
                 a_n = (v_safe - v_n) / dt;
         }
         else
         {
-            const float v_safe = __builtin_sqrtf(2.0f * a_lat_max * d_neg);
+            const float v_safe = __builtin_sqrtf(2.0f * a_lat_max * g.d_neg);
             if (-v_next > v_safe)
                 a_n = (-v_safe - v_n) / dt;
         }
     }
 
-    const float a_n_tot = fmaxf(-maxAccel, fminf(maxAccel, a_n + a_cent));
+    const float kappa = g.kappa, v_t = g.v_t;
+    const float a_n_tot = fmaxf(-maxAccel, fminf(maxAccel, a_n + g.a_cent));
     const float a_t_max = __builtin_sqrtf(fmaxf(maxAccel * maxAccel - a_n_tot * a_n_tot, 0.0f));
     const float kappa_eff = kappa / (1.0f - kappa * latDis + 1e-6f);
     float pathSpeed = fminf(speedGoal, __builtin_sqrtf(maxAccel / fmaxf(__builtin_fabsf(kappa_eff), 1e-12f)));
     if (__builtin_fabsf(a_n) > a_lat_max * 0.5f)
         pathSpeed = pathSpeed * fmaxf(0.4f, 1.5f - __builtin_fabsf(a_n) / a_lat_max);
-    float a_t = E.kv * (pathSpeed - v_t) - v_n * kappa * v_t / metric;
+    float a_t = E.kv * (pathSpeed - v_t) - v_n * kappa * v_t / g.metric;
     a_t = fmaxf(-a_t_max, fminf(a_t_max, a_t));
-    out[0] = a_t * tang[0] + a_n_tot * nx;
-    out[1] = a_t * tang[1] + a_n_tot * ny;
+    out[0] = a_t * g.tang[0] + a_n_tot * g.nx;
+    out[1] = a_t * g.tang[1] + a_n_tot * g.ny;
+}
+
+DR_HD void opp_action(const Tables& tb, const dr_sr_env& E, const dr_sr_opp_config& cfg, const St& s, float out[2])
+{
+    const OppGeom g = opp_geom(tb, E, s);
+    opp_action_g(E, cfg, s, g, out);
 }
 
 // Sequential standard normals of one (sample, step): block b of four from
@@ -373,18 +412,31 @@ DR_HD bool is_opp_winner(const dr_sr_env& E, const St& s)
     return s.laps[1] >= E.n_win_laps;
 }
 
+// Per-model constants of updateBelief: sqSigma = 0.2^2 + actionNoise^2 and
+// log(sqSigma).
+DR_HD void belief_consts(const dr_sr_env& E, int th, float* sq_sigma, float* log_sq)
+{
+    const float sigmaEnv = 0.2f;
+    const float an = E.opp[th].action_noise;
+    *sq_sigma = sigmaEnv * sigmaEnv + an * an;
+    *log_sq = logf_ieee(*sq_sigma);
+}
+
 // updateBelief (env_stratrace.h:166-222) for one opponent: the observed
 // opponent action against each model's nominal action (scaled to the
 // opponent's max acceleration), Gaussian likelihood with variance
 // 0.2^2 + actionNoise^2, product and normalisation (uniform below 1e-20).
-// log and exp are the build's (dr_rng.h).
-DR_HD void update_belief(const dr_sr_env& E, float* belief, const float obs[2], const float (*nom)[2])
+// log and exp are the build's (dr_rng.h).  bc = {sqSigma[4], log sqSigma[4]}
+// from belief_consts, or NULL to form them here.
+DR_HD void update_belief(const dr_sr_env& E, float* belief, const float obs[2], const float (*nom)[2], const float* bc = nullptr)
 {
-    const float sigmaEnv = 0.2f;
     const float log2pi = logf_ieee(2.0f * 3.14159265358979323846f);
     float sum = 0.0f;
-    for (int th = 0; th < E.n_models; ++th)
+#pragma unroll
+    for (int th = 0; th < DR_SR_MAX_MODELS; ++th)
     {
+        if (th >= E.n_models)
+            break;
         const float sqA = nom[th][0] * nom[th][0] + nom[th][1] * nom[th][1];
         float scale = 1.0f;
         if (sqA > E.max_accel[1] * E.max_accel[1])
@@ -392,21 +444,31 @@ DR_HD void update_belief(const dr_sr_env& E, float* belief, const float obs[2], 
         const float dx = obs[0] - nom[th][0] * scale;
         const float dy = obs[1] - nom[th][1] * scale;
         const float sqDist = dx * dx + dy * dy;
-        const float an = E.opp[th].action_noise;
-        const float sqSigma = sigmaEnv * sigmaEnv + an * an;
-        const float ll = -0.5f * sqDist / sqSigma - log2pi - logf_ieee(sqSigma);
+        float sqSigma, logSq;
+        if (bc)
+        {
+            sqSigma = bc[th];
+            logSq = bc[DR_SR_MAX_MODELS + th];
+        }
+        else
+            belief_consts(E, th, &sqSigma, &logSq);
+        const float ll = -0.5f * sqDist / sqSigma - log2pi - logSq;
         belief[th] = belief[th] * expf_ieee(ll);
         sum = sum + belief[th];
     }
     if (sum < 1e-20f)
     {
         const float uniform = 1.0f / (float)E.n_models;
-        for (int th = 0; th < E.n_models; ++th)
-            belief[th] = uniform;
+#pragma unroll
+        for (int th = 0; th < DR_SR_MAX_MODELS; ++th)
+            if (th < E.n_models)
+                belief[th] = uniform;
     }
     else
-        for (int th = 0; th < E.n_models; ++th)
-            belief[th] = belief[th] / sum;
+#pragma unroll
+        for (int th = 0; th < DR_SR_MAX_MODELS; ++th)
+            if (th < E.n_models)
+                belief[th] = belief[th] / sum;
 }
 
 // findConfident (state.h:93-115) for one model factor: theta + 1 for the
@@ -420,32 +482,41 @@ DR_HD int find_confident(const float* belief, int n_models, float threshold)
 }
 
 // environmentStep<BELIEF, false, false, lose_on_opp_win> with applyNoise =
-// true.  With BELIEF the nominal opponent action of every model is formed
-// and the belief is updated from the opponent's applied action.
+// true.  With BELIEF and `update`, the nominal opponent action of every
+// model is formed and the belief is updated from the opponent's applied
+// action; with `update` false the step is the plain one (a caller that no
+// longer reads the belief skips its update).  bc: see update_belief.
 template <bool BELIEF>
 DR_HD int env_step_t(const Tables& tb, const dr_sr_env& E, int theta, const float ego[2], St& s, NormalSeq& rng, bool lose_on_opp_win,
-                     float* belief)
+                     float* belief, const float* bc = nullptr, bool update = true)
 {
     const float prevS0 = s.S[0], prevS1 = s.S[1];
+    const OppGeom g = opp_geom(tb, E, s);
     float act[4];
     act[0] = ego[0];
     act[1] = ego[1];
     float noms[DR_SR_MAX_MODELS][2];
-    if (BELIEF)
+    const bool upd = BELIEF && update;
+    float nom[2];
+    if (upd)
     {
-        for (int th = 0; th < E.n_models; ++th)
-            opp_action(tb, E, E.opp[th], s, noms[th]);
+        // constant indices only: the model tables stay in registers
+#pragma unroll
+        for (int th = 0; th < DR_SR_MAX_MODELS; ++th)
+            if (th < E.n_models)
+            {
+                opp_action_g(E, E.opp[th], s, g, noms[th]);
+                if (th == theta)
+                {
+                    nom[0] = noms[th][0];
+                    nom[1] = noms[th][1];
+                }
+            }
     }
+    else
+        opp_action_g(E, E.opp[theta], s, g, nom);
     {
         const dr_sr_opp_config& cfg = E.opp[theta];
-        float nom[2];
-        if (BELIEF)
-        {
-            nom[0] = noms[theta][0];
-            nom[1] = noms[theta][1];
-        }
-        else
-            opp_action(tb, E, cfg, s, nom);
 #pragma unroll
         for (int d = 0; d < 2; ++d)
         {
@@ -473,12 +544,10 @@ DR_HD int env_step_t(const Tables& tb, const dr_sr_env& E, int theta, const floa
     }
     for (int i = 0; i < 4; ++i)
         s.pos[i] = s.pos[i] + E.dt * s.vel[i];
-    {
-        const float sd0 = sdot(tb, s, 0);
-        s.S[0] = s.S[0] + E.dt * sd0;
-        const float sd1 = sdot(tb, s, 1);
-        s.S[1] = s.S[1] + E.dt * sd1;
-    }
+    // progress rates of the state before the position update (computeSdot
+    // reads S, lat and vel only)
+    s.S[0] = s.S[0] + E.dt * g.sdot_j;
+    s.S[1] = s.S[1] + E.dt * g.sdot_i;
     for (int i = 0; i < 4; ++i)
         s.vel[i] = s.vel[i] + E.dt * act[i];
 #pragma unroll
@@ -517,10 +586,10 @@ DR_HD int env_step_t(const Tables& tb, const dr_sr_env& E, int theta, const floa
         if (s.S[a] > prev[a] + tb.L * 0.5f)
             s.laps[a] = s.laps[a] - 1;
     }
-    if (BELIEF)
+    if (upd)
     {
         const float obs[2] = {act[2], act[3]};
-        update_belief(E, belief, obs, noms);
+        update_belief(E, belief, obs, noms, bc);
     }
     if (boundary_dist(E, s, lose_on_opp_win) < E.drone_radius)
         return SR_LOSE;

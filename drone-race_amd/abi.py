@@ -187,6 +187,7 @@ SIGNATURES = {
     "dr_pr_set_nominals": (_I, [_P, _I, _F, _F, _I]),
     "dr_pr_get_nominals": (_I, [_P, _I, _F, _F, _I]),
     "dr_pr_set_belief": (_I, [_P, _I, _F, _I]),
+    "dr_pr_get_belief": (_I, [_P, _I, _F, _I]),
     "dr_pr_step": (_I, [_P, _I, _I, _F]),
     "dr_pr_get_diag": (_I, [_P, C.POINTER(PrDiag), _I]),
     "dr_pr_get_noise": (_I, [_P, _I, _F, C.c_size_t]),

@@ -39,7 +39,7 @@ hipError_t launch_prsr_rollout(int waves, int grid, hipStream_t st, const PrInst
 hipError_t launch_prsr_update(int n_inst, hipStream_t st, const PrInstDev* insts, PrDyn* dyn, const double* part, const double* den,
                               float* work, float* plan, float* cand, int* models, float* actions, const dr_sr_env* env, const dr_sr_cost& KC,
                               const dr_sr_state* states);
-hipError_t launch_prsr_advance(int n_inst, hipStream_t st, const PrInstDev* insts, const PrDyn* dyn, const dr_sr_env* env, dr_sr_state* states,
+hipError_t launch_prsr_advance(int n_inst, hipStream_t st, const PrInstDev* insts, PrDyn* dyn, const dr_sr_env* env, dr_sr_state* states,
                                const float* actions, int* events);
 
 } // namespace dr
@@ -664,6 +664,19 @@ int dr_pr_set_belief(dr_prengine* e, int instance, const float* belief, int n_br
     float b[DR_MAX_BRANCHES] = {0};
     std::memcpy(b, belief, n_branches * sizeof(float));
     return copy_in(e, e->d_dyn.p[instance].belief, b, (size_t)DR_MAX_BRANCHES);
+}
+
+int dr_pr_get_belief(dr_prengine* e, int instance, float* belief, int n_branches)
+{
+    if (int rc = check_pr_inst(e, instance))
+        return rc;
+    if (!belief || n_branches != e->idev[instance].B)
+        return host_fail(DR_EINVAL, "get_belief: expected n_branches values");
+    float b[DR_MAX_BRANCHES];
+    if (int rc = copy_out(e, b, e->d_dyn.p[instance].belief, (size_t)DR_MAX_BRANCHES))
+        return rc;
+    std::memcpy(belief, b, n_branches * sizeof(float));
+    return DR_OK;
 }
 
 int dr_pr_step_async(dr_prengine* e, int n_steps, int advance)

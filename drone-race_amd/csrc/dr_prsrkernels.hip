@@ -204,7 +204,7 @@ __global__ __launch_bounds__(256) void k_prsr_update(const PrInstDev* __restrict
 // Closed loop of each instance's world (SimulationEngine::dynStep with
 // loseOnOppWin = false, environment noise at counter (2^32 - 1, 4 n + b, 0,
 // STREAM_ENV) for control step n); a world stops at its first event.
-__global__ __launch_bounds__(64) void k_prsr_advance(const PrInstDev* __restrict__ insts, const PrDyn* __restrict__ dyn,
+__global__ __launch_bounds__(64) void k_prsr_advance(const PrInstDev* __restrict__ insts, PrDyn* __restrict__ dyn,
                                                      const dr_sr_env* __restrict__ env, dr_sr_state* __restrict__ states,
                                                      const float* __restrict__ actions, int* __restrict__ events, int n_inst)
 {
@@ -217,8 +217,16 @@ __global__ __launch_bounds__(64) void k_prsr_advance(const PrInstDev* __restrict
     St s = state_from_abi(states[I.drone]);
     NormalSeq rng = normal_seq(I.key0, I.key1, 0xFFFFFFFFu, dyn[i].step - 1u, 0u);
     const float u[2] = {actions[i * 4], actions[i * 4 + 1]};
-    const int term = env_step(tb, E, I.world_theta, u, s, rng, false);
+    // SimulationEngine::dynStep (engine.cpp:286-312): the controller's belief
+    // is updated from the world opponent's applied action
+    float belief[DR_SR_MAX_MODELS];
+    for (int th = 0; th < DR_SR_MAX_MODELS; ++th)
+        belief[th] = dyn[i].belief[th];
+    const int term = env_step_t<true>(tb, E, I.world_theta, u, s, rng, false, belief);
     state_to_abi(s, states[I.drone]);
+    for (int th = 0; th < DR_SR_MAX_MODELS; ++th)
+        if (th < E.n_models)
+            dyn[i].belief[th] = belief[th];
     events[i] = term;
 }
 
@@ -244,7 +252,7 @@ hipError_t launch_prsr_update(int n_inst, hipStream_t st, const PrInstDev* insts
     return hipGetLastError();
 }
 
-hipError_t launch_prsr_advance(int n_inst, hipStream_t st, const PrInstDev* insts, const PrDyn* dyn, const dr_sr_env* env, dr_sr_state* states,
+hipError_t launch_prsr_advance(int n_inst, hipStream_t st, const PrInstDev* insts, PrDyn* dyn, const dr_sr_env* env, dr_sr_state* states,
                                const float* actions, int* events)
 {
     k_prsr_advance<<<(n_inst + 63) / 64, 64, 0, st>>>(insts, dyn, env, states, actions, events, n_inst);

@@ -261,6 +261,11 @@ class PREngine:
         a = np.ascontiguousarray(belief, dtype=np.float32)
         check(self.lib.dr_pr_set_belief(self.h, i, _fptr(a), a.size))
 
+    def belief(self, i, n_branches):
+        a = np.empty(n_branches, dtype=np.float32)
+        check(self.lib.dr_pr_get_belief(self.h, i, _fptr(a), a.size))
+        return a
+
     def step(self, n_steps=1, advance=False):
         a = np.empty(self.n_inst * 4, dtype=np.float32)
         check(self.lib.dr_pr_step(self.h, n_steps, 1 if advance else 0, _fptr(a)))

@@ -287,6 +287,9 @@ int dr_pr_set_nominals(dr_prengine* e, int instance, const float* nom, const flo
 /* The plans after the last step (unshifted, as h_nom_nominal / h_rob_nominal). */
 int dr_pr_get_nominals(dr_prengine* e, int instance, float* nom, float* rob, int n_floats);
 int dr_pr_set_belief(dr_prengine* e, int instance, const float* belief, int n_branches);
+/* The belief as the engine holds it (on StratRace2D the closed-loop world step updates it, as
+ * SimulationEngine::dynStep updates h_belief, engine.cpp:286-312). */
+int dr_pr_get_belief(dr_prengine* e, int instance, float* belief, int n_branches);
 int dr_pr_step(dr_prengine* e, int n_steps, int advance, float* actions);
 int dr_pr_get_diag(dr_prengine* e, dr_pr_diag* diag, int n_instances);
 /* Results of the last step for tests: noise [k][t][4]; per-sample

@@ -32,10 +32,10 @@ def shifted(plan, fill=(0.0, 0.0)):
     return np.vstack([plan[1:], np.asarray(fill, np.float32)[None]]).astype(np.float32)
 
 
-def check_step(oracle, eng, sc, states, plans, acts, step=0):
+def check_step(oracle, eng, sc, states, plans, acts, step=0, beliefs=None):
     diag = eng.diag()
     for i, ins in enumerate(sc.instances):
-        belief = np.array(ins.belief[:sc.env.n_models], np.float32)
+        belief = np.array(ins.belief[:sc.env.n_models], np.float32) if beliefs is None else beliefs[i]
         lam = [ins.lambda_] * 3
         nom_w, rob_w = shifted(plans[i][0]), shifted(plans[i][1])
         r = oracle.prsr_step(0, sc.env, sc.cost, states[i], ins, belief, step, lam, nom_w, rob_w)
@@ -83,26 +83,30 @@ def test_prsr_step_parity(pkg, oracle, sampled):
 
 
 def test_prsr_closed_loop(pkg, oracle):
-    say("[pr-sr closed loop] 2 worlds, 4 steps")
+    say("[pr-sr closed loop] 2 worlds, 4 steps, belief updated by the world step")
     sc = pkg.scenarios.pr_sr_race(n_inst=2, n_samples=512, n_timesteps=25)
     eng = pkg.PRSREngine(sc)
+    NT = sc.env.n_models
     plans = [(np.zeros((25, 2), np.float32), np.zeros((25, 2), np.float32)) for _ in range(2)]
     states = [copy.deepcopy(s) for s in eng.get_states()]
+    beliefs = [np.array(ins.belief[:NT], np.float32) for ins in sc.instances]
     for n in range(4):
         acts = eng.step(1, advance=True)
-        check_step(oracle, eng, sc, states, plans, acts, step=n)
+        check_step(oracle, eng, sc, states, plans, acts, step=n, beliefs=beliefs)
         g = eng.get_states()
         ev = eng.events()
         for i, ins in enumerate(sc.instances):
-            s2, term = oracle.sr_env_step(0, sc.env, ins.true_theta, acts[i], states[i], ins.seed, 0xFFFFFFFF, n, 0,
-                                          lose_on_opp_win=False)
+            s2, b2, term = oracle.sr_env_step_belief(0, sc.env, ins.true_theta, acts[i], states[i], beliefs[i], ins.seed, 0xFFFFFFFF, n,
+                                                     0, lose_on_opp_win=False)
             got = np.array(list(g[i].pos) + list(g[i].vel) + list(g[i].s) + list(g[i].lat_dist))
             want = np.array(list(s2.pos) + list(s2.vel) + list(s2.s) + list(s2.lat_dist))
             np.testing.assert_array_equal(got, want)
+            assert np.array_equal(eng.belief(i, NT).view(np.uint32), b2.view(np.uint32)), "belief not bit-exact"
             assert ev[i] == term
-            states[i] = s2
+            states[i], beliefs[i] = s2, b2
         plans = [eng.get_nominals(i) for i in range(2)]
-        say(f"  step {n}: worlds identical to the oracle")
+        say(f"  step {n}: worlds and beliefs identical to the oracle; beliefs "
+            + " ".join(np.array2string(b, precision=3) for b in beliefs))
 
 
 def test_prsr_reference_size_step(pkg, oracle):

@@ -5,10 +5,12 @@ demand: `from drone_race_amd.autograd import rollout_cost`.
 
 The compute path is the HIP library libdr_amd.so (csrc/, C ABI in
 include/dr_api.h); this package is its Python host side: ctypes bindings,
-scenario builders and the engine handles.  There is no CPU fallback.
+scenario builders, the engine handles and the wire protocol / simulation
+session of the reference's ZMQ front-end (wire.py, serve.py).  There is no
+CPU fallback.
 """
-from . import abi, philox, scenarios
+from . import abi, philox, scenarios, wire
 from .abi import EngineError, load
 from .engine import BMEngine, Engine, MPPI, PREngine, PRSREngine, SREngine
 
-__all__ = ["abi", "philox", "scenarios", "BMEngine", "Engine", "MPPI", "PREngine", "PRSREngine", "SREngine", "EngineError", "load"]
+__all__ = ["abi", "philox", "scenarios", "wire", "BMEngine", "Engine", "MPPI", "PREngine", "PRSREngine", "SREngine", "EngineError", "load"]

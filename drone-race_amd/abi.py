@@ -140,6 +140,25 @@ class BmDiag(C.Structure):
                 ("branch", C.c_int32), ("_pad", C.c_int32)]
 
 
+MSG_HEADER, MSG_STATE, MSG_EVENT, MSG_DONE = 0, 1, 2, 3
+EVT_OUTSIDE, EVT_WINNER, EVT_OPP_WINNER, EVT_TRUNCATED = 0, 1, 2, 3
+CONT_MPPI, CONT_PRMPPI = 0, 1
+ENV_STRATRACE = 2
+
+
+class WireEnvInfo(C.Structure):
+    _fields_ = [("seed", C.c_int32), ("true_theta", C.c_int32), ("send_states", C.c_int32), ("n_models", C.c_int32),
+                ("init_belief", C.c_float * SR_MAX_MODELS), ("arena_min", C.c_float * 2), ("arena_max", C.c_float * 2)]
+
+
+class WireController(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("n_samples", C.c_int32), ("n_timesteps", C.c_int32), ("n_knots", C.c_int32),
+                ("knots", C.c_int32 * MAX_KNOTS), ("use_splines", C.c_int32), ("lambda_", C.c_float), ("sigma", C.c_float),
+                ("gate_traversal_margin", C.c_float), ("cost", SrCost), ("min_confidence", C.c_float), ("n_verif_samples", C.c_int32),
+                ("beta", C.c_float), ("verif_horizon", C.c_int32), ("max_verif_eps", C.c_float), ("safety_weight", C.c_float),
+                ("min_safe_dist", C.c_float), ("delta", C.c_float), ("P", C.c_int32)]
+
+
 # Every symbol include/dr_api.h declares, with its ctypes signature.
 _P = C.c_void_p
 _I = C.c_int
@@ -220,6 +239,12 @@ SIGNATURES = {
     "dr_pr_set_sr_states": (_I, [_P, C.POINTER(SrState), _I]),
     "dr_pr_get_sr_states": (_I, [_P, C.POINTER(SrState), _I]),
     "dr_pr_get_events": (_I, [_P, C.POINTER(C.c_int32), _I]),
+    "dr_wire_decode_env": (_I, [C.c_void_p, C.c_size_t, C.POINTER(SrEnv), C.POINTER(WireEnvInfo)]),
+    "dr_wire_decode_controller": (_I, [C.c_void_p, C.c_size_t, C.POINTER(WireController)]),
+    "dr_wire_decode_state": (_I, [C.c_void_p, C.c_size_t, C.POINTER(SrState)]),
+    "dr_session_create": (_I, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int32, C.POINTER(_P)]),
+    "dr_session_destroy": (None, [_P]),
+    "dr_session_next": (_I, [_P, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t), C.POINTER(C.c_int32)]),
     "dr_bm_create": (_I, [C.POINTER(SrEnv), C.POINTER(SrCost), C.POINTER(BmInstance), _I, C.POINTER(_P)]),
     "dr_bm_destroy": (None, [_P]),
     "dr_bm_set_states": (_I, [_P, C.POINTER(SrState), _I]),

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdr_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["dr_kernels.hip", "dr_engine.hip", "dr_prkernels.hip", "dr_prengine.hip", "dr_srkernels.hip", "dr_srengine.hip", "dr_prsrkernels.hip", "dr_bmkernels.hip", "dr_bmengine.hip"]
+SOURCES = ["dr_kernels.hip", "dr_engine.hip", "dr_prkernels.hip", "dr_prengine.hip", "dr_srkernels.hip", "dr_srengine.hip",
+           "dr_prsrkernels.hip", "dr_bmkernels.hip", "dr_bmengine.hip", "dr_session.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",

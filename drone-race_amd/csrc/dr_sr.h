@@ -8,8 +8,10 @@
 //   computeSdot                                :302-315
 //   computeOppTargetLatDist (one opponent)     :318-461
 //   computeOpponentAction                      :577-706
-//   environmentStep (no belief update)         :711-894
-// and the running/terminal costs of include/costs.cuh:7-39.
+//   updateBelief                               :166-222
+//   environmentStep                            :711-894
+// and the running/terminal costs of include/costs.cuh:7-39 (and the PR-MPPI
+// costs :44-86); findConfident of include/state.h:93-115.
 //
 // Arithmetic follows the reference expression by expression (compiled with
 // -ffp-contract=off, so a*b + c is two roundings); the CPU oracle repeats
@@ -19,9 +21,10 @@
 //   * powf(x, 2) is x * x (one rounding, the exact square rounded);
 //   * a track index that would reach 512 (s / L rounding up to 1) is held at
 //     511, where the reference reads one past its grid;
-//   * the opponent's nominal action is computed for the followed model only:
-//     the other models' actions feed the belief update alone, which the
-//     plain-MPPI path does not run;
+//   * the opponent's nominal action is computed for the followed model only
+//     where no belief is updated (the other models' actions feed the belief
+//     update alone); the model-independent part of the opponent controller
+//     is formed once per step and shared;
 //   * the tangent and curvature sampled and never used at env_stratrace.h:368
 //     and :783-786 are skipped,
 //     and computeSdot reuses the tangent its caller has just sampled at the
@@ -482,13 +485,15 @@ DR_HD int find_confident(const float* belief, int n_models, float threshold)
 }
 
 // environmentStep<BELIEF, false, false, lose_on_opp_win> with applyNoise =
-// true.  With BELIEF and `update`, the nominal opponent action of every
-// model is formed and the belief is updated from the opponent's applied
-// action; with `update` false the step is the plain one (a caller that no
-// longer reads the belief skips its update).  bc: see update_belief.
+// apply_noise (every noise draw is skipped when false: the display
+// predictions of SimulationEngine::sendState).  With BELIEF and `update`, the
+// nominal opponent action of every model is formed and the belief is
+// updated from the opponent's applied action; with `update` false the step
+// is the plain one (a caller that no longer reads the belief skips its
+// update).  bc: see update_belief.
 template <bool BELIEF>
 DR_HD int env_step_t(const Tables& tb, const dr_sr_env& E, int theta, const float ego[2], St& s, NormalSeq& rng, bool lose_on_opp_win,
-                     float* belief, const float* bc = nullptr, bool update = true)
+                     float* belief, const float* bc = nullptr, bool update = true, bool apply_noise = true)
 {
     const float prevS0 = s.S[0], prevS1 = s.S[1];
     const OppGeom g = opp_geom(tb, E, s);
@@ -521,7 +526,7 @@ DR_HD int env_step_t(const Tables& tb, const dr_sr_env& E, int theta, const floa
         for (int d = 0; d < 2; ++d)
         {
             float u = nom[d];
-            if (cfg.action_noise != 0.0f)
+            if (apply_noise && cfg.action_noise != 0.0f)
                 u = u + cfg.action_noise * rng.next();
             act[2 + d] = u;
         }
@@ -537,7 +542,7 @@ DR_HD int env_step_t(const Tables& tb, const dr_sr_env& E, int theta, const floa
         for (int d = 0; d < 2; ++d)
         {
             float v = act[2 * a + d] * factor;
-            if (E.action_noise != 0.0f)
+            if (apply_noise && E.action_noise != 0.0f)
                 v = v + rng.next() * E.action_noise;
             act[2 * a + d] = v;
         }
@@ -561,7 +566,7 @@ DR_HD int env_step_t(const Tables& tb, const dr_sr_env& E, int theta, const floa
             s.vel[2 * a + 1] = s.vel[2 * a + 1] * sc;
         }
     }
-    if (E.pos_noise != 0.0f || E.speed_noise != 0.0f)
+    if (apply_noise && (E.pos_noise != 0.0f || E.speed_noise != 0.0f))
 #pragma unroll
         for (int i = 0; i < 4; ++i)
         {

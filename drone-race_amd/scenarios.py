@@ -244,10 +244,11 @@ def c3_pr_four_drones(seed=2, n_samples=8192, n_timesteps=80, steps=500, sampled
 # ---------------------------------------------------------------------------
 # StratRace2D (the reference's compiled 2-D race)
 
-def sr_track_grids(track_fn, n=512):
+def sr_track_grids(track_fn, n=512, with_dp=False):
     """Track tables as the reference's StratRaceEnvironmentConfig.__post_init__ builds them
     (python-src/utils.py:382-428): polyline of track_fn(k / n), arc length, periodic cubic
-    splines x(s), y(s) on a uniform arc-length grid; unit tangent; signed curvature."""
+    splines x(s), y(s) on a uniform arc-length grid; unit tangent; signed curvature
+    (and the first derivatives dp when with_dp)."""
     from scipy.interpolate import CubicSpline
     q = np.array([track_fn(k / n) for k in range(n)], dtype=np.float64)
     d = np.linalg.norm(np.diff(q, axis=0, append=q[0:1]), axis=1)
@@ -264,6 +265,8 @@ def sr_track_grids(track_fn, n=512):
     t = dp / np.maximum(np.linalg.norm(dp, axis=1, keepdims=True), 1e-6)
     ddx, ddy = cx(s, 2), cy(s, 2)
     kappa = (dx * ddy - dy * ddx) / np.maximum((dx ** 2 + dy ** 2) ** 1.5, 1e-6)
+    if with_dp:
+        return L, p, t, kappa, dp
     return L, p, t, kappa
 
 

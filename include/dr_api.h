@@ -519,6 +519,74 @@ int dr_bm_set_timing(dr_bm_engine* e, int enable);
 /* Device ms summed over the last timed run: rollout, aggregation + masked minima, weighted averages, update + world, step. */
 int dr_bm_last_timing(dr_bm_engine* e, float* ms, int n);
 
+/* ---------------------------------------------------------------------
+ * Wire protocol and simulation session of the reference's front-end for the
+ * StratRace2D build: the byte codec (include/protocol.h:9-142,
+ * src/protocol.cpp, python-src/protocol.py:13-116), the header decoders
+ * (src/environments/env_stratrace.cpp:98-226, src/config.cpp:5-115) and
+ * SimulationEngine (include/engine.h, src/engine.cpp:14-363).  Frames are
+ * little-endian; an array is an int32 length followed by that many float32
+ * values (integer arrays included). */
+enum { DR_MSG_HEADER = 0, DR_MSG_STATE = 1, DR_MSG_EVENT = 2, DR_MSG_DONE = 3 };                /* MsgType, protocol.h:9-15 */
+enum { DR_EVT_OUTSIDE = 0, DR_EVT_WINNER = 1, DR_EVT_OPP_WINNER = 2, DR_EVT_TRUNCATED = 3 };  /* EventType, protocol.h:17-23 */
+enum { DR_CONT_MPPI = 0, DR_CONT_PRMPPI = 1 };                                               /* CONTROLLER_KIND, utils.py:31-34 */
+#define DR_ENV_STRATRACE 2                                                                    /* ENV_KIND, utils.py:37-41 */
+
+/* Environment-header fields that are not part of dr_sr_env. */
+typedef struct dr_wire_env_info
+{
+    int32_t seed;          /* controller and world seed */
+    int32_t true_theta;    /* opponent model of the world */
+    int32_t send_states;   /* 0: no STATE frames */
+    int32_t n_models;      /* modelSizes[0] */
+    float init_belief[DR_SR_MAX_MODELS];
+    float arena_min[2];
+    float arena_max[2];
+} dr_wire_env_info;
+
+/* Controller header (MPPIConfig / PRMPPIConfig, config.h:116-192). */
+typedef struct dr_wire_controller
+{
+    int32_t kind;                   /* DR_CONT_MPPI or DR_CONT_PRMPPI */
+    int32_t n_samples;
+    int32_t n_timesteps;
+    int32_t n_knots;                /* MPPI */
+    int32_t knots[DR_MAX_KNOTS];    /* MPPI */
+    int32_t use_splines;            /* MPPI; 1 (the reference is built with USE_SPLINES) */
+    float lambda;                   /* invTemperature */
+    float sigma;                    /* samplingNoise */
+    float gate_traversal_margin;    /* unused by StratRace2D */
+    dr_sr_cost cost;                /* outside_cost is MPPI only */
+    float min_confidence;           /* MPPI */
+    int32_t n_verif_samples;        /* MPPI plan verification (not run) */
+    float beta;
+    int32_t verif_horizon;
+    float max_verif_eps;
+    float safety_weight;            /* PR-MPPI */
+    float min_safe_dist;            /* PR-MPPI, unused by StratRace2D */
+    float delta;                    /* PR-MPPI */
+    int32_t P;                      /* PR-MPPI models drawn per step */
+} dr_wire_controller;
+
+/* Header decoders (unpackEnvConfig env_stratrace.cpp:127-226, loadControllerConfig config.cpp:86-115,
+ * unpackSimState env_stratrace.cpp:98-118): DR_EINVAL with the reason in dr_last_error() on any
+ * mismatch of message type, kind, sizes, array lengths or trailing bytes. */
+int dr_wire_decode_env(const void* frame, size_t len, dr_sr_env* env, dr_wire_env_info* info);
+int dr_wire_decode_controller(const void* frame, size_t len, dr_wire_controller* cont);
+int dr_wire_decode_state(const void* frame, size_t len, dr_sr_state* state);
+
+/* SimulationEngine(envConfig, contConfig, initState, seed, trueTheta) from the three header frames
+ * (protocol.py:225-229); the MPPI header selects the branching spline engine (dr_bm_*), the PR-MPPI
+ * header the StratRace2D PR-MPPI engine (dr_prsr_create).  max_steps <= 0 selects 500 (main.cpp:11). */
+typedef struct dr_session dr_session;
+int dr_session_create(const void* env_frame, size_t env_len, const void* cont_frame, size_t cont_len, const void* state_frame,
+                      size_t state_len, int32_t max_steps, dr_session** out);
+void dr_session_destroy(dr_session* s);
+/* The next frame SimulationEngine::run sends (engine.cpp:314-352): the step-0 STATE, then per control
+ * step getControl + dynStep + STATE, an EVENT on termination or at max_steps, then DONE.  *frame stays
+ * valid until the next call on s; after DONE, *frame = NULL, *len = 0 and *msg_type = -1. */
+int dr_session_next(dr_session* s, const uint8_t** frame, size_t* len, int32_t* msg_type);
+
 /* Single-controller entry points with the reference's shape. */
 typedef struct dr_mppi_cfg
 {

@@ -444,11 +444,11 @@ int find_confident(const S* belief, int NT, float threshold)
     return 0;
 }
 
-// environmentStep<belief != NULL, false, false, lose>(applyNoise = true);
+// environmentStep<belief != NULL, false, false, lose>(applyNoise = noise);
 // returns 0 none, 1 lose, 2 win, 3 opponent wins.  With a belief, it is
 // updated from the applied opponent action (env_stratrace.h:856-859).
 template <class S>
-int env_step(const Env<S>& E, int theta, const S ego[2], SState<S>& s, Normals& rng, bool lose, S* belief = nullptr)
+int env_step(const Env<S>& E, int theta, const S ego[2], SState<S>& s, Normals& rng, bool lose, S* belief = nullptr, bool noise = true)
 {
     const S prevS[2] = {s.Sa[0], s.Sa[1]};
     S act[4] = {ego[0], ego[1], 0, 0};
@@ -462,7 +462,7 @@ int env_step(const Env<S>& E, int theta, const S ego[2], SState<S>& s, Normals& 
     for (int d = 0; d < 2; ++d)
     {
         S u = nom[d];
-        if (c.action_noise != 0.0f)
+        if (noise && c.action_noise != 0.0f)
             u = u + (S)c.action_noise * (S)rng.next();
         act[2 + d] = u;
     }
@@ -478,7 +478,7 @@ int env_step(const Env<S>& E, int theta, const S ego[2], SState<S>& s, Normals& 
         for (int d = 0; d < 2; ++d)
         {
             S v = act[2 * a + d] * factor;
-            if (E.e.action_noise != 0.0f)
+            if (noise && E.e.action_noise != 0.0f)
                 v = v + (S)rng.next() * (S)E.e.action_noise;
             act[2 * a + d] = v;
         }
@@ -502,7 +502,7 @@ int env_step(const Env<S>& E, int theta, const S ego[2], SState<S>& s, Normals& 
             s.vel[2 * a + 1] = s.vel[2 * a + 1] * sc;
         }
     }
-    if (E.e.pos_noise != 0.0f || E.e.speed_noise != 0.0f)
+    if (noise && (E.e.pos_noise != 0.0f || E.e.speed_noise != 0.0f))
         for (int a = 0; a < 2; ++a)
             for (int d = 0; d < 2; ++d)
             {
@@ -1471,6 +1471,89 @@ int or_bm_step(int f64, const dr_sr_env* env, const dr_sr_cost* K, const dr_sr_s
         bm_step<float>(*env, *K, *x0, *inst, belief, step, lambda, knots, n_threads, noise_out, ctrue_out, used_out, btime_out, costs_out,
                        mins_out, nu_out, knots_out, dense_out, res_out);
     return 0;
+}
+
+// Display predictions of SimulationEngine::sendState (engine.cpp:96-166 for
+// MPPI, :168-257 for PR-MPPI): the world from x0 under model theta, no noise,
+// the opponent's win not a loss.  branching: MPPI (plans [NB][T][2], action
+// of the current branch at local time t - t_origin, belief update and
+// branching with min_conf); otherwise one plan [T][2] and no belief.
+// full_pos [T][4] (the stop state repeated after a terminal step), actions
+// [T][2] (zero after a terminal step); info = initial branch, branching time, final branch, stop reason
+// (event code, 3 = truncated), stop time (-1 = none).
+int or_sr_predict(const dr_sr_env* env, const dr_sr_state* x0, int theta, const float* plans, int T, int branching, const float* belief,
+                  float min_conf, float* full_pos, float* actions, int* info)
+{
+    if (!env || !x0 || !plans || !full_pos || !actions || !info || T < 1)
+        return 1;
+    const Env<float> E(*env);
+    const int NT = env->n_models;
+    SState<float> s = load<float>(*x0);
+    float b[DR_SR_MAX_MODELS] = {0, 0, 0, 0};
+    int pred = 0, time = -1;
+    if (branching)
+    {
+        for (int q = 0; q < NT; ++q)
+            b[q] = belief[q];
+        pred = find_confident(b, NT, min_conf);
+        time = pred == 0 ? -1 : 0;
+    }
+    info[0] = pred;
+    int origin = 0, stopReason = 3, stopTime = -1;
+    Normals rng{0, 0, 0, 0, 0};
+    int t = 0;
+    for (; t < T; ++t)
+    {
+        const float* u = branching ? plans + ((size_t)pred * T + (t - origin)) * 2 : plans + (size_t)t * 2;
+        actions[t * 2] = u[0];
+        actions[t * 2 + 1] = u[1];
+        const float ego[2] = {u[0], u[1]};
+        const int term = env_step(E, theta, ego, s, rng, false, branching ? b : nullptr, false);
+        if (branching && pred == 0)
+        {
+            const int np = find_confident(b, NT, min_conf);
+            if (np != 0)
+            {
+                pred = np;
+                time = t + 1;
+                origin = t + 1;
+            }
+        }
+        for (int i = 0; i < 4; ++i)
+            full_pos[t * 4 + i] = s.pos[i];
+        if (term != 0)
+        {
+            stopReason = term - 1;
+            stopTime = t;
+            break;
+        }
+    }
+    for (int r = t + 1; r < T; ++r) // after a stop: the stop state, zero actions
+    {
+        for (int i = 0; i < 4; ++i)
+            full_pos[r * 4 + i] = s.pos[i];
+        actions[r * 2] = 0.0f;
+        actions[r * 2 + 1] = 0.0f;
+    }
+    info[1] = time;
+    info[2] = pred;
+    info[3] = stopReason;
+    info[4] = stopTime;
+    return 0;
+}
+
+// Opponent target lateral offset (computeOppTargetLatDist, env_stratrace.h:318-461) at st under theta.
+double or_sr_opp_target(int f64, const dr_sr_env* env, int theta, const dr_sr_state* st)
+{
+    if (f64)
+    {
+        double lat, goal;
+        opp_target(Env<double>(*env), env->opp[theta], load<double>(*st), &lat, &goal);
+        return lat;
+    }
+    float lat, goal;
+    opp_target(Env<float>(*env), env->opp[theta], load<float>(*st), &lat, &goal);
+    return lat;
 }
 
 } // extern "C"

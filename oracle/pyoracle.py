@@ -75,6 +75,9 @@ def lib(abi):
     L.or_sr_env_step_belief.argtypes = [I, C.POINTER(abi.SrEnv), I, F, C.POINTER(abi.SrState), F, C.c_uint64, U, U, U, I]
     L.or_bm_step.argtypes = [I, C.POINTER(abi.SrEnv), C.POINTER(abi.SrCost), C.POINTER(abi.SrState), C.POINTER(abi.BmInstance), F,
                              U, C.c_float, F, I, F, D, C.POINTER(I), C.POINTER(I), D, D, D, D, D, D]
+    L.or_sr_predict.argtypes = [C.POINTER(abi.SrEnv), C.POINTER(abi.SrState), I, F, I, I, F, C.c_float, F, F, C.POINTER(I)]
+    L.or_sr_opp_target.restype = C.c_double
+    L.or_sr_opp_target.argtypes = [I, C.POINTER(abi.SrEnv), I, C.POINTER(abi.SrState)]
     _lib = L
     return L
 
@@ -438,3 +441,23 @@ class Oracle:
                     used=used.reshape(NT, K), btime=bt.reshape(NT, K), costs=costs.reshape(NB, K), mins=mins.reshape(NB, M),
                     nu=nu, knots=kout.reshape(NB, M, 2), dense=dense.reshape(NB, T, 2), action=res[:2], lambda_=float(res[2]),
                     branch=int(res[3]))
+
+    # ---------------------------------------------------------- session display predictions
+
+    def sr_predict(self, env, state, theta, plans, T, branching, belief=None, min_conf=0.0):
+        """SimulationEngine::sendState's noise-free prediction: (full_pos [T][4], actions [T][2],
+        (initial branch, branching time, final branch, stop reason, stop time))."""
+        pl = np.ascontiguousarray(plans, dtype=np.float32).reshape(-1)
+        b = np.zeros(self.abi.SR_MAX_MODELS, dtype=np.float32)
+        if belief is not None:
+            b[:len(belief)] = belief
+        pos = np.empty(T * 4, dtype=np.float32)
+        act = np.empty(T * 2, dtype=np.float32)
+        info = (C.c_int * 5)()
+        rc = self.L.or_sr_predict(C.byref(env), C.byref(state), theta, _f(pl), T, 1 if branching else 0, _f(b), min_conf, _f(pos),
+                                  _f(act), info)
+        assert rc == 0
+        return pos.reshape(T, 4), act.reshape(T, 2), tuple(info[:])
+
+    def sr_opp_target(self, f64, env, theta, state):
+        return self.L.or_sr_opp_target(int(f64), C.byref(env), theta, C.byref(state))

@@ -243,6 +243,32 @@ def bm_race(pkg, steps=30, warmup=3):
             "final_branch": int(d.branch), "steps": steps}
 
 
+def session_race(pkg, steps=30):
+    """The simulation session behind the wire protocol (SimulationEngine::run) at the reference's
+    settings: the branching spline MPPI header (K = 2^16, T = 80, 20 knots) and the PR-MPPI header
+    (K = 2^16, T = 60, P = 9), two opponent models; per control step getControl + dynStep + the
+    STATE frame with its display predictions, in process (no socket)."""
+    wire = pkg.wire
+    e = wire.stratrace_env(n_models=2, seed=42)
+    out = {}
+    for name, cont in (("mppi", wire.mppi_header()), ("prmppi", wire.prmppi_header())):
+        sess = wire.Session(wire.env_header(e), cont, wire.init_state_header(e), max_steps=steps)
+        first = sess.next_frame()  # step-0 STATE
+        t0 = time.perf_counter()
+        n_state, n_bytes, last = 0, 0, None
+        for t, frame in sess:
+            if t == 1:
+                n_state += 1
+                n_bytes += len(frame)
+            last = t
+        elapsed = time.perf_counter() - t0
+        sess.close()
+        out[name] = {"control_steps_per_s": n_state / elapsed, "ms_per_step": 1e3 * elapsed / max(n_state, 1),
+                     "state_frame_bytes": n_bytes / max(n_state, 1), "steps": n_state, "first_frame_bytes": len(first[1]),
+                     "last_frame_type": last}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +286,7 @@ def main():
     ap.add_argument("--no-sr", action="store_true")
     ap.add_argument("--sr-only", action="store_true", help="run only the StratRace2D section (for profiling)")
     ap.add_argument("--bm-only", action="store_true", help="run only the branching spline MPPI section (for profiling)")
+    ap.add_argument("--session-only", action="store_true", help="run only the wire-protocol session section")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 512 MiB buffer 3 times (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -281,6 +308,9 @@ def main():
 
     pkg = package()
     sc = pkg.scenarios
+    if args.session_only:
+        print(json.dumps({"session": session_race(pkg, steps=min(args.steps, 30))}), flush=True)
+        return
     if args.bm_only:
         print(json.dumps({"bm_race": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup)}), flush=True)
         return
@@ -387,6 +417,9 @@ def main():
         log("bench: branching spline MPPI on StratRace2D, K=65536, T=80, 20 knots ...")
         out["bm_race"] = bm_race(pkg)
         log(f"bench: bm_race {json.dumps(out['bm_race'])}")
+        log("bench: wire-protocol session (MPPI and PR-MPPI headers at the reference settings) ...")
+        out["session"] = session_race(pkg)
+        log(f"bench: session {json.dumps(out['session'])}")
     if not args.no_batch64:
         log("bench: 64 uneven controllers, batched vs per-instance ...")
         out["batch64"] = batch_vs_per_instance(pkg)

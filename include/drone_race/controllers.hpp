@@ -322,6 +322,98 @@ class MPPIController
     dr_sr_engine* h = nullptr;
 };
 
+// MPPIConfig of the reference's default controller: branching over the
+// opponent models with spline plans (config.h:116-154, USE_SPLINES).
+// knots[0..nKnots) = 0 selects the default spacing m * floor((T - 2) / (M - 1))
+// (utils.py:496-497).
+struct BranchingMPPIConfig
+{
+    int nSamples = 65536;
+    int nTimesteps = 80;
+    int nKnots = 20;
+    std::array<int, DR_MAX_KNOTS> knots{};
+    float invTemperature = 10.0f;
+    float samplingNoise = 0.5f;
+    float minConfidence = 0.95f;
+    dr_sr_cost cost{1.4f, 50.0f, 1.8f, 1e7f, 1e5f, 1e5f, 50.0f, 0.0f};
+};
+
+// MPPIController of the StratRace2D build (controllers.h:23-111) over the
+// branching engine (dr_bm_*).  h_belief is the controller's belief, written
+// by the caller's dynStep as in the reference (engine.cpp:332); h_nominal is
+// the dense plans of every branch after the step ([nModels + 1][T][2], the
+// chosen branch shifted).  Plan verification is not run: failCount stays
+// empty, epsilon / epsilonPartial / certifiedLoss 0 and useNewPlan true.
+class BranchingMPPIController
+{
+  public:
+    BranchingMPPIController(const EnvironmentConfig& c, const BranchingMPPIConfig& mc, int seed, int trueTheta = 0,
+                            std::optional<std::vector<float>> nominal = std::nullopt)
+        : mppiConfig(mc), nModels(c.n_models)
+    {
+        dr_bm_instance in{};
+        in.n_samples = mc.nSamples;
+        in.n_timesteps = mc.nTimesteps;
+        in.n_knots = mc.nKnots;
+        bool given = false;
+        for (int m = 0; m < mc.nKnots && m < DR_MAX_KNOTS; ++m)
+            given = given || mc.knots[m] != 0;
+        for (int m = 0; m < mc.nKnots && m < DR_MAX_KNOTS; ++m)
+            in.knots[m] = given ? mc.knots[m] : m * ((mc.nTimesteps - 2) / (mc.nKnots > 1 ? mc.nKnots - 1 : 1));
+        in.seed = (uint64_t)(uint32_t)seed;
+        in.lambda = mc.invTemperature;
+        in.sigma = mc.samplingNoise;
+        in.min_confidence = mc.minConfidence;
+        in.true_theta = trueTheta;
+        in.adapt_lambda = 1; // mppi_controller.cu:413-424
+        for (int th = 0; th < c.n_models && th < DR_SR_MAX_MODELS; ++th)
+            in.init_belief[th] = 1.0f / (float)c.n_models;
+        check(dr_bm_create(&c, &mc.cost, &in, 1, &h), "MPPIController");
+        const size_t n = (size_t)(nModels + 1) * mc.nTimesteps * ACTION_DIM;
+        if (nominal && nominal->size() != n)
+        {
+            dr_bm_destroy(h);
+            throw std::runtime_error("Invalid MPPI construction: expected nominal size " + std::to_string(n) + ", got " +
+                                     std::to_string(nominal->size()));
+        }
+        h_nominal = nominal ? *nominal : std::vector<float>(n, 0.0f);
+        h_belief.fill(0.0f);
+        for (int th = 0; th < nModels; ++th)
+            h_belief[th] = in.init_belief[th];
+    }
+
+    ~BranchingMPPIController() { dr_bm_destroy(h); }
+    BranchingMPPIController(const BranchingMPPIController&) = delete;
+    BranchingMPPIController& operator=(const BranchingMPPIController&) = delete;
+
+    void getControl(const SimState& state, float* outAction)
+    {
+        check(dr_bm_set_belief(h, 0, h_belief.data(), nModels), "getControl");
+        check(dr_bm_set_states(h, &state, 1), "getControl");
+        check(dr_bm_step(h, 1, 0, outAction), "getControl");
+        check(dr_bm_get_nominal(h, 0, h_nominal.data(), (int)h_nominal.size()), "getControl");
+        dr_bm_diag d{};
+        check(dr_bm_get_diag(h, &d, 1), "getControl");
+        mppiConfig.invTemperature = d.lambda;
+        chosenBranch = d.branch;
+        lastNu = d.nu;
+        lastMinCost = d.min_cost;
+    }
+
+    BranchingMPPIConfig mppiConfig;
+    std::vector<float> h_nominal;                    // (nModels + 1, T, 2)
+    std::array<float, DR_SR_MAX_MODELS> h_belief{};  // written by the caller's world step
+    std::vector<unsigned> failCount;                 // no verification
+    double epsilon = 0.0, epsilonPartial = 0.0, certifiedLoss = 0.0;
+    bool useNewPlan = true;
+    int chosenBranch = 0; // 0 nominal, theta + 1 committed
+    float lastNu = 0.0f, lastMinCost = 0.0f;
+
+  private:
+    int nModels;
+    dr_bm_engine* h = nullptr;
+};
+
 } // namespace stratrace
 
 } // namespace drone_race

@@ -122,6 +122,22 @@ int main(int argc, char** argv)
             sr.getControl(ss, a);
             std::printf("sr_action %d %.9g %.9g\n", i, a[0], a[1]);
         }
+        // the reference's default controller: branching over the models with spline plans
+        stratrace::BranchingMPPIConfig bc;
+        bc.nSamples = 2048;
+        bc.nTimesteps = 30;
+        bc.nKnots = 8;
+        bc.minConfidence = 0.6f;
+        stratrace::BranchingMPPIController bm(se, bc, 9, 1);
+        const float beliefs[3][2] = {{0.5f, 0.5f}, {0.3f, 0.7f}, {0.9f, 0.1f}};
+        for (int i = 0; i < steps; ++i)
+        {
+            bm.h_belief[0] = beliefs[i % 3][0];
+            bm.h_belief[1] = beliefs[i % 3][1];
+            float a[2];
+            bm.getControl(ss, a);
+            std::printf("bm_action %d %.9g %.9g branch %d lambda %.9g\n", i, a[0], a[1], bm.chosenBranch, bm.mppiConfig.invTemperature);
+        }
     }
     catch (const std::exception& e)
     {

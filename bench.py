@@ -218,12 +218,14 @@ def pr_sr_race(pkg, steps=30, warmup=3):
                                    "update_choice_world": float(ms[3]), "step": float(ms[4])}, "steps": steps}
 
 
-def bm_race(pkg, steps=30, warmup=3):
+def bm_race(pkg, steps=30, warmup=3, n_verif=0):
     """Branching spline MPPI (the reference's default MPPI controller) on its StratRace2D race at
     the spline setting (K = 2^16, T = 80, 20 knots, two opponent models, uniform initial belief,
-    temperature 10 adapted, sigma 0.5), closed loop with belief updates, one world."""
+    temperature 10 adapted, sigma 0.5), closed loop with belief updates, one world; n_verif > 0
+    adds the plan verification at mainGate's setting (2^17 samples, horizon 40, beta 1e-5,
+    maxVerifEps 0.001)."""
     sc = pkg.scenarios
-    scn = sc.bm_race(n_inst=1)
+    scn = sc.bm_race(n_inst=1, n_verif=n_verif)
     eng = pkg.BMEngine(scn)
     eng.step(warmup, advance=True)
     t0 = time.perf_counter()
@@ -239,8 +241,9 @@ def bm_race(pkg, steps=30, warmup=3):
     return {"workload": scn.name, "control_steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
             "rollouts_per_step": rollouts, "rollout_env_steps_per_s": rollouts * ins.n_timesteps * steps / elapsed,
             "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate_min": float(ms[1]), "weighted_avg": float(ms[2]),
-                                   "update_world": float(ms[3]), "step": float(ms[4])},
-            "final_branch": int(d.branch), "steps": steps}
+                                   "update_verify": float(ms[3]), "choice_world": float(ms[4]), "step": float(ms[5])},
+            "final_branch": int(d.branch), "use_new_plan": int(d.use_new_plan), "certified_loss": float(d.certified_loss),
+            "verification_samples": n_verif, "steps": steps}
 
 
 def session_race(pkg, steps=30):
@@ -312,12 +315,16 @@ def main():
         print(json.dumps({"session": session_race(pkg, steps=min(args.steps, 30))}), flush=True)
         return
     if args.bm_only:
-        print(json.dumps({"bm_race": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup)}), flush=True)
+        print(json.dumps({"bm_race": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup),
+                          "bm_race_verified": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup, n_verif=2 ** 17)}),
+              flush=True)
         return
     if args.sr_only:
         print(json.dumps({"sr_race": sr_race(pkg, steps=args.steps, warmup=args.warmup),
                           "pr_sr_race": pr_sr_race(pkg, steps=min(args.steps, 30), warmup=args.warmup),
-                          "bm_race": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup)}), flush=True)
+                          "bm_race": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup),
+                          "bm_race_verified": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup, n_verif=2 ** 17)}),
+              flush=True)
         return
     if args.pr_only:
         print(json.dumps({"pr_mppi": pr_mppi(pkg, steps=args.steps, warmup=args.warmup)}), flush=True)
@@ -417,6 +424,9 @@ def main():
         log("bench: branching spline MPPI on StratRace2D, K=65536, T=80, 20 knots ...")
         out["bm_race"] = bm_race(pkg)
         log(f"bench: bm_race {json.dumps(out['bm_race'])}")
+        log("bench: branching spline MPPI with plan verification (2^17 samples, horizon 40) ...")
+        out["bm_race_verified"] = bm_race(pkg, n_verif=2 ** 17)
+        log(f"bench: bm_race_verified {json.dumps(out['bm_race_verified'])}")
         log("bench: wire-protocol session (MPPI and PR-MPPI headers at the reference settings) ...")
         out["session"] = session_race(pkg)
         log(f"bench: session {json.dumps(out['session'])}")

@@ -132,12 +132,13 @@ class BmInstance(C.Structure):
     _fields_ = [("n_samples", C.c_int32), ("n_timesteps", C.c_int32), ("n_knots", C.c_int32), ("knots", C.c_int32 * MAX_KNOTS),
                 ("seed", C.c_uint64), ("lambda_", C.c_float), ("sigma", C.c_float), ("min_confidence", C.c_float),
                 ("true_theta", C.c_int32), ("adapt_lambda", C.c_int32), ("init_belief", C.c_float * SR_MAX_MODELS),
-                ("_pad", C.c_int32)]
+                ("n_verif_samples", C.c_int32), ("verif_horizon", C.c_int32), ("beta", C.c_float), ("max_verif_eps", C.c_float)]
 
 
 class BmDiag(C.Structure):
     _fields_ = [("lambda_", C.c_float), ("nu", C.c_float), ("min_cost", C.c_float), ("step", C.c_uint32),
-                ("branch", C.c_int32), ("_pad", C.c_int32)]
+                ("branch", C.c_int32), ("use_new_plan", C.c_int32), ("fail_count", C.c_uint32), ("_pad", C.c_int32),
+                ("epsilon", C.c_double), ("epsilon_partial", C.c_double), ("certified_loss", C.c_double)]
 
 
 MSG_HEADER, MSG_STATE, MSG_EVENT, MSG_DONE = 0, 1, 2, 3
@@ -264,6 +265,7 @@ SIGNATURES = {
     "dr_bm_get_min_costs": (_I, [_P, _I, _F, _I]),
     "dr_bm_get_nu": (_I, [_P, _I, _F, _I]),
     "dr_bm_get_diag": (_I, [_P, C.POINTER(BmDiag), _I]),
+    "dr_bm_get_fail_counts": (_I, [_P, _I, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _I]),
     "dr_bm_set_timing": (_I, [_P, _I]),
     "dr_bm_last_timing": (_I, [_P, _F, _I]),
 }

@@ -12,9 +12,11 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "dr_bmkernels.h"
+#include "dr_cp.h"
 #include "dr_host.h"
 
 namespace dr {
@@ -26,7 +28,11 @@ hipError_t launch_bm_reduce(int n_inst, int max_nbk, int n_rtiles, hipStream_t s
                             const float* ctrue, float* costs, const int* used, const int* btime, float* pmin, float* mins,
                             const float2* noise, double* part, hipEvent_t mid);
 hipError_t launch_bm_update(int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const double* part, const float* basis,
-                            float* knots, float* dense, float* nu, float* actions, float max_accel);
+                            float* knots, float* dnew, float* nu, int* fail, float max_accel);
+hipError_t launch_bm_verify(int n_tiles, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmVTile* tiles, const dr_sr_env* env,
+                            const dr_sr_state* states, const float* dnew, const float* dense, int* fail);
+hipError_t launch_bm_certify(int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const int* verdict, const float* dnew,
+                             float* dense, float* actions);
 hipError_t launch_bm_advance(int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const dr_sr_env* env, dr_sr_state* states,
                              const float* actions, int* events);
 
@@ -99,6 +105,20 @@ std::vector<float> spline_basis(const int* tau, int M, int T)
 
 using namespace dr;
 
+// Host side of the plan verification of one instance: the Clopper-Pearson
+// quantiles (memoised per failure count; n and beta are fixed) and the
+// outcome of the last step (computeCertifiedLoss / computeEpsilon,
+// mppi_controller.cu:468-514).
+struct BmVerif
+{
+    std::unordered_map<uint32_t, double> lo_half, up_half, up_full;
+    double certified = -1.0, epsilon = 0.0, eps_partial = 0.0;
+    uint32_t fail_total = 0;
+    int use_new = 1;
+};
+
+constexpr int BM_NEV = 6; // timing events per step
+
 struct dr_bm_engine
 {
     int device = 0;
@@ -109,45 +129,57 @@ struct dr_bm_engine
     std::vector<BmInstDev> idev;
     std::vector<BmTile> tiles;
     std::vector<BmRTile> rtiles;
+    std::vector<BmVTile> vtiles;
+    std::vector<BmVerif> verif;
+    bool any_verif = false;
     int max_m = 0, max_tm = 0, max_nbk = 0;
     long long noise_total = 0, part_total = 0;
-    int ct_total = 0, cost_total = 0, min_total = 0, pmin_total = 0, knot_total = 0, dense_total = 0, B_total = 0, nu_total = 0;
+    int ct_total = 0, cost_total = 0, min_total = 0, pmin_total = 0, knot_total = 0, dense_total = 0, B_total = 0, nu_total = 0,
+        fail_total = 0;
 
     DevBuf<BmInstDev> d_inst;
     DevBuf<BmDyn> d_dyn;
     DevBuf<BmTile> d_tiles;
     DevBuf<BmRTile> d_rt;
+    DevBuf<BmVTile> d_vt;
     DevBuf<dr_sr_env> d_env;
     DevBuf<dr_sr_state> d_states;
-    DevBuf<float> d_knots, d_dense, d_B, d_ctrue, d_costs, d_mins, d_pmin, d_nu, d_actions;
-    DevBuf<int> d_used, d_btime, d_events;
+    DevBuf<float> d_knots, d_dense, d_dnew, d_B, d_ctrue, d_costs, d_mins, d_pmin, d_nu, d_actions;
+    DevBuf<int> d_used, d_btime, d_events, d_fail, d_verdict;
     DevBuf<float2> d_noise;
     DevBuf<double> d_part;
 
-    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    // [advance][part]: part 0 = sample .. verification (or the whole step
+    // without verification), part 1 = plan choice + world step
+    hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     bool timing = false;
     std::vector<hipEvent_t> events;
     int timed_steps = 0;
-    float last_ms[5] = {0, 0, 0, 0, 0};
+    float last_ms[BM_NEV] = {0, 0, 0, 0, 0, 0};
+    std::vector<int> h_fail, h_verdict;
 
     ~dr_bm_engine()
     {
-        for (auto& g : graph_exec)
-            if (g)
-                (void)hipGraphExecDestroy(g);
+        for (auto& row : graph_exec)
+            for (auto& g : row)
+                if (g)
+                    (void)hipGraphExecDestroy(g);
         for (auto ev : events)
             (void)hipEventDestroy(ev);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
-    int enqueue_step(bool advance, int timed_index);
+    int enqueue_a(hipEvent_t* ev);
+    int enqueue_b(bool advance, hipEvent_t* ev);
+    int certify_host();
+    int graph(int gi, int part, bool advance);
     int run(int n_steps, bool advance);
 };
 
-int dr_bm_engine::enqueue_step(bool advance, int timed_index)
+// sample, rollout, reduce, update, verification
+int dr_bm_engine::enqueue_a(hipEvent_t* ev)
 {
     const int n = (int)inst.size();
-    hipEvent_t* ev = (timed_index >= 0) ? &events[(size_t)timed_index * 5] : nullptr;
     if (ev)
         HIPCHK(hipEventRecord(ev[0], stream));
     HIPCHK(launch_bm_rollout(max_m, max_tm, 0, stream, d_inst.p, d_dyn.p, d_tiles.p, (int)tiles.size(), d_env.p, cost, d_states.p, d_knots.p, d_B.p,
@@ -158,39 +190,91 @@ int dr_bm_engine::enqueue_step(bool advance, int timed_index)
                             d_pmin.p, d_mins.p, d_noise.p, d_part.p, ev ? ev[2] : nullptr));
     if (ev)
         HIPCHK(hipEventRecord(ev[3], stream));
-    HIPCHK(launch_bm_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_B.p, d_knots.p, d_dense.p, d_nu.p, d_actions.p, env.max_accel[0]));
-    if (advance)
-        HIPCHK(launch_bm_advance(n, stream, d_inst.p, d_dyn.p, d_env.p, d_states.p, d_actions.p, d_events.p));
+    HIPCHK(launch_bm_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_B.p, d_knots.p, d_dnew.p, d_nu.p, d_fail.p, env.max_accel[0]));
+    HIPCHK(launch_bm_verify((int)vtiles.size(), stream, d_inst.p, d_dyn.p, d_vt.p, d_env.p, d_states.p, d_dnew.p, d_dense.p, d_fail.p));
     if (ev)
         HIPCHK(hipEventRecord(ev[4], stream));
     return DR_OK;
 }
 
-int dr_bm_engine::run(int n_steps, bool advance)
+// plan choice, action, shift, world step
+int dr_bm_engine::enqueue_b(bool advance, hipEvent_t* ev)
 {
-    if (n_steps < 0)
-        return host_fail(DR_EINVAL, "n_steps < 0");
-    timed_steps = 0;
-    if (timing)
+    const int n = (int)inst.size();
+    HIPCHK(launch_bm_certify(n, stream, d_inst.p, d_dyn.p, d_verdict.p, d_dnew.p, d_dense.p, d_actions.p));
+    if (advance)
+        HIPCHK(launch_bm_advance(n, stream, d_inst.p, d_dyn.p, d_env.p, d_states.p, d_actions.p, d_events.p));
+    if (ev)
+        HIPCHK(hipEventRecord(ev[5], stream));
+    return DR_OK;
+}
+
+// computeCertifiedLoss + computeEpsilon (mppi_controller.cu:468-514) from
+// the failure counts of this step; writes the verdicts to the device.
+int dr_bm_engine::certify_host()
+{
+    HIPCHK(hipMemcpyAsync(h_fail.data(), d_fail.p, h_fail.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (size_t i = 0; i < inst.size(); ++i)
     {
-        while (events.size() < (size_t)n_steps * 5)
+        const BmInstDev& I = idev[i];
+        BmVerif& V = verif[i];
+        if (I.n_verif <= 0)
         {
-            hipEvent_t ev;
-            HIPCHK(hipEventCreate(&ev));
-            events.push_back(ev);
+            h_verdict[i] = 1;
+            continue;
         }
-        for (int s = 0; s < n_steps; ++s)
-            if (int rc = enqueue_step(advance, s))
-                return rc;
-        timed_steps = n_steps;
-        return DR_OK;
+        const dr_bm_instance& S = inst[i];
+        const unsigned n = (unsigned)I.n_verif;
+        const double half = (double)S.beta / 2.0;
+        auto memo = [](std::unordered_map<uint32_t, double>& m, uint32_t k, double v_fn(unsigned, unsigned, double), unsigned nn,
+                       double a) {
+            auto it = m.find(k);
+            if (it != m.end())
+                return it->second;
+            const double v = v_fn(k, nn, a);
+            m.emplace(k, v);
+            return v;
+        };
+        const int* fnew = h_fail.data() + I.fail_off;
+        const int* fold = fnew + I.T;
+        double certified = -1.0;
+        uint32_t totNew = 0, totOld = 0;
+        for (int t = 0; t < I.T; ++t)
+        {
+            totNew += (uint32_t)fnew[t];
+            totOld += (uint32_t)fold[t];
+            const double e1 = memo(V.up_half, totNew, cp::cp_upper, n, half);
+            const double e2 = memo(V.lo_half, totOld, cp::cp_lower, n, half);
+            if (e1 - e2 > certified)
+                certified = e1 - e2;
+        }
+        V.certified = certified;
+        V.use_new = certified < (double)S.max_verif_eps ? 1 : 0;
+        V.fail_total = V.use_new ? totNew : totOld;
+        V.eps_partial = memo(V.up_full, V.fail_total, cp::cp_upper, n, (double)S.beta);
+        V.epsilon = V.eps_partial + (double)S.verif_horizon * (double)S.max_verif_eps;
+        h_verdict[i] = V.use_new;
     }
-    const int gi = advance ? 1 : 0;
-    if (!graph_exec[gi])
+    HIPCHK(hipMemcpyAsync(d_verdict.p, h_verdict.data(), h_verdict.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    return DR_OK;
+}
+
+int dr_bm_engine::graph(int gi, int part, bool advance)
+{
+    if (!graph_exec[gi][part])
     {
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_step(advance, -1);
+        int rc = DR_OK;
+        if (part == 0)
+        {
+            rc = enqueue_a(nullptr);
+            if (!rc && !any_verif)
+                rc = enqueue_b(advance, nullptr);
+        }
+        else
+            rc = enqueue_b(advance, nullptr);
         hipError_t ce = hipStreamEndCapture(stream, &g);
         if (rc)
         {
@@ -200,13 +284,55 @@ int dr_bm_engine::run(int n_steps, bool advance)
         }
         if (ce != hipSuccess)
             return host_fail(DR_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-        hipError_t ie = hipGraphInstantiate(&graph_exec[gi], g, nullptr, nullptr, 0);
+        hipError_t ie = hipGraphInstantiate(&graph_exec[gi][part], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (ie != hipSuccess)
             return host_fail(DR_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
     }
+    HIPCHK(hipGraphLaunch(graph_exec[gi][part], stream));
+    return DR_OK;
+}
+
+// Without verification one control step is one graph replay; with it, the
+// step pauses after the verification for the host's Clopper-Pearson bounds
+// (as the reference does) and resumes with the plan choice.
+int dr_bm_engine::run(int n_steps, bool advance)
+{
+    if (n_steps < 0)
+        return host_fail(DR_EINVAL, "n_steps < 0");
+    timed_steps = 0;
+    const int gi = advance ? 1 : 0;
+    if (timing)
+        while (events.size() < (size_t)n_steps * BM_NEV)
+        {
+            hipEvent_t ev;
+            HIPCHK(hipEventCreate(&ev));
+            events.push_back(ev);
+        }
     for (int s = 0; s < n_steps; ++s)
-        HIPCHK(hipGraphLaunch(graph_exec[gi], stream));
+    {
+        hipEvent_t* ev = timing ? &events[(size_t)s * BM_NEV] : nullptr;
+        if (timing)
+        {
+            if (int rc = enqueue_a(ev))
+                return rc;
+        }
+        else if (int rc = graph(gi, 0, advance))
+            return rc;
+        if (any_verif)
+            if (int rc = certify_host())
+                return rc;
+        if (timing)
+        {
+            if (int rc = enqueue_b(advance, ev))
+                return rc;
+        }
+        else if (any_verif)
+            if (int rc = graph(gi, 1, advance))
+                return rc;
+    }
+    if (timing)
+        timed_steps = n_steps;
     return DR_OK;
 }
 
@@ -280,6 +406,11 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
         for (int th = 0; th < env->n_models; ++th)
             if (!(s.init_belief[th] >= 0.0f) || !std::isfinite(s.init_belief[th]))
                 return host_fail(DR_EINVAL, w + "belief entries must be finite and >= 0");
+        if (s.n_verif_samples < 0 || s.n_verif_samples > (1 << 22))
+            return host_fail(DR_EINVAL, w + "n_verif_samples must be in [0, 2^22]");
+        if (s.n_verif_samples > 0 &&
+            (s.verif_horizon < 1 || s.verif_horizon > s.n_timesteps || !(s.beta > 0.0f && s.beta < 1.0f) || !std::isfinite(s.max_verif_eps)))
+            return host_fail(DR_EINVAL, w + "verification needs verif_horizon in [1, T], beta in (0, 1) and a finite max_verif_eps");
     }
     dr_bm_engine* e = new (std::nothrow) dr_bm_engine();
     if (!e)
@@ -338,6 +469,17 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
         e->pmin_total += NB * I.M * I.n_chunks;
         I.nu_off = e->nu_total;
         e->nu_total += NB;
+        I.n_verif = s.n_verif_samples;
+        I.verif_h = s.n_verif_samples > 0 ? s.verif_horizon : 0;
+        I.fail_off = e->fail_total;
+        e->fail_total += 2 * I.T;
+        if (I.n_verif > 0)
+        {
+            e->any_verif = true;
+            for (int set = 0; set < 2; ++set)
+                for (int s0 = 0; s0 < I.n_verif; s0 += BM_VBLOCK)
+                    e->vtiles.push_back(BmVTile{i, set, s0, 0});
+        }
         e->max_m = std::max(e->max_m, I.M);
         e->max_tm = std::max(e->max_tm, I.T * I.M);
         e->max_nbk = std::max(e->max_nbk, NB * I.K);
@@ -362,6 +504,8 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
     };
     if (!chk(e->d_inst.alloc(n), "alloc inst") || !chk(e->d_dyn.alloc(n), "alloc dyn") || !chk(e->d_tiles.alloc(e->tiles.size()), "alloc tiles") ||
         !chk(e->d_rt.alloc(e->rtiles.size()), "alloc rt") || !chk(e->d_pmin.alloc(e->pmin_total), "alloc pmin") ||
+        !chk(e->d_vt.alloc(e->vtiles.size()), "alloc vtiles") || !chk(e->d_dnew.alloc(e->dense_total), "alloc dnew") ||
+        !chk(e->d_fail.alloc(e->fail_total), "alloc fail") || !chk(e->d_verdict.alloc(n), "alloc verdict") ||
         !chk(e->d_env.alloc(1), "alloc env") || !chk(e->d_states.alloc(n), "alloc states") || !chk(e->d_knots.alloc(e->knot_total), "alloc knots") ||
         !chk(e->d_dense.alloc(e->dense_total), "alloc dense") || !chk(e->d_B.alloc(e->B_total), "alloc B") ||
         !chk(e->d_ctrue.alloc(e->ct_total), "alloc ctrue") || !chk(e->d_costs.alloc(e->cost_total), "alloc costs") ||
@@ -375,6 +519,7 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
     {
         std::memset(&dyn[i], 0, sizeof(BmDyn));
         dyn[i].lambda = instances[i].lambda;
+        dyn[i].use_new = 1;
         for (int th = 0; th < NT; ++th)
             dyn[i].belief[th] = instances[i].init_belief[th];
     }
@@ -386,13 +531,22 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
         !chk(hipMemcpy(e->d_dyn.p, dyn.data(), n * sizeof(BmDyn), hipMemcpyHostToDevice), "copy dyn") ||
         !chk(hipMemcpy(e->d_tiles.p, e->tiles.data(), e->tiles.size() * sizeof(BmTile), hipMemcpyHostToDevice), "copy tiles") ||
         !chk(hipMemcpy(e->d_rt.p, e->rtiles.data(), e->rtiles.size() * sizeof(BmRTile), hipMemcpyHostToDevice), "copy rt") ||
+        (!e->vtiles.empty() &&
+         !chk(hipMemcpy(e->d_vt.p, e->vtiles.data(), e->vtiles.size() * sizeof(BmVTile), hipMemcpyHostToDevice), "copy vtiles")) ||
+        !chk(hipMemcpy(e->d_fail.p, std::vector<int>(e->fail_total, 0).data(), e->fail_total * sizeof(int), hipMemcpyHostToDevice),
+             "copy fail") ||
+        !chk(hipMemcpy(e->d_verdict.p, std::vector<int>(n, 1).data(), n * sizeof(int), hipMemcpyHostToDevice), "copy verdict") ||
         !chk(hipMemcpy(e->d_env.p, env, sizeof(dr_sr_env), hipMemcpyHostToDevice), "copy env") ||
         !chk(hipMemcpy(e->d_states.p, st.data(), n * sizeof(dr_sr_state), hipMemcpyHostToDevice), "copy states") ||
         !chk(hipMemcpy(e->d_knots.p, zk.data(), zk.size() * sizeof(float), hipMemcpyHostToDevice), "copy knots") ||
         !chk(hipMemcpy(e->d_dense.p, zd.data(), zd.size() * sizeof(float), hipMemcpyHostToDevice), "copy dense") ||
+        !chk(hipMemcpy(e->d_dnew.p, zd.data(), zd.size() * sizeof(float), hipMemcpyHostToDevice), "copy dnew") ||
         !chk(hipMemcpy(e->d_B.p, Ball.data(), Ball.size() * sizeof(float), hipMemcpyHostToDevice), "copy B") ||
         !chk(hipMemcpy(e->d_events.p, ev.data(), n * sizeof(int), hipMemcpyHostToDevice), "copy events"))
         return bail(DR_EHIP);
+    e->verif.assign(n, BmVerif{});
+    e->h_fail.assign(e->fail_total, 0);
+    e->h_verdict.assign(n, 1);
     *out = e;
     return DR_OK;
 }
@@ -499,19 +653,19 @@ int dr_bm_step(dr_bm_engine* e, int n_steps, int advance, float* actions)
     HIPCHK(hipStreamSynchronize(e->stream));
     if (e->timed_steps > 0)
     {
-        float acc[5] = {0, 0, 0, 0, 0};
+        float acc[BM_NEV] = {0, 0, 0, 0, 0, 0};
         for (int s = 0; s < e->timed_steps; ++s)
         {
-            hipEvent_t* ev = &e->events[(size_t)s * 5];
-            for (int q = 0; q < 4; ++q)
+            hipEvent_t* ev = &e->events[(size_t)s * BM_NEV];
+            for (int q = 0; q < BM_NEV - 1; ++q)
             {
                 float v;
                 HIPCHK(hipEventElapsedTime(&v, ev[q], ev[q + 1]));
                 acc[q] += v;
             }
             float t;
-            HIPCHK(hipEventElapsedTime(&t, ev[0], ev[4]));
-            acc[4] += t;
+            HIPCHK(hipEventElapsedTime(&t, ev[0], ev[BM_NEV - 1]));
+            acc[BM_NEV - 1] += t;
         }
         std::memcpy(e->last_ms, acc, sizeof(acc));
         e->timed_steps = 0;
@@ -611,8 +765,26 @@ int dr_bm_get_diag(dr_bm_engine* e, dr_bm_diag* diag, int n)
         diag[i].branch = dyn[i].branch;
         diag[i].nu = nu[I.nu_off + dyn[i].branch];
         diag[i].min_cost = mins[I.min_off + dyn[i].branch * I.M];
+        const BmVerif& V = e->verif[i];
+        diag[i].use_new_plan = dyn[i].use_new;
+        diag[i].fail_count = V.fail_total;
+        diag[i].epsilon = V.epsilon;
+        diag[i].epsilon_partial = V.eps_partial;
+        diag[i].certified_loss = I.n_verif > 0 ? V.certified : 0.0;
     }
     return DR_OK;
+}
+
+int dr_bm_get_fail_counts(dr_bm_engine* e, int instance, int32_t* fail_new, int32_t* fail_old, int n)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    const BmInstDev& I = e->idev[instance];
+    if (!fail_new || !fail_old || n != I.T)
+        return host_fail(DR_EINVAL, "get_fail_counts: expected T values each");
+    if (int rc = bm_out(e, fail_new, e->d_fail.p + I.fail_off, (size_t)n))
+        return rc;
+    return bm_out(e, fail_old, e->d_fail.p + I.fail_off + I.T, (size_t)n);
 }
 
 int dr_bm_set_timing(dr_bm_engine* e, int on)
@@ -625,8 +797,8 @@ int dr_bm_set_timing(dr_bm_engine* e, int on)
 
 int dr_bm_last_timing(dr_bm_engine* e, float* ms, int n)
 {
-    if (!e || !ms || n != 5)
-        return host_fail(DR_EINVAL, "last_timing: expected 5 floats");
+    if (!e || !ms || n != BM_NEV)
+        return host_fail(DR_EINVAL, "last_timing: expected 6 floats");
     std::memcpy(ms, e->last_ms, sizeof(e->last_ms));
     return DR_OK;
 }

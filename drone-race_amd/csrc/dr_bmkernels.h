@@ -12,6 +12,9 @@
 //   costs  float[NB][K]         belief-weighted branch costs
 //   mins   float[NB][M]         masked minima per (plan, knot)
 //   pmin   float[NB][M][chunks] masked minima per sample chunk
+//   dnew   float[NB][T][2]      interpolated plans of this step (before verification)
+//   fail   int[2][T]            verification failures per step: new plan, previous plan
+//   dense (above) holds the executed plans: the previous step's, then this step's choice
 //   part   double[NB][M][chunks][3]  weighted-average partials (num x, num y, den)
 #pragma once
 
@@ -25,6 +28,7 @@ constexpr int BM_BLOCK = 128;        // rollout lanes per workgroup
 constexpr int BM_MAX_TM = 8192;      // T * M held in LDS
 constexpr int BM_RED_BLOCK = 256;    // workgroup of the masked minima and weighted averages
 constexpr int BM_CHUNK = 4096;       // samples per weighted-average workgroup
+constexpr int BM_VBLOCK = 128;       // verification lanes per workgroup
 
 struct BmInstDev
 {
@@ -44,6 +48,9 @@ struct BmInstDev
     long long part_off;  // double units, NB * M * n_chunks * 3
     int pmin_off;        // NB * M * n_chunks
     int nu_off;          // NB
+    int n_verif;         // verification samples (0: none)
+    int verif_h;         // verification horizon
+    int fail_off;        // 2 * T
 };
 
 struct BmDyn
@@ -51,8 +58,16 @@ struct BmDyn
     uint32_t step;
     float lambda;
     float belief[DR_SR_MAX_MODELS];
-    int branch; // branch index of the last action
-    float _pad[2];
+    int branch;  // branch index of the last action
+    int use_new; // the last step executed the new plan
+    float _pad;
+};
+
+// Verification tile: BM_VBLOCK samples of one instance against one plan set
+// (0 = this step's plans, 1 = the previous step's).
+struct BmVTile
+{
+    int inst, set, s0, _pad;
 };
 
 // Rollout tile: BM_BLOCK consecutive samples of one instance under one true model.

@@ -1,7 +1,8 @@
 // Branching spline MPPI on StratRace2D for gfx950: the reference's default
 // MPPI controller (USE_SPLINES, one model factor) without plan verification.
 // One control step of a batch:
-//   k_bm_rollout -> k_bm_aggregate -> k_bm_min -> k_bm_wavg -> k_bm_update -> k_bm_advance
+//   k_bm_rollout -> k_bm_aggregate -> k_bm_min -> k_bm_wavg -> k_bm_update
+//   [-> k_bm_verify, Clopper-Pearson on the host] -> k_bm_certify -> k_bm_advance
 //
 // Reference kernels these replace (src/kernels.cu, src/mppi_controller.cu):
 //   generateNoiseKernel :18-31 + fullRolloutKernel :34-171  -> k_bm_rollout
@@ -11,6 +12,8 @@
 //   interpolateSplineKernel :411-431, clampNominalKernel :434-456,
 //   shiftSplineKernel :461-486, host action / temperature / shift
 //   mppi_controller.cu:385-457                              -> k_bm_update
+//   verifyNominalFailureKernel :488-551                     -> k_bm_verify
+//   plan choice, action, shift mppi_controller.cu:379-436   -> k_bm_certify
 //   SimulationEngine::dynStep (engine.cpp:286-312)          -> k_bm_advance
 //
 // Arithmetic: sums written a += b * c in the reference are fused
@@ -340,15 +343,14 @@ __global__ __launch_bounds__(BM_RED_BLOCK) void k_bm_wavg(const BmInstDev* __res
     }
 }
 
-// Per instance: knots += num/den (den > 1e-30), dense plans = B * knots,
-// L2 clamp of the knots to the ego's max acceleration, action = first step
-// of the dense plan of the confident branch, temperature update from that
-// branch's nu, zero-fill shift of that dense plan, spline shift of its knots
-// (knot m <- spline at min(tau_m + 1, T - 1)).
+// Per instance: knots += num/den (den > 1e-30), this step's dense plans
+// dnew = B * knots, L2 clamp of the knots to the ego's max acceleration,
+// temperature update from the confident branch's nu, spline shift of that
+// branch's knots (knot m <- spline at min(tau_m + 1, T - 1)); clears the
+// verification counters.  The action and the executed plans are k_bm_certify's.
 __global__ __launch_bounds__(256) void k_bm_update(const BmInstDev* __restrict__ insts, BmDyn* __restrict__ dyn,
                                                    const double* __restrict__ part, const float* __restrict__ basis, float* __restrict__ knots,
-                                                   float* __restrict__ dense, float* __restrict__ nu, float* __restrict__ actions,
-                                                   float max_accel)
+                                                   float* __restrict__ dnew, float* __restrict__ nu, int* __restrict__ fail, float max_accel)
 {
     __shared__ float s_kn[(DR_SR_MAX_MODELS + 1) * DR_MAX_KNOTS * 2];
     __shared__ float s_sh[DR_MAX_KNOTS * 2];
@@ -391,8 +393,11 @@ __global__ __launch_bounds__(256) void k_bm_update(const BmInstDev* __restrict__
         float sum = 0.0f;
         for (int m = 0; m < M; ++m)
             sum = fma_(B[t * M + m], s_kn[(b * M + m) * 2 + d], sum);
-        dense[I.dense_off + i] = sum;
+        dnew[I.dense_off + i] = sum;
     }
+    if (I.n_verif > 0)
+        for (int i = tid; i < 2 * T; i += blockDim.x)
+            fail[I.fail_off + i] = 0;
     __syncthreads();
     // L2 clamp of every knot
     for (int bm = tid; bm < NB * M; bm += blockDim.x)
@@ -408,11 +413,8 @@ __global__ __launch_bounds__(256) void k_bm_update(const BmInstDev* __restrict__
     }
     __syncthreads();
     const int br = s_branch;
-    const int off = I.dense_off + br * T * 2;
     if (tid == 0)
     {
-        actions[inst * 2] = dense[off];
-        actions[inst * 2 + 1] = dense[off + 1];
         const float usedNu = nu[I.nu_off + br];
         if (I.adapt)
         {
@@ -422,7 +424,6 @@ __global__ __launch_bounds__(256) void k_bm_update(const BmInstDev* __restrict__
                 Y.lambda = Y.lambda * 1.2f;
         }
         Y.branch = br;
-        Y.step = Y.step + 1u;
     }
     // spline shift of the chosen branch from the clamped knots
     for (int i = tid; i < M * 2; i += blockDim.x)
@@ -437,20 +438,138 @@ __global__ __launch_bounds__(256) void k_bm_update(const BmInstDev* __restrict__
         s_sh[i] = sum;
     }
     __syncthreads();
-    // zero-fill shift of the chosen dense plan (reads the whole row first)
-    float shifted[4];
-    const int n_sh = T * 2;
-    int cnt = 0;
-    for (int i = tid; i < n_sh && cnt < 4; i += blockDim.x, ++cnt)
-        shifted[cnt] = (i + 2 < n_sh) ? dense[off + i + 2] : 0.0f;
-    __syncthreads();
-    cnt = 0;
-    for (int i = tid; i < n_sh && cnt < 4; i += blockDim.x, ++cnt)
-        dense[off + i] = shifted[cnt];
     for (int i = tid; i < NB * M * 2; i += blockDim.x)
     {
         const int b = i / (M * 2);
         knots[I.knot_off + i] = (b == br) ? s_sh[i - b * M * 2] : s_kn[i];
+    }
+}
+
+// verifyNominalFailureKernel (kernels.cu:488-551): one lane per verification
+// sample and plan set.  The opponent model is drawn from the controller's
+// belief; the world runs the executed-plan rule of the controller (action of
+// the current branch at local time, belief update and branching, noise on,
+// the opponent's win not a loss) for verif_h steps; a loss at step t counts
+// one failure at t.
+__global__ __launch_bounds__(BM_VBLOCK) void k_bm_verify(const BmInstDev* __restrict__ insts, const BmDyn* __restrict__ dyn,
+                                                         const BmVTile* __restrict__ tiles, int n_tiles, const dr_sr_env* __restrict__ env,
+                                                         const dr_sr_state* __restrict__ states, const float* __restrict__ dnew,
+                                                         const float* __restrict__ dense, int* __restrict__ fail)
+{
+    __shared__ float s_p[NS * 2];
+    __shared__ float s_t[NS * 2];
+    __shared__ float s_k[NS];
+    __shared__ float s_plan[(DR_SR_MAX_MODELS + 1) * DR_MAX_T * 2];
+    __shared__ int s_fail[DR_MAX_T];
+    __shared__ float s_bc[2 * DR_SR_MAX_MODELS];
+
+    const int tid = threadIdx.x;
+    const dr_sr_env& E = *env;
+    for (int i = tid; i < NS * 2; i += BM_VBLOCK)
+    {
+        s_p[i] = (&E.p_grid[0][0])[i];
+        s_t[i] = (&E.t_grid[0][0])[i];
+    }
+    for (int i = tid; i < NS; i += BM_VBLOCK)
+        s_k[i] = E.kappa_grid[i];
+    if (tid < E.n_models)
+        belief_consts(E, tid, &s_bc[tid], &s_bc[DR_SR_MAX_MODELS + tid]);
+    const Tables tb{s_p, s_t, s_k, E.track_length};
+
+    for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
+    {
+        const BmVTile vt = tiles[tile];
+        const BmInstDev& I = insts[vt.inst];
+        const BmDyn& Y = dyn[vt.inst];
+        const int T = I.T, NB = I.NB, NT = I.NT, H = I.verif_h;
+        const float* plans = (vt.set == 0 ? dnew : dense) + I.dense_off;
+        __syncthreads(); // previous tile done with the LDS plans and counters
+        for (int i = tid; i < NB * T * 2; i += BM_VBLOCK)
+            s_plan[i] = plans[i];
+        for (int i = tid; i < H; i += BM_VBLOCK)
+            s_fail[i] = 0;
+        __syncthreads();
+
+        const int sidx = vt.s0 + tid;
+        if (sidx < I.n_verif)
+        {
+            const uint32_t step = Y.step, word = (uint32_t)sidx | ((uint32_t)vt.set << 30);
+            float belief[DR_SR_MAX_MODELS];
+            for (int q = 0; q < DR_SR_MAX_MODELS; ++q)
+                belief[q] = Y.belief[q];
+            int pred = find_confident(belief, NT, I.min_conf);
+            const U4 r = philox4x32_10(U4{word, 0xFFFFFFFFu, step, STREAM_VERIFY}, I.key0, I.key1);
+            const int theta = model_from_belief(uniform_open0(r.x), belief, NT);
+            St s = state_from_abi(states[vt.inst]);
+            int origin = 0;
+            for (int t = 0; t < H; ++t)
+            {
+                const float* u = s_plan + (pred * T + (t - origin)) * 2;
+                const float ego[2] = {u[0], u[1]};
+                NormalSeq rng = normal_seq(I.key0, I.key1, word, (uint32_t)t, step, STREAM_VERIFY);
+                const int term = env_step_t<true>(tb, E, theta, ego, s, rng, false, belief, s_bc, pred == 0);
+                if (pred == 0)
+                {
+                    const int np = find_confident(belief, NT, I.min_conf);
+                    if (np != 0)
+                    {
+                        pred = np;
+                        origin = t + 1;
+                    }
+                }
+                if (term == SR_LOSE)
+                {
+                    atomicAdd(&s_fail[t], 1);
+                    break;
+                }
+                if (term != SR_NONE)
+                    break;
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < H; i += BM_VBLOCK)
+            if (s_fail[i] != 0)
+                atomicAdd(&fail[I.fail_off + vt.set * T + i], s_fail[i]);
+    }
+}
+
+// The plan choice (mppi_controller.cu:379-436): the executed plans are this
+// step's when verdict[i] is 1 (or no verification runs), else the previous
+// step's; the action is the first step of the confident branch, whose row is
+// then shifted left with a zero fill.
+__global__ __launch_bounds__(256) void k_bm_certify(const BmInstDev* __restrict__ insts, BmDyn* __restrict__ dyn,
+                                                    const int* __restrict__ verdict, const float* __restrict__ dnew,
+                                                    float* __restrict__ dense, float* __restrict__ actions)
+{
+    const int inst = blockIdx.x;
+    const BmInstDev& I = insts[inst];
+    BmDyn& Y = dyn[inst];
+    const int T = I.T, NB = I.NB, tid = threadIdx.x;
+    const int use_new = I.n_verif > 0 ? verdict[inst] : 1;
+    const int br = Y.branch;
+    const float* src = (use_new ? dnew : dense) + I.dense_off;
+    float* dst = dense + I.dense_off;
+    const int n = NB * T * 2;
+    float v[2 * (DR_SR_MAX_MODELS + 1) * DR_MAX_T / 256];
+    int cnt = 0;
+    for (int i = tid; i < n; i += 256, ++cnt)
+    {
+        const int b = i / (T * 2);
+        v[cnt] = b != br ? src[i] : (i + 2 < (b + 1) * T * 2 ? src[i + 2] : 0.0f);
+    }
+    if (tid == 0)
+    {
+        actions[inst * 2] = src[br * T * 2];
+        actions[inst * 2 + 1] = src[br * T * 2 + 1];
+    }
+    __syncthreads();
+    cnt = 0;
+    for (int i = tid; i < n; i += 256, ++cnt)
+        dst[i] = v[cnt];
+    if (tid == 0)
+    {
+        Y.use_new = use_new;
+        Y.step = Y.step + 1u;
     }
 }
 
@@ -515,9 +634,24 @@ hipError_t launch_bm_reduce(int n_inst, int max_nbk, int n_rtiles, hipStream_t s
 }
 
 hipError_t launch_bm_update(int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const double* part, const float* basis,
-                            float* knots, float* dense, float* nu, float* actions, float max_accel)
+                            float* knots, float* dnew, float* nu, int* fail, float max_accel)
 {
-    k_bm_update<<<n_inst, 256, 0, st>>>(insts, dyn, part, basis, knots, dense, nu, actions, max_accel);
+    k_bm_update<<<n_inst, 256, 0, st>>>(insts, dyn, part, basis, knots, dnew, nu, fail, max_accel);
+    return hipGetLastError();
+}
+
+hipError_t launch_bm_verify(int n_tiles, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmVTile* tiles, const dr_sr_env* env,
+                            const dr_sr_state* states, const float* dnew, const float* dense, int* fail)
+{
+    if (n_tiles > 0)
+        k_bm_verify<<<n_tiles, BM_VBLOCK, 0, st>>>(insts, dyn, tiles, n_tiles, env, states, dnew, dense, fail);
+    return hipGetLastError();
+}
+
+hipError_t launch_bm_certify(int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const int* verdict, const float* dnew,
+                             float* dense, float* actions)
+{
+    k_bm_certify<<<n_inst, 256, 0, st>>>(insts, dyn, verdict, dnew, dense, actions);
     return hipGetLastError();
 }
 

@@ -34,6 +34,8 @@ enum : uint32_t
     STREAM_BRANCH = 2u,  // PR-MPPI branch beliefs and opponent velocity shifts
     STREAM_MODEL = 3u,   // PR-MPPI opponent-model draws, counter (m, 0, step, 3)
     STREAM_ENV = 4u,     // environment noise of a 2-D race rollout, counter (k, 4 t + block, step, 4)
+    STREAM_VERIFY = 5u,  // plan verification: model draw (s | set << 30, 0xFFFFFFFF, step, 5),
+                         // environment noise (s | set << 30, 4 t + block, step, 5)
 };
 
 struct U4
@@ -198,13 +200,11 @@ DR_HD float expf_ieee(float x)
     return p * u2f((uint32_t)(n + 127) << 23);
 }
 
-// Opponent model m of a sampled PR-MPPI step: inverse CDF of the belief at
-// u in (0, 1] (prmppi_kernels.cu:34-47: running FP32 sum, first i with
-// u <= sum, else the last model).
-DR_HD int sample_branch(uint32_t k0, uint32_t k1, uint32_t m, uint32_t step, const float* belief, int n_branches)
+// Inverse CDF of the belief at u in (0, 1] (prmppi_kernels.cu:34-47,
+// sampleModelFromBelief state.h:118-131): running FP32 sum, first i with
+// u <= sum, else the last model.
+DR_HD int model_from_belief(float u, const float* belief, int n_branches)
 {
-    const U4 r = philox4x32_10(U4{m, 0u, step, STREAM_MODEL}, k0, k1);
-    const float u = uniform_open0(r.x);
     float acc = 0.0f;
     for (int i = 0; i < n_branches; ++i)
     {
@@ -213,6 +213,13 @@ DR_HD int sample_branch(uint32_t k0, uint32_t k1, uint32_t m, uint32_t step, con
             return i;
     }
     return n_branches - 1;
+}
+
+// Opponent model m of a sampled PR-MPPI step.
+DR_HD int sample_branch(uint32_t k0, uint32_t k1, uint32_t m, uint32_t step, const float* belief, int n_branches)
+{
+    const U4 r = philox4x32_10(U4{m, 0u, step, STREAM_MODEL}, k0, k1);
+    return model_from_belief(uniform_open0(r.x), belief, n_branches);
 }
 
 } // namespace dr

@@ -16,9 +16,9 @@
 //
 // Differences from the reference, by design: the world's noise comes from
 // the engine's Philox stream (the reference draws it from std::mt19937 with
-// std::normal_distribution); plan verification is not run, so a STATE frame
-// carries failCount 0, epsilon 0, epsilonPartial 0, useNewPlan 1 and
-// certifiedLoss 0; nothing is printed.
+// std::normal_distribution); with nVerifSamples = 0 no verification runs and
+// the new plan is always executed (STATE frames carry failCount 0, epsilon 0,
+// epsilonPartial 0, useNewPlan 1, certifiedLoss 0); nothing is printed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -285,12 +285,15 @@ struct dr_session
         const int T = this->T(), NT = this->NT();
         if (bm)
         {
+            dr_bm_diag d;
+            if (int rc = dr_bm_get_diag(bm, &d, 1))
+                return rc;
             f.arr(b, NT);
-            f.i32(0);      // failCount (no verification)
-            f.f32(0.0f);   // epsilon
-            f.f32(0.0f);   // epsilonPartial
-            f.i32(1);      // useNewPlan
-            f.f32(0.0f);   // certifiedLoss
+            f.i32((int32_t)d.fail_count);
+            f.f32((float)d.epsilon);
+            f.f32((float)d.epsilon_partial);
+            f.i32(d.use_new_plan);
+            f.f32((float)d.certified_loss);
             f.i32(NT);
             std::vector<float> nominal((size_t)(NT + 1) * T * 2);
             if (int rc = dr_bm_get_nominal(bm, 0, nominal.data(), (int)nominal.size()))
@@ -607,6 +610,12 @@ int dr_session_create(const void* env_frame, size_t env_len, const void* cont_fr
         ins.true_theta = s->info.true_theta;
         ins.adapt_lambda = 1; // mppi_controller.cu:413-424
         std::memcpy(ins.init_belief, s->info.init_belief, sizeof(ins.init_belief));
+        ins.n_verif_samples = k.n_verif_samples;
+        // a horizon beyond T would read past the plans (kernels.cu:525-531 indexes the
+        // nominal at local time < verifHorizon); it is capped at T
+        ins.verif_horizon = std::min(k.verif_horizon, k.n_timesteps);
+        ins.beta = k.beta;
+        ins.max_verif_eps = k.max_verif_eps;
         if (int rc = dr_bm_create(&s->env, &k.cost, &ins, 1, &s->bm))
             return bail(rc);
         if (int rc = dr_bm_set_states(s->bm, &s->state, 1))

@@ -360,10 +360,10 @@ DR_HD void opp_action(const Tables& tb, const dr_sr_env& E, const dr_sr_opp_conf
 }
 
 // Sequential standard normals of one (sample, step): block b of four from
-// Philox counter (k, 4 t + b, step, STREAM_ENV).
+// Philox counter (k, 4 t + b, step, stream) (STREAM_ENV unless stated).
 struct NormalSeq
 {
-    uint32_t key0, key1, k, c1, step;
+    uint32_t key0, key1, k, c1, step, stream;
     int n;
     float z0, z1, z2, z3;
     DR_HD float next()
@@ -372,7 +372,7 @@ struct NormalSeq
         if (j == 0)
         {
             float z[4];
-            normal4(key0, key1, k, c1 + (uint32_t)(n >> 2), step, STREAM_ENV, z);
+            normal4(key0, key1, k, c1 + (uint32_t)(n >> 2), step, stream, z);
             z0 = z[0];
             z1 = z[1];
             z2 = z[2];
@@ -383,7 +383,7 @@ struct NormalSeq
     }
 };
 
-DR_HD NormalSeq normal_seq(uint32_t key0, uint32_t key1, uint32_t k, uint32_t t, uint32_t step)
+DR_HD NormalSeq normal_seq(uint32_t key0, uint32_t key1, uint32_t k, uint32_t t, uint32_t step, uint32_t stream = STREAM_ENV)
 {
     NormalSeq q;
     q.key0 = key0;
@@ -391,6 +391,7 @@ DR_HD NormalSeq normal_seq(uint32_t key0, uint32_t key1, uint32_t k, uint32_t t,
     q.k = k;
     q.c1 = 4u * t;
     q.step = step;
+    q.stream = stream;
     q.n = 0;
     q.z0 = q.z1 = q.z2 = q.z3 = 0.0f;
     return q;

@@ -615,8 +615,18 @@ class BMEngine:
     def set_timing(self, on=True):
         check(self.lib.dr_bm_set_timing(self.h, 1 if on else 0))
 
+    def fail_counts(self, i):
+        """Verification failures per horizon step of the last step: (new plan, previous plan)."""
+        T = self.sc.instances[i].n_timesteps
+        a = np.empty(T, dtype=np.int32)
+        b = np.empty(T, dtype=np.int32)
+        ip = C.POINTER(C.c_int32)
+        check(self.lib.dr_bm_get_fail_counts(self.h, i, a.ctypes.data_as(ip), b.ctypes.data_as(ip), T))
+        return a, b
+
     def last_timing(self):
-        """Device ms summed over the last timed run: rollout, aggregation + minima, weighted averages, update + world, step."""
-        a = np.empty(5, dtype=np.float32)
-        check(self.lib.dr_bm_last_timing(self.h, _fptr(a), 5))
+        """Device ms summed over the last timed run: rollout, aggregation + minima, weighted averages,
+        update + verification, plan choice + world, step (host Clopper-Pearson time included)."""
+        a = np.empty(6, dtype=np.float32)
+        check(self.lib.dr_bm_last_timing(self.h, _fptr(a), 6))
         return a

@@ -384,10 +384,15 @@ def bm_knots(n_timesteps, n_knots):
 
 
 def bm_instance(n_samples, n_timesteps, seed, n_knots=20, knots=None, lam=10.0, sigma=0.5, min_confidence=0.95, true_theta=0,
-                adapt=1, belief=(0.5, 0.5)):
+                adapt=1, belief=(0.5, 0.5), n_verif=0, verif_horizon=40, beta=1e-5, max_verif_eps=0.001):
+    """One branching spline MPPI controller; n_verif > 0 adds the plan verification (the
+    defaults are mainGate's: 2^17 samples, beta 1e-5, horizon 40, maxVerifEps 0.001,
+    python-src/main.py:1022-1026)."""
     tau = bm_knots(n_timesteps, n_knots) if knots is None else list(knots)
     ins = abi.BmInstance(n_samples=n_samples, n_timesteps=n_timesteps, n_knots=len(tau), seed=seed, lambda_=lam, sigma=sigma,
-                         min_confidence=min_confidence, true_theta=true_theta, adapt_lambda=adapt)
+                         min_confidence=min_confidence, true_theta=true_theta, adapt_lambda=adapt, n_verif_samples=n_verif,
+                         verif_horizon=min(verif_horizon, n_timesteps) if n_verif else 0, beta=beta if n_verif else 0.0,
+                         max_verif_eps=max_verif_eps if n_verif else 0.0)
     for m, t in enumerate(tau):
         ins.knots[m] = int(t)
     for b, p in enumerate(belief):
@@ -402,17 +407,18 @@ def bm_cost():
                       win_cost=1e5, opp_win_cost=1e5, final_adv_weight=50.0, final_opp_adv_weight=0.0)
 
 
-def bm_race(n_inst=1, n_samples=65536, n_timesteps=80, n_knots=20, seed=42, steps=300, n_models=2, **env_kw):
+def bm_race(n_inst=1, n_samples=65536, n_timesteps=80, n_knots=20, seed=42, steps=300, n_models=2, n_verif=0, **env_kw):
     """The reference's default branching spline MPPI race on StratRace2D (stratRaceEnv with
     splines, main.py:944-968): K = 2^16, T = 80, 20 knots, temperature 10 (adapted), sigma 0.5,
     confidence 0.95, two opponent models with a uniform initial belief; world i follows model
-    i % n_models."""
+    i % n_models; n_verif > 0 adds mainGate's plan verification (main.py:1022-1026)."""
     env = sr_env(n_models=n_models, **env_kw)
     belief = [1.0 / n_models] * n_models
-    inst = [bm_instance(n_samples, n_timesteps, seed_mix(seed, 500 + i), n_knots=n_knots, true_theta=i % n_models, belief=belief)
-            for i in range(n_inst)]
+    inst = [bm_instance(n_samples, n_timesteps, seed_mix(seed, 500 + i), n_knots=n_knots, true_theta=i % n_models, belief=belief,
+                        n_verif=n_verif) for i in range(n_inst)]
     st = [sr_start_state(env) for _ in range(n_inst)]
-    return SrScenario(f"bm_race_{n_inst}x_k{n_samples}_t{n_timesteps}_m{n_knots}", env, bm_cost(), inst, st, steps,
+    tag = f"_v{n_verif}" if n_verif else ""
+    return SrScenario(f"bm_race_{n_inst}x_k{n_samples}_t{n_timesteps}_m{n_knots}{tag}", env, bm_cost(), inst, st, steps,
                       f"branching spline MPPI on StratRace2D, {n_inst} worlds, K={n_samples}, T={n_timesteps}, {n_knots} knots, "
                       f"{n_models} opponent models")
 

@@ -473,7 +473,13 @@ typedef struct dr_bm_instance
     int32_t true_theta;              /* opponent model of the instance's world */
     int32_t adapt_lambda;
     float init_belief[DR_SR_MAX_MODELS];
-    int32_t _pad;
+    /* plan verification (verifyNominalFailureKernel + Clopper-Pearson, kernels.cu:488-551,
+     * mppi_controller.cu:352-377, 468-514); n_verif_samples = 0 runs none and always uses the
+     * new plan (the reference would keep the previous plan, its certified loss being 1) */
+    int32_t n_verif_samples;         /* nVerifSamples, 0..2^22 */
+    int32_t verif_horizon;           /* verifHorizon, 1..T */
+    float beta;                      /* confidence parameter of the bounds */
+    float max_verif_eps;             /* maxVerifEps: the new plan is used iff its certified loss is below */
 } dr_bm_instance;
 
 typedef struct dr_bm_engine dr_bm_engine;
@@ -506,17 +512,26 @@ int dr_bm_get_min_costs(dr_bm_engine* e, int instance, float* mins, int n);
 int dr_bm_get_nu(dr_bm_engine* e, int instance, float* nu, int n);
 typedef struct dr_bm_diag
 {
-    float lambda;    /* temperature of the next step */
-    float nu;        /* nu of the chosen branch in the last step */
-    float min_cost;  /* masked minimum of the chosen branch at knot 0 */
-    uint32_t step;   /* control steps taken */
-    int32_t branch;  /* branch of the last action: 0 nominal, theta + 1 committed */
+    float lambda;           /* temperature of the next step */
+    float nu;               /* nu of the chosen branch in the last step */
+    float min_cost;         /* masked minimum of the chosen branch at knot 0 */
+    uint32_t step;          /* control steps taken */
+    int32_t branch;         /* branch of the last action: 0 nominal, theta + 1 committed */
+    int32_t use_new_plan;   /* 1: the new plan was executed, 0: the previous one */
+    uint32_t fail_count;    /* verification failures of the executed plan */
     int32_t _pad;
+    double epsilon;         /* epsilonPartial + verifHorizon * maxVerifEps */
+    double epsilon_partial; /* Clopper-Pearson upper bound of the failure rate */
+    double certified_loss;  /* max over t of upper(new) - lower(old); -1 before any test */
 } dr_bm_diag;
 /* Per-instance diagnostics (what getControl prints, mppi_controller.cu:394-424). */
 int dr_bm_get_diag(dr_bm_engine* e, dr_bm_diag* diag, int n);
+/* Verification failures per horizon step of the last step, new and previous plan ([T] each). */
+int dr_bm_get_fail_counts(dr_bm_engine* e, int instance, int32_t* fail_new, int32_t* fail_old, int n);
 int dr_bm_set_timing(dr_bm_engine* e, int enable);
-/* Device ms summed over the last timed run: rollout, aggregation + masked minima, weighted averages, update + world, step. */
+/* Device ms summed over the last timed run (6 values): rollout, aggregation + masked minima, weighted
+ * averages, update + verification, plan choice + world, step (with verification the step includes the
+ * host's Clopper-Pearson bounds). */
 int dr_bm_last_timing(dr_bm_engine* e, float* ms, int n);
 
 /* ---------------------------------------------------------------------

@@ -34,6 +34,9 @@
 #include <string.h>
 
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "oracle.h"
 
@@ -358,16 +361,17 @@ void opp_action(const Env<S>& E, const dr_sr_opp_config& c, const SState<S>& s, 
     out[1] = a_t * tang[1] + a_tot * norm[1];
 }
 
-// Sequential normals of one (sample, step) from Philox blocks (k, 4 t + b, step, 4).
+// Sequential normals of one (sample, step) from Philox blocks (k, 4 t + b, step, stream).
 struct Normals
 {
     uint32_t k0, k1, k, c1, step;
+    uint32_t stream = 4u; // 5: plan verification
     int n = 0;
     float z[4] = {0, 0, 0, 0};
     float next()
     {
         if ((n & 3) == 0)
-            normals4(k0, k1, k, c1 + (uint32_t)(n >> 2), step, 4u, z);
+            normals4(k0, k1, k, c1 + (uint32_t)(n >> 2), step, stream, z);
         const float v = z[n & 3];
         n = n + 1;
         return v;
@@ -985,13 +989,93 @@ void bm_rollout(const Env<S>& E, const dr_sr_cost& K, const dr_sr_state& x0, con
     }
 }
 
+// verifyNominalFailureKernel (kernels.cu:488-551) for one plan set (0: this
+// step's plans, 1: the previous step's): per verification sample, an
+// opponent model drawn from the belief (Philox (s | set << 30, 0xFFFFFFFF,
+// step, 5), inverse CDF), then verif_horizon world steps under the plan of
+// the current branch at local time with belief update and branching, noise
+// from stream 5, the opponent's win not a loss; a loss at step t adds one
+// failure at t.
+template <class S>
+void bm_verify(const Env<S>& E, const dr_sr_state& x0, const dr_bm_instance& I, const float* belief0f, uint32_t step, int set,
+               const S* plans, int n_threads, int* fail)
+{
+    const int NT = E.e.n_models, T = I.n_timesteps, H = I.verif_horizon;
+    const uint32_t key0 = (uint32_t)(I.seed & 0xffffffffu), key1 = (uint32_t)(I.seed >> 32);
+    for (int t = 0; t < T; ++t)
+        fail[t] = 0;
+    const int nt = n_threads > 0 ? n_threads : 1;
+    std::vector<int> local((size_t)nt * T, 0);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
+    for (int sidx = 0; sidx < I.n_verif_samples; ++sidx)
+    {
+#ifdef _OPENMP
+        int* cnt = &local[(size_t)omp_get_thread_num() * T];
+#else
+        int* cnt = &local[0];
+#endif
+        const uint32_t word = (uint32_t)sidx | ((uint32_t)set << 30);
+        S belief[DR_SR_MAX_MODELS];
+        float bf[DR_SR_MAX_MODELS];
+        for (int q = 0; q < NT; ++q)
+        {
+            belief[q] = (S)belief0f[q];
+            bf[q] = belief0f[q];
+        }
+        int pred = find_confident(belief, NT, I.min_confidence);
+        const W4 r = philox(W4{{word, 0xFFFFFFFFu, step, 5u}}, key0, key1);
+        const float u = (float)((r.v[0] >> 8) + 1u) * 5.9604644775390625e-08f;
+        int theta = NT - 1;
+        float acc = 0.0f;
+        for (int q = 0; q < NT; ++q)
+        {
+            acc = acc + bf[q];
+            if (u <= acc)
+            {
+                theta = q;
+                break;
+            }
+        }
+        SState<S> s = load<S>(x0);
+        int origin = 0;
+        for (int t = 0; t < H; ++t)
+        {
+            const S* pl = plans + ((size_t)pred * T + (t - origin)) * 2;
+            const S ego[2] = {pl[0], pl[1]};
+            Normals rng{key0, key1, word, 4u * (uint32_t)t, step};
+            rng.stream = 5u;
+            const int term = env_step(E, theta, ego, s, rng, false, belief);
+            if (pred == 0)
+            {
+                const int np = find_confident(belief, NT, I.min_confidence);
+                if (np != 0)
+                {
+                    pred = np;
+                    origin = t + 1;
+                }
+            }
+            if (term == 1)
+            {
+                cnt[t] += 1;
+                break;
+            }
+            if (term != 0)
+                break;
+        }
+    }
+    for (int q = 0; q < nt; ++q)
+        for (int t = 0; t < T; ++t)
+            fail[t] += local[(size_t)q * T + t];
+}
+
 // One control step of one branching spline MPPI controller (kernels.cu:175-486,
 // mppi_controller.cu:250-466 without verification).  In FP32 the weights
 // are FP32 and the weighted sums FP64, as in the engine.
 template <class S>
 void bm_step(const dr_sr_env& env, const dr_sr_cost& K, const dr_sr_state& x0, const dr_bm_instance& I, const float* belief, uint32_t step,
-             float lambda, const float* knots, int n_threads, float* noise_out, double* ctrue_out, int* used_out, int* btime_out,
-             double* costs_out, double* mins_out, double* nu_out, double* knots_out, double* dense_out, double* res_out)
+             float lambda, const float* knots, const float* prev_dense, int n_threads, float* noise_out, double* ctrue_out, int* used_out,
+             int* btime_out, double* costs_out, double* mins_out, double* nu_out, double* knots_out, double* dense_out,
+             double* dense_new_out, int* fail_out, double* res_out)
 {
     const Env<S> E(env);
     const int NT = env.n_models, NB = NT + 1, T = I.n_timesteps, M = I.n_knots, NK = I.n_samples;
@@ -1114,6 +1198,18 @@ void bm_step(const dr_sr_env& env, const dr_sr_cost& K, const dr_sr_state& x0, c
                     sum = fm(B[(size_t)t * M + m], kn[((size_t)b * M + m) * 2 + d], sum);
                 dense[((size_t)b * T + t) * 2 + d] = sum;
             }
+    if (dense_new_out)
+        for (size_t i = 0; i < dense.size(); ++i)
+            dense_new_out[i] = (double)dense[i];
+    if (fail_out && I.n_verif_samples > 0)
+    {
+        std::vector<S> prev(dense.size(), (S)0);
+        if (prev_dense)
+            for (size_t i = 0; i < prev.size(); ++i)
+                prev[i] = (S)prev_dense[i];
+        bm_verify<S>(E, x0, I, belief, step, 0, dense.data(), n_threads, fail_out);
+        bm_verify<S>(E, x0, I, belief, step, 1, prev.data(), n_threads, fail_out + T);
+    }
     // clampNominalKernel (kernels.cu:434-456) on the knots
     const S ma = (S)env.max_accel[0];
     for (int bm = 0; bm < NB * M; ++bm)
@@ -1422,11 +1518,13 @@ int or_sr_update_belief(int f64, const dr_sr_env* env, double* belief, const dou
 }
 
 // One world step with belief update (SimulationEngine::dynStep,
-// engine.cpp:286-312): the opponent follows theta; belief [n_models] in/out.
+// engine.cpp:286-312): the opponent follows theta; belief [n_models] in/out;
+// environment noise from Philox stream `stream` (4; 5 for the verification).
 int or_sr_env_step_belief(int f64, const dr_sr_env* env, int theta, const float* ego, dr_sr_state* st, float* belief, uint64_t seed,
-                          uint32_t k, uint32_t t, uint32_t step, int lose_on_opp_win)
+                          uint32_t k, uint32_t t, uint32_t step, int lose_on_opp_win, uint32_t stream)
 {
     Normals rng{(uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32), k, 4u * t, step};
+    rng.stream = stream;
     const int NT = env->n_models;
     if (f64)
     {
@@ -1455,21 +1553,27 @@ int or_sr_env_step_belief(int f64, const dr_sr_env* env, int theta, const float*
 // costs / branch used / branching time [n_models][K], aggregated costs
 // [n_models + 1][K], masked minima [n_models + 1][M], nu [n_models + 1],
 // knots after update + clamp + shift, dense plans [n_models + 1][T][2]
-// after the shift, res = action[2], next lambda, chosen branch.
+// after the shift (the new plan executed), res = action[2], next lambda,
+// chosen branch.  With verification: the unshifted new plans (dense_new_out)
+// and the failure counts [2][T] of the new plans and of prev_dense (the
+// previous step's executed plans, [n_models + 1][T][2]; NULL = zero).
 int or_bm_step(int f64, const dr_sr_env* env, const dr_sr_cost* K, const dr_sr_state* x0, const dr_bm_instance* inst, const float* belief,
-               uint32_t step, float lambda, const float* knots, int n_threads, float* noise_out, double* ctrue_out, int* used_out,
-               int* btime_out, double* costs_out, double* mins_out, double* nu_out, double* knots_out, double* dense_out, double* res_out)
+               uint32_t step, float lambda, const float* knots, const float* prev_dense, int n_threads, float* noise_out, double* ctrue_out,
+               int* used_out, int* btime_out, double* costs_out, double* mins_out, double* nu_out, double* knots_out, double* dense_out,
+               double* dense_new_out, int* fail_out, double* res_out)
 {
     if (!env || !K || !x0 || !inst || !belief || !knots || !mins_out || !nu_out || !knots_out || !dense_out || !res_out)
         return 1;
     if (inst->n_knots < 2 || inst->n_knots > DR_MAX_KNOTS || env->n_models < 1 || env->n_models > DR_SR_MAX_MODELS)
         return 1;
+    if (inst->n_verif_samples > 0 && (inst->verif_horizon < 1 || inst->verif_horizon > inst->n_timesteps))
+        return 1;
     if (f64)
-        bm_step<double>(*env, *K, *x0, *inst, belief, step, lambda, knots, n_threads, noise_out, ctrue_out, used_out, btime_out, costs_out,
-                        mins_out, nu_out, knots_out, dense_out, res_out);
+        bm_step<double>(*env, *K, *x0, *inst, belief, step, lambda, knots, prev_dense, n_threads, noise_out, ctrue_out, used_out, btime_out,
+                        costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, res_out);
     else
-        bm_step<float>(*env, *K, *x0, *inst, belief, step, lambda, knots, n_threads, noise_out, ctrue_out, used_out, btime_out, costs_out,
-                       mins_out, nu_out, knots_out, dense_out, res_out);
+        bm_step<float>(*env, *K, *x0, *inst, belief, step, lambda, knots, prev_dense, n_threads, noise_out, ctrue_out, used_out, btime_out,
+                       costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, res_out);
     return 0;
 }
 

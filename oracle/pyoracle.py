@@ -72,9 +72,9 @@ def lib(abi):
                                  F, C.POINTER(I), D, U, F, D, D, D, D]
     L.or_bm_basis.argtypes = [I, C.POINTER(I), I, I, D]
     L.or_sr_update_belief.argtypes = [I, C.POINTER(abi.SrEnv), D, D, D]
-    L.or_sr_env_step_belief.argtypes = [I, C.POINTER(abi.SrEnv), I, F, C.POINTER(abi.SrState), F, C.c_uint64, U, U, U, I]
+    L.or_sr_env_step_belief.argtypes = [I, C.POINTER(abi.SrEnv), I, F, C.POINTER(abi.SrState), F, C.c_uint64, U, U, U, I, U]
     L.or_bm_step.argtypes = [I, C.POINTER(abi.SrEnv), C.POINTER(abi.SrCost), C.POINTER(abi.SrState), C.POINTER(abi.BmInstance), F,
-                             U, C.c_float, F, I, F, D, C.POINTER(I), C.POINTER(I), D, D, D, D, D, D]
+                             U, C.c_float, F, F, I, F, D, C.POINTER(I), C.POINTER(I), D, D, D, D, D, D, C.POINTER(I), D]
     L.or_sr_predict.argtypes = [C.POINTER(abi.SrEnv), C.POINTER(abi.SrState), I, F, I, I, F, C.c_float, F, F, C.POINTER(I)]
     L.or_sr_opp_target.restype = C.c_double
     L.or_sr_opp_target.argtypes = [I, C.POINTER(abi.SrEnv), I, C.POINTER(abi.SrState)]
@@ -404,22 +404,26 @@ class Oracle:
         assert self.L.or_sr_update_belief(int(f64), C.byref(env), _d(b), _d(o), _d(n)) == 0
         return b
 
-    def sr_env_step_belief(self, f64, env, theta, ego, state, belief, seed, k, t, step, lose_on_opp_win=True):
+    def sr_env_step_belief(self, f64, env, theta, ego, state, belief, seed, k, t, step, lose_on_opp_win=True, stream=4):
         """One environment step with belief update; returns (state, belief, terminal code)."""
         st = self.abi.SrState.from_buffer_copy(state)
         u = np.ascontiguousarray(ego, dtype=np.float32)
         b = np.zeros(self.abi.SR_MAX_MODELS, dtype=np.float32)
         b[:len(belief)] = belief
         r = self.L.or_sr_env_step_belief(int(f64), C.byref(env), theta, _f(u), C.byref(st), _f(b), seed, k, t, step,
-                                         1 if lose_on_opp_win else 0)
+                                         1 if lose_on_opp_win else 0, stream)
         return st, b[:env.n_models].copy(), r
 
-    def bm_step(self, f64, env, cost, state, ins, belief, step, lam, knots, threads=8, want_noise=True):
-        """One branching spline MPPI step of a StratRace2D controller from its belief and knots [n_models + 1][M][2]."""
+    def bm_step(self, f64, env, cost, state, ins, belief, step, lam, knots, threads=8, want_noise=True, prev_dense=None):
+        """One branching spline MPPI step of a StratRace2D controller from its belief and knots [n_models + 1][M][2]
+        (and, with verification, the previous step's executed plans [n_models + 1][T][2])."""
         K, T, M, NT = ins.n_samples, ins.n_timesteps, ins.n_knots, env.n_models
         NB = NT + 1
         kn = np.ascontiguousarray(knots, dtype=np.float32).reshape(-1)
         assert kn.size == NB * M * 2
+        prev = None if prev_dense is None else np.ascontiguousarray(prev_dense, dtype=np.float32).reshape(-1)
+        dnew = np.empty(NB * T * 2)
+        fail = np.zeros(2 * T, dtype=np.int32)
         bel = np.zeros(self.abi.SR_MAX_MODELS, dtype=np.float32)
         bel[:NT] = belief[:NT]
         noise = np.empty(NB * M * K * 2, dtype=np.float32) if want_noise else None
@@ -433,14 +437,15 @@ class Oracle:
         dense = np.empty(NB * T * 2)
         res = np.empty(4)
         ip = C.POINTER(C.c_int)
-        rc = self.L.or_bm_step(int(f64), C.byref(env), C.byref(cost), C.byref(state), C.byref(ins), _f(bel), step, lam, _f(kn), threads,
+        rc = self.L.or_bm_step(int(f64), C.byref(env), C.byref(cost), C.byref(state), C.byref(ins), _f(bel), step, lam, _f(kn),
+                               _f(prev) if prev is not None else None, threads,
                                _f(noise) if noise is not None else None, _d(ct), used.ctypes.data_as(ip), bt.ctypes.data_as(ip),
-                               _d(costs), _d(mins), _d(nu), _d(kout), _d(dense), _d(res))
+                               _d(costs), _d(mins), _d(nu), _d(kout), _d(dense), _d(dnew), fail.ctypes.data_as(ip), _d(res))
         assert rc == 0
         return dict(noise=noise.reshape(NB, M, K, 2) if noise is not None else None, ctrue=ct.reshape(NT, K),
                     used=used.reshape(NT, K), btime=bt.reshape(NT, K), costs=costs.reshape(NB, K), mins=mins.reshape(NB, M),
                     nu=nu, knots=kout.reshape(NB, M, 2), dense=dense.reshape(NB, T, 2), action=res[:2], lambda_=float(res[2]),
-                    branch=int(res[3]))
+                    branch=int(res[3]), dense_new=dnew.reshape(NB, T, 2), fail_new=fail[:T].copy(), fail_old=fail[T:].copy())
 
     # ---------------------------------------------------------- session display predictions
 

@@ -130,3 +130,96 @@ def test_bm_reference_size_step(pkg, oracle):
     acts = eng.step(1, advance=False)
     ins = sc.instances[0]
     check_step(oracle, eng, sc, states, [eng.belief(0)], [np.zeros((3, ins.n_knots, 2), np.float32)], [ins.lambda_], acts, 0)
+
+
+def cp_bounds(n):
+    from scipy.stats import beta as bd
+
+    def upper(k, a):
+        return 1.0 if k == n else float(bd.ppf(1 - a, k + 1, n - k))
+
+    def lower(k, a):
+        return 0.0 if k == 0 else float(bd.ppf(a, k, n - k + 1))
+    return upper, lower
+
+
+def check_verification(eng, i, ins, r, prev, T):
+    """Failure counts bit-exact; certified loss, decision, epsilon against SciPy's beta quantiles
+    (computeCertifiedLoss / computeEpsilon, mppi_controller.cu:468-514); the executed plans."""
+    fnew, fold = eng.fail_counts(i)
+    np.testing.assert_array_equal(fnew, r["fail_new"])
+    np.testing.assert_array_equal(fold, r["fail_old"])
+    n = ins.n_verif_samples
+    upper, lower = cp_bounds(n)
+    half = float(np.float32(ins.beta)) / 2
+    cn, co = np.cumsum(fnew), np.cumsum(fold)
+    cert = max([-1.0] + [upper(int(cn[t]), half) - lower(int(co[t]), half) for t in range(T)])
+    d = eng.diag()[i]
+    assert d.certified_loss == pytest.approx(cert, rel=1e-9, abs=1e-12)
+    use_new = int(cert < float(np.float32(ins.max_verif_eps)))
+    assert d.use_new_plan == use_new
+    total = int(cn[-1] if use_new else co[-1])
+    assert d.fail_count == total
+    eps_p = upper(total, float(np.float32(ins.beta)))
+    assert d.epsilon_partial == pytest.approx(eps_p, rel=1e-9, abs=1e-12)
+    assert d.epsilon == pytest.approx(eps_p + ins.verif_horizon * float(np.float32(ins.max_verif_eps)), rel=1e-9)
+    chosen = (r["dense_new"] if use_new else prev.astype(np.float64)).copy()
+    br = r["branch"]
+    want = chosen.copy()
+    want[br, :-1] = chosen[br, 1:]
+    want[br, -1] = 0.0
+    assert rel_err(eng.nominal(i), want) <= REL
+    return use_new, chosen[br, 0], cert
+
+
+@pytest.mark.parametrize("plan", ["safe", "crash"])
+def test_bm_verification_parity(pkg, oracle, plan):
+    say(f"[bm verification] {plan} new plan")
+    sc = scenario(pkg, [(1024, 30, 8)], min_conf=0.6, sigma=0.2)
+    ins = sc.instances[0]
+    # 4096 samples certify no loss below upper(0) = 1 - (beta / 2)^(1/4096) ~ 3e-3: the bound is
+    # maxVerifEps = 0.01 (MPPIConfig's default, utils.py:488)
+    ins.n_verif_samples, ins.verif_horizon, ins.beta, ins.max_verif_eps = 4096, 24, 1e-5, 0.01
+    eng = pkg.BMEngine(sc)
+    env = sc.env
+    tang = np.array(env.t_grid[77][:])
+    normal = np.float32([-tang[1], tang[0]])
+    kn = np.zeros((3, 8, 2), np.float32)
+    if plan == "crash":
+        kn[:] = 1.9 * normal
+    eng.set_knots(0, kn)
+    prev = eng.nominal(0)
+    states = [copy.deepcopy(s) for s in eng.get_states()]
+    belief = eng.belief(0)
+    acts = eng.step(1, advance=False)
+    r = oracle.bm_step(0, env, sc.cost, states[0], ins, belief, 0, ins.lambda_, kn, prev_dense=prev)
+    assert np.array_equal(eng.noise(0).view(np.uint32), r["noise"].view(np.uint32))
+    np.testing.assert_array_equal(eng.costs_true(0).astype(np.float64), r["ctrue"])
+    use_new, a0, cert = check_verification(eng, 0, ins, r, prev, 30)
+    assert rel_err(acts[0], a0) <= REL
+    say(f"  failures new {int(r['fail_new'].sum())} old {int(r['fail_old'].sum())}; certified loss {cert:.3e} -> "
+        f"{'new' if use_new else 'previous'} plan; bit-exact counts")
+    assert use_new == (1 if plan == "safe" else 0)
+
+
+def test_bm_verification_closed_loop(pkg, oracle):
+    say("[bm verification closed loop] 3 steps")
+    sc = scenario(pkg, [(768, 30, 8)], min_conf=0.8, sigma=0.5)
+    ins = sc.instances[0]
+    ins.n_verif_samples, ins.verif_horizon, ins.beta, ins.max_verif_eps = 2048, 20, 1e-5, 0.05
+    eng = pkg.BMEngine(sc)
+    state = copy.deepcopy(eng.get_states()[0])
+    belief = eng.belief(0)
+    knots, prev, lam = eng.knots(0), eng.nominal(0), ins.lambda_
+    for step in range(3):
+        acts = eng.step(1, advance=True)
+        r = oracle.bm_step(0, sc.env, sc.cost, state, ins, belief, step, lam, knots, prev_dense=prev)
+        use_new, a0, cert = check_verification(eng, 0, ins, r, prev, 30)
+        assert rel_err(acts[0], a0) <= REL
+        s2, b2, term = oracle.sr_env_step_belief(0, sc.env, ins.true_theta, acts[0], state, belief, ins.seed, 0xFFFFFFFF, step, 0,
+                                                 lose_on_opp_win=False)
+        assert bytes(eng.get_states()[0]) == bytes(s2)
+        assert np.array_equal(eng.belief(0).view(np.uint32), b2.view(np.uint32))
+        state, belief = s2, b2
+        knots, prev, lam = eng.knots(0), eng.nominal(0), eng.diag()[0].lambda_
+        say(f"  step {step}: certified loss {cert:.3e}, {'new' if use_new else 'previous'} plan, world identical")

@@ -50,7 +50,7 @@ def tail_frames(wire, step, ev, max_steps):
 
 def test_mppi_session_frames(pkg, oracle):
     wire, abi = pkg.wire, pkg.abi
-    say("[session] branching spline MPPI, K=1024 T=30 M=8, 6 steps")
+    say("[session] branching spline MPPI with plan verification (2^17 samples), K=1024 T=30 M=8, 6 steps")
     e = wire.stratrace_env(n_models=2, seed=77, true_theta=1)
     env_f, cont_f, st_f = wire.env_header(e), wire.mppi_header(n_samples=1024, n_timesteps=30, n_knots=8), wire.init_state_header(e)
     sess = wire.Session(env_f, cont_f, st_f, max_steps=6)
@@ -61,7 +61,9 @@ def test_mppi_session_frames(pkg, oracle):
     cont = wire.decode_controller(cont_f)
     x0 = wire.decode_state(st_f)
     ins = pkg.scenarios.bm_instance(1024, 30, 77, knots=list(cont.knots[:8]), lam=cont.lambda_, sigma=cont.sigma,
-                                    min_confidence=cont.min_confidence, true_theta=1, adapt=1, belief=list(info.init_belief[:2]))
+                                    min_confidence=cont.min_confidence, true_theta=1, adapt=1, belief=list(info.init_belief[:2]),
+                                    n_verif=cont.n_verif_samples, verif_horizon=min(cont.verif_horizon, 30), beta=cont.beta,
+                                    max_verif_eps=cont.max_verif_eps)
     eng = pkg.BMEngine(pkg.scenarios.SrScenario("session", env, cont.cost, [ins], [x0], 6, ""))
     T, NT = 30, 2
 
@@ -69,11 +71,12 @@ def test_mppi_session_frames(pkg, oracle):
         P = wire.Packer()
         state_part(P, step, st, prev, action, env, 1, oracle)
         P.arr(belief)
-        P.i32(0)
-        P.f32(0)
-        P.f32(0)
-        P.i32(1)
-        P.f32(0)
+        d = eng.diag()[0]
+        P.i32(d.fail_count)
+        P.f32(d.epsilon)
+        P.f32(d.epsilon_partial)
+        P.i32(d.use_new_plan)
+        P.f32(d.certified_loss)
         P.i32(NT)
         for theta in range(NT):
             pos, act, inf = oracle.sr_predict(env, st, theta, nominal, T, True, belief, cont.min_confidence)
@@ -103,7 +106,8 @@ def test_mppi_session_frames(pkg, oracle):
         assert got == w, f"frame {n} (type {t}) differs"
     d = wire.decode_frame(frames[-3][1], abi.CONT_MPPI)
     say(f"  {len(frames)} frames identical; step {d['step']} belief {d['belief']} predicted branches "
-        f"{[int(p['predTheta'][0]) for p in d['preds']]}")
+        f"{[int(p['predTheta'][0]) for p in d['preds']]}; verification: failures {d['failCount']} useNewPlan {d['useNewPlan']} "
+        f"certified loss {d['certifiedLoss']:.3e} epsilon {d['epsilon']:.3e}")
 
 
 def test_prmppi_session_frames(pkg, oracle):

@@ -279,3 +279,78 @@ def test_masked_minima_and_averages_brute_force(oracle, pkg):
             if b != r["branch"]:
                 np.testing.assert_allclose(r["knots"][b, m], knots[b, m] + inc, rtol=1e-9, atol=1e-12)
     assert B.shape == (T, M)
+
+
+def verify_replay(oracle, env, ins, x0, belief, step, plans, set_):
+    """Failure counts of verifyNominalFailureKernel rebuilt sample by sample through the
+    belief-updating world step (noise stream 5) and a NumPy inverse-CDF model draw."""
+    T, H, NT = ins.n_timesteps, ins.verif_horizon, env.n_models
+    key = [ins.seed & 0xFFFFFFFF, ins.seed >> 32]
+    fail = np.zeros(T, np.int32)
+    for s in range(ins.n_verif_samples):
+        word = s | (set_ << 30)
+        r = oracle.philox([word, 0xFFFFFFFF, step, 5], key)
+        u = np.float32(((r[0] >> 8) + 1) * 2.0 ** -24)
+        acc, theta = np.float32(0), NT - 1
+        for q in range(NT):
+            acc = np.float32(acc + np.float32(belief[q]))
+            if u <= acc:
+                theta = q
+                break
+        st, b = x0, np.array(belief, np.float32)
+        pred = next((i + 1 for i in range(NT) if b[i] > ins.min_confidence), 0)
+        origin = 0
+        for t in range(H):
+            st, b, term = oracle.sr_env_step_belief(0, env, theta, plans[pred, t - origin], st, b, ins.seed, word, t, step,
+                                                    lose_on_opp_win=False, stream=5)
+            if pred == 0:
+                np_ = next((i + 1 for i in range(NT) if b[i] > ins.min_confidence), 0)
+                if np_:
+                    pred, origin = np_, t + 1
+            if term == 1:
+                fail[t] += 1
+                break
+            if term:
+                break
+    return fail
+
+
+@pytest.mark.parametrize("start,belief", [((0.15, 0.2), (0.5, 0.5)), ((0.18, 0.2), (0.3, 0.7))])
+def test_verification_counts_replay(oracle, pkg, start, belief):
+    env = pkg.scenarios.sr_env(n_models=2, action_noise=0.3)
+    cost = pkg.scenarios.bm_cost()
+    K, T, tau = 32, 24, [0, 5, 11, 16, 22]
+    ins = make(pkg, K, T, tau, sigma=0.5, min_conf=0.6, belief=belief)
+    ins.n_verif_samples, ins.verif_horizon, ins.beta, ins.max_verif_eps = 48, 18, 1e-5, 0.001
+    x0 = pkg.scenarios.sr_start_state(env, start=start)
+    rng = np.random.default_rng(8)
+    knots = rng.normal(0, 1.2, (3, len(tau), 2)).astype(np.float32)
+    prev = rng.normal(0, 1.5, (3, T, 2)).astype(np.float32)
+    b = np.float32(belief)
+    r = oracle.bm_step(0, env, cost, x0, ins, b, 6, 10.0, knots, prev_dense=prev)
+    assert np.array_equal(r["fail_new"], verify_replay(oracle, env, ins, x0, b, 6, r["dense_new"].astype(np.float32), 0))
+    assert np.array_equal(r["fail_old"], verify_replay(oracle, env, ins, x0, b, 6, prev, 1))
+    assert np.all(r["fail_new"][ins.verif_horizon:] == 0)
+    # the executed plans with the new plan chosen: dense_new with the chosen branch shifted left, zero fill
+    want = r["dense_new"].copy()
+    want[r["branch"], :-1] = r["dense_new"][r["branch"], 1:]
+    want[r["branch"], -1] = 0.0
+    np.testing.assert_array_equal(r["dense"], want)
+
+
+def test_verification_closed_forms(oracle, pkg):
+    """Deterministic worlds: a plan that leaves the track fails every sample at the same step;
+    a standing plan fails none; the counts do not depend on the plan set's noise draw."""
+    env = quiet_env(pkg)
+    cost = pkg.scenarios.bm_cost()
+    K, T, tau = 8, 30, [0, 7, 14, 21, 28]
+    ins = make(pkg, K, T, tau, sigma=0.0, min_conf=2.0)
+    ins.n_verif_samples, ins.verif_horizon, ins.beta, ins.max_verif_eps = 64, 30, 1e-5, 0.001
+    x0 = pkg.scenarios.sr_start_state(env, start=(0.15, 0.2))
+    out = np.zeros((3, T, 2), np.float32)
+    tang = np.array(env.t_grid[77][:])
+    normal = np.array([-tang[1], tang[0]], np.float32)
+    out[:] = 2.0 * normal  # full acceleration across the track
+    r = oracle.bm_step(0, env, cost, x0, ins, np.float32([0.5, 0.5]), 0, 10.0, np.zeros((3, len(tau), 2), np.float32), prev_dense=out)
+    assert r["fail_old"].sum() == 64 and np.count_nonzero(r["fail_old"]) == 1
+    assert r["fail_new"].sum() == 0  # zero knots, sigma 0: the ego stands still, nothing fails within 30 steps

@@ -336,14 +336,21 @@ struct BranchingMPPIConfig
     float samplingNoise = 0.5f;
     float minConfidence = 0.95f;
     dr_sr_cost cost{1.4f, 50.0f, 1.8f, 1e7f, 1e5f, 1e5f, 50.0f, 0.0f};
+    // plan verification (0 samples: none, the new plan is always executed)
+    int nVerifSamples = 0;
+    float beta = 1e-6f;
+    int verifHorizon = 40;
+    float maxVerifEps = 0.01f;
 };
 
 // MPPIController of the StratRace2D build (controllers.h:23-111) over the
 // branching engine (dr_bm_*).  h_belief is the controller's belief, written
 // by the caller's dynStep as in the reference (engine.cpp:332); h_nominal is
-// the dense plans of every branch after the step ([nModels + 1][T][2], the
-// chosen branch shifted).  Plan verification is not run: failCount stays
-// empty, epsilon / epsilonPartial / certifiedLoss 0 and useNewPlan true.
+// the executed dense plans of every branch after the step ([nModels + 1][T][2],
+// the chosen branch shifted).  With nVerifSamples > 0 the plan verification
+// runs and failCount (per horizon step, of the executed plan), epsilon,
+// epsilonPartial, certifiedLoss and useNewPlan are its outcome; without it
+// failCount is all zero, the three bounds 0 and useNewPlan true.
 class BranchingMPPIController
 {
   public:
@@ -368,7 +375,12 @@ class BranchingMPPIController
         in.adapt_lambda = 1; // mppi_controller.cu:413-424
         for (int th = 0; th < c.n_models && th < DR_SR_MAX_MODELS; ++th)
             in.init_belief[th] = 1.0f / (float)c.n_models;
+        in.n_verif_samples = mc.nVerifSamples;
+        in.verif_horizon = mc.verifHorizon;
+        in.beta = mc.beta;
+        in.max_verif_eps = mc.maxVerifEps;
         check(dr_bm_create(&c, &mc.cost, &in, 1, &h), "MPPIController");
+        failCount.assign((size_t)mc.nTimesteps, 0u);
         const size_t n = (size_t)(nModels + 1) * mc.nTimesteps * ACTION_DIM;
         if (nominal && nominal->size() != n)
         {
@@ -398,12 +410,23 @@ class BranchingMPPIController
         chosenBranch = d.branch;
         lastNu = d.nu;
         lastMinCost = d.min_cost;
+        useNewPlan = d.use_new_plan != 0;
+        epsilon = d.epsilon;
+        epsilonPartial = d.epsilon_partial;
+        certifiedLoss = d.certified_loss;
+        if (mppiConfig.nVerifSamples > 0)
+        {
+            std::vector<int32_t> fn(failCount.size()), fo(failCount.size());
+            check(dr_bm_get_fail_counts(h, 0, fn.data(), fo.data(), (int)failCount.size()), "getControl");
+            for (size_t t = 0; t < failCount.size(); ++t)
+                failCount[t] = (unsigned)(useNewPlan ? fn[t] : fo[t]);
+        }
     }
 
     BranchingMPPIConfig mppiConfig;
     std::vector<float> h_nominal;                    // (nModels + 1, T, 2)
     std::array<float, DR_SR_MAX_MODELS> h_belief{};  // written by the caller's world step
-    std::vector<unsigned> failCount;                 // no verification
+    std::vector<unsigned> failCount;                 // (T) verification failures of the executed plan
     double epsilon = 0.0, epsilonPartial = 0.0, certifiedLoss = 0.0;
     bool useNewPlan = true;
     int chosenBranch = 0; // 0 nominal, theta + 1 committed

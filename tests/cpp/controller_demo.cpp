@@ -128,6 +128,10 @@ int main(int argc, char** argv)
         bc.nTimesteps = 30;
         bc.nKnots = 8;
         bc.minConfidence = 0.6f;
+        bc.nVerifSamples = 4096;
+        bc.verifHorizon = 20;
+        bc.beta = 1e-5f;
+        bc.maxVerifEps = 0.01f;
         stratrace::BranchingMPPIController bm(se, bc, 9, 1);
         const float beliefs[3][2] = {{0.5f, 0.5f}, {0.3f, 0.7f}, {0.9f, 0.1f}};
         for (int i = 0; i < steps; ++i)
@@ -136,7 +140,11 @@ int main(int argc, char** argv)
             bm.h_belief[1] = beliefs[i % 3][1];
             float a[2];
             bm.getControl(ss, a);
-            std::printf("bm_action %d %.9g %.9g branch %d lambda %.9g\n", i, a[0], a[1], bm.chosenBranch, bm.mppiConfig.invTemperature);
+            unsigned fails = 0;
+            for (unsigned f : bm.failCount)
+                fails += f;
+            std::printf("bm_action %d %.9g %.9g branch %d lambda %.9g use_new %d cert %.17g fails %u\n", i, a[0], a[1], bm.chosenBranch,
+                        bm.mppiConfig.invTemperature, (int)bm.useNewPlan, bm.certifiedLoss, fails);
         }
     }
     catch (const std::exception& e)

@@ -110,15 +110,19 @@ def test_shim_matches_python_handle(pkg):
     # branching spline MPPI shim (the reference's default MPPIController)
     bm_lines = [l.split() for l in res if l.startswith("bm_action")]
     bm_acts = np.array([[float(x) for x in l[2:4]] for l in bm_lines], dtype=np.float32)
-    ins = sc.bm_instance(2048, 30, 9, n_knots=8, lam=10.0, sigma=0.5, min_confidence=0.6, true_theta=1, adapt=1)
+    ins = sc.bm_instance(2048, 30, 9, n_knots=8, lam=10.0, sigma=0.5, min_confidence=0.6, true_theta=1, adapt=1, n_verif=4096,
+                         verif_horizon=20, beta=1e-5, max_verif_eps=0.01)
     beng = pkg.BMEngine(sc.SrScenario("shim_bm", env, abi.SrCost(1.4, 50.0, 1.8, 1e7, 1e5, 1e5, 50.0, 0.0), [ins], [st], 1, ""))
-    py_bm, py_branch = [], []
+    py_bm, py_branch, py_verif = [], [], []
     for i, b in enumerate([[0.5, 0.5], [0.3, 0.7], [0.9, 0.1]][:3]):
         beng.set_belief(0, np.float32(b))
         beng.set_states([st])
         py_bm.append(beng.step(1, advance=False)[0])
-        py_branch.append(beng.diag()[0].branch)
+        d = beng.diag()[0]
+        py_branch.append(d.branch)
+        py_verif.append((d.use_new_plan, d.certified_loss, d.fail_count))
     beng.close()
     assert np.array_equal(bm_acts, np.stack(py_bm))
     assert [int(l[5]) for l in bm_lines] == py_branch
+    assert [(int(l[9]), float(l[11]), int(l[13])) for l in bm_lines] == py_verif
 

@@ -28,6 +28,11 @@ FLOPS_PER_SAMPLE_STEP = 270.07
 # HBM bytes per rollout sample-step the rollout kernel must move: the float4
 # noise row it writes (read back once by the weighted average).
 ROLLOUT_BYTES_PER_SAMPLE_STEP = 16.0
+# FP32 flops per rollout-step of the StratRace2D workloads (oracle op counter at the first control
+# step, plus noise; oracle/count_ops.py count_sr) and, for the branching rollout, the fraction of
+# the K * models * T rollout-steps actually run there (rollouts stop at terminal events).
+SR_FLOPS_PER_STEP = {"sr_race": 1044.30, "pr_sr_race": 1010.14, "bm_race": 1513.46}
+BM_STEPS_FRACTION = 0.73
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
@@ -73,6 +78,62 @@ def cpu_baseline(pkg, scn, budget_s):
             "sample": f"{n} closed-loop batch steps ({n_inst} controllers each) of the same workload after 1 warm-up "
                       f"step, oracle FP32 restatement, OpenMP {threads} threads, {dt:.1f} s",
             "sample_steps_per_s": n * scn.sample_steps_per_step / dt}
+
+
+def cpu_stratrace(pkg, budget_s=12.0):
+    """The oracle's FP32 restatement of the three StratRace2D controllers at the reference sizes,
+    OpenMP over samples on this host: one control step (rollouts + reductions + update) from the
+    start state, repeated within about budget_s / 3 seconds each."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+
+    oracle = pyoracle.Oracle(pkg.abi)
+    threads = min(16, os.cpu_count() or 1)
+    sc = pkg.scenarios
+    out = {}
+
+    def timed(fn, work):
+        t0 = time.perf_counter()
+        fn()
+        one = time.perf_counter() - t0
+        n = max(1, int(budget_s / 3 / max(one, 1e-6)))
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        dt = time.perf_counter() - t0
+        return {"control_steps_per_s": n / dt, "rollout_steps_per_s": n * work / dt, "cores": threads, "kind": "port",
+                "sample": f"{n} control steps after 1 warm-up step, oracle FP32, OpenMP {threads} threads, {dt:.1f} s"}
+
+    s1 = sc.sr_race(n_inst=1)
+    i1 = s1.instances[0]
+    zero = np.zeros((i1.n_timesteps, 2), np.float32)
+
+    def sr_step():
+        costs, noise = oracle.sr_rollout(0, s1.env, s1.cost, s1.states[0], i1.true_theta, zero, i1.n_samples, i1.n_timesteps, i1.seed,
+                                         0, i1.sigma, threads=threads)
+        oracle.sr_update(0, costs, noise, zero, i1.lambda_, s1.env.max_accel[0])
+    out["sr_race"] = timed(sr_step, i1.n_samples * i1.n_timesteps)
+
+    s2 = sc.pr_sr_race(n_inst=1)
+    i2 = s2.instances[0]
+    z2 = np.zeros((i2.n_timesteps, 2), np.float32)
+    belief = np.array(i2.belief[:s2.env.n_models], np.float32)
+
+    def prsr_step():
+        oracle.prsr_step(0, s2.env, s2.cost, s2.states[0], i2, belief, 0, [i2.lambda_] * 3, z2, z2, threads=threads, want_noise=False,
+                         want_pairs=False)
+    out["pr_sr_race"] = timed(prsr_step, 2 * i2.n_draws * i2.n_samples * i2.n_timesteps)
+
+    s3 = sc.bm_race(n_inst=1)
+    i3 = s3.instances[0]
+    kn = np.zeros((s3.env.n_models + 1, i3.n_knots, 2), np.float32)
+    b3 = np.array(i3.init_belief[:s3.env.n_models], np.float32)
+
+    def bm_step():
+        oracle.bm_step(0, s3.env, s3.cost, s3.states[0], i3, b3, 0, i3.lambda_, kn, threads=threads, want_noise=False)
+    out["bm_race"] = timed(bm_step, s3.env.n_models * i3.n_samples * i3.n_timesteps)
+    return out
 
 
 def batch_vs_per_instance(pkg, steps=50, warmup=5):
@@ -174,6 +235,16 @@ def pr_mppi(pkg, steps=100, warmup=10):
             "use_nom_plan": [int(d.use_nom_plan) for d in diag], "steps": steps}
 
 
+def sr_roofline(key, rollout_steps, rollout_ms, kernel):
+    """FP32 roofline of a StratRace2D rollout kernel: counted flops per rollout-step times the
+    rollout-steps of one launch, over its HIP-event time."""
+    flops = SR_FLOPS_PER_STEP[key] * rollout_steps
+    achieved = flops / (rollout_ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None, "kernel": kernel, "flops_per_launch": flops,
+            "note": "FP32 vector (VALU) work, no MFMA; oracle-counted flops per rollout-step"}
+
+
 def sr_race(pkg, steps=50, warmup=5, n_inst=1):
     """Plain MPPI on the reference's StratRace2D race at its dense setting
     (K = 2^16, T = 70, temperature 10, sigma 1.5, round track r = 6 m), closed
@@ -193,7 +264,8 @@ def sr_race(pkg, steps=50, warmup=5, n_inst=1):
     return {"workload": scn.name, "control_steps_per_s": n_inst * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
             "sample_steps_per_s": ss * steps / elapsed,
             "kernel_ms_per_step": {"rollout": float(ms[0]), "weighted_avg": float(ms[1]), "update_world": float(ms[2]),
-                                   "step": float(ms[3])}, "steps": steps}
+                                   "step": float(ms[3])},
+            "roofline": sr_roofline("sr_race", ss, float(ms[0]), "k_sr_rollout"), "steps": steps}
 
 
 def pr_sr_race(pkg, steps=30, warmup=3):
@@ -215,7 +287,8 @@ def pr_sr_race(pkg, steps=30, warmup=3):
     return {"workload": scn.name, "control_steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
             "rollouts_per_step": rollouts, "rollout_env_steps_per_s": rollouts * ins.n_timesteps * steps / elapsed,
             "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate": float(ms[1]), "weighted_avg": float(ms[2]),
-                                   "update_choice_world": float(ms[3]), "step": float(ms[4])}, "steps": steps}
+                                   "update_choice_world": float(ms[3]), "step": float(ms[4])},
+            "roofline": sr_roofline("pr_sr_race", rollouts * ins.n_timesteps, float(ms[0]), "k_prsr_rollout"), "steps": steps}
 
 
 def bm_race(pkg, steps=30, warmup=3, n_verif=0):
@@ -243,6 +316,7 @@ def bm_race(pkg, steps=30, warmup=3, n_verif=0):
             "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate_min": float(ms[1]), "weighted_avg": float(ms[2]),
                                    "update_verify": float(ms[3]), "choice_world": float(ms[4]), "step": float(ms[5])},
             "final_branch": int(d.branch), "use_new_plan": int(d.use_new_plan), "certified_loss": float(d.certified_loss),
+            "roofline": sr_roofline("bm_race", BM_STEPS_FRACTION * rollouts * ins.n_timesteps, float(ms[0]), "k_bm_rollout"),
             "verification_samples": n_verif, "steps": steps}
 
 
@@ -437,6 +511,10 @@ def main():
     if not args.no_cpu:
         log("bench: CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(pkg, scn, args.cpu_budget)
+        if not args.no_sr:
+            log("bench: CPU baselines of the StratRace2D controllers ...")
+            out["cpu_baseline_stratrace"] = cpu_stratrace(pkg, args.cpu_budget)
+            log(f"bench: cpu_baseline_stratrace {json.dumps(out['cpu_baseline_stratrace'])}")
         log(f"bench: CPU baseline {out['cpu_baseline']['value']:.2f} control steps/s")
     print(json.dumps(out), flush=True)
 

@@ -65,6 +65,59 @@ def count(scn, n=1024):
     return flops / steps + NOISE_FLOPS_PER_STEP
 
 
+# Noise cost per StratRace2D rollout-step read off the code: one Box-Muller
+# pair (log 20, sincos 21, 5 more) and two sigma scalings for the control
+# noise of a (sample, step) = 48; the environment's opponent-action noise is
+# drawn four normals at a time (two pairs) = 92 per step.
+SR_CONTROL_NOISE = (20 + 21 + 5) + 2
+SR_ENV_NOISE = 2 * (20 + 21 + 5)
+
+
+def count_sr(pkg, n=256):
+    """FP32 flops per rollout-step of the three StratRace2D workloads at their first control step
+    (oracle op counter over n samples, early stops included), plus the noise per rollout-step:
+    plain (control noise per step), PR-MPPI (control noise shared by the 2*P rollouts of a
+    sample), branching (the NB*M knot-noise pairs of a sample shared by its model rollouts)."""
+    sys.path.insert(0, HERE)
+    import pyoracle
+    abi = pkg.abi
+    L = pyoracle.lib(abi)
+    L.or_count_sr_ops.restype = C.c_longlong
+    L.or_count_sr_ops.argtypes = [C.c_int, C.POINTER(abi.SrEnv), C.POINTER(abi.SrCost), C.POINTER(abi.SrState), C.c_int,
+                                  C.POINTER(C.c_float), C.c_int, C.c_int, C.c_uint64, C.POINTER(abi.BmInstance),
+                                  C.POINTER(C.c_float), C.POINTER(C.c_longlong)]
+    fp = C.POINTER(C.c_float)
+    sc = pkg.scenarios
+    out = {}
+    steps = C.c_longlong()
+    s1 = sc.sr_race(n_inst=1)
+    i1 = s1.instances[0]
+    nom = np.zeros(i1.n_timesteps * 2, np.float32)
+    f = L.or_count_sr_ops(0, C.byref(s1.env), C.byref(s1.cost), C.byref(s1.states[0]), i1.true_theta, nom.ctypes.data_as(fp), n,
+                          i1.n_timesteps, i1.seed, None, None, C.byref(steps))
+    out["sr_race"] = f / steps.value + SR_CONTROL_NOISE + SR_ENV_NOISE
+    s2 = sc.pr_sr_race(n_inst=1)
+    i2 = s2.instances[0]
+    nom = np.zeros(i2.n_timesteps * 2, np.float32)
+    f = L.or_count_sr_ops(1, C.byref(s2.env), C.byref(s2.cost), C.byref(s2.states[0]), 0, nom.ctypes.data_as(fp), n,
+                          i2.n_timesteps, i2.seed, None, None, C.byref(steps))
+    out["pr_sr_race"] = f / steps.value + SR_CONTROL_NOISE / (2 * i2.n_draws) + SR_ENV_NOISE
+    s3 = sc.bm_race(n_inst=1)
+    i3 = s3.instances[0]
+    NT = s3.env.n_models
+    kn = np.zeros((NT + 1) * i3.n_knots * 2, np.float32)
+    b = np.zeros(abi.SR_MAX_MODELS, np.float32)
+    b[:NT] = i3.init_belief[:NT]
+    f = L.or_count_sr_ops(2, C.byref(s3.env), C.byref(s3.cost), C.byref(s3.states[0]), 0, kn.ctypes.data_as(fp), n,
+                          i3.n_timesteps, i3.seed, C.byref(i3), b.ctypes.data_as(fp), C.byref(steps))
+    knot_noise = (NT + 1) * i3.n_knots * SR_CONTROL_NOISE * n / steps.value
+    out["bm_race"] = f / steps.value + knot_noise + SR_ENV_NOISE
+    out["bm_race_steps_fraction"] = steps.value / (n * NT * i3.n_timesteps)
+    return out
+
+
 if __name__ == "__main__":
     pkg = package()
     print(f"{count(pkg.scenarios.c1_four_drones()):.2f} FP32 flops per sample-step (4 drones, K=16384, T=100)")
+    for k, v in count_sr(pkg).items():
+        print(f"{k}: {v:.2f}")

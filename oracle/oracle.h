@@ -488,6 +488,7 @@ struct Cn
     Cn(double x) : v(x) {}
     explicit operator double() const { return v; }
     explicit operator float() const { return (float)v; }
+    explicit operator int() const { return (int)v; }
 };
 inline Cn operator+(Cn a, Cn b) { ++g_tally.flops; return Cn(a.v + b.v); }
 inline Cn operator-(Cn a, Cn b) { ++g_tally.flops; return Cn(a.v - b.v); }
@@ -497,6 +498,7 @@ inline Cn operator-(Cn a) { return Cn(-a.v); }
 inline bool operator<(Cn a, Cn b) { ++g_tally.flops; return a.v < b.v; }
 inline bool operator>(Cn a, Cn b) { ++g_tally.flops; return a.v > b.v; }
 inline bool operator<=(Cn a, Cn b) { ++g_tally.flops; return a.v <= b.v; }
+inline bool operator>=(Cn a, Cn b) { ++g_tally.flops; return a.v >= b.v; }
 template <>
 inline Cn sq_root<Cn>(Cn x) { ++g_tally.flops; return Cn(sqrt(x.v)); }
 inline bool tiny(Cn x) { ++g_tally.flops; return !(fabs(x.v) >= 1e-10); }

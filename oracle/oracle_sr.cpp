@@ -55,6 +55,12 @@ inline float cps(float a, float b) { return copysignf(a, b); }
 inline double cps(double a, double b) { return copysign(a, b); }
 inline float ab(float a) { return fabsf(a); }
 inline double ab(double a) { return fabs(a); }
+// the op-counting scalar: exp, log and floor count one each; copysign and
+// fabs are sign-bit operations and count none
+inline Cn ex(Cn x) { ++g_tally.flops; return Cn(exp(x.v)); }
+inline Cn flo(Cn x) { ++g_tally.flops; return Cn(floor(x.v)); }
+inline Cn cps(Cn a, Cn b) { return Cn(copysign(a.v, b.v)); }
+inline Cn ab(Cn a) { return Cn(fabs(a.v)); }
 
 template <class S>
 struct Env
@@ -399,6 +405,7 @@ bool winner(const Env<S>& E, const SState<S>& s)
 
 inline float lg(float x) { return log32(x); }
 inline double lg(double x) { return log(x); }
+inline Cn lg(Cn x) { ++g_tally.flops; return Cn(log(x.v)); }
 
 // updateBelief (env_stratrace.h:166-222), one opponent: Gaussian likelihood
 // of the applied opponent action `obs` under each model's nominal action
@@ -458,11 +465,16 @@ int env_step(const Env<S>& E, int theta, const S ego[2], SState<S>& s, Normals& 
     S act[4] = {ego[0], ego[1], 0, 0};
     const dr_sr_opp_config& c = E.e.opp[theta];
     S noms[DR_SR_MAX_MODELS][2];
+    S nom[2];
     if (belief)
+    {
         for (int th = 0; th < E.e.n_models; ++th)
             opp_action(E, E.e.opp[th], s, noms[th]);
-    S nom[2];
-    opp_action(E, c, s, nom);
+        nom[0] = noms[theta][0];
+        nom[1] = noms[theta][1];
+    }
+    else
+        opp_action(E, c, s, nom);
     for (int d = 0; d < 2; ++d)
     {
         S u = nom[d];
@@ -929,7 +941,7 @@ inline bool bm_spline_contributed(int b, int m, int btime, int used, int T, int 
 // step updates the belief; a branch commits when the belief is confident.
 template <class S>
 void bm_rollout(const Env<S>& E, const dr_sr_cost& K, const dr_sr_state& x0, const dr_bm_instance& I, const float* belief0f, const float* knots,
-                const S* B, uint32_t step, uint32_t k, S* ctrue, int* used, int* btime)
+                const S* B, uint32_t step, uint32_t k, S* ctrue, int* used, int* btime, long long* n_steps = nullptr)
 {
     const int NT = E.e.n_models, T = I.n_timesteps, M = I.n_knots, NK = I.n_samples;
     const uint32_t key0 = (uint32_t)(I.seed & 0xffffffffu), key1 = (uint32_t)(I.seed >> 32);
@@ -980,6 +992,8 @@ void bm_rollout(const Env<S>& E, const dr_sr_cost& K, const dr_sr_state& x0, con
             }
             acc = fm(state_cost(E, K, s), decay, acc);
             decay = decay * (S)0.97f;
+            if (n_steps)
+                ++*n_steps;
             if (term != 0)
                 break;
         }
@@ -1658,6 +1672,65 @@ double or_sr_opp_target(int f64, const dr_sr_env* env, int theta, const dr_sr_st
     float lat, goal;
     opp_target(Env<float>(*env), env->opp[theta], load<float>(*st), &lat, &goal);
     return lat;
+}
+
+// Algorithmic FP32 work of StratRace2D rollouts, counted with the
+// op-counting scalar (add, sub, mul, div, sqrt, exp, log, floor, min/max and
+// comparisons 1 each, fma 2) over n samples from x0: mode 0 the plain
+// rollout (one model, dense plan, environment step + stateCost), mode 1 the
+// PR-MPPI rollout (PR costs and safety), mode 2 the branching rollout of
+// model theta (belief update of every model each step, spline action with
+// the basis B [T][M] and knots [NB][M][2]).  Noise generation is not
+// counted.  Returns the total count; steps_out = rollout-steps taken.
+long long or_count_sr_ops(int mode, const dr_sr_env* env, const dr_sr_cost* K, const dr_sr_state* x0, int theta, const float* nominal,
+                          int n, int T, uint64_t seed, const dr_bm_instance* bm, const float* belief, long long* steps_out)
+{
+    g_tally.flops = 0;
+    const Env<Cn> E(*env);
+    const uint32_t key0 = (uint32_t)(seed & 0xffffffffu), key1 = (uint32_t)(seed >> 32);
+    long long steps = 0;
+    if (mode == 2 && bm)
+    {
+        std::vector<double> Bd((size_t)bm->n_timesteps * bm->n_knots);
+        spline_basis<double>(bm->knots, bm->n_knots, bm->n_timesteps, Bd.data());
+        std::vector<Cn> B(Bd.begin(), Bd.end());
+        g_tally.flops = 0;
+        for (int k = 0; k < n; ++k)
+        {
+            Cn c[DR_SR_MAX_MODELS];
+            int u[DR_SR_MAX_MODELS], t[DR_SR_MAX_MODELS];
+            bm_rollout<Cn>(E, *K, *x0, *bm, belief, nominal, B.data(), 0, (uint32_t)k, c, u, t, &steps);
+        }
+    }
+    else
+    {
+        for (int k = 0; k < n; ++k)
+        {
+            SState<Cn> s = load<Cn>(*x0);
+            Cn acc = 0, decay = 1, smax = -(Cn)INFINITY;
+            for (int t = 0; t < T; ++t)
+            {
+                const Cn u[2] = {Cn(nominal[t * 2]), Cn(nominal[t * 2 + 1])};
+                Normals rng{key0, key1, (uint32_t)k, 4u * (uint32_t)t, 0u};
+                const int term = env_step(E, theta, u, s, rng, true);
+                if (mode == 0)
+                    acc = fm(state_cost(E, *K, s), decay, acc);
+                else
+                {
+                    acc = fm(pr_state_cost(E, *K, s), decay, acc);
+                    smax = mx(smax, pr_safety(E, *K, s));
+                }
+                decay = decay * Cn(0.97f);
+                ++steps;
+                if (term)
+                    break;
+            }
+            acc = acc + final_cost(E, *K, s);
+        }
+    }
+    if (steps_out)
+        *steps_out = steps;
+    return g_tally.flops;
 }
 
 } // extern "C"

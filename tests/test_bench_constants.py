@@ -18,3 +18,12 @@ def test_flops_per_sample_step_matches_counter(pkg):
     count_ops = _load(os.path.join(ROOT, "oracle", "count_ops.py"), "count_ops_mod")
     measured = count_ops.count(pkg.scenarios.c1_four_drones())
     assert abs(bench.FLOPS_PER_SAMPLE_STEP - measured) < 0.01, measured
+
+
+def test_stratrace_flops_match_counter(pkg):
+    bench = _load(os.path.join(ROOT, "bench.py"), "bench_mod")
+    count_ops = _load(os.path.join(ROOT, "oracle", "count_ops.py"), "count_ops_mod")
+    measured = count_ops.count_sr(pkg)
+    for key, v in bench.SR_FLOPS_PER_STEP.items():
+        assert abs(v - measured[key]) < 0.01, (key, measured[key])
+    assert abs(bench.BM_STEPS_FRACTION - measured["bm_race_steps_fraction"]) < 0.005

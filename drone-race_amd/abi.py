@@ -141,6 +141,19 @@ class BmDiag(C.Structure):
                 ("epsilon", C.c_double), ("epsilon_partial", C.c_double), ("certified_loss", C.c_double)]
 
 
+class HoEnv(C.Structure):
+    _fields_ = [("arena_min", C.c_float * 2), ("arena_max", C.c_float * 2), ("dt", C.c_float), ("drone_radius", C.c_float),
+                ("pos_noise", C.c_float), ("speed_noise", C.c_float), ("action_noise", C.c_float), ("max_speed", C.c_float),
+                ("max_accel", C.c_float), ("n_win_laps", C.c_int32), ("gate_centers", (C.c_float * 2) * 3),
+                ("gate_vectors", (C.c_float * 2) * 3), ("gate_radius", C.c_float * 3), ("rect_obstacles", (C.c_float * 4) * 3),
+                ("hidden_centers", (C.c_float * 2) * 2), ("hidden_radius", C.c_float * 2), ("dbl_hp_limits", C.c_float * 4),
+                ("dbl_hp_lambdas", C.c_float * 4), ("annulus_center", C.c_float * 2), ("annulus_radius", C.c_float * 2)]
+
+
+class HoState(C.Structure):
+    _fields_ = [("pos", C.c_float * 2), ("vel", C.c_float * 2), ("laps", C.c_int32), ("gates", C.c_int32)]
+
+
 MSG_HEADER, MSG_STATE, MSG_EVENT, MSG_DONE = 0, 1, 2, 3
 EVT_OUTSIDE, EVT_WINNER, EVT_OPP_WINNER, EVT_TRUNCATED = 0, 1, 2, 3
 CONT_MPPI, CONT_PRMPPI = 0, 1
@@ -268,6 +281,9 @@ SIGNATURES = {
     "dr_bm_get_fail_counts": (_I, [_P, _I, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _I]),
     "dr_bm_set_timing": (_I, [_P, _I]),
     "dr_bm_last_timing": (_I, [_P, _F, _I]),
+    "dr_bmho_create": (_I, [C.POINTER(HoEnv), C.POINTER(SrCost), C.POINTER(BmInstance), _I, C.POINTER(_P)]),
+    "dr_bmho_set_states": (_I, [_P, C.POINTER(HoState), _I]),
+    "dr_bmho_get_states": (_I, [_P, C.POINTER(HoState), _I]),
 }
 
 _lib = None

@@ -1,4 +1,4 @@
-// Host side of the branching spline MPPI engine on StratRace2D: the natural
+// Host side of the branching spline MPPI engine on StratRace2D and HiddenObs: the natural
 // cubic spline basis, descriptor tables, device buffers, the per-step
 // launch sequence, hipGraph capture and the dr_bm_* C ABI.
 //
@@ -21,19 +21,19 @@
 
 namespace dr {
 
-hipError_t launch_bm_rollout(int max_m, int max_tm, int grid, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmTile* tiles,
-                             int n_tiles, const dr_sr_env* env, const dr_sr_cost& KC, const dr_sr_state* states, const float* knots,
-                             const float* basis, float2* noise, float* ctrue, int* used, int* btime);
+hipError_t launch_bm_rollout(int env_kind, int max_m, int max_tm, int grid, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn,
+                             const BmTile* tiles, int n_tiles, const void* env, const dr_sr_cost& KC, const void* states,
+                             const float* knots, const float* basis, float2* noise, float* ctrue, int* used, int* btime);
 hipError_t launch_bm_reduce(int n_inst, int max_nbk, int n_rtiles, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const BmRTile* rt,
                             const float* ctrue, float* costs, const int* used, const int* btime, float* pmin, float* mins,
                             const float2* noise, double* part, hipEvent_t mid);
 hipError_t launch_bm_update(int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const double* part, const float* basis,
                             float* knots, float* dnew, float* nu, int* fail, float max_accel);
-hipError_t launch_bm_verify(int n_tiles, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmVTile* tiles, const dr_sr_env* env,
-                            const dr_sr_state* states, const float* dnew, const float* dense, int* fail);
+hipError_t launch_bm_verify(int env_kind, int n_tiles, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmVTile* tiles,
+                            const void* env, const void* states, const float* dnew, const float* dense, int* fail);
 hipError_t launch_bm_certify(int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const int* verdict, const float* dnew,
                              float* dense, float* actions);
-hipError_t launch_bm_advance(int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const dr_sr_env* env, dr_sr_state* states,
+hipError_t launch_bm_advance(int env_kind, int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const void* env, void* states,
                              const float* actions, int* events);
 
 // Natural cubic spline basis B[t][i] = spline through e_i at the knots,
@@ -123,7 +123,10 @@ struct dr_bm_engine
 {
     int device = 0;
     hipStream_t stream = nullptr;
-    dr_sr_env env{};
+    int env_kind = BM_ENV_SR;
+    int n_models = 0;
+    float max_accel = 0.0f;
+    size_t state_size = 0; // bytes of one instance's world state
     dr_sr_cost cost{};
     std::vector<dr_bm_instance> inst;
     std::vector<BmInstDev> idev;
@@ -142,8 +145,8 @@ struct dr_bm_engine
     DevBuf<BmTile> d_tiles;
     DevBuf<BmRTile> d_rt;
     DevBuf<BmVTile> d_vt;
-    DevBuf<dr_sr_env> d_env;
-    DevBuf<dr_sr_state> d_states;
+    DevBuf<uint8_t> d_env;    // dr_sr_env or dr_ho_env
+    DevBuf<uint8_t> d_states; // dr_sr_state or dr_ho_state per instance
     DevBuf<float> d_knots, d_dense, d_dnew, d_B, d_ctrue, d_costs, d_mins, d_pmin, d_nu, d_actions;
     DevBuf<int> d_used, d_btime, d_events, d_fail, d_verdict;
     DevBuf<float2> d_noise;
@@ -182,16 +185,16 @@ int dr_bm_engine::enqueue_a(hipEvent_t* ev)
     const int n = (int)inst.size();
     if (ev)
         HIPCHK(hipEventRecord(ev[0], stream));
-    HIPCHK(launch_bm_rollout(max_m, max_tm, 0, stream, d_inst.p, d_dyn.p, d_tiles.p, (int)tiles.size(), d_env.p, cost, d_states.p, d_knots.p, d_B.p,
-                             d_noise.p, d_ctrue.p, d_used.p, d_btime.p));
+    HIPCHK(launch_bm_rollout(env_kind, max_m, max_tm, 0, stream, d_inst.p, d_dyn.p, d_tiles.p, (int)tiles.size(), d_env.p, cost, d_states.p,
+                             d_knots.p, d_B.p, d_noise.p, d_ctrue.p, d_used.p, d_btime.p));
     if (ev)
         HIPCHK(hipEventRecord(ev[1], stream));
     HIPCHK(launch_bm_reduce(n, max_nbk, (int)rtiles.size(), stream, d_inst.p, d_dyn.p, d_rt.p, d_ctrue.p, d_costs.p, d_used.p, d_btime.p,
                             d_pmin.p, d_mins.p, d_noise.p, d_part.p, ev ? ev[2] : nullptr));
     if (ev)
         HIPCHK(hipEventRecord(ev[3], stream));
-    HIPCHK(launch_bm_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_B.p, d_knots.p, d_dnew.p, d_nu.p, d_fail.p, env.max_accel[0]));
-    HIPCHK(launch_bm_verify((int)vtiles.size(), stream, d_inst.p, d_dyn.p, d_vt.p, d_env.p, d_states.p, d_dnew.p, d_dense.p, d_fail.p));
+    HIPCHK(launch_bm_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_B.p, d_knots.p, d_dnew.p, d_nu.p, d_fail.p, max_accel));
+    HIPCHK(launch_bm_verify(env_kind, (int)vtiles.size(), stream, d_inst.p, d_dyn.p, d_vt.p, d_env.p, d_states.p, d_dnew.p, d_dense.p, d_fail.p));
     if (ev)
         HIPCHK(hipEventRecord(ev[4], stream));
     return DR_OK;
@@ -203,7 +206,7 @@ int dr_bm_engine::enqueue_b(bool advance, hipEvent_t* ev)
     const int n = (int)inst.size();
     HIPCHK(launch_bm_certify(n, stream, d_inst.p, d_dyn.p, d_verdict.p, d_dnew.p, d_dense.p, d_actions.p));
     if (advance)
-        HIPCHK(launch_bm_advance(n, stream, d_inst.p, d_dyn.p, d_env.p, d_states.p, d_actions.p, d_events.p));
+        HIPCHK(launch_bm_advance(env_kind, n, stream, d_inst.p, d_dyn.p, d_env.p, d_states.p, d_actions.p, d_events.p));
     if (ev)
         HIPCHK(hipEventRecord(ev[5], stream));
     return DR_OK;
@@ -336,6 +339,39 @@ int dr_bm_engine::run(int n_steps, bool advance)
     return DR_OK;
 }
 
+namespace dr {
+
+// HiddenObs configuration: finite geometry, positive step, radius and
+// limits, nonnegative noise levels, consecutive gates apart (getAdvance
+// divides by their distance).
+int check_ho_env(const dr_ho_env& env)
+{
+    const float* f = &env.arena_min[0];
+    const float* end = reinterpret_cast<const float*>(&env.n_win_laps);
+    for (; f < end; ++f)
+        if (!std::isfinite(*f))
+            return host_fail(DR_EINVAL, "HiddenObs configuration values must be finite");
+    for (f = &env.gate_centers[0][0]; f < &env.annulus_radius[0] + 2; ++f)
+        if (!std::isfinite(*f))
+            return host_fail(DR_EINVAL, "HiddenObs geometry must be finite");
+    if (!(env.dt > 0.0f) || !(env.drone_radius > 0.0f) || !(env.max_speed > 0.0f) || !(env.max_accel > 0.0f))
+        return host_fail(DR_EINVAL, "dt, drone_radius, max_speed and max_accel must be > 0");
+    if (env.pos_noise < 0.0f || env.speed_noise < 0.0f || env.action_noise < 0.0f)
+        return host_fail(DR_EINVAL, "noise levels must be >= 0");
+    if (env.n_win_laps < 1)
+        return host_fail(DR_EINVAL, "n_win_laps must be >= 1");
+    for (int g = 0; g < 3; ++g)
+    {
+        const float* a = env.gate_centers[g];
+        const float* b = env.gate_centers[(g + 1) % 3];
+        if (a[0] == b[0] && a[1] == b[1])
+            return host_fail(DR_EINVAL, "consecutive gate centres must differ");
+    }
+    return DR_OK;
+}
+
+} // namespace dr
+
 namespace {
 
 int check_bm(dr_bm_engine* e, int i)
@@ -373,17 +409,11 @@ int get_dyn(dr_bm_engine* e, std::vector<BmDyn>& dyn)
 
 extern "C" {
 
-int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_instance* instances, int n_instances, dr_bm_engine** out)
+// Shared by dr_bm_create and dr_bmho_create: env points at a validated
+// environment configuration of env_size bytes with n_models opponent models.
+static int bm_create(int env_kind, const void* env, size_t env_size, size_t state_size, int n_models, float max_accel, const dr_sr_cost* cost,
+                     const dr_bm_instance* instances, int n_instances, dr_bm_engine** out)
 {
-    if (!out)
-        return host_fail(DR_EINVAL, "out is NULL");
-    *out = nullptr;
-    if (!env || !cost || !instances)
-        return host_fail(DR_EINVAL, "NULL argument");
-    if (n_instances < 1)
-        return host_fail(DR_EINVAL, "need at least one instance");
-    if (int rc = check_sr_env(*env))
-        return rc;
     for (int i = 0; i < n_instances; ++i)
     {
         const dr_bm_instance& s = instances[i];
@@ -401,9 +431,9 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
                 return host_fail(DR_EINVAL, w + "knots must be strictly increasing");
         if (!(s.lambda > 0.0f) || !(s.sigma >= 0.0f) || !std::isfinite(s.sigma) || !std::isfinite(s.min_confidence))
             return host_fail(DR_EINVAL, w + "lambda must be > 0, sigma finite and >= 0, min_confidence finite");
-        if (s.true_theta < 0 || s.true_theta >= env->n_models)
+        if (s.true_theta < 0 || s.true_theta >= n_models)
             return host_fail(DR_EINVAL, w + "true_theta out of range");
-        for (int th = 0; th < env->n_models; ++th)
+        for (int th = 0; th < n_models; ++th)
             if (!(s.init_belief[th] >= 0.0f) || !std::isfinite(s.init_belief[th]))
                 return host_fail(DR_EINVAL, w + "belief entries must be finite and >= 0");
         if (s.n_verif_samples < 0 || s.n_verif_samples > (1 << 22))
@@ -423,10 +453,13 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
         return bail(host_fail(DR_EHIP, "no HIP device"));
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(host_fail(DR_EHIP, "stream creation failed"));
-    e->env = *env;
+    e->env_kind = env_kind;
+    e->n_models = n_models;
+    e->max_accel = max_accel;
+    e->state_size = state_size;
     e->cost = *cost;
     e->inst.assign(instances, instances + n_instances);
-    const int n = n_instances, NT = env->n_models, NB = NT + 1;
+    const int n = n_instances, NT = n_models, NB = NT + 1;
     e->idev.assign(n, BmInstDev{});
     std::vector<float> Ball;
     for (int i = 0; i < n; ++i)
@@ -506,7 +539,7 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
         !chk(e->d_rt.alloc(e->rtiles.size()), "alloc rt") || !chk(e->d_pmin.alloc(e->pmin_total), "alloc pmin") ||
         !chk(e->d_vt.alloc(e->vtiles.size()), "alloc vtiles") || !chk(e->d_dnew.alloc(e->dense_total), "alloc dnew") ||
         !chk(e->d_fail.alloc(e->fail_total), "alloc fail") || !chk(e->d_verdict.alloc(n), "alloc verdict") ||
-        !chk(e->d_env.alloc(1), "alloc env") || !chk(e->d_states.alloc(n), "alloc states") || !chk(e->d_knots.alloc(e->knot_total), "alloc knots") ||
+        !chk(e->d_env.alloc(env_size), "alloc env") || !chk(e->d_states.alloc(n * state_size), "alloc states") || !chk(e->d_knots.alloc(e->knot_total), "alloc knots") ||
         !chk(e->d_dense.alloc(e->dense_total), "alloc dense") || !chk(e->d_B.alloc(e->B_total), "alloc B") ||
         !chk(e->d_ctrue.alloc(e->ct_total), "alloc ctrue") || !chk(e->d_costs.alloc(e->cost_total), "alloc costs") ||
         !chk(e->d_mins.alloc(e->min_total), "alloc mins") || !chk(e->d_nu.alloc(e->nu_total), "alloc nu") ||
@@ -524,8 +557,7 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
             dyn[i].belief[th] = instances[i].init_belief[th];
     }
     std::vector<float> zk(e->knot_total, 0.0f), zd(e->dense_total, 0.0f);
-    std::vector<dr_sr_state> st(n);
-    std::memset(st.data(), 0, st.size() * sizeof(dr_sr_state));
+    std::vector<uint8_t> st(n * state_size, 0);
     std::vector<int> ev(n, 0);
     if (!chk(hipMemcpy(e->d_inst.p, e->idev.data(), n * sizeof(BmInstDev), hipMemcpyHostToDevice), "copy inst") ||
         !chk(hipMemcpy(e->d_dyn.p, dyn.data(), n * sizeof(BmDyn), hipMemcpyHostToDevice), "copy dyn") ||
@@ -536,8 +568,8 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
         !chk(hipMemcpy(e->d_fail.p, std::vector<int>(e->fail_total, 0).data(), e->fail_total * sizeof(int), hipMemcpyHostToDevice),
              "copy fail") ||
         !chk(hipMemcpy(e->d_verdict.p, std::vector<int>(n, 1).data(), n * sizeof(int), hipMemcpyHostToDevice), "copy verdict") ||
-        !chk(hipMemcpy(e->d_env.p, env, sizeof(dr_sr_env), hipMemcpyHostToDevice), "copy env") ||
-        !chk(hipMemcpy(e->d_states.p, st.data(), n * sizeof(dr_sr_state), hipMemcpyHostToDevice), "copy states") ||
+        !chk(hipMemcpy(e->d_env.p, env, env_size, hipMemcpyHostToDevice), "copy env") ||
+        !chk(hipMemcpy(e->d_states.p, st.data(), st.size(), hipMemcpyHostToDevice), "copy states") ||
         !chk(hipMemcpy(e->d_knots.p, zk.data(), zk.size() * sizeof(float), hipMemcpyHostToDevice), "copy knots") ||
         !chk(hipMemcpy(e->d_dense.p, zd.data(), zd.size() * sizeof(float), hipMemcpyHostToDevice), "copy dense") ||
         !chk(hipMemcpy(e->d_dnew.p, zd.data(), zd.size() * sizeof(float), hipMemcpyHostToDevice), "copy dnew") ||
@@ -551,6 +583,31 @@ int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_insta
     return DR_OK;
 }
 
+int dr_bm_create(const dr_sr_env* env, const dr_sr_cost* cost, const dr_bm_instance* instances, int n_instances, dr_bm_engine** out)
+{
+    if (!out)
+        return host_fail(DR_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!env || !cost || !instances)
+        return host_fail(DR_EINVAL, "NULL argument");
+    if (int rc = check_sr_env(*env))
+        return rc;
+    return bm_create(BM_ENV_SR, env, sizeof(dr_sr_env), sizeof(dr_sr_state), env->n_models, env->max_accel[0], cost, instances, n_instances,
+                     out);
+}
+
+int dr_bmho_create(const dr_ho_env* env, const dr_sr_cost* cost, const dr_bm_instance* instances, int n_instances, dr_bm_engine** out)
+{
+    if (!out)
+        return host_fail(DR_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!env || !cost || !instances)
+        return host_fail(DR_EINVAL, "NULL argument");
+    if (int rc = check_ho_env(*env))
+        return rc;
+    return bm_create(BM_ENV_HO, env, sizeof(dr_ho_env), sizeof(dr_ho_state), 2, env->max_accel, cost, instances, n_instances, out);
+}
+
 void dr_bm_destroy(dr_bm_engine* e)
 {
     if (e)
@@ -560,28 +617,40 @@ void dr_bm_destroy(dr_bm_engine* e)
     }
 }
 
-int dr_bm_set_states(dr_bm_engine* e, const dr_sr_state* states, int n)
+static int bm_set_states(dr_bm_engine* e, int kind, const void* states, int n)
 {
     if (!e || !states || n != (int)e->inst.size())
         return host_fail(DR_EINVAL, "set_states: bad handle, pointer or count");
-    if (int rc = bm_in(e, e->d_states.p, states, (size_t)n))
+    if (e->env_kind != kind)
+        return host_fail(DR_EINVAL, "set_states: state type does not match the engine's environment");
+    if (int rc = bm_in(e, e->d_states.p, static_cast<const uint8_t*>(states), (size_t)n * e->state_size))
         return rc;
     std::vector<int> ev(n, 0);
     return bm_in(e, e->d_events.p, ev.data(), ev.size());
 }
 
-int dr_bm_get_states(dr_bm_engine* e, dr_sr_state* states, int n)
+static int bm_get_states(dr_bm_engine* e, int kind, void* states, int n)
 {
     if (!e || !states || n != (int)e->inst.size())
         return host_fail(DR_EINVAL, "get_states: bad handle, pointer or count");
-    return bm_out(e, states, e->d_states.p, (size_t)n);
+    if (e->env_kind != kind)
+        return host_fail(DR_EINVAL, "get_states: state type does not match the engine's environment");
+    return bm_out(e, static_cast<uint8_t*>(states), e->d_states.p, (size_t)n * e->state_size);
 }
+
+int dr_bm_set_states(dr_bm_engine* e, const dr_sr_state* states, int n) { return bm_set_states(e, BM_ENV_SR, states, n); }
+
+int dr_bm_get_states(dr_bm_engine* e, dr_sr_state* states, int n) { return bm_get_states(e, BM_ENV_SR, states, n); }
+
+int dr_bmho_set_states(dr_bm_engine* e, const dr_ho_state* states, int n) { return bm_set_states(e, BM_ENV_HO, states, n); }
+
+int dr_bmho_get_states(dr_bm_engine* e, dr_ho_state* states, int n) { return bm_get_states(e, BM_ENV_HO, states, n); }
 
 int dr_bm_set_belief(dr_bm_engine* e, int instance, const float* belief, int n_models)
 {
     if (int rc = check_bm(e, instance))
         return rc;
-    if (!belief || n_models != e->env.n_models)
+    if (!belief || n_models != e->n_models)
         return host_fail(DR_EINVAL, "set_belief: expected n_models values");
     for (int th = 0; th < n_models; ++th)
         if (!(belief[th] >= 0.0f) || !std::isfinite(belief[th]))
@@ -595,7 +664,7 @@ int dr_bm_get_belief(dr_bm_engine* e, int instance, float* belief, int n_models)
 {
     if (int rc = check_bm(e, instance))
         return rc;
-    if (!belief || n_models != e->env.n_models)
+    if (!belief || n_models != e->n_models)
         return host_fail(DR_EINVAL, "get_belief: expected n_models values");
     std::vector<BmDyn> dyn;
     if (int rc = get_dyn(e, dyn))

@@ -29,6 +29,8 @@ constexpr int BM_MAX_TM = 8192;      // T * M held in LDS
 constexpr int BM_RED_BLOCK = 256;    // workgroup of the masked minima and weighted averages
 constexpr int BM_CHUNK = 4096;       // samples per weighted-average workgroup
 constexpr int BM_VBLOCK = 128;       // verification lanes per workgroup
+constexpr int BM_ENV_SR = 0;         // StratRace2D (dr_sr_env, dr_sr_state)
+constexpr int BM_ENV_HO = 1;         // HiddenObs (dr_ho_env, dr_ho_state)
 
 struct BmInstDev
 {

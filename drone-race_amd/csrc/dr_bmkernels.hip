@@ -22,7 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dr_bmkernels.h"
-#include "dr_sr.h"
+#include "dr_envpolicy.h"
 #include "dr_wave.h"
 
 namespace dr {
@@ -67,33 +67,21 @@ __device__ __forceinline__ bool spline_contributed(int b, int m, int btime, int 
 // (and M <= MAXM) the current branch's knot noise is held in registers;
 // with MAXM == 0 it is read back from the noise rows in HBM.
 // The spline basis sits in dynamic LDS (T * M floats of the largest instance).
-template <int MAXM>
+template <int MAXM, class P>
 __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __restrict__ insts, const BmDyn* __restrict__ dyn,
-                                                         const BmTile* __restrict__ tiles, int n_tiles, const dr_sr_env* __restrict__ env,
-                                                         dr_sr_cost KC, const dr_sr_state* __restrict__ states,
+                                                         const BmTile* __restrict__ tiles, int n_tiles, const typename P::Env* __restrict__ env,
+                                                         dr_sr_cost KC, const typename P::Abi* __restrict__ states,
                                                          const float* __restrict__ knots, const float* __restrict__ basis,
                                                          float2* __restrict__ noise, float* __restrict__ ctrue, int* __restrict__ used,
                                                          int* __restrict__ btime)
 {
-    __shared__ float s_p[NS * 2];
-    __shared__ float s_t[NS * 2];
-    __shared__ float s_k[NS];
+    __shared__ float s_env[P::LDS_FLOATS];
     extern __shared__ float s_B[];
     __shared__ float s_kn[(DR_SR_MAX_MODELS + 1) * DR_MAX_KNOTS * 2];
-    __shared__ float s_bc[2 * DR_SR_MAX_MODELS];
 
     const int tid = threadIdx.x;
-    const dr_sr_env& E = *env;
-    if (tid < E.n_models)
-        belief_consts(E, tid, &s_bc[tid], &s_bc[DR_SR_MAX_MODELS + tid]);
-    for (int i = tid; i < NS * 2; i += BM_BLOCK)
-    {
-        s_p[i] = (&E.p_grid[0][0])[i];
-        s_t[i] = (&E.t_grid[0][0])[i];
-    }
-    for (int i = tid; i < NS; i += BM_BLOCK)
-        s_k[i] = E.kappa_grid[i];
-    const Tables tb{s_p, s_t, s_k, E.track_length};
+    const typename P::Env& E = *env;
+    const typename P::Ctx ctx = P::stage(s_env, E, tid, BM_BLOCK);
 
     for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
     {
@@ -130,10 +118,10 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
             ctrue[I.ct_off + th * K + k] = __builtin_inff();
             used[I.ct_off + th * K + k] = firstPred;
             btime[I.ct_off + th * K + k] = firstTime;
-            const St x0 = state_from_abi(states[bt.inst]);
+            const typename P::State x0 = P::load(states[bt.inst]);
             if (firstPred == 0 || firstPred == th + 1)
             {
-                St s = x0;
+                typename P::State s = x0;
                 float belief[DR_SR_MAX_MODELS];
                 for (int q = 0; q < DR_SR_MAX_MODELS; ++q)
                     belief[q] = belief0[q];
@@ -175,7 +163,7 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                     const float ego[2] = {u0, u1};
                     NormalSeq rng = normal_seq(key0, key1, (uint32_t)(th * K + k), (uint32_t)t, step);
                     // once committed, the rollout no longer reads its belief
-                    const int term = env_step_t<true>(tb, E, th, ego, s, rng, true, belief, s_bc, pred == 0);
+                    const int term = P::step(ctx, E, th, ego, s, rng, true, belief, pred == 0);
                     const int np = pred == 0 ? find_confident(belief, NT, I.min_conf) : 0;
                     if (np != 0)
                     {
@@ -189,12 +177,12 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                                 e[m] = m < M ? knot_noise(key0, key1, (uint32_t)k, pred, m, step, I.sigma) : make_float2(0.0f, 0.0f);
                         }
                     }
-                    acc = fma_(state_cost(E, KC, s), decay, acc);
+                    acc = fma_(P::stage_cost(ctx, E, KC, s, th), decay, acc);
                     decay = decay * DECAY;
                     if (term != SR_NONE)
                         break;
                 }
-                acc = acc + final_cost(E, KC, s, tb.L);
+                acc = acc + P::final_cost(ctx, E, KC, s);
                 ctrue[I.ct_off + th * K + k] = acc;
                 used[I.ct_off + th * K + k] = pred;
                 btime[I.ct_off + th * K + k] = time;
@@ -451,30 +439,20 @@ __global__ __launch_bounds__(256) void k_bm_update(const BmInstDev* __restrict__
 // the current branch at local time, belief update and branching, noise on,
 // the opponent's win not a loss) for verif_h steps; a loss at step t counts
 // one failure at t.
+template <class P>
 __global__ __launch_bounds__(BM_VBLOCK) void k_bm_verify(const BmInstDev* __restrict__ insts, const BmDyn* __restrict__ dyn,
-                                                         const BmVTile* __restrict__ tiles, int n_tiles, const dr_sr_env* __restrict__ env,
-                                                         const dr_sr_state* __restrict__ states, const float* __restrict__ dnew,
+                                                         const BmVTile* __restrict__ tiles, int n_tiles,
+                                                         const typename P::Env* __restrict__ env,
+                                                         const typename P::Abi* __restrict__ states, const float* __restrict__ dnew,
                                                          const float* __restrict__ dense, int* __restrict__ fail)
 {
-    __shared__ float s_p[NS * 2];
-    __shared__ float s_t[NS * 2];
-    __shared__ float s_k[NS];
+    __shared__ float s_env[P::LDS_FLOATS];
     __shared__ float s_plan[(DR_SR_MAX_MODELS + 1) * DR_MAX_T * 2];
     __shared__ int s_fail[DR_MAX_T];
-    __shared__ float s_bc[2 * DR_SR_MAX_MODELS];
 
     const int tid = threadIdx.x;
-    const dr_sr_env& E = *env;
-    for (int i = tid; i < NS * 2; i += BM_VBLOCK)
-    {
-        s_p[i] = (&E.p_grid[0][0])[i];
-        s_t[i] = (&E.t_grid[0][0])[i];
-    }
-    for (int i = tid; i < NS; i += BM_VBLOCK)
-        s_k[i] = E.kappa_grid[i];
-    if (tid < E.n_models)
-        belief_consts(E, tid, &s_bc[tid], &s_bc[DR_SR_MAX_MODELS + tid]);
-    const Tables tb{s_p, s_t, s_k, E.track_length};
+    const typename P::Env& E = *env;
+    const typename P::Ctx ctx = P::stage(s_env, E, tid, BM_VBLOCK);
 
     for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
     {
@@ -500,14 +478,14 @@ __global__ __launch_bounds__(BM_VBLOCK) void k_bm_verify(const BmInstDev* __rest
             int pred = find_confident(belief, NT, I.min_conf);
             const U4 r = philox4x32_10(U4{word, 0xFFFFFFFFu, step, STREAM_VERIFY}, I.key0, I.key1);
             const int theta = model_from_belief(uniform_open0(r.x), belief, NT);
-            St s = state_from_abi(states[vt.inst]);
+            typename P::State s = P::load(states[vt.inst]);
             int origin = 0;
             for (int t = 0; t < H; ++t)
             {
                 const float* u = s_plan + (pred * T + (t - origin)) * 2;
                 const float ego[2] = {u[0], u[1]};
                 NormalSeq rng = normal_seq(I.key0, I.key1, word, (uint32_t)t, step, STREAM_VERIFY);
-                const int term = env_step_t<true>(tb, E, theta, ego, s, rng, false, belief, s_bc, pred == 0);
+                const int term = P::step(ctx, E, theta, ego, s, rng, false, belief, pred == 0);
                 if (pred == 0)
                 {
                     const int np = find_confident(belief, NT, I.min_conf);
@@ -575,25 +553,26 @@ __global__ __launch_bounds__(256) void k_bm_certify(const BmInstDev* __restrict_
 
 // Closed loop (SimulationEngine::dynStep): the world's opponent follows its
 // model; the controller's belief is updated from the opponent's action.
+template <class P>
 __global__ __launch_bounds__(64) void k_bm_advance(const BmInstDev* __restrict__ insts, BmDyn* __restrict__ dyn,
-                                                   const dr_sr_env* __restrict__ env, dr_sr_state* __restrict__ states,
+                                                   const typename P::Env* __restrict__ env, typename P::Abi* __restrict__ states,
                                                    const float* __restrict__ actions, int* __restrict__ events, int n_inst)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_inst || events[i] != 0)
         return;
-    const dr_sr_env& E = *env;
-    const Tables tb{&E.p_grid[0][0], &E.t_grid[0][0], E.kappa_grid, E.track_length};
+    const typename P::Env& E = *env;
+    const typename P::Ctx ctx = P::direct(E);
     const BmInstDev& I = insts[i];
     BmDyn& Y = dyn[i];
-    St s = state_from_abi(states[i]);
+    typename P::State s = P::load(states[i]);
     float belief[DR_SR_MAX_MODELS];
     for (int th = 0; th < DR_SR_MAX_MODELS; ++th)
         belief[th] = Y.belief[th];
     NormalSeq rng = normal_seq(I.key0, I.key1, 0xFFFFFFFFu, Y.step - 1u, 0u);
     const float u[2] = {actions[i * 2], actions[i * 2 + 1]};
-    const int term = env_step_t<true>(tb, E, I.true_theta, u, s, rng, false, belief);
-    state_to_abi(s, states[i]);
+    const int term = P::step(ctx, E, I.true_theta, u, s, rng, false, belief, true);
+    P::store(s, states[i]);
     for (int th = 0; th < DR_SR_MAX_MODELS; ++th)
         Y.belief[th] = belief[th];
     events[i] = term;
@@ -602,18 +581,35 @@ __global__ __launch_bounds__(64) void k_bm_advance(const BmInstDev* __restrict__
 // ---------------------------------------------------------------------------
 // launch wrappers (host)
 
-hipError_t launch_bm_rollout(int max_m, int max_tm, int grid, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmTile* tiles,
-                             int n_tiles, const dr_sr_env* env, const dr_sr_cost& KC, const dr_sr_state* states, const float* knots,
-                             const float* basis, float2* noise, float* ctrue, int* used, int* btime)
+template <class P>
+static void rollout_t(int max_m, int g, size_t lds, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmTile* tiles,
+                      int n_tiles, const void* env, const dr_sr_cost& KC, const void* states, const float* knots, const float* basis,
+                      float2* noise, float* ctrue, int* used, int* btime)
+{
+    const auto* E = static_cast<const typename P::Env*>(env);
+    const auto* S = static_cast<const typename P::Abi*>(states);
+    if (max_m <= 20)
+        k_bm_rollout<20, P><<<g, BM_BLOCK, lds, st>>>(insts, dyn, tiles, n_tiles, E, KC, S, knots, basis, noise, ctrue, used, btime);
+    else
+        k_bm_rollout<0, P><<<g, BM_BLOCK, lds, st>>>(insts, dyn, tiles, n_tiles, E, KC, S, knots, basis, noise, ctrue, used, btime);
+}
+
+// env_kind: BM_ENV_SR (env dr_sr_env, states dr_sr_state) or BM_ENV_HO
+// (dr_ho_env, dr_ho_state)
+hipError_t launch_bm_rollout(int env_kind, int max_m, int max_tm, int grid, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn,
+                             const BmTile* tiles, int n_tiles, const void* env, const dr_sr_cost& KC, const void* states,
+                             const float* knots, const float* basis, float2* noise, float* ctrue, int* used, int* btime)
 {
     const int g = grid > 0 ? (grid < n_tiles ? grid : n_tiles) : n_tiles;
     if (max_m > DR_MAX_KNOTS || max_tm > BM_MAX_TM || max_tm < 1)
         return hipErrorInvalidValue;
     const size_t lds = (size_t)max_tm * sizeof(float);
-    if (max_m <= 20)
-        k_bm_rollout<20><<<g, BM_BLOCK, lds, st>>>(insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime);
+    if (env_kind == BM_ENV_SR)
+        rollout_t<SrPolicy>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime);
+    else if (env_kind == BM_ENV_HO)
+        rollout_t<HoPolicy>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime);
     else
-        k_bm_rollout<0><<<g, BM_BLOCK, lds, st>>>(insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime);
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -640,11 +636,19 @@ hipError_t launch_bm_update(int n_inst, hipStream_t st, const BmInstDev* insts, 
     return hipGetLastError();
 }
 
-hipError_t launch_bm_verify(int n_tiles, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmVTile* tiles, const dr_sr_env* env,
-                            const dr_sr_state* states, const float* dnew, const float* dense, int* fail)
+hipError_t launch_bm_verify(int env_kind, int n_tiles, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmVTile* tiles,
+                            const void* env, const void* states, const float* dnew, const float* dense, int* fail)
 {
-    if (n_tiles > 0)
-        k_bm_verify<<<n_tiles, BM_VBLOCK, 0, st>>>(insts, dyn, tiles, n_tiles, env, states, dnew, dense, fail);
+    if (n_tiles <= 0)
+        return hipGetLastError();
+    if (env_kind == BM_ENV_SR)
+        k_bm_verify<SrPolicy><<<n_tiles, BM_VBLOCK, 0, st>>>(insts, dyn, tiles, n_tiles, static_cast<const dr_sr_env*>(env),
+                                                             static_cast<const dr_sr_state*>(states), dnew, dense, fail);
+    else if (env_kind == BM_ENV_HO)
+        k_bm_verify<HoPolicy><<<n_tiles, BM_VBLOCK, 0, st>>>(insts, dyn, tiles, n_tiles, static_cast<const dr_ho_env*>(env),
+                                                             static_cast<const dr_ho_state*>(states), dnew, dense, fail);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -655,10 +659,18 @@ hipError_t launch_bm_certify(int n_inst, hipStream_t st, const BmInstDev* insts,
     return hipGetLastError();
 }
 
-hipError_t launch_bm_advance(int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const dr_sr_env* env, dr_sr_state* states,
+hipError_t launch_bm_advance(int env_kind, int n_inst, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const void* env, void* states,
                              const float* actions, int* events)
 {
-    k_bm_advance<<<(n_inst + 63) / 64, 64, 0, st>>>(insts, dyn, env, states, actions, events, n_inst);
+    const int g = (n_inst + 63) / 64;
+    if (env_kind == BM_ENV_SR)
+        k_bm_advance<SrPolicy><<<g, 64, 0, st>>>(insts, dyn, static_cast<const dr_sr_env*>(env), static_cast<dr_sr_state*>(states), actions,
+                                                 events, n_inst);
+    else if (env_kind == BM_ENV_HO)
+        k_bm_advance<HoPolicy><<<g, 64, 0, st>>>(insts, dyn, static_cast<const dr_ho_env*>(env), static_cast<dr_ho_state*>(states), actions,
+                                                 events, n_inst);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -493,6 +493,8 @@ class BMEngine:
     """Branching spline MPPI controllers on StratRace2D worlds (the reference's default
     MPPIController with splines, without plan verification), one world each."""
 
+    _create, _set, _get, _state = "dr_bm_create", "dr_bm_set_states", "dr_bm_get_states", abi.SrState
+
     def __init__(self, scenario):
         lib = abi.load()
         self.lib = lib
@@ -500,10 +502,15 @@ class BMEngine:
         n = len(scenario.instances)
         self._inst = (abi.BmInstance * n)(*scenario.instances)
         self.n_inst = n
+        self.n_models = self._n_models(scenario.env)
         h = C.c_void_p()
-        check(lib.dr_bm_create(C.byref(scenario.env), C.byref(scenario.cost), self._inst, n, C.byref(h)))
+        check(getattr(lib, self._create)(C.byref(scenario.env), C.byref(scenario.cost), self._inst, n, C.byref(h)))
         self.h = h
         self.set_states(scenario.states)
+
+    @staticmethod
+    def _n_models(env):
+        return env.n_models
 
     def close(self):
         if self.h:
@@ -518,15 +525,15 @@ class BMEngine:
 
     def _dims(self, i):
         ins = self.sc.instances[i]
-        return ins.n_samples, ins.n_timesteps, ins.n_knots, self.sc.env.n_models
+        return ins.n_samples, ins.n_timesteps, ins.n_knots, self.n_models
 
     def set_states(self, states):
-        arr = (abi.SrState * self.n_inst)(*states)
-        check(self.lib.dr_bm_set_states(self.h, arr, self.n_inst))
+        arr = (self._state * self.n_inst)(*states)
+        check(getattr(self.lib, self._set)(self.h, arr, self.n_inst))
 
     def get_states(self):
-        arr = (abi.SrState * self.n_inst)()
-        check(self.lib.dr_bm_get_states(self.h, arr, self.n_inst))
+        arr = (self._state * self.n_inst)()
+        check(getattr(self.lib, self._get)(self.h, arr, self.n_inst))
         return arr
 
     def set_belief(self, i, belief):
@@ -534,7 +541,7 @@ class BMEngine:
         check(self.lib.dr_bm_set_belief(self.h, i, _fptr(b), b.size))
 
     def belief(self, i):
-        b = np.empty(self.sc.env.n_models, dtype=np.float32)
+        b = np.empty(self.n_models, dtype=np.float32)
         check(self.lib.dr_bm_get_belief(self.h, i, _fptr(b), b.size))
         return b
 
@@ -603,7 +610,7 @@ class BMEngine:
         return a.reshape(NT + 1, M)
 
     def nu(self, i):
-        a = np.empty(self.sc.env.n_models + 1, dtype=np.float32)
+        a = np.empty(self.n_models + 1, dtype=np.float32)
         check(self.lib.dr_bm_get_nu(self.h, i, _fptr(a), a.size))
         return a
 
@@ -630,3 +637,14 @@ class BMEngine:
         a = np.empty(6, dtype=np.float32)
         check(self.lib.dr_bm_last_timing(self.h, _fptr(a), 6))
         return a
+
+
+class HoBMEngine(BMEngine):
+    """Branching spline MPPI controllers on HiddenObs worlds (two models: which hidden
+    obstacle exists); the same engine and calls, with HiddenObs states."""
+
+    _create, _set, _get, _state = "dr_bmho_create", "dr_bmho_set_states", "dr_bmho_get_states", abi.HoState
+
+    @staticmethod
+    def _n_models(env):
+        return 2

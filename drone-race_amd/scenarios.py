@@ -423,6 +423,68 @@ def bm_race(n_inst=1, n_samples=65536, n_timesteps=80, n_knots=20, seed=42, step
                       f"{n_models} opponent models")
 
 
+# ------------------------------------------------------------------ HiddenObs
+
+def ho_env(pos_noise=0.0, speed_noise=0.0, action_noise=0.0, n_win_laps=1):
+    """The HiddenObs course of hiddenObsEnv (python-src/main.py:705-781): a quarter-annulus
+    corridor, two diagonal double half-planes, three rectangles, three gates and two hidden
+    circular obstacles (model 0: the bottom one at (9, 5.5), model 1: the top one at (9, 8.5))."""
+    min_r, max_r, diag_w, middle_w = 3.0, 5.0, 1.5, 4.0
+    hid_w, hid_r, small_w, small_off = 4.0, 0.8, 4.0, 1.5
+    corr_h, small_h, small_hoff, hid_h = 3.0, 1.0, 0.0, 1.5
+    avg_r, neg_avg_r = (min_r + max_r) / 2, (max_r - min_r) / 2
+    maxx, maxy = max_r + diag_w + middle_w + diag_w, corr_h + max_r + diag_w
+    e = abi.HoEnv()
+    e.arena_min[:] = [0.0, 0.0]
+    e.arena_max[:] = [maxx + 1, maxy]
+    e.dt, e.drone_radius, e.max_speed, e.max_accel, e.n_win_laps = 0.1, 0.3, 2.0, 3.0, n_win_laps
+    e.pos_noise, e.speed_noise, e.action_noise = pos_noise, speed_noise, action_noise
+    for g, (c, v) in enumerate(zip([(maxx, corr_h + avg_r), (neg_avg_r, corr_h), (max_r, corr_h + avg_r)], [(1, 0), (0, 1), (1, 0)])):
+        e.gate_centers[g][:] = c
+        e.gate_vectors[g][:] = v
+        e.gate_radius[g] = (max_r - min_r) / 2
+    rects = [(max_r - min_r, 0, maxx, corr_h), (max_r, corr_h, maxx, corr_h + min_r - diag_w),
+             (max_r + small_off, corr_h + avg_r + small_hoff - small_h / 2, max_r + small_off + small_w, corr_h + avg_r + small_hoff + small_h / 2)]
+    for r, box in enumerate(rects):
+        e.rect_obstacles[r][:] = box
+    e.hidden_centers[0][:] = [max_r + hid_w, avg_r + corr_h - hid_h]
+    e.hidden_centers[1][:] = [max_r + hid_w, avg_r + corr_h + hid_h]
+    e.hidden_radius[:] = [hid_r, hid_r]
+    e.dbl_hp_limits[:] = [max_r, max_r + diag_w, max_r + diag_w + middle_w, maxx]
+    e.dbl_hp_lambdas[:] = [corr_h, -corr_h - max_r - min_r, maxx + corr_h + max_r, maxx - corr_h - min_r]
+    e.annulus_center[:] = [max_r, corr_h]
+    e.annulus_radius[:] = [min_r, max_r]
+    return e
+
+
+def ho_start_state(env):
+    """At rest at ((max_r - min_r) / 2, 1.5 drone radii), before gate 1 (main.py:783)."""
+    s = abi.HoState()
+    s.pos[:] = [1.0, env.drone_radius * 1.5]
+    return s
+
+
+def ho_cost():
+    """The spline MPPIConfig of hiddenObsEnv (python-src/main.py:811-833): collision factor 1.2,
+    boundary cost 10 (factor 1.4), outside 1e7, win 1e6, terminal advance weight 500."""
+    return abi.SrCost(coll_dist_factor=1.2, boundary_cost=10.0, boundary_threshold_factor=1.4, outside_cost=1e7,
+                      win_cost=1e6, opp_win_cost=0.0, final_adv_weight=500.0, final_opp_adv_weight=0.0)
+
+
+def ho_race(n_inst=1, n_samples=65536, n_timesteps=80, n_knots=20, seed=2, steps=300, n_verif=0, **env_kw):
+    """The reference's branching spline MPPI on HiddenObs (hiddenObsEnv with splines,
+    main.py:811-833): K = 2^16, T = 80, 20 knots, temperature 10 (adapted), sigma 0.8,
+    confidence 0.95, uniform initial belief over which hidden obstacle exists; world i has
+    obstacle i % 2."""
+    env = ho_env(**env_kw)
+    inst = [bm_instance(n_samples, n_timesteps, seed_mix(seed, 700 + i), n_knots=n_knots, lam=10.0, sigma=0.8, true_theta=i % 2,
+                        belief=(0.5, 0.5), n_verif=n_verif) for i in range(n_inst)]
+    st = [ho_start_state(env) for _ in range(n_inst)]
+    tag = f"_v{n_verif}" if n_verif else ""
+    return SrScenario(f"ho_race_{n_inst}x_k{n_samples}_t{n_timesteps}_m{n_knots}{tag}", env, ho_cost(), inst, st, steps,
+                      f"branching spline MPPI on HiddenObs, {n_inst} worlds, K={n_samples}, T={n_timesteps}, {n_knots} knots")
+
+
 def seed_mix(seed, i):
     """64-bit Philox key for instance i of a scenario seeded `seed`."""
     return ((int(seed) * 0x9E3779B97F4A7C15) ^ (0xA5A5A5A5 + 0x632BE59BD9B4E019 * (i + 1))) & 0xFFFFFFFFFFFFFFFF

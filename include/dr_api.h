@@ -535,6 +535,50 @@ int dr_bm_set_timing(dr_bm_engine* e, int enable);
 int dr_bm_last_timing(dr_bm_engine* e, float* ms, int n);
 
 /* ---------------------------------------------------------------------
+ * HiddenObs: the reference's second 2-D environment (include/environments/
+ * env_hiddenobs_defs.h:48-105, env_hiddenobs.h): one drone, three gates,
+ * three rectangles, two double half-planes, a quarter annulus and two hidden
+ * circular obstacles of which model theta in {0, 1} exists; the belief
+ * collapses on the true model once either hidden obstacle is in sight.
+ * The branching spline MPPI engine runs on it (dr_bmho_create; every other
+ * dr_bm_* call applies, with two models). */
+typedef struct dr_ho_env
+{
+    float arena_min[2];
+    float arena_max[2];
+    float dt;
+    float drone_radius;
+    float pos_noise;
+    float speed_noise;
+    float action_noise;
+    float max_speed;               /* L2 */
+    float max_accel;               /* L2 */
+    int32_t n_win_laps;
+    float gate_centers[3][2];
+    float gate_vectors[3][2];
+    float gate_radius[3];
+    float rect_obstacles[3][4];    /* xmin, ymin, xmax, ymax */
+    float hidden_centers[2][2];
+    float hidden_radius[2];
+    float dbl_hp_limits[4];        /* minx1, maxx1, minx2, maxx2 */
+    float dbl_hp_lambdas[4];       /* lambda1top, lambda1bot, lambda2top, lambda2bot */
+    float annulus_center[2];
+    float annulus_radius[2];       /* inner, outer */
+} dr_ho_env;
+
+typedef struct dr_ho_state
+{
+    float pos[2];
+    float vel[2];
+    int32_t laps;
+    int32_t gates;
+} dr_ho_state;
+
+int dr_bmho_create(const dr_ho_env* env, const dr_sr_cost* cost, const dr_bm_instance* instances, int n_instances, dr_bm_engine** out);
+int dr_bmho_set_states(dr_bm_engine* e, const dr_ho_state* states, int n);
+int dr_bmho_get_states(dr_bm_engine* e, dr_ho_state* states, int n);
+
+/* ---------------------------------------------------------------------
  * Wire protocol and simulation session of the reference's front-end for the
  * StratRace2D build: the byte codec (include/protocol.h:9-142,
  * src/protocol.cpp, python-src/protocol.py:13-116), the header decoders

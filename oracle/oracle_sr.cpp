@@ -844,6 +844,386 @@ void prsr_choose(const dr_sr_env& env, const dr_sr_cost& K, const dr_sr_state& x
 
 // ------------------------------------------------------------ branching spline MPPI
 
+// ------------------------------------------------------------ HiddenObs
+// env_hiddenobs.h restated on the scalar S (every constant rounded to float
+// first, as the engine holds it).
+
+template <class S>
+struct HoEnv
+{
+    const dr_ho_env& e;
+    explicit HoEnv(const dr_ho_env& env) : e(env) {}
+    S f(float v) const { return (S)v; }
+};
+
+template <class S>
+struct HState
+{
+    S pos[2], vel[2];
+    int laps, gates;
+};
+
+template <class S>
+HState<S> ho_load(const dr_ho_state& a)
+{
+    HState<S> s;
+    for (int d = 0; d < 2; ++d)
+    {
+        s.pos[d] = (S)a.pos[d];
+        s.vel[d] = (S)a.vel[d];
+    }
+    s.laps = a.laps;
+    s.gates = a.gates;
+    return s;
+}
+
+template <class S>
+void ho_store(const HState<S>& s, dr_ho_state& a)
+{
+    for (int d = 0; d < 2; ++d)
+    {
+        a.pos[d] = (float)s.pos[d];
+        a.vel[d] = (float)s.vel[d];
+    }
+    a.laps = s.laps;
+    a.gates = s.gates;
+}
+
+// sqDistToRectangle (env_hiddenobs.h:27-34): clamp the point into the box
+template <class S>
+S ho_sq_rect(S x, S y, S x0, S y0, S x1, S y1)
+{
+    const S dx = x - mn(mx(x, x0), x1);
+    const S dy = y - mn(mx(y, y0), y1);
+    return dx * dx + dy * dy;
+}
+
+// sqDistToHalfPlane (env_hiddenobs.h:36-60): region x0 <= x <= x1, x + y >= lam;
+// outside it, the nearest of the diagonal edge and the two vertical rays
+template <class S>
+S ho_sq_hp(S x, S y, S x0, S x1, S lam)
+{
+    if (x0 <= x && x <= x1 && x + y >= lam)
+        return (S)0;
+    const S px = mx(mn((x - y + lam) * (S)0.5f, x1), x0);
+    const S py = lam - px;
+    S best = (x - px) * (x - px) + (y - py) * (y - py);
+    const S yl = mx(y, lam - x0);
+    best = mn(best, (x - x0) * (x - x0) + (y - yl) * (y - yl));
+    const S yr = mx(y, lam - x1);
+    best = mn(best, (x - x1) * (x - x1) + (y - yr) * (y - yr));
+    return best;
+}
+
+// distToAnnulus (env_hiddenobs.h:62-86): top-left quadrant of the centre,
+// obstacle where r <= rmin or r >= rmax; +inf outside the quadrant
+template <class S>
+S ho_annulus(S x, S y, S cx, S cy, S r0, S r1)
+{
+    const S ux = x - cx, uy = y - cy;
+    const S rr = ux * ux + uy * uy;
+    const bool quad = ux <= (S)0 && uy >= (S)0;
+    if (quad && (rr <= r0 * r0 || rr >= r1 * r1))
+        return (S)0;
+    if (quad)
+    {
+        const S r = sq_root(rr);
+        return mn(r - r0, r1 - r);
+    }
+    return (S)INFINITY;
+}
+
+// trackBoundaryDist (env_hiddenobs.h:93-132)
+template <class S>
+S ho_bdist(const HoEnv<S>& H, const HState<S>& s, int theta)
+{
+    const dr_ho_env& e = H.e;
+    const S x = s.pos[0], y = s.pos[1];
+    S sq = (S)INFINITY;
+    for (int r = 0; r < 3; ++r)
+        sq = mn(sq, ho_sq_rect(x, y, H.f(e.rect_obstacles[r][0]), H.f(e.rect_obstacles[r][1]), H.f(e.rect_obstacles[r][2]),
+                               H.f(e.rect_obstacles[r][3])));
+    const S a0 = H.f(e.dbl_hp_limits[0]), a1 = H.f(e.dbl_hp_limits[1]), b0 = H.f(e.dbl_hp_limits[2]), b1 = H.f(e.dbl_hp_limits[3]);
+    // first double half-plane mirrored in x, second as is; the bottom halves mirrored in y
+    const S h1 = mn(ho_sq_hp(-x, y, -a1, -a0, H.f(e.dbl_hp_lambdas[0])), ho_sq_hp(-x, -y, -a1, -a0, H.f(e.dbl_hp_lambdas[1])));
+    const S h2 = mn(ho_sq_hp(x, y, b0, b1, H.f(e.dbl_hp_lambdas[2])), ho_sq_hp(x, -y, b0, b1, H.f(e.dbl_hp_lambdas[3])));
+    sq = mn(sq, mn(h1, h2));
+    S d = sq_root(sq);
+    const S walls = mn(mn(x - H.f(e.arena_min[0]), H.f(e.arena_max[0]) - x), mn(y - H.f(e.arena_min[1]), H.f(e.arena_max[1]) - y));
+    d = mn(d, walls);
+    d = mn(d, ho_annulus(x, y, H.f(e.annulus_center[0]), H.f(e.annulus_center[1]), H.f(e.annulus_radius[0]), H.f(e.annulus_radius[1])));
+    const S hx = x - H.f(e.hidden_centers[theta][0]), hy = y - H.f(e.hidden_centers[theta][1]);
+    d = mn(d, sq_root(hx * hx + hy * hy) - H.f(e.hidden_radius[theta]));
+    return d;
+}
+
+// segmentIntersectsRectangle (env_hiddenobs.h:147-201): Liang-Barsky
+// clipping of the parameter interval [0, 1] against both slabs
+template <class S>
+bool ho_seg_rect(S px, S py, S qx, S qy, S x0, S y0, S x1, S y1)
+{
+    S lo = 0, hi = 1;
+    const S p[2] = {px, py}, dq[2] = {qx - px, qy - py}, bmin[2] = {x0, y0}, bmax[2] = {x1, y1};
+    for (int a = 0; a < 2; ++a)
+    {
+        if (ab(dq[a]) < (S)1e-12f)
+        {
+            if (p[a] < bmin[a] || p[a] > bmax[a])
+                return false;
+            continue;
+        }
+        S ta = (bmin[a] - p[a]) / dq[a], tb = (bmax[a] - p[a]) / dq[a];
+        if (ta > tb)
+        {
+            const S t = ta;
+            ta = tb;
+            tb = t;
+        }
+        lo = mx(lo, ta);
+        hi = mn(hi, tb);
+        if (lo > hi)
+            return false;
+    }
+    return true;
+}
+
+// segmentIntersectsAnnulus (env_hiddenobs.h:203-276): clip the segment to
+// the quadrant, then its nearest point against rmin and its ends against
+// rmax (|x(t)|^2 is convex in t)
+template <class S>
+bool ho_seg_annulus(S px, S py, S qx, S qy, S cx, S cy, S r0, S r1)
+{
+    px = px - cx;
+    py = py - cy;
+    qx = qx - cx;
+    qy = qy - cy;
+    S lo = 0, hi = 1;
+    if (ab(px - qx) > (S)1e-12f)
+    {
+        const S c = px / (px - qx);
+        if (qx > px)
+            hi = mn(hi, c);
+        else
+            lo = mx(lo, c);
+    }
+    else if (px > (S)0)
+        return false;
+    if (hi < lo)
+        return false;
+    if (ab(py - qy) > (S)1e-12f)
+    {
+        const S c = py / (py - qy);
+        if (qy > py)
+            lo = mx(lo, c);
+        else
+            hi = mn(hi, c);
+    }
+    else if (py > (S)0)
+        return false;
+    if (hi < lo)
+        return false;
+    const S ddx = px - qx, ddy = py - qy;
+    const S len2 = ddx * ddx + ddy * ddy;
+    if (len2 > (S)1e-12f)
+    {
+        const S tc = mx(lo, mn(hi, (px * ddx + py * ddy) / len2));
+        const S xm = px + tc * (qx - px), ym = py + tc * (qy - py);
+        if (xm * xm + ym * ym < r0 * r0)
+            return true;
+    }
+    for (int end = 0; end < 2; ++end)
+    {
+        const S t = end == 0 ? lo : hi;
+        const S xe = px + t * (qx - px), ye = py + t * (qy - py);
+        if (xe * xe + ye * ye > r1 * r1)
+            return true;
+    }
+    return false;
+}
+
+// canSeeHiddenObs (env_hiddenobs.h:278-295)
+template <class S>
+bool ho_can_see(const HoEnv<S>& H, const HState<S>& s, int id)
+{
+    const dr_ho_env& e = H.e;
+    const S qx = H.f(e.hidden_centers[id][0]), qy = H.f(e.hidden_centers[id][1]);
+    for (int r = 0; r < 3; ++r)
+        if (ho_seg_rect(s.pos[0], s.pos[1], qx, qy, H.f(e.rect_obstacles[r][0]), H.f(e.rect_obstacles[r][1]), H.f(e.rect_obstacles[r][2]),
+                        H.f(e.rect_obstacles[r][3])))
+            return false;
+    return !ho_seg_annulus(s.pos[0], s.pos[1], qx, qy, H.f(e.annulus_center[0]), H.f(e.annulus_center[1]), H.f(e.annulus_radius[0]),
+                           H.f(e.annulus_radius[1]));
+}
+
+// getAdvance (env_hiddenobs.h:297-316)
+template <class S>
+S ho_advance(const HoEnv<S>& H, const HState<S>& s)
+{
+    const dr_ho_env& e = H.e;
+    const int nx = (s.gates + 1) % 3;
+    S g2 = 0, c2 = 0;
+    for (int d = 0; d < 2; ++d)
+    {
+        const S a = H.f(e.gate_centers[nx][d]) - s.pos[d];
+        const S b = H.f(e.gate_centers[nx][d]) - H.f(e.gate_centers[s.gates][d]);
+        g2 = g2 + a * a;
+        c2 = c2 + b * b;
+    }
+    return (S)(float)(s.gates + 3 * s.laps) + (S)0.8f * ((S)1 - sq_root(g2 / c2));
+}
+
+// updateGates<false> (env_hiddenobs.h:330-376)
+template <class S>
+void ho_gates(const HoEnv<S>& H, HState<S>& s, const S prev[2])
+{
+    const dr_ho_env& e = H.e;
+    const int nx = (s.gates + 1) % 3;
+    S num = 0, den = 0;
+    for (int d = 0; d < 2; ++d)
+    {
+        num = num + H.f(e.gate_vectors[nx][d]) * (H.f(e.gate_centers[nx][d]) - prev[d]);
+        den = den + H.f(e.gate_vectors[nx][d]) * (s.pos[d] - prev[d]);
+    }
+    if (ab(den) < (S)1e-10f)
+        return;
+    const S lam = num / den;
+    if (lam < (S)0 || lam > (S)1)
+        return;
+    S sq = 0;
+    for (int d = 0; d < 2; ++d)
+    {
+        const S v = ((S)1 - lam) * prev[d] + lam * s.pos[d] - H.f(e.gate_centers[nx][d]);
+        sq = sq + v * v;
+    }
+    if (sq <= H.f(e.gate_radius[nx]) * H.f(e.gate_radius[nx]))
+    {
+        s.gates = s.gates + 1;
+        if (s.gates == 3)
+        {
+            s.gates = 0;
+            s.laps = s.laps + 1;
+        }
+    }
+}
+
+// environmentStep<belief != NULL, false, false> (env_hiddenobs.h:378-483):
+// returns 0 none, 1 lose, 2 win.  With a belief, it collapses on theta when
+// either hidden obstacle is in sight while no model has 0.9.
+template <class S>
+int env_step(const HoEnv<S>& H, int theta, const S ego[2], HState<S>& s, Normals& rng, bool /*lose*/, S* belief = nullptr, bool noise = true)
+{
+    const dr_ho_env& e = H.e;
+    const S prev[2] = {s.pos[0], s.pos[1]};
+    const S ma = H.f(e.max_accel), vmax = H.f(e.max_speed), dt = H.f(e.dt);
+    S u[2] = {ego[0], ego[1]};
+    const S n2 = u[0] * u[0] + u[1] * u[1];
+    const S scale = n2 > ma * ma ? ma / sq_root(n2) : (S)1;
+    for (int d = 0; d < 2; ++d)
+    {
+        u[d] = u[d] * scale;
+        if (noise && e.action_noise != 0.0f)
+            u[d] = u[d] + (S)rng.next() * H.f(e.action_noise);
+    }
+    for (int d = 0; d < 2; ++d)
+        s.pos[d] = s.pos[d] + dt * s.vel[d];
+    for (int d = 0; d < 2; ++d)
+        s.vel[d] = s.vel[d] + dt * u[d];
+    const S v2 = s.vel[0] * s.vel[0] + s.vel[1] * s.vel[1];
+    if (v2 > vmax * vmax)
+    {
+        const S c = vmax / sq_root(v2);
+        for (int d = 0; d < 2; ++d)
+            s.vel[d] = s.vel[d] * c;
+    }
+    if (noise && (e.pos_noise != 0.0f || e.speed_noise != 0.0f))
+        for (int d = 0; d < 2; ++d)
+        {
+            if (e.pos_noise != 0.0f)
+                s.pos[d] = s.pos[d] + (S)rng.next() * H.f(e.pos_noise);
+            if (e.speed_noise != 0.0f)
+                s.vel[d] = s.vel[d] + (S)rng.next() * H.f(e.speed_noise);
+        }
+    ho_gates(H, s, prev);
+    if (belief && belief[0] < (S)0.9f && belief[1] < (S)0.9f && (ho_can_see(H, s, theta) || ho_can_see(H, s, 1 - theta)))
+    {
+        belief[theta] = (S)1;
+        belief[1 - theta] = (S)0;
+    }
+    if (ho_bdist(H, s, theta) < H.f(e.drone_radius))
+        return 1;
+    if (s.laps >= e.n_win_laps)
+        return 2;
+    return 0;
+}
+
+// stateCost / finalCost (costs.cuh:7-39) on HiddenObs (no opponent)
+template <class S>
+S stage_cost(const HoEnv<S>& H, const dr_sr_cost& K, const HState<S>& s, int theta)
+{
+    const S r = H.f(H.e.drone_radius);
+    const S bd = ho_bdist(H, s, theta);
+    S cost = 0;
+    if (bd < (S)K.boundary_threshold_factor * r)
+    {
+        const S q = bd / r;
+        cost = cost + (S)K.boundary_cost * (q * q);
+    }
+    if (bd < r * (S)K.coll_dist_factor)
+        cost = cost + (S)K.outside_cost;
+    if (s.laps >= H.e.n_win_laps)
+        cost = cost - (S)K.win_cost;
+    return cost;
+}
+
+template <class S>
+S final_cost(const HoEnv<S>& H, const dr_sr_cost& K, const HState<S>& s)
+{
+    S cost = -(S)K.final_adv_weight * ho_advance(H, s);
+    if (K.final_opp_adv_weight != 0.0f)
+        cost = cost + (S)K.final_opp_adv_weight * (S)0;
+    return cost;
+}
+
+// The environment-specific pieces the branching MPPI step needs, for both
+// environments.
+template <class S>
+S stage_cost(const Env<S>& E, const dr_sr_cost& K, const SState<S>& s, int)
+{
+    return state_cost(E, K, s);
+}
+template <class S>
+int n_models(const Env<S>& E)
+{
+    return E.e.n_models;
+}
+template <class S>
+int n_models(const HoEnv<S>&)
+{
+    return 2;
+}
+template <class S>
+SState<S> load_st(const Env<S>&, const dr_sr_state& a)
+{
+    return load<S>(a);
+}
+template <class S>
+HState<S> load_st(const HoEnv<S>&, const dr_ho_state& a)
+{
+    return ho_load<S>(a);
+}
+inline float max_accel_of(const dr_sr_env& e) { return e.max_accel[0]; }
+inline float max_accel_of(const dr_ho_env& e) { return e.max_accel; }
+template <class S>
+Env<S> make_env(const dr_sr_env& e)
+{
+    return Env<S>(e);
+}
+template <class S>
+HoEnv<S> make_env(const dr_ho_env& e)
+{
+    return HoEnv<S>(e);
+}
+
 // Natural cubic spline basis B[t][i] (buildSplineMatrix,
 // mppi_controller.cu:67-141): the spline through e_i at the knots tau
 // (second derivatives by the Thomas algorithm, zero at both ends), read at
@@ -939,11 +1319,11 @@ inline bool bm_spline_contributed(int b, int m, int btime, int used, int T, int 
 // commitment (fullRolloutKernel, kernels.cu:34-171): action = sum_m
 // B[t - t_origin][m] (knot + noise) of the current branch; the environment
 // step updates the belief; a branch commits when the belief is confident.
-template <class S>
-void bm_rollout(const Env<S>& E, const dr_sr_cost& K, const dr_sr_state& x0, const dr_bm_instance& I, const float* belief0f, const float* knots,
+template <class S, class EnvT, class Abi>
+void bm_rollout(const EnvT& E, const dr_sr_cost& K, const Abi& x0, const dr_bm_instance& I, const float* belief0f, const float* knots,
                 const S* B, uint32_t step, uint32_t k, S* ctrue, int* used, int* btime, long long* n_steps = nullptr)
 {
-    const int NT = E.e.n_models, T = I.n_timesteps, M = I.n_knots, NK = I.n_samples;
+    const int NT = n_models(E), T = I.n_timesteps, M = I.n_knots, NK = I.n_samples;
     const uint32_t key0 = (uint32_t)(I.seed & 0xffffffffu), key1 = (uint32_t)(I.seed >> 32);
     S belief0[DR_SR_MAX_MODELS];
     for (int th = 0; th < NT; ++th)
@@ -960,7 +1340,7 @@ void bm_rollout(const Env<S>& E, const dr_sr_cost& K, const dr_sr_state& x0, con
     {
         if (firstPred != 0 && firstPred != th + 1)
             continue;
-        SState<S> s = load<S>(x0);
+        auto s = load_st(E, x0);
         S belief[DR_SR_MAX_MODELS];
         for (int q = 0; q < NT; ++q)
             belief[q] = belief0[q];
@@ -990,7 +1370,7 @@ void bm_rollout(const Env<S>& E, const dr_sr_cost& K, const dr_sr_state& x0, con
                 time = t + 1;
                 origin = t + 1;
             }
-            acc = fm(state_cost(E, K, s), decay, acc);
+            acc = fm(stage_cost(E, K, s, th), decay, acc);
             decay = decay * (S)0.97f;
             if (n_steps)
                 ++*n_steps;
@@ -1010,11 +1390,11 @@ void bm_rollout(const Env<S>& E, const dr_sr_cost& K, const dr_sr_state& x0, con
 // the current branch at local time with belief update and branching, noise
 // from stream 5, the opponent's win not a loss; a loss at step t adds one
 // failure at t.
-template <class S>
-void bm_verify(const Env<S>& E, const dr_sr_state& x0, const dr_bm_instance& I, const float* belief0f, uint32_t step, int set,
-               const S* plans, int n_threads, int* fail)
+template <class S, class EnvT, class Abi>
+void bm_verify(const EnvT& E, const Abi& x0, const dr_bm_instance& I, const float* belief0f, uint32_t step, int set, const S* plans,
+               int n_threads, int* fail)
 {
-    const int NT = E.e.n_models, T = I.n_timesteps, H = I.verif_horizon;
+    const int NT = n_models(E), T = I.n_timesteps, H = I.verif_horizon;
     const uint32_t key0 = (uint32_t)(I.seed & 0xffffffffu), key1 = (uint32_t)(I.seed >> 32);
     for (int t = 0; t < T; ++t)
         fail[t] = 0;
@@ -1050,7 +1430,7 @@ void bm_verify(const Env<S>& E, const dr_sr_state& x0, const dr_bm_instance& I, 
                 break;
             }
         }
-        SState<S> s = load<S>(x0);
+        auto s = load_st(E, x0);
         int origin = 0;
         for (int t = 0; t < H; ++t)
         {
@@ -1085,14 +1465,14 @@ void bm_verify(const Env<S>& E, const dr_sr_state& x0, const dr_bm_instance& I, 
 // One control step of one branching spline MPPI controller (kernels.cu:175-486,
 // mppi_controller.cu:250-466 without verification).  In FP32 the weights
 // are FP32 and the weighted sums FP64, as in the engine.
-template <class S>
-void bm_step(const dr_sr_env& env, const dr_sr_cost& K, const dr_sr_state& x0, const dr_bm_instance& I, const float* belief, uint32_t step,
+template <class S, class Cfg, class Abi>
+void bm_step(const Cfg& env, const dr_sr_cost& K, const Abi& x0, const dr_bm_instance& I, const float* belief, uint32_t step,
              float lambda, const float* knots, const float* prev_dense, int n_threads, float* noise_out, double* ctrue_out, int* used_out,
              int* btime_out, double* costs_out, double* mins_out, double* nu_out, double* knots_out, double* dense_out,
              double* dense_new_out, int* fail_out, double* res_out)
 {
-    const Env<S> E(env);
-    const int NT = env.n_models, NB = NT + 1, T = I.n_timesteps, M = I.n_knots, NK = I.n_samples;
+    const auto E = make_env<S>(env);
+    const int NT = n_models(E), NB = NT + 1, T = I.n_timesteps, M = I.n_knots, NK = I.n_samples;
     const uint32_t key0 = (uint32_t)(I.seed & 0xffffffffu), key1 = (uint32_t)(I.seed >> 32);
     std::vector<S> B((size_t)T * M);
     spline_basis<S>(I.knots, M, T, B.data());
@@ -1225,7 +1605,7 @@ void bm_step(const dr_sr_env& env, const dr_sr_cost& K, const dr_sr_state& x0, c
         bm_verify<S>(E, x0, I, belief, step, 1, prev.data(), n_threads, fail_out + T);
     }
     // clampNominalKernel (kernels.cu:434-456) on the knots
-    const S ma = (S)env.max_accel[0];
+    const S ma = (S)max_accel_of(env);
     for (int bm = 0; bm < NB * M; ++bm)
     {
         const S a = kn[bm * 2], c = kn[bm * 2 + 1];
@@ -1727,6 +2107,147 @@ long long or_count_sr_ops(int mode, const dr_sr_env* env, const dr_sr_cost* K, c
             }
             acc = acc + final_cost(E, *K, s);
         }
+    }
+    if (steps_out)
+        *steps_out = steps;
+    return g_tally.flops;
+}
+
+// HiddenObs geometry primitives on the scalar of f64: which 0 squared
+// distance to a rectangle (x, y, xmin, ymin, xmax, ymax), 1 squared distance
+// to a half-plane strip (x, y, xmin, xmax, lambda), 2 distance to the
+// annulus (x, y, cx, cy, rmin, rmax), 3 segment crosses a rectangle (px, py,
+// qx, qy, xmin, ymin, xmax, ymax), 4 segment crosses the annulus (px, py, qx,
+// qy, cx, cy, rmin, rmax); the predicates return 0 or 1.
+double or_ho_prim(int f64, int which, const double* a)
+{
+    if (f64)
+        switch (which)
+        {
+        case 0: return ho_sq_rect(a[0], a[1], a[2], a[3], a[4], a[5]);
+        case 1: return ho_sq_hp(a[0], a[1], a[2], a[3], a[4]);
+        case 2: return ho_annulus(a[0], a[1], a[2], a[3], a[4], a[5]);
+        case 3: return ho_seg_rect(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7]) ? 1.0 : 0.0;
+        case 4: return ho_seg_annulus(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7]) ? 1.0 : 0.0;
+        default: return NAN;
+        }
+    float f[8];
+    for (int i = 0; i < 8; ++i)
+        f[i] = (float)a[i];
+    switch (which)
+    {
+    case 0: return ho_sq_rect(f[0], f[1], f[2], f[3], f[4], f[5]);
+    case 1: return ho_sq_hp(f[0], f[1], f[2], f[3], f[4]);
+    case 2: return ho_annulus(f[0], f[1], f[2], f[3], f[4], f[5]);
+    case 3: return ho_seg_rect(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7]) ? 1.0 : 0.0;
+    case 4: return ho_seg_annulus(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7]) ? 1.0 : 0.0;
+    default: return NAN;
+    }
+}
+
+// HiddenObs state functions: what 0 boundary distance under model theta,
+// 1 canSeeHiddenObs(theta), 2 getAdvance, 3 stateCost under theta (needs K),
+// 4 finalCost (needs K).
+double or_ho_eval(int f64, int what, const dr_ho_env* env, const dr_sr_cost* K, const dr_ho_state* st, int theta)
+{
+    if (!env || !st || theta < 0 || theta > 1 || ((what == 3 || what == 4) && !K))
+        return NAN;
+    if (f64)
+    {
+        const HoEnv<double> H(*env);
+        const HState<double> s = ho_load<double>(*st);
+        switch (what)
+        {
+        case 0: return ho_bdist(H, s, theta);
+        case 1: return ho_can_see(H, s, theta) ? 1.0 : 0.0;
+        case 2: return ho_advance(H, s);
+        case 3: return stage_cost(H, *K, s, theta);
+        case 4: return final_cost(H, *K, s);
+        default: return NAN;
+        }
+    }
+    const HoEnv<float> H(*env);
+    const HState<float> s = ho_load<float>(*st);
+    switch (what)
+    {
+    case 0: return ho_bdist(H, s, theta);
+    case 1: return ho_can_see(H, s, theta) ? 1.0 : 0.0;
+    case 2: return ho_advance(H, s);
+    case 3: return stage_cost(H, *K, s, theta);
+    case 4: return final_cost(H, *K, s);
+    default: return NAN;
+    }
+}
+
+// One HiddenObs environment step under model theta from *st (updated in
+// place), noise from Philox (k, 4 t + block, step, stream); belief [2] is
+// updated in place when not NULL.  Returns 0 none, 1 lose, 2 win.
+int or_ho_env_step(int f64, const dr_ho_env* env, int theta, const float* ego, dr_ho_state* st, float* belief, uint64_t seed, uint32_t k,
+                   uint32_t t, uint32_t step, uint32_t stream)
+{
+    if (!env || !ego || !st || theta < 0 || theta > 1)
+        return -1;
+    Normals rng{(uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32), k, 4u * t, step};
+    rng.stream = stream;
+    if (f64)
+    {
+        const HoEnv<double> H(*env);
+        HState<double> s = ho_load<double>(*st);
+        const double u[2] = {ego[0], ego[1]};
+        double b[2] = {belief ? belief[0] : 0.0, belief ? belief[1] : 0.0};
+        const int r = env_step(H, theta, u, s, rng, true, belief ? b : nullptr);
+        ho_store(s, *st);
+        if (belief)
+            for (int q = 0; q < 2; ++q)
+                belief[q] = (float)b[q];
+        return r;
+    }
+    const HoEnv<float> H(*env);
+    HState<float> s = ho_load<float>(*st);
+    const float u[2] = {ego[0], ego[1]};
+    const int r = env_step(H, theta, u, s, rng, true, belief);
+    ho_store(s, *st);
+    return r;
+}
+
+// or_bm_step on HiddenObs (two models).
+int or_bmho_step(int f64, const dr_ho_env* env, const dr_sr_cost* K, const dr_ho_state* x0, const dr_bm_instance* inst, const float* belief,
+                 uint32_t step, float lambda, const float* knots, const float* prev_dense, int n_threads, float* noise_out, double* ctrue_out,
+                 int* used_out, int* btime_out, double* costs_out, double* mins_out, double* nu_out, double* knots_out, double* dense_out,
+                 double* dense_new_out, int* fail_out, double* res_out)
+{
+    if (!env || !K || !x0 || !inst || !belief || !knots || !mins_out || !nu_out || !knots_out || !dense_out || !res_out)
+        return 1;
+    if (inst->n_knots < 2 || inst->n_knots > DR_MAX_KNOTS)
+        return 1;
+    if (inst->n_verif_samples > 0 && (inst->verif_horizon < 1 || inst->verif_horizon > inst->n_timesteps))
+        return 1;
+    if (f64)
+        bm_step<double>(*env, *K, *x0, *inst, belief, step, lambda, knots, prev_dense, n_threads, noise_out, ctrue_out, used_out, btime_out,
+                        costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, res_out);
+    else
+        bm_step<float>(*env, *K, *x0, *inst, belief, step, lambda, knots, prev_dense, n_threads, noise_out, ctrue_out, used_out, btime_out,
+                       costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, res_out);
+    return 0;
+}
+
+// Algorithmic FP32 work of n branching HiddenObs rollouts (sample k, every
+// model compatible with the belief) from x0, counted as or_count_sr_ops
+// counts; steps_out = rollout-steps taken.
+long long or_count_ho_ops(const dr_ho_env* env, const dr_sr_cost* K, const dr_ho_state* x0, const float* knots, int n,
+                          const dr_bm_instance* bm, const float* belief, long long* steps_out)
+{
+    const HoEnv<Cn> H(*env);
+    std::vector<double> Bd((size_t)bm->n_timesteps * bm->n_knots);
+    spline_basis<double>(bm->knots, bm->n_knots, bm->n_timesteps, Bd.data());
+    std::vector<Cn> B(Bd.begin(), Bd.end());
+    g_tally.flops = 0;
+    long long steps = 0;
+    for (int k = 0; k < n; ++k)
+    {
+        Cn c[DR_SR_MAX_MODELS];
+        int u[DR_SR_MAX_MODELS], t[DR_SR_MAX_MODELS];
+        bm_rollout<Cn>(H, *K, *x0, *bm, belief, knots, B.data(), 0, (uint32_t)k, c, u, t, &steps);
     }
     if (steps_out)
         *steps_out = steps;

@@ -78,6 +78,16 @@ def lib(abi):
     L.or_sr_predict.argtypes = [C.POINTER(abi.SrEnv), C.POINTER(abi.SrState), I, F, I, I, F, C.c_float, F, F, C.POINTER(I)]
     L.or_sr_opp_target.restype = C.c_double
     L.or_sr_opp_target.argtypes = [I, C.POINTER(abi.SrEnv), I, C.POINTER(abi.SrState)]
+    L.or_ho_prim.restype = C.c_double
+    L.or_ho_prim.argtypes = [I, I, D]
+    L.or_ho_eval.restype = C.c_double
+    L.or_ho_eval.argtypes = [I, I, C.POINTER(abi.HoEnv), C.POINTER(abi.SrCost), C.POINTER(abi.HoState), I]
+    L.or_ho_env_step.argtypes = [I, C.POINTER(abi.HoEnv), I, F, C.POINTER(abi.HoState), F, C.c_uint64, U, U, U, U]
+    L.or_bmho_step.argtypes = [I, C.POINTER(abi.HoEnv), C.POINTER(abi.SrCost), C.POINTER(abi.HoState), C.POINTER(abi.BmInstance), F,
+                               U, C.c_float, F, F, I, F, D, C.POINTER(I), C.POINTER(I), D, D, D, D, D, D, C.POINTER(I), D]
+    L.or_count_ho_ops.restype = C.c_longlong
+    L.or_count_ho_ops.argtypes = [C.POINTER(abi.HoEnv), C.POINTER(abi.SrCost), C.POINTER(abi.HoState), F, I, C.POINTER(abi.BmInstance), F,
+                                  C.POINTER(C.c_longlong)]
     _lib = L
     return L
 
@@ -415,9 +425,10 @@ class Oracle:
         return st, b[:env.n_models].copy(), r
 
     def bm_step(self, f64, env, cost, state, ins, belief, step, lam, knots, threads=8, want_noise=True, prev_dense=None):
-        """One branching spline MPPI step of a StratRace2D controller from its belief and knots [n_models + 1][M][2]
-        (and, with verification, the previous step's executed plans [n_models + 1][T][2])."""
-        K, T, M, NT = ins.n_samples, ins.n_timesteps, ins.n_knots, env.n_models
+        """One branching spline MPPI step of a StratRace2D or HiddenObs controller from its belief and knots
+        [n_models + 1][M][2] (and, with verification, the previous step's executed plans [n_models + 1][T][2])."""
+        ho = isinstance(env, self.abi.HoEnv)
+        K, T, M, NT = ins.n_samples, ins.n_timesteps, ins.n_knots, 2 if ho else env.n_models
         NB = NT + 1
         kn = np.ascontiguousarray(knots, dtype=np.float32).reshape(-1)
         assert kn.size == NB * M * 2
@@ -437,7 +448,7 @@ class Oracle:
         dense = np.empty(NB * T * 2)
         res = np.empty(4)
         ip = C.POINTER(C.c_int)
-        rc = self.L.or_bm_step(int(f64), C.byref(env), C.byref(cost), C.byref(state), C.byref(ins), _f(bel), step, lam, _f(kn),
+        rc = (self.L.or_bmho_step if ho else self.L.or_bm_step)(int(f64), C.byref(env), C.byref(cost), C.byref(state), C.byref(ins), _f(bel), step, lam, _f(kn),
                                _f(prev) if prev is not None else None, threads,
                                _f(noise) if noise is not None else None, _d(ct), used.ctypes.data_as(ip), bt.ctypes.data_as(ip),
                                _d(costs), _d(mins), _d(nu), _d(kout), _d(dense), _d(dnew), fail.ctypes.data_as(ip), _d(res))
@@ -466,3 +477,36 @@ class Oracle:
 
     def sr_opp_target(self, f64, env, theta, state):
         return self.L.or_sr_opp_target(int(f64), C.byref(env), theta, C.byref(state))
+
+    # ---------------------------------------------------------- HiddenObs
+
+    HO_PRIMS = {"sq_rect": 0, "sq_half_plane": 1, "annulus": 2, "seg_rect": 3, "seg_annulus": 4}
+
+    def ho_prim(self, f64, name, *args):
+        """A HiddenObs geometry primitive (sq_rect, sq_half_plane, annulus, seg_rect, seg_annulus)."""
+        a = np.zeros(8)
+        a[:len(args)] = args
+        return self.L.or_ho_prim(int(f64), self.HO_PRIMS[name], _d(a))
+
+    def ho_eval(self, f64, what, env, state, theta=0, cost=None):
+        """bdist, can_see, advance, state_cost or final_cost of a HiddenObs state."""
+        w = {"bdist": 0, "can_see": 1, "advance": 2, "state_cost": 3, "final_cost": 4}[what]
+        return self.L.or_ho_eval(int(f64), w, C.byref(env), C.byref(cost) if cost is not None else None, C.byref(state), theta)
+
+    def ho_env_step(self, f64, env, theta, ego, state, belief=None, seed=0, k=0, t=0, step=0, stream=4):
+        """One HiddenObs environment step; returns (state, belief or None, terminal code)."""
+        st = self.abi.HoState.from_buffer_copy(state)
+        u = np.ascontiguousarray(ego, dtype=np.float32)
+        b = None if belief is None else np.ascontiguousarray(belief, dtype=np.float32).copy()
+        r = self.L.or_ho_env_step(int(f64), C.byref(env), theta, _f(u), C.byref(st), _f(b) if b is not None else None, seed, k, t, step,
+                                  stream)
+        return st, b, r
+
+    def count_ho(self, env, cost, state, ins, knots, belief, n):
+        """Counted FP32 work of n branching HiddenObs rollouts: (ops, rollout-steps)."""
+        kn = np.ascontiguousarray(knots, dtype=np.float32).reshape(-1)
+        b = np.zeros(self.abi.SR_MAX_MODELS, dtype=np.float32)
+        b[:2] = belief[:2]
+        steps = C.c_longlong(0)
+        ops = self.L.or_count_ho_ops(C.byref(env), C.byref(cost), C.byref(state), _f(kn), n, C.byref(ins), _f(b), C.byref(steps))
+        return ops, steps.value

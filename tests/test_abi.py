@@ -17,7 +17,7 @@ STRUCTS = {
     "dr_launch_cfg": "LaunchCfg", "dr_mppi_cfg": "MPPICfg", "dr_pr_instance": "PrInstance", "dr_pr_diag": "PrDiag",
     "dr_sr_opp_config": "SrOppConfig", "dr_sr_env": "SrEnv", "dr_sr_state": "SrState", "dr_sr_cost": "SrCost",
     "dr_sr_instance": "SrInstance", "dr_prsr_instance": "PrsrInstance", "dr_bm_instance": "BmInstance", "dr_bm_diag": "BmDiag",
-    "dr_wire_env_info": "WireEnvInfo", "dr_wire_controller": "WireController",
+    "dr_wire_env_info": "WireEnvInfo", "dr_wire_controller": "WireController", "dr_ho_env": "HoEnv", "dr_ho_state": "HoState",
 }
 RENAMED = {"lambda_": "lambda"}
 

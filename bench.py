@@ -28,11 +28,11 @@ FLOPS_PER_SAMPLE_STEP = 270.07
 # HBM bytes per rollout sample-step the rollout kernel must move: the float4
 # noise row it writes (read back once by the weighted average).
 ROLLOUT_BYTES_PER_SAMPLE_STEP = 16.0
-# FP32 flops per rollout-step of the StratRace2D workloads (oracle op counter at the first control
-# step, plus noise; oracle/count_ops.py count_sr) and, for the branching rollout, the fraction of
-# the K * models * T rollout-steps actually run there (rollouts stop at terminal events).
-SR_FLOPS_PER_STEP = {"sr_race": 1044.30, "pr_sr_race": 1010.14, "bm_race": 1513.46}
-BM_STEPS_FRACTION = 0.73
+# FP32 flops per rollout-step of the StratRace2D and HiddenObs workloads (oracle op counter at the
+# first control step, plus noise; oracle/count_ops.py count_sr and count_ho).  The branching
+# rollouts stop at terminal events: their launches are priced by the rollout-steps they ran, which
+# the engine reports per rollout.
+SR_FLOPS_PER_STEP = {"sr_race": 1044.30, "pr_sr_race": 1010.14, "bm_race": 1513.46, "ho_race": 689.45}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
@@ -133,6 +133,15 @@ def cpu_stratrace(pkg, budget_s=12.0):
     def bm_step():
         oracle.bm_step(0, s3.env, s3.cost, s3.states[0], i3, b3, 0, i3.lambda_, kn, threads=threads, want_noise=False)
     out["bm_race"] = timed(bm_step, s3.env.n_models * i3.n_samples * i3.n_timesteps)
+
+    s4 = sc.ho_race(n_inst=1)
+    i4 = s4.instances[0]
+    k4 = np.zeros((3, i4.n_knots, 2), np.float32)
+    b4 = np.array(i4.init_belief[:2], np.float32)
+
+    def ho_step():
+        oracle.bm_step(0, s4.env, s4.cost, s4.states[0], i4, b4, 0, i4.lambda_, k4, threads=threads, want_noise=False)
+    out["ho_race"] = timed(ho_step, 2 * i4.n_samples * i4.n_timesteps)
     return out
 
 
@@ -291,6 +300,21 @@ def pr_sr_race(pkg, steps=30, warmup=3):
             "roofline": sr_roofline("pr_sr_race", rollouts * ins.n_timesteps, float(ms[0]), "k_prsr_rollout"), "steps": steps}
 
 
+def timed_branching_steps(eng, n=5):
+    """n more closed-loop steps, one at a time, with HIP-event timing: the mean device ms of each
+    phase and the mean rollout-steps the rollout launch ran."""
+    import numpy as np
+    eng.set_timing(True)
+    ms = np.zeros(6)
+    ran = 0
+    for _ in range(n):
+        eng.step(1, advance=True)
+        ms += eng.last_timing()
+        ran += int(eng.rollout_lengths(0).sum())
+    eng.set_timing(False)
+    return ms / n, ran / n
+
+
 def bm_race(pkg, steps=30, warmup=3, n_verif=0):
     """Branching spline MPPI (the reference's default MPPI controller) on its StratRace2D race at
     the spline setting (K = 2^16, T = 80, 20 knots, two opponent models, uniform initial belief,
@@ -304,9 +328,7 @@ def bm_race(pkg, steps=30, warmup=3, n_verif=0):
     t0 = time.perf_counter()
     eng.step(steps, advance=True)
     elapsed = time.perf_counter() - t0
-    eng.set_timing(True)
-    eng.step(5, advance=True)
-    ms = eng.last_timing() / 5
+    ms, ran = timed_branching_steps(eng)
     d = eng.diag()[0]
     eng.close()
     ins = scn.instances[0]
@@ -316,8 +338,40 @@ def bm_race(pkg, steps=30, warmup=3, n_verif=0):
             "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate_min": float(ms[1]), "weighted_avg": float(ms[2]),
                                    "update_verify": float(ms[3]), "choice_world": float(ms[4]), "step": float(ms[5])},
             "final_branch": int(d.branch), "use_new_plan": int(d.use_new_plan), "certified_loss": float(d.certified_loss),
-            "roofline": sr_roofline("bm_race", BM_STEPS_FRACTION * rollouts * ins.n_timesteps, float(ms[0]), "k_bm_rollout"),
+            "rollout_steps_per_launch": ran, "rollout_steps_fraction": ran / (rollouts * ins.n_timesteps),
+            "roofline": sr_roofline("bm_race", ran, float(ms[0]), "k_bm_rollout"),
             "verification_samples": n_verif, "steps": steps}
+
+
+def ho_race(pkg, steps=30, warmup=3):
+    """Branching spline MPPI on the reference's HiddenObs course at its spline setting
+    (hiddenObsEnv, main.py:705-833: K = 2^16, T = 80, 20 knots, temperature 10 adapted,
+    sigma 0.8, uniform belief over which hidden obstacle exists), closed loop, one world."""
+    sc = pkg.scenarios
+    scn = sc.ho_race(n_inst=1)
+    eng = pkg.HoBMEngine(scn)
+    eng.step(warmup, advance=True)
+    t0 = time.perf_counter()
+    eng.step(steps, advance=True)
+    elapsed = time.perf_counter() - t0
+    ms, ran = timed_branching_steps(eng)
+    d = eng.diag()[0]
+    st = eng.get_states()[0]
+    belief = eng.belief(0)
+    events = int(eng.events()[0])
+    eng.close()
+    ins = scn.instances[0]
+    rollouts = 2 * ins.n_samples
+    return {"workload": scn.name, "control_steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+            "rollouts_per_step": rollouts, "rollout_env_steps_per_s": rollouts * ins.n_timesteps * steps / elapsed,
+            "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate_min": float(ms[1]), "weighted_avg": float(ms[2]),
+                                   "update_verify": float(ms[3]), "choice_world": float(ms[4]), "step": float(ms[5])},
+            "final_branch": int(d.branch), "final_state": {"pos": [float(st.pos[0]), float(st.pos[1])], "gates": int(st.gates),
+                                                           "laps": int(st.laps), "event": events},
+            "final_belief": [float(b) for b in belief],
+            "rollout_steps_per_launch": ran, "rollout_steps_fraction": ran / (rollouts * ins.n_timesteps),
+            "roofline": sr_roofline("ho_race", ran, float(ms[0]), "k_bm_rollout<HoPolicy>"),
+            "steps": steps}
 
 
 def session_race(pkg, steps=30):
@@ -364,6 +418,7 @@ def main():
     ap.add_argument("--sr-only", action="store_true", help="run only the StratRace2D section (for profiling)")
     ap.add_argument("--bm-only", action="store_true", help="run only the branching spline MPPI section (for profiling)")
     ap.add_argument("--session-only", action="store_true", help="run only the wire-protocol session section")
+    ap.add_argument("--ho-only", action="store_true", help="run only the HiddenObs section (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 512 MiB buffer 3 times (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -387,6 +442,9 @@ def main():
     sc = pkg.scenarios
     if args.session_only:
         print(json.dumps({"session": session_race(pkg, steps=min(args.steps, 30))}), flush=True)
+        return
+    if args.ho_only:
+        print(json.dumps({"ho_race": ho_race(pkg, steps=min(args.steps, 30), warmup=args.warmup)}), flush=True)
         return
     if args.bm_only:
         print(json.dumps({"bm_race": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup),
@@ -501,6 +559,9 @@ def main():
         log("bench: branching spline MPPI with plan verification (2^17 samples, horizon 40) ...")
         out["bm_race_verified"] = bm_race(pkg, n_verif=2 ** 17)
         log(f"bench: bm_race_verified {json.dumps(out['bm_race_verified'])}")
+        log("bench: branching spline MPPI on HiddenObs, K=65536, T=80, 20 knots ...")
+        out["ho_race"] = ho_race(pkg)
+        log(f"bench: ho_race {json.dumps(out['ho_race'])}")
         log("bench: wire-protocol session (MPPI and PR-MPPI headers at the reference settings) ...")
         out["session"] = session_race(pkg)
         log(f"bench: session {json.dumps(out['session'])}")

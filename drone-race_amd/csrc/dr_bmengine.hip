@@ -23,7 +23,7 @@ namespace dr {
 
 hipError_t launch_bm_rollout(int env_kind, int max_m, int max_tm, int grid, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn,
                              const BmTile* tiles, int n_tiles, const void* env, const dr_sr_cost& KC, const void* states,
-                             const float* knots, const float* basis, float2* noise, float* ctrue, int* used, int* btime);
+                             const float* knots, const float* basis, float2* noise, float* ctrue, int* used, int* btime, int* rlen);
 hipError_t launch_bm_reduce(int n_inst, int max_nbk, int n_rtiles, hipStream_t st, const BmInstDev* insts, BmDyn* dyn, const BmRTile* rt,
                             const float* ctrue, float* costs, const int* used, const int* btime, float* pmin, float* mins,
                             const float2* noise, double* part, hipEvent_t mid);
@@ -148,7 +148,7 @@ struct dr_bm_engine
     DevBuf<uint8_t> d_env;    // dr_sr_env or dr_ho_env
     DevBuf<uint8_t> d_states; // dr_sr_state or dr_ho_state per instance
     DevBuf<float> d_knots, d_dense, d_dnew, d_B, d_ctrue, d_costs, d_mins, d_pmin, d_nu, d_actions;
-    DevBuf<int> d_used, d_btime, d_events, d_fail, d_verdict;
+    DevBuf<int> d_used, d_btime, d_rlen, d_events, d_fail, d_verdict;
     DevBuf<float2> d_noise;
     DevBuf<double> d_part;
 
@@ -186,7 +186,7 @@ int dr_bm_engine::enqueue_a(hipEvent_t* ev)
     if (ev)
         HIPCHK(hipEventRecord(ev[0], stream));
     HIPCHK(launch_bm_rollout(env_kind, max_m, max_tm, 0, stream, d_inst.p, d_dyn.p, d_tiles.p, (int)tiles.size(), d_env.p, cost, d_states.p,
-                             d_knots.p, d_B.p, d_noise.p, d_ctrue.p, d_used.p, d_btime.p));
+                             d_knots.p, d_B.p, d_noise.p, d_ctrue.p, d_used.p, d_btime.p, d_rlen.p));
     if (ev)
         HIPCHK(hipEventRecord(ev[1], stream));
     HIPCHK(launch_bm_reduce(n, max_nbk, (int)rtiles.size(), stream, d_inst.p, d_dyn.p, d_rt.p, d_ctrue.p, d_costs.p, d_used.p, d_btime.p,
@@ -544,7 +544,7 @@ static int bm_create(int env_kind, const void* env, size_t env_size, size_t stat
         !chk(e->d_ctrue.alloc(e->ct_total), "alloc ctrue") || !chk(e->d_costs.alloc(e->cost_total), "alloc costs") ||
         !chk(e->d_mins.alloc(e->min_total), "alloc mins") || !chk(e->d_nu.alloc(e->nu_total), "alloc nu") ||
         !chk(e->d_actions.alloc((size_t)n * 2), "alloc actions") || !chk(e->d_used.alloc(e->ct_total), "alloc used") ||
-        !chk(e->d_btime.alloc(e->ct_total), "alloc btime") || !chk(e->d_events.alloc(n), "alloc events") ||
+        !chk(e->d_btime.alloc(e->ct_total), "alloc btime") || !chk(e->d_rlen.alloc(e->ct_total), "alloc rlen") || !chk(e->d_events.alloc(n), "alloc events") ||
         !chk(e->d_noise.alloc(e->noise_total), "alloc noise") || !chk(e->d_part.alloc(e->part_total), "alloc part"))
         return bail(DR_ENOMEM);
     std::vector<BmDyn> dyn(n);
@@ -781,6 +781,16 @@ int dr_bm_get_branches(dr_bm_engine* e, int instance, int32_t* used, int32_t* ti
     if (int rc = bm_out(e, used, e->d_used.p + I.ct_off, (size_t)n))
         return rc;
     return bm_out(e, time, e->d_btime.p + I.ct_off, (size_t)n);
+}
+
+int dr_bm_get_rollout_lengths(dr_bm_engine* e, int instance, int32_t* lengths, int n)
+{
+    if (int rc = check_bm(e, instance))
+        return rc;
+    const BmInstDev& I = e->idev[instance];
+    if (!lengths || n != I.NT * I.K)
+        return host_fail(DR_EINVAL, "get_rollout_lengths: expected n_models * K values");
+    return bm_out(e, lengths, e->d_rlen.p + I.ct_off, (size_t)n);
 }
 
 int dr_bm_get_costs(dr_bm_engine* e, int instance, float* costs, int n)

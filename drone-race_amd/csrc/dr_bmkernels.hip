@@ -65,7 +65,8 @@ __device__ __forceinline__ bool spline_contributed(int b, int m, int btime, int 
 // branch, the environment step updates the belief, and a branch commits
 // when the belief becomes confident (t_origin = t + 1).  With MAXM > 0
 // (and M <= MAXM) the current branch's knot noise is held in registers;
-// with MAXM == 0 it is read back from the noise rows in HBM.
+// with MAXM == 0 it is read back from the noise rows in HBM; the
+// environment policy picks one (P::REG_NOISE), by measurement.
 // The spline basis sits in dynamic LDS (T * M floats of the largest instance).
 template <int MAXM, class P>
 __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __restrict__ insts, const BmDyn* __restrict__ dyn,
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                                                          dr_sr_cost KC, const typename P::Abi* __restrict__ states,
                                                          const float* __restrict__ knots, const float* __restrict__ basis,
                                                          float2* __restrict__ noise, float* __restrict__ ctrue, int* __restrict__ used,
-                                                         int* __restrict__ btime)
+                                                         int* __restrict__ btime, int* __restrict__ rlen)
 {
     __shared__ float s_env[P::LDS_FLOATS];
     extern __shared__ float s_B[];
@@ -118,6 +119,7 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
             ctrue[I.ct_off + th * K + k] = __builtin_inff();
             used[I.ct_off + th * K + k] = firstPred;
             btime[I.ct_off + th * K + k] = firstTime;
+            rlen[I.ct_off + th * K + k] = 0;
             const typename P::State x0 = P::load(states[bt.inst]);
             if (firstPred == 0 || firstPred == th + 1)
             {
@@ -135,6 +137,7 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                         e[m] = m < M ? knot_noise(key0, key1, (uint32_t)k, pred, m, step, I.sigma) : make_float2(0.0f, 0.0f);
                 }
                 float acc = 0.0f, decay = 1.0f;
+                int executed = T;
                 for (int t = 0; t < T; ++t)
                 {
                     const float* Brow = s_B + (t - origin) * M;
@@ -180,12 +183,16 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                     acc = fma_(P::stage_cost(ctx, E, KC, s, th), decay, acc);
                     decay = decay * DECAY;
                     if (term != SR_NONE)
+                    {
+                        executed = t + 1;
                         break;
+                    }
                 }
                 acc = acc + P::final_cost(ctx, E, KC, s);
                 ctrue[I.ct_off + th * K + k] = acc;
                 used[I.ct_off + th * K + k] = pred;
                 btime[I.ct_off + th * K + k] = time;
+                rlen[I.ct_off + th * K + k] = executed;
             }
         }
     }
@@ -584,30 +591,36 @@ __global__ __launch_bounds__(64) void k_bm_advance(const BmInstDev* __restrict__
 template <class P>
 static void rollout_t(int max_m, int g, size_t lds, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmTile* tiles,
                       int n_tiles, const void* env, const dr_sr_cost& KC, const void* states, const float* knots, const float* basis,
-                      float2* noise, float* ctrue, int* used, int* btime)
+                      float2* noise, float* ctrue, int* used, int* btime, int* rlen)
 {
     const auto* E = static_cast<const typename P::Env*>(env);
     const auto* S = static_cast<const typename P::Abi*>(states);
-    if (max_m <= 20)
-        k_bm_rollout<20, P><<<g, BM_BLOCK, lds, st>>>(insts, dyn, tiles, n_tiles, E, KC, S, knots, basis, noise, ctrue, used, btime);
-    else
-        k_bm_rollout<0, P><<<g, BM_BLOCK, lds, st>>>(insts, dyn, tiles, n_tiles, E, KC, S, knots, basis, noise, ctrue, used, btime);
+    if constexpr (P::REG_NOISE)
+    {
+        if (max_m <= 20)
+        {
+            k_bm_rollout<20, P><<<g, BM_BLOCK, lds, st>>>(insts, dyn, tiles, n_tiles, E, KC, S, knots, basis, noise, ctrue, used, btime,
+                                                          rlen);
+            return;
+        }
+    }
+    k_bm_rollout<0, P><<<g, BM_BLOCK, lds, st>>>(insts, dyn, tiles, n_tiles, E, KC, S, knots, basis, noise, ctrue, used, btime, rlen);
 }
 
 // env_kind: BM_ENV_SR (env dr_sr_env, states dr_sr_state) or BM_ENV_HO
 // (dr_ho_env, dr_ho_state)
 hipError_t launch_bm_rollout(int env_kind, int max_m, int max_tm, int grid, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn,
                              const BmTile* tiles, int n_tiles, const void* env, const dr_sr_cost& KC, const void* states,
-                             const float* knots, const float* basis, float2* noise, float* ctrue, int* used, int* btime)
+                             const float* knots, const float* basis, float2* noise, float* ctrue, int* used, int* btime, int* rlen)
 {
     const int g = grid > 0 ? (grid < n_tiles ? grid : n_tiles) : n_tiles;
     if (max_m > DR_MAX_KNOTS || max_tm > BM_MAX_TM || max_tm < 1)
         return hipErrorInvalidValue;
     const size_t lds = (size_t)max_tm * sizeof(float);
     if (env_kind == BM_ENV_SR)
-        rollout_t<SrPolicy>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime);
+        rollout_t<SrPolicy>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime, rlen);
     else if (env_kind == BM_ENV_HO)
-        rollout_t<HoPolicy>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime);
+        rollout_t<HoPolicy>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime, rlen);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

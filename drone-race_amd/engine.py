@@ -597,6 +597,13 @@ class BMEngine:
         check(self.lib.dr_bm_get_branches(self.h, i, u.ctypes.data_as(ip), t.ctypes.data_as(ip), u.size))
         return u.reshape(NT, K), t.reshape(NT, K)
 
+    def rollout_lengths(self, i):
+        """Environment steps each rollout of the last step ran [n_models][K]."""
+        K, T, M, NT = self._dims(i)
+        a = np.empty(NT * K, dtype=np.int32)
+        check(self.lib.dr_bm_get_rollout_lengths(self.h, i, a.ctypes.data_as(C.POINTER(C.c_int32)), a.size))
+        return a.reshape(NT, K)
+
     def costs(self, i):
         K, T, M, NT = self._dims(i)
         a = np.empty((NT + 1) * K, dtype=np.float32)

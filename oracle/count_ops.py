@@ -116,8 +116,23 @@ def count_sr(pkg, n=256):
     return out
 
 
+def count_ho(pkg, n=256):
+    """FP32 flops per rollout-step of the branching HiddenObs workload at its first control step
+    (oracle op counter over n samples, early stops included) plus the knot noise of a sample shared
+    by its model rollouts; the course has no environment noise."""
+    sys.path.insert(0, HERE)
+    import pyoracle
+    oracle = pyoracle.Oracle(pkg.abi)
+    s = pkg.scenarios.ho_race(n_inst=1)
+    ins = s.instances[0]
+    kn = np.zeros(3 * ins.n_knots * 2, np.float32)
+    f, steps = oracle.count_ho(s.env, s.cost, s.states[0], ins, kn, np.float32(ins.init_belief[:2]), n)
+    knot_noise = 3 * ins.n_knots * SR_CONTROL_NOISE * n / steps
+    return {"ho_race": f / steps + knot_noise, "ho_race_steps_fraction": steps / (n * 2 * ins.n_timesteps)}
+
+
 if __name__ == "__main__":
     pkg = package()
     print(f"{count(pkg.scenarios.c1_four_drones()):.2f} FP32 flops per sample-step (4 drones, K=16384, T=100)")
-    for k, v in count_sr(pkg).items():
+    for k, v in {**count_sr(pkg), **count_ho(pkg)}.items():
         print(f"{k}: {v:.2f}")

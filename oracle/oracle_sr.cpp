@@ -1321,7 +1321,7 @@ inline bool bm_spline_contributed(int b, int m, int btime, int used, int T, int 
 // step updates the belief; a branch commits when the belief is confident.
 template <class S, class EnvT, class Abi>
 void bm_rollout(const EnvT& E, const dr_sr_cost& K, const Abi& x0, const dr_bm_instance& I, const float* belief0f, const float* knots,
-                const S* B, uint32_t step, uint32_t k, S* ctrue, int* used, int* btime, long long* n_steps = nullptr)
+                const S* B, uint32_t step, uint32_t k, S* ctrue, int* used, int* btime, int* len = nullptr, long long* n_steps = nullptr)
 {
     const int NT = n_models(E), T = I.n_timesteps, M = I.n_knots, NK = I.n_samples;
     const uint32_t key0 = (uint32_t)(I.seed & 0xffffffffu), key1 = (uint32_t)(I.seed >> 32);
@@ -1335,6 +1335,8 @@ void bm_rollout(const EnvT& E, const dr_sr_cost& K, const Abi& x0, const dr_bm_i
         ctrue[th] = (S)INFINITY;
         used[th] = firstPred;
         btime[th] = firstTime;
+        if (len)
+            len[th] = 0;
     }
     for (int th = 0; th < NT; ++th)
     {
@@ -1374,6 +1376,8 @@ void bm_rollout(const EnvT& E, const dr_sr_cost& K, const Abi& x0, const dr_bm_i
             decay = decay * (S)0.97f;
             if (n_steps)
                 ++*n_steps;
+            if (len)
+                len[th] = t + 1;
             if (term != 0)
                 break;
         }
@@ -1469,7 +1473,7 @@ template <class S, class Cfg, class Abi>
 void bm_step(const Cfg& env, const dr_sr_cost& K, const Abi& x0, const dr_bm_instance& I, const float* belief, uint32_t step,
              float lambda, const float* knots, const float* prev_dense, int n_threads, float* noise_out, double* ctrue_out, int* used_out,
              int* btime_out, double* costs_out, double* mins_out, double* nu_out, double* knots_out, double* dense_out,
-             double* dense_new_out, int* fail_out, double* res_out)
+             double* dense_new_out, int* fail_out, int* len_out, double* res_out)
 {
     const auto E = make_env<S>(env);
     const int NT = n_models(E), NB = NT + 1, T = I.n_timesteps, M = I.n_knots, NK = I.n_samples;
@@ -1478,7 +1482,7 @@ void bm_step(const Cfg& env, const dr_sr_cost& K, const Abi& x0, const dr_bm_ins
     spline_basis<S>(I.knots, M, T, B.data());
     std::vector<float> noise((size_t)NB * M * NK * 2);
     std::vector<S> ct((size_t)NT * NK);
-    std::vector<int> used((size_t)NT * NK), bt((size_t)NT * NK);
+    std::vector<int> used((size_t)NT * NK), bt((size_t)NT * NK), lens((size_t)NT * NK);
 #pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads > 0 ? n_threads : 1)
     for (int k = 0; k < NK; ++k)
     {
@@ -1487,12 +1491,14 @@ void bm_step(const Cfg& env, const dr_sr_cost& K, const Abi& x0, const dr_bm_ins
                 bm_knot_noise(key0, key1, (uint32_t)k, b, m, step, I.sigma, &noise[(((size_t)b * M + m) * NK + k) * 2]);
         S c[DR_SR_MAX_MODELS];
         int u[DR_SR_MAX_MODELS], t[DR_SR_MAX_MODELS];
-        bm_rollout<S>(E, K, x0, I, belief, knots, B.data(), step, (uint32_t)k, c, u, t);
+        int ln[DR_SR_MAX_MODELS];
+        bm_rollout<S>(E, K, x0, I, belief, knots, B.data(), step, (uint32_t)k, c, u, t, ln);
         for (int th = 0; th < NT; ++th)
         {
             ct[(size_t)th * NK + k] = c[th];
             used[(size_t)th * NK + k] = u[th];
             bt[(size_t)th * NK + k] = t[th];
+            lens[(size_t)th * NK + k] = ln[th];
         }
     }
     // aggregateBranchCostsKernel (kernels.cu:175-217)
@@ -1659,6 +1665,8 @@ void bm_step(const Cfg& env, const dr_sr_cost& K, const Abi& x0, const dr_bm_ins
             used_out[i] = used[i];
         if (btime_out)
             btime_out[i] = bt[i];
+        if (len_out)
+            len_out[i] = lens[i];
     }
     if (costs_out)
         for (size_t i = 0; i < costs.size(); ++i)
@@ -1951,10 +1959,11 @@ int or_sr_env_step_belief(int f64, const dr_sr_env* env, int theta, const float*
 // chosen branch.  With verification: the unshifted new plans (dense_new_out)
 // and the failure counts [2][T] of the new plans and of prev_dense (the
 // previous step's executed plans, [n_models + 1][T][2]; NULL = zero).
+// len_out [n_models][K]: environment steps each rollout ran.
 int or_bm_step(int f64, const dr_sr_env* env, const dr_sr_cost* K, const dr_sr_state* x0, const dr_bm_instance* inst, const float* belief,
                uint32_t step, float lambda, const float* knots, const float* prev_dense, int n_threads, float* noise_out, double* ctrue_out,
                int* used_out, int* btime_out, double* costs_out, double* mins_out, double* nu_out, double* knots_out, double* dense_out,
-               double* dense_new_out, int* fail_out, double* res_out)
+               double* dense_new_out, int* fail_out, int* len_out, double* res_out)
 {
     if (!env || !K || !x0 || !inst || !belief || !knots || !mins_out || !nu_out || !knots_out || !dense_out || !res_out)
         return 1;
@@ -1964,10 +1973,10 @@ int or_bm_step(int f64, const dr_sr_env* env, const dr_sr_cost* K, const dr_sr_s
         return 1;
     if (f64)
         bm_step<double>(*env, *K, *x0, *inst, belief, step, lambda, knots, prev_dense, n_threads, noise_out, ctrue_out, used_out, btime_out,
-                        costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, res_out);
+                        costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, len_out, res_out);
     else
         bm_step<float>(*env, *K, *x0, *inst, belief, step, lambda, knots, prev_dense, n_threads, noise_out, ctrue_out, used_out, btime_out,
-                       costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, res_out);
+                       costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, len_out, res_out);
     return 0;
 }
 
@@ -2079,7 +2088,7 @@ long long or_count_sr_ops(int mode, const dr_sr_env* env, const dr_sr_cost* K, c
         {
             Cn c[DR_SR_MAX_MODELS];
             int u[DR_SR_MAX_MODELS], t[DR_SR_MAX_MODELS];
-            bm_rollout<Cn>(E, *K, *x0, *bm, belief, nominal, B.data(), 0, (uint32_t)k, c, u, t, &steps);
+            bm_rollout<Cn>(E, *K, *x0, *bm, belief, nominal, B.data(), 0, (uint32_t)k, c, u, t, nullptr, &steps);
         }
     }
     else
@@ -2214,7 +2223,7 @@ int or_ho_env_step(int f64, const dr_ho_env* env, int theta, const float* ego, d
 int or_bmho_step(int f64, const dr_ho_env* env, const dr_sr_cost* K, const dr_ho_state* x0, const dr_bm_instance* inst, const float* belief,
                  uint32_t step, float lambda, const float* knots, const float* prev_dense, int n_threads, float* noise_out, double* ctrue_out,
                  int* used_out, int* btime_out, double* costs_out, double* mins_out, double* nu_out, double* knots_out, double* dense_out,
-                 double* dense_new_out, int* fail_out, double* res_out)
+                 double* dense_new_out, int* fail_out, int* len_out, double* res_out)
 {
     if (!env || !K || !x0 || !inst || !belief || !knots || !mins_out || !nu_out || !knots_out || !dense_out || !res_out)
         return 1;
@@ -2224,10 +2233,10 @@ int or_bmho_step(int f64, const dr_ho_env* env, const dr_sr_cost* K, const dr_ho
         return 1;
     if (f64)
         bm_step<double>(*env, *K, *x0, *inst, belief, step, lambda, knots, prev_dense, n_threads, noise_out, ctrue_out, used_out, btime_out,
-                        costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, res_out);
+                        costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, len_out, res_out);
     else
         bm_step<float>(*env, *K, *x0, *inst, belief, step, lambda, knots, prev_dense, n_threads, noise_out, ctrue_out, used_out, btime_out,
-                       costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, res_out);
+                       costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, len_out, res_out);
     return 0;
 }
 
@@ -2247,7 +2256,7 @@ long long or_count_ho_ops(const dr_ho_env* env, const dr_sr_cost* K, const dr_ho
     {
         Cn c[DR_SR_MAX_MODELS];
         int u[DR_SR_MAX_MODELS], t[DR_SR_MAX_MODELS];
-        bm_rollout<Cn>(H, *K, *x0, *bm, belief, knots, B.data(), 0, (uint32_t)k, c, u, t, &steps);
+        bm_rollout<Cn>(H, *K, *x0, *bm, belief, knots, B.data(), 0, (uint32_t)k, c, u, t, nullptr, &steps);
     }
     if (steps_out)
         *steps_out = steps;

@@ -74,7 +74,7 @@ def lib(abi):
     L.or_sr_update_belief.argtypes = [I, C.POINTER(abi.SrEnv), D, D, D]
     L.or_sr_env_step_belief.argtypes = [I, C.POINTER(abi.SrEnv), I, F, C.POINTER(abi.SrState), F, C.c_uint64, U, U, U, I, U]
     L.or_bm_step.argtypes = [I, C.POINTER(abi.SrEnv), C.POINTER(abi.SrCost), C.POINTER(abi.SrState), C.POINTER(abi.BmInstance), F,
-                             U, C.c_float, F, F, I, F, D, C.POINTER(I), C.POINTER(I), D, D, D, D, D, D, C.POINTER(I), D]
+                             U, C.c_float, F, F, I, F, D, C.POINTER(I), C.POINTER(I), D, D, D, D, D, D, C.POINTER(I), C.POINTER(I), D]
     L.or_sr_predict.argtypes = [C.POINTER(abi.SrEnv), C.POINTER(abi.SrState), I, F, I, I, F, C.c_float, F, F, C.POINTER(I)]
     L.or_sr_opp_target.restype = C.c_double
     L.or_sr_opp_target.argtypes = [I, C.POINTER(abi.SrEnv), I, C.POINTER(abi.SrState)]
@@ -84,7 +84,7 @@ def lib(abi):
     L.or_ho_eval.argtypes = [I, I, C.POINTER(abi.HoEnv), C.POINTER(abi.SrCost), C.POINTER(abi.HoState), I]
     L.or_ho_env_step.argtypes = [I, C.POINTER(abi.HoEnv), I, F, C.POINTER(abi.HoState), F, C.c_uint64, U, U, U, U]
     L.or_bmho_step.argtypes = [I, C.POINTER(abi.HoEnv), C.POINTER(abi.SrCost), C.POINTER(abi.HoState), C.POINTER(abi.BmInstance), F,
-                               U, C.c_float, F, F, I, F, D, C.POINTER(I), C.POINTER(I), D, D, D, D, D, D, C.POINTER(I), D]
+                               U, C.c_float, F, F, I, F, D, C.POINTER(I), C.POINTER(I), D, D, D, D, D, D, C.POINTER(I), C.POINTER(I), D]
     L.or_count_ho_ops.restype = C.c_longlong
     L.or_count_ho_ops.argtypes = [C.POINTER(abi.HoEnv), C.POINTER(abi.SrCost), C.POINTER(abi.HoState), F, I, C.POINTER(abi.BmInstance), F,
                                   C.POINTER(C.c_longlong)]
@@ -441,6 +441,7 @@ class Oracle:
         ct = np.empty(NT * K)
         used = np.empty(NT * K, dtype=np.int32)
         bt = np.empty(NT * K, dtype=np.int32)
+        lens = np.empty(NT * K, dtype=np.int32)
         costs = np.empty(NB * K)
         mins = np.empty(NB * M)
         nu = np.empty(NB)
@@ -451,10 +452,11 @@ class Oracle:
         rc = (self.L.or_bmho_step if ho else self.L.or_bm_step)(int(f64), C.byref(env), C.byref(cost), C.byref(state), C.byref(ins), _f(bel), step, lam, _f(kn),
                                _f(prev) if prev is not None else None, threads,
                                _f(noise) if noise is not None else None, _d(ct), used.ctypes.data_as(ip), bt.ctypes.data_as(ip),
-                               _d(costs), _d(mins), _d(nu), _d(kout), _d(dense), _d(dnew), fail.ctypes.data_as(ip), _d(res))
+                               _d(costs), _d(mins), _d(nu), _d(kout), _d(dense), _d(dnew), fail.ctypes.data_as(ip),
+                               lens.ctypes.data_as(ip), _d(res))
         assert rc == 0
         return dict(noise=noise.reshape(NB, M, K, 2) if noise is not None else None, ctrue=ct.reshape(NT, K),
-                    used=used.reshape(NT, K), btime=bt.reshape(NT, K), costs=costs.reshape(NB, K), mins=mins.reshape(NB, M),
+                    used=used.reshape(NT, K), btime=bt.reshape(NT, K), len=lens.reshape(NT, K), costs=costs.reshape(NB, K), mins=mins.reshape(NB, M),
                     nu=nu, knots=kout.reshape(NB, M, 2), dense=dense.reshape(NB, T, 2), action=res[:2], lambda_=float(res[2]),
                     branch=int(res[3]), dense_new=dnew.reshape(NB, T, 2), fail_new=fail[:T].copy(), fail_old=fail[T:].copy())
 

@@ -24,6 +24,12 @@ def test_stratrace_flops_match_counter(pkg):
     bench = _load(os.path.join(ROOT, "bench.py"), "bench_mod")
     count_ops = _load(os.path.join(ROOT, "oracle", "count_ops.py"), "count_ops_mod")
     measured = count_ops.count_sr(pkg)
-    for key, v in bench.SR_FLOPS_PER_STEP.items():
-        assert abs(v - measured[key]) < 0.01, (key, measured[key])
-    assert abs(bench.BM_STEPS_FRACTION - measured["bm_race_steps_fraction"]) < 0.005
+    for key in ("sr_race", "pr_sr_race", "bm_race"):
+        assert abs(bench.SR_FLOPS_PER_STEP[key] - measured[key]) < 0.01, (key, measured[key])
+
+
+def test_hiddenobs_flops_match_counter(pkg):
+    bench = _load(os.path.join(ROOT, "bench.py"), "bench_mod")
+    count_ops = _load(os.path.join(ROOT, "oracle", "count_ops.py"), "count_ops_mod")
+    measured = count_ops.count_ho(pkg)
+    assert abs(bench.SR_FLOPS_PER_STEP["ho_race"] - measured["ho_race"]) < 0.01, measured

@@ -47,6 +47,7 @@ def check_step(oracle, eng, sc, states, beliefs, knots, lambdas, acts, step, wan
         used, bt = eng.branches(i)
         np.testing.assert_array_equal(used, r["used"])
         np.testing.assert_array_equal(bt, r["btime"])
+        np.testing.assert_array_equal(eng.rollout_lengths(i), r["len"])
         g_ct = eng.costs_true(i)
         np.testing.assert_array_equal(g_ct.astype(np.float64), r["ctrue"])
         np.testing.assert_array_equal(eng.costs(i).astype(np.float64), r["costs"])

@@ -110,15 +110,17 @@ def replay(oracle, env, ins, x0, th, k, c, step, belief):
     st, b = x0, np.array(belief, np.float32)
     pred = next((i + 1 for i in range(env.n_models) if b[i] > ins.min_confidence), 0)
     time = -1 if pred == 0 else 0
+    n = 0
     for t in range(ins.n_timesteps):
         st, b, term = oracle.sr_env_step_belief(1, env, th, c, st, b, ins.seed, th * ins.n_samples + k, t, step)
+        n += 1
         if pred == 0:
             np_ = next((i + 1 for i in range(env.n_models) if b[i] > ins.min_confidence), 0)
             if np_:
                 pred, time = np_, t + 1
         if term:
             break
-    return pred, time
+    return pred, time, n
 
 
 @pytest.mark.parametrize("min_conf,belief", [(2.0, (0.5, 0.5)), (0.95, (0.99, 0.01)), (0.52, (0.5, 0.5)), (0.7, (0.4, 0.6))])
@@ -135,12 +137,12 @@ def test_constant_plan_rollouts_are_plain_rollouts(oracle, pkg, min_conf, belief
     for th in range(2):
         if committed and committed != th + 1:
             assert np.all(np.isinf(r["ctrue"][th]))
-            assert np.all(r["used"][th] == committed) and np.all(r["btime"][th] == 0)
+            assert np.all(r["used"][th] == committed) and np.all(r["btime"][th] == 0) and np.all(r["len"][th] == 0)
             continue
         plain, _ = oracle.sr_rollout(1, env, cost, x0, th, np.tile(c, (T, 1)), 2 * K, T, ins.seed, 7, 0.0, want_noise=False)
         np.testing.assert_allclose(r["ctrue"][th], plain[th * K:(th + 1) * K], rtol=1e-9)
         for k in range(K):
-            assert (r["used"][th, k], r["btime"][th, k]) == replay(oracle, env, ins, x0, th, k, c, 7, belief)
+            assert (r["used"][th, k], r["btime"][th, k], r["len"][th, k]) == replay(oracle, env, ins, x0, th, k, c, 7, belief)
     if min_conf == 0.52:
         assert np.any(r["btime"] >= 1), "some rollouts commit on the way"
 

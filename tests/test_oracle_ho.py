@@ -284,8 +284,9 @@ def replay(oracle, pkg, e, cost, ins, x0, th, actions_of, step, belief, f64=1):
     st, b = x0, np.array(belief, np.float32)
     pred = next((i + 1 for i in range(2) if b[i] > ins.min_confidence), 0)
     time, origin = (-1 if pred == 0 else 0), 0
-    acc, decay = 0.0, 1.0
+    acc, decay, n = 0.0, 1.0, 0
     for t in range(ins.n_timesteps):
+        n += 1
         u = actions_of(pred, t - origin)
         st, b, term = oracle.ho_env_step(f64, e, th, u, st, belief=b, seed=ins.seed, k=th * ins.n_samples + ins._k, t=t, step=step)
         if pred == 0:
@@ -296,7 +297,7 @@ def replay(oracle, pkg, e, cost, ins, x0, th, actions_of, step, belief, f64=1):
         decay *= float(np.float32(0.97))
         if term:
             break
-    return acc + oracle.ho_eval(f64, "final_cost", e, st, th, cost), pred, time
+    return acc + oracle.ho_eval(f64, "final_cost", e, st, th, cost), pred, time, n
 
 
 @pytest.mark.parametrize("start,belief", [((1.0, 4.6), (0.5, 0.5)), ((6.0, 5.4), (0.5, 0.5)), ((6.0, 5.4), (1.0, 0.0))])
@@ -311,14 +312,15 @@ def test_constant_plan_rollouts_replay(oracle, pkg, start, belief):
     committed = next((i + 1 for i in range(2) if belief[i] > 0.95), 0)
     for th in range(2):
         if committed and committed != th + 1:
-            assert np.all(np.isinf(r["ctrue"][th]))
+            assert np.all(np.isinf(r["ctrue"][th])) and np.all(r["len"][th] == 0)
             continue
         for k in range(K):
             ins._k = k
             want = replay(oracle, pkg, e, cost, ins, x0, th, lambda p, t: c, 5, belief)
+            assert r["len"][th, k] == want[3]
             # the replay holds the world in FP32 between steps
             assert r["ctrue"][th, k] == pytest.approx(want[0], rel=1e-6)
-            assert (r["used"][th, k], r["btime"][th, k]) == want[1:]
+            assert (r["used"][th, k], r["btime"][th, k]) == want[1:3]
     if start == (6.0, 5.4) and not committed:
         assert np.all(r["btime"] >= 1), "the bottom obstacle comes into sight on the way"
 
@@ -338,8 +340,9 @@ def test_random_knot_rollouts_replay(oracle, pkg):
             v = knots.astype(np.float64) + r["noise"][:, :, k, :].astype(np.float64)
             ins._k = k
             want = replay(oracle, pkg, e, cost, ins, x0, th, lambda p, t: (B[t] @ v[p]).astype(np.float32), 2, (0.5, 0.5))
-            assert (r["used"][th, k], r["btime"][th, k]) == want[1:]
+            assert (r["used"][th, k], r["btime"][th, k], r["len"][th, k]) == want[1:]
             assert r["ctrue"][th, k] == pytest.approx(want[0], rel=1e-6)
+    assert np.any(r["len"] < T), "some rollouts end early"
     assert 0 < np.mean(r["btime"] > 0) < 1, "some rollouts branch, some do not"
 
 

@@ -302,8 +302,9 @@ def pr_sr_race(pkg, steps=30, warmup=3):
 
 def timed_branching_steps(eng, n=5):
     """n more closed-loop steps, one at a time, with HIP-event timing: the mean device ms of each
-    phase and the mean rollout-steps the rollout launch ran."""
+    phase and the mean rollout-steps the rollout launch ran (the recording rollout is the timed one)."""
     import numpy as np
+    eng.set_record_lengths(True)
     eng.set_timing(True)
     ms = np.zeros(6)
     ran = 0
@@ -312,6 +313,7 @@ def timed_branching_steps(eng, n=5):
         ms += eng.last_timing()
         ran += int(eng.rollout_lengths(0).sum())
     eng.set_timing(False)
+    eng.set_record_lengths(False)
     return ms / n, ran / n
 
 

@@ -277,6 +277,7 @@ SIGNATURES = {
     "dr_bm_get_costs": (_I, [_P, _I, _F, _I]),
     "dr_bm_get_min_costs": (_I, [_P, _I, _F, _I]),
     "dr_bm_get_nu": (_I, [_P, _I, _F, _I]),
+    "dr_bm_set_record_lengths": (_I, [_P, _I]),
     "dr_bm_get_rollout_lengths": (_I, [_P, _I, C.POINTER(C.c_int32), _I]),
     "dr_bm_get_diag": (_I, [_P, C.POINTER(BmDiag), _I]),
     "dr_bm_get_fail_counts": (_I, [_P, _I, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _I]),

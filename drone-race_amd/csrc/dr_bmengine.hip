@@ -156,6 +156,7 @@ struct dr_bm_engine
     // without verification), part 1 = plan choice + world step
     hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     bool timing = false;
+    bool record_lengths = false; // rollout lengths into d_rlen (a separate kernel instantiation)
     std::vector<hipEvent_t> events;
     int timed_steps = 0;
     float last_ms[BM_NEV] = {0, 0, 0, 0, 0, 0};
@@ -186,7 +187,7 @@ int dr_bm_engine::enqueue_a(hipEvent_t* ev)
     if (ev)
         HIPCHK(hipEventRecord(ev[0], stream));
     HIPCHK(launch_bm_rollout(env_kind, max_m, max_tm, 0, stream, d_inst.p, d_dyn.p, d_tiles.p, (int)tiles.size(), d_env.p, cost, d_states.p,
-                             d_knots.p, d_B.p, d_noise.p, d_ctrue.p, d_used.p, d_btime.p, d_rlen.p));
+                             d_knots.p, d_B.p, d_noise.p, d_ctrue.p, d_used.p, d_btime.p, record_lengths ? d_rlen.p : nullptr));
     if (ev)
         HIPCHK(hipEventRecord(ev[1], stream));
     HIPCHK(launch_bm_reduce(n, max_nbk, (int)rtiles.size(), stream, d_inst.p, d_dyn.p, d_rt.p, d_ctrue.p, d_costs.p, d_used.p, d_btime.p,
@@ -787,6 +788,8 @@ int dr_bm_get_rollout_lengths(dr_bm_engine* e, int instance, int32_t* lengths, i
 {
     if (int rc = check_bm(e, instance))
         return rc;
+    if (!e->record_lengths)
+        return host_fail(DR_EINVAL, "get_rollout_lengths: recording is off (dr_bm_set_record_lengths)");
     const BmInstDev& I = e->idev[instance];
     if (!lengths || n != I.NT * I.K)
         return host_fail(DR_EINVAL, "get_rollout_lengths: expected n_models * K values");
@@ -871,6 +874,31 @@ int dr_bm_set_timing(dr_bm_engine* e, int on)
     if (!e)
         return host_fail(DR_EINVAL, "NULL handle");
     e->timing = on != 0;
+    return DR_OK;
+}
+
+int dr_bm_set_record_lengths(dr_bm_engine* e, int on)
+{
+    if (!e)
+        return host_fail(DR_EINVAL, "NULL handle");
+    if (e->record_lengths != (on != 0))
+    {
+        // the captured graphs hold the other rollout instantiation
+        HIPCHK(hipStreamSynchronize(e->stream));
+        for (auto& row : e->graph_exec)
+            for (auto& g : row)
+                if (g)
+                {
+                    (void)hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
+        if (on)
+        {
+            std::vector<int> z(e->d_rlen.n, 0);
+            HIPCHK(hipMemcpy(e->d_rlen.p, z.data(), z.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+    }
+    e->record_lengths = on != 0;
     return DR_OK;
 }
 

@@ -66,9 +66,11 @@ __device__ __forceinline__ bool spline_contributed(int b, int m, int btime, int 
 // when the belief becomes confident (t_origin = t + 1).  With MAXM > 0
 // (and M <= MAXM) the current branch's knot noise is held in registers;
 // with MAXM == 0 it is read back from the noise rows in HBM; the
-// environment policy picks one (P::REG_NOISE), by measurement.
+// environment policy picks one (P::REG_NOISE), by measurement.  With RLEN
+// each lane also records how many environment steps it ran (a separate
+// instantiation: the extra live value slows the loop measurably).
 // The spline basis sits in dynamic LDS (T * M floats of the largest instance).
-template <int MAXM, class P>
+template <int MAXM, bool RLEN, class P>
 __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __restrict__ insts, const BmDyn* __restrict__ dyn,
                                                          const BmTile* __restrict__ tiles, int n_tiles, const typename P::Env* __restrict__ env,
                                                          dr_sr_cost KC, const typename P::Abi* __restrict__ states,
@@ -119,7 +121,8 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
             ctrue[I.ct_off + th * K + k] = __builtin_inff();
             used[I.ct_off + th * K + k] = firstPred;
             btime[I.ct_off + th * K + k] = firstTime;
-            rlen[I.ct_off + th * K + k] = 0;
+            if constexpr (RLEN)
+                rlen[I.ct_off + th * K + k] = 0;
             const typename P::State x0 = P::load(states[bt.inst]);
             if (firstPred == 0 || firstPred == th + 1)
             {
@@ -137,7 +140,7 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                         e[m] = m < M ? knot_noise(key0, key1, (uint32_t)k, pred, m, step, I.sigma) : make_float2(0.0f, 0.0f);
                 }
                 float acc = 0.0f, decay = 1.0f;
-                int executed = T;
+                [[maybe_unused]] int executed = T;
                 for (int t = 0; t < T; ++t)
                 {
                     const float* Brow = s_B + (t - origin) * M;
@@ -184,7 +187,8 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                     decay = decay * DECAY;
                     if (term != SR_NONE)
                     {
-                        executed = t + 1;
+                        if constexpr (RLEN)
+                            executed = t + 1;
                         break;
                     }
                 }
@@ -192,7 +196,8 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                 ctrue[I.ct_off + th * K + k] = acc;
                 used[I.ct_off + th * K + k] = pred;
                 btime[I.ct_off + th * K + k] = time;
-                rlen[I.ct_off + th * K + k] = executed;
+                if constexpr (RLEN)
+                    rlen[I.ct_off + th * K + k] = executed;
             }
         }
     }
@@ -588,7 +593,7 @@ __global__ __launch_bounds__(64) void k_bm_advance(const BmInstDev* __restrict__
 // ---------------------------------------------------------------------------
 // launch wrappers (host)
 
-template <class P>
+template <bool RLEN, class P>
 static void rollout_t(int max_m, int g, size_t lds, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmTile* tiles,
                       int n_tiles, const void* env, const dr_sr_cost& KC, const void* states, const float* knots, const float* basis,
                       float2* noise, float* ctrue, int* used, int* btime, int* rlen)
@@ -599,16 +604,28 @@ static void rollout_t(int max_m, int g, size_t lds, hipStream_t st, const BmInst
     {
         if (max_m <= 20)
         {
-            k_bm_rollout<20, P><<<g, BM_BLOCK, lds, st>>>(insts, dyn, tiles, n_tiles, E, KC, S, knots, basis, noise, ctrue, used, btime,
-                                                          rlen);
+            k_bm_rollout<20, RLEN, P><<<g, BM_BLOCK, lds, st>>>(insts, dyn, tiles, n_tiles, E, KC, S, knots, basis, noise, ctrue, used,
+                                                                btime, rlen);
             return;
         }
     }
-    k_bm_rollout<0, P><<<g, BM_BLOCK, lds, st>>>(insts, dyn, tiles, n_tiles, E, KC, S, knots, basis, noise, ctrue, used, btime, rlen);
+    k_bm_rollout<0, RLEN, P><<<g, BM_BLOCK, lds, st>>>(insts, dyn, tiles, n_tiles, E, KC, S, knots, basis, noise, ctrue, used, btime,
+                                                       rlen);
+}
+
+template <class P>
+static void rollout_p(int max_m, int g, size_t lds, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn, const BmTile* tiles,
+                      int n_tiles, const void* env, const dr_sr_cost& KC, const void* states, const float* knots, const float* basis,
+                      float2* noise, float* ctrue, int* used, int* btime, int* rlen)
+{
+    if (rlen)
+        rollout_t<true, P>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime, rlen);
+    else
+        rollout_t<false, P>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime, rlen);
 }
 
 // env_kind: BM_ENV_SR (env dr_sr_env, states dr_sr_state) or BM_ENV_HO
-// (dr_ho_env, dr_ho_state)
+// (dr_ho_env, dr_ho_state); rlen may be NULL (no rollout lengths recorded)
 hipError_t launch_bm_rollout(int env_kind, int max_m, int max_tm, int grid, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn,
                              const BmTile* tiles, int n_tiles, const void* env, const dr_sr_cost& KC, const void* states,
                              const float* knots, const float* basis, float2* noise, float* ctrue, int* used, int* btime, int* rlen)
@@ -618,9 +635,9 @@ hipError_t launch_bm_rollout(int env_kind, int max_m, int max_tm, int grid, hipS
         return hipErrorInvalidValue;
     const size_t lds = (size_t)max_tm * sizeof(float);
     if (env_kind == BM_ENV_SR)
-        rollout_t<SrPolicy>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime, rlen);
+        rollout_p<SrPolicy>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime, rlen);
     else if (env_kind == BM_ENV_HO)
-        rollout_t<HoPolicy>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime, rlen);
+        rollout_p<HoPolicy>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime, rlen);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

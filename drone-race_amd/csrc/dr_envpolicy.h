@@ -13,9 +13,9 @@ struct SrPolicy
 {
     // the rollout holds its branch's knot noise in registers (true) or reads
     // it back from HBM each step (false); measured on MI355X at K = 2^16,
-    // T = 80, 20 knots: HBM 0.855 ms, registers 0.90 ms per rollout launch
-    // (171 VGPRs, and the lanes that branch regenerate 20 draws divergently)
-    static constexpr bool REG_NOISE = false;
+    // T = 80, 20 knots over 30 closed-loop steps from the start: registers
+    // 1062 control steps/s (645 with plan verification), HBM 1059 (589)
+    static constexpr bool REG_NOISE = true;
     using Env = dr_sr_env;
     using State = sr::St;
     using Abi = dr_sr_state;
@@ -63,7 +63,8 @@ struct SrPolicy
 
 struct HoPolicy
 {
-    // HiddenObs: registers 0.123 ms, HBM 0.144 ms per launch at the same size
+    // HiddenObs: registers 0.123 ms, HBM 0.144 ms per rollout launch at the
+    // same size
     static constexpr bool REG_NOISE = true;
     using Env = dr_ho_env;
     using State = ho::St;

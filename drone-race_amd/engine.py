@@ -503,6 +503,7 @@ class BMEngine:
         self._inst = (abi.BmInstance * n)(*scenario.instances)
         self.n_inst = n
         self.n_models = self._n_models(scenario.env)
+        self.recording = False
         h = C.c_void_p()
         check(getattr(lib, self._create)(C.byref(scenario.env), C.byref(scenario.cost), self._inst, n, C.byref(h)))
         self.h = h
@@ -597,8 +598,13 @@ class BMEngine:
         check(self.lib.dr_bm_get_branches(self.h, i, u.ctypes.data_as(ip), t.ctypes.data_as(ip), u.size))
         return u.reshape(NT, K), t.reshape(NT, K)
 
+    def set_record_lengths(self, on=True):
+        """Record the rollout lengths of the following steps (a slower rollout instantiation)."""
+        check(self.lib.dr_bm_set_record_lengths(self.h, 1 if on else 0))
+        self.recording = bool(on)
+
     def rollout_lengths(self, i):
-        """Environment steps each rollout of the last step ran [n_models][K]."""
+        """Environment steps each rollout of the last step ran [n_models][K] (recording on)."""
         K, T, M, NT = self._dims(i)
         a = np.empty(NT * K, dtype=np.int32)
         check(self.lib.dr_bm_get_rollout_lengths(self.h, i, a.ctypes.data_as(C.POINTER(C.c_int32)), a.size))

@@ -511,8 +511,10 @@ int dr_bm_get_costs(dr_bm_engine* e, int instance, float* costs, int n);
 int dr_bm_get_min_costs(dr_bm_engine* e, int instance, float* mins, int n);
 int dr_bm_get_nu(dr_bm_engine* e, int instance, float* nu, int n);
 /* Environment steps each rollout of the last step ran, per model [n_models][K] (T unless a terminal
- * event ended it; 0 for a model skipped by the commitment).  No counterpart in the reference: it
- * measures the work of a rollout launch. */
+ * event ended it; 0 for a model skipped by the commitment), recorded while dr_bm_set_record_lengths
+ * is on (off by default: recording runs a slower rollout instantiation).  No counterpart in the
+ * reference: it measures the work of a rollout launch. */
+int dr_bm_set_record_lengths(dr_bm_engine* e, int enable);
 int dr_bm_get_rollout_lengths(dr_bm_engine* e, int instance, int32_t* lengths, int n);
 typedef struct dr_bm_diag
 {

@@ -43,6 +43,8 @@ def test_ho_step_parity(pkg, oracle, case):
     else:
         sc = ho_scenario(pkg, [(1000, 40, 10), (2048, 48, 20)])
     eng = pkg.HoBMEngine(sc)
+    if case in ("branching", "noisy"):  # the rollout instantiation that records its lengths
+        eng.set_record_lengths(True)
     rng = np.random.default_rng(17)
     knots = []
     for i, ins in enumerate(sc.instances):
@@ -71,6 +73,7 @@ def test_ho_closed_loop(pkg, oracle):
     knots = [eng.knots(i) for i in range(n)]
     lambdas = [ins.lambda_ for ins in sc.instances]
     for step in range(8):
+        eng.set_record_lengths(step % 3 == 1)
         acts = eng.step(1, advance=True)
         check_step(oracle, eng, sc, states, beliefs, knots, lambdas, acts, step)
         g = eng.get_states()

@@ -68,7 +68,7 @@ __device__ __forceinline__ bool spline_contributed(int b, int m, int btime, int 
 // with MAXM == 0 it is read back from the noise rows in HBM; the
 // environment policy picks one (P::REG_NOISE), by measurement.  With RLEN
 // each lane also records how many environment steps it ran (a separate
-// instantiation: the extra live value slows the loop measurably).
+// instantiation, so the plain loop carries nothing extra).
 // The spline basis sits in dynamic LDS (T * M floats of the largest instance).
 template <int MAXM, bool RLEN, class P>
 __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __restrict__ insts, const BmDyn* __restrict__ dyn,
@@ -140,8 +140,8 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                         e[m] = m < M ? knot_noise(key0, key1, (uint32_t)k, pred, m, step, I.sigma) : make_float2(0.0f, 0.0f);
                 }
                 float acc = 0.0f, decay = 1.0f;
-                [[maybe_unused]] int executed = T;
-                for (int t = 0; t < T; ++t)
+                int t = 0;
+                for (; t < T; ++t)
                 {
                     const float* Brow = s_B + (t - origin) * M;
                     const float* kn = s_kn + pred * M * 2;
@@ -186,18 +186,14 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                     acc = fma_(P::stage_cost(ctx, E, KC, s, th), decay, acc);
                     decay = decay * DECAY;
                     if (term != SR_NONE)
-                    {
-                        if constexpr (RLEN)
-                            executed = t + 1;
                         break;
-                    }
                 }
                 acc = acc + P::final_cost(ctx, E, KC, s);
                 ctrue[I.ct_off + th * K + k] = acc;
                 used[I.ct_off + th * K + k] = pred;
                 btime[I.ct_off + th * K + k] = time;
                 if constexpr (RLEN)
-                    rlen[I.ct_off + th * K + k] = executed;
+                    rlen[I.ct_off + th * K + k] = t < T ? t + 1 : T;
             }
         }
     }

@@ -11,6 +11,9 @@
 //   PRMPPIController::h_nom_nominal, h_rob_nominal, resetNom, useNomPlan controllers.h:128-135
 //   stratrace::MPPIController: the same MPPIController interface over the
 //   reference's own StratRace2D environment (env_stratrace_defs.h:42-96).
+//   stratrace::BranchingMPPIController, hiddenobs::BranchingMPPIController:
+//   the reference's default (branching, spline) MPPIController on
+//   StratRace2D and on HiddenObs (env_hiddenobs_defs.h:48-105).
 //
 // Errors from the C ABI are rethrown as std::runtime_error, as the
 // reference's constructors and checks throw (mppi_controller.cu:27-29).
@@ -343,20 +346,45 @@ struct BranchingMPPIConfig
     float maxVerifEps = 0.01f;
 };
 
-// MPPIController of the StratRace2D build (controllers.h:23-111) over the
-// branching engine (dr_bm_*).  h_belief is the controller's belief, written
-// by the caller's dynStep as in the reference (engine.cpp:332); h_nominal is
-// the executed dense plans of every branch after the step ([nModels + 1][T][2],
-// the chosen branch shifted).  With nVerifSamples > 0 the plan verification
-// runs and failCount (per horizon step, of the executed plan), epsilon,
-// epsilonPartial, certifiedLoss and useNewPlan are its outcome; without it
-// failCount is all zero, the three bounds 0 and useNewPlan true.
-class BranchingMPPIController
+namespace detail {
+
+// The environment-specific calls of the branching engine: StratRace2D
+// (dr_bm_*) and HiddenObs (dr_bmho_*); the rest of the dr_bm_* calls are
+// shared.
+struct BmStratRace
+{
+    using Env = dr_sr_env;
+    using State = dr_sr_state;
+    static int models(const Env& e) { return e.n_models; }
+    static int create(const Env* e, const dr_sr_cost* k, const dr_bm_instance* in, dr_bm_engine** h) { return dr_bm_create(e, k, in, 1, h); }
+    static int set_state(dr_bm_engine* h, const State* s) { return dr_bm_set_states(h, s, 1); }
+};
+
+struct BmHiddenObs
+{
+    using Env = dr_ho_env;
+    using State = dr_ho_state;
+    static int models(const Env&) { return 2; }
+    static int create(const Env* e, const dr_sr_cost* k, const dr_bm_instance* in, dr_bm_engine** h) { return dr_bmho_create(e, k, in, 1, h); }
+    static int set_state(dr_bm_engine* h, const State* s) { return dr_bmho_set_states(h, s, 1); }
+};
+
+// MPPIController of the reference's branching build (controllers.h:23-111,
+// USE_SPLINES) over the branching engine.  h_belief is the controller's
+// belief, written by the caller's dynStep as in the reference
+// (engine.cpp:332); h_nominal is the executed dense plans of every branch
+// after the step ([nModels + 1][T][2], the chosen branch shifted).  With
+// nVerifSamples > 0 the plan verification runs and failCount (per horizon
+// step, of the executed plan), epsilon, epsilonPartial, certifiedLoss and
+// useNewPlan are its outcome; without it failCount is all zero, the three
+// bounds 0 and useNewPlan true.
+template <class E, class Config>
+class BranchingMPPI
 {
   public:
-    BranchingMPPIController(const EnvironmentConfig& c, const BranchingMPPIConfig& mc, int seed, int trueTheta = 0,
-                            std::optional<std::vector<float>> nominal = std::nullopt)
-        : mppiConfig(mc), nModels(c.n_models)
+    BranchingMPPI(const typename E::Env& c, const Config& mc, int seed, int trueTheta = 0,
+                  std::optional<std::vector<float>> nominal = std::nullopt)
+        : mppiConfig(mc), nModels(E::models(c))
     {
         dr_bm_instance in{};
         in.n_samples = mc.nSamples;
@@ -373,15 +401,15 @@ class BranchingMPPIController
         in.min_confidence = mc.minConfidence;
         in.true_theta = trueTheta;
         in.adapt_lambda = 1; // mppi_controller.cu:413-424
-        for (int th = 0; th < c.n_models && th < DR_SR_MAX_MODELS; ++th)
-            in.init_belief[th] = 1.0f / (float)c.n_models;
+        for (int th = 0; th < nModels && th < DR_SR_MAX_MODELS; ++th)
+            in.init_belief[th] = 1.0f / (float)nModels;
         in.n_verif_samples = mc.nVerifSamples;
         in.verif_horizon = mc.verifHorizon;
         in.beta = mc.beta;
         in.max_verif_eps = mc.maxVerifEps;
-        check(dr_bm_create(&c, &mc.cost, &in, 1, &h), "MPPIController");
+        check(E::create(&c, &mc.cost, &in, &h), "MPPIController");
         failCount.assign((size_t)mc.nTimesteps, 0u);
-        const size_t n = (size_t)(nModels + 1) * mc.nTimesteps * ACTION_DIM;
+        const size_t n = (size_t)(nModels + 1) * mc.nTimesteps * 2;
         if (nominal && nominal->size() != n)
         {
             dr_bm_destroy(h);
@@ -394,14 +422,14 @@ class BranchingMPPIController
             h_belief[th] = in.init_belief[th];
     }
 
-    ~BranchingMPPIController() { dr_bm_destroy(h); }
-    BranchingMPPIController(const BranchingMPPIController&) = delete;
-    BranchingMPPIController& operator=(const BranchingMPPIController&) = delete;
+    ~BranchingMPPI() { dr_bm_destroy(h); }
+    BranchingMPPI(const BranchingMPPI&) = delete;
+    BranchingMPPI& operator=(const BranchingMPPI&) = delete;
 
-    void getControl(const SimState& state, float* outAction)
+    void getControl(const typename E::State& state, float* outAction)
     {
         check(dr_bm_set_belief(h, 0, h_belief.data(), nModels), "getControl");
-        check(dr_bm_set_states(h, &state, 1), "getControl");
+        check(E::set_state(h, &state), "getControl");
         check(dr_bm_step(h, 1, 0, outAction), "getControl");
         check(dr_bm_get_nominal(h, 0, h_nominal.data(), (int)h_nominal.size()), "getControl");
         dr_bm_diag d{};
@@ -423,7 +451,7 @@ class BranchingMPPIController
         }
     }
 
-    BranchingMPPIConfig mppiConfig;
+    Config mppiConfig;
     std::vector<float> h_nominal;                    // (nModels + 1, T, 2)
     std::array<float, DR_SR_MAX_MODELS> h_belief{};  // written by the caller's world step
     std::vector<unsigned> failCount;                 // (T) verification failures of the executed plan
@@ -437,6 +465,36 @@ class BranchingMPPIController
     dr_bm_engine* h = nullptr;
 };
 
+} // namespace detail
+
+// The reference's default controller on StratRace2D: branching over the
+// opponent models.
+using BranchingMPPIController = detail::BranchingMPPI<detail::BmStratRace, BranchingMPPIConfig>;
+
 } // namespace stratrace
+
+namespace hiddenobs {
+
+// The reference's SimState / EnvironmentConfig of the HiddenObs build
+// (env_hiddenobs_defs.h:48-105, without the init belief, which the
+// controller holds).
+using SimState = dr_ho_state;
+using EnvironmentConfig = dr_ho_env;
+
+// MPPIConfig of hiddenObsEnv with splines (python-src/main.py:811-833).
+struct BranchingMPPIConfig : stratrace::BranchingMPPIConfig
+{
+    BranchingMPPIConfig()
+    {
+        samplingNoise = 0.8f;
+        cost = dr_sr_cost{1.2f, 10.0f, 1.4f, 1e7f, 1e6f, 0.0f, 500.0f, 0.0f};
+    }
+};
+
+// MPPIController of the HiddenObs build: branching over which hidden
+// obstacle exists (two models).
+using BranchingMPPIController = stratrace::detail::BranchingMPPI<stratrace::detail::BmHiddenObs, BranchingMPPIConfig>;
+
+} // namespace hiddenobs
 
 } // namespace drone_race

@@ -146,6 +146,58 @@ int main(int argc, char** argv)
             std::printf("bm_action %d %.9g %.9g branch %d lambda %.9g use_new %d cert %.17g fails %u\n", i, a[0], a[1], bm.chosenBranch,
                         bm.mppiConfig.invTemperature, (int)bm.useNewPlan, bm.certifiedLoss, fails);
         }
+        // the same controller on HiddenObs (hiddenObsEnv's course)
+        hiddenobs::EnvironmentConfig he{};
+        he.arena_max[0] = 13.0f;
+        he.arena_max[1] = 9.5f;
+        he.dt = 0.1f;
+        he.drone_radius = 0.3f;
+        he.max_speed = 2.0f;
+        he.max_accel = 3.0f;
+        he.n_win_laps = 1;
+        const float gc[3][2] = {{12.0f, 7.0f}, {1.0f, 3.0f}, {5.0f, 7.0f}}, gv[3][2] = {{1, 0}, {0, 1}, {1, 0}};
+        for (int g = 0; g < 3; ++g)
+            for (int d = 0; d < 2; ++d)
+            {
+                he.gate_centers[g][d] = gc[g][d];
+                he.gate_vectors[g][d] = gv[g][d];
+                he.gate_radius[g] = 1.0f;
+            }
+        const float rect[3][4] = {{2, 0, 12, 3}, {5, 3, 12, 4.5f}, {6.5f, 6.5f, 10.5f, 7.5f}};
+        for (int r = 0; r < 3; ++r)
+            for (int j = 0; j < 4; ++j)
+                he.rect_obstacles[r][j] = rect[r][j];
+        he.hidden_centers[0][0] = 9.0f;
+        he.hidden_centers[0][1] = 5.5f;
+        he.hidden_centers[1][0] = 9.0f;
+        he.hidden_centers[1][1] = 8.5f;
+        he.hidden_radius[0] = he.hidden_radius[1] = 0.8f;
+        const float lim[4] = {5, 6.5f, 10.5f, 12}, lam[4] = {3, -11, 20, 6};
+        for (int j = 0; j < 4; ++j)
+        {
+            he.dbl_hp_limits[j] = lim[j];
+            he.dbl_hp_lambdas[j] = lam[j];
+        }
+        he.annulus_center[0] = 5.0f;
+        he.annulus_center[1] = 3.0f;
+        he.annulus_radius[0] = 3.0f;
+        he.annulus_radius[1] = 5.0f;
+        hiddenobs::BranchingMPPIConfig hc;
+        hc.nSamples = 2048;
+        hc.nTimesteps = 40;
+        hc.nKnots = 10;
+        hiddenobs::BranchingMPPIController ho(he, hc, 13, 0);
+        hiddenobs::SimState hs{};
+        hs.pos[0] = 1.0f;
+        hs.pos[1] = 4.0f;
+        hs.vel[0] = 0.3f;
+        hs.vel[1] = 1.0f;
+        for (int i = 0; i < steps; ++i)
+        {
+            float a[2];
+            ho.getControl(hs, a);
+            std::printf("ho_action %d %.9g %.9g branch %d lambda %.9g\n", i, a[0], a[1], ho.chosenBranch, ho.mppiConfig.invTemperature);
+        }
     }
     catch (const std::exception& e)
     {

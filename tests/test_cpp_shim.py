@@ -125,4 +125,21 @@ def test_shim_matches_python_handle(pkg):
     assert np.array_equal(bm_acts, np.stack(py_bm))
     assert [int(l[5]) for l in bm_lines] == py_branch
     assert [(int(l[9]), float(l[11]), int(l[13])) for l in bm_lines] == py_verif
-
+    # the same controller on HiddenObs
+    ho_lines = [l.split() for l in res if l.startswith("ho_action")]
+    ho_acts = np.array([[float(x) for x in l[2:4]] for l in ho_lines], dtype=np.float32)
+    he = sc.ho_env()
+    hs = abi.HoState()
+    hs.pos[:] = [1.0, 4.0]
+    hs.vel[:] = [0.3, 1.0]
+    hins = sc.bm_instance(2048, 40, 13, n_knots=10, lam=10.0, sigma=0.8, min_confidence=0.95, true_theta=0, adapt=1)
+    heng = pkg.HoBMEngine(sc.SrScenario("shim_ho", he, sc.ho_cost(), [hins], [hs], 1, ""))
+    py_ho, py_hb = [], []
+    for _ in range(3):
+        heng.set_belief(0, np.float32([0.5, 0.5]))
+        heng.set_states([hs])
+        py_ho.append(heng.step(1, advance=False)[0])
+        py_hb.append(heng.diag()[0].branch)
+    heng.close()
+    assert np.array_equal(ho_acts, np.stack(py_ho))
+    assert [int(l[5]) for l in ho_lines] == py_hb

@@ -11,6 +11,6 @@ CPU fallback.
 """
 from . import abi, philox, scenarios, wire
 from .abi import EngineError, load
-from .engine import BMEngine, Engine, HoBMEngine, MPPI, PREngine, PRSREngine, SREngine
+from .engine import BMEngine, DraceBMEngine, Engine, HoBMEngine, MPPI, PREngine, PRSREngine, SREngine
 
-__all__ = ["abi", "philox", "scenarios", "wire", "BMEngine", "Engine", "HoBMEngine", "MPPI", "PREngine", "PRSREngine", "SREngine", "EngineError", "load"]
+__all__ = ["abi", "philox", "scenarios", "wire", "BMEngine", "DraceBMEngine", "Engine", "HoBMEngine", "MPPI", "PREngine", "PRSREngine", "SREngine", "EngineError", "load"]

@@ -154,6 +154,31 @@ class HoState(C.Structure):
     _fields_ = [("pos", C.c_float * 2), ("vel", C.c_float * 2), ("laps", C.c_int32), ("gates", C.c_int32)]
 
 
+DRACE_SAMPLES, DRACE_MAX_GATES, DRACE_MAX_OBSTACLES, DRACE_MAX_RACELINES = 512, 8, 4, 4
+
+
+class DracePid(C.Structure):
+    _fields_ = [("kp", C.c_float), ("kd", C.c_float), ("repulsion_factor", C.c_float), ("repulsion_power", C.c_float),
+                ("repulsion_dist_fact", C.c_float), ("raceline_index", C.c_int32), ("action_noise", C.c_float)]
+
+
+class DraceEnv(C.Structure):
+    _fields_ = [("arena_min", C.c_float * 2), ("arena_max", C.c_float * 2), ("dt", C.c_float), ("drone_radius", C.c_float),
+                ("pos_noise", C.c_float), ("speed_noise", C.c_float), ("action_noise", C.c_float), ("max_speed", C.c_float * 2),
+                ("max_accel", C.c_float * 2), ("n_win_laps", C.c_int32), ("target_distance", C.c_float), ("n_gates", C.c_int32),
+                ("gate_centers", (C.c_float * 2) * DRACE_MAX_GATES), ("gate_vectors", (C.c_float * 2) * DRACE_MAX_GATES),
+                ("gate_radius", C.c_float * DRACE_MAX_GATES), ("n_obstacles", C.c_int32),
+                ("obstacles", (C.c_float * 4) * DRACE_MAX_OBSTACLES), ("n_round_obstacles", C.c_int32),
+                ("round_centers", (C.c_float * 2) * DRACE_MAX_OBSTACLES), ("round_radius", C.c_float * DRACE_MAX_OBSTACLES),
+                ("n_racelines", C.c_int32), ("n_models", C.c_int32), ("opp_pid", DracePid * SR_MAX_MODELS), ("i_mppi", C.c_int32),
+                ("track_points", ((C.c_float * 2) * DRACE_SAMPLES) * DRACE_MAX_RACELINES)]
+
+
+class DraceState(C.Structure):
+    _fields_ = [("pos", C.c_float * 4), ("vel", C.c_float * 4), ("s", (C.c_float * DRACE_MAX_RACELINES) * 2), ("laps", C.c_int32 * 2),
+                ("gates", C.c_int32 * 2)]
+
+
 MSG_HEADER, MSG_STATE, MSG_EVENT, MSG_DONE = 0, 1, 2, 3
 EVT_OUTSIDE, EVT_WINNER, EVT_OPP_WINNER, EVT_TRUNCATED = 0, 1, 2, 3
 CONT_MPPI, CONT_PRMPPI = 0, 1
@@ -286,6 +311,9 @@ SIGNATURES = {
     "dr_bmho_create": (_I, [C.POINTER(HoEnv), C.POINTER(SrCost), C.POINTER(BmInstance), _I, C.POINTER(_P)]),
     "dr_bmho_set_states": (_I, [_P, C.POINTER(HoState), _I]),
     "dr_bmho_get_states": (_I, [_P, C.POINTER(HoState), _I]),
+    "dr_bmdrace_create": (_I, [C.POINTER(DraceEnv), C.POINTER(SrCost), C.POINTER(BmInstance), _I, C.POINTER(_P)]),
+    "dr_bmdrace_set_states": (_I, [_P, C.POINTER(DraceState), _I]),
+    "dr_bmdrace_get_states": (_I, [_P, C.POINTER(DraceState), _I]),
 }
 
 _lib = None

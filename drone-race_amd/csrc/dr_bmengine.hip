@@ -1,4 +1,4 @@
-// Host side of the branching spline MPPI engine on StratRace2D and HiddenObs: the natural
+// Host side of the branching spline MPPI engine on StratRace2D, HiddenObs and DroneRace: the natural
 // cubic spline basis, descriptor tables, device buffers, the per-step
 // launch sequence, hipGraph capture and the dr_bm_* C ABI.
 //
@@ -371,6 +371,59 @@ int check_ho_env(const dr_ho_env& env)
     return DR_OK;
 }
 
+// DroneRace configuration: counts within the compiled maxima, finite
+// values, positive step, radius and limits, nonnegative noise levels,
+// racelines of the opponent models that exist, consecutive gates apart.
+int check_drace_env(const dr_drace_env& e)
+{
+    if (e.n_gates < 2 || e.n_gates > DR_DRACE_MAX_GATES || e.n_obstacles < 0 || e.n_obstacles > DR_DRACE_MAX_OBSTACLES ||
+        e.n_round_obstacles < 0 || e.n_round_obstacles > DR_DRACE_MAX_OBSTACLES || e.n_racelines < 1 || e.n_racelines > DR_DRACE_MAX_RACELINES ||
+        e.n_models < 1 || e.n_models > DR_SR_MAX_MODELS || (e.i_mppi != 0 && e.i_mppi != 1))
+        return host_fail(DR_EINVAL, "DroneRace counts out of range (gates 2..8, obstacles 0..4, racelines 1..4, models 1..4, i_mppi 0/1)");
+    const float* f = &e.arena_min[0];
+    for (; f < reinterpret_cast<const float*>(&e.n_win_laps); ++f)
+        if (!std::isfinite(*f))
+            return host_fail(DR_EINVAL, "DroneRace configuration values must be finite");
+    if (!(e.dt > 0.0f) || !(e.drone_radius > 0.0f) || e.pos_noise < 0.0f || e.speed_noise < 0.0f || e.action_noise < 0.0f)
+        return host_fail(DR_EINVAL, "dt and drone_radius must be > 0, noise levels >= 0");
+    for (int a = 0; a < 2; ++a)
+        if (!(e.max_speed[a] > 0.0f) || !(e.max_accel[a] > 0.0f))
+            return host_fail(DR_EINVAL, "max_speed and max_accel must be > 0");
+    if (e.n_win_laps < 1 || !std::isfinite(e.target_distance))
+        return host_fail(DR_EINVAL, "n_win_laps must be >= 1 and target_distance finite");
+    for (int g = 0; g < e.n_gates; ++g)
+    {
+        const float* a = e.gate_centers[g];
+        const float* b = e.gate_centers[(g + 1) % e.n_gates];
+        if (!std::isfinite(a[0]) || !std::isfinite(a[1]) || !std::isfinite(e.gate_vectors[g][0]) || !std::isfinite(e.gate_vectors[g][1]) ||
+            !std::isfinite(e.gate_radius[g]))
+            return host_fail(DR_EINVAL, "gates must be finite");
+        if (a[0] == b[0] && a[1] == b[1])
+            return host_fail(DR_EINVAL, "consecutive gate centres must differ");
+    }
+    for (int o = 0; o < e.n_obstacles; ++o)
+        for (int j = 0; j < 4; ++j)
+            if (!std::isfinite(e.obstacles[o][j]))
+                return host_fail(DR_EINVAL, "obstacles must be finite");
+    for (int o = 0; o < e.n_round_obstacles; ++o)
+        if (!std::isfinite(e.round_centers[o][0]) || !std::isfinite(e.round_centers[o][1]) || !std::isfinite(e.round_radius[o]))
+            return host_fail(DR_EINVAL, "round obstacles must be finite");
+    for (int m = 0; m < e.n_models; ++m)
+    {
+        const dr_drace_pid& p = e.opp_pid[m];
+        if (p.raceline_index < 0 || p.raceline_index >= e.n_racelines)
+            return host_fail(DR_EINVAL, "opponent model raceline_index must name one of the racelines");
+        if (!std::isfinite(p.kp) || !std::isfinite(p.kd) || !std::isfinite(p.repulsion_factor) || !std::isfinite(p.repulsion_power) ||
+            !std::isfinite(p.repulsion_dist_fact) || !(p.action_noise >= 0.0f) || !std::isfinite(p.action_noise))
+            return host_fail(DR_EINVAL, "opponent PD configurations must be finite, action_noise >= 0");
+    }
+    for (int r = 0; r < e.n_racelines; ++r)
+        for (int i = 0; i < DR_DRACE_SAMPLES; ++i)
+            if (!std::isfinite(e.track_points[r][i][0]) || !std::isfinite(e.track_points[r][i][1]))
+                return host_fail(DR_EINVAL, "track points must be finite");
+    return DR_OK;
+}
+
 } // namespace dr
 
 namespace {
@@ -639,6 +692,19 @@ static int bm_get_states(dr_bm_engine* e, int kind, void* states, int n)
     return bm_out(e, static_cast<uint8_t*>(states), e->d_states.p, (size_t)n * e->state_size);
 }
 
+int dr_bmdrace_create(const dr_drace_env* env, const dr_sr_cost* cost, const dr_bm_instance* instances, int n_instances, dr_bm_engine** out)
+{
+    if (!out)
+        return host_fail(DR_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!env || !cost || !instances)
+        return host_fail(DR_EINVAL, "NULL argument");
+    if (int rc = check_drace_env(*env))
+        return rc;
+    return bm_create(BM_ENV_DRACE, env, sizeof(dr_drace_env), sizeof(dr_drace_state), env->n_models, env->max_accel[env->i_mppi], cost,
+                     instances, n_instances, out);
+}
+
 int dr_bm_set_states(dr_bm_engine* e, const dr_sr_state* states, int n) { return bm_set_states(e, BM_ENV_SR, states, n); }
 
 int dr_bm_get_states(dr_bm_engine* e, dr_sr_state* states, int n) { return bm_get_states(e, BM_ENV_SR, states, n); }
@@ -646,6 +712,10 @@ int dr_bm_get_states(dr_bm_engine* e, dr_sr_state* states, int n) { return bm_ge
 int dr_bmho_set_states(dr_bm_engine* e, const dr_ho_state* states, int n) { return bm_set_states(e, BM_ENV_HO, states, n); }
 
 int dr_bmho_get_states(dr_bm_engine* e, dr_ho_state* states, int n) { return bm_get_states(e, BM_ENV_HO, states, n); }
+
+int dr_bmdrace_set_states(dr_bm_engine* e, const dr_drace_state* states, int n) { return bm_set_states(e, BM_ENV_DRACE, states, n); }
+
+int dr_bmdrace_get_states(dr_bm_engine* e, dr_drace_state* states, int n) { return bm_get_states(e, BM_ENV_DRACE, states, n); }
 
 int dr_bm_set_belief(dr_bm_engine* e, int instance, const float* belief, int n_models)
 {

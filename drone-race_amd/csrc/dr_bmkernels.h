@@ -31,6 +31,7 @@ constexpr int BM_CHUNK = 4096;       // samples per weighted-average workgroup
 constexpr int BM_VBLOCK = 128;       // verification lanes per workgroup
 constexpr int BM_ENV_SR = 0;         // StratRace2D (dr_sr_env, dr_sr_state)
 constexpr int BM_ENV_HO = 1;         // HiddenObs (dr_ho_env, dr_ho_state)
+constexpr int BM_ENV_DRACE = 2;      // DroneRace (dr_drace_env, dr_drace_state)
 
 struct BmInstDev
 {

@@ -620,8 +620,9 @@ static void rollout_p(int max_m, int g, size_t lds, hipStream_t st, const BmInst
         rollout_t<false, P>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime, rlen);
 }
 
-// env_kind: BM_ENV_SR (env dr_sr_env, states dr_sr_state) or BM_ENV_HO
-// (dr_ho_env, dr_ho_state); rlen may be NULL (no rollout lengths recorded)
+// env_kind: BM_ENV_SR (env dr_sr_env, states dr_sr_state), BM_ENV_HO
+// (dr_ho_env, dr_ho_state) or BM_ENV_DRACE (dr_drace_env, dr_drace_state);
+// rlen may be NULL (no rollout lengths recorded)
 hipError_t launch_bm_rollout(int env_kind, int max_m, int max_tm, int grid, hipStream_t st, const BmInstDev* insts, const BmDyn* dyn,
                              const BmTile* tiles, int n_tiles, const void* env, const dr_sr_cost& KC, const void* states,
                              const float* knots, const float* basis, float2* noise, float* ctrue, int* used, int* btime, int* rlen)
@@ -634,6 +635,9 @@ hipError_t launch_bm_rollout(int env_kind, int max_m, int max_tm, int grid, hipS
         rollout_p<SrPolicy>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime, rlen);
     else if (env_kind == BM_ENV_HO)
         rollout_p<HoPolicy>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime, rlen);
+    else if (env_kind == BM_ENV_DRACE)
+        rollout_p<DracePolicy>(max_m, g, lds, st, insts, dyn, tiles, n_tiles, env, KC, states, knots, basis, noise, ctrue, used, btime,
+                               rlen);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -673,6 +677,9 @@ hipError_t launch_bm_verify(int env_kind, int n_tiles, hipStream_t st, const BmI
     else if (env_kind == BM_ENV_HO)
         k_bm_verify<HoPolicy><<<n_tiles, BM_VBLOCK, 0, st>>>(insts, dyn, tiles, n_tiles, static_cast<const dr_ho_env*>(env),
                                                              static_cast<const dr_ho_state*>(states), dnew, dense, fail);
+    else if (env_kind == BM_ENV_DRACE)
+        k_bm_verify<DracePolicy><<<n_tiles, BM_VBLOCK, 0, st>>>(insts, dyn, tiles, n_tiles, static_cast<const dr_drace_env*>(env),
+                                                                static_cast<const dr_drace_state*>(states), dnew, dense, fail);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -695,6 +702,9 @@ hipError_t launch_bm_advance(int env_kind, int n_inst, hipStream_t st, const BmI
     else if (env_kind == BM_ENV_HO)
         k_bm_advance<HoPolicy><<<g, 64, 0, st>>>(insts, dyn, static_cast<const dr_ho_env*>(env), static_cast<dr_ho_state*>(states), actions,
                                                  events, n_inst);
+    else if (env_kind == BM_ENV_DRACE)
+        k_bm_advance<DracePolicy><<<g, 64, 0, st>>>(insts, dyn, static_cast<const dr_drace_env*>(env), static_cast<dr_drace_state*>(states),
+                                                    actions, events, n_inst);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

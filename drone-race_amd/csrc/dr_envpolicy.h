@@ -1,9 +1,11 @@
 // Environment policies of the branching MPPI kernels: what a rollout lane
 // needs from an environment (its configuration, state, LDS staging, one
 // step with belief update, the running and terminal costs), so one set of
-// kernels serves StratRace2D (dr_sr.h) and HiddenObs (dr_ho.h).
+// kernels serves StratRace2D (dr_sr.h), HiddenObs (dr_ho.h) and DroneRace
+// (dr_drace.h).
 #pragma once
 
+#include "dr_drace.h"
 #include "dr_ho.h"
 #include "dr_sr.h"
 
@@ -88,6 +90,42 @@ struct HoPolicy
     DR_HD static float final_cost(const Ctx&, const Env& E, const dr_sr_cost& K, const State& s) { return ho::final_cost(E, K, s); }
     DR_HD static State load(const Abi& a) { return ho::state_from_abi(a); }
     DR_HD static void store(const State& s, Abi& a) { ho::state_to_abi(s, a); }
+};
+
+struct DracePolicy
+{
+    static constexpr bool REG_NOISE = true;
+    using Env = dr_drace_env;
+    using State = drace::St;
+    using Abi = dr_drace_state;
+    // racelines (up to 4 x 512 x 2) and the belief constants
+    static constexpr int LDS_FLOATS = DR_DRACE_MAX_RACELINES * DR_DRACE_SAMPLES * 2 + 2 * DR_SR_MAX_MODELS;
+    struct Ctx
+    {
+        drace::Lines L;
+        const float* bc;
+    };
+    DR_HD static Ctx stage(float* sh, const Env& E, int tid, int nthr)
+    {
+        const int n = E.n_racelines * DR_DRACE_SAMPLES * 2;
+        for (int i = tid; i < n; i += nthr)
+            sh[i] = (&E.track_points[0][0][0])[i];
+        float* bc = sh + DR_DRACE_MAX_RACELINES * DR_DRACE_SAMPLES * 2;
+        if (tid < E.n_models)
+            drace::belief_consts(E, tid, &bc[tid], &bc[DR_SR_MAX_MODELS + tid]);
+        return Ctx{drace::Lines{sh}, bc};
+    }
+    DR_HD static Ctx direct(const Env& E) { return Ctx{drace::Lines{&E.track_points[0][0][0]}, nullptr}; }
+    DR_HD static int n_models(const Env& E) { return E.n_models; }
+    DR_HD static int step(const Ctx& c, const Env& E, int theta, const float ego[2], State& s, sr::NormalSeq& rng, bool lose_on_opp_win,
+                          float* belief, bool update)
+    {
+        return drace::env_step_t<true>(c.L, E, theta, ego, s, rng, lose_on_opp_win, belief, c.bc, update);
+    }
+    DR_HD static float stage_cost(const Ctx&, const Env& E, const dr_sr_cost& K, const State& s, int) { return drace::state_cost(E, K, s); }
+    DR_HD static float final_cost(const Ctx&, const Env& E, const dr_sr_cost& K, const State& s) { return drace::final_cost(E, K, s); }
+    DR_HD static State load(const Abi& a) { return drace::state_from_abi(a); }
+    DR_HD static void store(const State& s, Abi& a) { drace::state_to_abi(s, a); }
 };
 
 } // namespace dr

@@ -20,6 +20,7 @@ int check_track(const dr_track& t, int idx);
 // DR_OK, or DR_EINVAL naming the first invalid field of a StratRace2D environment.
 int check_sr_env(const dr_sr_env& env);
 int check_ho_env(const dr_ho_env& env);
+int check_drace_env(const dr_drace_env& env);
 
 template <class T>
 struct DevBuf

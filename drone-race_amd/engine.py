@@ -661,3 +661,10 @@ class HoBMEngine(BMEngine):
     @staticmethod
     def _n_models(env):
         return 2
+
+
+class DraceBMEngine(BMEngine):
+    """Branching spline MPPI controllers on DroneRace worlds (model theta: the opponent's PD
+    configuration); the same engine and calls, with DroneRace states."""
+
+    _create, _set, _get, _state = "dr_bmdrace_create", "dr_bmdrace_set_states", "dr_bmdrace_get_states", abi.DraceState

@@ -485,6 +485,94 @@ def ho_race(n_inst=1, n_samples=65536, n_timesteps=80, n_knots=20, seed=2, steps
                       f"branching spline MPPI on HiddenObs, {n_inst} worlds, K={n_samples}, T={n_timesteps}, {n_knots} knots")
 
 
+# ------------------------------------------------------------------ DroneRace
+
+def drace_env(round_obstacle=True, action_noise=0.1, pid_noise=1.0, i_mppi=0, repulsion=0.0):
+    """The DroneRace course of tinyGateEnv (python-src/main.py:40-161): a 25 m straight with
+    two gates ((25, 0) and (0.25, 0), normal +x, radius 1), one round obstacle of radius 0.25
+    at its middle (or a 5 m x 1 m box), and two racelines passing above and below it
+    (y = +-1.5 sin^2 ramps); the opponent models are PD controllers (kp 5, kd 20, action
+    noise 1) on raceline 0 ("Top") and 1 ("Bottom")."""
+    length = 25.0 if round_obstacle else 30.0
+    height, obs_size, race_delay, margin, height_factor = 0.5, 0.2, 0.1, 0.05, 3.0
+    e = abi.DraceEnv()
+    e.arena_min[:] = [-0.1 * length, -2 * height * height_factor]
+    e.arena_max[:] = [1.1 * length, 2 * height * height_factor]
+    e.dt, e.drone_radius, e.action_noise = 0.1, 0.75, action_noise
+    e.max_speed[:] = [2.05, 2.5]
+    e.max_accel[:] = [3.0, 3.0]
+    e.n_win_laps, e.target_distance, e.n_gates = 1, 0.05, 2
+    e.gate_centers[0][:] = [length, 0.0]
+    e.gate_centers[1][:] = [length * 0.01, 0.0]
+    for g in range(2):
+        e.gate_vectors[g][:] = [1.0, 0.0]
+        e.gate_radius[g] = 1.0
+    if round_obstacle:
+        e.n_round_obstacles = 1
+        e.round_centers[0][:] = [length / 2, 0.0]
+        e.round_radius[0] = height * 0.5
+    else:
+        e.n_obstacles = 1
+        e.obstacles[0][:] = [length * (0.5 - obs_size / 2), -height, length * (0.5 + obs_size / 2), height]
+    e.n_racelines, e.n_models, e.i_mppi = 2, 2, i_mppi
+    for m in range(2):
+        e.opp_pid[m] = abi.DracePid(kp=5.0, kd=20.0, repulsion_factor=repulsion, repulsion_power=2.0, repulsion_dist_fact=5.0,
+                                    raceline_index=m, action_noise=pid_noise)
+
+    def lift(x):
+        xn = x / length
+        if xn > 1.0:
+            return 0.0
+        if xn > 0.5:
+            xn = 1 - xn
+        if xn < 0.5 - obs_size / 2 - race_delay:
+            return 0.0
+        if xn < 0.5 - obs_size / 2.0 - margin:
+            return np.sin(np.pi / 2.0 * (xn - (0.5 - obs_size / 2 - race_delay)) / (race_delay - margin)) ** 2
+        return 1.0
+
+    xs = np.linspace(0.0, length * 1.1, abi.DRACE_SAMPLES)
+    ys = np.array([lift(x) for x in xs]) * height * height_factor
+    for i in range(abi.DRACE_SAMPLES):
+        e.track_points[0][i][:] = [xs[i], ys[i]]
+        e.track_points[1][i][:] = [xs[i], -ys[i]]
+    return e
+
+
+def drace_start_state(env, start=(0.1, 0.02)):
+    """Both agents on raceline 0 at samples int(512 s), at rest, progress s on both racelines,
+    next gate 0 (main.py:132-138)."""
+    st = abi.DraceState()
+    for a, s0 in enumerate(start):
+        i = int(s0 * abi.DRACE_SAMPLES)
+        st.pos[2 * a], st.pos[2 * a + 1] = env.track_points[0][i][0], env.track_points[0][i][1]
+        for r in range(abi.DRACE_MAX_RACELINES):
+            st.s[a][r] = s0 if r < env.n_racelines else 0.0
+        st.gates[a] = 1
+    return st
+
+
+def drace_cost():
+    """The spline MPPIConfig of tinyGateEnv (main.py:140-161): collision factor 1.1, boundary
+    cost 0 (factor 1.7), outside 1e6, win 1e6, opponent win 1000 (MPPIConfig's default),
+    terminal advance weight 200."""
+    return abi.SrCost(coll_dist_factor=1.1, boundary_cost=0.0, boundary_threshold_factor=1.7, outside_cost=1e6, win_cost=1e6,
+                      opp_win_cost=1000.0, final_adv_weight=200.0, final_opp_adv_weight=0.0)
+
+
+def drace_race(n_inst=1, n_samples=65536, n_timesteps=60, n_knots=10, seed=42, steps=300, n_verif=0, **env_kw):
+    """The reference's branching spline MPPI on DroneRace (tinyGateEnv with splines): K = 2^16,
+    T = 60, 10 knots, temperature 10 (adapted), sigma 2, confidence 0.95, uniform belief over the
+    opponent's two PD models; world i's opponent follows model i % 2."""
+    env = drace_env(**env_kw)
+    inst = [bm_instance(n_samples, n_timesteps, seed_mix(seed, 900 + i), n_knots=n_knots, lam=10.0, sigma=2.0, true_theta=i % 2,
+                        belief=(0.5, 0.5), n_verif=n_verif) for i in range(n_inst)]
+    st = [drace_start_state(env) for _ in range(n_inst)]
+    tag = f"_v{n_verif}" if n_verif else ""
+    return SrScenario(f"drace_race_{n_inst}x_k{n_samples}_t{n_timesteps}_m{n_knots}{tag}", env, drace_cost(), inst, st, steps,
+                      f"branching spline MPPI on DroneRace, {n_inst} worlds, K={n_samples}, T={n_timesteps}, {n_knots} knots")
+
+
 def seed_mix(seed, i):
     """64-bit Philox key for instance i of a scenario seeded `seed`."""
     return ((int(seed) * 0x9E3779B97F4A7C15) ^ (0xA5A5A5A5 + 0x632BE59BD9B4E019 * (i + 1))) & 0xFFFFFFFFFFFFFFFF

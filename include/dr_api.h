@@ -585,6 +585,75 @@ int dr_bmho_set_states(dr_bm_engine* e, const dr_ho_state* states, int n);
 int dr_bmho_get_states(dr_bm_engine* e, dr_ho_state* states, int n);
 
 /* ---------------------------------------------------------------------
+ * DroneRace: the reference's third 2-D environment (include/environments/
+ * env_dronerace_defs.h:45-112, env_dronerace.h): two agents race through
+ * gates; the opponent is a PD controller (PIDConfig) following one of
+ * several racelines sampled at 512 points, model theta is its PD
+ * configuration; the belief is updated from the opponent's applied action.
+ * The reference fixes the counts at compile time (N_GATES, N_OBSTACLES,
+ * N_ROUND_OBSTACLES, N_RACELINES, N_TRUE_MODELS); here they are fields with
+ * the maxima below.  The branching spline MPPI engine runs on it
+ * (dr_bmdrace_create; every other dr_bm_* call applies). */
+#define DR_DRACE_SAMPLES 512
+#define DR_DRACE_MAX_GATES 8
+#define DR_DRACE_MAX_OBSTACLES 4
+#define DR_DRACE_MAX_RACELINES 4
+
+typedef struct dr_drace_pid           /* PIDConfig, env_dronerace_defs.h:45-58 */
+{
+    float kp;
+    float kd;
+    float repulsion_factor;
+    float repulsion_power;
+    float repulsion_dist_fact;
+    int32_t raceline_index;
+    float action_noise;
+} dr_drace_pid;
+
+typedef struct dr_drace_env           /* EnvironmentConfig, env_dronerace_defs.h:61-97 */
+{
+    float arena_min[2];
+    float arena_max[2];
+    float dt;
+    float drone_radius;
+    float pos_noise;
+    float speed_noise;
+    float action_noise;
+    float max_speed[2];                /* per agent, L2 */
+    float max_accel[2];                /* per agent, L2 */
+    int32_t n_win_laps;
+    float target_distance;             /* PD target at S + targetDistance */
+    int32_t n_gates;                   /* 2..DR_DRACE_MAX_GATES */
+    float gate_centers[DR_DRACE_MAX_GATES][2];
+    float gate_vectors[DR_DRACE_MAX_GATES][2];
+    float gate_radius[DR_DRACE_MAX_GATES];
+    int32_t n_obstacles;               /* rectangles, 0..4 */
+    float obstacles[DR_DRACE_MAX_OBSTACLES][4];  /* xmin, ymin, xmax, ymax */
+    int32_t n_round_obstacles;         /* 0..4 */
+    float round_centers[DR_DRACE_MAX_OBSTACLES][2];
+    float round_radius[DR_DRACE_MAX_OBSTACLES];
+    int32_t n_racelines;               /* 1..DR_DRACE_MAX_RACELINES */
+    int32_t n_models;                  /* N_TRUE_MODELS, 1..DR_SR_MAX_MODELS */
+    dr_drace_pid opp_pid[DR_SR_MAX_MODELS];
+    int32_t i_mppi;                    /* the controlled agent (0 or 1) */
+    float track_points[DR_DRACE_MAX_RACELINES][DR_DRACE_SAMPLES][2];
+} dr_drace_env;
+
+typedef struct dr_drace_state         /* SimState, env_dronerace_defs.h:99-107 */
+{
+    float pos[4];                      /* agent-major (x, y) */
+    float vel[4];
+    float s[2][DR_DRACE_MAX_RACELINES];/* progress on each raceline in [0, 1); < 0: not tracked */
+    int32_t laps[2];
+    int32_t gates[2];
+} dr_drace_state;
+
+int dr_bmdrace_create(const dr_drace_env* env, const dr_sr_cost* cost, const dr_bm_instance* instances, int n_instances,
+                      dr_bm_engine** out);
+int dr_bmdrace_set_states(dr_bm_engine* e, const dr_drace_state* states, int n);
+int dr_bmdrace_get_states(dr_bm_engine* e, dr_drace_state* states, int n);
+
+/* ---------------------------------------------------------------------
  * Wire protocol and simulation session of the reference's front-end for the
  * StratRace2D build: the byte codec (include/protocol.h:9-142,
  * src/protocol.cpp, python-src/protocol.py:13-116), the header decoders

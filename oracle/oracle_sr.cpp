@@ -1224,6 +1224,489 @@ HoEnv<S> make_env(const dr_ho_env& e)
     return HoEnv<S>(e);
 }
 
+// ------------------------------------------------------------ DroneRace
+// env_dronerace.h restated on the scalar S (configuration values rounded to
+// float first, as the engine holds them).
+
+template <class S>
+struct DrEnv
+{
+    const dr_drace_env& e;
+    explicit DrEnv(const dr_drace_env& env) : e(env) {}
+    S f(float v) const { return (S)v; }
+    const float* line(int r) const { return &e.track_points[r][0][0]; }
+};
+
+template <class S>
+struct DState
+{
+    S pos[4], vel[4], sp[2][DR_DRACE_MAX_RACELINES];
+    int laps[2], gates[2];
+};
+
+template <class S>
+DState<S> dr_load(const dr_drace_state& a)
+{
+    DState<S> s;
+    for (int i = 0; i < 4; ++i)
+    {
+        s.pos[i] = (S)a.pos[i];
+        s.vel[i] = (S)a.vel[i];
+    }
+    for (int g = 0; g < 2; ++g)
+    {
+        for (int r = 0; r < DR_DRACE_MAX_RACELINES; ++r)
+            s.sp[g][r] = (S)a.s[g][r];
+        s.laps[g] = a.laps[g];
+        s.gates[g] = a.gates[g];
+    }
+    return s;
+}
+
+template <class S>
+void dr_store(const DState<S>& s, dr_drace_state& a)
+{
+    for (int i = 0; i < 4; ++i)
+    {
+        a.pos[i] = (float)s.pos[i];
+        a.vel[i] = (float)s.vel[i];
+    }
+    for (int g = 0; g < 2; ++g)
+    {
+        for (int r = 0; r < DR_DRACE_MAX_RACELINES; ++r)
+            a.s[g][r] = (float)s.sp[g][r];
+        a.laps[g] = s.laps[g];
+        a.gates[g] = s.gates[g];
+    }
+}
+
+// sqDistToSample (env_dronerace.h:96-102)
+template <class S>
+S dr_sq(const float* tp, int i, const S* pos)
+{
+    S sq = 0;
+    for (int d = 0; d < 2; ++d)
+    {
+        const S dx = (S)tp[i * 2 + d] - pos[d];
+        sq = sq + dx * dx;
+    }
+    return sq;
+}
+
+// projectOnTrack (:69-93): first sample of least distance, s = i / 512
+template <class S>
+S dr_project_full(const float* tp, const S* pos)
+{
+    S bestS = 0, bestD2 = (S)3.402823466e+38f;
+    for (int i = 0; i < DR_DRACE_SAMPLES; ++i)
+    {
+        const S d2 = dr_sq(tp, i, pos);
+        if (d2 < bestD2)
+        {
+            bestD2 = d2;
+            bestS = (S)i * ((S)1 / (S)DR_DRACE_SAMPLES);
+        }
+    }
+    return bestS;
+}
+
+// findMinAlongDirection (:105-139): greedy walk, then 10 samples of margin
+template <class S>
+int dr_find_min(const float* tp, int i0, const S* pos, int delta, S base, S* best)
+{
+    const int N = DR_DRACE_SAMPLES;
+    int bestI = i0;
+    S bestD = base;
+    int cur = (i0 + delta + N) % N;
+    S curD = dr_sq(tp, cur, pos);
+    while (curD < bestD)
+    {
+        bestI = cur;
+        bestD = curD;
+        cur = (cur + delta + N) % N;
+        curD = dr_sq(tp, cur, pos);
+    }
+    for (int i = 0; i < 10; ++i)
+    {
+        curD = dr_sq(tp, cur, pos);
+        if (curD < bestD)
+        {
+            bestI = cur;
+            bestD = curD;
+        }
+        cur = (cur + delta + N) % N;
+    }
+    *best = bestD;
+    return bestI;
+}
+
+// fastProjectOnTrack (:143-209) from prevS (< 0: full scan)
+template <class S>
+S dr_fast_project(const float* tp, const S* pos, S prevS)
+{
+    const int N = DR_DRACE_SAMPLES;
+    if (prevS < (S)0)
+        return dr_project_full(tp, pos);
+    const int baseI = (int)(prevS * (S)N + (S)0.5f) % N;
+    const S base = dr_sq(tp, baseI, pos);
+    S bD, fD;
+    const int bI = dr_find_min(tp, baseI, pos, -1, base, &bD);
+    const int fI = dr_find_min(tp, baseI, pos, +1, base, &fD);
+    return fD < bD ? (S)fI / (S)N : (S)bI / (S)N;
+}
+
+// sampleCenterline (:55-66), index held at 511
+template <class S>
+void dr_sample_line(const float* tp, S s, S out[2])
+{
+    const int N = DR_DRACE_SAMPLES;
+    s = s - flo(s);
+    const S idx = s * (S)N;
+    int i0 = (int)idx;
+    if (i0 > N - 1)
+        i0 = N - 1;
+    const int i1 = (i0 + 1) % N;
+    const S t = idx - (S)i0;
+    for (int d = 0; d < 2; ++d)
+        out[d] = ((S)1 - t) * (S)tp[i0 * 2 + d] + t * (S)tp[i1 * 2 + d];
+}
+
+// computePIDAction (:473-541); powf(x, y) = exp(y log x)
+template <class S>
+void dr_pid(const DrEnv<S>& E, const DState<S>& s, int agent, const dr_drace_pid& pid, S out[2])
+{
+    S target[2];
+    dr_sample_line(E.line(pid.raceline_index), s.sp[agent][pid.raceline_index] + E.f(E.e.target_distance), target);
+    const S* p = s.pos + agent * 2;
+    const S* v = s.vel + agent * 2;
+    S sqErr = 0;
+    for (int d = 0; d < 2; ++d)
+    {
+        const S e = target[d] - p[d];
+        sqErr = sqErr + e * e;
+    }
+    const S invDist = (S)1 / sq_root(sqErr + (S)1e-5f);
+    S vPar = 0;
+    for (int d = 0; d < 2; ++d)
+        vPar = vPar + v[d] * ((target[d] - p[d]) * invDist);
+    for (int d = 0; d < 2; ++d)
+    {
+        const S err = (target[d] - p[d]) * invDist;
+        const S lat = v[d] - vPar * err;
+        out[d] = E.f(pid.kp) * err + E.f(pid.kd) * (-lat);
+    }
+    if (pid.repulsion_factor != 0.0f)
+    {
+        const int o = 1 - agent;
+        S diff[2], d2 = 0;
+        for (int d = 0; d < 2; ++d)
+        {
+            diff[d] = s.pos[o * 2 + d] - p[d];
+            d2 = d2 + diff[d] * diff[d];
+        }
+        const S dist = sq_root(d2) + (S)1e-8f;
+        if (dist < E.f(pid.repulsion_dist_fact) * E.f(E.e.drone_radius) * (S)2)
+        {
+            const S x = dist * (S)0.5f / E.f(E.e.drone_radius);
+            const S sc = E.f(pid.repulsion_factor) / ex((E.f(pid.repulsion_power) + (S)1) * lg(x));
+            for (int d = 0; d < 2; ++d)
+                out[d] = out[d] - sc * diff[d];
+        }
+    }
+    S sqA = 0;
+    for (int d = 0; d < 2; ++d)
+        sqA = sqA + out[d] * out[d];
+    const S ma = E.f(E.e.max_accel[agent]);
+    if (sqA > ma * ma)
+    {
+        const S fct = ma / sq_root(sqA);
+        for (int d = 0; d < 2; ++d)
+            out[d] = out[d] * fct;
+    }
+}
+
+// updateBelief (:212-259), the likelihood in the log domain
+template <class S>
+void dr_update_belief(const DrEnv<S>& E, S* belief, const S obs[2], const S (*nom)[2], S maxA)
+{
+    const S sigmaEnv = (S)0.2f;
+    const S log2pi = lg((S)2.0f * (S)3.14159265358979323846f);
+    const int NT = E.e.n_models;
+    S sum = 0;
+    for (int th = 0; th < NT; ++th)
+    {
+        const S sqA = nom[th][0] * nom[th][0] + nom[th][1] * nom[th][1];
+        S scale = 1;
+        if (sqA > maxA * maxA)
+            scale = maxA / sq_root(sqA);
+        const S dx = obs[0] - nom[th][0] * scale;
+        const S dy = obs[1] - nom[th][1] * scale;
+        const S sqDist = dx * dx + dy * dy;
+        const S an = E.f(E.e.opp_pid[th].action_noise);
+        const S sqSigma = sigmaEnv * sigmaEnv + an * an;
+        const S ll = (S)-0.5f * sqDist / sqSigma - log2pi - lg(sqSigma);
+        belief[th] = belief[th] * ex(ll);
+        sum = sum + belief[th];
+    }
+    if (sum < (S)1e-20f)
+        for (int th = 0; th < NT; ++th)
+            belief[th] = (S)1 / (S)NT;
+    else
+        for (int th = 0; th < NT; ++th)
+            belief[th] = belief[th] / sum;
+}
+
+// squared distance to rectangle o (:274-283)
+template <class S>
+S dr_sq_rect(const DrEnv<S>& E, int o, const S* c)
+{
+    S sq = 0;
+    for (int d = 0; d < 2; ++d)
+    {
+        const S dist = mx((S)0, mx(E.f(E.e.obstacles[o][d]) - c[d], c[d] - E.f(E.e.obstacles[o][2 + d])));
+        sq = sq + dist * dist;
+    }
+    return sq;
+}
+
+// trackBoundaryDist (:262-308)
+template <class S>
+S dr_bdist(const DrEnv<S>& E, const DState<S>& s, bool lose)
+{
+    const int me = E.e.i_mppi;
+    if (lose && s.laps[1 - me] >= E.e.n_win_laps)
+        return (S)-1;
+    const S* c = s.pos + me * 2;
+    S m = (S)INFINITY;
+    for (int d = 0; d < 2; ++d)
+        m = mn(m, mn(c[d] - E.f(E.e.arena_min[d]), E.f(E.e.arena_max[d]) - c[d]));
+    for (int o = 0; o < E.e.n_obstacles; ++o)
+        m = mn(m, sq_root(dr_sq_rect(E, o, c)));
+    for (int o = 0; o < E.e.n_round_obstacles; ++o)
+    {
+        S sq = 0;
+        for (int d = 0; d < 2; ++d)
+        {
+            const S dx = c[d] - E.f(E.e.round_centers[o][d]);
+            sq = sq + dx * dx;
+        }
+        m = mn(m, sq_root(sq) - E.f(E.e.round_radius[o]));
+    }
+    S sq = 0;
+    for (int d = 0; d < 2; ++d)
+    {
+        const S dx = s.pos[d] - s.pos[d + 2];
+        sq = sq + dx * dx;
+    }
+    return mn(m, sq_root(sq) - E.f(E.e.drone_radius));
+}
+
+// isWinner (:316-359), the opponent's wall test as written (+ radius on both sides)
+template <class S>
+bool dr_winner(const DrEnv<S>& E, const DState<S>& s)
+{
+    const int me = E.e.i_mppi;
+    if (s.laps[me] >= E.e.n_win_laps)
+        return true;
+    const S* c = s.pos + (1 - me) * 2;
+    const S r = E.f(E.e.drone_radius);
+    for (int d = 0; d < 2; ++d)
+        if (c[d] < E.f(E.e.arena_min[d]) + r || c[d] > E.f(E.e.arena_max[d]) + r)
+            return true;
+    for (int o = 0; o < E.e.n_obstacles; ++o)
+        if (dr_sq_rect(E, o, c) < r * r)
+            return true;
+    for (int o = 0; o < E.e.n_round_obstacles; ++o)
+    {
+        S sq = 0;
+        for (int d = 0; d < 2; ++d)
+        {
+            const S dx = c[d] - E.f(E.e.round_centers[o][d]);
+            sq = sq + dx * dx;
+        }
+        const S rr = r + E.f(E.e.round_radius[o]);
+        if (sq < rr * rr)
+            return true;
+    }
+    return false;
+}
+
+// getAnyAdvance (:374-394)
+template <class S>
+S dr_advance(const DrEnv<S>& E, const DState<S>& s, int agent)
+{
+    const int g = s.gates[agent], next = (g + 1) % E.e.n_gates;
+    S g2 = 0, c2 = 0;
+    for (int d = 0; d < 2; ++d)
+    {
+        const S a = E.f(E.e.gate_centers[next][d]) - s.pos[agent * 2 + d];
+        const S b = E.f(E.e.gate_centers[next][d]) - E.f(E.e.gate_centers[g][d]);
+        g2 = g2 + a * a;
+        c2 = c2 + b * b;
+    }
+    return (S)(float)(g + E.e.n_gates * s.laps[agent]) + (S)0.8f * ((S)1 - sq_root(g2 / c2));
+}
+
+// updateGates<false> (:412-470)
+template <class S>
+void dr_gates(const DrEnv<S>& E, DState<S>& s, const S prev[4])
+{
+    for (int a = 0; a < 2; ++a)
+    {
+        if (s.sp[a][0] >= (S)0)
+            for (int r = 0; r < E.e.n_racelines; ++r)
+                s.sp[a][r] = dr_fast_project(E.line(r), s.pos + a * 2, s.sp[a][r]);
+        const int next = (s.gates[a] + 1) % E.e.n_gates;
+        S num = 0, den = 0;
+        for (int d = 0; d < 2; ++d)
+        {
+            num = num + E.f(E.e.gate_vectors[next][d]) * (E.f(E.e.gate_centers[next][d]) - prev[a * 2 + d]);
+            den = den + E.f(E.e.gate_vectors[next][d]) * (s.pos[a * 2 + d] - prev[a * 2 + d]);
+        }
+        if (ab(den) < (S)1e-10f)
+            continue;
+        const S lam = num / den;
+        if (lam < (S)0 || lam > (S)1)
+            continue;
+        S sq = 0;
+        for (int d = 0; d < 2; ++d)
+        {
+            const S v = ((S)1 - lam) * prev[a * 2 + d] + lam * s.pos[a * 2 + d] - E.f(E.e.gate_centers[next][d]);
+            sq = sq + v * v;
+        }
+        if (sq <= E.f(E.e.gate_radius[next]) * E.f(E.e.gate_radius[next]))
+        {
+            s.gates[a] = s.gates[a] + 1;
+            if (s.gates[a] == E.e.n_gates)
+            {
+                s.gates[a] = 0;
+                s.laps[a] = s.laps[a] + 1;
+            }
+        }
+    }
+}
+
+// environmentStep<belief != NULL, false, false, lose> (:546-680): returns
+// 0 none, 1 lose, 2 win, 3 opponent wins (when !lose)
+template <class S>
+int env_step(const DrEnv<S>& E, int theta, const S ego[2], DState<S>& s, Normals& rng, bool lose, S* belief = nullptr, bool noise = true)
+{
+    const dr_drace_env& e = E.e;
+    const int me = e.i_mppi, opp = 1 - me;
+    S prev[4], act[4], noms[DR_SR_MAX_MODELS][2];
+    for (int i = 0; i < 4; ++i)
+        prev[i] = s.pos[i];
+    for (int th = 0; th < e.n_models; ++th)
+        if (belief || th == theta)
+            dr_pid(E, s, opp, e.opp_pid[th], noms[th]);
+    act[me * 2] = ego[0];
+    act[me * 2 + 1] = ego[1];
+    for (int d = 0; d < 2; ++d)
+    {
+        S u = noms[theta][d];
+        if (noise && e.opp_pid[theta].action_noise != 0.0f)
+            u = u + E.f(e.opp_pid[theta].action_noise) * (S)rng.next();
+        act[opp * 2 + d] = u;
+    }
+    for (int a = 0; a < 2; ++a)
+    {
+        const S ma = E.f(e.max_accel[a]);
+        S sq = 0;
+        for (int d = 0; d < 2; ++d)
+            sq = sq + act[a * 2 + d] * act[a * 2 + d];
+        const S factor = sq > ma * ma ? ma / sq_root(sq) : (S)1;
+        for (int d = 0; d < 2; ++d)
+        {
+            S v = act[a * 2 + d] * factor;
+            if (e.action_noise != 0.0f && noise)
+                v = v + (S)rng.next() * E.f(e.action_noise);
+            act[a * 2 + d] = v;
+        }
+    }
+    for (int i = 0; i < 4; ++i)
+        s.pos[i] = s.pos[i] + E.f(e.dt) * s.vel[i];
+    for (int i = 0; i < 4; ++i)
+        s.vel[i] = s.vel[i] + E.f(e.dt) * act[i];
+    for (int a = 0; a < 2; ++a)
+    {
+        const S spd = sq_root(s.vel[a * 2] * s.vel[a * 2] + s.vel[a * 2 + 1] * s.vel[a * 2 + 1]);
+        if (spd > E.f(e.max_speed[a]))
+        {
+            const S sc = E.f(e.max_speed[a]) / spd;
+            for (int d = 0; d < 2; ++d)
+                s.vel[a * 2 + d] = s.vel[a * 2 + d] * sc;
+        }
+    }
+    if ((e.pos_noise != 0.0f || e.speed_noise != 0.0f) && noise)
+        for (int i = 0; i < 4; ++i)
+        {
+            if (e.pos_noise != 0.0f)
+                s.pos[i] = s.pos[i] + (S)rng.next() * E.f(e.pos_noise);
+            if (e.speed_noise != 0.0f)
+                s.vel[i] = s.vel[i] + (S)rng.next() * E.f(e.speed_noise);
+        }
+    dr_gates(E, s, prev);
+    if (belief)
+    {
+        const S obs[2] = {act[opp * 2], act[opp * 2 + 1]};
+        dr_update_belief(E, belief, obs, noms, E.f(e.max_accel[opp]));
+    }
+    if (dr_bdist(E, s, lose) < E.f(e.drone_radius))
+        return 1;
+    if (dr_winner(E, s))
+        return 2;
+    if (!lose && dr_bdist(E, s, true) < E.f(e.drone_radius))
+        return 3;
+    return 0;
+}
+
+// stateCost / finalCost (costs.cuh:7-39) on DroneRace
+template <class S>
+S stage_cost(const DrEnv<S>& E, const dr_sr_cost& K, const DState<S>& s, int)
+{
+    const S r = E.f(E.e.drone_radius);
+    const S bd = dr_bdist(E, s, false);
+    S cost = 0;
+    if (bd < (S)K.boundary_threshold_factor * r)
+    {
+        const S q = bd / r;
+        cost = cost + (S)K.boundary_cost * (q * q);
+    }
+    if (bd < r * (S)K.coll_dist_factor)
+        cost = cost + (S)K.outside_cost;
+    if (s.laps[1 - E.e.i_mppi] >= E.e.n_win_laps)
+        cost = cost + (S)K.opp_win_cost;
+    if (dr_winner(E, s))
+        cost = cost - (S)K.win_cost;
+    return cost;
+}
+
+template <class S>
+S final_cost(const DrEnv<S>& E, const dr_sr_cost& K, const DState<S>& s)
+{
+    S cost = -(S)K.final_adv_weight * dr_advance(E, s, E.e.i_mppi);
+    if (K.final_opp_adv_weight != 0.0f)
+        cost = cost + (S)K.final_opp_adv_weight * dr_advance(E, s, 1 - E.e.i_mppi);
+    return cost;
+}
+
+template <class S>
+int n_models(const DrEnv<S>& E)
+{
+    return E.e.n_models;
+}
+template <class S>
+DState<S> load_st(const DrEnv<S>&, const dr_drace_state& a)
+{
+    return dr_load<S>(a);
+}
+inline float max_accel_of(const dr_drace_env& e) { return e.max_accel[e.i_mppi]; }
+template <class S>
+DrEnv<S> make_env(const dr_drace_env& e)
+{
+    return DrEnv<S>(e);
+}
+
 // Natural cubic spline basis B[t][i] (buildSplineMatrix,
 // mppi_controller.cu:67-141): the spline through e_i at the knots tau
 // (second derivatives by the Thomas algorithm, zero at both ends), read at
@@ -2257,6 +2740,149 @@ long long or_count_ho_ops(const dr_ho_env* env, const dr_sr_cost* K, const dr_ho
         Cn c[DR_SR_MAX_MODELS];
         int u[DR_SR_MAX_MODELS], t[DR_SR_MAX_MODELS];
         bm_rollout<Cn>(H, *K, *x0, *bm, belief, knots, B.data(), 0, (uint32_t)k, c, u, t, nullptr, &steps);
+    }
+    if (steps_out)
+        *steps_out = steps;
+    return g_tally.flops;
+}
+
+// DroneRace projection of (x, y) on the 512 samples tp [512][2]: the full
+// scan when prevS < 0, else the greedy search from prevS.
+double or_drace_project(int f64, const float* tp, double x, double y, double prevS)
+{
+    if (!tp)
+        return NAN;
+    if (f64)
+    {
+        const double p[2] = {x, y};
+        return dr_fast_project(tp, p, prevS);
+    }
+    const float p[2] = {(float)x, (float)y};
+    return dr_fast_project(tp, p, (float)prevS);
+}
+
+// DroneRace PD action of `agent` under opponent model `model`'s configuration.
+int or_drace_pid(int f64, const dr_drace_env* env, const dr_drace_state* st, int agent, int model, double* out)
+{
+    if (!env || !st || !out || agent < 0 || agent > 1 || model < 0 || model >= env->n_models)
+        return 1;
+    if (f64)
+    {
+        const DrEnv<double> E(*env);
+        double o[2];
+        dr_pid(E, dr_load<double>(*st), agent, env->opp_pid[model], o);
+        out[0] = o[0];
+        out[1] = o[1];
+        return 0;
+    }
+    const DrEnv<float> E(*env);
+    float o[2];
+    dr_pid(E, dr_load<float>(*st), agent, env->opp_pid[model], o);
+    out[0] = o[0];
+    out[1] = o[1];
+    return 0;
+}
+
+// DroneRace state functions: what 0 boundary distance, 1 boundary distance
+// with the opponent's win a loss, 2 isWinner, 3 isOppWinner, 4 advance of the
+// controlled agent, 5 of the opponent, 6 stateCost (needs K), 7 finalCost.
+double or_drace_eval(int f64, int what, const dr_drace_env* env, const dr_sr_cost* K, const dr_drace_state* st)
+{
+    if (!env || !st || ((what == 6 || what == 7) && !K))
+        return NAN;
+    auto run = [&](auto zero) -> double {
+        using S = decltype(zero);
+        const DrEnv<S> E(*env);
+        const DState<S> s = dr_load<S>(*st);
+        switch (what)
+        {
+        case 0: return (double)dr_bdist(E, s, false);
+        case 1: return (double)dr_bdist(E, s, true);
+        case 2: return dr_winner(E, s) ? 1.0 : 0.0;
+        case 3: return s.laps[1 - env->i_mppi] >= env->n_win_laps ? 1.0 : 0.0;
+        case 4: return (double)dr_advance(E, s, env->i_mppi);
+        case 5: return (double)dr_advance(E, s, 1 - env->i_mppi);
+        case 6: return (double)stage_cost(E, *K, s, 0);
+        case 7: return (double)final_cost(E, *K, s);
+        default: return NAN;
+        }
+    };
+    return f64 ? run(0.0) : run(0.0f);
+}
+
+// One DroneRace environment step under opponent model theta from *st
+// (updated in place), noise from Philox (k, 4 t + block, step, stream);
+// belief [n_models] updated in place when not NULL.  Returns 0 none, 1 lose,
+// 2 win, 3 opponent wins (lose_on_opp_win = 0).
+int or_drace_env_step(int f64, const dr_drace_env* env, int theta, const float* ego, dr_drace_state* st, float* belief, uint64_t seed,
+                      uint32_t k, uint32_t t, uint32_t step, uint32_t stream, int lose_on_opp_win)
+{
+    if (!env || !ego || !st || theta < 0 || theta >= env->n_models)
+        return -1;
+    Normals rng{(uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32), k, 4u * t, step};
+    rng.stream = stream;
+    const int NT = env->n_models;
+    if (f64)
+    {
+        const DrEnv<double> E(*env);
+        DState<double> s = dr_load<double>(*st);
+        const double u[2] = {ego[0], ego[1]};
+        double b[DR_SR_MAX_MODELS] = {0, 0, 0, 0};
+        if (belief)
+            for (int q = 0; q < NT; ++q)
+                b[q] = belief[q];
+        const int r = env_step(E, theta, u, s, rng, lose_on_opp_win != 0, belief ? b : nullptr);
+        dr_store(s, *st);
+        if (belief)
+            for (int q = 0; q < NT; ++q)
+                belief[q] = (float)b[q];
+        return r;
+    }
+    const DrEnv<float> E(*env);
+    DState<float> s = dr_load<float>(*st);
+    const float u[2] = {ego[0], ego[1]};
+    const int r = env_step(E, theta, u, s, rng, lose_on_opp_win != 0, belief);
+    dr_store(s, *st);
+    return r;
+}
+
+// or_bm_step on DroneRace.
+int or_bmdrace_step(int f64, const dr_drace_env* env, const dr_sr_cost* K, const dr_drace_state* x0, const dr_bm_instance* inst,
+                    const float* belief, uint32_t step, float lambda, const float* knots, const float* prev_dense, int n_threads,
+                    float* noise_out, double* ctrue_out, int* used_out, int* btime_out, double* costs_out, double* mins_out, double* nu_out,
+                    double* knots_out, double* dense_out, double* dense_new_out, int* fail_out, int* len_out, double* res_out)
+{
+    if (!env || !K || !x0 || !inst || !belief || !knots || !mins_out || !nu_out || !knots_out || !dense_out || !res_out)
+        return 1;
+    if (inst->n_knots < 2 || inst->n_knots > DR_MAX_KNOTS || env->n_models < 1 || env->n_models > DR_SR_MAX_MODELS)
+        return 1;
+    if (inst->n_verif_samples > 0 && (inst->verif_horizon < 1 || inst->verif_horizon > inst->n_timesteps))
+        return 1;
+    if (f64)
+        bm_step<double>(*env, *K, *x0, *inst, belief, step, lambda, knots, prev_dense, n_threads, noise_out, ctrue_out, used_out, btime_out,
+                        costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, len_out, res_out);
+    else
+        bm_step<float>(*env, *K, *x0, *inst, belief, step, lambda, knots, prev_dense, n_threads, noise_out, ctrue_out, used_out, btime_out,
+                       costs_out, mins_out, nu_out, knots_out, dense_out, dense_new_out, fail_out, len_out, res_out);
+    return 0;
+}
+
+// Algorithmic FP32 work of n branching DroneRace rollouts, counted as
+// or_count_sr_ops counts; steps_out = rollout-steps taken.
+long long or_count_drace_ops(const dr_drace_env* env, const dr_sr_cost* K, const dr_drace_state* x0, const float* knots, int n,
+                             const dr_bm_instance* bm, const float* belief, long long* steps_out)
+{
+    const DrEnv<Cn> E(*env);
+    std::vector<double> Bd((size_t)bm->n_timesteps * bm->n_knots);
+    spline_basis<double>(bm->knots, bm->n_knots, bm->n_timesteps, Bd.data());
+    std::vector<Cn> B(Bd.begin(), Bd.end());
+    g_tally.flops = 0;
+    long long steps = 0;
+    for (int k = 0; k < n; ++k)
+    {
+        Cn c[DR_SR_MAX_MODELS];
+        int u[DR_SR_MAX_MODELS], t[DR_SR_MAX_MODELS];
+        bm_rollout<Cn>(E, *K, *x0, *bm, belief, knots, B.data(), 0, (uint32_t)k, c, u, t, nullptr, &steps);
     }
     if (steps_out)
         *steps_out = steps;

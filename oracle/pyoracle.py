@@ -88,6 +88,18 @@ def lib(abi):
     L.or_count_ho_ops.restype = C.c_longlong
     L.or_count_ho_ops.argtypes = [C.POINTER(abi.HoEnv), C.POINTER(abi.SrCost), C.POINTER(abi.HoState), F, I, C.POINTER(abi.BmInstance), F,
                                   C.POINTER(C.c_longlong)]
+    L.or_drace_project.restype = C.c_double
+    L.or_drace_project.argtypes = [I, F, C.c_double, C.c_double, C.c_double]
+    L.or_drace_pid.argtypes = [I, C.POINTER(abi.DraceEnv), C.POINTER(abi.DraceState), I, I, D]
+    L.or_drace_eval.restype = C.c_double
+    L.or_drace_eval.argtypes = [I, I, C.POINTER(abi.DraceEnv), C.POINTER(abi.SrCost), C.POINTER(abi.DraceState)]
+    L.or_drace_env_step.argtypes = [I, C.POINTER(abi.DraceEnv), I, F, C.POINTER(abi.DraceState), F, C.c_uint64, U, U, U, U, I]
+    L.or_bmdrace_step.argtypes = [I, C.POINTER(abi.DraceEnv), C.POINTER(abi.SrCost), C.POINTER(abi.DraceState), C.POINTER(abi.BmInstance),
+                                  F, U, C.c_float, F, F, I, F, D, C.POINTER(I), C.POINTER(I), D, D, D, D, D, D, C.POINTER(I),
+                                  C.POINTER(I), D]
+    L.or_count_drace_ops.restype = C.c_longlong
+    L.or_count_drace_ops.argtypes = [C.POINTER(abi.DraceEnv), C.POINTER(abi.SrCost), C.POINTER(abi.DraceState), F, I,
+                                     C.POINTER(abi.BmInstance), F, C.POINTER(C.c_longlong)]
     _lib = L
     return L
 
@@ -428,6 +440,7 @@ class Oracle:
         """One branching spline MPPI step of a StratRace2D or HiddenObs controller from its belief and knots
         [n_models + 1][M][2] (and, with verification, the previous step's executed plans [n_models + 1][T][2])."""
         ho = isinstance(env, self.abi.HoEnv)
+        drc = isinstance(env, self.abi.DraceEnv)
         K, T, M, NT = ins.n_samples, ins.n_timesteps, ins.n_knots, 2 if ho else env.n_models
         NB = NT + 1
         kn = np.ascontiguousarray(knots, dtype=np.float32).reshape(-1)
@@ -449,7 +462,7 @@ class Oracle:
         dense = np.empty(NB * T * 2)
         res = np.empty(4)
         ip = C.POINTER(C.c_int)
-        rc = (self.L.or_bmho_step if ho else self.L.or_bm_step)(int(f64), C.byref(env), C.byref(cost), C.byref(state), C.byref(ins), _f(bel), step, lam, _f(kn),
+        rc = (self.L.or_bmho_step if ho else self.L.or_bmdrace_step if drc else self.L.or_bm_step)(int(f64), C.byref(env), C.byref(cost), C.byref(state), C.byref(ins), _f(bel), step, lam, _f(kn),
                                _f(prev) if prev is not None else None, threads,
                                _f(noise) if noise is not None else None, _d(ct), used.ctypes.data_as(ip), bt.ctypes.data_as(ip),
                                _d(costs), _d(mins), _d(nu), _d(kout), _d(dense), _d(dnew), fail.ctypes.data_as(ip),
@@ -511,4 +524,41 @@ class Oracle:
         b[:2] = belief[:2]
         steps = C.c_longlong(0)
         ops = self.L.or_count_ho_ops(C.byref(env), C.byref(cost), C.byref(state), _f(kn), n, C.byref(ins), _f(b), C.byref(steps))
+        return ops, steps.value
+
+    # ---------------------------------------------------------- DroneRace
+
+    def drace_project(self, f64, line, x, y, prev_s=-1.0):
+        """Progress in [0, 1) of (x, y) on one raceline [512][2] (full scan when prev_s < 0)."""
+        tp = np.ascontiguousarray(line, dtype=np.float32).reshape(-1)
+        return self.L.or_drace_project(int(f64), _f(tp), x, y, prev_s)
+
+    def drace_pid(self, f64, env, state, agent, model):
+        out = np.empty(2)
+        assert self.L.or_drace_pid(int(f64), C.byref(env), C.byref(state), agent, model, _d(out)) == 0
+        return out
+
+    DRACE_EVAL = {"bdist": 0, "bdist_lose": 1, "winner": 2, "opp_winner": 3, "advance": 4, "opp_advance": 5, "state_cost": 6,
+                  "final_cost": 7}
+
+    def drace_eval(self, f64, what, env, state, cost=None):
+        return self.L.or_drace_eval(int(f64), self.DRACE_EVAL[what], C.byref(env), C.byref(cost) if cost is not None else None,
+                                    C.byref(state))
+
+    def drace_env_step(self, f64, env, theta, ego, state, belief=None, seed=0, k=0, t=0, step=0, stream=4, lose_on_opp_win=True):
+        """One DroneRace environment step; returns (state, belief or None, terminal code)."""
+        st = self.abi.DraceState.from_buffer_copy(state)
+        u = np.ascontiguousarray(ego, dtype=np.float32)
+        b = None if belief is None else np.ascontiguousarray(belief, dtype=np.float32).copy()
+        r = self.L.or_drace_env_step(int(f64), C.byref(env), theta, _f(u), C.byref(st), _f(b) if b is not None else None, seed, k, t,
+                                     step, stream, 1 if lose_on_opp_win else 0)
+        return st, b, r
+
+    def count_drace(self, env, cost, state, ins, knots, belief, n):
+        """Counted FP32 work of n branching DroneRace rollouts: (ops, rollout-steps)."""
+        kn = np.ascontiguousarray(knots, dtype=np.float32).reshape(-1)
+        b = np.zeros(self.abi.SR_MAX_MODELS, dtype=np.float32)
+        b[:env.n_models] = belief[:env.n_models]
+        steps = C.c_longlong(0)
+        ops = self.L.or_count_drace_ops(C.byref(env), C.byref(cost), C.byref(state), _f(kn), n, C.byref(ins), _f(b), C.byref(steps))
         return ops, steps.value

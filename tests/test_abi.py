@@ -18,6 +18,7 @@ STRUCTS = {
     "dr_sr_opp_config": "SrOppConfig", "dr_sr_env": "SrEnv", "dr_sr_state": "SrState", "dr_sr_cost": "SrCost",
     "dr_sr_instance": "SrInstance", "dr_prsr_instance": "PrsrInstance", "dr_bm_instance": "BmInstance", "dr_bm_diag": "BmDiag",
     "dr_wire_env_info": "WireEnvInfo", "dr_wire_controller": "WireController", "dr_ho_env": "HoEnv", "dr_ho_state": "HoState",
+    "dr_drace_pid": "DracePid", "dr_drace_env": "DraceEnv", "dr_drace_state": "DraceState",
 }
 RENAMED = {"lambda_": "lambda"}
 

@@ -28,11 +28,12 @@ FLOPS_PER_SAMPLE_STEP = 270.07
 # HBM bytes per rollout sample-step the rollout kernel must move: the float4
 # noise row it writes (read back once by the weighted average).
 ROLLOUT_BYTES_PER_SAMPLE_STEP = 16.0
-# FP32 flops per rollout-step of the StratRace2D and HiddenObs workloads (oracle op counter at the
-# first control step, plus noise; oracle/count_ops.py count_sr and count_ho).  The branching
+# FP32 flops per rollout-step of the StratRace2D, HiddenObs and DroneRace workloads (oracle op
+# counter at the first control step, plus noise; oracle/count_ops.py count_sr, count_ho and
+# count_drace).  The branching
 # rollouts stop at terminal events: their launches are priced by the rollout-steps they ran, which
 # the engine reports per rollout.
-SR_FLOPS_PER_STEP = {"sr_race": 1044.30, "pr_sr_race": 1010.14, "bm_race": 1513.46, "ho_race": 689.45}
+SR_FLOPS_PER_STEP = {"sr_race": 1044.30, "pr_sr_race": 1010.14, "bm_race": 1513.46, "ho_race": 689.45, "drace_race": 1320.55}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
@@ -142,6 +143,15 @@ def cpu_stratrace(pkg, budget_s=12.0):
     def ho_step():
         oracle.bm_step(0, s4.env, s4.cost, s4.states[0], i4, b4, 0, i4.lambda_, k4, threads=threads, want_noise=False)
     out["ho_race"] = timed(ho_step, 2 * i4.n_samples * i4.n_timesteps)
+
+    s5 = sc.drace_race(n_inst=1)
+    i5 = s5.instances[0]
+    k5 = np.zeros((3, i5.n_knots, 2), np.float32)
+    b5 = np.array(i5.init_belief[:2], np.float32)
+
+    def drace_step():
+        oracle.bm_step(0, s5.env, s5.cost, s5.states[0], i5, b5, 0, i5.lambda_, k5, threads=threads, want_noise=False)
+    out["drace_race"] = timed(drace_step, 2 * i5.n_samples * i5.n_timesteps)
     return out
 
 
@@ -376,6 +386,37 @@ def ho_race(pkg, steps=30, warmup=3):
             "steps": steps}
 
 
+def drace_race(pkg, steps=30, warmup=3):
+    """Branching spline MPPI on the reference's DroneRace course at its spline setting
+    (tinyGateEnv, main.py:40-161: K = 2^16, T = 60, 10 knots, temperature 10 adapted, sigma 2,
+    two PD opponent models on the upper and lower raceline, uniform belief), closed loop, one
+    world."""
+    sc = pkg.scenarios
+    scn = sc.drace_race(n_inst=1)
+    eng = pkg.DraceBMEngine(scn)
+    eng.step(warmup, advance=True)
+    t0 = time.perf_counter()
+    eng.step(steps, advance=True)
+    elapsed = time.perf_counter() - t0
+    ms, ran = timed_branching_steps(eng)
+    d = eng.diag()[0]
+    st = eng.get_states()[0]
+    belief = eng.belief(0)
+    eng.close()
+    ins = scn.instances[0]
+    rollouts = scn.env.n_models * ins.n_samples
+    return {"workload": scn.name, "control_steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+            "rollouts_per_step": rollouts, "rollout_env_steps_per_s": rollouts * ins.n_timesteps * steps / elapsed,
+            "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate_min": float(ms[1]), "weighted_avg": float(ms[2]),
+                                   "update_verify": float(ms[3]), "choice_world": float(ms[4]), "step": float(ms[5])},
+            "final_branch": int(d.branch), "final_state": {"pos": [float(v) for v in st.pos], "gates": list(st.gates),
+                                                           "laps": list(st.laps)},
+            "final_belief": [float(b) for b in belief],
+            "rollout_steps_per_launch": ran, "rollout_steps_fraction": ran / (rollouts * ins.n_timesteps),
+            "roofline": sr_roofline("drace_race", ran, float(ms[0]), "k_bm_rollout<DracePolicy>"),
+            "steps": steps}
+
+
 def session_race(pkg, steps=30):
     """The simulation session behind the wire protocol (SimulationEngine::run) at the reference's
     settings: the branching spline MPPI header (K = 2^16, T = 80, 20 knots) and the PR-MPPI header
@@ -420,7 +461,7 @@ def main():
     ap.add_argument("--sr-only", action="store_true", help="run only the StratRace2D section (for profiling)")
     ap.add_argument("--bm-only", action="store_true", help="run only the branching spline MPPI section (for profiling)")
     ap.add_argument("--session-only", action="store_true", help="run only the wire-protocol session section")
-    ap.add_argument("--ho-only", action="store_true", help="run only the HiddenObs section (for profiling)")
+    ap.add_argument("--ho-only", action="store_true", help="run only the HiddenObs and DroneRace sections (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 512 MiB buffer 3 times (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -446,7 +487,8 @@ def main():
         print(json.dumps({"session": session_race(pkg, steps=min(args.steps, 30))}), flush=True)
         return
     if args.ho_only:
-        print(json.dumps({"ho_race": ho_race(pkg, steps=min(args.steps, 30), warmup=args.warmup)}), flush=True)
+        print(json.dumps({"ho_race": ho_race(pkg, steps=min(args.steps, 30), warmup=args.warmup),
+                          "drace_race": drace_race(pkg, steps=min(args.steps, 30), warmup=args.warmup)}), flush=True)
         return
     if args.bm_only:
         print(json.dumps({"bm_race": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup),
@@ -564,6 +606,9 @@ def main():
         log("bench: branching spline MPPI on HiddenObs, K=65536, T=80, 20 knots ...")
         out["ho_race"] = ho_race(pkg)
         log(f"bench: ho_race {json.dumps(out['ho_race'])}")
+        log("bench: branching spline MPPI on DroneRace, K=65536, T=60, 10 knots ...")
+        out["drace_race"] = drace_race(pkg)
+        log(f"bench: drace_race {json.dumps(out['drace_race'])}")
         log("bench: wire-protocol session (MPPI and PR-MPPI headers at the reference settings) ...")
         out["session"] = session_race(pkg)
         log(f"bench: session {json.dumps(out['session'])}")

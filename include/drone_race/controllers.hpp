@@ -11,9 +11,10 @@
 //   PRMPPIController::h_nom_nominal, h_rob_nominal, resetNom, useNomPlan controllers.h:128-135
 //   stratrace::MPPIController: the same MPPIController interface over the
 //   reference's own StratRace2D environment (env_stratrace_defs.h:42-96).
-//   stratrace::BranchingMPPIController, hiddenobs::BranchingMPPIController:
-//   the reference's default (branching, spline) MPPIController on
-//   StratRace2D and on HiddenObs (env_hiddenobs_defs.h:48-105).
+//   stratrace::BranchingMPPIController, hiddenobs::BranchingMPPIController,
+//   dronerace::BranchingMPPIController: the reference's default (branching,
+//   spline) MPPIController on StratRace2D, HiddenObs (env_hiddenobs_defs.h)
+//   and DroneRace (env_dronerace_defs.h).
 //
 // Errors from the C ABI are rethrown as std::runtime_error, as the
 // reference's constructors and checks throw (mppi_controller.cu:27-29).
@@ -360,6 +361,18 @@ struct BmStratRace
     static int set_state(dr_bm_engine* h, const State* s) { return dr_bm_set_states(h, s, 1); }
 };
 
+struct BmDroneRace
+{
+    using Env = dr_drace_env;
+    using State = dr_drace_state;
+    static int models(const Env& e) { return e.n_models; }
+    static int create(const Env* e, const dr_sr_cost* k, const dr_bm_instance* in, dr_bm_engine** h)
+    {
+        return dr_bmdrace_create(e, k, in, 1, h);
+    }
+    static int set_state(dr_bm_engine* h, const State* s) { return dr_bmdrace_set_states(h, s, 1); }
+};
+
 struct BmHiddenObs
 {
     using Env = dr_ho_env;
@@ -496,5 +509,31 @@ struct BranchingMPPIConfig : stratrace::BranchingMPPIConfig
 using BranchingMPPIController = stratrace::detail::BranchingMPPI<stratrace::detail::BmHiddenObs, BranchingMPPIConfig>;
 
 } // namespace hiddenobs
+
+namespace dronerace {
+
+// The reference's SimState / EnvironmentConfig of the DroneRace build
+// (env_dronerace_defs.h:45-107; counts as fields, racelines held by value).
+using SimState = dr_drace_state;
+using EnvironmentConfig = dr_drace_env;
+using PIDConfig = dr_drace_pid;
+
+// MPPIConfig of tinyGateEnv with splines (python-src/main.py:140-161).
+struct BranchingMPPIConfig : stratrace::BranchingMPPIConfig
+{
+    BranchingMPPIConfig()
+    {
+        nTimesteps = 60;
+        nKnots = 10;
+        samplingNoise = 2.0f;
+        cost = dr_sr_cost{1.1f, 0.0f, 1.7f, 1e6f, 1e6f, 1000.0f, 200.0f, 0.0f};
+    }
+};
+
+// MPPIController of the DroneRace build: branching over the opponent's PD
+// configurations.
+using BranchingMPPIController = stratrace::detail::BranchingMPPI<stratrace::detail::BmDroneRace, BranchingMPPIConfig>;
+
+} // namespace dronerace
 
 } // namespace drone_race

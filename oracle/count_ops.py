@@ -131,8 +131,26 @@ def count_ho(pkg, n=256):
     return {"ho_race": f / steps + knot_noise, "ho_race_steps_fraction": steps / (n * 2 * ins.n_timesteps)}
 
 
+def count_drace(pkg, n=256):
+    """FP32 flops per rollout-step of the branching DroneRace workload at its first control step
+    (oracle op counter over n samples) plus the knot noise of a sample shared by its model
+    rollouts and the environment noise per step (the opponent's PD noise pair and the action
+    noise of both agents: three Box-Muller pairs)."""
+    sys.path.insert(0, HERE)
+    import pyoracle
+    oracle = pyoracle.Oracle(pkg.abi)
+    s = pkg.scenarios.drace_race(n_inst=1)
+    ins = s.instances[0]
+    NT = s.env.n_models
+    kn = np.zeros((NT + 1) * ins.n_knots * 2, np.float32)
+    f, steps = oracle.count_drace(s.env, s.cost, s.states[0], ins, kn, np.float32(ins.init_belief[:NT]), n)
+    knot_noise = (NT + 1) * ins.n_knots * SR_CONTROL_NOISE * n / steps
+    env_noise = 3 * (20 + 21 + 5)
+    return {"drace_race": f / steps + knot_noise + env_noise, "drace_race_steps_fraction": steps / (n * NT * ins.n_timesteps)}
+
+
 if __name__ == "__main__":
     pkg = package()
     print(f"{count(pkg.scenarios.c1_four_drones()):.2f} FP32 flops per sample-step (4 drones, K=16384, T=100)")
-    for k, v in {**count_sr(pkg), **count_ho(pkg)}.items():
+    for k, v in {**count_sr(pkg), **count_ho(pkg), **count_drace(pkg)}.items():
         print(f"{k}: {v:.2f}")

@@ -33,3 +33,10 @@ def test_hiddenobs_flops_match_counter(pkg):
     count_ops = _load(os.path.join(ROOT, "oracle", "count_ops.py"), "count_ops_mod")
     measured = count_ops.count_ho(pkg)
     assert abs(bench.SR_FLOPS_PER_STEP["ho_race"] - measured["ho_race"]) < 0.01, measured
+
+
+def test_dronerace_flops_match_counter(pkg):
+    bench = _load(os.path.join(ROOT, "bench.py"), "bench_mod")
+    count_ops = _load(os.path.join(ROOT, "oracle", "count_ops.py"), "count_ops_mod")
+    measured = count_ops.count_drace(pkg)
+    assert abs(bench.SR_FLOPS_PER_STEP["drace_race"] - measured["drace_race"]) < 0.01, measured

@@ -321,20 +321,21 @@ def timed_branching_steps(eng, n=5):
     for _ in range(n):
         eng.step(1, advance=True)
         ms += eng.last_timing()
-        ran += int(eng.rollout_lengths(0).sum())
+        ran += sum(int(eng.rollout_lengths(i).sum()) for i in range(eng.n_inst))
     eng.set_timing(False)
     eng.set_record_lengths(False)
     return ms / n, ran / n
 
 
-def bm_race(pkg, steps=30, warmup=3, n_verif=0):
+def bm_race(pkg, steps=30, warmup=3, n_verif=0, n_inst=1):
     """Branching spline MPPI (the reference's default MPPI controller) on its StratRace2D race at
     the spline setting (K = 2^16, T = 80, 20 knots, two opponent models, uniform initial belief,
-    temperature 10 adapted, sigma 0.5), closed loop with belief updates, one world; n_verif > 0
+    temperature 10 adapted, sigma 0.5), closed loop with belief updates, n_inst worlds batched in
+    one graph (control steps/s counts every world's step); n_verif > 0
     adds the plan verification at mainGate's setting (2^17 samples, horizon 40, beta 1e-5,
     maxVerifEps 0.001)."""
     sc = pkg.scenarios
-    scn = sc.bm_race(n_inst=1, n_verif=n_verif)
+    scn = sc.bm_race(n_inst=n_inst, n_verif=n_verif)
     eng = pkg.BMEngine(scn)
     eng.step(warmup, advance=True)
     t0 = time.perf_counter()
@@ -344,8 +345,8 @@ def bm_race(pkg, steps=30, warmup=3, n_verif=0):
     d = eng.diag()[0]
     eng.close()
     ins = scn.instances[0]
-    rollouts = scn.env.n_models * ins.n_samples
-    return {"workload": scn.name, "control_steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+    rollouts = n_inst * scn.env.n_models * ins.n_samples
+    return {"workload": scn.name, "control_steps_per_s": n_inst * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
             "rollouts_per_step": rollouts, "rollout_env_steps_per_s": rollouts * ins.n_timesteps * steps / elapsed,
             "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate_min": float(ms[1]), "weighted_avg": float(ms[2]),
                                    "update_verify": float(ms[3]), "choice_world": float(ms[4]), "step": float(ms[5])},
@@ -600,6 +601,9 @@ def main():
         log("bench: branching spline MPPI on StratRace2D, K=65536, T=80, 20 knots ...")
         out["bm_race"] = bm_race(pkg)
         log(f"bench: bm_race {json.dumps(out['bm_race'])}")
+        log("bench: branching spline MPPI, 8 worlds in one batch ...")
+        out["bm_race_8x"] = bm_race(pkg, n_inst=8)
+        log(f"bench: bm_race_8x {json.dumps(out['bm_race_8x'])}")
         log("bench: branching spline MPPI with plan verification (2^17 samples, horizon 40) ...")
         out["bm_race_verified"] = bm_race(pkg, n_verif=2 ** 17)
         log(f"bench: bm_race_verified {json.dumps(out['bm_race_verified'])}")

@@ -132,19 +132,27 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                     belief[q] = belief0[q];
                 int pred = firstPred, time = firstTime, origin = 0;
                 constexpr int RM = MAXM > 0 ? MAXM : 1;
+                // knot + noise of the current branch, formed once per branch
+                // (the same float sums the step would form)
                 float2 e[RM];
                 if constexpr (MAXM > 0)
                 {
+                    const float* kb = s_kn + pred * M * 2;
 #pragma unroll
                     for (int m = 0; m < MAXM; ++m)
-                        e[m] = m < M ? knot_noise(key0, key1, (uint32_t)k, pred, m, step, I.sigma) : make_float2(0.0f, 0.0f);
+                        if (m < M)
+                        {
+                            const float2 z = knot_noise(key0, key1, (uint32_t)k, pred, m, step, I.sigma);
+                            e[m] = make_float2(kb[m * 2] + z.x, kb[m * 2 + 1] + z.y);
+                        }
+                        else
+                            e[m] = make_float2(0.0f, 0.0f);
                 }
                 float acc = 0.0f, decay = 1.0f;
                 int t = 0;
                 for (; t < T; ++t)
                 {
                     const float* Brow = s_B + (t - origin) * M;
-                    const float* kn = s_kn + pred * M * 2;
                     float u0 = 0.0f, u1 = 0.0f;
                     if constexpr (MAXM > 0)
                     {
@@ -152,12 +160,13 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                         for (int m = 0; m < MAXM; ++m)
                             if (m < M)
                             {
-                                u0 = fma_(Brow[m], kn[m * 2] + e[m].x, u0);
-                                u1 = fma_(Brow[m], kn[m * 2 + 1] + e[m].y, u1);
+                                u0 = fma_(Brow[m], e[m].x, u0);
+                                u1 = fma_(Brow[m], e[m].y, u1);
                             }
                     }
                     else
                     {
+                        const float* kn = s_kn + pred * M * 2;
                         const float2* nrow = noise + I.noise_off + (long long)pred * M * K + k;
                         for (int m = 0; m < M; ++m)
                         {
@@ -178,9 +187,16 @@ __global__ __launch_bounds__(BM_BLOCK) void k_bm_rollout(const BmInstDev* __rest
                         origin = t + 1;
                         if constexpr (MAXM > 0)
                         {
+                            const float* kb = s_kn + pred * M * 2;
 #pragma unroll
                             for (int m = 0; m < MAXM; ++m)
-                                e[m] = m < M ? knot_noise(key0, key1, (uint32_t)k, pred, m, step, I.sigma) : make_float2(0.0f, 0.0f);
+                                if (m < M)
+                                {
+                                    const float2 z = knot_noise(key0, key1, (uint32_t)k, pred, m, step, I.sigma);
+                                    e[m] = make_float2(kb[m * 2] + z.x, kb[m * 2 + 1] + z.y);
+                                }
+                                else
+                                    e[m] = make_float2(0.0f, 0.0f);
                         }
                     }
                     acc = fma_(P::stage_cost(ctx, E, KC, s, th), decay, acc);

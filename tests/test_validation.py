@@ -106,3 +106,76 @@ def test_pr_null_handles_are_rejected(pkg):
     assert lib.dr_pr_set_belief(None, 0, None, 0) == pkg.abi.DR_EINVAL
     assert lib.dr_pr_set_launch(None, None) == pkg.abi.DR_EINVAL
     lib.dr_pr_destroy(None)
+
+
+# ---------------------------------------------------------------- branching engine (all environments)
+
+def bm_create(pkg, kind, mutate_env=None, mutate_inst=None):
+    abi, sc = pkg.abi, pkg.scenarios
+    lib = abi.load()
+    if kind == "stratrace":
+        env, cost, fn = sc.sr_env(n_models=2), sc.bm_cost(), lib.dr_bm_create
+    elif kind == "hiddenobs":
+        env, cost, fn = sc.ho_env(), sc.ho_cost(), lib.dr_bmho_create
+    else:
+        env, cost, fn = sc.drace_env(), sc.drace_cost(), lib.dr_bmdrace_create
+    ins = sc.bm_instance(256, 30, 1, n_knots=6)
+    if mutate_env:
+        mutate_env(env)
+    if mutate_inst:
+        mutate_inst(ins)
+    arr = (abi.BmInstance * 1)(ins)
+    h = C.c_void_p()
+    rc = fn(C.byref(env), C.byref(cost), arr, 1, C.byref(h))
+    return rc, lib.dr_last_error().decode(), h
+
+
+INST_CASES = {
+    "knots not increasing": lambda i: setattr(i, "knots", (C.c_int32 * 60)(0, 5, 5, 10, 15, 20)),
+    "knot beyond the horizon": lambda i: i.knots.__setitem__(5, 30),
+    "too many knots x steps": lambda i: (setattr(i, "n_timesteps", 256), setattr(i, "n_knots", 40)),
+    "true model out of range": lambda i: setattr(i, "true_theta", 2),
+    "negative belief": lambda i: i.init_belief.__setitem__(0, -0.1),
+    "verification horizon beyond T": lambda i: (setattr(i, "n_verif_samples", 128), setattr(i, "verif_horizon", 31),
+                                                setattr(i, "beta", 1e-5)),
+    "verification beta of 1": lambda i: (setattr(i, "n_verif_samples", 128), setattr(i, "verif_horizon", 10), setattr(i, "beta", 1.0)),
+    "zero temperature": lambda i: setattr(i, "lambda_", 0.0),
+}
+
+
+@pytest.mark.parametrize("kind", ["stratrace", "hiddenobs", "dronerace"])
+@pytest.mark.parametrize("name", list(INST_CASES))
+def test_branching_create_rejects_instances(pkg, kind, name):
+    rc, msg, h = bm_create(pkg, kind, mutate_inst=INST_CASES[name])
+    assert rc == pkg.abi.DR_EINVAL and msg and not h.value, (rc, msg)
+
+
+def _nan(arr, i):
+    arr[i] = float("nan")
+
+
+ENV_CASES = {
+    "hiddenobs": {
+        "negative noise": lambda e: setattr(e, "pos_noise", -0.1),
+        "zero step": lambda e: setattr(e, "dt", 0.0),
+        "NaN geometry": lambda e: _nan(e.hidden_radius, 1),
+        "repeated gate": lambda e: (e.gate_centers[1].__setitem__(0, 12.0), e.gate_centers[1].__setitem__(1, 7.0)),
+        "no laps to win": lambda e: setattr(e, "n_win_laps", 0),
+    },
+    "dronerace": {
+        "nine gates": lambda e: setattr(e, "n_gates", 9),
+        "five racelines": lambda e: setattr(e, "n_racelines", 5),
+        "raceline of a model missing": lambda e: setattr(e.opp_pid[1], "raceline_index", 2),
+        "controlled agent 2": lambda e: setattr(e, "i_mppi", 2),
+        "five obstacles": lambda e: setattr(e, "n_obstacles", 5),
+        "NaN track point": lambda e: _nan(e.track_points[1][100], 0),
+        "zero max speed": lambda e: e.max_speed.__setitem__(1, 0.0),
+        "negative PD noise": lambda e: setattr(e.opp_pid[0], "action_noise", -1.0),
+    },
+}
+
+
+@pytest.mark.parametrize("kind,name", [(k, n) for k, cases in ENV_CASES.items() for n in cases])
+def test_branching_create_rejects_environments(pkg, kind, name):
+    rc, msg, h = bm_create(pkg, kind, mutate_env=ENV_CASES[kind][name])
+    assert rc == pkg.abi.DR_EINVAL and msg and not h.value, (rc, msg)

@@ -19,14 +19,15 @@
 namespace dr {
 
 // dJ/dp of the running cost at state s (post-step, opponents at this step).
-DR_HD void stage_cost_grad(const QuadConst& C, const CostConst& Q, const dr_cost_cfg& K, const dr_track& tr, const QState& s,
-                           const float4* opp, int n_opp, float weight, float g[3])
+template <class OppAt>
+DR_HD void stage_cost_grad_at(const QuadConst& C, const CostConst& Q, const dr_cost_cfg& K, const dr_track& tr, const QState& s,
+                              int n_opp, OppAt opp_at, float weight, float g[3])
 {
     float min_sq = __builtin_inff();
     int jmin = -1;
     for (int j = 0; j < n_opp; ++j)
     {
-        const float4 o = opp[j];
+        const float4 o = opp_at(j);
         const float dx = s.p[0] - o.x, dy = s.p[1] - o.y, dz = s.p[2] - o.z;
         const float d2 = fma_(dx, dx, fma_(dy, dy, dz * dz));
         if (d2 < min_sq)
@@ -57,7 +58,7 @@ DR_HD void stage_cost_grad(const QuadConst& C, const CostConst& Q, const dr_cost
         const float dbd = weight * (2.0f * K.boundary_cost * x) * (-Q.inv_danger);
         if (opp_side && jmin >= 0)
         {
-            const float4 o = opp[jmin];
+            const float4 o = opp_at(jmin);
             const float d = __builtin_sqrtf(min_sq);
             if (d > 0.0f)
             {
@@ -70,6 +71,12 @@ DR_HD void stage_cost_grad(const QuadConst& C, const CostConst& Q, const dr_cost
         else
             g[2] = g[2] + dbd;
     }
+}
+
+DR_HD void stage_cost_grad(const QuadConst& C, const CostConst& Q, const dr_cost_cfg& K, const dr_track& tr, const QState& s,
+                           const float4* opp, int n_opp, float weight, float g[3])
+{
+    stage_cost_grad_at(C, Q, K, tr, s, n_opp, [opp](int j) { return opp[j]; }, weight, g);
 }
 
 // dJ/dp of the terminal cost -final_adv_weight * progress.

@@ -36,9 +36,8 @@ hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstD
 hipError_t launch_costgrad(int lanes, hipStream_t st, const InstDev* insts, int inst, int T, int n_ck, int ck,
                            const dr_track* tracks, const dr_drone_state* drones, const float* U, int n_seq, float* J, float* G,
                            const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, int opp_stride);
-hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, int max_T, int max_slots, const dr_track* tracks,
-                         const dr_drone_state* drones, float* nominal, int n_steps, float step, int ck, const QuadConst& C,
-                         const CostConst& Q, const dr_cost_cfg& KC, int opp_stride);
+hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, const dr_track* tracks, const dr_drone_state* drones,
+                         float* nominal, int n_steps, float step, int ck, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC);
 
 } // namespace dr
 
@@ -158,15 +157,6 @@ struct dr_engine
     }
 
     int build_layout();
-    // state slots of one adjoint lane: checkpoints + one recomputed segment
-    int slots_for(int T, int ck) const { return (T + ck - 1) / ck + ck; }
-    int max_refine_slots() const
-    {
-        int m = 1;
-        for (const InstDev& I : idev)
-            m = std::max(m, slots_for(I.T, refine_ck));
-        return m;
-    }
     int enqueue_step(bool advance, int timed_index);
     int run(int n_steps, bool advance);
 };
@@ -292,8 +282,8 @@ int dr_engine::enqueue_step(bool advance, int timed_index)
     {
         HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, 0, d_drones.p,
                              d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin, 1));
-        HIPCHK(launch_refine(n, stream, d_inst.p, max_T, max_refine_slots(), d_tracks.p, d_drones.p, d_nominal.p, refine_steps,
-                             refine_step, refine_ck, C, Q, cost, opp_stride));
+        HIPCHK(launch_refine(n, stream, d_inst.p, d_tracks.p, d_drones.p, d_nominal.p, refine_steps, refine_step, refine_ck, C, Q,
+                             cost));
         HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, advance ? 1 : 0, d_drones.p,
                              d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin, 2));
     }
@@ -738,11 +728,6 @@ int dr_engine_set_refine(dr_engine* e, int n_grad_steps, float step, int ckpt_ev
         return fail(DR_EINVAL, "NULL handle");
     if (n_grad_steps < 0 || n_grad_steps > 100 || !(step >= 0.0f) || ckpt_every < 1 || ckpt_every > DR_MAX_T)
         return fail(DR_EINVAL, "set_refine: n_grad_steps in [0,100], step >= 0, ckpt_every in [1,256]");
-    size_t need = 0;
-    for (const InstDev& I : e->idev)
-        need = std::max(need, ((size_t)e->max_T * e->opp_stride + (size_t)e->slots_for(I.T, ckpt_every) * 4) * sizeof(float4));
-    if (need > kAdjLdsBudget)
-        return fail(DR_EINVAL, "set_refine: checkpoint interval needs too much LDS (use ckpt_every near sqrt(T))");
     HIPCHK(hipStreamSynchronize(e->stream));
     e->refine_steps = n_grad_steps;
     e->refine_step = step;

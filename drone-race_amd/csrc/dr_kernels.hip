@@ -628,45 +628,341 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
                             blockDim.x);
 }
 
-// Adjoint refinement of each instance's updated nominal (one workgroup per
-// instance, one lane): n_steps projected gradient steps
-// U <- clamp(U - step * dJ/dU), J and dJ/dU from cost_grad_lane.
-__global__ __launch_bounds__(64) void k_refine(const InstDev* __restrict__ insts, const dr_track* __restrict__ tracks,
-                                               const dr_drone_state* __restrict__ drones, float* __restrict__ nominal, int n_steps,
-                                               float step, int ck, QuadConst C, CostConst Q, dr_cost_cfg KC, int opp_stride)
+// Adjoint refinement of each instance's updated nominal: n_steps projected
+// gradient steps U <- clamp(U - step * dJ/dU), one workgroup of REF_BLOCK
+// lanes per instance, everything in LDS.  J and dJ/dU are those of
+// cost_grad_lane (same forward, same terminal rule, same vector-Jacobian
+// products); the work is spread over the lanes so that only the forward
+// recurrence of the dynamics stays serial:
+//  A  lane 0 runs the dynamics and the gate counter over all T steps and
+//     writes the state at every ck-th step (the checkpoints) into the state
+//     table; no cost is evaluated here.
+//  B  one lane per checkpoint segment recomputes the segment's states from
+//     its checkpoint into the table.
+//  C  one lane per step t: terminal test and dJ/dp of the running cost of
+//     the post-step state x_{t+1}; the first terminal step t_end ends the
+//     trajectory (cost_grad_lane's break).
+//  D  the reverse recurrence lam_t = A_t^T lam_{t+1} + g_t is linear in lam.
+//     The steps 0..t_end are cut into n_seg <= REF_SEGS reverse segments
+//     [a, b); per segment, 14 lanes carry lam_b = 0 with the g_t terms
+//     (giving psi) and lam_b = e_i, i < 13, without them (giving column i of
+//     Psi) down to a: lam_a = Psi lam_b + psi.
+//  E  wave 0, lane i holding component i: lam_b of every segment, from the
+//     terminal adjoint down to the first segment (one 13 x 13
+//     matrix-vector product per segment).
+//  F  one lane per reverse segment carries its exact lam_b down the segment
+//     and writes dJ/du_t and the projected step of u_t; steps after t_end
+//     have dJ/du = 0 (u_t is only clamped).
+// The order of the additions in D-F differs from the one-lane sweep, so the
+// gradient agrees with cost_grad_lane to rounding, not bit for bit.
+constexpr int REF_BLOCK = 256;
+constexpr int REF_SEGS = 16; // 14 * REF_SEGS <= REF_BLOCK
+constexpr int REF_COMP = 14; // affine part + 13 state directions
+
+// lam as 13 floats: p[3], q[4], v[3], w[3]
+__device__ __forceinline__ void grad_to_flat(const QGrad& g, float f[13])
 {
-    extern __shared__ float4 s_dyn[];
+    f[0] = g.p[0]; f[1] = g.p[1]; f[2] = g.p[2];
+    f[3] = g.q[0]; f[4] = g.q[1]; f[5] = g.q[2]; f[6] = g.q[3];
+    f[7] = g.v[0]; f[8] = g.v[1]; f[9] = g.v[2];
+    f[10] = g.w[0]; f[11] = g.w[1]; f[12] = g.w[2];
+}
+
+__device__ __forceinline__ QGrad grad_from_flat(const float f[13])
+{
+    QGrad g;
+    g.p[0] = f[0]; g.p[1] = f[1]; g.p[2] = f[2];
+    g.q[0] = f[3]; g.q[1] = f[4]; g.q[2] = f[5]; g.q[3] = f[6];
+    g.v[0] = f[7]; g.v[1] = f[8]; g.v[2] = f[9];
+    g.w[0] = f[10]; g.w[1] = f[11]; g.w[2] = f[12];
+    return g;
+}
+
+// State x_t and control u_t of one step, as read from the tables.
+struct RefRow
+{
+    float4 x[4], u;
+    __device__ static RefRow load(const float4* xs, const float4* us, int t)
+    {
+        RefRow r;
+        for (int k = 0; k < 4; ++k)
+            r.x[k] = xs[t * 4 + k];
+        r.u = us[t];
+        return r;
+    }
+    __device__ QState state() const
+    {
+        QState s;
+        s.p[0] = x[0].x; s.p[1] = x[0].y; s.p[2] = x[0].z; s.q[0] = x[0].w;
+        s.q[1] = x[1].x; s.q[2] = x[1].y; s.q[3] = x[1].z; s.v[0] = x[1].w;
+        s.v[1] = x[2].x; s.v[2] = x[2].y; s.w[0] = x[2].z; s.w[1] = x[2].w;
+        s.w[2] = x[3].x; s.gate = (int)f2u(x[3].y); s.laps = (int)f2u(x[3].z);
+        return s;
+    }
+};
+
+// One step of the forward pass (clamp, dynamics, gate counter).
+__device__ __forceinline__ void ref_step(const QuadConst& C, const dr_track& tr, float margin, NextGate& ng, QState& s, float4 u4)
+{
+    const float uin[4] = {u4.x, u4.y, u4.z, u4.w};
+    float uc[4];
+    clamp_control(C, uin, uc);
+    const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+    quad_step(C, s, uc);
+    ng.update(tr.gates, tr.n_gates, margin, p0, s);
+}
+
+__global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict__ insts, const dr_track* __restrict__ tracks,
+                                                      const dr_drone_state* __restrict__ drones, float* __restrict__ nominal,
+                                                      int n_steps, float step, int ck, QuadConst C, CostConst Q, dr_cost_cfg KC)
+{
     __shared__ dr_track s_track;
-    __shared__ float s_u[DR_MAX_T * 4];
-    __shared__ float s_g[DR_MAX_T * 4];
+    __shared__ float4 s_x[(DR_MAX_T + 1) * 4]; // state table, put_state layout
+    __shared__ float4 s_u[DR_MAX_T];           // the sequence being refined
+    __shared__ float4 s_gp[DR_MAX_T];          // dJ/dp of the running cost of x_{t+1}
+    __shared__ float s_w[DR_MAX_T];            // running-cost weight of step t
+    __shared__ float s_psi[REF_SEGS * REF_COMP * 16];
+    __shared__ float s_lam[REF_SEGS * 16];     // lam_b of each reverse segment
+    __shared__ float4 s_op[MAX_OPP], s_ov[MAX_OPP];
+    __shared__ int s_tend;
+
     const InstDev& I = insts[blockIdx.x];
-    const int T = I.T;
-    float4* s_opp = s_dyn;
-    float4* s_buf = s_dyn + T * opp_stride;
-    stage_inputs(I, tracks, drones, C, &s_track, s_opp, opp_stride);
-    for (int i = threadIdx.x; i < T * 4; i += blockDim.x)
-        s_u[i] = nominal[I.nom_off + i];
-    __syncthreads();
-    AdjCtx A{&s_track, s_opp, opp_stride, I.n_opp, T, ck, C, Q, KC};
+    const int T = I.T, n_opp = I.n_opp, tid = threadIdx.x;
+    const float margin = KC.gate_traversal_margin;
+    {
+        const float* src = reinterpret_cast<const float*>(tracks + I.track);
+        float* dst = reinterpret_cast<float*>(&s_track);
+        for (int i = tid; i < (int)(sizeof(dr_track) / 4); i += REF_BLOCK)
+            dst[i] = src[i];
+    }
+    if (tid < n_opp)
+    {
+        const dr_drone_state& o = drones[I.opp[tid]];
+        s_op[tid] = make_float4(o.p[0], o.p[1], o.p[2], 0.0f);
+        s_ov[tid] = make_float4(o.v[0], o.v[1], o.v[2], 0.0f);
+    }
+    for (int t = tid; t < T; t += REF_BLOCK)
+    {
+        const float* u = nominal + I.nom_off + t * 4;
+        s_u[t] = make_float4(u[0], u[1], u[2], u[3]);
+    }
+    if (tid == 0)
+    {
+        // the running-cost weights, the forward pass's decay products
+        float w = 1.0f;
+        for (int t = 0; t < T; ++t)
+        {
+            s_w[t] = w;
+            w = w * DECAY;
+        }
+    }
     const QState x0 = state_from_abi(drones[I.drone]);
+    const int n_ck = (T + ck - 1) / ck;
+    __syncthreads();
+
     for (int it = 0; it < n_steps; ++it)
     {
-        if (threadIdx.x == 0)
-            (void)cost_grad_lane(A, x0, s_u, s_g, s_buf, 1);
-        __syncthreads();
-        for (int t = threadIdx.x; t < T; t += blockDim.x)
+        // A: forward pass, checkpoints only
+        if (tid == 0)
         {
-            float in[4], out[4];
-            for (int d = 0; d < 4; ++d)
-                in[d] = fma_(-step, s_g[t * 4 + d], s_u[t * 4 + d]);
+            QState s = x0;
+            NextGate ng;
+            ng.load(s_track.gates, s_track.n_gates, s.gate);
+            int left = 0;
+            float4 un = s_u[0];
+            for (int t = 0; t < T; ++t)
+            {
+                const float4 u = un; // the next step's control is loaded one step ahead
+                if (t + 1 < T)
+                    un = s_u[t + 1];
+                if (left == 0)
+                {
+                    put_state(s_x, t, 1, s, 0.0f);
+                    left = ck;
+                }
+                --left;
+                ref_step(C, s_track, margin, ng, s, u);
+            }
+            s_tend = T - 1;
+        }
+        __syncthreads();
+        // B: states between checkpoints, one lane per segment
+        if (tid < n_ck)
+        {
+            const int t0 = tid * ck, t1 = min(t0 + ck, T);
+            float w;
+            QState s = get_state(s_x, t0, 1, &w);
+            NextGate ng;
+            ng.load(s_track.gates, s_track.n_gates, s.gate);
+            float4 un = s_u[t0];
+            for (int t = t0; t < t1; ++t)
+            {
+                const float4 u = un;
+                if (t + 1 < t1)
+                    un = s_u[t + 1];
+                ref_step(C, s_track, margin, ng, s, u);
+                if (t + 1 < t1 || t1 == T)
+                    put_state(s_x, t + 1, 1, s, 0.0f);
+            }
+        }
+        __syncthreads();
+        // C: terminal test and running-cost gradient, one lane per step
+        for (int t = tid; t < T; t += REF_BLOCK)
+        {
+            float w;
+            const QState s = get_state(s_x, t + 1, 1, &w);
+            const float tau = C.dt * (float)(t + 1);
+            const auto opp_at = [&](int j) {
+                const float4 p = s_op[j], v = s_ov[j];
+                return make_float4(fma_(v.x, tau, p.x), fma_(v.y, tau, p.y), fma_(v.z, tau, p.z), 0.0f);
+            };
+            int term;
+            (void)stage_cost_at(C, Q, KC, s_track, s, n_opp, opp_at, &term);
+            if (term != TERM_NONE)
+                atomicMin(&s_tend, t);
+            float g[3] = {0.0f, 0.0f, 0.0f};
+            stage_cost_grad_at(C, Q, KC, s_track, s, n_opp, opp_at, s_w[t], g);
+            s_gp[t] = make_float4(g[0], g[1], g[2], 0.0f);
+        }
+        __syncthreads();
+        const int te = s_tend;
+        const int R = (te + REF_SEGS) / REF_SEGS; // ceil((te + 1) / REF_SEGS)
+        const int n_seg = (te + R) / R;           // ceil((te + 1) / R)
+        if (tid == REF_BLOCK - 1)
+        {
+            // terminal adjoint: final-cost gradient, then the running cost of x_{te+1}
+            float w;
+            const QState s = get_state(s_x, te + 1, 1, &w);
+            const float tau = C.dt * (float)(te + 1);
+            const auto opp_at = [&](int j) {
+                const float4 p = s_op[j], v = s_ov[j];
+                return make_float4(fma_(v.x, tau, p.x), fma_(v.y, tau, p.y), fma_(v.z, tau, p.z), 0.0f);
+            };
+            float g[3] = {0.0f, 0.0f, 0.0f};
+            final_cost_grad(KC, s_track, s, g);
+            stage_cost_grad_at(C, Q, KC, s_track, s, n_opp, opp_at, s_w[te], g);
+            float* lam = s_lam + (n_seg - 1) * 16;
+            lam[0] = g[0];
+            lam[1] = g[1];
+            lam[2] = g[2];
+            for (int i = 3; i < 13; ++i)
+                lam[i] = 0.0f;
+        }
+        // D: per reverse segment, the affine map lam_a = Psi lam_b + psi
+        if (tid < REF_COMP * n_seg)
+        {
+            const int seg = tid / REF_COMP, comp = tid - seg * REF_COMP;
+            const int a = seg * R, b = min(a + R, te + 1);
+            float f[13];
+#pragma unroll
+            for (int i = 0; i < 13; ++i)
+                f[i] = (i + 1 == comp) ? 1.0f : 0.0f;
+            QGrad lam = grad_from_flat(f);
+            RefRow nx = RefRow::load(s_x, s_u, b - 1);
+            for (int t = b - 1; t >= a; --t)
+            {
+                const RefRow r = nx; // the previous step's row is loaded one step ahead
+                if (t > a)
+                    nx = RefRow::load(s_x, s_u, t - 1);
+                const QState x = r.state();
+                const float uin[4] = {r.u.x, r.u.y, r.u.z, r.u.w};
+                float uc[4], gu[4];
+                clamp_control(C, uin, uc);
+                quad_step_vjp(C, x, uc, lam, gu);
+                if (comp == 0 && t >= 1)
+                {
+                    const float4 gp = s_gp[t - 1];
+                    lam.p[0] = lam.p[0] + gp.x;
+                    lam.p[1] = lam.p[1] + gp.y;
+                    lam.p[2] = lam.p[2] + gp.z;
+                }
+            }
+            grad_to_flat(lam, f);
+            float* out = s_psi + (seg * REF_COMP + comp) * 16;
+#pragma unroll
+            for (int i = 0; i < 13; ++i)
+                out[i] = f[i];
+        }
+        __syncthreads();
+        // E: lam_b of every segment, last to first (wave 0; lane i = component i)
+        if (tid < 64)
+        {
+            const bool on = tid < 13;
+            float lam = on ? s_lam[(n_seg - 1) * 16 + tid] : 0.0f;
+            for (int sg = n_seg - 1; sg >= 1; --sg)
+            {
+                const float* P = s_psi + sg * REF_COMP * 16;
+                float acc = on ? P[tid] : 0.0f;
+#pragma unroll
+                for (int j = 0; j < 13; ++j)
+                {
+                    const float lj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lam), j));
+                    const float pij = on ? P[(1 + j) * 16 + tid] : 0.0f;
+                    acc = fma_(pij, lj, acc);
+                }
+                lam = acc;
+                if (on)
+                    s_lam[(sg - 1) * 16 + tid] = lam;
+            }
+        }
+        __syncthreads();
+        // F: exact chain per segment, gradient and projected step of u_t
+        if (tid < n_seg)
+        {
+            const int a = tid * R, b = min(a + R, te + 1);
+            float f[13];
+#pragma unroll
+            for (int i = 0; i < 13; ++i)
+                f[i] = s_lam[tid * 16 + i];
+            QGrad lam = grad_from_flat(f);
+            RefRow nx = RefRow::load(s_x, s_u, b - 1);
+            for (int t = b - 1; t >= a; --t)
+            {
+                const RefRow r = nx;
+                if (t > a)
+                    nx = RefRow::load(s_x, s_u, t - 1);
+                const QState x = r.state();
+                const float uin[4] = {r.u.x, r.u.y, r.u.z, r.u.w};
+                float uc[4], gu[4];
+                clamp_control(C, uin, uc);
+                quad_step_vjp(C, x, uc, lam, gu);
+                const float G[4] = {gu[0] * clamp_grad(uin[0], C.thrust_min, C.thrust_max),
+                                    gu[1] * clamp_grad(uin[1], -C.rate_max, C.rate_max),
+                                    gu[2] * clamp_grad(uin[2], -C.rate_max, C.rate_max),
+                                    gu[3] * clamp_grad(uin[3], -C.rate_max, C.rate_max)};
+                float in[4], out[4];
+                for (int d = 0; d < 4; ++d)
+                    in[d] = fma_(-step, G[d], uin[d]);
+                clamp_control(C, in, out);
+                s_u[t] = make_float4(out[0], out[1], out[2], out[3]);
+                if (t >= 1)
+                {
+                    const float4 gp = s_gp[t - 1];
+                    lam.p[0] = lam.p[0] + gp.x;
+                    lam.p[1] = lam.p[1] + gp.y;
+                    lam.p[2] = lam.p[2] + gp.z;
+                }
+            }
+        }
+        for (int t = te + 1 + tid; t < T; t += REF_BLOCK)
+        {
+            const float4 u4 = s_u[t];
+            const float in[4] = {u4.x, u4.y, u4.z, u4.w};
+            float out[4];
             clamp_control(C, in, out);
-            for (int d = 0; d < 4; ++d)
-                s_u[t * 4 + d] = out[d];
+            s_u[t] = make_float4(out[0], out[1], out[2], out[3]);
         }
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < T * 4; i += blockDim.x)
-        nominal[I.nom_off + i] = s_u[i];
+    for (int t = tid; t < T; t += REF_BLOCK)
+    {
+        const float4 u4 = s_u[t];
+        float* u = nominal + I.nom_off + t * 4;
+        u[0] = u4.x;
+        u[1] = u4.y;
+        u[2] = u4.z;
+        u[3] = u4.w;
+    }
 }
 
 // Weighted average over one chunk of samples [k0, k0 + n) and timesteps
@@ -1000,12 +1296,10 @@ hipError_t launch_costgrad(int lanes, hipStream_t st, const InstDev* insts, int 
     return hipGetLastError();
 }
 
-hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, int max_T, int max_slots, const dr_track* tracks,
-                         const dr_drone_state* drones, float* nominal, int n_steps, float step, int ck, const QuadConst& C,
-                         const CostConst& Q, const dr_cost_cfg& KC, int opp_stride)
+hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, const dr_track* tracks, const dr_drone_state* drones,
+                         float* nominal, int n_steps, float step, int ck, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC)
 {
-    const size_t shm = ((size_t)max_T * opp_stride + (size_t)max_slots * 4) * sizeof(float4);
-    k_refine<<<n_inst, 64, shm, st>>>(insts, tracks, drones, nominal, n_steps, step, ck, C, Q, KC, opp_stride);
+    k_refine<<<n_inst, REF_BLOCK, 0, st>>>(insts, tracks, drones, nominal, n_steps, step, ck, C, Q, KC);
     return hipGetLastError();
 }
 

@@ -231,14 +231,15 @@ DR_HD float advance(const dr_gate* gates, int n_gates, const QState& s)
 // opp holds the opponents' predicted positions at this step (x, y, z, -).
 // An opponent farther than sqrt(far2) cannot bring bd below the penalty band,
 // so its distance is taken as +inf without a square root.
-DR_HD float stage_cost(const QuadConst& C, const CostConst& Q, const dr_cost_cfg& K, const dr_track& tr, const QState& s,
-                       const float4* opp, int n_opp, int* term)
+template <class OppAt>
+DR_HD float stage_cost_at(const QuadConst& C, const CostConst& Q, const dr_cost_cfg& K, const dr_track& tr, const QState& s, int n_opp,
+                          OppAt opp_at, int* term)
 {
     float cost = 0.0f;
     float min_sq = __builtin_inff();
     for (int j = 0; j < n_opp; ++j)
     {
-        const float4 o = opp[j];
+        const float4 o = opp_at(j);
         const float dx = s.p[0] - o.x;
         const float dy = s.p[1] - o.y;
         const float dz = s.p[2] - o.z;
@@ -273,6 +274,12 @@ DR_HD float stage_cost(const QuadConst& C, const CostConst& Q, const dr_cost_cfg
     }
     *term = t;
     return cost;
+}
+
+DR_HD float stage_cost(const QuadConst& C, const CostConst& Q, const dr_cost_cfg& K, const dr_track& tr, const QState& s,
+                       const float4* opp, int n_opp, int* term)
+{
+    return stage_cost_at(C, Q, K, tr, s, n_opp, [opp](int j) { return opp[j]; }, term);
 }
 
 // PR-MPPI running cost and safety on the post-step state (costs.cuh:44-63

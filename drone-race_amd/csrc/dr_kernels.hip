@@ -678,16 +678,18 @@ __device__ __forceinline__ QGrad grad_from_flat(const float f[13])
     return g;
 }
 
-// State x_t and control u_t of one step, as read from the tables.
+// State x_t, control u_t and the running-cost gradient of x_t (row t - 1 of
+// the gradient table; row 0 at t = 0, where it is not used) of one step.
 struct RefRow
 {
-    float4 x[4], u;
-    __device__ static RefRow load(const float4* xs, const float4* us, int t)
+    float4 x[4], u, gp;
+    __device__ static RefRow load(const float4* xs, const float4* us, const float4* gps, int t)
     {
         RefRow r;
         for (int k = 0; k < 4; ++k)
             r.x[k] = xs[t * 4 + k];
         r.u = us[t];
+        r.gp = gps[t > 0 ? t - 1 : 0];
         return r;
     }
     __device__ QState state() const
@@ -701,15 +703,70 @@ struct RefRow
     }
 };
 
-// One step of the forward pass (clamp, dynamics, gate counter).
-__device__ __forceinline__ void ref_step(const QuadConst& C, const dr_track& tr, float margin, NextGate& ng, QState& s, float4 u4)
+// Gate k of a register-held gate table (center, normal, radius), by selects
+// over the whole table so that the table stays in registers.
+__device__ __forceinline__ void pick_gate(const float (&gv)[DR_MAX_GATES][7], int k, dr_gate& g)
 {
-    const float uin[4] = {u4.x, u4.y, u4.z, u4.w};
-    float uc[4];
-    clamp_control(C, uin, uc);
-    const float p0[3] = {s.p[0], s.p[1], s.p[2]};
-    quad_step(C, s, uc);
-    ng.update(tr.gates, tr.n_gates, margin, p0, s);
+    float v[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+        v[i] = gv[0][i];
+#pragma unroll
+    for (int j = 1; j < DR_MAX_GATES; ++j)
+#pragma unroll
+        for (int i = 0; i < 7; ++i)
+            v[i] = j == k ? gv[j][i] : v[i];
+    g.center[0] = v[0], g.center[1] = v[1], g.center[2] = v[2];
+    g.normal[0] = v[3], g.normal[1] = v[4], g.normal[2] = v[5];
+    g.radius = v[6];
+}
+
+// Steps [t0, t1) of the forward pass from s (clamp, dynamics, gate counter).
+// The state after step t goes to xs[t + 1] when it is kept (every `every`-th
+// step, and only for t + 1 < stop) and to the junk slot otherwise, and the
+// next control is loaded one step ahead: every LDS access of the loop is
+// issued on every step and none is waited for before its use.  The track's
+// gates are held in registers; the next gate is selected from them when one
+// is crossed.
+__device__ void ref_forward(const QuadConst& C, const dr_track& tr, float margin, const float4* us, float4* xs, float4* junk,
+                            int t0, int t1, int every, int stop, QState& s)
+{
+    float gv[DR_MAX_GATES][7]; // center, normal, radius
+#pragma unroll
+    for (int k = 0; k < DR_MAX_GATES; ++k)
+    {
+        const dr_gate& src = tr.gates[k];
+        gv[k][0] = src.center[0], gv[k][1] = src.center[1], gv[k][2] = src.center[2];
+        gv[k][3] = src.normal[0], gv[k][4] = src.normal[1], gv[k][5] = src.normal[2];
+        gv[k][6] = src.radius;
+    }
+    const int n_gates = tr.n_gates;
+    dr_gate g;
+    g._pad = 0.0f;
+    int nx = s.gate + 1 == n_gates ? 0 : s.gate + 1;
+    pick_gate(gv, nx, g);
+    float4 u = us[t0];
+    int left = every;
+    for (int t = t0; t < t1; ++t)
+    {
+        const float4 un = us[t + 1 < t1 ? t + 1 : t];
+        const float uin[4] = {u.x, u.y, u.z, u.w};
+        float uc[4];
+        clamp_control(C, uin, uc);
+        const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+        quad_step(C, s, uc);
+        cross_gate(g, nx, margin, p0, s);
+        if (s.gate == nx)
+        {
+            nx = s.gate + 1 == n_gates ? 0 : s.gate + 1;
+            pick_gate(gv, nx, g);
+        }
+        --left;
+        const bool keep = left == 0 && t + 1 < stop;
+        left = left == 0 ? every : left;
+        put_state(keep ? xs + (t + 1) * 4 : junk, 0, 1, s, 0.0f);
+        u = un;
+    }
 }
 
 __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict__ insts, const dr_track* __restrict__ tracks,
@@ -724,6 +781,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     __shared__ float s_psi[REF_SEGS * REF_COMP * 16];
     __shared__ float s_lam[REF_SEGS * 16];     // lam_b of each reverse segment
     __shared__ float4 s_op[MAX_OPP], s_ov[MAX_OPP];
+    __shared__ float4 s_junk[16]; // sink of the forward pass's unkept states
     __shared__ int s_tend;
 
     const InstDev& I = insts[blockIdx.x];
@@ -766,23 +824,8 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         if (tid == 0)
         {
             QState s = x0;
-            NextGate ng;
-            ng.load(s_track.gates, s_track.n_gates, s.gate);
-            int left = 0;
-            float4 un = s_u[0];
-            for (int t = 0; t < T; ++t)
-            {
-                const float4 u = un; // the next step's control is loaded one step ahead
-                if (t + 1 < T)
-                    un = s_u[t + 1];
-                if (left == 0)
-                {
-                    put_state(s_x, t, 1, s, 0.0f);
-                    left = ck;
-                }
-                --left;
-                ref_step(C, s_track, margin, ng, s, u);
-            }
+            put_state(s_x, 0, 1, s, 0.0f);
+            ref_forward(C, s_track, margin, s_u, s_x, s_junk, 0, T, ck, T + 1, s);
             s_tend = T - 1;
         }
         __syncthreads();
@@ -792,18 +835,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             const int t0 = tid * ck, t1 = min(t0 + ck, T);
             float w;
             QState s = get_state(s_x, t0, 1, &w);
-            NextGate ng;
-            ng.load(s_track.gates, s_track.n_gates, s.gate);
-            float4 un = s_u[t0];
-            for (int t = t0; t < t1; ++t)
-            {
-                const float4 u = un;
-                if (t + 1 < t1)
-                    un = s_u[t + 1];
-                ref_step(C, s_track, margin, ng, s, u);
-                if (t + 1 < t1 || t1 == T)
-                    put_state(s_x, t + 1, 1, s, 0.0f);
-            }
+            ref_forward(C, s_track, margin, s_u, s_x, s_junk + 4 * (tid & 3), t0, t1, 1, t1 == T ? T + 1 : t1, s);
         }
         __syncthreads();
         // C: terminal test and running-cost gradient, one lane per step
@@ -858,24 +890,20 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             for (int i = 0; i < 13; ++i)
                 f[i] = (i + 1 == comp) ? 1.0f : 0.0f;
             QGrad lam = grad_from_flat(f);
-            RefRow nx = RefRow::load(s_x, s_u, b - 1);
+            RefRow nx = RefRow::load(s_x, s_u, s_gp, b - 1);
             for (int t = b - 1; t >= a; --t)
             {
-                const RefRow r = nx; // the previous step's row is loaded one step ahead
-                if (t > a)
-                    nx = RefRow::load(s_x, s_u, t - 1);
+                const RefRow r = nx; // the previous step's row is loaded one step ahead, unconditionally
+                nx = RefRow::load(s_x, s_u, s_gp, t > a ? t - 1 : t);
                 const QState x = r.state();
                 const float uin[4] = {r.u.x, r.u.y, r.u.z, r.u.w};
                 float uc[4], gu[4];
                 clamp_control(C, uin, uc);
                 quad_step_vjp(C, x, uc, lam, gu);
-                if (comp == 0 && t >= 1)
-                {
-                    const float4 gp = s_gp[t - 1];
-                    lam.p[0] = lam.p[0] + gp.x;
-                    lam.p[1] = lam.p[1] + gp.y;
-                    lam.p[2] = lam.p[2] + gp.z;
-                }
+                const bool add = comp == 0 && t >= 1;
+                lam.p[0] = add ? lam.p[0] + r.gp.x : lam.p[0];
+                lam.p[1] = add ? lam.p[1] + r.gp.y : lam.p[1];
+                lam.p[2] = add ? lam.p[2] + r.gp.z : lam.p[2];
             }
             grad_to_flat(lam, f);
             float* out = s_psi + (seg * REF_COMP + comp) * 16;
@@ -887,21 +915,26 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         // E: lam_b of every segment, last to first (wave 0; lane i = component i)
         if (tid < 64)
         {
-            const bool on = tid < 13;
-            float lam = on ? s_lam[(n_seg - 1) * 16 + tid] : 0.0f;
+            // lanes 13..63 compute copies of rows 0..15 (row 13..15: unused
+            // padding) and are never read; every load is unconditional
+            const int i = tid & 15;
+            float lam = s_lam[(n_seg - 1) * 16 + i];
             for (int sg = n_seg - 1; sg >= 1; --sg)
             {
                 const float* P = s_psi + sg * REF_COMP * 16;
-                float acc = on ? P[tid] : 0.0f;
+                float pc[REF_COMP];
+#pragma unroll
+                for (int c = 0; c < REF_COMP; ++c)
+                    pc[c] = P[c * 16 + i];
+                float acc = pc[0];
 #pragma unroll
                 for (int j = 0; j < 13; ++j)
                 {
                     const float lj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lam), j));
-                    const float pij = on ? P[(1 + j) * 16 + tid] : 0.0f;
-                    acc = fma_(pij, lj, acc);
+                    acc = fma_(pc[1 + j], lj, acc);
                 }
                 lam = acc;
-                if (on)
+                if (tid < 13)
                     s_lam[(sg - 1) * 16 + tid] = lam;
             }
         }
@@ -915,12 +948,11 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             for (int i = 0; i < 13; ++i)
                 f[i] = s_lam[tid * 16 + i];
             QGrad lam = grad_from_flat(f);
-            RefRow nx = RefRow::load(s_x, s_u, b - 1);
+            RefRow nx = RefRow::load(s_x, s_u, s_gp, b - 1);
             for (int t = b - 1; t >= a; --t)
             {
                 const RefRow r = nx;
-                if (t > a)
-                    nx = RefRow::load(s_x, s_u, t - 1);
+                nx = RefRow::load(s_x, s_u, s_gp, t > a ? t - 1 : t);
                 const QState x = r.state();
                 const float uin[4] = {r.u.x, r.u.y, r.u.z, r.u.w};
                 float uc[4], gu[4];
@@ -935,13 +967,10 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                     in[d] = fma_(-step, G[d], uin[d]);
                 clamp_control(C, in, out);
                 s_u[t] = make_float4(out[0], out[1], out[2], out[3]);
-                if (t >= 1)
-                {
-                    const float4 gp = s_gp[t - 1];
-                    lam.p[0] = lam.p[0] + gp.x;
-                    lam.p[1] = lam.p[1] + gp.y;
-                    lam.p[2] = lam.p[2] + gp.z;
-                }
+                const bool add = t >= 1;
+                lam.p[0] = add ? lam.p[0] + r.gp.x : lam.p[0];
+                lam.p[1] = add ? lam.p[1] + r.gp.y : lam.p[1];
+                lam.p[2] = add ? lam.p[2] + r.gp.z : lam.p[2];
             }
         }
         for (int t = te + 1 + tid; t < T; t += REF_BLOCK)

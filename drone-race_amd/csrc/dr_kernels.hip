@@ -630,18 +630,21 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
 
 // Adjoint refinement of each instance's updated nominal: n_steps projected
 // gradient steps U <- clamp(U - step * dJ/dU), one workgroup of REF_BLOCK
-// lanes per instance, everything in LDS.  J and dJ/dU are those of
-// cost_grad_lane (same forward, same terminal rule, same vector-Jacobian
-// products); the work is spread over the lanes so that only the forward
-// recurrence of the dynamics stays serial:
+// lanes per instance, everything in LDS.  Every iterate's J is evaluated
+// (one more forward pass after the last step) and the lowest-J sequence is
+// kept, the plain MPPI update (iterate 0) on ties: a step that jumps over a
+// discontinuity of the cost (a lost gate, a collision) is not taken.  J and
+// dJ/dU are those of cost_grad_lane (same forward, same terminal rule, same
+// vector-Jacobian products); the work is spread over the lanes so that only
+// the forward recurrence of the dynamics stays serial:
 //  A  lane 0 runs the dynamics and the gate counter over all T steps and
 //     writes the state at every ck-th step (the checkpoints) into the state
 //     table; no cost is evaluated here.
 //  B  one lane per checkpoint segment recomputes the segment's states from
 //     its checkpoint into the table.
-//  C  one lane per step t: terminal test and dJ/dp of the running cost of
-//     the post-step state x_{t+1}; the first terminal step t_end ends the
-//     trajectory (cost_grad_lane's break).
+//  C  one lane per step t: terminal test, discounted running cost and dJ/dp
+//     of the running cost of the post-step state x_{t+1}; the first terminal
+//     step t_end ends the trajectory (cost_grad_lane's break); wave 0 sums J.
 //  D  the reverse recurrence lam_t = A_t^T lam_{t+1} + g_t is linear in lam.
 //     The steps 0..t_end are cut into n_seg <= REF_SEGS reverse segments
 //     [a, b); per segment, 14 lanes carry lam_b = 0 with the g_t terms
@@ -653,8 +656,8 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
 //  F  one lane per reverse segment carries its exact lam_b down the segment
 //     and writes dJ/du_t and the projected step of u_t; steps after t_end
 //     have dJ/du = 0 (u_t is only clamped).
-// The order of the additions in D-F differs from the one-lane sweep, so the
-// gradient agrees with cost_grad_lane to rounding, not bit for bit.
+// The order of the additions in C-F differs from the one-lane sweep, so J
+// and the gradient agree with cost_grad_lane to rounding, not bit for bit.
 constexpr int REF_BLOCK = 256;
 constexpr int REF_SEGS = 16; // 14 * REF_SEGS <= REF_BLOCK
 constexpr int REF_COMP = 14; // affine part + 13 state directions
@@ -782,7 +785,9 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     __shared__ float s_lam[REF_SEGS * 16];     // lam_b of each reverse segment
     __shared__ float4 s_op[MAX_OPP], s_ov[MAX_OPP];
     __shared__ float4 s_junk[16]; // sink of the forward pass's unkept states
-    __shared__ int s_tend;
+    __shared__ float4 s_ubest[DR_MAX_T]; // the lowest-cost sequence evaluated so far
+    __shared__ float s_jbest;
+    __shared__ int s_tend, s_better;
 
     const InstDev& I = insts[blockIdx.x];
     const int T = I.T, n_opp = I.n_opp, tid = threadIdx.x;
@@ -818,7 +823,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     const int n_ck = (T + ck - 1) / ck;
     __syncthreads();
 
-    for (int it = 0; it < n_steps; ++it)
+    for (int it = 0;; ++it)
     {
         // A: forward pass, checkpoints only
         if (tid == 0)
@@ -849,17 +854,43 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                 return make_float4(fma_(v.x, tau, p.x), fma_(v.y, tau, p.y), fma_(v.z, tau, p.z), 0.0f);
             };
             int term;
-            (void)stage_cost_at(C, Q, KC, s_track, s, n_opp, opp_at, &term);
+            const float c = stage_cost_at(C, Q, KC, s_track, s, n_opp, opp_at, &term);
             if (term != TERM_NONE)
                 atomicMin(&s_tend, t);
             float g[3] = {0.0f, 0.0f, 0.0f};
             stage_cost_grad_at(C, Q, KC, s_track, s, n_opp, opp_at, s_w[t], g);
-            s_gp[t] = make_float4(g[0], g[1], g[2], 0.0f);
+            s_gp[t] = make_float4(g[0], g[1], g[2], c * s_w[t]); // .w: discounted running cost
         }
         __syncthreads();
         const int te = s_tend;
         const int R = (te + REF_SEGS) / REF_SEGS; // ceil((te + 1) / REF_SEGS)
         const int n_seg = (te + R) / R;           // ceil((te + 1) / R)
+        if (tid < 64)
+        {
+            // J of this sequence (running costs up to te summed in FP64, then
+            // the terminal cost); the plain MPPI update (pass 0) is the
+            // default, a gradient step is kept only if its J is lower
+            double acc = 0.0;
+            for (int t = tid; t <= te; t += 64)
+                acc += (double)s_gp[t].w;
+            acc = wave_sum(acc);
+            if (tid == 0)
+            {
+                float w;
+                const QState s = get_state(s_x, te + 1, 1, &w);
+                const float J = (float)(acc + (double)final_cost(KC, s_track, s));
+                const bool better = it == 0 || J < s_jbest;
+                if (better)
+                    s_jbest = J;
+                s_better = better;
+            }
+        }
+        __syncthreads();
+        if (s_better)
+            for (int t = tid; t < T; t += REF_BLOCK)
+                s_ubest[t] = s_u[t];
+        if (it == n_steps)
+            break;
         if (tid == REF_BLOCK - 1)
         {
             // terminal adjoint: final-cost gradient, then the running cost of x_{te+1}
@@ -983,9 +1014,10 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         }
         __syncthreads();
     }
+    __syncthreads();
     for (int t = tid; t < T; t += REF_BLOCK)
     {
-        const float4 u4 = s_u[t];
+        const float4 u4 = s_ubest[t];
         float* u = nominal + I.nom_off + t * 4;
         u[0] = u4.x;
         u[1] = u4.y;

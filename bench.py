@@ -466,6 +466,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-batch64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint-refined vs plain section (for profiling)")
     ap.add_argument("--no-pr", action="store_true")
     ap.add_argument("--pr-only", action="store_true", help="run only the PR-MPPI section (for profiling)")
     ap.add_argument("--no-sr", action="store_true")
@@ -515,6 +516,9 @@ def main():
         return
     if args.pr_only:
         print(json.dumps({"pr_mppi": pr_mppi(pkg, steps=args.steps, warmup=args.warmup)}), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_vs_plain(pkg, steps=min(args.steps, 100), cost_steps=20)}), flush=True)
         return
     scn = {"c0": sc.c0_single, "c1": sc.c1_four_drones, "c2": sc.c2_batch64}[args.config]()
     n_inst = len(scn.instances)

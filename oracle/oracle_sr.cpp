@@ -2532,6 +2532,23 @@ int or_sr_predict(const dr_sr_env* env, const dr_sr_state* x0, int theta, const 
     return 0;
 }
 
+// Opponent target lateral offset and speed goal (computeOppTargetLatDist,
+// env_stratrace.h:318-461) at st under theta.
+double or_sr_opp_target_goal(int f64, const dr_sr_env* env, int theta, const dr_sr_state* st, double* goal)
+{
+    if (f64)
+    {
+        double lat, g;
+        opp_target(Env<double>(*env), env->opp[theta], load<double>(*st), &lat, &g);
+        *goal = g;
+        return lat;
+    }
+    float lat, g;
+    opp_target(Env<float>(*env), env->opp[theta], load<float>(*st), &lat, &g);
+    *goal = g;
+    return lat;
+}
+
 // Opponent target lateral offset (computeOppTargetLatDist, env_stratrace.h:318-461) at st under theta.
 double or_sr_opp_target(int f64, const dr_sr_env* env, int theta, const dr_sr_state* st)
 {

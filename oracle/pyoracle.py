@@ -78,6 +78,8 @@ def lib(abi):
     L.or_sr_predict.argtypes = [C.POINTER(abi.SrEnv), C.POINTER(abi.SrState), I, F, I, I, F, C.c_float, F, F, C.POINTER(I)]
     L.or_sr_opp_target.restype = C.c_double
     L.or_sr_opp_target.argtypes = [I, C.POINTER(abi.SrEnv), I, C.POINTER(abi.SrState)]
+    L.or_sr_opp_target_goal.restype = C.c_double
+    L.or_sr_opp_target_goal.argtypes = [I, C.POINTER(abi.SrEnv), I, C.POINTER(abi.SrState), C.POINTER(C.c_double)]
     L.or_ho_prim.restype = C.c_double
     L.or_ho_prim.argtypes = [I, I, D]
     L.or_ho_eval.restype = C.c_double
@@ -492,6 +494,12 @@ class Oracle:
 
     def sr_opp_target(self, f64, env, theta, state):
         return self.L.or_sr_opp_target(int(f64), C.byref(env), theta, C.byref(state))
+
+    def sr_opp_target_goal(self, f64, env, theta, state):
+        """(target lateral offset, speed goal) of computeOppTargetLatDist."""
+        goal = C.c_double()
+        lat = self.L.or_sr_opp_target_goal(int(f64), C.byref(env), theta, C.byref(state), C.byref(goal))
+        return lat, goal.value
 
     # ---------------------------------------------------------- HiddenObs
 

@@ -174,3 +174,44 @@ def test_refined_step_matches_oracle(pkg, oracle, step, kept):
     best, _ = check_refined(eng, act, Us, Js)
     assert best == kept
     eng.close()
+
+
+@pytest.mark.parametrize("T,ck,collide", [(37, 7, "mid"), (50, 1, "none"), (40, 64, "mid"), (256, 16, "none"), (30, 10, "start")])
+def test_refined_step_edge_cases(pkg, oracle, T, ck, collide):
+    """The refinement at horizons that are not a multiple of the checkpoint
+    interval, at an interval of 1 and one longer than T, at the largest
+    horizon, and on trajectories that end in a collision mid-horizon (an
+    opponent closing head-on) or at the first step (an opponent 0.3 m away):
+    the reverse segments then cover fewer steps than the horizon."""
+    say(f"case: refined step T={T} ck={ck} collision={collide}")
+    sc = pkg.scenarios
+    K = 1024
+    scn, opp = setup(pkg, T)
+    if collide != "none":
+        d0, d1 = scn.drones[0], scn.drones[1]
+        gap = 0.3 if collide == "start" else 1.2
+        d1.p[0], d1.p[1], d1.p[2] = d0.p[0] + gap, d0.p[1], d0.p[2]
+        d1.v[0], d1.v[1], d1.v[2] = (-3.0 if collide == "mid" else 0.0), 0.0, 0.0
+        for c in range(3):
+            opp[0].p[c], opp[0].v[c] = d1.p[c], d1.v[c]
+    scn.instances[0] = sc.instance(K, T, 0, 0, 23)
+    eng = pkg.Engine(scn)
+    eng.set_refine(3, 0.05, ck)
+    nom0 = eng.get_nominal(0).copy()
+    act = eng.step(1)
+    ins = scn.instances[0]
+    costs, noise, _ = oracle.rollout(0, scn.params, scn.cost, scn.tracks[0], scn.drones[0], opp, nom0, K, T, ins.seed, 0,
+                                     np.array(ins.sigma[:], np.float32))
+    upd = oracle.update(0, scn.params, costs, noise, nom0, ins.lambda_, np.array(ins.shift_fill[:], np.float32))
+    U0 = upd["updated"].astype(np.float32)
+    _, G, _ = oracle.cost_grad(0, scn.params, scn.cost, scn.tracks[0], scn.drones[0], opp, U0[None], ck=ck)
+    live = np.nonzero(np.abs(G[0]).sum(axis=1))[0]
+    last = int(live.max()) if live.size else -1
+    say(f"  last step with a nonzero gradient: {last} of {T - 1}")
+    if collide == "mid":
+        assert 0 <= last < T - 8, "the trajectory ends in the collision"
+    if collide == "start":
+        assert last < 1
+    Us, Js = refine_oracle(oracle, scn, opp, U0, 3, 0.05, ck=ck)
+    check_refined(eng, act, Us, Js)
+    eng.close()

@@ -706,24 +706,6 @@ struct RefRow
     }
 };
 
-// Gate k of a register-held gate table (center, normal, radius), by selects
-// over the whole table so that the table stays in registers.
-__device__ __forceinline__ void pick_gate(const float (&gv)[DR_MAX_GATES][7], int k, dr_gate& g)
-{
-    float v[7];
-#pragma unroll
-    for (int i = 0; i < 7; ++i)
-        v[i] = gv[0][i];
-#pragma unroll
-    for (int j = 1; j < DR_MAX_GATES; ++j)
-#pragma unroll
-        for (int i = 0; i < 7; ++i)
-            v[i] = j == k ? gv[j][i] : v[i];
-    g.center[0] = v[0], g.center[1] = v[1], g.center[2] = v[2];
-    g.normal[0] = v[3], g.normal[1] = v[4], g.normal[2] = v[5];
-    g.radius = v[6];
-}
-
 // Steps [t0, t1) of the forward pass from s (clamp, dynamics, gate counter).
 // The state after step t goes to xs[t + 1] when it is kept (every `every`-th
 // step, and only for t + 1 < stop) and to the junk slot otherwise, and the
@@ -734,20 +716,8 @@ __device__ __forceinline__ void pick_gate(const float (&gv)[DR_MAX_GATES][7], in
 __device__ void ref_forward(const QuadConst& C, const dr_track& tr, float margin, const float4* us, float4* xs, float4* junk,
                             int t0, int t1, int every, int stop, QState& s)
 {
-    float gv[DR_MAX_GATES][7]; // center, normal, radius
-#pragma unroll
-    for (int k = 0; k < DR_MAX_GATES; ++k)
-    {
-        const dr_gate& src = tr.gates[k];
-        gv[k][0] = src.center[0], gv[k][1] = src.center[1], gv[k][2] = src.center[2];
-        gv[k][3] = src.normal[0], gv[k][4] = src.normal[1], gv[k][5] = src.normal[2];
-        gv[k][6] = src.radius;
-    }
-    const int n_gates = tr.n_gates;
-    dr_gate g;
-    g._pad = 0.0f;
-    int nx = s.gate + 1 == n_gates ? 0 : s.gate + 1;
-    pick_gate(gv, nx, g);
+    GateRegs gr;
+    gr.init(tr, s.gate);
     float4 u = us[t0];
     int left = every;
     for (int t = t0; t < t1; ++t)
@@ -758,12 +728,7 @@ __device__ void ref_forward(const QuadConst& C, const dr_track& tr, float margin
         clamp_control(C, uin, uc);
         const float p0[3] = {s.p[0], s.p[1], s.p[2]};
         quad_step(C, s, uc);
-        cross_gate(g, nx, margin, p0, s);
-        if (s.gate == nx)
-        {
-            nx = s.gate + 1 == n_gates ? 0 : s.gate + 1;
-            pick_gate(gv, nx, g);
-        }
+        gr.step(margin, p0, s);
         --left;
         const bool keep = left == 0 && t + 1 < stop;
         left = left == 0 ? every : left;

@@ -377,6 +377,8 @@ __global__ __launch_bounds__(PR_WAVG_BLOCK) void k_pr_wavg(const PrInstDev* __re
 // when any model finds the chosen plan unsafe), the three temperature
 // updates (prmppi_controller.cu:328-346), and the plans for the next step:
 // unshifted (plan) and shifted with shift_fill (work).
+constexpr int UPD_CH = 8; // steps per chunk of the candidate rollouts' two-wave pipeline
+
 __global__ __launch_bounds__(256) void k_pr_update(const PrInstDev* __restrict__ insts, PrDyn* __restrict__ dyn,
                                                    const double* __restrict__ part, const double* __restrict__ den,
                                                    float* __restrict__ work, float* __restrict__ plan, float* __restrict__ cand,
@@ -386,6 +388,7 @@ __global__ __launch_bounds__(256) void k_pr_update(const PrInstDev* __restrict__
     __shared__ PrUpdShared sh;
     __shared__ float4 s_pv[DR_MAX_MODELS][MAX_OPP][2];
     __shared__ dr_track s_track;
+    __shared__ float4 s_ring[2][UPD_CH][2 * DR_MAX_MODELS]; // position, gate | laps << 8
 
     const int inst = blockIdx.x;
     const PrInstDev& I = insts[inst];
@@ -409,60 +412,101 @@ __global__ __launch_bounds__(256) void k_pr_update(const PrInstDev* __restrict__
     pr_candidates(I, part, work, sh, tid);
     __syncthreads();
 
-    if (tid < 2 * M)
+    // Noise-free rollouts, one (candidate, model) per lane l < 2M, in two
+    // waves joined by a double-buffered LDS ring of UPD_CH steps: lane l of
+    // wave 0 integrates the dynamics and the gate counter (the latency-bound
+    // recurrence; the next control in a register) and writes each step's
+    // position, gate and laps (the track's gates in registers); lane l of
+    // wave 1 evaluates the running cost,
+    // safety and terminal test of the previous chunk with the model's
+    // opponents in registers.  One barrier per chunk.  The state integrates
+    // every step; the cost, the safety and the state the terminal cost reads
+    // are frozen at the first terminal step.
     {
-        // One latency-bound lane per (candidate, model): the model's
-        // opponents, the next gate and the next control are kept in
-        // registers so that no step waits on LDS.
-        const int which = tid / M, m = tid - which * M;
-        const float4* U = reinterpret_cast<const float4*>(sh.c[which]);
-        RegOpp ro;
-#pragma unroll
-        for (int j = 0; j < MAX_OPP; ++j)
-        {
-            ro.p[j] = j < n_opp ? s_pv[m][j][0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            ro.v[j] = j < n_opp ? s_pv[m][j][1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
+        const int n_lanes = 2 * M;
+        const int n_ch = (T + UPD_CH - 1) / UPD_CH;
+        const bool dyn_lane = tid < n_lanes;
+        const int cl = tid - WAVE;
+        const bool cost_lane = cl >= 0 && cl < n_lanes;
         QState s = state_from_abi(drones[I.drone]);
-        NextGate ng;
-        ng.load(s_track.gates, s_track.n_gates, s.gate);
-        float acc = 0.0f, decay = 1.0f, smax = -__builtin_inff();
-        // The state integrates every step; the cost, the safety and the state
-        // the terminal cost reads are frozen at the first terminal step, so
-        // the dynamics of step t+1 do not wait for the cost of step t.
-        QState fin = s;
-        bool stop = false;
-        float4 un = U[0];
-        for (int t = 0; t < T; ++t)
+        GateRegs gr;
+        const float4* U = nullptr;
+        float4 un = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (dyn_lane)
         {
-            const float4 u4 = un;
-            if (t + 1 < T)
-                un = U[t + 1];
-            float uin[4] = {u4.x + 0.0f, u4.y + 0.0f, u4.z + 0.0f, u4.w + 0.0f}, uc[4];
-            clamp_control(C, uin, uc);
-            const float p0[3] = {s.p[0], s.p[1], s.p[2]};
-            quad_step(C, s, uc);
-            ng.update(s_track.gates, s_track.n_gates, KC.gate_traversal_margin, p0, s);
-            ro.tau = C.dt * (float)(t + 1);
-            float sf;
-            int term;
-            const float c = pr_stage_cost(C, Q, KC, s_track, s, n_opp, ro, &sf, &term);
-            if (!stop)
+            U = reinterpret_cast<const float4*>(sh.c[tid / M]);
+            gr.init(s_track, s.gate);
+            un = U[0];
+        }
+        RegOpp ro;
+        if (cost_lane)
+        {
+            const int m = cl - (cl / M) * M;
+#pragma unroll
+            for (int j = 0; j < MAX_OPP; ++j)
             {
-                acc = fma_(c, decay, acc);
-                smax = fmaxf(smax, sf);
-                decay = decay * DECAY;
-                fin.p[0] = s.p[0];
-                fin.p[1] = s.p[1];
-                fin.p[2] = s.p[2];
-                fin.gate = s.gate;
-                fin.laps = s.laps;
-                stop = term != TERM_NONE;
+                ro.p[j] = j < n_opp ? s_pv[m][j][0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                ro.v[j] = j < n_opp ? s_pv[m][j][1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
         }
-        acc = acc + final_cost(KC, s_track, fin);
-        sh.cc[tid] = smax > 0.0f ? acc + I.safety_weight : acc;
-        sh.cs[tid] = smax;
+        float acc = 0.0f, decay = 1.0f, smax = -__builtin_inff();
+        QState fin = s;
+        bool stop = false;
+        for (int k = 0; k <= n_ch; ++k)
+        {
+            if (dyn_lane && k < n_ch)
+            {
+                const int t0 = k * UPD_CH, t1 = min(t0 + UPD_CH, T);
+                for (int t = t0; t < t1; ++t)
+                {
+                    const float4 u4 = un;
+                    un = U[t + 1 < T ? t + 1 : t]; // issued every step, so never waited for early
+                    float uin[4] = {u4.x + 0.0f, u4.y + 0.0f, u4.z + 0.0f, u4.w + 0.0f}, uc[4];
+                    clamp_control(C, uin, uc);
+                    const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+                    quad_step(C, s, uc);
+                    gr.step(KC.gate_traversal_margin, p0, s);
+                    s_ring[k & 1][t - t0][tid] = make_float4(s.p[0], s.p[1], s.p[2], u2f((uint32_t)s.gate | ((uint32_t)s.laps << 8)));
+                }
+            }
+            else if (cost_lane && k >= 1)
+            {
+                const int t0 = (k - 1) * UPD_CH, t1 = min(t0 + UPD_CH, T);
+                for (int t = t0; t < t1; ++t)
+                {
+                    const float4 r = s_ring[(k - 1) & 1][t - t0][cl];
+                    const uint32_t gl = f2u(r.w);
+                    s.p[0] = r.x;
+                    s.p[1] = r.y;
+                    s.p[2] = r.z;
+                    s.gate = (int)(gl & 0xffu);
+                    s.laps = (int)(gl >> 8);
+                    ro.tau = C.dt * (float)(t + 1);
+                    float sf;
+                    int term;
+                    const float c = pr_stage_cost(C, Q, KC, s_track, s, n_opp, ro, &sf, &term);
+                    if (!stop)
+                    {
+                        acc = fma_(c, decay, acc);
+                        smax = fmaxf(smax, sf);
+                        decay = decay * DECAY;
+                        fin.p[0] = s.p[0];
+                        fin.p[1] = s.p[1];
+                        fin.p[2] = s.p[2];
+                        fin.gate = s.gate;
+                        fin.laps = s.laps;
+                        stop = term != TERM_NONE;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (cost_lane)
+        {
+            acc = acc + final_cost(KC, s_track, fin);
+            sh.cc[cl] = smax > 0.0f ? acc + I.safety_weight : acc;
+            sh.cs[cl] = smax;
+        }
     }
     __syncthreads();
     pr_finish(I, Y, step, sh, work, plan, cand, models, actions, inst, tid);

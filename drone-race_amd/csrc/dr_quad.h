@@ -208,6 +208,57 @@ struct NextGate
 
 // Race progress (env_dronerace.h:374-394): gate + n_gates*laps +
 // 0.8 * (1 - |p - c_next| / |c_next - c_gate|).
+// Gate counter with the track's gates held in registers (center, normal,
+// radius): update_gate's test for the move p0 -> s.p; the next gate is
+// selected from the table, by selects over the whole of it so that it stays
+// in registers, when one is crossed.  No memory access per step.
+struct GateRegs
+{
+    float gv[DR_MAX_GATES][7];
+    int n_gates, nx;
+    dr_gate g;
+
+    __device__ void pick()
+    {
+        float v[7];
+#pragma unroll
+        for (int i = 0; i < 7; ++i)
+            v[i] = gv[0][i];
+#pragma unroll
+        for (int j = 1; j < DR_MAX_GATES; ++j)
+#pragma unroll
+            for (int i = 0; i < 7; ++i)
+                v[i] = j == nx ? gv[j][i] : v[i];
+        g.center[0] = v[0], g.center[1] = v[1], g.center[2] = v[2];
+        g.normal[0] = v[3], g.normal[1] = v[4], g.normal[2] = v[5];
+        g.radius = v[6];
+        g._pad = 0.0f;
+    }
+    __device__ void init(const dr_track& tr, int gate)
+    {
+#pragma unroll
+        for (int k = 0; k < DR_MAX_GATES; ++k)
+        {
+            const dr_gate& src = tr.gates[k];
+            gv[k][0] = src.center[0], gv[k][1] = src.center[1], gv[k][2] = src.center[2];
+            gv[k][3] = src.normal[0], gv[k][4] = src.normal[1], gv[k][5] = src.normal[2];
+            gv[k][6] = src.radius;
+        }
+        n_gates = tr.n_gates;
+        nx = gate + 1 == n_gates ? 0 : gate + 1;
+        pick();
+    }
+    __device__ void step(float margin, const float p0[3], QState& s)
+    {
+        cross_gate(g, nx, margin, p0, s);
+        if (s.gate == nx)
+        {
+            nx = s.gate + 1 == n_gates ? 0 : s.gate + 1;
+            pick();
+        }
+    }
+};
+
 DR_HD float advance(const dr_gate* gates, int n_gates, const QState& s)
 {
     int next = s.gate + 1;

@@ -285,7 +285,10 @@ __global__ __launch_bounds__(256) void k_rollout_p4(const InstDev* __restrict__ 
 
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
-    const int role = tid / WAVE;
+    // The stage of each wave rotates with the workgroup: the waves of a
+    // workgroup go to the CU's SIMDs in order, so a fixed assignment would put
+    // every workgroup's dynamics wave (the heaviest stage) on one SIMD.
+    const int role = ((tid / WAVE) + (int)blockIdx.x) & 3;
     for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
     {
         const RTile rt = tiles[tile];

@@ -5,6 +5,7 @@ differences (h = 1e-3) of the FP64 oracle on 256 random control sequences
 +-h are excluded and counted.  Also: checkpoint invariance, the autograd
 wrapper, and the adjoint-refined control step."""
 import sys
+from types import SimpleNamespace
 
 import numpy as np
 import pytest
@@ -215,4 +216,31 @@ def test_refined_step_edge_cases(pkg, oracle, T, ck, collide):
         assert last < 1
     Us, Js = refine_oracle(oracle, scn, opp, U0, 3, 0.05, ck=ck)
     check_refined(eng, act, Us, Js)
+    eng.close()
+
+
+def test_refined_batch_of_uneven_instances(pkg, oracle):
+    """Three controllers in one engine (K = 512, 1000, 2048; T = 30, 57, 100;
+    each one's opponents are the other drones): every instance's refined
+    nominal and action against the oracle's MPPI update and refinement."""
+    from test_gpu_parity import opponents_of, small_scenario
+    say("case: refinement of a batch of uneven instances")
+    sc = pkg.scenarios
+    scn = small_scenario(sc, n_samples=512, n_timesteps=30, n_drones=4, controlled=3, seed=9, track="lemniscate")
+    scn.instances[1] = sc.instance(1000, 57, 0, 1, sc.seed_mix(9, 1))
+    scn.instances[2] = sc.instance(2048, 100, 0, 2, sc.seed_mix(9, 2))
+    eng = pkg.Engine(scn)
+    eng.set_refine(3, 0.05, 10)
+    noms = [eng.get_nominal(i).copy() for i in range(3)]
+    acts = eng.step(1)
+    for i, ins in enumerate(scn.instances):
+        K, T = ins.n_samples, ins.n_timesteps
+        opp = opponents_of(pkg, scn, ins.drone)
+        costs, noise, _ = oracle.rollout(0, scn.params, scn.cost, scn.tracks[0], scn.drones[ins.drone], opp, noms[i], K, T,
+                                         ins.seed, 0, np.array(ins.sigma[:], np.float32))
+        upd = oracle.update(0, scn.params, costs, noise, noms[i], ins.lambda_, np.array(ins.shift_fill[:], np.float32))
+        one = SimpleNamespace(params=scn.params, cost=scn.cost, tracks=scn.tracks, drones=[scn.drones[ins.drone]])
+        Us, Js = refine_oracle(oracle, one, opp, upd["updated"].astype(np.float32), 3, 0.05)
+        view = SimpleNamespace(get_nominal=lambda _, i=i: eng.get_nominal(i))  # instance i seen as instance 0
+        check_refined(view, acts[i:i + 1], Us, Js)
     eng.close()

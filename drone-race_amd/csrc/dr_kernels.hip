@@ -1104,7 +1104,7 @@ __global__ __launch_bounds__(WAVG_BLOCK) void k_wavg(const InstDev* __restrict__
 // mode 0: update, clamp, action, shift, advance; mode 1: update and clamp
 // only (the unshifted nominal is written back for the adjoint refinement);
 // mode 2: action, shift, advance from the (refined) unshifted nominal.
-__global__ __launch_bounds__(256) void k_update(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
+__global__ __launch_bounds__(512) void k_update(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
                                                 const double* __restrict__ part, const double* __restrict__ den,
                                                 float* __restrict__ nominal, float* __restrict__ actions, QuadConst C, int advance,
                                                 dr_drone_state* __restrict__ drones, const int* __restrict__ drone_inst, int n_drones,
@@ -1311,7 +1311,7 @@ hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstD
                          float* nominal, float* actions, const QuadConst& C, int advance, dr_drone_state* drones, const int* drone_inst,
                          int n_drones, const dr_track* tracks, float margin, int mode)
 {
-    k_update<<<n_inst, 256, 0, st>>>(insts, dyn, part, den, nominal, actions, C, advance, drones, drone_inst, n_drones, tracks, margin,
+    k_update<<<n_inst, 512, 0, st>>>(insts, dyn, part, den, nominal, actions, C, advance, drones, drone_inst, n_drones, tracks, margin,
                                      mode);
     return hipGetLastError();
 }

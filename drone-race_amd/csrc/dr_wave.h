@@ -25,18 +25,18 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 // Sum of n values p[0], p[stride], ... in index order; loads are issued in
-// batches of 8 so that their latencies overlap (same order as a plain loop).
+// batches of 16 so that their latencies overlap (same order as a plain loop).
 __device__ __forceinline__ double sum_in_order(const double* __restrict__ p, int n, long long stride)
 {
     double acc = 0.0;
-    for (int c0 = 0; c0 < n; c0 += 8)
+    for (int c0 = 0; c0 < n; c0 += 16)
     {
-        double v[8];
+        double v[16];
 #pragma unroll
-        for (int b = 0; b < 8; ++b)
+        for (int b = 0; b < 16; ++b)
             v[b] = (c0 + b < n) ? p[(long long)(c0 + b) * stride] : 0.0;
 #pragma unroll
-        for (int b = 0; b < 8; ++b)
+        for (int b = 0; b < 16; ++b)
             acc += v[b];
     }
     return acc;

@@ -488,7 +488,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (each controller batch runs on a "
-              f"single MI355X; see DESIGN.md for how the path would shard)", file=sys.stderr)
+              f"single MI355X; sharding across GPUs is designed but not built)", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -225,8 +225,8 @@ SIGNATURES = {
     "dr_engine_autotune": (_I, [_P, _I, C.POINTER(LaunchCfg), _F]),
     "dr_engine_set_timing": (_I, [_P, _I]),
     "dr_engine_set_eval": (_I, [_P, _I]),
-    "dr_engine_cost_grad": (_I, [_P, _I, _F, _I, _F, _F, _I]),
-    "dr_engine_cost_grad_device": (_I, [_P, _I, _P, _I, _P, _P, _I, _P]),
+    "dr_engine_cost_grad": (_I, [_P, _I, _F, _I, C.c_size_t, _F, _F, _I]),
+    "dr_engine_cost_grad_device": (_I, [_P, _I, _P, _I, C.c_size_t, _P, _P, _I, _P]),
     "dr_engine_set_refine": (_I, [_P, _I, C.c_float, _I]),
     "dr_engine_last_timing": (_I, [_P, _F, _F, _F, _F]),
     "dr_mppi_create": (_I, [C.POINTER(QuadParams), C.POINTER(CostCfg), C.POINTER(Track), C.POINTER(MPPICfg), C.c_uint64,

@@ -736,14 +736,16 @@ int dr_engine_set_refine(dr_engine* e, int n_grad_steps, float step, int ckpt_ev
     return DR_OK;
 }
 
-int dr_engine_cost_grad_device(dr_engine* e, int instance, const float* U, int n_seq, float* J, float* dJdU, int ckpt_every,
-                               void* stream)
+int dr_engine_cost_grad_device(dr_engine* e, int instance, const float* U, int n_seq, size_t n_floats, float* J, float* dJdU,
+                               int ckpt_every, void* stream)
 {
     if (int rc = check_inst(e, instance))
         return rc;
     if (!U || !J || n_seq < 1 || ckpt_every < 1)
         return fail(DR_EINVAL, "cost_grad: bad pointers or sizes");
     const InstDev& I = e->idev[instance];
+    if (n_floats != (size_t)n_seq * I.T * 4)
+        return fail(DR_EINVAL, "cost_grad: n_floats must be n_seq * T * 4 of the instance (U and dJdU are [n_seq][T][4])");
     const int ck = std::min(ckpt_every, I.T);
     const int n_ck = (I.T + ck - 1) / ck;
     const size_t table = (size_t)I.T * e->opp_stride * sizeof(float4);
@@ -759,13 +761,15 @@ int dr_engine_cost_grad_device(dr_engine* e, int instance, const float* U, int n
     return DR_OK;
 }
 
-int dr_engine_cost_grad(dr_engine* e, int instance, const float* U, int n_seq, float* J, float* dJdU, int ckpt_every)
+int dr_engine_cost_grad(dr_engine* e, int instance, const float* U, int n_seq, size_t n_floats, float* J, float* dJdU, int ckpt_every)
 {
     if (int rc = check_inst(e, instance))
         return rc;
     if (!U || !J || n_seq < 1)
         return fail(DR_EINVAL, "cost_grad: bad pointers or sizes");
     const size_t n = (size_t)n_seq * e->idev[instance].T * 4;
+    if (n_floats != n)
+        return fail(DR_EINVAL, "cost_grad: n_floats must be n_seq * T * 4 of the instance (U and dJdU are [n_seq][T][4])");
     if (e->d_gU.n < n)
     {
         HIPCHK(e->d_gU.alloc(n));
@@ -774,7 +778,7 @@ int dr_engine_cost_grad(dr_engine* e, int instance, const float* U, int n_seq, f
     if (e->d_gJ.n < (size_t)n_seq)
         HIPCHK(e->d_gJ.alloc(n_seq));
     HIPCHK(hipMemcpyAsync(e->d_gU.p, U, n * sizeof(float), hipMemcpyHostToDevice, e->stream));
-    int rc = dr_engine_cost_grad_device(e, instance, e->d_gU.p, n_seq, e->d_gJ.p, dJdU ? e->d_gG.p : nullptr, ckpt_every, nullptr);
+    int rc = dr_engine_cost_grad_device(e, instance, e->d_gU.p, n_seq, n, e->d_gJ.p, dJdU ? e->d_gG.p : nullptr, ckpt_every, nullptr);
     if (rc)
         return rc;
     HIPCHK(hipMemcpyAsync(J, e->d_gJ.p, n_seq * sizeof(float), hipMemcpyDeviceToHost, e->stream));

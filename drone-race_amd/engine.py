@@ -123,17 +123,26 @@ class Engine:
     def set_eval(self, on):
         check(self.lib.dr_engine_set_eval(self.h, 1 if on else 0))
 
+    def horizon(self, i):
+        """T of instance i."""
+        if not 0 <= i < self.n_inst:
+            raise IndexError(f"instance {i} out of range [0, {self.n_inst})")
+        return self.sc.instances[i].n_timesteps
+
     def cost_grad(self, i, U, ckpt_every=10, grad=True):
         """J and dJ/dU of control sequences U[n][T][4] of instance i (noise-free adjoint rollout)."""
         U = np.ascontiguousarray(U, dtype=np.float32)
+        T = self.horizon(i)
+        if U.ndim != 3 or U.shape[0] < 1 or U.shape[1:] != (T, 4):
+            raise ValueError(f"U must have shape (n, {T}, 4) for instance {i}, got {U.shape}")
         n = U.shape[0]
         J = np.empty(n, dtype=np.float32)
         G = np.empty_like(U) if grad else None
-        check(self.lib.dr_engine_cost_grad(self.h, i, _fptr(U), n, _fptr(J), _fptr(G) if grad else None, ckpt_every))
+        check(self.lib.dr_engine_cost_grad(self.h, i, _fptr(U), n, U.size, _fptr(J), _fptr(G) if grad else None, ckpt_every))
         return J, G
 
-    def cost_grad_device(self, i, u_ptr, n_seq, j_ptr, g_ptr, ckpt_every, stream):
-        check(self.lib.dr_engine_cost_grad_device(self.h, i, u_ptr, n_seq, j_ptr, g_ptr, ckpt_every, stream))
+    def cost_grad_device(self, i, u_ptr, n_seq, n_floats, j_ptr, g_ptr, ckpt_every, stream):
+        check(self.lib.dr_engine_cost_grad_device(self.h, i, u_ptr, n_seq, n_floats, j_ptr, g_ptr, ckpt_every, stream))
 
     def set_refine(self, n_grad_steps, step=0.05, ckpt_every=10):
         check(self.lib.dr_engine_set_refine(self.h, n_grad_steps, step, ckpt_every))

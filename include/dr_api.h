@@ -218,13 +218,16 @@ int dr_engine_last_timing(dr_engine* e, float* rollout_ms, float* reduce_ms, flo
 /* Adjoint of the noise-free rollout cost.  For n_seq control sequences
  * U[n_seq][T][4] of `instance` (from its current drone state, opponents and
  * track) writes J[n_seq] and dJdU[n_seq][T][4] (may be NULL), checkpointing
- * every ckpt_every steps.  Host pointers; synchronous.  No reference
- * counterpart: the reference has no differentiable rollout. */
-int dr_engine_cost_grad(dr_engine* e, int instance, const float* U, int n_seq, float* J, float* dJdU, int ckpt_every);
+ * every ckpt_every steps.  n_floats is the length of U (and of dJdU): it must
+ * equal n_seq * T * 4 for the instance's T, else DR_EINVAL.  Host pointers;
+ * synchronous.  No reference counterpart: the reference has no
+ * differentiable rollout. */
+int dr_engine_cost_grad(dr_engine* e, int instance, const float* U, int n_seq, size_t n_floats, float* J, float* dJdU,
+                        int ckpt_every);
 /* Same with device pointers, enqueued on `stream` (a hipStream_t; NULL = the
  * engine's stream).  Used by the PyTorch autograd wrapper. */
-int dr_engine_cost_grad_device(dr_engine* e, int instance, const float* U, int n_seq, float* J, float* dJdU, int ckpt_every,
-                               void* stream);
+int dr_engine_cost_grad_device(dr_engine* e, int instance, const float* U, int n_seq, size_t n_floats, float* J, float* dJdU,
+                               int ckpt_every, void* stream);
 /* Adjoint refinement inside every control step: after the MPPI update,
  * n_grad_steps projected gradient steps U <- clamp(U - step * dJ/dU) on each
  * instance's nominal (checkpoints every ckpt_every steps), the J of every

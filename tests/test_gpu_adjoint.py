@@ -244,3 +244,31 @@ def test_refined_batch_of_uneven_instances(pkg, oracle):
         view = SimpleNamespace(get_nominal=lambda _, i=i: eng.get_nominal(i))  # instance i seen as instance 0
         check_refined(view, acts[i:i + 1], Us, Js)
     eng.close()
+
+
+def test_cost_grad_rejects_mismatched_shapes(pkg):
+    """U and dJ/dU carry n_seq * T * 4 floats; any other length is rejected by
+    the C ABI (DR_EINVAL, nothing read or written) and any other shape by the
+    host and autograd wrappers."""
+    import ctypes as C
+    import torch
+    from drone_race_amd.autograd import rollout_cost
+    T = 20
+    scn, _ = setup(pkg, T)
+    eng = pkg.Engine(scn)
+    U = sequences(pkg, 4, T + 1, 1)  # room for the over-long cases
+    J = np.full(4, 7.0, np.float32)
+    G = np.full_like(U, 7.0)
+    f = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    for n_floats in (4 * T * 4 - 4, 4 * T * 4 + 4, 4 * (T + 1) * 4, 0):
+        rc = eng.lib.dr_engine_cost_grad(eng.h, 0, f(U), 4, n_floats, f(J), f(G), 10)
+        assert rc == pkg.abi.DR_EINVAL, n_floats
+        assert "n_floats" in eng.lib.dr_last_error().decode()
+    assert np.all(J == 7.0) and np.all(G == 7.0)
+    assert eng.lib.dr_engine_cost_grad(eng.h, 0, f(U), 4, 4 * T * 4, f(J), f(G), 10) == pkg.abi.DR_OK
+    for bad in (U[:, :T - 1], U, U[:, :T, :3], U[0, :T]):
+        with pytest.raises(ValueError):
+            eng.cost_grad(0, bad)
+    with pytest.raises(ValueError):
+        rollout_cost(torch.zeros((2, T + 1, 4), device="cuda:0"), eng, 0, 10)
+    eng.close()

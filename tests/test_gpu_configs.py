@@ -1,0 +1,135 @@
+"""GPU parity at the benchmark configurations' own workloads (HIP engine vs the
+FP32 oracle, FP64 distance printed):
+
+  single quadrotor   1 drone, K = 1024, T = 50, 8-gate circle r = 20 m, seed 0:
+                     one step sample by sample, then the closed loop
+  64 uneven          64 single-drone controllers, K ~ logU[1024, 32768],
+                     T ~ U{30..150}, each on its own circle: one step of all 64
+                     sample by sample, three closed-loop steps against the
+                     oracle's closed loop, then a fourth step sample by sample
+                     from the engine's own states
+  8 drones refined   8 drones, K = 4096, T = 100, MPPI + 3 projected gradient
+                     steps (0.05, checkpoints every 10): every controller's kept
+                     iterate against the oracle's MPPI update and refinement
+
+Bar: noise and costs bit-exact, nominals / actions / min cost / nu within
+1e-5 relative; closed-loop actions and drone states within 1e-4 relative (the
+FP32 update may differ from the oracle's by one rounding per step); refined
+plans within 1e-4 of the oracle iterate of lowest J.
+"""
+import time
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from test_gpu_adjoint import check_refined, refine_oracle
+from test_gpu_parity import check_instance, make_engine, opponents_of, rel_err, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _state_rows(drones):
+    return np.array([list(d.p) + list(d.q) + list(d.v) + list(d.w) for d in drones], dtype=np.float64)
+
+
+def _flat_nominals(scn):
+    return np.concatenate([np.tile(np.array(i.shift_fill[:], np.float32), (i.n_timesteps, 1)).reshape(-1)
+                           for i in scn.instances]).astype(np.float32)
+
+
+def test_single_quadrotor_workload(pkg, oracle):
+    sc = pkg.scenarios
+    scn = sc.c0_single()
+    ins = scn.instances[0]
+    say(f"case: {scn.name}: K={ins.n_samples} T={ins.n_timesteps}, one step sample by sample")
+    assert (ins.n_samples, ins.n_timesteps, scn.tracks[0].n_gates) == (1024, 50, 8)
+    eng = make_engine(pkg, scn)
+    nom0 = eng.get_nominal(0).copy()
+    acts = eng.step(1)
+    upd = check_instance(pkg, oracle, eng, scn, 0, nom0)
+    assert rel_err(acts[0], upd["action"]) <= 1e-5
+    eng.close()
+
+    n_steps = 100
+    say(f"case: {scn.name}: closed loop, {n_steps} steps against the oracle's closed loop")
+    eng = pkg.Engine(scn)
+    g_acts = eng.step(n_steps, advance=True)
+    g_drones = eng.get_drones()
+    drones = (pkg.abi.DroneState * 1)(*scn.drones)
+    o_acts, _ = oracle.world_run(scn, drones, _flat_nominals(scn), [0], [ins.lambda_], n_steps, threads=16)
+    e_act, e_state = rel_err(g_acts, o_acts), rel_err(_state_rows(g_drones), _state_rows(drones))
+    say(f"  after {n_steps} steps: last action max rel {e_act:.2e}, drone state max rel {e_state:.2e}, gate "
+        f"{g_drones[0].gate} (oracle {drones[0].gate}), laps {g_drones[0].laps}")
+    assert e_act <= 1e-4 and e_state <= 1e-4
+    assert (g_drones[0].gate, g_drones[0].laps) == (drones[0].gate, drones[0].laps)
+    eng.close()
+
+
+def test_batch64_uneven_workload(pkg, oracle):
+    sc = pkg.scenarios
+    scn = sc.c2_batch64()
+    n = len(scn.instances)
+    Ks = [i.n_samples for i in scn.instances]
+    Ts = [i.n_timesteps for i in scn.instances]
+    say(f"case: {scn.name}: {n} instances, K in [{min(Ks)}, {max(Ks)}], T in [{min(Ts)}, {max(Ts)}], "
+        f"{scn.sample_steps_per_step} sample-steps per step; one step, every instance sample by sample")
+    assert n == 64 and 1024 <= min(Ks) and max(Ks) <= 32768 and 30 <= min(Ts) and max(Ts) <= 150
+    eng = make_engine(pkg, scn)
+    noms = [eng.get_nominal(i).copy() for i in range(n)]
+    eng.step(1)
+    for i in range(n):
+        check_instance(pkg, oracle, eng, scn, i, noms[i])
+    eng.close()
+
+    n_loop = 3
+    say(f"case: {scn.name}: {n_loop} closed-loop steps against the oracle's closed loop")
+    eng = pkg.Engine(scn)
+    t0 = time.perf_counter()
+    g_acts = eng.step(n_loop, advance=True)
+    g_drones = eng.get_drones()
+    drones = (pkg.abi.DroneState * n)(*scn.drones)
+    o_acts, _ = oracle.world_run(scn, drones, _flat_nominals(scn), [0] * n, [i.lambda_ for i in scn.instances], n_loop,
+                                 threads=16)
+    e_act, e_state = rel_err(g_acts, o_acts), rel_err(_state_rows(g_drones), _state_rows(drones))
+    say(f"  after {n_loop} steps: actions max rel {e_act:.2e}, drone states max rel {e_state:.2e} "
+        f"({time.perf_counter() - t0:.1f} s with the oracle)")
+    assert e_act <= 1e-4 and e_state <= 1e-4
+
+    say(f"case: {scn.name}: step {n_loop + 1} sample by sample from the engine's own states and plans")
+    here = SimpleNamespace(params=scn.params, cost=scn.cost, tracks=scn.tracks, instances=scn.instances,
+                           drone_world=scn.drone_world, drones=(pkg.abi.DroneState * n)(*g_drones))
+    noms = [eng.get_nominal(i).copy() for i in range(n)]
+    eng.set_eval(True)
+    eng.step(1)
+    for i in range(n):
+        check_instance(pkg, oracle, eng, here, i, noms[i], step=n_loop)
+    eng.close()
+
+
+def test_eight_drone_refined_workload(pkg, oracle):
+    sc = pkg.scenarios
+    scn = sc.c4_eight_drones(4096)
+    n = len(scn.instances)
+    say(f"case: {scn.name}: MPPI K=4096 T=100 + 3 gradient steps (0.05, checkpoints every 10), every controller")
+    assert n == 8 and all((i.n_samples, i.n_timesteps) == (4096, 100) for i in scn.instances)
+    eng = pkg.Engine(scn)
+    eng.set_refine(3, 0.05, 10)
+    noms = [eng.get_nominal(i).copy() for i in range(n)]
+    acts = eng.step(1)
+    kept = []
+    for i, ins in enumerate(scn.instances):
+        opp = opponents_of(pkg, scn, ins.drone)
+        costs, noise, _ = oracle.rollout(0, scn.params, scn.cost, scn.tracks[0], scn.drones[ins.drone], opp, noms[i],
+                                         ins.n_samples, ins.n_timesteps, ins.seed, 0, np.array(ins.sigma[:], np.float32),
+                                         threads=16)
+        assert np.array_equal(eng.costs(i).astype(np.float64), costs), f"instance {i}: costs are expected to be bit-exact"
+        assert np.array_equal(eng.noise(i).view(np.uint32), noise.view(np.uint32)), f"instance {i}: noise not bit-exact"
+        upd = oracle.update(0, scn.params, costs, noise, noms[i], ins.lambda_, np.array(ins.shift_fill[:], np.float32))
+        one = SimpleNamespace(params=scn.params, cost=scn.cost, tracks=scn.tracks, drones=[scn.drones[ins.drone]])
+        Us, Js = refine_oracle(oracle, one, opp, upd["updated"].astype(np.float32), 3, 0.05)
+        say(f"  instance {i}:")
+        view = SimpleNamespace(get_nominal=lambda _, i=i: eng.get_nominal(i))
+        kept.append(check_refined(view, acts[i:i + 1], Us, Js)[1])
+    say(f"  kept iterates {kept}")
+    eng.close()

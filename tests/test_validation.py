@@ -179,3 +179,19 @@ ENV_CASES = {
 def test_branching_create_rejects_environments(pkg, kind, name):
     rc, msg, h = bm_create(pkg, kind, mutate_env=ENV_CASES[kind][name])
     assert rc == pkg.abi.DR_EINVAL and msg and not h.value, (rc, msg)
+
+
+def test_cost_grad_wrappers_check_shapes_before_any_call(pkg):
+    """Engine.cost_grad rejects a U whose shape is not (n, T_i, 4) before it
+    calls the library (the engine object is never created: no GPU needed)."""
+    import numpy as np
+    sc = pkg.scenarios
+    scn = sc.Scenario("v", [sc.track_circle(0)], [sc.instance(128, 10, 0, 0, 1)], [0], sc.start_states(sc.track_circle(0), 1),
+                      1, "validation")
+    eng = object.__new__(pkg.Engine)
+    eng.sc, eng.n_inst, eng.h, eng.lib = scn, 1, None, None
+    for shape in ((3, 9, 4), (3, 11, 4), (3, 10, 3), (10, 4), (0, 10, 4)):
+        with pytest.raises(ValueError):
+            eng.cost_grad(0, np.zeros(shape, np.float32))
+    with pytest.raises(IndexError):
+        eng.cost_grad(1, np.zeros((1, 10, 4), np.float32))

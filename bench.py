@@ -1,14 +1,26 @@
-"""Benchmark of the plain-MPPI control step on one MI355X.
+"""Benchmark of the MPPI control step on one MI355X.
 
-Workload (default): 4 drones racing on a 12-gate lemniscate (half-width
-30 m), one controller each, K=16384 samples, T=100 steps, FP32, closed loop
-(each control step ends with every drone integrated with its action).  A
-"step" is one control step of the whole batch; `value` counts control steps
-of every controller (4 per batch step).
+Headline (the metric: control steps/s of batched adjoint-refined MPPI against
+plain MPPI at equal mean trajectory cost): 8 drones racing on a 12-gate
+lemniscate (half-width 30 m), one controller each, T=100, FP32, closed loop
+(each control step ends with every drone integrated with its action).
+  adjoint_k4096  MPPI at K=4096 followed by 3 projected gradient steps (0.05,
+                 checkpoints every 10) on the weighted nominal: `value`
+  plain_k16384   plain MPPI at K=16384: the comparison (`vs_baseline` =
+                 value / its control steps/s, measured in the same run)
+  plain_k4096    plain MPPI at K=4096: what the gradient steps add
+A "step" is one control step of the whole batch; control steps/s count every
+controller (8 per batch step).  Quality: the mean trajectory cost of the
+executed closed loop over 1000 steps and 4 seeds per leg.
+
+Also reported: plain MPPI on 4 drones at K=16384 (c1), the 64-instance batch
+against per-instance launches (c2), PR-MPPI (c3), the single-quadrotor
+workload's 500 steps on host and GPU (c0), the reference's own StratRace2D
+controllers, and the oracle's OpenMP CPU runs of the same workloads.
 
 Output: one JSON line (driver contract) with a roofline object for the
-dominant kernel (the fused rollout) and the oracle's OpenMP CPU run of the
-same workload as the baseline.
+dominant kernel of the headline step and the CPU run of the headline workload
+as the baseline.
 """
 import argparse
 import importlib.util
@@ -25,6 +37,11 @@ METRIC = "control steps/s, batched+adjoint vs plain MPPI at equal mean trajector
 # 96 for the Philox Box-Muller noise).  tests/test_bench_constants.py keeps it
 # equal to oracle/count_ops.py.
 FLOPS_PER_SAMPLE_STEP = 270.07
+# The same for the 8-drone workload at K=4096 (seven opponents in every running cost), and the
+# oracle-counted flops of one adjoint-refinement launch over its 8 controllers (oracle/count_ops.py
+# count_refine); tests/test_bench_constants.py keeps both equal to the counter.
+C4_FLOPS_PER_SAMPLE_STEP = 334.12
+C4_REFINE_FLOPS = 2220640
 # HBM bytes per rollout sample-step the rollout kernel must move: the float4
 # noise row it writes (read back once by the weighted average).
 ROLLOUT_BYTES_PER_SAMPLE_STEP = 16.0
@@ -35,6 +52,8 @@ ROLLOUT_BYTES_PER_SAMPLE_STEP = 16.0
 # the engine reports per rollout.
 SR_FLOPS_PER_STEP = {"sr_race": 1044.30, "pr_sr_race": 1010.14, "bm_race": 1513.46, "ho_race": 689.45, "drace_race": 1320.55}
 PEAK_FP32_TFLOPS = 157.3
+# Without packed FP32 (this build compiles with it off) a lane retires one FP32 FMA per cycle: half of the published figure.
+PEAK_FP32_UNPACKED_TFLOPS = PEAK_FP32_TFLOPS / 2
 PEAK_HBM_GBS = 8000.0
 
 
@@ -60,7 +79,8 @@ def cpu_baseline(pkg, scn, budget_s):
     import numpy as np
     import pyoracle
 
-    oracle = pyoracle.Oracle(pkg.abi)
+    path, flags = pyoracle.timing_build()
+    oracle = pyoracle.Oracle(pkg.abi, path)
     threads = min(16, os.cpu_count() or 1)
     drones = (pkg.abi.DroneState * len(scn.drone_world))(*scn.drones)
     nominal = np.concatenate([np.tile(np.array(i.shift_fill[:], np.float32), (i.n_timesteps, 1)).reshape(-1)
@@ -77,7 +97,7 @@ def cpu_baseline(pkg, scn, budget_s):
     n_inst = len(scn.instances)
     return {"value": n_inst * n / dt, "unit": "control steps/s", "cores": threads, "kind": "port",
             "sample": f"{n} closed-loop batch steps ({n_inst} controllers each) of the same workload after 1 warm-up "
-                      f"step, oracle FP32 restatement, OpenMP {threads} threads, {dt:.1f} s",
+                      f"step, oracle FP32 restatement built {flags}, OpenMP {threads} threads, {dt:.1f} s",
             "sample_steps_per_s": n * scn.sample_steps_per_step / dt}
 
 
@@ -89,7 +109,8 @@ def cpu_stratrace(pkg, budget_s=12.0):
     import numpy as np
     import pyoracle
 
-    oracle = pyoracle.Oracle(pkg.abi)
+    path, flags = pyoracle.timing_build()
+    oracle = pyoracle.Oracle(pkg.abi, path)
     threads = min(16, os.cpu_count() or 1)
     sc = pkg.scenarios
     out = {}
@@ -104,7 +125,8 @@ def cpu_stratrace(pkg, budget_s=12.0):
             fn()
         dt = time.perf_counter() - t0
         return {"control_steps_per_s": n / dt, "rollout_steps_per_s": n * work / dt, "cores": threads, "kind": "port",
-                "sample": f"{n} control steps after 1 warm-up step, oracle FP32, OpenMP {threads} threads, {dt:.1f} s"}
+                "sample": f"{n} control steps after 1 warm-up step, oracle FP32 built {flags}, OpenMP {threads} threads, "
+                          f"{dt:.1f} s"}
 
     s1 = sc.sr_race(n_inst=1)
     i1 = s1.instances[0]
@@ -191,49 +213,210 @@ def batch_vs_per_instance(pkg, steps=50, warmup=5):
             "sample_steps_per_batch_step": scn.sample_steps_per_step, "steps": steps}
 
 
-def adjoint_vs_plain(pkg, steps=100, warmup=10, cost_steps=100, autotune=True):
-    """8 drones on the 12-gate lemniscate: plain MPPI at K=16384 against MPPI at
-    K=4096 followed by 3 adjoint gradient steps (step 0.05, checkpoints every
-    10) on the weighted nominal; plain K=4096 alongside, to show what the
-    gradient steps add.  Throughput: closed-loop control steps/s (graph
-    replays), every leg with its own autotuned launch shape.  Quality: mean
-    over controllers and the first cost_steps steps of the nominal's
-    noise-free cost (the plan each step executes), from the same start, plus
-    gates passed at the end."""
+# The three legs of the adjoint-vs-plain comparison: name, samples per controller, gradient steps.
+C4_LEGS = (("plain_k16384", 16384, 0), ("plain_k4096", 4096, 0), ("adjoint_k4096", 4096, 3))
+C4_GRAD_STEP, C4_CKPT = 0.05, 10
+
+
+def start_progress(scn):
+    """Race progress of every controlled drone at the start (gate + n_gates laps + 0.8 of the way from
+    the last gate passed to the next, env_dronerace.h:374-394), for the progress made in a run."""
+    import math
+    out = []
+    for ins in scn.instances:
+        d, tr = scn.drones[ins.drone], scn.tracks[ins.track]
+        nxt = 0 if d.gate + 1 == tr.n_gates else d.gate + 1
+        a, b = tr.gates[d.gate].center, tr.gates[nxt].center
+        g2 = sum((b[c] - d.p[c]) ** 2 for c in range(3))
+        c2 = sum((b[c] - a[c]) ** 2 for c in range(3))
+        out.append(d.gate + tr.n_gates * d.laps + 0.8 * (1.0 - math.sqrt(g2 / c2)))
+    return out
+
+
+def c4_speed(pkg, steps, warmup, autotune=True):
+    """Throughput of each leg on 8 drones (12-gate lemniscate, seed 1), closed loop, graph replays:
+    autotuned launch shape, `warmup` untimed steps, then exactly `steps` timed batch steps between two
+    synchronisations; then the same number of steps again with HIP events between the kernels for
+    the per-kernel device time."""
+    import torch
     sc = pkg.scenarios
     out = {}
-    for name, K, refine in (("plain_k16384", 16384, 0), ("plain_k4096", 4096, 0), ("adjoint_k4096", 4096, 3)):
+    for name, K, refine in C4_LEGS:
         scn = sc.c4_eight_drones(K)
         n = len(scn.instances)
         eng = pkg.Engine(scn)
-        eng.set_refine(refine, 0.05, 10)
+        eng.set_refine(refine, C4_GRAD_STEP, C4_CKPT)
         tuned = None
         if autotune:
             tuned, _ = eng.autotune(trials=5)
+        launch = {f: getattr(eng.get_launch(), f) for f, _ in pkg.abi.LaunchCfg._fields_}
         eng.step(warmup, advance=True)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        eng.step(steps, advance=True)
+        eng.step_async(steps, advance=True)
+        eng.synchronize()
+        torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        n_timed = max(steps, 20)
+        eng.set_timing(True)
+        eng.step(n_timed, advance=True)
+        eng.set_timing(False)
+        kt = {k: v / n_timed for k, v in eng.last_timing().items()}
         eng.close()
-        eng = pkg.Engine(scn)
-        eng.set_refine(refine, 0.05, 10)
-        eng.set_eval(True)
-        costs = []
-        for _ in range(cost_steps):
-            eng.step(1, advance=True)
-            costs.append(sum(d["traj_cost"] for d in eng.diag()) / n)
-        prog = [d.gate + d.laps * scn.tracks[0].n_gates for d in eng.get_drones()]
-        eng.close()
-        out[name] = {"control_steps_per_s": n * steps / dt, "ms_per_batch_step": 1e3 * dt / steps,
-                     "mean_traj_cost": sum(costs) / len(costs),
-                     "gates_passed_mean": sum(prog) / len(prog) - (scn.tracks[0].n_gates - 1)}
-        if tuned is not None:
-            out[name]["launch"] = tuned
-    out["speedup"] = out["adjoint_k4096"]["control_steps_per_s"] / out["plain_k16384"]["control_steps_per_s"]
-    # costs are negative (progress rewards): the refined plan reaches the
-    # plain K=16384 cost when this ratio is >= 1
-    out["cost_ratio"] = out["adjoint_k4096"]["mean_traj_cost"] / out["plain_k16384"]["mean_traj_cost"]
+        out[name] = {"samples": K, "grad_steps": refine, "control_steps_per_s": n * steps / dt, "ms_per_batch_step": 1e3 * dt / steps,
+                     "kernel_ms_per_step": kt, "launch": launch, "autotuned": tuned is not None,
+                     "sample_steps_per_s": scn.sample_steps_per_step * steps / dt}
+        log(f"bench: {name}: {out[name]['control_steps_per_s']:.0f} control steps/s ({out[name]['ms_per_batch_step']:.4f} ms per "
+            f"batch step of {n}); device ms per step {json.dumps({k: round(v, 4) for k, v in kt.items()})}")
     return out
+
+
+def c4_quality(pkg, launches, seeds=(1, 2, 3, 4), n_steps=1000):
+    """Mean trajectory cost of each leg over n_steps closed-loop control steps from the start of the
+    8-drone race, for each seed (the seed draws the track's gate jitter and the controllers' noise
+    keys): per controller, the running cost of every executed state (undiscounted, the engine's own
+    cost function on the device) minus the final progress weight times the progress made (gates
+    passed, fractional), i.e. the controllers' objective evaluated on the trajectory actually
+    flown.  Lower is better.  Reported: the mean over seeds and controllers, the per-seed means and
+    their spread, gates passed and executed states in collision."""
+    import numpy as np
+    sc = pkg.scenarios
+    out = {}
+    for name, K, refine in C4_LEGS:
+        per_seed, gates, colls, costs = [], [], [], []
+        for seed in seeds:
+            scn = sc.c4_eight_drones(K, seed=seed)
+            eng = pkg.Engine(scn)
+            eng.set_launch(**launches[name])
+            eng.set_refine(refine, C4_GRAD_STEP, C4_CKPT)
+            eng.set_exec_cost(True)
+            p0 = start_progress(scn)
+            eng.step(n_steps, advance=True)
+            d = eng.diag()
+            eng.close()
+            w = scn.cost.final_adv_weight
+            prog = [x["exec_progress"] - a for x, a in zip(d, p0)]
+            J = [x["exec_cost"] - w * g for x, g in zip(d, prog)]
+            per_seed.append(float(np.mean(J)))
+            gates += prog
+            costs += J
+            colls += [x["exec_collisions"] for x in d]
+        out[name] = {"mean_traj_cost": float(np.mean(costs)), "per_seed_mean_cost": per_seed,
+                     "seed_std": float(np.std(per_seed, ddof=1)) if len(per_seed) > 1 else 0.0,
+                     "gates_passed_mean": float(np.mean(gates)), "gates_passed_min": float(np.min(gates)),
+                     "collision_states_mean": float(np.mean(colls)), "controllers": len(costs)}
+        log(f"bench: quality {name}: mean trajectory cost {out[name]['mean_traj_cost']:.2f} (per seed "
+            f"{[round(x, 2) for x in per_seed]}), gates passed {out[name]['gates_passed_mean']:.2f}, "
+            f"collision states {out[name]['collision_states_mean']:.2f}")
+    out["seeds"] = list(seeds)
+    out["steps"] = n_steps
+    return out
+
+
+def vector_roofline(kernel, flops, ms, note):
+    """FP32 vector (VALU) roofline of one kernel: algorithmic flops per launch over its HIP-event time.
+    peak = the published FP32 vector figure, which counts packed FP32 (two lanes per instruction);
+    this build compiles with packed FP32 off, so the unpacked ceiling is half of it."""
+    achieved = flops / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+    return {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+            "peak_unpacked": PEAK_FP32_UNPACKED_TFLOPS, "frac_unpacked": achieved / PEAK_FP32_UNPACKED_TFLOPS, "traffic": None,
+            "kernel": kernel, "flops_per_launch": flops, "ms_per_launch": ms, "note": note}
+
+
+def c4_rooflines(pkg, adj):
+    """Rooflines of the adjoint leg's kernels from its per-kernel HIP-event times; the dominant one
+    (largest device time per step) is the headline roofline."""
+    sc = pkg.scenarios
+    scn = sc.c4_eight_drones(4096)
+    ss = scn.sample_steps_per_step
+    kt = adj["kernel_ms_per_step"]
+    r = {"k_rollout": vector_roofline("k_rollout", C4_FLOPS_PER_SAMPLE_STEP * ss, kt["rollout_ms"],
+                                      "oracle-counted flops per sample-step (8 drones: 7 opponents in the cost)"),
+         "k_refine": vector_roofline("k_refine", C4_REFINE_FLOPS, kt["refine_ms"],
+                                     "oracle-counted flops of the one-lane adjoint (3 iterates of J and dJ/dU + the last J) per "
+                                     "launch of 8 controllers; bound by its serial per-step dependency chain, not by issue")}
+    wavg_bytes = ROLLOUT_BYTES_PER_SAMPLE_STEP * ss
+    ms = kt["reduce_ms"]
+    ach = wavg_bytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    r["k_wavg"] = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": None,
+                   "kernel": "k_wavg", "bytes_per_launch": wavg_bytes, "ms_per_launch": ms,
+                   "note": "reads every noise row once (16 B per sample-step)"}
+    dom = max(("k_rollout", kt["rollout_ms"]), ("k_wavg", kt["reduce_ms"]), ("k_refine", kt["refine_ms"]), key=lambda x: x[1])[0]
+    tfile = os.path.join(ROOT, "profiles", "traffic_c4_8drones_k4096_t100.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tr = json.load(f)
+        for k in r:
+            if k in tr:
+                r[k]["traffic"] = tr[k]
+    return dom, r
+
+
+def cpu_c4(pkg, budget_s):
+    """The adjoint leg on the host: the oracle's FP32 restatement of MPPI at K=4096 followed by the
+    same 3 projected gradient steps (FP32 adjoint, lowest J kept), closed loop, 8 drones, OpenMP
+    over samples, timed build (-O3) of the oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+
+    path, flags = pyoracle.timing_build()
+    oracle = pyoracle.Oracle(pkg.abi, path)
+    threads = min(16, os.cpu_count() or 1)
+    scn = pkg.scenarios.c4_eight_drones(4096)
+    n_inst = len(scn.instances)
+    drones = (pkg.abi.DroneState * len(scn.drone_world))(*scn.drones)
+    nominal = np.concatenate([np.tile(np.array(i.shift_fill[:], np.float32), (i.n_timesteps, 1)).reshape(-1)
+                              for i in scn.instances]).astype(np.float32)
+    steps, lams = [0] * n_inst, [i.lambda_ for i in scn.instances]
+    refine = (3, C4_GRAD_STEP, C4_CKPT)
+    t0 = time.perf_counter()
+    oracle.world_run(scn, drones, nominal, steps, lams, 1, threads, refine=refine)
+    one = time.perf_counter() - t0
+    n = max(1, int(budget_s / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    oracle.world_run(scn, drones, nominal, steps, lams, n, threads, refine=refine)
+    dt = time.perf_counter() - t0
+    return {"value": n_inst * n / dt, "unit": "control steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} closed-loop batch steps ({n_inst} controllers each: MPPI K=4096 T=100 + 3 gradient steps) after "
+                      f"1 warm-up step, oracle FP32 restatement built {flags}, OpenMP {threads} threads, {dt:.1f} s",
+            "sample_steps_per_s": n * scn.sample_steps_per_step / dt}
+
+
+def c0_runs(pkg, n_steps=500):
+    """The single-quadrotor workload (K=1024, T=50, 8-gate circle r=20 m, seed 0) for its 500 control
+    steps: on the host (oracle FP32 restatement, timed build, OpenMP) and on the GPU (graph replays)."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    path, flags = pyoracle.timing_build()
+    oracle = pyoracle.Oracle(pkg.abi, path)
+    threads = min(16, os.cpu_count() or 1)
+    scn = pkg.scenarios.c0_single()
+    ins = scn.instances[0]
+    drones = (pkg.abi.DroneState * 1)(*scn.drones)
+    nominal = np.tile(np.array(ins.shift_fill[:], np.float32), (ins.n_timesteps, 1)).reshape(-1)
+    t0 = time.perf_counter()
+    oracle.world_run(scn, drones, nominal, [0], [ins.lambda_], n_steps, threads)
+    cpu_dt = time.perf_counter() - t0
+    cpu_gates = drones[0].gate + scn.tracks[0].n_gates * drones[0].laps
+    eng = pkg.Engine(scn)
+    eng.step(1, advance=True)
+    eng.close()
+    eng = pkg.Engine(scn)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.step_async(n_steps, advance=True)
+    eng.synchronize()
+    gpu_dt = time.perf_counter() - t0
+    g = eng.get_drones()[0]
+    eng.close()
+    return {"workload": scn.name, "steps": n_steps,
+            "cpu": {"control_steps_per_s": n_steps / cpu_dt, "seconds": cpu_dt, "cores": threads, "kind": "port",
+                    "build": flags, "final_gate_count": cpu_gates},
+            "gpu": {"control_steps_per_s": n_steps / gpu_dt, "seconds": gpu_dt,
+                    "final_gate_count": g.gate + scn.tracks[0].n_gates * g.laps}}
 
 
 def pr_mppi(pkg, steps=100, warmup=10):
@@ -267,11 +450,8 @@ def pr_mppi(pkg, steps=100, warmup=10):
 def sr_roofline(key, rollout_steps, rollout_ms, kernel):
     """FP32 roofline of a StratRace2D rollout kernel: counted flops per rollout-step times the
     rollout-steps of one launch, over its HIP-event time."""
-    flops = SR_FLOPS_PER_STEP[key] * rollout_steps
-    achieved = flops / (rollout_ms * 1e-3) / 1e12
-    return {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None, "kernel": kernel, "flops_per_launch": flops,
-            "note": "FP32 vector (VALU) work, no MFMA; oracle-counted flops per rollout-step"}
+    return vector_roofline(kernel, SR_FLOPS_PER_STEP[key] * rollout_steps, rollout_ms,
+                           "FP32 vector (VALU) work, no MFMA; oracle-counted flops per rollout-step")
 
 
 def sr_race(pkg, steps=50, warmup=5, n_inst=1):
@@ -454,22 +634,63 @@ def session_race(pkg, steps=30):
     return out
 
 
+def plain_run(pkg, scn, steps, warmup, launch=None, autotune=True):
+    """Plain MPPI on a workload: launch shape (given or autotuned), `warmup` steps, exactly `steps`
+    timed closed-loop batch steps between two synchronisations, then per-kernel device time from HIP
+    events over as many steps, and the rollout kernel's roofline."""
+    import torch
+    n_inst = len(scn.instances)
+    eng = pkg.Engine(scn)
+    tuned = None
+    if launch:
+        eng.set_launch(**launch)
+    elif autotune:
+        tuned, ms = eng.autotune(trials=5)
+        log(f"bench: {scn.name}: autotuned launch {tuned} ({ms:.3f} ms/step)")
+    used = {f: getattr(eng.get_launch(), f) for f, _ in pkg.abi.LaunchCfg._fields_}
+    eng.step(warmup, advance=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.step_async(steps, advance=True)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    n_timed = min(max(steps, 20), 200)
+    eng.set_timing(True)
+    eng.step(n_timed, advance=True)
+    eng.set_timing(False)
+    kt = {k: v / n_timed for k, v in eng.last_timing().items()}
+    eng.close()
+    ss = scn.sample_steps_per_step
+    flops = FLOPS_PER_SAMPLE_STEP * ss
+    roof = vector_roofline("k_rollout", flops, kt["rollout_ms"], "oracle-counted flops per sample-step (dynamics, gates, costs, noise)")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{scn.name}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            roof["traffic"] = json.load(f).get("rollout_hbm_bytes_per_launch")
+    return {"workload": scn.name, "control_steps_per_s": n_inst * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+            "sample_steps_per_s": ss * steps / elapsed, "kernel_ms_per_step": kt, "launch": used, "autotuned": tuned is not None,
+            "roofline": roof, "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2"])
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--launch", default="", help="launch shape as JSON (skips the autotuner)")
+    ap.add_argument("--launch", default="", help="launch shape of the plain 4-drone run as JSON (skips its autotuner)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--cpu-budget", type=float, default=8.0)
+    ap.add_argument("--no-quality", action="store_true", help="skip the closed-loop trajectory-cost runs of the three legs")
+    ap.add_argument("--quality-steps", type=int, default=1000)
+    ap.add_argument("--quality-seeds", type=int, default=4)
     ap.add_argument("--no-batch64", action="store_true")
-    ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint-refined vs plain section (for profiling)")
     ap.add_argument("--no-pr", action="store_true")
-    ap.add_argument("--pr-only", action="store_true", help="run only the PR-MPPI section (for profiling)")
     ap.add_argument("--no-sr", action="store_true")
+    ap.add_argument("--plain-only", action="store_true", help="run only plain MPPI on 4 drones, K=16384 (for profiling)")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the throughput of the three 8-drone legs (for profiling)")
+    ap.add_argument("--pr-only", action="store_true", help="run only the PR-MPPI section (for profiling)")
     ap.add_argument("--sr-only", action="store_true", help="run only the StratRace2D section (for profiling)")
     ap.add_argument("--bm-only", action="store_true", help="run only the branching spline MPPI section (for profiling)")
     ap.add_argument("--session-only", action="store_true", help="run only the wire-protocol session section")
@@ -495,6 +716,7 @@ def main():
 
     pkg = package()
     sc = pkg.scenarios
+    torch.cuda.set_device(0)
     if args.session_only:
         print(json.dumps({"session": session_race(pkg, steps=min(args.steps, 30))}), flush=True)
         return
@@ -517,90 +739,64 @@ def main():
     if args.pr_only:
         print(json.dumps({"pr_mppi": pr_mppi(pkg, steps=args.steps, warmup=args.warmup)}), flush=True)
         return
-    if args.adjoint_only:
-        print(json.dumps({"adjoint": adjoint_vs_plain(pkg, steps=min(args.steps, 100), cost_steps=20)}), flush=True)
+    if args.plain_only:
+        launch = json.loads(args.launch) if args.launch else None
+        print(json.dumps({"c1": plain_run(pkg, sc.c1_four_drones(), args.steps, args.warmup, launch, not args.no_autotune)}), flush=True)
         return
-    scn = {"c0": sc.c0_single, "c1": sc.c1_four_drones, "c2": sc.c2_batch64}[args.config]()
-    n_inst = len(scn.instances)
-    log(f"bench: workload {scn.name}: {scn.description}")
-    torch.cuda.set_device(0)
-    eng = pkg.Engine(scn)
-    tuned = None
-    if args.launch:
-        eng.set_launch(**json.loads(args.launch))
-    elif not args.no_autotune:
-        tuned, ms = eng.autotune(trials=5)
-        log(f"bench: autotuned launch {tuned} ({ms:.3f} ms/step)")
-    launch = {f: getattr(eng.get_launch(), f) for f, _ in pkg.abi.LaunchCfg._fields_}
+    if args.adjoint_only:
+        print(json.dumps({"c4": c4_speed(pkg, args.steps, args.warmup, not args.no_autotune)}), flush=True)
+        return
 
-    eng.step(args.warmup, advance=True)
-    log(f"bench: warm-up {args.warmup} steps done")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    eng.step_async(args.steps, advance=True)
-    eng.synchronize()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    log(f"bench: timed {args.steps} steps in {elapsed:.3f} s")
-
-    # Per-kernel device time from HIP events recorded on the engine's stream
-    # between the kernels (stream launches), over the same number of steps.
-    n_timed = min(args.steps, 200)
-    eng.set_timing(True)
-    eng.step(n_timed, advance=True)
-    eng.set_timing(False)
-    kt = eng.last_timing()
-    rollout_s = kt["rollout_ms"] / n_timed * 1e-3
-    log(f"bench: per-step device time rollout {kt['rollout_ms'] / n_timed:.4f} ms, reduce {kt['reduce_ms'] / n_timed:.4f} ms, "
-        f"update+advance {kt['update_ms'] / n_timed:.4f} ms")
-    # plan quality: noise-free cost of each controller's nominal, mean over 20 steps
-    eng.set_eval(True)
-    traj = []
-    for _ in range(20):
-        eng.step(1, advance=True)
-        traj.append(sum(d["traj_cost"] for d in eng.diag()) / n_inst)
-    eng.set_eval(False)
-    eng.close()
-
-    ss = scn.sample_steps_per_step
-    flops = FLOPS_PER_SAMPLE_STEP * ss
-    achieved = flops / rollout_s / 1e12
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{scn.name}.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("rollout_hbm_bytes_per_launch")
+    log("bench: 8 drones, K=16384 plain / K=4096 plain / K=4096 + 3 adjoint gradient steps: throughput ...")
+    legs = c4_speed(pkg, args.steps, args.warmup, not args.no_autotune)
+    adj, base = legs["adjoint_k4096"], legs["plain_k16384"]
+    dom, roofs = c4_rooflines(pkg, adj)
+    scn4 = sc.c4_eight_drones(4096)
     out = {
         "metric": METRIC,
-        "value": n_inst * args.steps / elapsed,
+        "value": adj["control_steps_per_s"],
         "unit": "control steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": adj["ms_per_batch_step"],
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": adj["control_steps_per_s"] / base["control_steps_per_s"],
+        "vs_baseline_ref": "plain MPPI at K=16384 on the same 8-drone workload, measured in this run (no published figure "
+                           "exists); the comparison the metric names",
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": scn.name, "controllers": n_inst,
-                   "samples": [i.n_samples for i in scn.instances][:8],
-                   "horizon": [i.n_timesteps for i in scn.instances][:8],
-                   "closed_loop": True, "launch": launch},
-        "sample_steps_per_s": ss * args.steps / elapsed,
-        "kernel_ms_per_step": {k: v / n_timed for k, v in kt.items()},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "kernel": "k_rollout", "flops_per_launch": flops,
-                     "note": "FP32 vector (VALU) work, no MFMA; peak is the published FP32 figure"},
-        "mean_traj_cost": sum(traj) / len(traj),
+        "config": {"workload": "c4_8drones_k4096_t100_adjoint3", "controllers": len(scn4.instances), "samples": 4096, "horizon": 100,
+                   "grad_steps": 3, "grad_step": C4_GRAD_STEP, "ckpt_every": C4_CKPT, "closed_loop": True, "launch": adj["launch"],
+                   "autotuned": adj["autotuned"], "baseline_workload": "c4_8drones_k16384_t100 (plain MPPI)"},
+        "legs": legs,
+        "speedup_vs_plain_k16384": adj["control_steps_per_s"] / base["control_steps_per_s"],
+        "speedup_plain_k4096_vs_plain_k16384": legs["plain_k4096"]["control_steps_per_s"] / base["control_steps_per_s"],
+        "roofline": roofs[dom],
+        "rooflines": roofs,
+        "sample_steps_per_s": adj["sample_steps_per_s"],
     }
-    if tuned:
-        out["config"]["autotuned"] = True
-    if not args.no_adjoint:
-        log("bench: 8 drones, adjoint-refined K=4096 vs plain K=16384 ...")
-        out["adjoint"] = adjoint_vs_plain(pkg)
-        log(f"bench: adjoint {json.dumps(out['adjoint'])}")
+    if not args.no_quality:
+        log(f"bench: trajectory cost of the three legs, {args.quality_steps} closed-loop steps x {args.quality_seeds} seeds ...")
+        q = c4_quality(pkg, {k: v["launch"] for k, v in legs.items()}, seeds=tuple(range(1, 1 + args.quality_seeds)),
+                       n_steps=args.quality_steps)
+        out["quality"] = q
+        ja, jb = q["adjoint_k4096"]["mean_traj_cost"], q["plain_k16384"]["mean_traj_cost"]
+        out["cost_ratio"] = ja / jb if jb != 0 else None
+        out["equal_cost"] = bool(ja <= jb)
+        out["cost_ratio_note"] = ("mean trajectory cost of the adjoint leg over that of plain K=16384; costs are negative "
+                                  "(progress rewards), so >= 1 means the adjoint leg is at least as good; equal_cost: its "
+                                  "mean cost is <= plain K=16384's")
+    log("bench: plain MPPI, 4 drones, K=16384, T=100 ...")
+    launch = json.loads(args.launch) if args.launch else None
+    out["c1"] = plain_run(pkg, sc.c1_four_drones(), args.steps, args.warmup, launch, not args.no_autotune)
+    log(f"bench: c1 {json.dumps(out['c1'])}")
+    if not args.no_batch64:
+        log("bench: 64 uneven controllers, batched vs per-instance ...")
+        out["batch64"] = batch_vs_per_instance(pkg)
+        out["batch64_speedup"] = out["batch64"]["speedup"]
+        log(f"bench: batch64 speedup {out['batch64']['speedup']:.2f}x")
     if not args.no_pr:
         log("bench: PR-MPPI, 4 drones, K=8192 per branch, T=80 ...")
         out["pr_mppi"] = pr_mppi(pkg)
@@ -608,40 +804,31 @@ def main():
     if not args.no_sr:
         log("bench: StratRace2D plain MPPI, K=65536, T=70 ...")
         out["sr_race"] = sr_race(pkg)
-        log(f"bench: sr_race {json.dumps(out['sr_race'])}")
         log("bench: PR-MPPI on StratRace2D, K=65536, T=60, P=9 ...")
         out["pr_sr_race"] = pr_sr_race(pkg)
-        log(f"bench: pr_sr_race {json.dumps(out['pr_sr_race'])}")
         log("bench: branching spline MPPI on StratRace2D, K=65536, T=80, 20 knots ...")
         out["bm_race"] = bm_race(pkg)
-        log(f"bench: bm_race {json.dumps(out['bm_race'])}")
         log("bench: branching spline MPPI, 8 worlds in one batch ...")
         out["bm_race_8x"] = bm_race(pkg, n_inst=8)
-        log(f"bench: bm_race_8x {json.dumps(out['bm_race_8x'])}")
         log("bench: branching spline MPPI with plan verification (2^17 samples, horizon 40) ...")
         out["bm_race_verified"] = bm_race(pkg, n_verif=2 ** 17)
-        log(f"bench: bm_race_verified {json.dumps(out['bm_race_verified'])}")
         log("bench: branching spline MPPI on HiddenObs, K=65536, T=80, 20 knots ...")
         out["ho_race"] = ho_race(pkg)
-        log(f"bench: ho_race {json.dumps(out['ho_race'])}")
         log("bench: branching spline MPPI on DroneRace, K=65536, T=60, 10 knots ...")
         out["drace_race"] = drace_race(pkg)
-        log(f"bench: drace_race {json.dumps(out['drace_race'])}")
         log("bench: wire-protocol session (MPPI and PR-MPPI headers at the reference settings) ...")
         out["session"] = session_race(pkg)
-        log(f"bench: session {json.dumps(out['session'])}")
-    if not args.no_batch64:
-        log("bench: 64 uneven controllers, batched vs per-instance ...")
-        out["batch64"] = batch_vs_per_instance(pkg)
-        log(f"bench: batch64 speedup {out['batch64']['speedup']:.2f}x")
     if not args.no_cpu:
-        log("bench: CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline(pkg, scn, args.cpu_budget)
+        log("bench: CPU baseline of the headline workload (adjoint leg on the host) ...")
+        out["cpu_baseline"] = cpu_c4(pkg, args.cpu_budget)
+        log(f"bench: CPU baseline {out['cpu_baseline']['value']:.2f} control steps/s")
+        log("bench: single-quadrotor workload, 500 steps on host and GPU ...")
+        out["c0"] = c0_runs(pkg)
+        log(f"bench: c0 {json.dumps(out['c0'])}")
+        out["c1"]["cpu_baseline"] = cpu_baseline(pkg, sc.c1_four_drones(), args.cpu_budget / 2)
         if not args.no_sr:
             log("bench: CPU baselines of the StratRace2D controllers ...")
             out["cpu_baseline_stratrace"] = cpu_stratrace(pkg, args.cpu_budget)
-            log(f"bench: cpu_baseline_stratrace {json.dumps(out['cpu_baseline_stratrace'])}")
-        log(f"bench: CPU baseline {out['cpu_baseline']['value']:.2f} control steps/s")
     print(json.dumps(out), flush=True)
 
 

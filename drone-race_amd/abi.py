@@ -60,7 +60,7 @@ class Instance(C.Structure):
 
 class Diag(C.Structure):
     _fields_ = [("min_cost", C.c_float), ("nu", C.c_float), ("lambda_", C.c_float), ("traj_cost", C.c_float),
-                ("step", C.c_uint32), ("_pad", C.c_int32)]
+                ("step", C.c_uint32), ("exec_collisions", C.c_uint32), ("exec_cost", C.c_float), ("exec_progress", C.c_float)]
 
 
 class LaunchCfg(C.Structure):
@@ -225,10 +225,11 @@ SIGNATURES = {
     "dr_engine_autotune": (_I, [_P, _I, C.POINTER(LaunchCfg), _F]),
     "dr_engine_set_timing": (_I, [_P, _I]),
     "dr_engine_set_eval": (_I, [_P, _I]),
+    "dr_engine_set_exec_cost": (_I, [_P, _I]),
     "dr_engine_cost_grad": (_I, [_P, _I, _F, _I, C.c_size_t, _F, _F, _I]),
     "dr_engine_cost_grad_device": (_I, [_P, _I, _P, _I, C.c_size_t, _P, _P, _I, _P]),
     "dr_engine_set_refine": (_I, [_P, _I, C.c_float, _I]),
-    "dr_engine_last_timing": (_I, [_P, _F, _F, _F, _F]),
+    "dr_engine_last_timing": (_I, [_P, _F, _I]),
     "dr_mppi_create": (_I, [C.POINTER(QuadParams), C.POINTER(CostCfg), C.POINTER(Track), C.POINTER(MPPICfg), C.c_uint64,
                             C.POINTER(_P)]),
     "dr_mppi_destroy": (None, [_P]),

@@ -28,6 +28,8 @@ hipError_t launch_rollout(int block, int nb, int split, int grid, hipStream_t st
 hipError_t launch_eval(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const dr_track* tracks,
                        const dr_drone_state* drones, const float* nominal, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC,
                        int max_T, int opp_stride);
+hipError_t launch_exec(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const dr_track* tracks,
+                       const dr_drone_state* drones, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC);
 hipError_t launch_wavg(int chunk, int n_wtiles, hipStream_t st, const InstDev* insts, InstDyn* dyn, const WTile* wtiles,
                        const float* costs, const float4* noise, const float* tmin, double* part, double* den);
 hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
@@ -46,6 +48,10 @@ using namespace dr;
 namespace {
 
 thread_local std::string g_err;
+
+// HIP events per timed step: start, after the rollout, after the weighted
+// average, before and after the adjoint refinement, end.
+constexpr int kStepEvents = 6;
 
 int fail(int code, const std::string& msg)
 {
@@ -82,6 +88,7 @@ struct dr_engine
     hipStream_t side = nullptr; // forked stream of the nominal's noise-free rollout
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool eval = false;
+    bool exec_cost = false; // k_exec after every closed-loop step
     int refine_steps = 0; // adjoint gradient steps per control step (0: plain MPPI)
     float refine_step = 0.05f;
     int refine_ck = 10;
@@ -128,7 +135,7 @@ struct dr_engine
     bool timing = false;
     std::vector<hipEvent_t> events;
     int timed_steps = 0;
-    float last_ms[4] = {0, 0, 0, 0};
+    float last_ms[5] = {0, 0, 0, 0, 0}; // rollout, reduce, update (+ advance), refinement, step
 
     ~dr_engine()
     {
@@ -257,7 +264,7 @@ int dr_engine::build_layout()
 int dr_engine::enqueue_step(bool advance, int timed_index)
 {
     const int n = (int)inst.size();
-    hipEvent_t* ev = (timed_index >= 0) ? &events[(size_t)timed_index * 4] : nullptr;
+    hipEvent_t* ev = (timed_index >= 0) ? &events[(size_t)timed_index * kStepEvents] : nullptr;
     if (ev)
         HIPCHK(hipEventRecord(ev[0], stream));
     if (eval)
@@ -282,16 +289,29 @@ int dr_engine::enqueue_step(bool advance, int timed_index)
     {
         HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, 0, d_drones.p,
                              d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin, 1));
+        if (ev)
+            HIPCHK(hipEventRecord(ev[3], stream));
         HIPCHK(launch_refine(n, stream, d_inst.p, d_tracks.p, d_drones.p, d_nominal.p, refine_steps, refine_step, refine_ck, C, Q,
                              cost));
+        if (ev)
+            HIPCHK(hipEventRecord(ev[4], stream));
         HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, advance ? 1 : 0, d_drones.p,
                              d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin, 2));
     }
     else
+    {
+        if (ev)
+        {
+            HIPCHK(hipEventRecord(ev[3], stream));
+            HIPCHK(hipEventRecord(ev[4], stream));
+        }
         HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, advance ? 1 : 0, d_drones.p,
                              d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin, 0));
+    }
+    if (exec_cost && advance)
+        HIPCHK(launch_exec(n, stream, d_inst.p, d_dyn.p, d_tracks.p, d_drones.p, C, Q, cost));
     if (ev)
-        HIPCHK(hipEventRecord(ev[3], stream));
+        HIPCHK(hipEventRecord(ev[5], stream));
     return DR_OK;
 }
 
@@ -302,7 +322,7 @@ int dr_engine::run(int n_steps, bool advance)
     timed_steps = 0;
     if (timing)
     {
-        const size_t need = (size_t)n_steps * 4;
+        const size_t need = (size_t)n_steps * kStepEvents;
         while (events.size() < need)
         {
             hipEvent_t ev;
@@ -579,19 +599,22 @@ int dr_engine_synchronize(dr_engine* e)
     HIPCHK(hipStreamSynchronize(e->stream));
     if (e->timed_steps > 0)
     {
-        float acc[4] = {0, 0, 0, 0};
+        float acc[5] = {0, 0, 0, 0, 0};
         for (int s = 0; s < e->timed_steps; ++s)
         {
-            hipEvent_t* ev = &e->events[(size_t)s * 4];
-            float a, b, c, t;
+            hipEvent_t* ev = &e->events[(size_t)s * kStepEvents];
+            float a, b, u1, r, u2, t;
             HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
             HIPCHK(hipEventElapsedTime(&b, ev[1], ev[2]));
-            HIPCHK(hipEventElapsedTime(&c, ev[2], ev[3]));
-            HIPCHK(hipEventElapsedTime(&t, ev[0], ev[3]));
+            HIPCHK(hipEventElapsedTime(&u1, ev[2], ev[3]));
+            HIPCHK(hipEventElapsedTime(&r, ev[3], ev[4]));
+            HIPCHK(hipEventElapsedTime(&u2, ev[4], ev[5]));
+            HIPCHK(hipEventElapsedTime(&t, ev[0], ev[5]));
             acc[0] += a;
             acc[1] += b;
-            acc[2] += c;
-            acc[3] += t;
+            acc[2] += u1 + u2;
+            acc[3] += r;
+            acc[4] += t;
         }
         std::memcpy(e->last_ms, acc, sizeof(acc));
         e->timed_steps = 0;
@@ -659,7 +682,9 @@ int dr_engine_get_diag(dr_engine* e, dr_diag* diag, int n_instances)
         diag[i].lambda = y[i].lambda;
         diag[i].traj_cost = y[i].traj_cost;
         diag[i].step = y[i].step;
-        diag[i]._pad = 0;
+        diag[i].exec_collisions = y[i].exec_collisions;
+        diag[i].exec_cost = y[i].exec_cost;
+        diag[i].exec_progress = y[i].exec_progress;
     }
     return DR_OK;
 }
@@ -721,6 +746,32 @@ int dr_engine_set_eval(dr_engine* e, int on)
 }
 
 static const size_t kAdjLdsBudget = 64 * 1024;
+
+int dr_engine_set_exec_cost(dr_engine* e, int on)
+{
+    if (!e)
+        return fail(DR_EINVAL, "NULL handle");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (on)
+    {
+        const int n = (int)e->inst.size();
+        std::vector<InstDyn> y(n);
+        HIPCHK(hipMemcpy(y.data(), e->d_dyn.p, n * sizeof(InstDyn), hipMemcpyDeviceToHost));
+        for (InstDyn& d : y)
+        {
+            d.exec_cost = 0.0f;
+            d.exec_progress = 0.0f;
+            d.exec_collisions = 0u;
+        }
+        HIPCHK(hipMemcpy(e->d_dyn.p, y.data(), n * sizeof(InstDyn), hipMemcpyHostToDevice));
+    }
+    if (e->exec_cost != (on != 0))
+    {
+        e->exec_cost = on != 0;
+        e->drop_graphs();
+    }
+    return DR_OK;
+}
 
 int dr_engine_set_refine(dr_engine* e, int n_grad_steps, float step, int ckpt_every)
 {
@@ -788,18 +839,11 @@ int dr_engine_cost_grad(dr_engine* e, int instance, const float* U, int n_seq, s
     return DR_OK;
 }
 
-int dr_engine_last_timing(dr_engine* e, float* rollout_ms, float* reduce_ms, float* update_ms, float* total_ms)
+int dr_engine_last_timing(dr_engine* e, float* ms, int n)
 {
-    if (!e)
-        return fail(DR_EINVAL, "NULL handle");
-    if (rollout_ms)
-        *rollout_ms = e->last_ms[0];
-    if (reduce_ms)
-        *reduce_ms = e->last_ms[1];
-    if (update_ms)
-        *update_ms = e->last_ms[2];
-    if (total_ms)
-        *total_ms = e->last_ms[3];
+    if (!e || !ms || n < 1 || n > 5)
+        return fail(DR_EINVAL, "last_timing: bad handle, pointer or count (1..5)");
+    std::memcpy(ms, e->last_ms, n * sizeof(float));
     return DR_OK;
 }
 

@@ -49,7 +49,9 @@ struct InstDyn
     float min_cost;
     float nu;
     float traj_cost;
-    float _pad[3];
+    float exec_cost;          // running cost of the executed closed-loop states, summed (k_exec)
+    float exec_progress;      // race progress of the last executed state (k_exec)
+    uint32_t exec_collisions; // executed states in collision (k_exec)
 };
 
 struct RTile

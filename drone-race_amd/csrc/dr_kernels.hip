@@ -1198,6 +1198,34 @@ __global__ __launch_bounds__(512) void k_update(const InstDev* __restrict__ inst
     }
 }
 
+// Quality of the executed closed loop: after the drones have moved, the
+// running cost of each controlled drone's new state (stage_cost, the other
+// drones of its world where they now are), summed undiscounted over the
+// steps, the number of those states in collision, and the race progress of
+// the latest one.  One lane per instance.
+__global__ __launch_bounds__(64) void k_exec(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn, int n_inst,
+                                             const dr_track* __restrict__ tracks, const dr_drone_state* __restrict__ drones,
+                                             QuadConst C, CostConst Q, dr_cost_cfg KC)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst)
+        return;
+    const InstDev& I = insts[i];
+    const dr_track& tr = tracks[I.track];
+    const QState s = state_from_abi(drones[I.drone]);
+    const auto opp_at = [&](int j) {
+        const dr_drone_state& o = drones[I.opp[j]];
+        return make_float4(o.p[0], o.p[1], o.p[2], 0.0f);
+    };
+    int term;
+    const float c = stage_cost_at(C, Q, KC, tr, s, I.n_opp, opp_at, &term);
+    InstDyn& Y = dyn[i];
+    Y.exec_cost = Y.exec_cost + c;
+    Y.exec_progress = advance(tr.gates, tr.n_gates, s);
+    if (term == TERM_LOSE)
+        Y.exec_collisions = Y.exec_collisions + 1u;
+}
+
 // ---------------------------------------------------------------------------
 // launch wrappers (host)
 
@@ -1286,6 +1314,13 @@ hipError_t launch_rollout(int block, int nb, int split, int grid, hipStream_t st
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_exec(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const dr_track* tracks,
+                       const dr_drone_state* drones, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC)
+{
+    hipLaunchKernelGGL(k_exec, dim3((n_inst + 63) / 64), dim3(64), 0, st, insts, dyn, n_inst, tracks, drones, C, Q, KC);
     return hipGetLastError();
 }
 

@@ -101,7 +101,8 @@ class Engine:
     def diag(self):
         d = (abi.Diag * self.n_inst)()
         check(self.lib.dr_engine_get_diag(self.h, d, self.n_inst))
-        return [dict(min_cost=x.min_cost, nu=x.nu, lam=x.lambda_, traj_cost=x.traj_cost, step=x.step) for x in d]
+        return [dict(min_cost=x.min_cost, nu=x.nu, lam=x.lambda_, traj_cost=x.traj_cost, step=x.step, exec_cost=x.exec_cost,
+                     exec_progress=x.exec_progress, exec_collisions=x.exec_collisions) for x in d]
 
     def set_launch(self, **kw):
         cfg = self.get_launch()
@@ -122,6 +123,10 @@ class Engine:
 
     def set_eval(self, on):
         check(self.lib.dr_engine_set_eval(self.h, 1 if on else 0))
+
+    def set_exec_cost(self, on):
+        """Track the executed closed loop's running cost, collisions and progress (diag exec_*); on clears them."""
+        check(self.lib.dr_engine_set_exec_cost(self.h, 1 if on else 0))
 
     def horizon(self, i):
         """T of instance i."""
@@ -151,9 +156,10 @@ class Engine:
         check(self.lib.dr_engine_set_timing(self.h, 1 if on else 0))
 
     def last_timing(self):
-        v = [C.c_float() for _ in range(4)]
-        check(self.lib.dr_engine_last_timing(self.h, *[C.byref(x) for x in v]))
-        return dict(zip(("rollout_ms", "reduce_ms", "update_ms", "total_ms"), [x.value for x in v]))
+        """Device ms of the last timed run, summed over its steps, per phase."""
+        a = np.zeros(5, dtype=np.float32)
+        check(self.lib.dr_engine_last_timing(self.h, _fptr(a), 5))
+        return dict(zip(("rollout_ms", "reduce_ms", "update_ms", "refine_ms", "total_ms"), [float(x) for x in a]))
 
 
 class MPPI:

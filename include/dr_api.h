@@ -146,7 +146,9 @@ typedef struct dr_diag
     float lambda;     /* temperature used by the next step */
     float traj_cost;  /* noise-free cost of the nominal the last rollout started from */
     uint32_t step;    /* control steps taken */
-    int32_t _pad;
+    uint32_t exec_collisions; /* closed-loop steps whose executed state was a collision (dr_engine_set_exec_cost) */
+    float exec_cost;          /* running cost of the executed states, summed over closed-loop steps (dr_engine_set_exec_cost) */
+    float exec_progress;      /* race progress of the last executed state (dr_engine_set_exec_cost) */
 } dr_diag;
 
 /* Launch shape of the control step; chosen by dr_engine_autotune or set by
@@ -206,14 +208,25 @@ int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* bes
  * (dr_diag.traj_cost), on a second stream concurrently with the sampled
  * rollout.  Off by default; traj_cost is not updated while off. */
 int dr_engine_set_eval(dr_engine* e, int on);
+/* on != 0: every later closed-loop step (advance != 0) also evaluates, for
+ * each controlled drone, the running cost of the state it was just moved to
+ * (opponents where the other drones of its world now are, undiscounted) and
+ * adds it to dr_diag.exec_cost, counts the states in collision in
+ * exec_collisions and writes the state's race progress to exec_progress: the
+ * quality of the executed trajectory.  Turning it on clears the three.  Off
+ * by default (one more small launch per step).  No reference counterpart:
+ * the reference prints the executed state instead (engine.cpp:336-338). */
+int dr_engine_set_exec_cost(dr_engine* e, int on);
 /* on != 0: later steps run as plain stream launches with HIP events
  * between the kernels (no graph), so that dr_engine_last_timing can report
  * per-kernel device time. */
 int dr_engine_set_timing(dr_engine* e, int on);
 /* Device time of the last timed dr_engine_step / step_async batch, per
- * kernel kind, from HIP events on the engine stream (ms, summed over steps;
- * update_ms includes the closed-loop advance when it ran). */
-int dr_engine_last_timing(dr_engine* e, float* rollout_ms, float* reduce_ms, float* update_ms, float* total_ms);
+ * kernel kind, from HIP events on the engine stream (ms, summed over steps),
+ * the first n (1..5) of: rollout, weighted average, update (with the
+ * closed-loop advance when it ran), adjoint refinement (0 without it),
+ * whole step. */
+int dr_engine_last_timing(dr_engine* e, float* ms, int n);
 
 /* Adjoint of the noise-free rollout cost.  For n_seq control sequences
  * U[n_seq][T][4] of `instance` (from its current drone state, opponents and

@@ -65,6 +65,41 @@ def count(scn, n=1024):
     return flops / steps + NOISE_FLOPS_PER_STEP
 
 
+def count_refine(scn, n_grad=3, ck=10):
+    """Algorithmic FP32 flops of one adjoint-refinement launch over every controller of the
+    workload, counted by the oracle's op-counting scalar on each controller's hover plan from its
+    start state: n_grad iterates of J and dJ/dU (forward with checkpoints, segment recompute,
+    reverse) plus the J of the last iterate, plus the projected step (fma and two-sided clamp,
+    4 flops per control value) of each iterate."""
+    sys.path.insert(0, HERE)
+    import pyoracle
+    pkg = package()
+    abi = pkg.abi
+    L = pyoracle.lib(abi)
+    L.or_count_cost_grad_ops.argtypes = [C.POINTER(abi.QuadParams), C.POINTER(abi.CostCfg), C.POINTER(abi.Track),
+                                         C.POINTER(abi.DroneState), C.POINTER(abi.Opponent), C.c_int, C.POINTER(C.c_float),
+                                         C.c_int, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    total = 0
+    for ins in scn.instances:
+        opp = []
+        for d in range(len(scn.drone_world)):
+            if d != ins.drone and scn.drone_world[d] == scn.drone_world[ins.drone]:
+                o = abi.Opponent()
+                for c in range(3):
+                    o.p[c], o.v[c] = scn.drones[d].p[c], scn.drones[d].v[c]
+                opp.append(o)
+        oa = (abi.Opponent * max(1, len(opp)))(*opp)
+        T = ins.n_timesteps
+        U = np.tile(np.array(ins.shift_fill[:], np.float32), (T, 1)).reshape(-1)
+        fwd, full = C.c_longlong(), C.c_longlong()
+        rc = L.or_count_cost_grad_ops(C.byref(scn.params), C.byref(scn.cost), C.byref(scn.tracks[ins.track]),
+                                      C.byref(scn.drones[ins.drone]), oa, len(opp), U.ctypes.data_as(C.POINTER(C.c_float)), T, ck,
+                                      C.byref(fwd), C.byref(full))
+        assert rc == 0
+        total += n_grad * (full.value + 4 * T * 4) + fwd.value
+    return total
+
+
 # Noise cost per StratRace2D rollout-step read off the code: one Box-Muller
 # pair (log 20, sincos 21, 5 more) and two sigma scalings for the control
 # noise of a (sample, step) = 48; the environment's opponent-action noise is
@@ -152,5 +187,7 @@ def count_drace(pkg, n=256):
 if __name__ == "__main__":
     pkg = package()
     print(f"{count(pkg.scenarios.c1_four_drones()):.2f} FP32 flops per sample-step (4 drones, K=16384, T=100)")
+    print(f"{count(pkg.scenarios.c4_eight_drones(4096)):.2f} FP32 flops per sample-step (8 drones, K=4096, T=100)")
+    print(f"{count_refine(pkg.scenarios.c4_eight_drones(4096))} FP32 flops per refinement launch (8 drones, T=100, 3 iterates)")
     for k, v in {**count_sr(pkg), **count_ho(pkg), **count_drace(pkg)}.items():
         print(f"{k}: {v:.2f}")

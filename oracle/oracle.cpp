@@ -557,11 +557,19 @@ int or_mppi_update(int f64, const dr_quad_params* P, const double* costs, const 
 // updates, clamps, takes its action and shifts; then every controlled drone
 // is integrated with its action.  nominal holds sum_i T_i*4 floats in
 // instance order; steps and lambdas are per instance and updated in place.
+//
+// n_grad_steps > 0 adds the adjoint refinement after each update (the
+// engine's dr_engine_set_refine): projected gradient steps
+// U <- clamp(U - grad_step dJ/dU) from the clamped update, checkpoints every
+// ck steps, J of every iterate, the lowest kept (the update itself on ties),
+// then action and shift.  The FP32 adjoint here sums in the one-lane order,
+// so the refined plans agree with the engine's to rounding, not bit for bit.
 int or_world_run(const dr_quad_params* P, const dr_cost_cfg* K, const dr_track* tracks, const dr_instance* inst, int n_inst,
                  const int32_t* drone_world, dr_drone_state* drones, int n_drones, float* nominal, uint32_t* steps, float* lambdas,
-                 int n_steps, int n_threads, float* actions_out, float* traj_costs_out)
+                 int n_steps, int n_threads, float* actions_out, float* traj_costs_out, int n_grad_steps, float grad_step, int ck)
 {
-    if (!P || !K || !tracks || !inst || !drone_world || !drones || !nominal || !steps || !lambdas || n_inst < 1 || n_steps < 0)
+    if (!P || !K || !tracks || !inst || !drone_world || !drones || !nominal || !steps || !lambdas || n_inst < 1 || n_steps < 0 ||
+        n_grad_steps < 0 || (n_grad_steps > 0 && ck < 1))
         return 1;
     const Model<float> M(*P);
     std::vector<int> nom_off(n_inst);
@@ -601,6 +609,41 @@ int or_world_run(const dr_quad_params* P, const dr_cost_cfg* K, const dr_track* 
             double a[4], mc, nu;
             update<float>(*P, costs.data(), noise.data(), nominal + nom_off[i], Kn, T, lambdas[i], I.shift_fill, upd.data(), a, sh.data(),
                           &mc, &nu, nullptr);
+            if (n_grad_steps > 0)
+            {
+                // refine the clamped update, keep the lowest-J iterate, then act and shift
+                std::vector<float> U((size_t)T * 4), best;
+                for (int j = 0; j < T * 4; ++j)
+                    U[j] = (float)upd[j];
+                std::vector<float> G((size_t)T * 4);
+                float j_best = 0.0f;
+                for (int it = 0; it <= n_grad_steps; ++it)
+                {
+                    const bool last = it == n_grad_steps;
+                    const float J = cost_grad<float>(*P, *K, tracks[I.track], snap[I.drone], opp.data(), (int)opp.size(), U.data(), T, ck,
+                                                     last ? nullptr : G.data(), nullptr);
+                    if (it == 0 || J < j_best)
+                    {
+                        j_best = J;
+                        best = U;
+                    }
+                    if (last)
+                        break;
+                    for (int t = 0; t < T; ++t)
+                    {
+                        float in[4], out[4];
+                        for (int d = 0; d < 4; ++d)
+                            in[d] = fmaf(-grad_step, G[(size_t)t * 4 + d], U[(size_t)t * 4 + d]);
+                        clamp_u(M, in, out);
+                        for (int d = 0; d < 4; ++d)
+                            U[(size_t)t * 4 + d] = out[d];
+                    }
+                }
+                for (int d = 0; d < 4; ++d)
+                    a[d] = best[d];
+                for (int j = 0; j < T * 4; ++j)
+                    sh[j] = (j + 4 < T * 4) ? best[j + 4] : I.shift_fill[j & 3];
+            }
             for (int j = 0; j < T * 4; ++j)
                 nominal[nom_off[i] + j] = (float)sh[j];
             for (int d = 0; d < 4; ++d)
@@ -678,6 +721,25 @@ int or_count_rollout_ops(const dr_quad_params* P, const dr_cost_cfg* K, const dr
         (void)rollout_one<Cn>(*P, M, *K, *tr, *ego, opp, n_opp, nominal, T, eps.data(), 4);
     }
     *flops_out = g_tally.flops;
+    return 0;
+}
+
+// Algorithmic FP work of the adjoint of one control sequence U (T x 4):
+// flops of the forward pass alone (J) and of forward + reverse (J and dJ/dU),
+// op-counting scalar, checkpoints every ck steps (the recomputed segments
+// counted too).
+int or_count_cost_grad_ops(const dr_quad_params* P, const dr_cost_cfg* K, const dr_track* tr, const dr_drone_state* ego,
+                           const dr_opponent* opp, int n_opp, const float* U, int T, int ck, long long* fwd_flops, long long* full_flops)
+{
+    if (!P || !K || !tr || !ego || !U || !fwd_flops || !full_flops || T < 1 || ck < 1 || (n_opp > 0 && !opp))
+        return 1;
+    std::vector<Cn> g((size_t)T * 4);
+    g_tally.flops = 0;
+    (void)cost_grad<Cn>(*P, *K, *tr, *ego, opp, n_opp, U, T, ck, nullptr, nullptr);
+    *fwd_flops = g_tally.flops;
+    g_tally.flops = 0;
+    (void)cost_grad<Cn>(*P, *K, *tr, *ego, opp, n_opp, U, T, ck, g.data(), nullptr);
+    *full_flops = g_tally.flops;
     return 0;
 }
 

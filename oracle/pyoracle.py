@@ -19,14 +19,36 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def lib(abi):
-    """Load liboracle.so; `abi` is the engine's ctypes struct module."""
+def timing_build():
+    """Path of the -O3 build used to time the CPU baseline: x86-64-v4 (AVX-512) when this host's CPU has
+    AVX-512 F/BW/CD/DQ/VL, else x86-64-v3; and the flag set it was built with."""
+    flags = set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+    except OSError:
+        pass
+    v4 = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= flags
+    name = "liboracle_t4.so" if v4 else "liboracle_t3.so"
+    return os.path.join(HERE, name), ("-O3 -march=x86-64-v4" if v4 else "-O3 -march=x86-64-v3") + " -ffp-contract=off -fopenmp"
+
+
+_libs = {}
+
+
+def lib(abi, path=None):
+    """Load liboracle.so (or another build of the same sources at `path`); `abi` is the engine's ctypes
+    struct module."""
     global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
+    path = path or LIB_PATH
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
         build()
-    L = C.CDLL(LIB_PATH)
+    L = C.CDLL(path)
     F, D, I, U = C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_int, C.c_uint32
     L.or_philox4x32_10.argtypes = [C.POINTER(U), C.POINTER(U), C.POINTER(U)]
     L.or_normals4.argtypes = [U, U, U, U, U, U, F]
@@ -47,7 +69,7 @@ def lib(abi):
     L.or_mppi_update.argtypes = [I, C.POINTER(abi.QuadParams), D, F, F, I, I, C.c_float, F, D, D, D, D, D, D]
     L.or_world_run.argtypes = [C.POINTER(abi.QuadParams), C.POINTER(abi.CostCfg), C.POINTER(abi.Track),
                                C.POINTER(abi.Instance), I, C.POINTER(C.c_int32), C.POINTER(abi.DroneState), I, F,
-                               C.POINTER(U), F, I, I, F, F]
+                               C.POINTER(U), F, I, I, F, F, I, C.c_float, I]
     L.or_cost_grad.argtypes = [I, C.POINTER(abi.QuadParams), C.POINTER(abi.CostCfg), C.POINTER(abi.Track),
                                C.POINTER(abi.DroneState), C.POINTER(abi.Opponent), I, F, I, I, I, D, D,
                                C.POINTER(C.c_uint64), I]
@@ -102,7 +124,9 @@ def lib(abi):
     L.or_count_drace_ops.restype = C.c_longlong
     L.or_count_drace_ops.argtypes = [C.POINTER(abi.DraceEnv), C.POINTER(abi.SrCost), C.POINTER(abi.DraceState), F, I,
                                      C.POINTER(abi.BmInstance), F, C.POINTER(C.c_longlong)]
-    _lib = L
+    _libs[path] = L
+    if path == LIB_PATH:
+        _lib = L
     return L
 
 
@@ -115,9 +139,9 @@ def _d(a):
 
 
 class Oracle:
-    def __init__(self, abi):
+    def __init__(self, abi, path=None):
         self.abi = abi
-        self.L = lib(abi)
+        self.L = lib(abi, path)
 
     def philox(self, ctr, key):
         c = (C.c_uint32 * 4)(*ctr)
@@ -208,8 +232,9 @@ class Oracle:
         assert rc == 0
         return J, G, sig
 
-    def world_run(self, sc, drones, nominal, steps, lambdas, n_steps, threads=1):
-        """Closed-loop FP32 run; drones/nominal/steps/lambdas are updated in place."""
+    def world_run(self, sc, drones, nominal, steps, lambdas, n_steps, threads=1, refine=(0, 0.05, 10)):
+        """Closed-loop FP32 run; drones/nominal/steps/lambdas are updated in place.  refine =
+        (gradient steps, step size, checkpoint interval) of the adjoint refinement (0: plain MPPI)."""
         abi = self.abi
         n_t, n_i, n_d = len(sc.tracks), len(sc.instances), len(sc.drone_world)
         tracks = (abi.Track * n_t)(*sc.tracks)
@@ -220,7 +245,8 @@ class Oracle:
         acts = np.empty(n_i * 4, dtype=np.float32)
         traj = np.empty(n_i, dtype=np.float32)
         rc = self.L.or_world_run(C.byref(sc.params), C.byref(sc.cost), tracks, inst, n_i, world, drones, n_d,
-                                 _f(nominal), st, _f(lam), n_steps, threads, _f(acts), _f(traj))
+                                 _f(nominal), st, _f(lam), n_steps, threads, _f(acts), _f(traj), int(refine[0]),
+                                 float(refine[1]), int(refine[2]))
         assert rc == 0
         steps[:] = list(st)
         lambdas[:] = lam

@@ -40,3 +40,12 @@ def test_dronerace_flops_match_counter(pkg):
     count_ops = _load(os.path.join(ROOT, "oracle", "count_ops.py"), "count_ops_mod")
     measured = count_ops.count_drace(pkg)
     assert abs(bench.SR_FLOPS_PER_STEP["drace_race"] - measured["drace_race"]) < 0.01, measured
+
+
+def test_eight_drone_flops_match_counter(pkg):
+    bench = _load(os.path.join(ROOT, "bench.py"), "bench_mod")
+    count_ops = _load(os.path.join(ROOT, "oracle", "count_ops.py"), "count_ops_mod")
+    scn = pkg.scenarios.c4_eight_drones(4096)
+    measured = count_ops.count(scn)
+    assert abs(bench.C4_FLOPS_PER_SAMPLE_STEP - measured) < 0.01, measured
+    assert bench.C4_REFINE_FLOPS == count_ops.count_refine(scn, n_grad=3, ck=10)

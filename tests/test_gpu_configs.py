@@ -133,3 +133,72 @@ def test_eight_drone_refined_workload(pkg, oracle):
         kept.append(check_refined(view, acts[i:i + 1], Us, Js)[1])
     say(f"  kept iterates {kept}")
     eng.close()
+
+
+def test_executed_trajectory_cost(pkg, oracle):
+    """dr_engine_set_exec_cost: after every closed-loop step each controller adds the running cost of
+    its drone's new state (the other drones where they now are) and counts collisions; the progress is
+    that of the latest state.  Checked step by step against the FP32 oracle's running cost and progress
+    on the engine's own drone states (bit-exact), on the 8-drone workload and on two drones placed in
+    collision."""
+    sc = pkg.scenarios
+    for name, scn in (("8 drones", sc.c4_eight_drones(512, n_timesteps=30)),
+                      ("collision", sc.c4_eight_drones(256, n_timesteps=20))):
+        if name == "collision":
+            d0, d1 = scn.drones[0], scn.drones[1]
+            d1.p[0], d1.p[1], d1.p[2] = d0.p[0] + 0.2, d0.p[1], d0.p[2]
+        say(f"case: executed trajectory cost, {name}")
+        eng = pkg.Engine(scn)
+        eng.set_exec_cost(True)
+        n = len(scn.instances)
+        acc = np.zeros(n, np.float32)
+        colls = np.zeros(n, np.int64)
+        for step in range(6):
+            eng.step(1, advance=True)
+            drones = eng.get_drones()
+            d = eng.diag()
+            for i, ins in enumerate(scn.instances):
+                me = drones[ins.drone]
+                x = np.array(list(me.p) + list(me.q) + list(me.v) + list(me.w), np.float64)
+                opp = []
+                for j in range(len(drones)):
+                    if j != ins.drone:
+                        o = pkg.abi.Opponent()
+                        for c in range(3):
+                            o.p[c], o.v[c] = drones[j].p[c], 0.0
+                        opp.append(o)
+                c, term = oracle.running_cost(0, scn.params, scn.cost, scn.tracks[0], x, me.gate, me.laps, opp, 0.0)
+                acc[i] = np.float32(acc[i] + np.float32(c))
+                colls[i] += int(term == 1)
+                prog = oracle.progress(0, scn.tracks[0], x, me.gate, me.laps)
+                assert d[i]["exec_cost"] == acc[i], (name, step, i, d[i]["exec_cost"], acc[i])
+                assert d[i]["exec_collisions"] == colls[i]
+                assert d[i]["exec_progress"] == np.float32(prog)
+        say(f"  after 6 steps: exec cost {acc.round(3).tolist()}, collisions {colls.tolist()}")
+        if name == "collision":
+            assert colls[0] >= 1
+        eng.set_exec_cost(True)
+        assert all(x["exec_cost"] == 0.0 and x["exec_collisions"] == 0 for x in eng.diag())
+        eng.close()
+
+
+def test_eight_drone_refined_closed_loop(pkg, oracle):
+    """Three closed-loop steps of the 8-drone refined workload (K = 4096, T = 100, 3 gradient steps)
+    against the oracle's closed loop with the same refinement (the CPU baseline's code): actions
+    and drone states within 1e-3 relative (the refined plans agree to rounding, which the closed
+    loop carries forward)."""
+    sc = pkg.scenarios
+    scn = sc.c4_eight_drones(4096)
+    n = len(scn.instances)
+    say(f"case: {scn.name}: 3 refined closed-loop steps against the oracle's")
+    eng = pkg.Engine(scn)
+    eng.set_refine(3, 0.05, 10)
+    g_acts = eng.step(3, advance=True)
+    g_drones = eng.get_drones()
+    eng.close()
+    drones = (pkg.abi.DroneState * n)(*scn.drones)
+    o_acts, _ = oracle.world_run(scn, drones, _flat_nominals(scn), [0] * n, [i.lambda_ for i in scn.instances], 3, threads=16,
+                                 refine=(3, 0.05, 10))
+    e_act, e_state = rel_err(g_acts, o_acts), rel_err(_state_rows(g_drones), _state_rows(drones))
+    say(f"  after 3 steps: actions max rel {e_act:.2e}, drone states max rel {e_state:.2e}")
+    assert e_act <= 1e-3 and e_state <= 1e-3

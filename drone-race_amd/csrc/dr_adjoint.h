@@ -127,7 +127,7 @@ DR_HD void quad_step_vjp(const QuadConst& C, const QState& x, const float u[4], 
     const float nx = fma_(C.half_dt, dqx, qx);
     const float ny = fma_(C.half_dt, dqy, qy);
     const float nz = fma_(C.half_dt, dqz, qz);
-    const float inv = 1.0f / __builtin_sqrtf(fma_(nw, nw, fma_(nx, nx, fma_(ny, ny, nz * nz))));
+    const float inv = rsqrt_nr(fma_(nw, nw, fma_(nx, nx, fma_(ny, ny, nz * nz))));
     const float o0 = nw * inv, o1 = nx * inv, o2 = ny * inv, o3 = nz * inv;
 
     // w' = w + k (u - w)

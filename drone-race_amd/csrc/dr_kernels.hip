@@ -640,11 +640,16 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
 // dJ/dU are those of cost_grad_lane (same forward, same terminal rule, same
 // vector-Jacobian products); the work is spread over the lanes so that only
 // the forward recurrence of the dynamics stays serial:
-//  A  lane 0 runs the dynamics and the gate counter over all T steps and
-//     writes the state at every ck-th step (the checkpoints) into the state
-//     table; no cost is evaluated here.
+//  P  the iterate's controls are clamped, one lane per step.
+//  A  lane 0 runs the dynamics (attitude, rates, velocity, position) over all
+//     T steps and writes the state at every ck-th step (the checkpoints) into
+//     the state table; neither the gate counter nor any cost is evaluated
+//     here, so the serial chain is the recurrence of the dynamics alone.
 //  B  one lane per checkpoint segment recomputes the segment's states from
 //     its checkpoint into the table.
+//  G  one lane per step tests the step's move against every gate; lane 0
+//     then runs the gate counter over those crossing bits (a few integer
+//     operations per step) and writes gate and laps into the table.
 //  C  one lane per step t: terminal test, discounted running cost and dJ/dp
 //     of the running cost of the post-step state x_{t+1}; the first terminal
 //     step t_end ends the trajectory (cost_grad_lane's break); wave 0 sums J.
@@ -709,33 +714,41 @@ struct RefRow
     }
 };
 
-// Steps [t0, t1) of the forward pass from s (clamp, dynamics, gate counter).
-// The state after step t goes to xs[t + 1] when it is kept (every `every`-th
-// step, and only for t + 1 < stop) and to the junk slot otherwise, and the
-// next control is loaded one step ahead: every LDS access of the loop is
-// issued on every step and none is waited for before its use.  The track's
-// gates are held in registers; the next gate is selected from them when one
-// is crossed.
-__device__ void ref_forward(const QuadConst& C, const dr_track& tr, float margin, const float4* us, float4* xs, float4* junk,
-                            int t0, int t1, int every, int stop, QState& s)
+// The dynamic part (p, q, v, w) of state slot t of the state table; the
+// fourth word's gate and lap fields are left for the gate pass.
+__device__ __forceinline__ void put_dyn(float4* xs, int t, const QState& s)
 {
-    GateRegs gr;
-    gr.init(tr, s.gate);
-    float4 u = us[t0];
-    int left = every;
+    xs[t * 4 + 0] = make_float4(s.p[0], s.p[1], s.p[2], s.q[0]);
+    xs[t * 4 + 1] = make_float4(s.q[1], s.q[2], s.q[3], s.v[0]);
+    xs[t * 4 + 2] = make_float4(s.v[1], s.v[2], s.w[0], s.w[1]);
+    xs[t * 4 + 3].x = s.w[2];
+}
+
+__device__ __forceinline__ void get_dyn(const float4* xs, int t, QState& s)
+{
+    const float4 a = xs[t * 4 + 0], b = xs[t * 4 + 1], c = xs[t * 4 + 2];
+    s.p[0] = a.x; s.p[1] = a.y; s.p[2] = a.z; s.q[0] = a.w;
+    s.q[1] = b.x; s.q[2] = b.y; s.q[3] = b.z; s.v[0] = b.w;
+    s.v[1] = c.x; s.v[2] = c.y; s.w[0] = c.z; s.w[1] = c.w;
+    s.w[2] = xs[t * 4 + 3].x;
+}
+
+// Dynamics of steps [t0, t1) from s over the clamped controls uc: the
+// attitude, rates, velocity and position recurrences only (the gate counter
+// does not feed back into them and is run by the gate pass).  With keep_all
+// every state x_{t+1}, t + 1 < t1 or t1 == T, is written to the table; else
+// none is.  The next control is loaded one step ahead.
+__device__ __forceinline__ void ref_dynamics(const QuadConst& C, const float4* uc, float4* xs, int t0, int t1, int T, bool keep_all,
+                                             QState& s)
+{
+    float4 u = uc[t0];
     for (int t = t0; t < t1; ++t)
     {
-        const float4 un = us[t + 1 < t1 ? t + 1 : t];
-        const float uin[4] = {u.x, u.y, u.z, u.w};
-        float uc[4];
-        clamp_control(C, uin, uc);
-        const float p0[3] = {s.p[0], s.p[1], s.p[2]};
-        quad_step(C, s, uc);
-        gr.step(margin, p0, s);
-        --left;
-        const bool keep = left == 0 && t + 1 < stop;
-        left = left == 0 ? every : left;
-        put_state(keep ? xs + (t + 1) * 4 : junk, 0, 1, s, 0.0f);
+        const float4 un = uc[t + 1 < t1 ? t + 1 : t];
+        const float uu[4] = {u.x, u.y, u.z, u.w};
+        quad_step(C, s, uu);
+        if (keep_all && (t + 1 < t1 || t1 == T))
+            put_dyn(xs, t + 1, s);
         u = un;
     }
 }
@@ -752,7 +765,8 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     __shared__ float s_psi[REF_SEGS * REF_COMP * 16];
     __shared__ float s_lam[REF_SEGS * 16];     // lam_b of each reverse segment
     __shared__ float4 s_op[MAX_OPP], s_ov[MAX_OPP];
-    __shared__ float4 s_junk[16]; // sink of the forward pass's unkept states
+    __shared__ float4 s_uc[DR_MAX_T];    // the clamped controls the forward pass integrates
+    __shared__ uint32_t s_cross[DR_MAX_T]; // bit g: the move x_t -> x_{t+1} crosses gate g
     __shared__ float4 s_ubest[DR_MAX_T]; // the lowest-cost sequence evaluated so far
     __shared__ float s_jbest;
     __shared__ int s_tend, s_better;
@@ -793,22 +807,73 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
 
     for (int it = 0;; ++it)
     {
-        // A: forward pass, checkpoints only
+        // P: the clamped controls of this iterate
+        for (int t = tid; t < T; t += REF_BLOCK)
+        {
+            const float4 u4 = s_u[t];
+            const float in[4] = {u4.x, u4.y, u4.z, u4.w};
+            float out[4];
+            clamp_control(C, in, out);
+            s_uc[t] = make_float4(out[0], out[1], out[2], out[3]);
+        }
+        __syncthreads();
+        // A: forward pass of the dynamics, writing the checkpoints
         if (tid == 0)
         {
             QState s = x0;
-            put_state(s_x, 0, 1, s, 0.0f);
-            ref_forward(C, s_track, margin, s_u, s_x, s_junk, 0, T, ck, T + 1, s);
+            for (int c = 0; c < n_ck; ++c)
+            {
+                const int t0 = c * ck;
+                put_dyn(s_x, t0, s);
+                ref_dynamics(C, s_uc, s_x, t0, min(t0 + ck, T), T, false, s);
+            }
             s_tend = T - 1;
         }
         __syncthreads();
         // B: states between checkpoints, one lane per segment
         if (tid < n_ck)
         {
-            const int t0 = tid * ck, t1 = min(t0 + ck, T);
-            float w;
-            QState s = get_state(s_x, t0, 1, &w);
-            ref_forward(C, s_track, margin, s_u, s_x, s_junk + 4 * (tid & 3), t0, t1, 1, t1 == T ? T + 1 : t1, s);
+            const int t0 = tid * ck;
+            QState s;
+            get_dyn(s_x, t0, s);
+            ref_dynamics(C, s_uc, s_x, t0, min(t0 + ck, T), T, true, s);
+        }
+        __syncthreads();
+        // G: gate counter.  One lane per step tests the move x_t -> x_{t+1}
+        // against every gate (the test reads only the two positions and the
+        // gate); then lane 0 runs the counter over the crossing bits, exactly
+        // as the per-step update (next gate = gate + 1 mod n, a crossing of
+        // gate 0 counts a lap), and writes gate and laps into the table.
+        for (int t = tid; t < T; t += REF_BLOCK)
+        {
+            const float4 a = s_x[t * 4], b = s_x[(t + 1) * 4];
+            const float p0[3] = {a.x, a.y, a.z}, p1[3] = {b.x, b.y, b.z};
+            uint32_t bits = 0u;
+            for (int g = 0; g < s_track.n_gates; ++g)
+                bits |= gate_crossed(s_track.gates[g], margin, p0, p1) ? (1u << g) : 0u;
+            s_cross[t] = bits;
+        }
+        __syncthreads();
+        if (tid == 0)
+        {
+            int gate = x0.gate, laps = x0.laps;
+            const int ng = s_track.n_gates;
+            s_x[3].y = u2f((uint32_t)gate);
+            s_x[3].z = u2f((uint32_t)laps);
+            uint32_t bits = s_cross[0];
+            for (int t = 0; t < T; ++t)
+            {
+                const uint32_t nb = s_cross[t + 1 < T ? t + 1 : t];
+                const int nx = gate + 1 == ng ? 0 : gate + 1;
+                if ((bits >> nx) & 1u)
+                {
+                    gate = nx;
+                    laps += nx == 0 ? 1 : 0;
+                }
+                s_x[(t + 1) * 4 + 3].y = u2f((uint32_t)gate);
+                s_x[(t + 1) * 4 + 3].z = u2f((uint32_t)laps);
+                bits = nb;
+            }
         }
         __syncthreads();
         // C: terminal test and running-cost gradient, one lane per step

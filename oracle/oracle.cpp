@@ -197,7 +197,7 @@ void vjp_step(const Model<S>& M, const St<S>& x, const S u[4], G13<S>& gx, S gu[
     S n[4];
     for (int i = 0; i < 4; ++i)
         n[i] = fm(M.half_dt, dq[i], x.q[i]);
-    const S inv = one / sq_root(fm(n[0], n[0], fm(n[1], n[1], fm(n[2], n[2], n[3] * n[3]))));
+    const S inv = inv_sqrt(fm(n[0], n[0], fm(n[1], n[1], fm(n[2], n[2], n[3] * n[3]))));
     S o[4];
     for (int i = 0; i < 4; ++i)
         o[i] = n[i] * inv;
@@ -468,6 +468,8 @@ void or_normals4(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t c2
 float or_log32(float x) { return log32(x); }
 float or_exp32(float x) { return exp32(x); }
 void or_sincos_turns(uint32_t k, float* s, float* c) { sincos_turns(k, s, c); }
+
+float or_inv_sqrt32(float x) { return inv_sqrt<float>(x); }
 
 // One clamped Euler step + gate update on a 13-vector (p, q, v, w).
 void or_quad_step(int f64, const dr_quad_params* P, const dr_track* tr, float margin, double* x, int* gate, int* laps, const double* u_in)

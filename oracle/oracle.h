@@ -183,6 +183,26 @@ inline float sq_root<float>(float x)
 inline float fm(float a, float b, float c) { return fmaf(a, b, c); }
 inline double fm(double a, double b, double c) { return fma(a, b, c); }
 
+// 1/sqrt(x) of the attitude renormalisation.  FP32: the engine's estimate
+// from the bit pattern (Lomont's constant 0x5f375a86, Lomont 2003) and three
+// Newton-Raphson steps in correction form y <- y + y (1/2 - (x/2) y^2), in
+// the same operations; FP64: the exact value.
+template <class S>
+inline S inv_sqrt(S x)
+{
+    return (S)1 / sq_root(x);
+}
+template <>
+inline float inv_sqrt<float>(float x)
+{
+    const float hx = 0.5f * x;
+    float y = from_bits(0x5f375a86u - (bits(x) >> 1));
+    y = fmaf(y, fmaf(-hx, y * y, 0.5f), y);
+    y = fmaf(y, fmaf(-hx, y * y, 0.5f), y);
+    y = fmaf(y, fmaf(-hx, y * y, 0.5f), y);
+    return y;
+}
+
 template <class S>
 inline S mn(S a, S b)
 {
@@ -276,7 +296,7 @@ inline void euler(const Model<S>& M, St<S>& s, const S u[4])
     S n[4];
     for (int i = 0; i < 4; ++i)
         n[i] = fm(M.half_dt, dq[i], s.q[i]);
-    const S inv = one / sq_root(fm(n[0], n[0], fm(n[1], n[1], fm(n[2], n[2], n[3] * n[3]))));
+    const S inv = inv_sqrt(fm(n[0], n[0], fm(n[1], n[1], fm(n[2], n[2], n[3] * n[3]))));
     for (int i = 0; i < 4; ++i)
         s.q[i] = n[i] * inv;
     s.w[0] = fm(M.rate_k, u[1] - wx, wx);
@@ -501,6 +521,9 @@ inline bool operator<=(Cn a, Cn b) { ++g_tally.flops; return a.v <= b.v; }
 inline bool operator>=(Cn a, Cn b) { ++g_tally.flops; return a.v >= b.v; }
 template <>
 inline Cn sq_root<Cn>(Cn x) { ++g_tally.flops; return Cn(sqrt(x.v)); }
+// the FP32 inverse square root's work: x/2, then per Newton step y^2 and two fmas (2 each)
+template <>
+inline Cn inv_sqrt<Cn>(Cn x) { g_tally.flops += 1 + 3 * 5; return Cn(1.0 / sqrt(x.v)); }
 inline bool tiny(Cn x) { ++g_tally.flops; return !(fabs(x.v) >= 1e-10); }
 inline Cn fm(Cn a, Cn b, Cn c) { g_tally.flops += 2; return Cn(fma(a.v, b.v, c.v)); }
 

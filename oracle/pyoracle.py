@@ -57,6 +57,8 @@ def lib(abi, path=None):
     L.or_exp32.restype = C.c_float
     L.or_exp32.argtypes = [C.c_float]
     L.or_sincos_turns.argtypes = [U, F, F]
+    L.or_inv_sqrt32.restype = C.c_float
+    L.or_inv_sqrt32.argtypes = [C.c_float]
     L.or_quad_step.argtypes = [I, C.POINTER(abi.QuadParams), C.POINTER(abi.Track), C.c_float, D, C.POINTER(I),
                                C.POINTER(I), D]
     L.or_progress.restype = C.c_double
@@ -160,6 +162,9 @@ class Oracle:
 
     def exp32(self, x):
         return self.L.or_exp32(x)
+
+    def inv_sqrt32(self, x):
+        return self.L.or_inv_sqrt32(x)
 
     def sincos(self, k):
         s, c = C.c_float(), C.c_float()

@@ -36,12 +36,12 @@ METRIC = "control steps/s, batched+adjoint vs plain MPPI at equal mean trajector
 # oracle's op-counting scalar on the first control step of this workload, plus
 # 96 for the Philox Box-Muller noise).  tests/test_bench_constants.py keeps it
 # equal to oracle/count_ops.py.
-FLOPS_PER_SAMPLE_STEP = 284.07
+FLOPS_PER_SAMPLE_STEP = 292.07
 # The same for the 8-drone workload at K=4096 (seven opponents in every running cost), and the
 # oracle-counted flops of one adjoint-refinement launch over its 8 controllers (oracle/count_ops.py
 # count_refine); tests/test_bench_constants.py keeps both equal to the counter.
-C4_FLOPS_PER_SAMPLE_STEP = 348.12
-C4_REFINE_FLOPS = 2332640
+C4_FLOPS_PER_SAMPLE_STEP = 356.12
+C4_REFINE_FLOPS = 2420640
 # HBM bytes per rollout sample-step the rollout kernel must move: the float4
 # noise row it writes (read back once by the weighted average).
 ROLLOUT_BYTES_PER_SAMPLE_STEP = 16.0

@@ -3,7 +3,8 @@
 // state, noise-free.  The forward pass is dr_quad.h's; this file holds the
 // vector-Jacobian products of its pieces:
 //   clamp        1 inside [lo, hi] (bounds included), 0 outside
-//   Euler step   quad_step's expressions (an fma differentiates as a*b + c)
+//   Euler step   quad_step's expressions (an fma differentiates as a*b + c;
+//                the inverse square root as the exact 1/sqrt)
 //   running cost opponent proximity, near-boundary term (gradient of the
 //                active side of bd = min(d_opp, d_ground)); collision and win
 //                terms are constants
@@ -118,17 +119,17 @@ DR_HD void quad_step_vjp(const QuadConst& C, const QState& x, const float u[4], 
     const float b3z = fma_(-2.0f, fma_(qx, qx, qy * qy), 1.0f);
     const float at = u[0] * C.inv_mass;
 
-    // recompute the normalisation of the attitude update
-    const float dqw = -fma_(qx, wx, fma_(qy, wy, qz * wz));
-    const float dqx = fma_(qw, wx, fma_(qy, wz, -(qz * wy)));
-    const float dqy = fma_(qw, wy, fma_(qz, wx, -(qx * wz)));
-    const float dqz = fma_(qw, wz, fma_(qx, wy, -(qy * wx)));
-    const float nw = fma_(C.half_dt, dqw, qw);
-    const float nx = fma_(C.half_dt, dqx, qx);
-    const float ny = fma_(C.half_dt, dqy, qy);
-    const float nz = fma_(C.half_dt, dqz, qz);
-    const float inv = rsqrt_nr(fma_(nw, nw, fma_(nx, nx, fma_(ny, ny, nz * nz))));
-    const float o0 = nw * inv, o1 = nx * inv, o2 = ny * inv, o3 = nz * inv;
+    // recompute the attitude step's pieces: increment (h w, rs), product n, correction c
+    const float h = C.half_dt;
+    const float hx = h * wx, hy = h * wy, hz = h * wz;
+    const float rs = rsqrt_nr(fma_(hx, hx, fma_(hy, hy, fma_(hz, hz, 1.0f))));
+    const float n0 = fma_(-qx, hx, fma_(-qy, hy, fma_(-qz, hz, qw)));
+    const float n1 = fma_(qw, hx, fma_(qy, hz, fma_(-qz, hy, qx)));
+    const float n2 = fma_(qw, hy, fma_(qz, hx, fma_(-qx, hz, qy)));
+    const float n3 = fma_(qw, hz, fma_(qx, hy, fma_(-qy, hx, qz)));
+    const float q2 = fma_(qw, qw, fma_(qx, qx, fma_(qy, qy, qz * qz)));
+    const float c = fma_(-0.5f, q2, 1.5f);
+    const float sc = rs * c;
 
     // w' = w + k (u - w)
     QGrad g;
@@ -140,22 +141,22 @@ DR_HD void quad_step_vjp(const QuadConst& C, const QState& x, const float u[4], 
     g.w[1] = (1.0f - kk) * gx.w[1];
     g.w[2] = (1.0f - kk) * gx.w[2];
 
-    // q' = n / |n|: dn = (dq' - q' (q'.dq')) / |n|
-    const float dot = fma_(o0, gx.q[0], fma_(o1, gx.q[1], fma_(o2, gx.q[2], o3 * gx.q[3])));
-    const float gn0 = (gx.q[0] - o0 * dot) * inv;
-    const float gn1 = (gx.q[1] - o1 * dot) * inv;
-    const float gn2 = (gx.q[2] - o2 * dot) * inv;
-    const float gn3 = (gx.q[3] - o3 * dot) * inv;
-    // n = q + h dq(q, w)
-    const float h = C.half_dt;
-    const float hw = h * gn0, hx = h * gn1, hy = h * gn2, hz = h * gn3;
-    g.q[0] = gn0 + (wx * hx + wy * hy + wz * hz);
-    g.q[1] = gn1 + (-wx * hw - wz * hy + wy * hz);
-    g.q[2] = gn2 + (-wy * hw + wz * hx - wx * hz);
-    g.q[3] = gn3 + (-wz * hw - wy * hx + wx * hy);
-    g.w[0] = g.w[0] + (-qx * hw + qw * hx + qz * hy - qy * hz);
-    g.w[1] = g.w[1] + (-qy * hw - qz * hx + qw * hy + qx * hz);
-    g.w[2] = g.w[2] + (-qz * hw + qy * hx - qx * hy + qw * hz);
+    // q' = n sc, sc = rs c: through n, then through sc to rs (r2 = 1 + |h w|^2) and c (c = 3/2 - |q|^2/2)
+    const float gn0 = sc * gx.q[0], gn1 = sc * gx.q[1], gn2 = sc * gx.q[2], gn3 = sc * gx.q[3];
+    const float gsc = fma_(n0, gx.q[0], fma_(n1, gx.q[1], fma_(n2, gx.q[2], n3 * gx.q[3])));
+    const float gc = rs * gsc;
+    const float gr2 = (-0.5f * (rs * rs * rs)) * (c * gsc);
+    // n = q (x) (1, h w)
+    g.q[0] = (gn0 + hx * gn1 + hy * gn2 + hz * gn3) - qw * gc;
+    g.q[1] = (-hx * gn0 + gn1 - hz * gn2 + hy * gn3) - qx * gc;
+    g.q[2] = (-hy * gn0 + hz * gn1 + gn2 - hx * gn3) - qy * gc;
+    g.q[3] = (-hz * gn0 - hy * gn1 + hx * gn2 + gn3) - qz * gc;
+    const float ghx = (-qx * gn0 + qw * gn1 + qz * gn2 - qy * gn3) + 2.0f * hx * gr2;
+    const float ghy = (-qy * gn0 - qz * gn1 + qw * gn2 + qx * gn3) + 2.0f * hy * gr2;
+    const float ghz = (-qz * gn0 + qy * gn1 - qx * gn2 + qw * gn3) + 2.0f * hz * gr2;
+    g.w[0] = g.w[0] + h * ghx;
+    g.w[1] = g.w[1] + h * ghy;
+    g.w[2] = g.w[2] + h * ghz;
 
     // v' = v + dt a, a = at b3 - drag v - g e3; p' = p + dt v
     const float ga0 = C.dt * gx.v[0], ga1 = C.dt * gx.v[1], ga2 = C.dt * gx.v[2];

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -76,6 +77,22 @@ int check_track(const dr_track& t, int idx)
     for (int g = 0; g < t.n_gates; ++g)
         if (!(t.gates[g].radius > 0.0f))
             return fail(DR_EINVAL, "track " + std::to_string(idx) + ": gate radius must be > 0");
+    return DR_OK;
+}
+
+int check_drones(const dr_drone_state* s, int n)
+{
+    for (int d = 0; d < n; ++d)
+    {
+        const dr_drone_state& x = s[d];
+        const float vals[13] = {x.p[0], x.p[1], x.p[2], x.q[0], x.q[1], x.q[2], x.q[3], x.v[0], x.v[1], x.v[2], x.w[0], x.w[1], x.w[2]};
+        for (float v : vals)
+            if (!std::isfinite(v))
+                return fail(DR_EINVAL, "drone " + std::to_string(d) + ": state is not finite");
+        const double q2 = (double)x.q[0] * x.q[0] + (double)x.q[1] * x.q[1] + (double)x.q[2] * x.q[2] + (double)x.q[3] * x.q[3];
+        if (!(std::fabs(q2 - 1.0) <= 1e-2))
+            return fail(DR_EINVAL, "drone " + std::to_string(d) + ": attitude quaternion must be a unit quaternion");
+    }
     return DR_OK;
 }
 
@@ -525,6 +542,8 @@ int dr_engine_set_drones(dr_engine* e, const dr_drone_state* states, int n_drone
 {
     if (!e || !states || n_drones != e->n_drones)
         return fail(DR_EINVAL, "set_drones: bad handle, pointer or count");
+    if (int rc = check_drones(states, n_drones))
+        return rc;
     HIPCHK(hipMemcpyAsync(e->d_drones.p, states, n_drones * sizeof(dr_drone_state), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return DR_OK;

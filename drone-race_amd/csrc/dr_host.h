@@ -16,6 +16,9 @@ int host_fail(int code, const std::string& msg);
 
 // DR_OK, or DR_EINVAL with a message naming track idx.
 int check_track(const dr_track& t, int idx);
+// Drone states must hold a unit attitude quaternion (|q|^2 within 1e-2 of 1)
+// and finite values: the attitude recurrence renormalises to first order.
+int check_drones(const dr_drone_state* s, int n);
 
 // DR_OK, or DR_EINVAL naming the first invalid field of a StratRace2D environment.
 int check_sr_env(const dr_sr_env& env);

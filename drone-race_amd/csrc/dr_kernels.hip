@@ -641,15 +641,19 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
 // vector-Jacobian products); the work is spread over the lanes so that only
 // the forward recurrence of the dynamics stays serial:
 //  P  the iterate's controls are clamped, one lane per step.
-//  A  lane 0 runs the dynamics (attitude, rates, velocity, position) over all
-//     T steps and writes the state at every ck-th step (the checkpoints) into
-//     the state table; neither the gate counter nor any cost is evaluated
-//     here, so the serial chain is the recurrence of the dynamics alone.
+//  A  the dynamics (attitude, rates, velocity, position) over all T steps,
+//     writing the state at every ck-th step (the checkpoints) into the state
+//     table, as a three-wave pipeline over chunks of ck steps: body rates and
+//     attitude increments (lane 128), attitude one chunk behind (lane 0),
+//     velocity and position two chunks behind (lane 64); neither the gate
+//     counter nor any cost is evaluated here, so each wave's serial chain is
+//     one part of the dynamics.
 //  B  one lane per checkpoint segment recomputes the segment's states from
 //     its checkpoint into the table.
-//  G  one lane per step tests the step's move against every gate; lane 0
-//     then runs the gate counter over those crossing bits (a few integer
-//     operations per step) and writes gate and laps into the table.
+//  G  every (step, gate) pair's crossing test in parallel; wave 0 then runs
+//     the gate counter by searching the crossing bits for the next gate's
+//     crossing (one ballot per 64 steps) and every state slot takes the gate
+//     and laps after the crossings before it.
 //  C  one lane per step t: terminal test, discounted running cost and dJ/dp
 //     of the running cost of the post-step state x_{t+1}; the first terminal
 //     step t_end ends the trajectory (cost_grad_lane's break); wave 0 sums J.
@@ -765,8 +769,12 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     __shared__ float s_psi[REF_SEGS * REF_COMP * 16];
     __shared__ float s_lam[REF_SEGS * 16];     // lam_b of each reverse segment
     __shared__ float4 s_op[MAX_OPP], s_ov[MAX_OPP];
-    __shared__ float4 s_uc[DR_MAX_T];    // the clamped controls the forward pass integrates
+    __shared__ float4 s_uc[DR_MAX_T + 1];  // the clamped controls the forward pass integrates (+1: read-ahead slot)
     __shared__ uint32_t s_cross[DR_MAX_T]; // bit g: the move x_t -> x_{t+1} crosses gate g
+    __shared__ float4 s_qa[DR_MAX_T + 1];  // attitude at the start of each step (forward pass; +1: read-ahead slot)
+    __shared__ float4 s_inc[DR_MAX_T + 1]; // attitude increment (h w, 1/|(1, h w)|) of each step (forward pass; +1: read-ahead)
+    __shared__ int s_xstep[DR_MAX_T], s_xgate[DR_MAX_T], s_xlaps[DR_MAX_T]; // gate crossings: step, gate, laps after
+    __shared__ int s_nx;
     __shared__ float4 s_ubest[DR_MAX_T]; // the lowest-cost sequence evaluated so far
     __shared__ float s_jbest;
     __shared__ int s_tend, s_better;
@@ -803,11 +811,14 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     }
     const QState x0 = state_from_abi(drones[I.drone]);
     const int n_ck = (T + ck - 1) / ck;
+    // forward-pass registers: attitude (lane 0), position and velocity (lane 64), body rates (lane 128)
+    float qa[4] = {x0.q[0], x0.q[1], x0.q[2], x0.q[3]}, wr[3] = {x0.w[0], x0.w[1], x0.w[2]};
+    float pt[3] = {x0.p[0], x0.p[1], x0.p[2]}, vt[3] = {x0.v[0], x0.v[1], x0.v[2]};
     __syncthreads();
 
     for (int it = 0;; ++it)
     {
-        // P: the clamped controls of this iterate
+        // P: the clamped controls of this iterate; crossing bits cleared
         for (int t = tid; t < T; t += REF_BLOCK)
         {
             const float4 u4 = s_u[t];
@@ -815,21 +826,95 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             float out[4];
             clamp_control(C, in, out);
             s_uc[t] = make_float4(out[0], out[1], out[2], out[3]);
+            s_cross[t] = 0u;
         }
         __syncthreads();
-        // A: forward pass of the dynamics, writing the checkpoints
+        // A: forward pass of the dynamics, writing the checkpoints, as a
+        // three-stage pipeline over chunks of ck steps (one barrier per chunk):
+        // lane 128 integrates the body rates (they read only the controls) and
+        // forms each step's attitude increment (h w, 1/|(1, h w)|) into s_inc;
+        // lane 0, one chunk behind, applies the increments to the attitude and
+        // passes the attitude of every step through s_qa; lane 64, two chunks
+        // behind, integrates velocity and position from that attitude.  Each
+        // writes its part of the checkpoint at the start of its chunk.  The
+        // operations are those of quad_step.
+        for (int c = 0; c <= n_ck + 1; ++c)
+        {
+            if (tid == 2 * WAVE && c < n_ck)
+            {
+                const int t0 = c * ck, t1 = min(t0 + ck, T);
+                s_x[t0 * 4 + 2].z = wr[0];
+                s_x[t0 * 4 + 2].w = wr[1];
+                s_x[t0 * 4 + 3].x = wr[2];
+                float4 u = s_uc[t0];
+#pragma unroll 2
+                for (int t = t0; t < t1; ++t)
+                {
+                    const float4 un = s_uc[t + 1]; // read ahead (slot T is never used)
+                    s_inc[t] = attitude_increment(C, wr);
+                    const float uu[4] = {u.x, u.y, u.z, u.w};
+                    rates_step(C, wr, uu);
+                    u = un;
+                }
+            }
+            if (tid == 0 && c >= 1 && c <= n_ck)
+            {
+                const int t0 = (c - 1) * ck, t1 = min(t0 + ck, T);
+                s_x[t0 * 4 + 0].w = qa[0];
+                s_x[t0 * 4 + 1].x = qa[1];
+                s_x[t0 * 4 + 1].y = qa[2];
+                s_x[t0 * 4 + 1].z = qa[3];
+                float4 inc = s_inc[t0];
+#pragma unroll 2
+                for (int t = t0; t < t1; ++t)
+                {
+                    const float4 nxt = s_inc[t + 1]; // read ahead (written in the next chunk or never used)
+                    s_qa[t] = make_float4(qa[0], qa[1], qa[2], qa[3]);
+                    attitude_apply(qa, inc);
+                    inc = nxt;
+                }
+            }
+            if (tid == WAVE && c >= 2)
+            {
+                const int t0 = (c - 2) * ck, t1 = min(t0 + ck, T);
+                s_x[t0 * 4 + 0].x = pt[0];
+                s_x[t0 * 4 + 0].y = pt[1];
+                s_x[t0 * 4 + 0].z = pt[2];
+                s_x[t0 * 4 + 1].w = vt[0];
+                s_x[t0 * 4 + 2].x = vt[1];
+                s_x[t0 * 4 + 2].y = vt[2];
+                float4 u = s_uc[t0];
+                float4 q = s_qa[t0];
+#pragma unroll 2
+                for (int t = t0; t < t1; ++t)
+                {
+                    const float4 un = s_uc[t + 1]; // read ahead (unused past the chunk)
+                    const float4 qn = s_qa[t + 1];
+                    const float uu[4] = {u.x, u.y, u.z, u.w};
+                    const float qq[4] = {q.x, q.y, q.z, q.w};
+                    quad_translation_step(C, qq, pt, vt, uu);
+                    u = un;
+                    q = qn;
+                }
+            }
+            __syncthreads();
+        }
         if (tid == 0)
         {
-            QState s = x0;
-            for (int c = 0; c < n_ck; ++c)
-            {
-                const int t0 = c * ck;
-                put_dyn(s_x, t0, s);
-                ref_dynamics(C, s_uc, s_x, t0, min(t0 + ck, T), T, false, s);
-            }
             s_tend = T - 1;
+            // the next pass starts from x0 again
+            for (int i = 0; i < 4; ++i)
+                qa[i] = x0.q[i];
         }
-        __syncthreads();
+        if (tid == 2 * WAVE)
+            for (int i = 0; i < 3; ++i)
+                wr[i] = x0.w[i];
+        if (tid == WAVE)
+            for (int i = 0; i < 3; ++i)
+            {
+                pt[i] = x0.p[i];
+                vt[i] = x0.v[i];
+            }
         // B: states between checkpoints, one lane per segment
         if (tid < n_ck)
         {
@@ -839,41 +924,74 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             ref_dynamics(C, s_uc, s_x, t0, min(t0 + ck, T), T, true, s);
         }
         __syncthreads();
-        // G: gate counter.  One lane per step tests the move x_t -> x_{t+1}
-        // against every gate (the test reads only the two positions and the
-        // gate); then lane 0 runs the counter over the crossing bits, exactly
-        // as the per-step update (next gate = gate + 1 mod n, a crossing of
-        // gate 0 counts a lap), and writes gate and laps into the table.
-        for (int t = tid; t < T; t += REF_BLOCK)
+        // G: gate counter.  The crossing test of a step's move reads only the
+        // two positions and the gate, so every (step, gate) pair is tested in
+        // parallel (lpt lanes per step).  Then wave 0 runs the counter: from
+        // step t the next crossing is the first t' >= t whose move crosses the
+        // next gate (the counter cannot change in between), found with one
+        // ballot per 64 steps; the crossings are listed, and every state slot
+        // takes the gate and laps after the crossings before it -- exactly the
+        // per-step update (next gate = gate + 1 mod n, a crossing of gate 0
+        // counts a lap).
         {
-            const float4 a = s_x[t * 4], b = s_x[(t + 1) * 4];
-            const float p0[3] = {a.x, a.y, a.z}, p1[3] = {b.x, b.y, b.z};
-            uint32_t bits = 0u;
-            for (int g = 0; g < s_track.n_gates; ++g)
-                bits |= gate_crossed(s_track.gates[g], margin, p0, p1) ? (1u << g) : 0u;
-            s_cross[t] = bits;
+            const int ng = s_track.n_gates;
+            const int lpt = T >= REF_BLOCK ? 1 : REF_BLOCK / T;
+            for (int i = tid; i < T * lpt; i += REF_BLOCK)
+            {
+                const int t = i / lpt, sub = i - t * lpt;
+                const float4 a4 = s_x[t * 4], b4 = s_x[(t + 1) * 4];
+                const float p0[3] = {a4.x, a4.y, a4.z}, p1[3] = {b4.x, b4.y, b4.z};
+                uint32_t bits = 0u;
+                for (int g = sub; g < ng; g += lpt)
+                    bits |= gate_crossed(s_track.gates[g], margin, p0, p1) ? (1u << g) : 0u;
+                if (bits)
+                    atomicOr(&s_cross[t], bits);
+            }
         }
         __syncthreads();
-        if (tid == 0)
+        if (tid < WAVE)
+        {
+            const int ng = s_track.n_gates;
+            int gate = x0.gate, laps = x0.laps, t = 0, n_x = 0;
+            while (t < T)
+            {
+                const int nx = gate + 1 == ng ? 0 : gate + 1;
+                int found = -1;
+                for (int c0 = t & ~(WAVE - 1); c0 < T && found < 0; c0 += WAVE)
+                {
+                    const int tt = c0 + tid;
+                    const bool hit = tt >= t && tt < T && ((s_cross[tt] >> nx) & 1u) != 0u;
+                    const unsigned long long m = __ballot(hit);
+                    if (m)
+                        found = c0 + __ffsll((long long)m) - 1;
+                }
+                if (found < 0)
+                    break;
+                gate = nx;
+                laps += nx == 0 ? 1 : 0;
+                if (tid == 0)
+                {
+                    s_xstep[n_x] = found;
+                    s_xgate[n_x] = gate;
+                    s_xlaps[n_x] = laps;
+                }
+                ++n_x;
+                t = found + 1;
+            }
+            if (tid == 0)
+                s_nx = n_x;
+        }
+        __syncthreads();
+        for (int k = tid; k <= T; k += REF_BLOCK)
         {
             int gate = x0.gate, laps = x0.laps;
-            const int ng = s_track.n_gates;
-            s_x[3].y = u2f((uint32_t)gate);
-            s_x[3].z = u2f((uint32_t)laps);
-            uint32_t bits = s_cross[0];
-            for (int t = 0; t < T; ++t)
+            for (int i = 0; i < s_nx && s_xstep[i] < k; ++i)
             {
-                const uint32_t nb = s_cross[t + 1 < T ? t + 1 : t];
-                const int nx = gate + 1 == ng ? 0 : gate + 1;
-                if ((bits >> nx) & 1u)
-                {
-                    gate = nx;
-                    laps += nx == 0 ? 1 : 0;
-                }
-                s_x[(t + 1) * 4 + 3].y = u2f((uint32_t)gate);
-                s_x[(t + 1) * 4 + 3].z = u2f((uint32_t)laps);
-                bits = nb;
+                gate = s_xgate[i];
+                laps = s_xlaps[i];
             }
+            s_x[k * 4 + 3].y = u2f((uint32_t)gate);
+            s_x[k * 4 + 3].z = u2f((uint32_t)laps);
         }
         __syncthreads();
         // C: terminal test and running-cost gradient, one lane per step

@@ -607,6 +607,8 @@ int dr_pr_set_drones(dr_prengine* e, const dr_drone_state* states, int n_drones)
         return host_fail(DR_ESTATE, "set_drones: a StratRace2D engine takes dr_pr_set_sr_states");
     if (!e || !states || n_drones != e->n_drones)
         return host_fail(DR_EINVAL, "set_drones: bad handle, pointer or count");
+    if (int rc = check_drones(states, n_drones))
+        return rc;
     return copy_in(e, e->d_drones.p, states, (size_t)n_drones);
 }
 

@@ -122,51 +122,83 @@ DR_HD void clamp_control(const QuadConst& C, const float in[4], float out[4])
     out[3] = clampf(in[3], -C.rate_max, C.rate_max);
 }
 
-// One explicit Euler step.  Position uses the old velocity, velocity the old
-// attitude, attitude the old body rates; the quaternion is renormalised
-// (rsqrt_nr).
+// One explicit Euler step, in parts that read disjoint parts of the state:
+//   translation  position from the old velocity, velocity from the old
+//                attitude and velocity;
+//   attitude     q <- q (x) (1, h w) / |(1, h w)| * (3 - |q|^2) / 2, h = dt/2:
+//                the Euler step q + h q (x) (0, w) renormalised.  The norm of a
+//                quaternion product is the product of the norms, so the
+//                normaliser is 1/|(1, h w)| (rsqrt_nr) times the first-order
+//                correction (3 - |q|^2)/2 of 1/|q| (exact to 1e-14 for the unit
+//                attitude the recurrence keeps; a small initial error decays
+//                quadratically).  The increment (h w, 1/|(1, h w)|) depends on
+//                the body rates alone, so a caller may form it apart from the
+//                attitude recurrence (attitude_increment, attitude_apply);
+//   body rates   first-order tracking of the commanded rates.
 // Products that feed a sum are fused (fmaf) in the order written.
-DR_HD void quad_step(const QuadConst& C, QState& s, const float u[4])
+DR_HD void quad_translation_step(const QuadConst& C, const float q[4], float p[3], float v[3], const float u[4])
 {
-    const float qw = s.q[0], qx = s.q[1], qy = s.q[2], qz = s.q[3];
-    const float wx = s.w[0], wy = s.w[1], wz = s.w[2];
-
+    const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
     // thrust direction R(q) e3
     const float b3x = 2.0f * fma_(qx, qz, qw * qy);
     const float b3y = 2.0f * fma_(qy, qz, -(qw * qx));
     const float b3z = fma_(-2.0f, fma_(qx, qx, qy * qy), 1.0f);
     const float at = u[0] * C.inv_mass;
-    const float ax = fma_(at, b3x, -(C.drag * s.v[0]));
-    const float ay = fma_(at, b3y, -(C.drag * s.v[1]));
-    const float az = fma_(at, b3z, -(C.drag * s.v[2])) - C.gravity;
+    const float ax = fma_(at, b3x, -(C.drag * v[0]));
+    const float ay = fma_(at, b3y, -(C.drag * v[1]));
+    const float az = fma_(at, b3z, -(C.drag * v[2])) - C.gravity;
 
-    s.p[0] = fma_(C.dt, s.v[0], s.p[0]);
-    s.p[1] = fma_(C.dt, s.v[1], s.p[1]);
-    s.p[2] = fma_(C.dt, s.v[2], s.p[2]);
+    p[0] = fma_(C.dt, v[0], p[0]);
+    p[1] = fma_(C.dt, v[1], p[1]);
+    p[2] = fma_(C.dt, v[2], p[2]);
 
-    s.v[0] = fma_(C.dt, ax, s.v[0]);
-    s.v[1] = fma_(C.dt, ay, s.v[1]);
-    s.v[2] = fma_(C.dt, az, s.v[2]);
+    v[0] = fma_(C.dt, ax, v[0]);
+    v[1] = fma_(C.dt, ay, v[1]);
+    v[2] = fma_(C.dt, az, v[2]);
+}
 
-    // q <- q + dt/2 * q (x) (0, w)
-    const float dqw = -fma_(qx, wx, fma_(qy, wy, qz * wz));
-    const float dqx = fma_(qw, wx, fma_(qy, wz, -(qz * wy)));
-    const float dqy = fma_(qw, wy, fma_(qz, wx, -(qx * wz)));
-    const float dqz = fma_(qw, wz, fma_(qx, wy, -(qy * wx)));
-    const float nw = fma_(C.half_dt, dqw, qw);
-    const float nx = fma_(C.half_dt, dqx, qx);
-    const float ny = fma_(C.half_dt, dqy, qy);
-    const float nz = fma_(C.half_dt, dqz, qz);
-    const float inv = rsqrt_nr(fma_(nw, nw, fma_(nx, nx, fma_(ny, ny, nz * nz))));
-    s.q[0] = nw * inv;
-    s.q[1] = nx * inv;
-    s.q[2] = ny * inv;
-    s.q[3] = nz * inv;
+// (h w, 1/|(1, h w)|) of the body rates w.
+DR_HD float4 attitude_increment(const QuadConst& C, const float w[3])
+{
+    const float hx = C.half_dt * w[0], hy = C.half_dt * w[1], hz = C.half_dt * w[2];
+    const float r2 = fma_(hx, hx, fma_(hy, hy, fma_(hz, hz, 1.0f)));
+    return make_float4(hx, hy, hz, rsqrt_nr(r2));
+}
 
-    // first-order body-rate tracking
-    s.w[0] = fma_(C.rate_k, u[1] - wx, wx);
-    s.w[1] = fma_(C.rate_k, u[2] - wy, wy);
-    s.w[2] = fma_(C.rate_k, u[3] - wz, wz);
+// q <- q (x) (1, h w) * (rs (3 - |q|^2)/2) for the increment (h w, rs).
+DR_HD void attitude_apply(float q[4], const float4 inc)
+{
+    const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
+    const float nw = fma_(-qx, inc.x, fma_(-qy, inc.y, fma_(-qz, inc.z, qw)));
+    const float nx = fma_(qw, inc.x, fma_(qy, inc.z, fma_(-qz, inc.y, qx)));
+    const float ny = fma_(qw, inc.y, fma_(qz, inc.x, fma_(-qx, inc.z, qy)));
+    const float nz = fma_(qw, inc.z, fma_(qx, inc.y, fma_(-qy, inc.x, qz)));
+    const float q2 = fma_(qw, qw, fma_(qx, qx, fma_(qy, qy, qz * qz)));
+    const float sc = inc.w * fma_(-0.5f, q2, 1.5f);
+    q[0] = nw * sc;
+    q[1] = nx * sc;
+    q[2] = ny * sc;
+    q[3] = nz * sc;
+}
+
+// first-order body-rate tracking
+DR_HD void rates_step(const QuadConst& C, float w[3], const float u[4])
+{
+    w[0] = fma_(C.rate_k, u[1] - w[0], w[0]);
+    w[1] = fma_(C.rate_k, u[2] - w[1], w[1]);
+    w[2] = fma_(C.rate_k, u[3] - w[2], w[2]);
+}
+
+DR_HD void quad_attitude_step(const QuadConst& C, float q[4], float w[3], const float u[4])
+{
+    attitude_apply(q, attitude_increment(C, w));
+    rates_step(C, w, u);
+}
+
+DR_HD void quad_step(const QuadConst& C, QState& s, const float u[4])
+{
+    quad_translation_step(C, s.q, s.p, s.v, u);
+    quad_attitude_step(C, s.q, s.w, u);
 }
 
 // Gate crossing test on the segment p0 -> p1 (env_dronerace.h:412-470),

@@ -192,15 +192,16 @@ void vjp_step(const Model<S>& M, const St<S>& x, const S u[4], G13<S>& gx, S gu[
     const S two = (S)2, one = (S)1;
     const S b3[3] = {two * fm(qx, qz, qw * qy), two * fm(qy, qz, -(qw * qx)), fm(-two, fm(qx, qx, qy * qy), one)};
     const S at = u[0] * M.inv_mass;
-    const S dq[4] = {-fm(qx, wx, fm(qy, wy, qz * wz)), fm(qw, wx, fm(qy, wz, -(qz * wy))), fm(qw, wy, fm(qz, wx, -(qx * wz))),
-                     fm(qw, wz, fm(qx, wy, -(qy * wx)))};
-    S n[4];
-    for (int i = 0; i < 4; ++i)
-        n[i] = fm(M.half_dt, dq[i], x.q[i]);
-    const S inv = inv_sqrt(fm(n[0], n[0], fm(n[1], n[1], fm(n[2], n[2], n[3] * n[3]))));
-    S o[4];
-    for (int i = 0; i < 4; ++i)
-        o[i] = n[i] * inv;
+    const S h = M.half_dt;
+    const S hx = h * wx, hy = h * wy, hz = h * wz;
+    const S rs = inv_sqrt(fm(hx, hx, fm(hy, hy, fm(hz, hz, one))));
+    const S n0 = fm(-qx, hx, fm(-qy, hy, fm(-qz, hz, qw)));
+    const S n1 = fm(qw, hx, fm(qy, hz, fm(-qz, hy, qx)));
+    const S n2 = fm(qw, hy, fm(qz, hx, fm(-qx, hz, qy)));
+    const S n3 = fm(qw, hz, fm(qx, hy, fm(-qy, hx, qz)));
+    const S q2 = fm(qw, qw, fm(qx, qx, fm(qy, qy, qz * qz)));
+    const S c = fm((S)-0.5f, q2, (S)1.5f);
+    const S sc = rs * c;
     G13<S> g;
     const S kk = M.rate_k;
     for (int i = 0; i < 3; ++i)
@@ -208,19 +209,20 @@ void vjp_step(const Model<S>& M, const St<S>& x, const S u[4], G13<S>& gx, S gu[
         gu[1 + i] = kk * gx.w[i];
         g.w[i] = (one - kk) * gx.w[i];
     }
-    const S dot = fm(o[0], gx.q[0], fm(o[1], gx.q[1], fm(o[2], gx.q[2], o[3] * gx.q[3])));
-    S gn[4];
-    for (int i = 0; i < 4; ++i)
-        gn[i] = (gx.q[i] - o[i] * dot) * inv;
-    const S h = M.half_dt;
-    const S hw = h * gn[0], hx = h * gn[1], hy = h * gn[2], hz = h * gn[3];
-    g.q[0] = gn[0] + (wx * hx + wy * hy + wz * hz);
-    g.q[1] = gn[1] + (-wx * hw - wz * hy + wy * hz);
-    g.q[2] = gn[2] + (-wy * hw + wz * hx - wx * hz);
-    g.q[3] = gn[3] + (-wz * hw - wy * hx + wx * hy);
-    g.w[0] = g.w[0] + (-qx * hw + qw * hx + qz * hy - qy * hz);
-    g.w[1] = g.w[1] + (-qy * hw - qz * hx + qw * hy + qx * hz);
-    g.w[2] = g.w[2] + (-qz * hw + qy * hx - qx * hy + qw * hz);
+    const S gn0 = sc * gx.q[0], gn1 = sc * gx.q[1], gn2 = sc * gx.q[2], gn3 = sc * gx.q[3];
+    const S gsc = fm(n0, gx.q[0], fm(n1, gx.q[1], fm(n2, gx.q[2], n3 * gx.q[3])));
+    const S gc = rs * gsc;
+    const S gr2 = ((S)-0.5f * (rs * rs * rs)) * (c * gsc);
+    g.q[0] = (gn0 + hx * gn1 + hy * gn2 + hz * gn3) - qw * gc;
+    g.q[1] = (-hx * gn0 + gn1 - hz * gn2 + hy * gn3) - qx * gc;
+    g.q[2] = (-hy * gn0 + hz * gn1 + gn2 - hx * gn3) - qy * gc;
+    g.q[3] = (-hz * gn0 - hy * gn1 + hx * gn2 + gn3) - qz * gc;
+    const S ghx = (-qx * gn0 + qw * gn1 + qz * gn2 - qy * gn3) + two * hx * gr2;
+    const S ghy = (-qy * gn0 - qz * gn1 + qw * gn2 + qx * gn3) + two * hy * gr2;
+    const S ghz = (-qz * gn0 + qy * gn1 - qx * gn2 + qw * gn3) + two * hz * gr2;
+    g.w[0] = g.w[0] + h * ghx;
+    g.w[1] = g.w[1] + h * ghy;
+    g.w[2] = g.w[2] + h * ghz;
     S ga[3];
     for (int i = 0; i < 3; ++i)
     {

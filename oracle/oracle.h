@@ -291,14 +291,15 @@ inline void euler(const Model<S>& M, St<S>& s, const S u[4])
         s.p[i] = fm(M.dt, s.v[i], s.p[i]);
     for (int i = 0; i < 3; ++i)
         s.v[i] = fm(M.dt, acc[i], s.v[i]);
-    const S dq[4] = {-fm(qx, wx, fm(qy, wy, qz * wz)), fm(qw, wx, fm(qy, wz, -(qz * wy))), fm(qw, wy, fm(qz, wx, -(qx * wz))),
-                     fm(qw, wz, fm(qx, wy, -(qy * wx)))};
-    S n[4];
+    // q <- q (x) (1, h w) / |(1, h w)| * (3 - |q|^2)/2, h = dt/2 (the Euler step q + h q (x) (0, w) renormalised)
+    const S hx = M.half_dt * wx, hy = M.half_dt * wy, hz = M.half_dt * wz;
+    const S rs = inv_sqrt(fm(hx, hx, fm(hy, hy, fm(hz, hz, one))));
+    const S n[4] = {fm(-qx, hx, fm(-qy, hy, fm(-qz, hz, qw))), fm(qw, hx, fm(qy, hz, fm(-qz, hy, qx))),
+                    fm(qw, hy, fm(qz, hx, fm(-qx, hz, qy))), fm(qw, hz, fm(qx, hy, fm(-qy, hx, qz)))};
+    const S q2 = fm(qw, qw, fm(qx, qx, fm(qy, qy, qz * qz)));
+    const S sc = rs * fm((S)-0.5f, q2, (S)1.5f);
     for (int i = 0; i < 4; ++i)
-        n[i] = fm(M.half_dt, dq[i], s.q[i]);
-    const S inv = inv_sqrt(fm(n[0], n[0], fm(n[1], n[1], fm(n[2], n[2], n[3] * n[3]))));
-    for (int i = 0; i < 4; ++i)
-        s.q[i] = n[i] * inv;
+        s.q[i] = n[i] * sc;
     s.w[0] = fm(M.rate_k, u[1] - wx, wx);
     s.w[1] = fm(M.rate_k, u[2] - wy, wy);
     s.w[2] = fm(M.rate_k, u[3] - wz, wz);

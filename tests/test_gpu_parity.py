@@ -251,3 +251,26 @@ def test_single_controller_api(pkg, oracle):
     assert rel_err(h.get_nominal(), upd["shifted"]) <= REL
     assert h.diag()["step"] == 1
     h.close()
+
+
+def test_rejects_non_unit_attitude(pkg):
+    """Drone states enter with a unit attitude quaternion (the attitude recurrence renormalises to
+    first order); a quaternion off by more than 1e-2 in |q|^2, or a non-finite value, is rejected
+    with DR_EINVAL and leaves the engine's states unchanged."""
+    sc = pkg.scenarios
+    scn = small_scenario(sc, n_samples=64, n_timesteps=5, n_drones=2)
+    eng = make_engine(pkg, scn)
+    before = [list(d.p) + list(d.q) for d in eng.get_drones()]
+    bad = (pkg.abi.DroneState * 2)(*scn.drones)
+    bad[1].q[0] = 1.2
+    with pytest.raises(pkg.EngineError, match="unit quaternion"):
+        eng.set_drones(bad)
+    bad[1].q[0] = 1.0
+    bad[0].p[2] = float("nan")
+    with pytest.raises(pkg.EngineError, match="not finite"):
+        eng.set_drones(bad)
+    assert [list(d.p) + list(d.q) for d in eng.get_drones()] == before
+    ok = (pkg.abi.DroneState * 2)(*scn.drones)
+    ok[1].q[0], ok[1].q[3] = 0.70710677, 0.70710677
+    eng.set_drones(ok)
+    eng.close()

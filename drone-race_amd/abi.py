@@ -15,6 +15,7 @@ MAX_T = 256
 MAX_WORLD_DRONES = 8
 
 MAX_BRANCHES = 8
+COSTGRAD_NOALLOC_MAX_FLOATS = 1 << 21  # dr_api.h DR_COSTGRAD_NOALLOC_MAX_FLOATS
 MAX_MODELS = 16
 SR_TRACK_SAMPLES = 512
 SR_MAX_MODELS = 4
@@ -228,6 +229,7 @@ SIGNATURES = {
     "dr_engine_set_exec_cost": (_I, [_P, _I]),
     "dr_engine_cost_grad": (_I, [_P, _I, _F, _I, C.c_size_t, _F, _F, _I]),
     "dr_engine_cost_grad_device": (_I, [_P, _I, _P, _I, C.c_size_t, _P, _P, _I, _P]),
+    "dr_engine_cost_grad_noalloc": (_I, [_P, _I, _F, _I, C.c_size_t, _F, _F, _I]),
     "dr_engine_set_refine": (_I, [_P, _I, C.c_float, _I]),
     "dr_engine_last_timing": (_I, [_P, _F, _I]),
     "dr_mppi_create": (_I, [C.POINTER(QuadParams), C.POINTER(CostCfg), C.POINTER(Track), C.POINTER(MPPICfg), C.c_uint64,

@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -854,6 +855,75 @@ int dr_engine_cost_grad(dr_engine* e, int instance, const float* U, int n_seq, s
     HIPCHK(hipMemcpyAsync(J, e->d_gJ.p, n_seq * sizeof(float), hipMemcpyDeviceToHost, e->stream));
     if (dJdU)
         HIPCHK(hipMemcpyAsync(dJdU, e->d_gG.p, n * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return DR_OK;
+}
+
+// Staging buffer of dr_engine_cost_grad_noalloc: one per GPU, held for the
+// life of the process, never allocated or freed by a call.  Host code uses
+// only its per-device address from hipGetSymbolAddress (the array's name is a
+// host-side stand-in, not a device address).
+static __device__ __align__(256) float scratch[DR_COSTGRAD_NOALLOC_MAX_FLOATS];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// The address of this device's staging buffer, looked up once per device with
+// that device current.
+int scratch_on(int device, float** out)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return fail(DR_EINVAL, "cost_grad_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device])
+    {
+        int cur = 0;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != device)
+            HIPCHK(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device)
+            HIPCHK(hipSetDevice(cur));
+        if (err != hipSuccess || !p)
+            return fail(DR_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err));
+        g_scratch_ptr[device] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return DR_OK;
+}
+
+} // namespace
+
+int dr_engine_cost_grad_noalloc(dr_engine* e, int instance, const float* U, int n_seq, size_t n_floats, float* J, float* dJdU,
+                                int ckpt_every)
+{
+    if (int rc = check_inst(e, instance))
+        return rc;
+    if (!U || !J || n_seq < 1)
+        return fail(DR_EINVAL, "cost_grad_noalloc: bad pointers or sizes");
+    const size_t n = (size_t)n_seq * e->idev[instance].T * 4;
+    if (n_floats != n)
+        return fail(DR_EINVAL, "cost_grad_noalloc: n_floats must be n_seq * T * 4 of the instance (U and dJdU are [n_seq][T][4])");
+    if (2 * n + (size_t)n_seq > (size_t)DR_COSTGRAD_NOALLOC_MAX_FLOATS)
+        return fail(DR_EINVAL, "cost_grad_noalloc: n_seq * T * 8 + n_seq exceeds the staging buffer of " +
+                                   std::to_string(DR_COSTGRAD_NOALLOC_MAX_FLOATS) + " floats");
+    float* buf = nullptr;
+    if (int rc = scratch_on(e->device, &buf))
+        return rc;
+    float* dU = buf;
+    float* dG = buf + n;
+    float* dJ = buf + 2 * n;
+    HIPCHK(hipMemcpyAsync(dU, U, n * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    int rc = dr_engine_cost_grad_device(e, instance, dU, n_seq, n, dJ, dJdU ? dG : nullptr, ckpt_every, nullptr);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemcpyAsync(J, dJ, n_seq * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    if (dJdU)
+        HIPCHK(hipMemcpyAsync(dJdU, dG, n * sizeof(float), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return DR_OK;
 }

@@ -241,6 +241,18 @@ int dr_engine_cost_grad(dr_engine* e, int instance, const float* U, int n_seq, s
  * engine's stream).  Used by the PyTorch autograd wrapper. */
 int dr_engine_cost_grad_device(dr_engine* e, int instance, const float* U, int n_seq, size_t n_floats, float* J, float* dJdU,
                                int ckpt_every, void* stream);
+/* No-allocation variant of dr_engine_cost_grad, for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid allocation): the same adjoint kernel, with U, J and
+ * dJdU staged through a fixed buffer of DR_COSTGRAD_NOALLOC_MAX_FLOATS floats
+ * (8 MiB) that the library holds on each GPU for the life of the process;
+ * the call makes no device allocation.  n_seq * T * 8 + n_seq must not exceed
+ * that size (DR_EINVAL otherwise).  Every call on a GPU shares that GPU's
+ * buffer: calls that overlap on one GPU (other threads, other handles) race.
+ * No counterpart in the reference. */
+#define DR_COSTGRAD_NOALLOC_MAX_FLOATS (1 << 21)
+int dr_engine_cost_grad_noalloc(dr_engine* e, int instance, const float* U, int n_seq, size_t n_floats, float* J, float* dJdU,
+                                int ckpt_every);
 /* Adjoint refinement inside every control step: after the MPPI update,
  * n_grad_steps projected gradient steps U <- clamp(U - step * dJ/dU) on each
  * instance's nominal (checkpoints every ckpt_every steps), the J of every

@@ -67,3 +67,14 @@ def build_demo(out=None):
            "-L", HERE, "-ldr_amd", "-Wl,-rpath," + HERE]
     subprocess.run(cmd, check=True)
     return out
+
+
+def build_reference_demo(out=None):
+    """Build tests/cpp/reference_engine_demo (the reference-engine binding check) against libdr_amd.so."""
+    root = os.path.dirname(HERE)
+    out = out or os.path.join(root, "tests", "cpp", "reference_engine_demo")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"),
+           os.path.join(root, "tests", "cpp", "reference_engine_demo.cpp"), "-o", out,
+           "-L", HERE, "-ldr_amd", "-Wl,-rpath," + HERE]
+    subprocess.run(cmd, check=True)
+    return out

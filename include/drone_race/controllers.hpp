@@ -25,6 +25,7 @@
 
 #include <array>
 #include <cmath>
+#include <memory>
 #include <optional>
 #include <stdexcept>
 #include <string>
@@ -73,6 +74,24 @@ inline void check(int rc, const char* what)
         throw std::runtime_error(std::string(what) + ": " + dr_last_error());
 }
 
+// Owning handles: a constructor that throws after creating its engine releases it.
+struct MppiDeleter
+{
+    void operator()(dr_mppi* h) const { dr_mppi_destroy(h); }
+};
+struct PrDeleter
+{
+    void operator()(dr_prengine* h) const { dr_pr_destroy(h); }
+};
+struct SrDeleter
+{
+    void operator()(dr_sr_engine* h) const { dr_sr_destroy(h); }
+};
+struct BmDeleter
+{
+    void operator()(dr_bm_engine* h) const { dr_bm_destroy(h); }
+};
+
 class Controller
 {
   public:
@@ -100,32 +119,25 @@ class MPPIController : public Controller
         }
         cfg.adapt_lambda = mc.adaptTemperature ? 1 : 0;
         cfg.max_opponents = c.maxOpponents;
-        check(dr_mppi_create(&c.quad, &c.cost, &c.track, &cfg, (uint64_t)(uint32_t)seed, &h), "MPPIController");
-        if (nominal)
-        {
-            if ((int)nominal->size() != mc.nTimesteps * ACTION_DIM)
-            {
-                dr_mppi_destroy(h);
-                throw std::runtime_error("Invalid MPPI construction: expected nominal size " +
-                                         std::to_string(mc.nTimesteps * ACTION_DIM) + ", got " + std::to_string(nominal->size()));
-            }
-            h_nominal = *nominal;
-        }
-        else
-            h_nominal.assign((size_t)mc.nTimesteps * ACTION_DIM, 0.0f);
-        check(dr_mppi_set_nominal(h, h_nominal.data(), (int)h_nominal.size()), "MPPIController");
+        if (nominal && (int)nominal->size() != mc.nTimesteps * ACTION_DIM)
+            throw std::runtime_error("Invalid MPPI construction: expected nominal size " + std::to_string(mc.nTimesteps * ACTION_DIM) +
+                                     ", got " + std::to_string(nominal->size()));
+        dr_mppi* raw = nullptr;
+        check(dr_mppi_create(&c.quad, &c.cost, &c.track, &cfg, (uint64_t)(uint32_t)seed, &raw), "MPPIController");
+        h.reset(raw);
+        h_nominal = nominal ? *nominal : std::vector<float>((size_t)mc.nTimesteps * ACTION_DIM, 0.0f);
+        check(dr_mppi_set_nominal(h.get(), h_nominal.data(), (int)h_nominal.size()), "MPPIController");
     }
 
-    ~MPPIController() override { dr_mppi_destroy(h); }
     MPPIController(const MPPIController&) = delete;
     MPPIController& operator=(const MPPIController&) = delete;
 
     void getControl(const SimState& state, float* outAction) override
     {
-        check(dr_mppi_step(h, &state.ego, state.opponents.data(), state.nOpponents, outAction), "getControl");
-        check(dr_mppi_get_nominal(h, h_nominal.data(), (int)h_nominal.size()), "getControl");
+        check(dr_mppi_step(h.get(), &state.ego, state.opponents.data(), state.nOpponents, outAction), "getControl");
+        check(dr_mppi_get_nominal(h.get(), h_nominal.data(), (int)h_nominal.size()), "getControl");
         dr_diag d{};
-        check(dr_mppi_get_diag(h, &d), "getControl");
+        check(dr_mppi_get_diag(h.get(), &d), "getControl");
         mppiConfig.invTemperature = d.lambda;
         lastMinCost = d.min_cost;
         lastNu = d.nu;
@@ -138,7 +150,7 @@ class MPPIController : public Controller
 
   private:
     EnvironmentConfig envConfig;
-    dr_mppi* h = nullptr;
+    std::unique_ptr<dr_mppi, MppiDeleter> h;
 };
 
 // PR-MPPI over opponent-velocity branches (the reference's PRMPPIConfig
@@ -197,20 +209,18 @@ class PRMPPIController : public Controller
         }
         const int nd = 1 + c.maxOpponents;
         std::vector<int32_t> world(nd, 0);
-        check(dr_pr_create(&c.quad, &c.cost, &c.track, 1, &in, 1, world.data(), nd, &h), "PRMPPIController");
         const size_t n = (size_t)mc.nTimesteps * ACTION_DIM;
         if (nominal && nominal->size() != n)
-        {
-            dr_pr_destroy(h);
             throw std::runtime_error("Invalid PRMPPI construction: expected nominal size " + std::to_string(n) + ", got " +
                                      std::to_string(nominal->size()));
-        }
+        dr_prengine* raw = nullptr;
+        check(dr_pr_create(&c.quad, &c.cost, &c.track, 1, &in, 1, world.data(), nd, &raw), "PRMPPIController");
+        h.reset(raw);
         h_nom_nominal = nominal ? *nominal : std::vector<float>(n, 0.0f);
         h_rob_nominal = h_nom_nominal;
-        check(dr_pr_set_nominals(h, 0, h_nom_nominal.data(), h_rob_nominal.data(), (int)n), "PRMPPIController");
+        check(dr_pr_set_nominals(h.get(), 0, h_nom_nominal.data(), h_rob_nominal.data(), (int)n), "PRMPPIController");
     }
 
-    ~PRMPPIController() override { dr_pr_destroy(h); }
     PRMPPIController(const PRMPPIController&) = delete;
     PRMPPIController& operator=(const PRMPPIController&) = delete;
 
@@ -230,11 +240,11 @@ class PRMPPIController : public Controller
                 w[1 + j].v[d] = state.opponents[j].v[d];
             }
         }
-        check(dr_pr_set_drones(h, w.data(), (int)w.size()), "getControl");
-        check(dr_pr_step(h, 1, 0, outAction), "getControl");
-        check(dr_pr_get_nominals(h, 0, h_nom_nominal.data(), h_rob_nominal.data(), (int)h_nom_nominal.size()), "getControl");
+        check(dr_pr_set_drones(h.get(), w.data(), (int)w.size()), "getControl");
+        check(dr_pr_step(h.get(), 1, 0, outAction), "getControl");
+        check(dr_pr_get_nominals(h.get(), 0, h_nom_nominal.data(), h_rob_nominal.data(), (int)h_nom_nominal.size()), "getControl");
         dr_pr_diag d{};
-        check(dr_pr_get_diag(h, &d, 1), "getControl");
+        check(dr_pr_get_diag(h.get(), &d, 1), "getControl");
         useNomPlan = d.use_nom_plan != 0;
         resetNom = d.reset_nom != 0;
         for (int j = 0; j < 3; ++j)
@@ -250,7 +260,7 @@ class PRMPPIController : public Controller
 
   private:
     EnvironmentConfig envConfig;
-    dr_prengine* h = nullptr;
+    std::unique_ptr<dr_prengine, PrDeleter> h;
 };
 
 namespace stratrace {
@@ -289,29 +299,27 @@ class MPPIController
         in.seed = (uint64_t)(uint32_t)seed;
         in.lambda = mc.invTemperature;
         in.sigma = mc.samplingNoise;
-        check(dr_sr_create(&c, &mc.cost, &in, 1, &h), "MPPIController");
         const size_t n = (size_t)mc.nTimesteps * ACTION_DIM;
         if (nominal && nominal->size() != n)
-        {
-            dr_sr_destroy(h);
             throw std::runtime_error("Invalid MPPI construction: expected nominal size " + std::to_string(n) + ", got " +
                                      std::to_string(nominal->size()));
-        }
+        dr_sr_engine* raw = nullptr;
+        check(dr_sr_create(&c, &mc.cost, &in, 1, &raw), "MPPIController");
+        h.reset(raw);
         h_nominal = nominal ? *nominal : std::vector<float>(n, 0.0f);
-        check(dr_sr_set_nominal(h, 0, h_nominal.data(), (int)n), "MPPIController");
+        check(dr_sr_set_nominal(h.get(), 0, h_nominal.data(), (int)n), "MPPIController");
     }
 
-    ~MPPIController() { dr_sr_destroy(h); }
     MPPIController(const MPPIController&) = delete;
     MPPIController& operator=(const MPPIController&) = delete;
 
     void getControl(const SimState& state, float* outAction)
     {
-        check(dr_sr_set_states(h, &state, 1), "getControl");
-        check(dr_sr_step(h, 1, 0, outAction), "getControl");
-        check(dr_sr_get_nominal(h, 0, h_nominal.data(), (int)h_nominal.size()), "getControl");
+        check(dr_sr_set_states(h.get(), &state, 1), "getControl");
+        check(dr_sr_step(h.get(), 1, 0, outAction), "getControl");
+        check(dr_sr_get_nominal(h.get(), 0, h_nominal.data(), (int)h_nominal.size()), "getControl");
         dr_diag d{};
-        check(dr_sr_get_diag(h, &d, 1), "getControl");
+        check(dr_sr_get_diag(h.get(), &d, 1), "getControl");
         mppiConfig.invTemperature = d.lambda;
         lastMinCost = d.min_cost;
         lastNu = d.nu;
@@ -323,7 +331,7 @@ class MPPIController
     float lastNu = 0.0f;
 
   private:
-    dr_sr_engine* h = nullptr;
+    std::unique_ptr<dr_sr_engine, SrDeleter> h;
 };
 
 // MPPIConfig of the reference's default controller: branching over the
@@ -420,33 +428,31 @@ class BranchingMPPI
         in.verif_horizon = mc.verifHorizon;
         in.beta = mc.beta;
         in.max_verif_eps = mc.maxVerifEps;
-        check(E::create(&c, &mc.cost, &in, &h), "MPPIController");
-        failCount.assign((size_t)mc.nTimesteps, 0u);
         const size_t n = (size_t)(nModels + 1) * mc.nTimesteps * 2;
         if (nominal && nominal->size() != n)
-        {
-            dr_bm_destroy(h);
             throw std::runtime_error("Invalid MPPI construction: expected nominal size " + std::to_string(n) + ", got " +
                                      std::to_string(nominal->size()));
-        }
+        dr_bm_engine* raw = nullptr;
+        check(E::create(&c, &mc.cost, &in, &raw), "MPPIController");
+        h.reset(raw);
+        failCount.assign((size_t)mc.nTimesteps, 0u);
         h_nominal = nominal ? *nominal : std::vector<float>(n, 0.0f);
         h_belief.fill(0.0f);
         for (int th = 0; th < nModels; ++th)
             h_belief[th] = in.init_belief[th];
     }
 
-    ~BranchingMPPI() { dr_bm_destroy(h); }
     BranchingMPPI(const BranchingMPPI&) = delete;
     BranchingMPPI& operator=(const BranchingMPPI&) = delete;
 
     void getControl(const typename E::State& state, float* outAction)
     {
-        check(dr_bm_set_belief(h, 0, h_belief.data(), nModels), "getControl");
-        check(E::set_state(h, &state), "getControl");
-        check(dr_bm_step(h, 1, 0, outAction), "getControl");
-        check(dr_bm_get_nominal(h, 0, h_nominal.data(), (int)h_nominal.size()), "getControl");
+        check(dr_bm_set_belief(h.get(), 0, h_belief.data(), nModels), "getControl");
+        check(E::set_state(h.get(), &state), "getControl");
+        check(dr_bm_step(h.get(), 1, 0, outAction), "getControl");
+        check(dr_bm_get_nominal(h.get(), 0, h_nominal.data(), (int)h_nominal.size()), "getControl");
         dr_bm_diag d{};
-        check(dr_bm_get_diag(h, &d, 1), "getControl");
+        check(dr_bm_get_diag(h.get(), &d, 1), "getControl");
         mppiConfig.invTemperature = d.lambda;
         chosenBranch = d.branch;
         lastNu = d.nu;
@@ -458,7 +464,7 @@ class BranchingMPPI
         if (mppiConfig.nVerifSamples > 0)
         {
             std::vector<int32_t> fn(failCount.size()), fo(failCount.size());
-            check(dr_bm_get_fail_counts(h, 0, fn.data(), fo.data(), (int)failCount.size()), "getControl");
+            check(dr_bm_get_fail_counts(h.get(), 0, fn.data(), fo.data(), (int)failCount.size()), "getControl");
             for (size_t t = 0; t < failCount.size(); ++t)
                 failCount[t] = (unsigned)(useNewPlan ? fn[t] : fo[t]);
         }
@@ -475,7 +481,7 @@ class BranchingMPPI
 
   private:
     int nModels;
-    dr_bm_engine* h = nullptr;
+    std::unique_ptr<dr_bm_engine, BmDeleter> h;
 };
 
 } // namespace detail

@@ -143,3 +143,143 @@ def test_shim_matches_python_handle(pkg):
     heng.close()
     assert np.array_equal(ho_acts, np.stack(py_ho))
     assert [int(l[5]) for l in ho_lines] == py_hb
+
+
+# --------------------------------------------------------------------------------------------
+# The reference-engine binding (include/drone_race/reference_controllers.hpp): controllers that
+# the reference's SimulationEngine holds as std::unique_ptr<Controller> and downcasts.
+
+REF_DEMO = os.path.join(ROOT, "tests", "cpp", "reference_engine_demo")
+
+
+def test_reference_binding_compiles_and_links(tmp_path):
+    out = _build_module().build_reference_demo(str(tmp_path / "refdemo"))
+    assert os.path.exists(out)
+
+
+def _ref_input(tmp_path, kind, env, seed, cfg, knots, belief0, records):
+    envf = tmp_path / "env.bin"
+    envf.write_bytes(bytes(env))
+    lines = [f"env {envf}", f"kind {kind}", f"seed {seed}", "belief0 " + " ".join(f"{b:.9g}" for b in belief0)]
+    lines += [f"{k} {v!r}" for k, v in cfg.items()]
+    if knots:
+        lines.append("knots " + " ".join(str(k) for k in knots))
+    for belief, st in records:
+        vals = list(belief) + list(st.pos) + list(st.vel) + list(st.s) + list(st.lat_dist)
+        lines.append("step " + " ".join(repr(float(v)) for v in vals) + " " + " ".join(str(int(x)) for x in st.laps))
+    p = tmp_path / f"{kind}.txt"
+    p.write_text("\n".join(lines) + "\n")
+    return p
+
+
+def _run_ref_demo(path):
+    exe = REF_DEMO if os.path.exists(REF_DEMO) else _build_module().build_reference_demo()
+    out = subprocess.run([exe, str(path)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    return out.stdout.split("\n")
+
+
+def _field(line, name):
+    toks = line.split()
+    return toks[toks.index(name) + 1]
+
+
+def _pad4(b):
+    out = [0.0] * 4
+    out[:len(b)] = [float(x) for x in b]
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n_verif", [0, 8192])
+def test_reference_binding_mppi_matches_python_handle(pkg, tmp_path, n_verif):
+    """MPPIController through std::unique_ptr<Controller> in an engine that mirrors the reference's
+    construction, downcasts and belief hand-off: over a closed loop recorded from the Python
+    branching-MPPI handle (without and with plan verification), the actions are bit-for-bit the
+    handle's, and failCount, epsilon, epsilonPartial, useNewPlan, certifiedLoss, the temperature and
+    the dense plans are the handle's diagnostics.  The controller's int seed s is the Philox key
+    (s, 0), as the handle is built here."""
+    sc = pkg.scenarios
+    n_steps = 4
+    scn = sc.bm_race(n_inst=1, n_samples=4096, n_timesteps=30, n_knots=6, n_verif=n_verif)
+    ins = scn.instances[0]
+    ins.seed = 1234567
+    eng = pkg.BMEngine(scn)
+    records, acts, diags, noms = [], [], [], []
+    for _ in range(n_steps):
+        records.append((_pad4(eng.belief(0)), eng.get_states()[0]))
+        acts.append(eng.step(1, advance=True)[0].copy())
+        diags.append(eng.diag()[0])
+        noms.append(eng.nominal(0).reshape(-1)[:2].copy())
+    eng.close()
+    cfg = dict(nSamples=ins.n_samples, nTimesteps=ins.n_timesteps, nKnots=ins.n_knots, invTemperature=ins.lambda_,
+               samplingNoise=ins.sigma, collDistFactor=scn.cost.coll_dist_factor, boundaryCost=scn.cost.boundary_cost,
+               boundaryThresholdFactor=scn.cost.boundary_threshold_factor, outsideCost=scn.cost.outside_cost,
+               winCost=scn.cost.win_cost, oppWinCost=scn.cost.opp_win_cost, finalAdvWeight=scn.cost.final_adv_weight,
+               finalOppAdvWeight=scn.cost.final_opp_adv_weight, minConfidence=ins.min_confidence,
+               nVerifSamples=ins.n_verif_samples, beta=ins.beta, verifHorizon=ins.verif_horizon, maxVerifEps=ins.max_verif_eps)
+    path = _ref_input(tmp_path, "mppi", scn.env, ins.seed, cfg, list(ins.knots[:ins.n_knots]), records[0][0], records)
+    out = _run_ref_demo(path)
+    act_lines = [l for l in out if " action " in l]
+    info = [l for l in out if " mppi " in l]
+    assert len(act_lines) == n_steps and len(info) == n_steps
+    got = np.array([[float(x) for x in l.split()[3:5]] for l in act_lines], np.float32)
+    assert np.array_equal(got, np.array(acts, np.float32)), (got, acts)
+    if n_verif == 0:
+        assert np.abs(got).max() > 0.0
+    for k, (l, d) in enumerate(zip(info, diags)):
+        assert int(_field(l, "use_new_plan")) == int(d.use_new_plan)
+        assert float(_field(l, "epsilon")) == d.epsilon and float(_field(l, "certified_loss")) == d.certified_loss
+        assert float(_field(l, "epsilon_partial")) == d.epsilon_partial
+        assert int(_field(l, "fail_count")) == int(d.fail_count)
+        assert np.float32(_field(l, "lambda")) == np.float32(d.lambda_)
+        assert int(_field(l, "back")) == 1
+        toks = l.split()
+        i = toks.index("nominal0")
+        assert np.array_equal(np.float32(toks[i + 1:i + 3]), noms[k])
+    assert any(l.startswith("throws:") for l in out)
+
+
+@pytest.mark.gpu
+def test_reference_binding_prmppi_matches_python_handle(pkg, tmp_path):
+    """PRMPPIController through the same engine: P = 9 models drawn from h_belief each step, the
+    actions bit-for-bit the Python PR-MPPI handle's over its recorded closed loop, and useNomPlan,
+    resetNom and both plans its diagnostics."""
+    sc = pkg.scenarios
+    n_steps = 4
+    scn = sc.pr_sr_race(n_inst=1, n_samples=2048, n_timesteps=20)
+    ins = scn.instances[0]
+    ins.seed = 7654321
+    ins.adapt_lambda = 1  # the reference adapts its three temperatures every step
+    eng = pkg.PRSREngine(scn)
+    n_models = scn.env.n_models
+    records, acts, diags, plans = [], [], [], []
+    for _ in range(n_steps):
+        records.append((_pad4(eng.belief(0, n_models)), eng.get_states()[0]))
+        acts.append(eng.step(1, advance=True)[0].copy())
+        diags.append(eng.diag()[0])
+        nom, rob = eng.get_nominals(0)
+        plans.append((nom.reshape(-1)[:2].copy(), rob.reshape(-1)[:2].copy()))
+    eng.close()
+    cfg = dict(nSamples=ins.n_samples, nTimesteps=ins.n_timesteps, invTemperature=ins.lambda_, samplingNoise=ins.sigma,
+               collDistFactor=scn.cost.coll_dist_factor, boundaryCost=scn.cost.boundary_cost,
+               boundaryThresholdFactor=scn.cost.boundary_threshold_factor, winCost=scn.cost.win_cost,
+               oppWinCost=scn.cost.opp_win_cost, finalAdvWeight=scn.cost.final_adv_weight,
+               finalOppAdvWeight=scn.cost.final_opp_adv_weight, safetyWeight=ins.safety_weight, P=float(ins.n_draws))
+    path = _ref_input(tmp_path, "prmppi", scn.env, ins.seed, cfg, None, records[0][0], records)
+    out = _run_ref_demo(path)
+    act_lines = [l for l in out if " action " in l]
+    info = [l for l in out if " prmppi " in l]
+    assert len(act_lines) == n_steps and len(info) == n_steps
+    got = np.array([[float(x) for x in l.split()[3:5]] for l in act_lines], np.float32)
+    assert np.array_equal(got, np.array(acts, np.float32)), (got, acts)
+    assert np.abs(got).max() > 0.0
+    for k, (l, d) in enumerate(zip(info, diags)):
+        assert int(_field(l, "use_nom_plan")) == int(d.use_nom_plan)
+        assert int(_field(l, "reset_nom")) == int(d.reset_nom)
+        toks = l.split()
+        i, j = toks.index("nom0"), toks.index("rob0")
+        assert np.array_equal(np.float32(toks[i + 1:i + 3]), plans[k][0])
+        assert np.array_equal(np.float32(toks[j + 1:j + 3]), plans[k][1])
+        assert int(_field(l, "back")) == 1
+    assert any(l.startswith("throws:") for l in out)

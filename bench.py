@@ -233,7 +233,7 @@ def start_progress(scn):
     return out
 
 
-def c4_speed(pkg, steps, warmup, autotune=True):
+def c4_speed(pkg, steps, warmup, autotune=True, launch=None, legs=None):
     """Throughput of each leg on 8 drones (12-gate lemniscate, seed 1), closed loop, graph replays:
     autotuned launch shape, `warmup` untimed steps, then exactly `steps` timed batch steps between two
     synchronisations; then the same number of steps again with HIP events between the kernels for
@@ -242,12 +242,16 @@ def c4_speed(pkg, steps, warmup, autotune=True):
     sc = pkg.scenarios
     out = {}
     for name, K, refine in C4_LEGS:
+        if legs and name not in legs:
+            continue
         scn = sc.c4_eight_drones(K)
         n = len(scn.instances)
         eng = pkg.Engine(scn)
         eng.set_refine(refine, C4_GRAD_STEP, C4_CKPT)
         tuned = None
-        if autotune:
+        if launch:
+            eng.set_launch(**launch)
+        elif autotune:
             tuned, _ = eng.autotune(trials=5)
         launch = {f: getattr(eng.get_launch(), f) for f, _ in pkg.abi.LaunchCfg._fields_}
         eng.step(warmup, advance=True)
@@ -690,6 +694,8 @@ def main():
     ap.add_argument("--no-sr", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="run only plain MPPI on 4 drones, K=16384 (for profiling)")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the throughput of the three 8-drone legs (for profiling)")
+    ap.add_argument("--c4-legs", default="", help="comma-separated subset of the 8-drone legs to run with --adjoint-only")
+    ap.add_argument("--c4-launch", default="", help="launch shape of the 8-drone legs as JSON (skips their autotuner)")
     ap.add_argument("--pr-only", action="store_true", help="run only the PR-MPPI section (for profiling)")
     ap.add_argument("--sr-only", action="store_true", help="run only the StratRace2D section (for profiling)")
     ap.add_argument("--bm-only", action="store_true", help="run only the branching spline MPPI section (for profiling)")
@@ -744,7 +750,9 @@ def main():
         print(json.dumps({"c1": plain_run(pkg, sc.c1_four_drones(), args.steps, args.warmup, launch, not args.no_autotune)}), flush=True)
         return
     if args.adjoint_only:
-        print(json.dumps({"c4": c4_speed(pkg, args.steps, args.warmup, not args.no_autotune)}), flush=True)
+        legs = [x for x in args.c4_legs.split(",") if x] or None
+        launch = json.loads(args.c4_launch) if args.c4_launch else None
+        print(json.dumps({"c4": c4_speed(pkg, args.steps, args.warmup, not args.no_autotune, launch, legs)}), flush=True)
         return
 
     log("bench: 8 drones, K=16384 plain / K=4096 plain / K=4096 + 3 adjoint gradient steps: throughput ...")

@@ -253,7 +253,7 @@ def c4_speed(pkg, steps, warmup, autotune=True, launch=None, legs=None):
             eng.set_launch(**launch)
         elif autotune:
             tuned, _ = eng.autotune(trials=5)
-        launch = {f: getattr(eng.get_launch(), f) for f, _ in pkg.abi.LaunchCfg._fields_}
+        used = {f: getattr(eng.get_launch(), f) for f, _ in pkg.abi.LaunchCfg._fields_}
         eng.step(warmup, advance=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -268,7 +268,7 @@ def c4_speed(pkg, steps, warmup, autotune=True, launch=None, legs=None):
         kt = {k: v / n_timed for k, v in eng.last_timing().items()}
         eng.close()
         out[name] = {"samples": K, "grad_steps": refine, "control_steps_per_s": n * steps / dt, "ms_per_batch_step": 1e3 * dt / steps,
-                     "kernel_ms_per_step": kt, "launch": launch, "autotuned": tuned is not None,
+                     "kernel_ms_per_step": kt, "launch": used, "autotuned": tuned is not None,
                      "sample_steps_per_s": scn.sample_steps_per_step * steps / dt}
         log(f"bench: {name}: {out[name]['control_steps_per_s']:.0f} control steps/s ({out[name]['ms_per_batch_step']:.4f} ms per "
             f"batch step of {n}); device ms per step {json.dumps({k: round(v, 4) for k, v in kt.items()})}")

@@ -642,14 +642,12 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
 // the forward recurrence of the dynamics stays serial:
 //  P  the iterate's controls are clamped, one lane per step.
 //  A  the dynamics (attitude, rates, velocity, position) over all T steps,
-//     writing the state at every ck-th step (the checkpoints) into the state
-//     table, as a three-wave pipeline over chunks of ck steps: body rates and
-//     attitude increments (lane 128), attitude one chunk behind (lane 0),
-//     velocity and position two chunks behind (lane 64); neither the gate
-//     counter nor any cost is evaluated here, so each wave's serial chain is
-//     one part of the dynamics.
-//  B  one lane per checkpoint segment recomputes the segment's states from
-//     its checkpoint into the table.
+//     every state into the state table, as a three-wave pipeline handing on
+//     blocks of ck steps through LDS progress counters: body rates and
+//     attitude increments (lane 128), attitude (lane 0), velocity and
+//     position (lane 64); neither the gate counter nor any cost is evaluated
+//     here, so each wave's serial chain is one part of the dynamics, and no
+//     workgroup barrier falls inside the pass.
 //  G  every (step, gate) pair's crossing test in parallel; wave 0 then runs
 //     the gate counter by searching the crossing bits for the next gate's
 //     crossing (one ballot per 64 steps) and every state slot takes the gate
@@ -737,24 +735,34 @@ __device__ __forceinline__ void get_dyn(const float4* xs, int t, QState& s)
     s.w[2] = xs[t * 4 + 3].x;
 }
 
-// Dynamics of steps [t0, t1) from s over the clamped controls uc: the
-// attitude, rates, velocity and position recurrences only (the gate counter
-// does not feed back into them and is run by the gate pass).  With keep_all
-// every state x_{t+1}, t + 1 < t1 or t1 == T, is written to the table; else
-// none is.  The next control is loaded one step ahead.
-__device__ __forceinline__ void ref_dynamics(const QuadConst& C, const float4* uc, float4* xs, int t0, int t1, int T, bool keep_all,
-                                             QState& s)
+// Lane i of each quad of lanes takes x from lane (CTRL >> 2i) & 3 of its quad
+// (a DPP quad permute).
+template <int CTRL>
+__device__ __forceinline__ float quad_perm(float x)
 {
-    float4 u = uc[t0];
-    for (int t = t0; t < t1; ++t)
-    {
-        const float4 un = uc[t + 1 < t1 ? t + 1 : t];
-        const float uu[4] = {u.x, u.y, u.z, u.w};
-        quad_step(C, s, uu);
-        if (keep_all && (t + 1 < t1 || t1 == T))
-            put_dyn(xs, t + 1, s);
-        u = un;
-    }
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, true));
+}
+
+// Bound on the polls of one wait of the forward pipeline.  Every producer
+// publishes unconditionally, so a wait ends long before it; reaching it
+// anyway marks the refinement failed (the plain update is kept) instead of
+// spinning on.
+constexpr int REF_SPIN_LIMIT = 1 << 22;
+
+// Waits until the progress counter *p reaches need (workgroup-scope acquire:
+// the LDS writes published with it are visible afterwards).  Returns false
+// once the poll bound is hit; a failed wait is not retried.
+__device__ __forceinline__ bool wait_progress(int* p, int need, bool ok)
+{
+    int polls = 0;
+    while (ok && __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need)
+        ok = ++polls < REF_SPIN_LIMIT;
+    return ok;
+}
+
+__device__ __forceinline__ void publish_progress(int* p, int done)
+{
+    __hip_atomic_store(p, done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict__ insts, const dr_track* __restrict__ tracks,
@@ -771,13 +779,15 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     __shared__ float4 s_op[MAX_OPP], s_ov[MAX_OPP];
     __shared__ float4 s_uc[DR_MAX_T + 1];  // the clamped controls the forward pass integrates (+1: read-ahead slot)
     __shared__ uint32_t s_cross[DR_MAX_T]; // bit g: the move x_t -> x_{t+1} crosses gate g
-    __shared__ float4 s_qa[DR_MAX_T + 1];  // attitude at the start of each step (forward pass; +1: read-ahead slot)
-    __shared__ float4 s_inc[DR_MAX_T + 1]; // attitude increment (h w, 1/|(1, h w)|) of each step (forward pass; +1: read-ahead)
+    __shared__ float4 s_coef[DR_MAX_T * 4]; // per step, per attitude lane: its three product coefficients and 1/|(1, h w)|
+    __shared__ float2 s_thr[DR_MAX_T * 4];  // per step, per axis lane: thrust acceleration and thrust direction component
     __shared__ int s_xstep[DR_MAX_T], s_xgate[DR_MAX_T], s_xlaps[DR_MAX_T]; // gate crossings: step, gate, laps after
     __shared__ int s_nx;
     __shared__ float4 s_ubest[DR_MAX_T]; // the lowest-cost sequence evaluated so far
     __shared__ float s_jbest;
     __shared__ int s_tend, s_better;
+    __shared__ int s_prog[2]; // forward pipeline: steps whose increments (0) / start attitudes (1) are in LDS
+    __shared__ int s_fail;    // a pipeline wait hit its bound: keep the plain update
 
     const InstDev& I = insts[blockIdx.x];
     const int T = I.T, n_opp = I.n_opp, tid = threadIdx.x;
@@ -809,11 +819,11 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             w = w * DECAY;
         }
     }
+    if (tid == 0)
+        s_fail = 0;
     const QState x0 = state_from_abi(drones[I.drone]);
-    const int n_ck = (T + ck - 1) / ck;
-    // forward-pass registers: attitude (lane 0), position and velocity (lane 64), body rates (lane 128)
-    float qa[4] = {x0.q[0], x0.q[1], x0.q[2], x0.q[3]}, wr[3] = {x0.w[0], x0.w[1], x0.w[2]};
-    float pt[3] = {x0.p[0], x0.p[1], x0.p[2]}, vt[3] = {x0.v[0], x0.v[1], x0.v[2]};
+    float* const s_xf = reinterpret_cast<float*>(s_x); // the state table as floats: slot t at 16 t (put_dyn's layout)
+    const float* const s_ucf = reinterpret_cast<const float*>(s_uc);
     __syncthreads();
 
     for (int it = 0;; ++it)
@@ -828,102 +838,164 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             s_uc[t] = make_float4(out[0], out[1], out[2], out[3]);
             s_cross[t] = 0u;
         }
-        __syncthreads();
-        // A: forward pass of the dynamics, writing the checkpoints, as a
-        // three-stage pipeline over chunks of ck steps (one barrier per chunk):
-        // lane 128 integrates the body rates (they read only the controls) and
-        // forms each step's attitude increment (h w, 1/|(1, h w)|) into s_inc;
-        // lane 0, one chunk behind, applies the increments to the attitude and
-        // passes the attitude of every step through s_qa; lane 64, two chunks
-        // behind, integrates velocity and position from that attitude.  Each
-        // writes its part of the checkpoint at the start of its chunk.  The
-        // operations are those of quad_step.
-        for (int c = 0; c <= n_ck + 1; ++c)
-        {
-            if (tid == 2 * WAVE && c < n_ck)
-            {
-                const int t0 = c * ck, t1 = min(t0 + ck, T);
-                s_x[t0 * 4 + 2].z = wr[0];
-                s_x[t0 * 4 + 2].w = wr[1];
-                s_x[t0 * 4 + 3].x = wr[2];
-                float4 u = s_uc[t0];
-#pragma unroll 2
-                for (int t = t0; t < t1; ++t)
-                {
-                    const float4 un = s_uc[t + 1]; // read ahead (slot T is never used)
-                    s_inc[t] = attitude_increment(C, wr);
-                    const float uu[4] = {u.x, u.y, u.z, u.w};
-                    rates_step(C, wr, uu);
-                    u = un;
-                }
-            }
-            if (tid == 0 && c >= 1 && c <= n_ck)
-            {
-                const int t0 = (c - 1) * ck, t1 = min(t0 + ck, T);
-                s_x[t0 * 4 + 0].w = qa[0];
-                s_x[t0 * 4 + 1].x = qa[1];
-                s_x[t0 * 4 + 1].y = qa[2];
-                s_x[t0 * 4 + 1].z = qa[3];
-                float4 inc = s_inc[t0];
-#pragma unroll 2
-                for (int t = t0; t < t1; ++t)
-                {
-                    const float4 nxt = s_inc[t + 1]; // read ahead (written in the next chunk or never used)
-                    s_qa[t] = make_float4(qa[0], qa[1], qa[2], qa[3]);
-                    attitude_apply(qa, inc);
-                    inc = nxt;
-                }
-            }
-            if (tid == WAVE && c >= 2)
-            {
-                const int t0 = (c - 2) * ck, t1 = min(t0 + ck, T);
-                s_x[t0 * 4 + 0].x = pt[0];
-                s_x[t0 * 4 + 0].y = pt[1];
-                s_x[t0 * 4 + 0].z = pt[2];
-                s_x[t0 * 4 + 1].w = vt[0];
-                s_x[t0 * 4 + 2].x = vt[1];
-                s_x[t0 * 4 + 2].y = vt[2];
-                float4 u = s_uc[t0];
-                float4 q = s_qa[t0];
-#pragma unroll 2
-                for (int t = t0; t < t1; ++t)
-                {
-                    const float4 un = s_uc[t + 1]; // read ahead (unused past the chunk)
-                    const float4 qn = s_qa[t + 1];
-                    const float uu[4] = {u.x, u.y, u.z, u.w};
-                    const float qq[4] = {q.x, q.y, q.z, q.w};
-                    quad_translation_step(C, qq, pt, vt, uu);
-                    u = un;
-                    q = qn;
-                }
-            }
-            __syncthreads();
-        }
         if (tid == 0)
         {
-            s_tend = T - 1;
-            // the next pass starts from x0 again
-            for (int i = 0; i < 4; ++i)
-                qa[i] = x0.q[i];
-        }
-        if (tid == 2 * WAVE)
-            for (int i = 0; i < 3; ++i)
-                wr[i] = x0.w[i];
-        if (tid == WAVE)
-            for (int i = 0; i < 3; ++i)
-            {
-                pt[i] = x0.p[i];
-                vt[i] = x0.v[i];
-            }
-        // B: states between checkpoints, one lane per segment
-        if (tid < n_ck)
-        {
-            const int t0 = tid * ck;
-            QState s;
-            get_dyn(s_x, t0, s);
-            ref_dynamics(C, s_uc, s_x, t0, min(t0 + ck, T), T, true, s);
+            s_prog[0] = 0;
+            s_prog[1] = 0;
         }
         __syncthreads();
+        // A: forward pass of the dynamics, every state into the state table,
+        // by three waves handing on blocks of ck steps through LDS progress
+        // counters (no workgroup barrier until the pass ends).  Each serial
+        // recurrence runs one lane per vector component and every per-step
+        // quantity that only reads an earlier stage's output is formed one
+        // lane per step; the operations (and so the results) are those of
+        // quad_step:
+        //  wave 2: body rates w (lanes 0..2), then each step's attitude
+        //          increment (h w, 1/|(1, h w)|) as the coefficients of the
+        //          quaternion product, one lane per step;
+        //  wave 0: attitude q (lanes 0..3, component i in lane i; the product
+        //          takes the other components with quad permutes);
+        //  wave 1: thrust acceleration and thrust direction of each step, one
+        //          lane per step, then velocity and position (lanes 0..2).
+        if (tid >= 2 * WAVE && tid < 3 * WAVE)
+        {
+            const int l = tid - 2 * WAVE;
+            float w = l < 3 ? x0.w[l] : 0.0f;
+            for (int t0 = 0; t0 < T; t0 += ck)
+            {
+                const int t1 = min(t0 + ck, T);
+                if (l < 3)
+                {
+                    const auto rts = [&](int t, float u) {
+                        s_xf[t * 16 + 10 + l] = w;
+                        w = fma_(C.rate_k, u - w, w); // rates_step
+                    };
+                    float ua = s_ucf[t0 * 4 + 1 + l];
+                    int t = t0;
+                    for (; t + 1 < t1; t += 2)
+                    {
+                        const float ub = s_ucf[(t + 1) * 4 + 1 + l];
+                        __builtin_amdgcn_sched_barrier(0); // keep the load a full step ahead of its use
+                        rts(t, ua);
+                        ua = s_ucf[(t + 2) * 4 + 1 + l]; // slot T is the read-ahead slot
+                        __builtin_amdgcn_sched_barrier(0);
+                        rts(t + 1, ub);
+                    }
+                    if (t < t1)
+                        rts(t, ua);
+                }
+                for (int t = t0 + l; t < t1; t += WAVE)
+                {
+                    const float wt[3] = {s_xf[t * 16 + 10], s_xf[t * 16 + 11], s_xf[t * 16 + 12]};
+                    const float4 inc = attitude_increment(C, wt);
+                    // lane i of the product: n_i = fma(o, q_o, fma(m, q_m, fma(c, q_c, q_i)))
+                    s_coef[t * 4 + 0] = make_float4(-inc.z, -inc.y, -inc.x, inc.w);
+                    s_coef[t * 4 + 1] = make_float4(-inc.y, inc.z, inc.x, inc.w);
+                    s_coef[t * 4 + 2] = make_float4(-inc.z, inc.x, inc.y, inc.w);
+                    s_coef[t * 4 + 3] = make_float4(-inc.x, inc.y, inc.z, inc.w);
+                }
+                if (l == 0)
+                    publish_progress(&s_prog[0], t1);
+            }
+            if (l < 3)
+                s_xf[T * 16 + 10 + l] = w;
+        }
+        if (tid < 4)
+        {
+            // attitude_apply with component i in lane i: the three product
+            // terms read q from the lanes the quad permutes name (inner
+            // (3,3,1,2), middle (2,2,3,1), outer (1,0,0,0)); |q|^2 is summed
+            // from broadcasts in attitude_apply's order
+            float q = x0.q[tid];
+            bool ok = true;
+            for (int t0 = 0; t0 < T; t0 += ck)
+            {
+                const int t1 = min(t0 + ck, T);
+                ok = wait_progress(&s_prog[0], t1, ok);
+                const auto att = [&](int t, const float4 cf) {
+                    s_xf[t * 16 + 3 + tid] = q;
+                    const float n = fma_(quad_perm<0x01>(q), cf.z, fma_(quad_perm<0x7a>(q), cf.y, fma_(quad_perm<0x9f>(q), cf.x, q)));
+                    const float q0 = quad_perm<0x00>(q), q1 = quad_perm<0x55>(q), q2 = quad_perm<0xaa>(q), q3 = quad_perm<0xff>(q);
+                    const float n2 = fma_(q0, q0, fma_(q1, q1, fma_(q2, q2, q3 * q3)));
+                    q = n * (cf.w * fma_(-0.5f, n2, 1.5f));
+                };
+                // two steps per trip, each step's coefficients loaded one step ahead
+                float4 ca = s_coef[t0 * 4 + tid];
+                int t = t0;
+                for (; t + 1 < t1; t += 2)
+                {
+                    const float4 cb = s_coef[(t + 1) * 4 + tid];
+                    __builtin_amdgcn_sched_barrier(0); // keep the load a full step ahead of its use
+                    att(t, ca);
+                    ca = s_coef[min(t + 2, t1 - 1) * 4 + tid];
+                    __builtin_amdgcn_sched_barrier(0);
+                    att(t + 1, cb);
+                }
+                if (t < t1)
+                    att(t, ca);
+                if (tid == 0)
+                    publish_progress(&s_prog[1], t1);
+            }
+            s_xf[T * 16 + 3 + tid] = q;
+            if (!ok)
+                s_fail = 1;
+        }
+        if (tid >= WAVE && tid < 2 * WAVE)
+        {
+            const int l = tid - WAVE;
+            float p = l < 3 ? x0.p[l] : 0.0f, v = l < 3 ? x0.v[l] : 0.0f;
+            const float g = l == 2 ? C.gravity : 0.0f; // x - 0 is x: only the z lane's subtraction changes anything
+            bool ok = true;
+            for (int t0 = 0; t0 < T; t0 += ck)
+            {
+                const int t1 = min(t0 + ck, T);
+                ok = wait_progress(&s_prog[1], t1, ok);
+                for (int t = t0 + l; t < t1; t += WAVE)
+                {
+                    const float qt[4] = {s_xf[t * 16 + 3], s_xf[t * 16 + 4], s_xf[t * 16 + 5], s_xf[t * 16 + 6]};
+                    float b3[3];
+                    thrust_dir(qt, b3);
+                    const float at = s_uc[t].x * C.inv_mass;
+                    s_thr[t * 4 + 0] = make_float2(at, b3[0]);
+                    s_thr[t * 4 + 1] = make_float2(at, b3[1]);
+                    s_thr[t * 4 + 2] = make_float2(at, b3[2]);
+                }
+                if (l < 3)
+                {
+                    const auto trn = [&](int t, const float2 ab) {
+                        s_xf[t * 16 + l] = p;
+                        s_xf[t * 16 + 7 + l] = v;
+                        const float a = fma_(ab.x, ab.y, -(C.drag * v)) - g;
+                        p = fma_(C.dt, v, p);
+                        v = fma_(C.dt, a, v);
+                    };
+                    float2 aa = s_thr[t0 * 4 + l];
+                    int t = t0;
+                    for (; t + 1 < t1; t += 2)
+                    {
+                        const float2 ab = s_thr[(t + 1) * 4 + l];
+                        __builtin_amdgcn_sched_barrier(0); // keep the load a full step ahead of its use
+                        trn(t, aa);
+                        aa = s_thr[min(t + 2, t1 - 1) * 4 + l];
+                        __builtin_amdgcn_sched_barrier(0);
+                        trn(t + 1, ab);
+                    }
+                    if (t < t1)
+                        trn(t, aa);
+                }
+            }
+            if (l < 3)
+            {
+                s_xf[T * 16 + l] = p;
+                s_xf[T * 16 + 7 + l] = v;
+            }
+            if (!ok)
+                s_fail = 1;
+        }
+        __syncthreads();
+        if (tid == 0)
+            s_tend = T - 1;
         // G: gate counter.  The crossing test of a step's move reads only the
         // two positions and the gate, so every (step, gate) pair is tested in
         // parallel (lpt lanes per step).  Then wave 0 runs the counter: from
@@ -1166,7 +1238,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         __syncthreads();
     }
     __syncthreads();
-    for (int t = tid; t < T; t += REF_BLOCK)
+    for (int t = tid; t < T && !s_fail; t += REF_BLOCK)
     {
         const float4 u4 = s_ubest[t];
         float* u = nominal + I.nom_off + t * 4;

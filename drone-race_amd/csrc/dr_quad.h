@@ -136,17 +136,23 @@ DR_HD void clamp_control(const QuadConst& C, const float in[4], float out[4])
 //                attitude recurrence (attitude_increment, attitude_apply);
 //   body rates   first-order tracking of the commanded rates.
 // Products that feed a sum are fused (fmaf) in the order written.
-DR_HD void quad_translation_step(const QuadConst& C, const float q[4], float p[3], float v[3], const float u[4])
+// thrust direction R(q) e3
+DR_HD void thrust_dir(const float q[4], float b3[3])
 {
     const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
-    // thrust direction R(q) e3
-    const float b3x = 2.0f * fma_(qx, qz, qw * qy);
-    const float b3y = 2.0f * fma_(qy, qz, -(qw * qx));
-    const float b3z = fma_(-2.0f, fma_(qx, qx, qy * qy), 1.0f);
+    b3[0] = 2.0f * fma_(qx, qz, qw * qy);
+    b3[1] = 2.0f * fma_(qy, qz, -(qw * qx));
+    b3[2] = fma_(-2.0f, fma_(qx, qx, qy * qy), 1.0f);
+}
+
+DR_HD void quad_translation_step(const QuadConst& C, const float q[4], float p[3], float v[3], const float u[4])
+{
+    float b3[3];
+    thrust_dir(q, b3);
     const float at = u[0] * C.inv_mass;
-    const float ax = fma_(at, b3x, -(C.drag * v[0]));
-    const float ay = fma_(at, b3y, -(C.drag * v[1]));
-    const float az = fma_(at, b3z, -(C.drag * v[2])) - C.gravity;
+    const float ax = fma_(at, b3[0], -(C.drag * v[0]));
+    const float ay = fma_(at, b3[1], -(C.drag * v[1]));
+    const float az = fma_(at, b3[2], -(C.drag * v[2])) - C.gravity;
 
     p[0] = fma_(C.dt, v[0], p[0]);
     p[1] = fma_(C.dt, v[1], p[1]);

@@ -255,8 +255,9 @@ int dr_engine_cost_grad_noalloc(dr_engine* e, int instance, const float* U, int 
                                 int ckpt_every);
 /* Adjoint refinement inside every control step: after the MPPI update,
  * n_grad_steps projected gradient steps U <- clamp(U - step * dJ/dU) on each
- * instance's nominal (checkpoints every ckpt_every steps), the J of every
- * iterate evaluated and the lowest kept (the MPPI update itself on ties),
+ * instance's nominal (the forward pass hands its states on between its
+ * waves in blocks of ckpt_every steps; the result does not depend on it),
+ * the J of every iterate evaluated and the lowest kept (the MPPI update itself on ties),
  * then action and shift.  n_grad_steps = 0 (default) is plain MPPI.  No
  * counterpart in the reference (its controllers have no gradient path). */
 int dr_engine_set_refine(dr_engine* e, int n_grad_steps, float step, int ckpt_every);

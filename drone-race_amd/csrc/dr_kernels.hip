@@ -659,13 +659,15 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
 //     The steps 0..t_end are cut into n_seg <= REF_SEGS reverse segments
 //     [a, b); per segment, 14 lanes carry lam_b = 0 with the g_t terms
 //     (giving psi) and lam_b = e_i, i < 13, without them (giving column i of
-//     Psi) down to a: lam_a = Psi lam_b + psi.
+//     Psi) down to a: lam_a = Psi lam_b + psi.  Each chain also keeps its
+//     dJ/du_t at every step of the segment.
 //  E  wave 0, lane i holding component i: lam_b of every segment, from the
 //     terminal adjoint down to the first segment (one 13 x 13
 //     matrix-vector product per segment).
-//  F  one lane per reverse segment carries its exact lam_b down the segment
-//     and writes dJ/du_t and the projected step of u_t; steps after t_end
-//     have dJ/du = 0 (u_t is only clamped).
+//  F  one lane per step: dJ/du_t is the segment's psi chain plus its Psi
+//     chains weighted by lam_b (the products are linear in lam), then the
+//     projected step of u_t; steps after t_end have dJ/du = 0 (u_t is only
+//     clamped).
 // The order of the additions in C-F differs from the one-lane sweep, so J
 // and the gradient agree with cost_grad_lane to rounding, not bit for bit.
 constexpr int REF_BLOCK = 256;
@@ -776,6 +778,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     __shared__ float s_w[DR_MAX_T];            // running-cost weight of step t
     __shared__ float s_psi[REF_SEGS * REF_COMP * 16];
     __shared__ float s_lam[REF_SEGS * 16];     // lam_b of each reverse segment
+    __shared__ float4 s_gu[DR_MAX_T * REF_COMP]; // per step t and chain c of its segment: dJ/du_t along chain c
     __shared__ float4 s_op[MAX_OPP], s_ov[MAX_OPP];
     __shared__ float4 s_uc[DR_MAX_T + 1];  // the clamped controls the forward pass integrates (+1: read-ahead slot)
     __shared__ uint32_t s_cross[DR_MAX_T]; // bit g: the move x_t -> x_{t+1} crosses gate g
@@ -1154,6 +1157,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                 float uc[4], gu[4];
                 clamp_control(C, uin, uc);
                 quad_step_vjp(C, x, uc, lam, gu);
+                s_gu[t * REF_COMP + comp] = make_float4(gu[0], gu[1], gu[2], gu[3]);
                 const bool add = comp == 0 && t >= 1;
                 lam.p[0] = add ? lam.p[0] + r.gp.x : lam.p[0];
                 lam.p[1] = add ? lam.p[1] + r.gp.y : lam.p[1];
@@ -1193,39 +1197,33 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             }
         }
         __syncthreads();
-        // F: exact chain per segment, gradient and projected step of u_t
-        if (tid < n_seg)
+        // F: dJ/du_t of every step t <= te, one lane per step: the chains of
+        // t's segment combined with the segment's lam_b (the vector-Jacobian
+        // products are linear in lam), then the projected step of u_t
+        for (int t = tid; t <= te; t += REF_BLOCK)
         {
-            const int a = tid * R, b = min(a + R, te + 1);
-            float f[13];
+            const float* lb = s_lam + (t / R) * 16;
+            const float4* gc = s_gu + t * REF_COMP;
+            float4 g = gc[0];
 #pragma unroll
             for (int i = 0; i < 13; ++i)
-                f[i] = s_lam[tid * 16 + i];
-            QGrad lam = grad_from_flat(f);
-            RefRow nx = RefRow::load(s_x, s_u, s_gp, b - 1);
-            for (int t = b - 1; t >= a; --t)
             {
-                const RefRow r = nx;
-                nx = RefRow::load(s_x, s_u, s_gp, t > a ? t - 1 : t);
-                const QState x = r.state();
-                const float uin[4] = {r.u.x, r.u.y, r.u.z, r.u.w};
-                float uc[4], gu[4];
-                clamp_control(C, uin, uc);
-                quad_step_vjp(C, x, uc, lam, gu);
-                const float G[4] = {gu[0] * clamp_grad(uin[0], C.thrust_min, C.thrust_max),
-                                    gu[1] * clamp_grad(uin[1], -C.rate_max, C.rate_max),
-                                    gu[2] * clamp_grad(uin[2], -C.rate_max, C.rate_max),
-                                    gu[3] * clamp_grad(uin[3], -C.rate_max, C.rate_max)};
-                float in[4], out[4];
-                for (int d = 0; d < 4; ++d)
-                    in[d] = fma_(-step, G[d], uin[d]);
-                clamp_control(C, in, out);
-                s_u[t] = make_float4(out[0], out[1], out[2], out[3]);
-                const bool add = t >= 1;
-                lam.p[0] = add ? lam.p[0] + r.gp.x : lam.p[0];
-                lam.p[1] = add ? lam.p[1] + r.gp.y : lam.p[1];
-                lam.p[2] = add ? lam.p[2] + r.gp.z : lam.p[2];
+                const float li = lb[i];
+                const float4 gi = gc[1 + i];
+                g.x = fma_(li, gi.x, g.x);
+                g.y = fma_(li, gi.y, g.y);
+                g.z = fma_(li, gi.z, g.z);
+                g.w = fma_(li, gi.w, g.w);
             }
+            const float4 u4 = s_u[t];
+            const float uin[4] = {u4.x, u4.y, u4.z, u4.w};
+            const float G[4] = {g.x * clamp_grad(uin[0], C.thrust_min, C.thrust_max), g.y * clamp_grad(uin[1], -C.rate_max, C.rate_max),
+                                g.z * clamp_grad(uin[2], -C.rate_max, C.rate_max), g.w * clamp_grad(uin[3], -C.rate_max, C.rate_max)};
+            float in[4], out[4];
+            for (int d = 0; d < 4; ++d)
+                in[d] = fma_(-step, G[d], uin[d]);
+            clamp_control(C, in, out);
+            s_u[t] = make_float4(out[0], out[1], out[2], out[3]);
         }
         for (int t = te + 1 + tid; t < T; t += REF_BLOCK)
         {

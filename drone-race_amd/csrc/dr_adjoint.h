@@ -36,15 +36,20 @@ DR_HD void stage_cost_grad_at(const QuadConst& C, const CostConst& Q, const dr_c
             min_sq = d2;
             jmin = j;
         }
-        if (d2 < Q.r2_prox && d2 > 0.0f)
+        const bool near = d2 < Q.r2_prox && d2 > 0.0f;
+        if (any_lane(near))
         {
-            const float d = __builtin_sqrtf(d2);
-            const float x = fma_(-d, Q.inv_prox, 1.0f);
-            // d/dp W x^2 = 2 W x * (-inv_prox) * (p - o) / d
-            const float f = weight * (2.0f * K.opp_prox_cost * x) * (-Q.inv_prox) / d;
-            g[0] = fma_(f, dx, g[0]);
-            g[1] = fma_(f, dy, g[1]);
-            g[2] = fma_(f, dz, g[2]);
+            keep_branch();
+            if (near)
+            {
+                const float d = __builtin_sqrtf(d2);
+                const float x = fma_(-d, Q.inv_prox, 1.0f);
+                // d/dp W x^2 = 2 W x * (-inv_prox) * (p - o) / d
+                const float f = weight * (2.0f * K.opp_prox_cost * x) * (-Q.inv_prox) / d;
+                g[0] = fma_(f, dx, g[0]);
+                g[1] = fma_(f, dy, g[1]);
+                g[2] = fma_(f, dz, g[2]);
+            }
         }
     }
     float d_opp = __builtin_inff();

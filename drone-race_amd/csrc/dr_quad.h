@@ -93,6 +93,28 @@ DR_HD float clampf(float x, float lo, float hi)
     return fminf(fmaxf(x, lo), hi);
 }
 
+// Whether c holds on any active lane of the wave (on the host: c itself).  A
+// branch on it is uniform, so a block that no lane needs is skipped rather
+// than computed and discarded.
+DR_HD bool any_lane(bool c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ballot_w64(c) != 0;
+#else
+    return c;
+#endif
+}
+
+// Placed first in a block under any_lane: keeps the compiler from hoisting
+// the block's work above the branch (computing it on every wave and
+// selecting the result), which would undo the branch.
+DR_HD void keep_branch()
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
 // 1/sqrt(x) for normal x > 0 (the attitude renormalisation): a first
 // estimate from the bit pattern with Lomont's constant 0x5f375a86 (C. Lomont,
 // "Fast Inverse Square Root", Purdue University, 2003; relative error below
@@ -352,23 +374,37 @@ DR_HD float stage_cost_at(const QuadConst& C, const CostConst& Q, const dr_cost_
 {
     float cost = 0.0f;
     float min_sq = __builtin_inff();
+    float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (n_opp > 0)
+        o = opp_at(0);
     for (int j = 0; j < n_opp; ++j)
     {
-        const float4 o = opp_at(j);
+        const float4 on = opp_at(j + 1 < n_opp ? j + 1 : j); // read ahead
         const float dx = s.p[0] - o.x;
         const float dy = s.p[1] - o.y;
         const float dz = s.p[2] - o.z;
         const float d2 = fma_(dx, dx, fma_(dy, dy, dz * dz));
         min_sq = fminf(min_sq, d2);
-        if (d2 < Q.r2_prox)
+        const bool near = d2 < Q.r2_prox;
+        if (any_lane(near))
         {
-            const float x = fma_(-__builtin_sqrtf(d2), Q.inv_prox, 1.0f);
-            cost = fma_(K.opp_prox_cost, x * x, cost);
+            keep_branch();
+            if (near)
+            {
+                const float x = fma_(-__builtin_sqrtf(d2), Q.inv_prox, 1.0f);
+                cost = fma_(K.opp_prox_cost, x * x, cost);
+            }
         }
+        o = on;
     }
     float d_opp = __builtin_inff();
-    if (min_sq < Q.far2)
-        d_opp = __builtin_sqrtf(min_sq) - C.drone_radius;
+    const bool reach = min_sq < Q.far2;
+    if (any_lane(reach))
+    {
+        keep_branch();
+        if (reach)
+            d_opp = __builtin_sqrtf(min_sq) - C.drone_radius;
+    }
     const float d_ground = s.p[2] - tr.ground_z;
     const float bd = fminf(d_opp, d_ground);
     if (bd < Q.danger)
@@ -421,10 +457,15 @@ DR_HD float pr_stage_cost(const QuadConst& C, const CostConst& Q, const dr_cost_
         const float dz = s.p[2] - o.z;
         const float d2 = fma_(dx, dx, fma_(dy, dy, dz * dz));
         min_sq = fminf(min_sq, d2);
-        if (d2 < Q.r2_prox)
+        const bool near = d2 < Q.r2_prox;
+        if (any_lane(near))
         {
-            const float x = fma_(-__builtin_sqrtf(d2), Q.inv_prox, 1.0f);
-            cost = fma_(K.opp_prox_cost, x * x, cost);
+            keep_branch();
+            if (near)
+            {
+                const float x = fma_(-__builtin_sqrtf(d2), Q.inv_prox, 1.0f);
+                cost = fma_(K.opp_prox_cost, x * x, cost);
+            }
         }
     }
     const float d_opp = __builtin_sqrtf(min_sq) - C.drone_radius;

@@ -726,9 +726,10 @@ int dr_engine_set_launch(dr_engine* e, const dr_launch_cfg* cfg)
         c.steps_per_graph = 1;
     if (!valid_block(c.rollout_block) || !valid_chunk(c.reduce_chunk) || !valid_nb(c.noise_batch) || c.rollout_grid < 0 ||
         c.steps_per_graph < 1 || c.steps_per_graph > 64 || c.reduce_tgroup < 1 || c.reduce_tgroup > DR_MAX_T ||
-        (c.rollout_split < 0 || c.rollout_split > 2))
+        (c.rollout_split < 0 || c.rollout_split > 3) || (c.rollout_split == 3 && c.noise_batch > 2))
         return fail(DR_EINVAL, "set_launch: rollout_block in {64,128,256}, reduce_chunk a multiple of 64 in [64,4096], "
-                               "noise_batch in {1,2,4}, steps_per_graph in [1,64], reduce_tgroup in [1,256], rollout_split in {0,1,2}");
+                               "noise_batch in {1,2,4} ({1,2} with rollout_split 3), steps_per_graph in [1,64], "
+                               "reduce_tgroup in [1,256], rollout_split in {0,1,2,3}");
     HIPCHK(hipStreamSynchronize(e->stream));
     const bool relayout = c.rollout_block != e->launch.rollout_block || c.reduce_chunk != e->launch.reduce_chunk ||
                           c.reduce_tgroup != e->launch.reduce_tgroup || c.rollout_split != e->launch.rollout_split;
@@ -957,17 +958,17 @@ int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* bes
         if (hipGetDeviceProperties(&prop, e->device) == hipSuccess)
             n_cu = prop.multiProcessorCount;
     }
-    // Candidate shapes: the three rollout structures (one lane per sample,
-    // noise wave + dynamics wave, four-stage pipeline) x noise batch x
-    // weighted-average chunk x timestep group.
+    // Candidate shapes: the four rollout structures (one lane per sample,
+    // noise wave + dynamics wave, four- and eight-stage pipelines) x noise
+    // batch x weighted-average chunk x timestep group.
     std::vector<dr_launch_cfg> cands;
-    for (int split : {0, 1, 2})
+    for (int split : {0, 1, 2, 3})
         for (int nb : {1, 2, 4})
             for (int ch : {512, 1024, 2048})
                 for (int tg : {16, 32})
                     for (int rb : {128, 256})
                     {
-                        if (split != 0 && rb != 128)
+                        if ((split != 0 && rb != 128) || (split == 3 && nb > 2))
                             continue;
                         dr_launch_cfg c = saved;
                         c.rollout_split = split;

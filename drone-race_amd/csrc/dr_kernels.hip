@@ -426,6 +426,227 @@ __global__ __launch_bounds__(256) void k_rollout_p4(const InstDev* __restrict__ 
     }
 }
 
+// Eight-stage pipelined rollout for small sample counts: a workgroup of eight
+// waves covers 64 samples, so that a launch with few tiles still gives every
+// SIMD four instruction streams.  Stages and the chunk each works on at
+// chunk period i (stages joined by LDS rings of CH steps, one barrier per
+// period):
+//   P   Philox4x32-10 words, even steps / odd steps (two waves)   chunk i
+//   B   Box-Muller, sigma, noise row store, even / odd (two)      chunk i - 1
+//   R   clamp(nominal + eps), body rates and attitude             chunk i - 2
+//   X   velocity, position from the pre-step attitude, gate       chunk i - 3
+//   C   running cost and terminal test, even / odd steps (two)    chunk i - 4
+//   S   (with the even cost wave) discounted sum in step order,
+//       stop at the first terminal state                          chunk i - 5
+// Wave w works as stage (w + blockIdx) mod 8 in the order P0 P1 B0 B1 C1 C0
+// X R: waves w and w + 4 share a SIMD, and the pairs (P, C), (B, X), (B, R)
+// carry about equal work whatever the rotation.  The arithmetic of every
+// sample is that of k_rollout.
+template <int CH>
+__global__ __launch_bounds__(512, 4) void k_rollout_p8(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
+                                                       const RTile* __restrict__ tiles, int n_tiles, const dr_track* __restrict__ tracks,
+                                                       const dr_drone_state* __restrict__ drones, const float* __restrict__ nominal,
+                                                       float* __restrict__ costs, float4* __restrict__ noise, float* __restrict__ tmin,
+                                                       QuadConst C, CostConst Q, dr_cost_cfg KC, int opp_stride)
+{
+    static_assert(CH % 2 == 0, "even and odd steps of a chunk");
+    extern __shared__ float4 s_opp[]; // [T][opp_stride]
+    __shared__ uint4 r_w[2][CH][WAVE];  // P -> B
+    __shared__ float4 r_e[2][CH][WAVE]; // B -> R: sigma * z
+    __shared__ float4 r_q[2][CH][WAVE]; // R -> X: attitude at the start of the step
+    __shared__ float r_f[2][CH][WAVE];  // R -> X: clamped collective thrust
+    __shared__ float4 r_s[3][CH][WAVE]; // X -> C, S: position after the step, gate | laps << 8
+    __shared__ float2 r_c[2][CH][WAVE]; // C -> S: running cost, terminal code
+    __shared__ float s_nom[DR_MAX_T * 4];
+    __shared__ dr_track s_track;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    // stage of this wave: 0 P-even 1 P-odd 2 B-even 3 B-odd 4 C-odd 5 C-even(+S) 6 X 7 R
+    const int role = ((tid / WAVE) + (int)blockIdx.x) & 7;
+    constexpr int kStage[8] = {0, 0, 1, 1, 4, 4, 3, 2};
+    const int stage = kStage[role];
+    const int parity = (role == 1 || role == 3 || role == 4) ? 1 : 0;
+    for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
+    {
+        const RTile rt = tiles[tile];
+        const InstDev& I = insts[rt.inst];
+        const int K = I.K, T = I.T;
+        const uint32_t step = dyn[rt.inst].step;
+        const int n_opp = I.n_opp;
+        const uint32_t key0 = I.key0, key1 = I.key1;
+        const float sg0 = I.sigma[0], sg1 = I.sigma[1], sg2 = I.sigma[2], sg3 = I.sigma[3];
+
+        __syncthreads(); // LDS of the previous tile is no longer read
+        for (int i = tid; i < T * 4; i += 512)
+            s_nom[i] = nominal[I.nom_off + i];
+        {
+            const float* src = reinterpret_cast<const float*>(tracks + I.track);
+            float* dst = reinterpret_cast<float*>(&s_track);
+            for (int i = tid; i < (int)(sizeof(dr_track) / 4); i += 512)
+                dst[i] = src[i];
+        }
+        for (int i = tid; i < T * n_opp; i += 512)
+        {
+            const int t = i / n_opp, j = i - t * n_opp;
+            s_opp[t * opp_stride + j] = opp_predict(drones[I.opp[j]], C.dt * (float)(t + 1));
+        }
+        __syncthreads();
+
+        const int k = rt.k0 + lane;
+        const bool live = k < K;
+        const int n_chunks = (T + CH - 1) / CH;
+
+        QState s; // R: attitude and rates; X: position, velocity, gate, laps
+        if (live)
+            s = state_from_abi(drones[I.drone]);
+        // S: discounted sum, the state the terminal cost reads (position and gate code)
+        float acc = 0.0f, decay = 1.0f;
+        float4 last = make_float4(s.p[0], s.p[1], s.p[2], u2f((uint32_t)(s.gate | (s.laps << 8))));
+        bool stop = false;
+
+        for (int i = 0; i < n_chunks + 5; ++i)
+        {
+            const int c = i - stage;
+            if (live && c >= 0 && c < n_chunks)
+            {
+                if (stage == 0)
+                {
+#pragma unroll
+                    for (int u = parity; u < CH; u += 2)
+                    {
+                        U4 ctr;
+                        ctr.x = (uint32_t)k;
+                        ctr.y = (uint32_t)(c * CH + u);
+                        ctr.z = step;
+                        ctr.w = STREAM_CONTROL;
+                        const U4 w = philox4x32_10(ctr, key0, key1);
+                        r_w[c & 1][u][lane] = make_uint4(w.x, w.y, w.z, w.w);
+                    }
+                }
+                else if (stage == 1)
+                {
+                    float4* nrow = noise + I.noise_off + (long long)k * T;
+#pragma unroll
+                    for (int u = parity; u < CH; u += 2)
+                    {
+                        const int t = c * CH + u;
+                        const uint4 w = r_w[c & 1][u][lane];
+                        float z[4];
+                        box_muller(w.x, w.y, &z[0], &z[1]);
+                        box_muller(w.z, w.w, &z[2], &z[3]);
+                        const float4 e = make_float4(sg0 * z[0], sg1 * z[1], sg2 * z[2], sg3 * z[3]);
+                        r_e[c & 1][u][lane] = e;
+                        if (t < T)
+                            nrow[t] = e;
+                    }
+                }
+                else if (stage == 2)
+                {
+#pragma unroll
+                    for (int u = 0; u < CH; ++u)
+                    {
+                        const int t = c * CH + u;
+                        if (t < T)
+                        {
+                            const float4 e = r_e[c & 1][u][lane];
+                            float uin[4], uc[4];
+                            uin[0] = s_nom[t * 4 + 0] + e.x;
+                            uin[1] = s_nom[t * 4 + 1] + e.y;
+                            uin[2] = s_nom[t * 4 + 2] + e.z;
+                            uin[3] = s_nom[t * 4 + 3] + e.w;
+                            clamp_control(C, uin, uc);
+                            r_q[c & 1][u][lane] = make_float4(s.q[0], s.q[1], s.q[2], s.q[3]);
+                            r_f[c & 1][u][lane] = uc[0];
+                            quad_attitude_step(C, s.q, s.w, uc);
+                        }
+                    }
+                }
+                else if (stage == 3)
+                {
+#pragma unroll
+                    for (int u = 0; u < CH; ++u)
+                    {
+                        const int t = c * CH + u;
+                        if (t < T)
+                        {
+                            const float4 q4 = r_q[c & 1][u][lane];
+                            const float q[4] = {q4.x, q4.y, q4.z, q4.w};
+                            const float uu[4] = {r_f[c & 1][u][lane], 0.0f, 0.0f, 0.0f}; // the translation reads the thrust only
+                            const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+                            quad_translation_step(C, q, s.p, s.v, uu);
+                            update_gate(s_track.gates, s_track.n_gates, KC.gate_traversal_margin, p0, s);
+                            r_s[c % 3][u][lane] = make_float4(s.p[0], s.p[1], s.p[2], u2f((uint32_t)(s.gate | (s.laps << 8))));
+                        }
+                    }
+                }
+                else
+                {
+#pragma unroll
+                    for (int u = parity; u < CH; u += 2)
+                    {
+                        const int t = c * CH + u;
+                        if (t < T)
+                        {
+                            const float4 v = r_s[c % 3][u][lane];
+                            const uint32_t code = f2u(v.w);
+                            QState x;
+                            x.p[0] = v.x;
+                            x.p[1] = v.y;
+                            x.p[2] = v.z;
+                            x.gate = (int)(code & 0xffu);
+                            x.laps = (int)code >> 8;
+                            int term;
+                            const float cst = stage_cost(C, Q, KC, s_track, x, s_opp + t * opp_stride, n_opp, &term);
+                            r_c[c & 1][u][lane] = make_float2(cst, u2f((uint32_t)term));
+                        }
+                    }
+                }
+            }
+            // S: the even cost wave sums the chunk costed one period earlier
+            const int cs = i - 5;
+            if (role == 5 && live && cs >= 0 && cs < n_chunks)
+            {
+#pragma unroll
+                for (int u = 0; u < CH; ++u)
+                {
+                    const int t = cs * CH + u;
+                    if (t < T && !stop)
+                    {
+                        const float2 ct = r_c[cs & 1][u][lane];
+                        last = r_s[cs % 3][u][lane];
+                        acc = fma_(ct.x, decay, acc);
+                        decay = decay * DECAY;
+                        stop = ((int)f2u(ct.y) != TERM_NONE);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+        if (role == 5)
+        {
+            float cost = __builtin_inff();
+            if (live)
+            {
+                const uint32_t code = f2u(last.w);
+                QState x;
+                x.p[0] = last.x;
+                x.p[1] = last.y;
+                x.p[2] = last.z;
+                x.gate = (int)(code & 0xffu);
+                x.laps = (int)code >> 8;
+                acc = acc + final_cost(KC, s_track, x);
+                costs[I.cost_off + k] = acc;
+                cost = acc;
+            }
+            const float wm = wave_min(cost);
+            if (lane == 0)
+                tmin[tile] = wm;
+        }
+    }
+}
+
 // Noise-free rollout of each instance's nominal from its current state (the
 // plan's own cost, reported as traj_cost): one workgroup per instance, the
 // opponents' predicted positions in LDS, one active lane.  It runs on a
@@ -1513,6 +1734,24 @@ hipError_t launch_rollout(int block, int nb, int split, int grid, hipStream_t st
         return hipErrorInvalidValue;
     const int g = grid > 0 ? (grid < n_tiles ? grid : n_tiles) : n_tiles;
     const size_t shm = (size_t)max_T * opp_stride * sizeof(float4);
+    if (split == 3)
+    {
+        // tiles of 64 samples, eight stage waves; chunk length = 2 * noise_batch steps
+        switch (nb)
+        {
+        case 1:
+            k_rollout_p8<2><<<g, 512, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                 opp_stride);
+            break;
+        case 2:
+            k_rollout_p8<4><<<g, 512, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                 opp_stride);
+            break;
+        default:
+            return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     if (split == 2)
     {
         // tiles of 64 samples, four stage waves; chunk length = 2 * noise_batch steps

@@ -162,7 +162,8 @@ typedef struct dr_launch_cfg
     int32_t steps_per_graph;  /* control steps captured per hipGraph */
     int32_t use_graph;        /* 0: plain stream launches */
     int32_t reduce_tgroup;    /* timesteps per weighted-average workgroup: 1..256 */
-    int32_t rollout_split;    /* 0: one lane per sample; 1: noise wave + dynamics wave; 2: four-stage wave pipeline */
+    int32_t rollout_split;    /* 0: one lane per sample; 1: noise wave + dynamics wave; 2: four-stage wave pipeline;
+                                 3: eight-stage wave pipeline (noise_batch 1 or 2) */
 } dr_launch_cfg;
 
 typedef struct dr_engine dr_engine;

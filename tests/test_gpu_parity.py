@@ -174,7 +174,8 @@ def test_closed_loop_matches_oracle(pkg, oracle):
 
 def test_launch_shapes_agree(pkg):
     """Every launch shape (rollout block, persistent grid, samples per lane,
-    timestep group, graph or stream) gives the same costs, noise and nominal."""
+    timestep group, graph or stream, rollout structure) gives the same costs,
+    noise and nominal."""
     sc = pkg.scenarios
     say("case: launch shapes on 3 uneven instances")
     scn = small_scenario(sc, n_samples=3000, n_timesteps=45, n_drones=3, controlled=3)
@@ -188,7 +189,8 @@ def test_launch_shapes_agree(pkg):
                                                     (64, 1, 192, 2, 0, 1, 1, 0), (128, 0, 1024, 1, 1, 1, 25, 1),
                                                     (128, 0, 512, 2, 0, 1, 16, 1), (128, 5, 1024, 4, 1, 2, 32, 1),
                                                     (128, 0, 1024, 1, 1, 1, 16, 2), (128, 0, 1024, 2, 0, 1, 16, 2),
-                                                    (128, 9, 256, 4, 1, 2, 25, 2)]]
+                                                    (128, 9, 256, 4, 1, 2, 25, 2), (128, 0, 1024, 1, 1, 1, 16, 3),
+                                                    (128, 0, 512, 2, 0, 1, 25, 3), (128, 7, 2048, 2, 1, 2, 32, 3)]]
     for shape in shapes:
         eng = make_engine(pkg, scn)
         eng.set_launch(**shape)

@@ -865,17 +865,20 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
 //  A  the dynamics (attitude, rates, velocity, position) over all T steps,
 //     every state into the state table, as a three-wave pipeline handing on
 //     blocks of ck steps through LDS progress counters: body rates and
-//     attitude increments (lane 128), attitude (lane 0), velocity and
-//     position (lane 64); neither the gate counter nor any cost is evaluated
-//     here, so each wave's serial chain is one part of the dynamics, and no
-//     workgroup barrier falls inside the pass.
-//  G  every (step, gate) pair's crossing test in parallel; wave 0 then runs
-//     the gate counter by searching the crossing bits for the next gate's
-//     crossing (one ballot per 64 steps) and every state slot takes the gate
-//     and laps after the crossings before it.
-//  C  one lane per step t: terminal test, discounted running cost and dJ/dp
-//     of the running cost of the post-step state x_{t+1}; the first terminal
-//     step t_end ends the trajectory (cost_grad_lane's break); wave 0 sums J.
+//     attitude increments (wave 2), attitude (wave 0), velocity and position
+//     (wave 1), each serial recurrence one lane per vector component; these
+//     waves evaluate neither the gate counter nor any cost, so each serial
+//     chain is one part of the dynamics, and no workgroup barrier falls
+//     inside the pass.  Behind the published positions, wave 3 runs every
+//     (step, gate) pair's crossing test and the running cost of each
+//     post-step state x_{t+1} without the winner test, and wave 2 (after its
+//     rates) the running cost's dJ/dp (both read the position only).
+//  G  wave 0 runs the gate counter by searching the crossing bits for the
+//     next gate's crossing (one ballot per 64 steps) and every state slot
+//     takes the gate and laps after the crossings before it.
+//  C  one lane per step t: the winner test on x_{t+1} (laps now known), the
+//     discounted running cost; the first terminal step t_end ends the
+//     trajectory (cost_grad_lane's break); wave 0 sums J.
 //  D  the reverse recurrence lam_t = A_t^T lam_{t+1} + g_t is linear in lam.
 //     The steps 0..t_end are cut into n_seg <= REF_SEGS reverse segments
 //     [a, b); per segment, 14 lanes carry lam_b = 0 with the g_t terms
@@ -1010,7 +1013,9 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     __shared__ float4 s_ubest[DR_MAX_T]; // the lowest-cost sequence evaluated so far
     __shared__ float s_jbest;
     __shared__ int s_tend, s_better;
-    __shared__ int s_prog[2]; // forward pipeline: steps whose increments (0) / start attitudes (1) are in LDS
+    __shared__ int s_prog[3]; // forward pipeline: steps whose increments (0) / start attitudes (1) / end positions (2) are in LDS
+    __shared__ float s_cb[DR_MAX_T]; // running cost of x_{t+1} without the winner test
+    __shared__ int s_tb[DR_MAX_T];   // its terminal code (collision or none)
     __shared__ int s_fail;    // a pipeline wait hit its bound: keep the plain update
 
     const InstDev& I = insts[blockIdx.x];
@@ -1066,6 +1071,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         {
             s_prog[0] = 0;
             s_prog[1] = 0;
+            s_prog[2] = 0;
         }
         __syncthreads();
         // A: forward pass of the dynamics, every state into the state table,
@@ -1170,6 +1176,11 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             const int l = tid - WAVE;
             float p = l < 3 ? x0.p[l] : 0.0f, v = l < 3 ? x0.v[l] : 0.0f;
             const float g = l == 2 ? C.gravity : 0.0f; // x - 0 is x: only the z lane's subtraction changes anything
+            if (l < 3)
+            {
+                s_xf[l] = p;
+                s_xf[7 + l] = v;
+            }
             bool ok = true;
             for (int t0 = 0; t0 < T; t0 += ck)
             {
@@ -1188,11 +1199,11 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                 if (l < 3)
                 {
                     const auto trn = [&](int t, const float2 ab) {
-                        s_xf[t * 16 + l] = p;
-                        s_xf[t * 16 + 7 + l] = v;
                         const float a = fma_(ab.x, ab.y, -(C.drag * v)) - g;
                         p = fma_(C.dt, v, p);
                         v = fma_(C.dt, a, v);
+                        s_xf[(t + 1) * 16 + l] = p;
+                        s_xf[(t + 1) * 16 + 7 + l] = v;
                     };
                     float2 aa = s_thr[t0 * 4 + l];
                     int t = t0;
@@ -1208,11 +1219,67 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                     if (t < t1)
                         trn(t, aa);
                 }
+                if (l == 0)
+                    publish_progress(&s_prog[2], t1);
             }
-            if (l < 3)
+            if (!ok)
+                s_fail = 1;
+        }
+        // Behind the translation wave, on published ranges of at least 32
+        // steps (up to 64; a lane does the same work for one step as for a
+        // whole range): wave 3 tests every (step, gate) pair's crossing and
+        // forms the running cost of x_{t+1} without the winner test; wave 2,
+        // once its rates are done, forms the running cost's dJ/dp of x_{t+1}
+        // (not needed on the last pass).  Both read positions only.
+        if (tid >= 3 * WAVE || (tid >= 2 * WAVE && it < n_steps))
+        {
+            const int l = tid & (WAVE - 1);
+            const bool crossings = tid >= 3 * WAVE;
+            const int ng = s_track.n_gates;
+            bool ok = true;
+            int done = 0;
+            while (done < T)
             {
-                s_xf[T * 16 + l] = p;
-                s_xf[T * 16 + 7 + l] = v;
+                ok = wait_progress(&s_prog[2], min(done + 32, T), ok);
+                const int avail = ok ? __hip_atomic_load(&s_prog[2], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) : T;
+                const int b = min(avail, done + WAVE);
+                if (crossings)
+                    for (int i = l; i < (b - done) * ng; i += WAVE)
+                    {
+                        const int t = done + i / ng, gi = i % ng;
+                        const float p0[3] = {s_xf[t * 16], s_xf[t * 16 + 1], s_xf[t * 16 + 2]};
+                        const float p1[3] = {s_xf[(t + 1) * 16], s_xf[(t + 1) * 16 + 1], s_xf[(t + 1) * 16 + 2]};
+                        if (gate_crossed(s_track.gates[gi], margin, p0, p1))
+                            atomicOr(&s_cross[t], 1u << gi);
+                    }
+                const int t = done + l;
+                if (t < b)
+                {
+                    QState x;
+                    x.p[0] = s_xf[(t + 1) * 16];
+                    x.p[1] = s_xf[(t + 1) * 16 + 1];
+                    x.p[2] = s_xf[(t + 1) * 16 + 2];
+                    const float tau = C.dt * (float)(t + 1);
+                    const auto opp_at = [&](int j) {
+                        const float4 o = s_op[j], ov = s_ov[j];
+                        return make_float4(fma_(ov.x, tau, o.x), fma_(ov.y, tau, o.y), fma_(ov.z, tau, o.z), 0.0f);
+                    };
+                    if (crossings)
+                    {
+                        int term;
+                        s_cb[t] = stage_cost_body_at(C, Q, KC, s_track, x, n_opp, opp_at, &term);
+                        s_tb[t] = term;
+                    }
+                    else
+                    {
+                        float gp[3] = {0.0f, 0.0f, 0.0f};
+                        stage_cost_grad_at(C, Q, KC, s_track, x, n_opp, opp_at, s_w[t], gp);
+                        s_gp[t].x = gp[0];
+                        s_gp[t].y = gp[1];
+                        s_gp[t].z = gp[2];
+                    }
+                }
+                done = b;
             }
             if (!ok)
                 s_fail = 1;
@@ -1221,30 +1288,14 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         if (tid == 0)
             s_tend = T - 1;
         // G: gate counter.  The crossing test of a step's move reads only the
-        // two positions and the gate, so every (step, gate) pair is tested in
-        // parallel (lpt lanes per step).  Then wave 0 runs the counter: from
+        // two positions and the gate, so every (step, gate) pair was tested
+        // during the forward pass.  Wave 0 runs the counter: from
         // step t the next crossing is the first t' >= t whose move crosses the
         // next gate (the counter cannot change in between), found with one
         // ballot per 64 steps; the crossings are listed, and every state slot
         // takes the gate and laps after the crossings before it -- exactly the
         // per-step update (next gate = gate + 1 mod n, a crossing of gate 0
         // counts a lap).
-        {
-            const int ng = s_track.n_gates;
-            const int lpt = T >= REF_BLOCK ? 1 : REF_BLOCK / T;
-            for (int i = tid; i < T * lpt; i += REF_BLOCK)
-            {
-                const int t = i / lpt, sub = i - t * lpt;
-                const float4 a4 = s_x[t * 4], b4 = s_x[(t + 1) * 4];
-                const float p0[3] = {a4.x, a4.y, a4.z}, p1[3] = {b4.x, b4.y, b4.z};
-                uint32_t bits = 0u;
-                for (int g = sub; g < ng; g += lpt)
-                    bits |= gate_crossed(s_track.gates[g], margin, p0, p1) ? (1u << g) : 0u;
-                if (bits)
-                    atomicOr(&s_cross[t], bits);
-            }
-        }
-        __syncthreads();
         if (tid < WAVE)
         {
             const int ng = s_track.n_gates;
@@ -1290,23 +1341,15 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             s_x[k * 4 + 3].z = u2f((uint32_t)laps);
         }
         __syncthreads();
-        // C: terminal test and running-cost gradient, one lane per step
+        // C: the winner test on the running cost of every x_{t+1} (laps now
+        // known), the first terminal step, and the discounted cost for J
         for (int t = tid; t < T; t += REF_BLOCK)
         {
-            float w;
-            const QState s = get_state(s_x, t + 1, 1, &w);
-            const float tau = C.dt * (float)(t + 1);
-            const auto opp_at = [&](int j) {
-                const float4 p = s_op[j], v = s_ov[j];
-                return make_float4(fma_(v.x, tau, p.x), fma_(v.y, tau, p.y), fma_(v.z, tau, p.z), 0.0f);
-            };
-            int term;
-            const float c = stage_cost_at(C, Q, KC, s_track, s, n_opp, opp_at, &term);
+            int term = s_tb[t];
+            const float c = stage_cost_win(KC, s_track, (int)f2u(s_x[(t + 1) * 4 + 3].z), s_cb[t], &term);
             if (term != TERM_NONE)
                 atomicMin(&s_tend, t);
-            float g[3] = {0.0f, 0.0f, 0.0f};
-            stage_cost_grad_at(C, Q, KC, s_track, s, n_opp, opp_at, s_w[t], g);
-            s_gp[t] = make_float4(g[0], g[1], g[2], c * s_w[t]); // .w: discounted running cost
+            s_gp[t].w = c * s_w[t]; // discounted running cost
         }
         __syncthreads();
         const int te = s_tend;

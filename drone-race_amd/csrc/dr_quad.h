@@ -368,9 +368,13 @@ DR_HD float advance(const dr_gate* gates, int n_gates, const QState& s)
 // opp holds the opponents' predicted positions at this step (x, y, z, -).
 // An opponent farther than sqrt(far2) cannot bring bd below the penalty band,
 // so its distance is taken as +inf without a square root.
+//
+// stage_cost_body_at: every term but the winner test (*term is TERM_LOSE
+// after a collision, else TERM_NONE; reads the position only);
+// stage_cost_win: the winner test on its result; stage_cost_at: both.
 template <class OppAt>
-DR_HD float stage_cost_at(const QuadConst& C, const CostConst& Q, const dr_cost_cfg& K, const dr_track& tr, const QState& s, int n_opp,
-                          OppAt opp_at, int* term)
+DR_HD float stage_cost_body_at(const QuadConst& C, const CostConst& Q, const dr_cost_cfg& K, const dr_track& tr, const QState& s,
+                               int n_opp, OppAt opp_at, int* term)
 {
     float cost = 0.0f;
     float min_sq = __builtin_inff();
@@ -418,13 +422,26 @@ DR_HD float stage_cost_at(const QuadConst& C, const CostConst& Q, const dr_cost_
         cost = cost + K.outside_cost;
         t = TERM_LOSE;
     }
-    else if (s.laps >= tr.n_win_laps)
-    {
-        cost = cost - K.win_cost;
-        t = TERM_WIN;
-    }
     *term = t;
     return cost;
+}
+
+DR_HD float stage_cost_win(const dr_cost_cfg& K, const dr_track& tr, int laps, float cost, int* term)
+{
+    if (*term == TERM_NONE && laps >= tr.n_win_laps)
+    {
+        cost = cost - K.win_cost;
+        *term = TERM_WIN;
+    }
+    return cost;
+}
+
+template <class OppAt>
+DR_HD float stage_cost_at(const QuadConst& C, const CostConst& Q, const dr_cost_cfg& K, const dr_track& tr, const QState& s, int n_opp,
+                          OppAt opp_at, int* term)
+{
+    const float cost = stage_cost_body_at(C, Q, K, tr, s, n_opp, opp_at, term);
+    return stage_cost_win(K, tr, s.laps, cost, term);
 }
 
 DR_HD float stage_cost(const QuadConst& C, const CostConst& Q, const dr_cost_cfg& K, const dr_track& tr, const QState& s,

@@ -612,6 +612,28 @@ def drace_race(pkg, steps=30, warmup=3):
             "steps": steps}
 
 
+def record_lengths_run(pkg, env, steps=4):
+    """`steps` closed-loop steps from the start of one branching race (bm: StratRace2D, ho:
+    HiddenObs, drace: DroneRace; one world at the reference sizes) with rollout-length recording on
+    from the first step: the rollout-steps each rollout launch ran, so that counter totals over
+    those launches can be put per rollout-step."""
+    sc = pkg.scenarios
+    scn, cls = {"bm": (sc.bm_race, pkg.BMEngine), "ho": (sc.ho_race, pkg.HoBMEngine),
+                "drace": (sc.drace_race, pkg.DraceBMEngine)}[env]
+    scn = scn(n_inst=1)
+    eng = cls(scn)
+    eng.set_record_lengths(True)
+    ran, slots = [], 0
+    for _ in range(steps):
+        eng.step(1, advance=True)
+        lens = [eng.rollout_lengths(i) for i in range(eng.n_inst)]
+        ran.append(sum(int(a.sum()) for a in lens))
+        slots = sum(int(a.size) for a in lens) * scn.instances[0].n_timesteps
+    eng.set_record_lengths(False)
+    eng.close()
+    return {"workload": scn.name, "env": env, "steps": steps, "rollout_steps_per_launch": ran, "rollout_slots_per_launch": slots}
+
+
 def session_race(pkg, steps=30):
     """The simulation session behind the wire protocol (SimulationEngine::run) at the reference's
     settings: the branching spline MPPI header (K = 2^16, T = 80, 20 knots) and the PR-MPPI header
@@ -701,6 +723,10 @@ def main():
     ap.add_argument("--bm-only", action="store_true", help="run only the branching spline MPPI section (for profiling)")
     ap.add_argument("--session-only", action="store_true", help="run only the wire-protocol session section")
     ap.add_argument("--ho-only", action="store_true", help="run only the HiddenObs and DroneRace sections (for profiling)")
+    ap.add_argument("--record-lengths", default="", choices=["", "bm", "ho", "drace"],
+                    help="only run 4 closed-loop steps of one branching race (bm: StratRace2D, ho: HiddenObs, drace: "
+                         "DroneRace; one world, reference sizes) with rollout-length recording on, and print the "
+                         "rollout-steps each rollout launch ran (the program profiled for the instruction-mix counters)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 512 MiB buffer 3 times (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -712,6 +738,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": src.numel(), "copies": 3}))
+        return
+    if args.record_lengths:
+        print(json.dumps(record_lengths_run(package(), args.record_lengths)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (each controller batch runs on a "

@@ -858,21 +858,18 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
 // (one more forward pass after the last step) and the lowest-J sequence is
 // kept, the plain MPPI update (iterate 0) on ties: a step that jumps over a
 // discontinuity of the cost (a lost gate, a collision) is not taken.  J and
-// dJ/dU are those of cost_grad_lane (same forward, same terminal rule, same
-// vector-Jacobian products); the work is spread over the lanes so that only
-// the forward recurrence of the dynamics stays serial:
+// dJ/dU are those of cost_grad_lane (the same dynamics, terminal rule and
+// vector-Jacobian products); the work is spread over the lanes, the time
+// recurrences included:
 //  P  the iterate's controls are clamped, one lane per step.
-//  A  the dynamics (attitude, rates, velocity, position) over all T steps,
-//     every state into the state table, as a three-wave pipeline handing on
-//     blocks of ck steps through LDS progress counters: body rates and
-//     attitude increments (wave 2), attitude (wave 0), velocity and position
-//     (wave 1), each serial recurrence one lane per vector component; these
-//     waves evaluate neither the gate counter nor any cost, so each serial
-//     chain is one part of the dynamics, and no workgroup barrier falls
-//     inside the pass.  Behind the published positions, wave 3 runs every
-//     (step, gate) pair's crossing test and the running cost of each
-//     post-step state x_{t+1} without the winner test, and wave 2 (after its
-//     rates) the running cost's dJ/dp (both read the position only).
+//  A  the dynamics over all T steps, every state into the state table, as
+//     block-wide scans (one lane per step): the body rates and the velocity
+//     are affine recurrences, the position a prefix sum and the attitude a
+//     product of unit quaternions, all associative in exact arithmetic, so
+//     the states are quad_step's to rounding.  Then, one lane per (step,
+//     gate) pair, the gate-crossing tests, and one lane per step the running
+//     cost of x_{t+1} without the winner test and its dJ/dp (both read the
+//     position only).
 //  G  wave 0 runs the gate counter by searching the crossing bits for the
 //     next gate's crossing (one ballot per 64 steps) and every state slot
 //     takes the gate and laps after the crossings before it.
@@ -892,8 +889,9 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
 //     chains weighted by lam_b (the products are linear in lam), then the
 //     projected step of u_t; steps after t_end have dJ/du = 0 (u_t is only
 //     clamped).
-// The order of the additions in C-F differs from the one-lane sweep, so J
-// and the gradient agree with cost_grad_lane to rounding, not bit for bit.
+// The forward states are reassociated (A) and the additions of C-F ordered
+// differently from the one-lane sweep, so J and the gradient agree with
+// cost_grad_lane to rounding, not bit for bit.
 constexpr int REF_BLOCK = 256;
 constexpr int REF_SEGS = 16; // 14 * REF_SEGS <= REF_BLOCK
 constexpr int REF_COMP = 14; // affine part + 13 state directions
@@ -961,35 +959,103 @@ __device__ __forceinline__ void get_dyn(const float4* xs, int t, QState& s)
     s.w[2] = xs[t * 4 + 3].x;
 }
 
-// Lane i of each quad of lanes takes x from lane (CTRL >> 2i) & 3 of its quad
-// (a DPP quad permute).
-template <int CTRL>
-__device__ __forceinline__ float quad_perm(float x)
+// Block-wide inclusive scans for the refinement's forward pass: lane t of
+// the REF_BLOCK-lane workgroup holds element t (identity past the last
+// element), and after the scan it holds e_0 o e_1 o ... o e_t for an
+// associative op(earlier, later).  Within a wave: row shifts by 1, 2, 4, 8
+// and the row broadcasts of lanes 15 and 31 (DPP, no LDS); across waves: each
+// wave's total through LDS.  A lane whose shifted source lies outside its row
+// receives the identity, and op(identity, x) = x.
+template <int CTRL, int ROWS, int N>
+__device__ __forceinline__ void dpp_shift(const float (&x)[N], const float (&id)[N], float (&out)[N])
 {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, true));
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        out[i] = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(id[i]), __float_as_int(x[i]), CTRL, ROWS, 0xf, false));
 }
 
-// Bound on the polls of one wait of the forward pipeline.  Every producer
-// publishes unconditionally, so a wait ends long before it; reaching it
-// anyway marks the refinement failed (the plain update is kept) instead of
-// spinning on.
-constexpr int REF_SPIN_LIMIT = 1 << 22;
-
-// Waits until the progress counter *p reaches need (workgroup-scope acquire:
-// the LDS writes published with it are visible afterwards).  Returns false
-// once the poll bound is hit; a failed wait is not retried.
-__device__ __forceinline__ bool wait_progress(int* p, int need, bool ok)
+template <int N, class Op>
+__device__ __forceinline__ void block_scan(float (&x)[N], const float (&id)[N], Op op, float* s_tot)
 {
-    int polls = 0;
-    while (ok && __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need)
-        ok = ++polls < REF_SPIN_LIMIT;
-    return ok;
+    float y[N];
+    dpp_shift<0x111, 0xf>(x, id, y); // row_shr:1
+    op(y, x);
+    dpp_shift<0x112, 0xf>(x, id, y); // row_shr:2
+    op(y, x);
+    dpp_shift<0x114, 0xf>(x, id, y); // row_shr:4
+    op(y, x);
+    dpp_shift<0x118, 0xf>(x, id, y); // row_shr:8
+    op(y, x);
+    dpp_shift<0x142, 0xa>(x, id, y); // row_bcast:15 into rows 1 and 3
+    op(y, x);
+    dpp_shift<0x143, 0xc>(x, id, y); // row_bcast:31 into rows 2 and 3
+    op(y, x);
+    const int wave = threadIdx.x / WAVE;
+    if ((threadIdx.x & (WAVE - 1)) == WAVE - 1)
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            s_tot[wave * N + i] = x[i];
+    __syncthreads();
+    if (wave > 0)
+    {
+        float pre[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            pre[i] = s_tot[i];
+        for (int w = 1; w < wave; ++w)
+        {
+            float t[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                t[i] = s_tot[w * N + i];
+            op(pre, t);
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                pre[i] = t[i];
+        }
+        op(pre, x);
+    }
+    __syncthreads(); // s_tot free for the next scan
 }
 
-__device__ __forceinline__ void publish_progress(int* p, int done)
+// op(e, l): l <- the affine map "l after e", maps x -> A x + B_i (one A for
+// the three axes): (A_l A_e, A_l B_e + B_l).
+struct AffineOp
 {
-    __hip_atomic_store(p, done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
+    __device__ void operator()(const float (&e)[4], float (&l)[4]) const
+    {
+        l[1] = fma_(l[0], e[1], l[1]);
+        l[2] = fma_(l[0], e[2], l[2]);
+        l[3] = fma_(l[0], e[3], l[3]);
+        l[0] = l[0] * e[0];
+    }
+};
+
+// op(e, l): l <- e (x) l (quaternion product, earlier factor on the left)
+struct QuatOp
+{
+    __device__ void operator()(const float (&e)[4], float (&l)[4]) const
+    {
+        const float w = e[0] * l[0] - e[1] * l[1] - e[2] * l[2] - e[3] * l[3];
+        const float x = e[0] * l[1] + e[1] * l[0] + e[2] * l[3] - e[3] * l[2];
+        const float y = e[0] * l[2] - e[1] * l[3] + e[2] * l[0] + e[3] * l[1];
+        const float z = e[0] * l[3] + e[1] * l[2] - e[2] * l[1] + e[3] * l[0];
+        l[0] = w;
+        l[1] = x;
+        l[2] = y;
+        l[3] = z;
+    }
+};
+
+struct AddOp
+{
+    __device__ void operator()(const float (&e)[3], float (&l)[3]) const
+    {
+        l[0] = e[0] + l[0];
+        l[1] = e[1] + l[1];
+        l[2] = e[2] + l[2];
+    }
+};
 
 __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict__ insts, const dr_track* __restrict__ tracks,
                                                       const dr_drone_state* __restrict__ drones, float* __restrict__ nominal,
@@ -1006,17 +1072,14 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     __shared__ float4 s_op[MAX_OPP], s_ov[MAX_OPP];
     __shared__ float4 s_uc[DR_MAX_T + 1];  // the clamped controls the forward pass integrates (+1: read-ahead slot)
     __shared__ uint32_t s_cross[DR_MAX_T]; // bit g: the move x_t -> x_{t+1} crosses gate g
-    __shared__ float4 s_coef[DR_MAX_T * 4]; // per step, per attitude lane: its three product coefficients and 1/|(1, h w)|
-    __shared__ float2 s_thr[DR_MAX_T * 4];  // per step, per axis lane: thrust acceleration and thrust direction component
+    __shared__ float s_tot[(REF_BLOCK / WAVE) * 4]; // per-wave totals of the forward pass's scans
     __shared__ int s_xstep[DR_MAX_T], s_xgate[DR_MAX_T], s_xlaps[DR_MAX_T]; // gate crossings: step, gate, laps after
     __shared__ int s_nx;
     __shared__ float4 s_ubest[DR_MAX_T]; // the lowest-cost sequence evaluated so far
     __shared__ float s_jbest;
     __shared__ int s_tend, s_better;
-    __shared__ int s_prog[3]; // forward pipeline: steps whose increments (0) / start attitudes (1) / end positions (2) are in LDS
     __shared__ float s_cb[DR_MAX_T]; // running cost of x_{t+1} without the winner test
     __shared__ int s_tb[DR_MAX_T];   // its terminal code (collision or none)
-    __shared__ int s_fail;    // a pipeline wait hit its bound: keep the plain update
 
     const InstDev& I = insts[blockIdx.x];
     const int T = I.T, n_opp = I.n_opp, tid = threadIdx.x;
@@ -1048,11 +1111,8 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             w = w * DECAY;
         }
     }
-    if (tid == 0)
-        s_fail = 0;
     const QState x0 = state_from_abi(drones[I.drone]);
     float* const s_xf = reinterpret_cast<float*>(s_x); // the state table as floats: slot t at 16 t (put_dyn's layout)
-    const float* const s_ucf = reinterpret_cast<const float*>(s_uc);
     __syncthreads();
 
     for (int it = 0;; ++it)
@@ -1067,222 +1127,149 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             s_uc[t] = make_float4(out[0], out[1], out[2], out[3]);
             s_cross[t] = 0u;
         }
-        if (tid == 0)
-        {
-            s_prog[0] = 0;
-            s_prog[1] = 0;
-            s_prog[2] = 0;
-        }
         __syncthreads();
-        // A: forward pass of the dynamics, every state into the state table,
-        // by three waves handing on blocks of ck steps through LDS progress
-        // counters (no workgroup barrier until the pass ends).  Each serial
-        // recurrence runs one lane per vector component and every per-step
-        // quantity that only reads an earlier stage's output is formed one
-        // lane per step; the operations (and so the results) are those of
-        // quad_step:
-        //  wave 2: body rates w (lanes 0..2), then each step's attitude
-        //          increment (h w, 1/|(1, h w)|) as the coefficients of the
-        //          quaternion product, one lane per step;
-        //  wave 0: attitude q (lanes 0..3, component i in lane i; the product
-        //          takes the other components with quad permutes);
-        //  wave 1: thrust acceleration and thrust direction of each step, one
-        //          lane per step, then velocity and position (lanes 0..2).
-        if (tid >= 2 * WAVE && tid < 3 * WAVE)
+        // A: forward pass of the dynamics, every state into the state table, as
+        // block-wide scans over the steps (lane t holds step t).  Each
+        // recurrence is associative in exact arithmetic: the body rates and the
+        // velocity are affine (w' = (1 - k) w + k u, v' = (1 - dt drag) v + dt (a_T
+        // R(q) e3 - g e3)), the position a prefix sum (p' = p + dt v) and the
+        // attitude a product of unit quaternions (q' = q (x) rs (1, h w): the
+        // renormalisation factor (3 - |q|^2)/2 of quad_step is 1 on the unit
+        // attitude).  So the states are quad_step's to rounding, and the pass is
+        // four scans instead of a T-step serial chain.
         {
-            const int l = tid - 2 * WAVE;
-            float w = l < 3 ? x0.w[l] : 0.0f;
-            for (int t0 = 0; t0 < T; t0 += ck)
+            const int t = tid;
+            const bool in = t < T;
+            float* xs = s_xf + (t + 1) * 16; // slot t + 1
             {
-                const int t1 = min(t0 + ck, T);
-                if (l < 3)
+                const float id[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+                float x[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+                if (in)
                 {
-                    const auto rts = [&](int t, float u) {
-                        s_xf[t * 16 + 10 + l] = w;
-                        w = fma_(C.rate_k, u - w, w); // rates_step
-                    };
-                    float ua = s_ucf[t0 * 4 + 1 + l];
-                    int t = t0;
-                    for (; t + 1 < t1; t += 2)
-                    {
-                        const float ub = s_ucf[(t + 1) * 4 + 1 + l];
-                        __builtin_amdgcn_sched_barrier(0); // keep the load a full step ahead of its use
-                        rts(t, ua);
-                        ua = s_ucf[(t + 2) * 4 + 1 + l]; // slot T is the read-ahead slot
-                        __builtin_amdgcn_sched_barrier(0);
-                        rts(t + 1, ub);
-                    }
-                    if (t < t1)
-                        rts(t, ua);
+                    const float4 u = s_uc[t];
+                    x[0] = 1.0f - C.rate_k;
+                    x[1] = C.rate_k * u.y;
+                    x[2] = C.rate_k * u.z;
+                    x[3] = C.rate_k * u.w;
                 }
-                for (int t = t0 + l; t < t1; t += WAVE)
+                block_scan(x, id, AffineOp(), s_tot);
+                if (in)
+                    for (int i = 0; i < 3; ++i)
+                        xs[10 + i] = fma_(x[0], x0.w[i], x[1 + i]);
+                if (t == 0)
+                    for (int i = 0; i < 3; ++i)
+                        s_xf[10 + i] = x0.w[i];
+            }
+            __syncthreads();
+            {
+                const float id[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+                float x[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+                if (in)
                 {
                     const float wt[3] = {s_xf[t * 16 + 10], s_xf[t * 16 + 11], s_xf[t * 16 + 12]};
                     const float4 inc = attitude_increment(C, wt);
-                    // lane i of the product: n_i = fma(o, q_o, fma(m, q_m, fma(c, q_c, q_i)))
-                    s_coef[t * 4 + 0] = make_float4(-inc.z, -inc.y, -inc.x, inc.w);
-                    s_coef[t * 4 + 1] = make_float4(-inc.y, inc.z, inc.x, inc.w);
-                    s_coef[t * 4 + 2] = make_float4(-inc.z, inc.x, inc.y, inc.w);
-                    s_coef[t * 4 + 3] = make_float4(-inc.x, inc.y, inc.z, inc.w);
+                    x[0] = inc.w;
+                    x[1] = inc.w * inc.x;
+                    x[2] = inc.w * inc.y;
+                    x[3] = inc.w * inc.z;
                 }
-                if (l == 0)
-                    publish_progress(&s_prog[0], t1);
-            }
-            if (l < 3)
-                s_xf[T * 16 + 10 + l] = w;
-        }
-        if (tid < 4)
-        {
-            // attitude_apply with component i in lane i: the three product
-            // terms read q from the lanes the quad permutes name (inner
-            // (3,3,1,2), middle (2,2,3,1), outer (1,0,0,0)); |q|^2 is summed
-            // from broadcasts in attitude_apply's order
-            float q = x0.q[tid];
-            bool ok = true;
-            for (int t0 = 0; t0 < T; t0 += ck)
-            {
-                const int t1 = min(t0 + ck, T);
-                ok = wait_progress(&s_prog[0], t1, ok);
-                const auto att = [&](int t, const float4 cf) {
-                    s_xf[t * 16 + 3 + tid] = q;
-                    const float n = fma_(quad_perm<0x01>(q), cf.z, fma_(quad_perm<0x7a>(q), cf.y, fma_(quad_perm<0x9f>(q), cf.x, q)));
-                    const float q0 = quad_perm<0x00>(q), q1 = quad_perm<0x55>(q), q2 = quad_perm<0xaa>(q), q3 = quad_perm<0xff>(q);
-                    const float n2 = fma_(q0, q0, fma_(q1, q1, fma_(q2, q2, q3 * q3)));
-                    q = n * (cf.w * fma_(-0.5f, n2, 1.5f));
-                };
-                // two steps per trip, each step's coefficients loaded one step ahead
-                float4 ca = s_coef[t0 * 4 + tid];
-                int t = t0;
-                for (; t + 1 < t1; t += 2)
+                block_scan(x, id, QuatOp(), s_tot);
+                if (in)
                 {
-                    const float4 cb = s_coef[(t + 1) * 4 + tid];
-                    __builtin_amdgcn_sched_barrier(0); // keep the load a full step ahead of its use
-                    att(t, ca);
-                    ca = s_coef[min(t + 2, t1 - 1) * 4 + tid];
-                    __builtin_amdgcn_sched_barrier(0);
-                    att(t + 1, cb);
+                    float q[4] = {x0.q[0], x0.q[1], x0.q[2], x0.q[3]};
+                    QuatOp()(q, x);
+                    for (int i = 0; i < 4; ++i)
+                        xs[3 + i] = x[i];
                 }
-                if (t < t1)
-                    att(t, ca);
-                if (tid == 0)
-                    publish_progress(&s_prog[1], t1);
+                if (t == 0)
+                    for (int i = 0; i < 4; ++i)
+                        s_xf[3 + i] = x0.q[i];
             }
-            s_xf[T * 16 + 3 + tid] = q;
-            if (!ok)
-                s_fail = 1;
-        }
-        if (tid >= WAVE && tid < 2 * WAVE)
-        {
-            const int l = tid - WAVE;
-            float p = l < 3 ? x0.p[l] : 0.0f, v = l < 3 ? x0.v[l] : 0.0f;
-            const float g = l == 2 ? C.gravity : 0.0f; // x - 0 is x: only the z lane's subtraction changes anything
-            if (l < 3)
+            __syncthreads();
             {
-                s_xf[l] = p;
-                s_xf[7 + l] = v;
-            }
-            bool ok = true;
-            for (int t0 = 0; t0 < T; t0 += ck)
-            {
-                const int t1 = min(t0 + ck, T);
-                ok = wait_progress(&s_prog[1], t1, ok);
-                for (int t = t0 + l; t < t1; t += WAVE)
+                const float id[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+                float x[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+                if (in)
                 {
                     const float qt[4] = {s_xf[t * 16 + 3], s_xf[t * 16 + 4], s_xf[t * 16 + 5], s_xf[t * 16 + 6]};
                     float b3[3];
                     thrust_dir(qt, b3);
                     const float at = s_uc[t].x * C.inv_mass;
-                    s_thr[t * 4 + 0] = make_float2(at, b3[0]);
-                    s_thr[t * 4 + 1] = make_float2(at, b3[1]);
-                    s_thr[t * 4 + 2] = make_float2(at, b3[2]);
+                    x[0] = fma_(-C.dt, C.drag, 1.0f);
+                    x[1] = C.dt * (at * b3[0]);
+                    x[2] = C.dt * (at * b3[1]);
+                    x[3] = C.dt * fma_(at, b3[2], -C.gravity);
                 }
-                if (l < 3)
-                {
-                    const auto trn = [&](int t, const float2 ab) {
-                        const float a = fma_(ab.x, ab.y, -(C.drag * v)) - g;
-                        p = fma_(C.dt, v, p);
-                        v = fma_(C.dt, a, v);
-                        s_xf[(t + 1) * 16 + l] = p;
-                        s_xf[(t + 1) * 16 + 7 + l] = v;
-                    };
-                    float2 aa = s_thr[t0 * 4 + l];
-                    int t = t0;
-                    for (; t + 1 < t1; t += 2)
-                    {
-                        const float2 ab = s_thr[(t + 1) * 4 + l];
-                        __builtin_amdgcn_sched_barrier(0); // keep the load a full step ahead of its use
-                        trn(t, aa);
-                        aa = s_thr[min(t + 2, t1 - 1) * 4 + l];
-                        __builtin_amdgcn_sched_barrier(0);
-                        trn(t + 1, ab);
-                    }
-                    if (t < t1)
-                        trn(t, aa);
-                }
-                if (l == 0)
-                    publish_progress(&s_prog[2], t1);
+                block_scan(x, id, AffineOp(), s_tot);
+                if (in)
+                    for (int i = 0; i < 3; ++i)
+                        xs[7 + i] = fma_(x[0], x0.v[i], x[1 + i]);
+                if (t == 0)
+                    for (int i = 0; i < 3; ++i)
+                        s_xf[7 + i] = x0.v[i];
             }
-            if (!ok)
-                s_fail = 1;
-        }
-        // Behind the translation wave, on published ranges of at least 32
-        // steps (up to 64; a lane does the same work for one step as for a
-        // whole range): wave 3 tests every (step, gate) pair's crossing and
-        // forms the running cost of x_{t+1} without the winner test; wave 2,
-        // once its rates are done, forms the running cost's dJ/dp of x_{t+1}
-        // (not needed on the last pass).  Both read positions only.
-        if (tid >= 3 * WAVE || (tid >= 2 * WAVE && it < n_steps))
-        {
-            const int l = tid & (WAVE - 1);
-            const bool crossings = tid >= 3 * WAVE;
-            const int ng = s_track.n_gates;
-            bool ok = true;
-            int done = 0;
-            while (done < T)
+            __syncthreads();
             {
-                ok = wait_progress(&s_prog[2], min(done + 32, T), ok);
-                const int avail = ok ? __hip_atomic_load(&s_prog[2], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) : T;
-                const int b = min(avail, done + WAVE);
-                if (crossings)
-                    for (int i = l; i < (b - done) * ng; i += WAVE)
-                    {
-                        const int t = done + i / ng, gi = i % ng;
-                        const float p0[3] = {s_xf[t * 16], s_xf[t * 16 + 1], s_xf[t * 16 + 2]};
-                        const float p1[3] = {s_xf[(t + 1) * 16], s_xf[(t + 1) * 16 + 1], s_xf[(t + 1) * 16 + 2]};
-                        if (gate_crossed(s_track.gates[gi], margin, p0, p1))
-                            atomicOr(&s_cross[t], 1u << gi);
-                    }
-                const int t = done + l;
-                if (t < b)
-                {
-                    QState x;
-                    x.p[0] = s_xf[(t + 1) * 16];
-                    x.p[1] = s_xf[(t + 1) * 16 + 1];
-                    x.p[2] = s_xf[(t + 1) * 16 + 2];
-                    const float tau = C.dt * (float)(t + 1);
-                    const auto opp_at = [&](int j) {
-                        const float4 o = s_op[j], ov = s_ov[j];
-                        return make_float4(fma_(ov.x, tau, o.x), fma_(ov.y, tau, o.y), fma_(ov.z, tau, o.z), 0.0f);
-                    };
-                    if (crossings)
-                    {
-                        int term;
-                        s_cb[t] = stage_cost_body_at(C, Q, KC, s_track, x, n_opp, opp_at, &term);
-                        s_tb[t] = term;
-                    }
-                    else
-                    {
-                        float gp[3] = {0.0f, 0.0f, 0.0f};
-                        stage_cost_grad_at(C, Q, KC, s_track, x, n_opp, opp_at, s_w[t], gp);
-                        s_gp[t].x = gp[0];
-                        s_gp[t].y = gp[1];
-                        s_gp[t].z = gp[2];
-                    }
-                }
-                done = b;
+                const float id[3] = {0.0f, 0.0f, 0.0f};
+                float x[3] = {0.0f, 0.0f, 0.0f};
+                if (in)
+                    for (int i = 0; i < 3; ++i)
+                        x[i] = C.dt * s_xf[t * 16 + 7 + i];
+                block_scan(x, id, AddOp(), s_tot);
+                if (in)
+                    for (int i = 0; i < 3; ++i)
+                        xs[i] = x0.p[i] + x[i];
+                if (t == 0)
+                    for (int i = 0; i < 3; ++i)
+                        s_xf[i] = x0.p[i];
             }
-            if (!ok)
-                s_fail = 1;
+            __syncthreads();
+        }
+        // Per step, all lanes: every (step, gate) pair's crossing test of the
+        // move x_t -> x_{t+1}; the running cost of x_{t+1} without the winner
+        // test, and its dJ/dp (not needed on the last pass).  Both read the
+        // position only.
+        {
+            const int ng = s_track.n_gates;
+            for (int i = tid; i < T * ng; i += REF_BLOCK)
+            {
+                const int t = i / ng, gi = i - t * ng;
+                const float p0[3] = {s_xf[t * 16], s_xf[t * 16 + 1], s_xf[t * 16 + 2]};
+                const float p1[3] = {s_xf[(t + 1) * 16], s_xf[(t + 1) * 16 + 1], s_xf[(t + 1) * 16 + 2]};
+                if (gate_crossed(s_track.gates[gi], margin, p0, p1))
+                    atomicOr(&s_cross[t], 1u << gi);
+            }
+            const int Tw = (T + WAVE - 1) & ~(WAVE - 1); // cost and gradient lanes in separate waves
+            for (int i = tid; i < 2 * Tw; i += REF_BLOCK)
+            {
+                const bool grad = i >= Tw;
+                const int t = grad ? i - Tw : i;
+                if (t >= T || (grad && it == n_steps))
+                    continue;
+                QState x;
+                x.p[0] = s_xf[(t + 1) * 16];
+                x.p[1] = s_xf[(t + 1) * 16 + 1];
+                x.p[2] = s_xf[(t + 1) * 16 + 2];
+                const float tau = C.dt * (float)(t + 1);
+                const auto opp_at = [&](int j) {
+                    const float4 o = s_op[j], ov = s_ov[j];
+                    return make_float4(fma_(ov.x, tau, o.x), fma_(ov.y, tau, o.y), fma_(ov.z, tau, o.z), 0.0f);
+                };
+                if (!grad)
+                {
+                    int term;
+                    s_cb[t] = stage_cost_body_at(C, Q, KC, s_track, x, n_opp, opp_at, &term);
+                    s_tb[t] = term;
+                }
+                else
+                {
+                    float gp[3] = {0.0f, 0.0f, 0.0f};
+                    stage_cost_grad_at(C, Q, KC, s_track, x, n_opp, opp_at, s_w[t], gp);
+                    s_gp[t].x = gp[0];
+                    s_gp[t].y = gp[1];
+                    s_gp[t].z = gp[2];
+                }
+            }
         }
         __syncthreads();
         if (tid == 0)
@@ -1500,7 +1487,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         __syncthreads();
     }
     __syncthreads();
-    for (int t = tid; t < T && !s_fail; t += REF_BLOCK)
+    for (int t = tid; t < T; t += REF_BLOCK)
     {
         const float4 u4 = s_ubest[t];
         float* u = nominal + I.nom_off + t * 4;

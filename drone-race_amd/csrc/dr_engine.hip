@@ -40,8 +40,9 @@ hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstD
 hipError_t launch_costgrad(int lanes, hipStream_t st, const InstDev* insts, int inst, int T, int n_ck, int ck,
                            const dr_track* tracks, const dr_drone_state* drones, const float* U, int n_seq, float* J, float* G,
                            const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, int opp_stride);
-hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, const dr_track* tracks, const dr_drone_state* drones,
-                         float* nominal, int n_steps, float step, int ck, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC);
+hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
+                         const dr_track* tracks, const dr_drone_state* drones, float* nominal, int n_steps, float step,
+                         const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC);
 
 } // namespace dr
 
@@ -305,12 +306,11 @@ int dr_engine::enqueue_step(bool advance, int timed_index)
         HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
     if (refine_steps > 0)
     {
-        HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, 0, d_drones.p,
-                             d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin, 1));
+        // the MPPI update runs inside the refinement kernel
         if (ev)
             HIPCHK(hipEventRecord(ev[3], stream));
-        HIPCHK(launch_refine(n, stream, d_inst.p, d_tracks.p, d_drones.p, d_nominal.p, refine_steps, refine_step, refine_ck, C, Q,
-                             cost));
+        HIPCHK(launch_refine(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_tracks.p, d_drones.p, d_nominal.p, refine_steps,
+                             refine_step, C, Q, cost));
         if (ev)
             HIPCHK(hipEventRecord(ev[4], stream));
         HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, advance ? 1 : 0, d_drones.p,

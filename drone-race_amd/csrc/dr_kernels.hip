@@ -1057,9 +1057,11 @@ struct AddOp
     }
 };
 
-__global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict__ insts, const dr_track* __restrict__ tracks,
-                                                      const dr_drone_state* __restrict__ drones, float* __restrict__ nominal,
-                                                      int n_steps, float step, int ck, QuadConst C, CostConst Q, dr_cost_cfg KC)
+__global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
+                                                      const double* __restrict__ part, const double* __restrict__ den,
+                                                      const dr_track* __restrict__ tracks, const dr_drone_state* __restrict__ drones,
+                                                      float* __restrict__ nominal, int n_steps, float step, QuadConst C, CostConst Q,
+                                                      dr_cost_cfg KC)
 {
     __shared__ dr_track s_track;
     __shared__ float4 s_x[(DR_MAX_T + 1) * 4]; // state table, put_state layout
@@ -1096,10 +1098,41 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         s_op[tid] = make_float4(o.p[0], o.p[1], o.p[2], 0.0f);
         s_ov[tid] = make_float4(o.v[0], o.v[1], o.v[2], 0.0f);
     }
-    for (int t = tid; t < T; t += REF_BLOCK)
     {
-        const float* u = nominal + I.nom_off + t * 4;
-        s_u[t] = make_float4(u[0], u[1], u[2], u[3]);
+        // the MPPI update (k_update's mode 1): nominal + num/den over the
+        // chunks in fixed order, box clamp; nu and the temperature rule
+        const double dn = sum_in_order(den + I.den_off, I.n_chunks, 1);
+        float* su = reinterpret_cast<float*>(s_u);
+        for (int i = tid; i < T * 4; i += REF_BLOCK)
+        {
+            const double num = sum_in_order(part + I.part_off + i, I.n_chunks, (long long)T * 4);
+            float u = nominal[I.nom_off + i];
+            if (dn > 1e-30)
+                u = u + (float)(num / dn);
+            su[i] = u;
+        }
+        __syncthreads();
+        for (int t = tid; t < T; t += REF_BLOCK)
+        {
+            const float4 u4 = s_u[t];
+            const float in[4] = {u4.x, u4.y, u4.z, u4.w};
+            float out[4];
+            clamp_control(C, in, out);
+            s_u[t] = make_float4(out[0], out[1], out[2], out[3]);
+        }
+        if (tid == 0)
+        {
+            InstDyn& Y = dyn[blockIdx.x];
+            const float nu = (float)dn;
+            Y.nu = nu;
+            if (I.adapt)
+            {
+                if (nu > 0.001f * (float)I.K)
+                    Y.lambda = Y.lambda * 0.9f;
+                else if (nu < 0.0005f * (float)I.K)
+                    Y.lambda = Y.lambda * 1.2f;
+            }
+        }
     }
     if (tid == 0)
     {
@@ -1882,10 +1915,11 @@ hipError_t launch_costgrad(int lanes, hipStream_t st, const InstDev* insts, int 
     return hipGetLastError();
 }
 
-hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, const dr_track* tracks, const dr_drone_state* drones,
-                         float* nominal, int n_steps, float step, int ck, const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC)
+hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
+                         const dr_track* tracks, const dr_drone_state* drones, float* nominal, int n_steps, float step,
+                         const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC)
 {
-    k_refine<<<n_inst, REF_BLOCK, 0, st>>>(insts, tracks, drones, nominal, n_steps, step, ck, C, Q, KC);
+    k_refine<<<n_inst, REF_BLOCK, 0, st>>>(insts, dyn, part, den, tracks, drones, nominal, n_steps, step, C, Q, KC);
     return hipGetLastError();
 }
 

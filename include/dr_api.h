@@ -225,8 +225,9 @@ int dr_engine_set_timing(dr_engine* e, int on);
 /* Device time of the last timed dr_engine_step / step_async batch, per
  * kernel kind, from HIP events on the engine stream (ms, summed over steps),
  * the first n (1..5) of: rollout, weighted average, update (with the
- * closed-loop advance when it ran), adjoint refinement (0 without it),
- * whole step. */
+ * closed-loop advance when it ran), adjoint refinement (0 without it; with
+ * it, the MPPI update runs inside the refinement kernel and is counted
+ * there), whole step. */
 int dr_engine_last_timing(dr_engine* e, float* ms, int n);
 
 /* Adjoint of the noise-free rollout cost.  For n_seq control sequences
@@ -256,9 +257,10 @@ int dr_engine_cost_grad_noalloc(dr_engine* e, int instance, const float* U, int 
                                 int ckpt_every);
 /* Adjoint refinement inside every control step: after the MPPI update,
  * n_grad_steps projected gradient steps U <- clamp(U - step * dJ/dU) on each
- * instance's nominal (the forward pass hands its states on between its
- * waves in blocks of ckpt_every steps; the result does not depend on it),
- * the J of every iterate evaluated and the lowest kept (the MPPI update itself on ties),
+ * instance's nominal (ckpt_every in 1..256 is accepted for the interface of
+ * the checkpointed adjoint; the refinement keeps every state of the
+ * horizon on chip, so it changes neither the result nor the work), the J of
+ * every iterate evaluated and the lowest kept (the MPPI update itself on ties),
  * then action and shift.  n_grad_steps = 0 (default) is plain MPPI.  No
  * counterpart in the reference (its controllers have no gradient path). */
 int dr_engine_set_refine(dr_engine* e, int n_grad_steps, float step, int ckpt_every);

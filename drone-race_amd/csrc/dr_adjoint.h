@@ -182,6 +182,89 @@ DR_HD void quad_step_vjp(const QuadConst& C, const QState& x, const float u[4], 
     gx = g;
 }
 
+// The same linearisation as quad_step_vjp, as the blocks of the step's
+// Jacobian that depend on the state, so that many adjoint vectors can be
+// carried through one step with no recomputation of the forward pieces.
+// With lam = (P, V, Q, W) = dJ/d(p, v, q, w) after the step, the adjoint
+// before it and dJ/du are
+//   P' = P,  V' = (1 - drag dt) V + dt P,
+//   Q'_i = sum_j Mq[i][j] Q_j + sum_a Mv[i][a] V_a,
+//   W'_k = (1 - k) W_k + sum_j Mw[k][j] Q_j,
+//   dJ/du = (bu . V, k W),
+// where, with n = q (x) (1, h w), c = (3 - |q|^2)/2, rs = 1/|(1, h w)|,
+// sc = rs c (the pieces quad_step_vjp recomputes):
+//   Mq[i][j] = sc B[i][j] - rs q_i n_j      (B: the product's matrix in h w)
+//   Mw[k][j] = h (sc Cq[k][j] - h_k rs^3 c n_j)   (Cq: the product's matrix in q)
+//   Mv[i][a] = a_T dt dR(q)e3_a/dq_i,  bu = dt R(q)e3 / m.
+// Layout of J: Mq row-major [0, 16), Mw [16, 28), Mv [28, 40), bu [40, 43).
+constexpr int QJAC = 43;
+
+DR_HD void quad_step_jacobian(const QuadConst& C, const QState& x, const float u[4], float J[QJAC])
+{
+    const float qw = x.q[0], qx = x.q[1], qy = x.q[2], qz = x.q[3];
+    const float h = C.half_dt;
+    const float hx = h * x.w[0], hy = h * x.w[1], hz = h * x.w[2];
+    const float rs = rsqrt_nr(fma_(hx, hx, fma_(hy, hy, fma_(hz, hz, 1.0f))));
+    const float n[4] = {fma_(-qx, hx, fma_(-qy, hy, fma_(-qz, hz, qw))), fma_(qw, hx, fma_(qy, hz, fma_(-qz, hy, qx))),
+                        fma_(qw, hy, fma_(qz, hx, fma_(-qx, hz, qy))), fma_(qw, hz, fma_(qx, hy, fma_(-qy, hx, qz)))};
+    const float q2 = fma_(qw, qw, fma_(qx, qx, fma_(qy, qy, qz * qz)));
+    const float c = fma_(-0.5f, q2, 1.5f);
+    const float sc = rs * c;
+    const float B[4][4] = {{1.0f, hx, hy, hz}, {-hx, 1.0f, -hz, hy}, {-hy, hz, 1.0f, -hx}, {-hz, -hy, hx, 1.0f}};
+    const float Cq[3][4] = {{-qx, qw, qz, -qy}, {-qy, -qz, qw, qx}, {-qz, qy, -qx, qw}};
+    const float q[4] = {qw, qx, qy, qz}, hv[3] = {hx, hy, hz};
+    const float r3c = (rs * rs * rs) * c;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            J[i * 4 + j] = fma_(sc, B[i][j], -(rs * q[i]) * n[j]);
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 4; ++j)
+            J[16 + k * 4 + j] = h * fma_(sc, Cq[k][j], -(hv[k] * r3c) * n[j]);
+    const float atd = (u[0] * C.inv_mass) * C.dt;
+    const float D[4][3] = {{2.0f * qy, -2.0f * qx, 0.0f}, {2.0f * qz, -2.0f * qw, -4.0f * qx},
+                           {2.0f * qw, 2.0f * qz, -4.0f * qy}, {2.0f * qx, 2.0f * qy, 0.0f}};
+    for (int i = 0; i < 4; ++i)
+        for (int a = 0; a < 3; ++a)
+            J[28 + i * 3 + a] = atd * D[i][a];
+    float b3[3];
+    thrust_dir(x.q, b3);
+    const float dm = C.dt * C.inv_mass;
+    J[40] = dm * b3[0];
+    J[41] = dm * b3[1];
+    J[42] = dm * b3[2];
+}
+
+// One step of the adjoint through J (quad_step_jacobian): lam (13 floats:
+// p, q, v, w, as grad_to_flat) after the step -> before it; gu = dJ/du.
+DR_HD void quad_step_jvp_t(const QuadConst& C, const float* J, float lam[13], float gu[4])
+{
+    const float P[3] = {lam[0], lam[1], lam[2]};
+    const float Qv[4] = {lam[3], lam[4], lam[5], lam[6]};
+    const float V[3] = {lam[7], lam[8], lam[9]};
+    const float W[3] = {lam[10], lam[11], lam[12]};
+    gu[0] = fma_(J[40], V[0], fma_(J[41], V[1], J[42] * V[2]));
+    const float kk = C.rate_k, dv = fma_(-C.drag, C.dt, 1.0f);
+    for (int k = 0; k < 3; ++k)
+    {
+        gu[1 + k] = kk * W[k];
+        float a = (1.0f - kk) * W[k];
+        for (int j = 0; j < 4; ++j)
+            a = fma_(J[16 + k * 4 + j], Qv[j], a);
+        lam[10 + k] = a;
+    }
+    for (int i = 0; i < 4; ++i)
+    {
+        float a = 0.0f;
+        for (int j = 0; j < 4; ++j)
+            a = fma_(J[i * 4 + j], Qv[j], a);
+        for (int b = 0; b < 3; ++b)
+            a = fma_(J[28 + i * 3 + b], V[b], a);
+        lam[3 + i] = a;
+    }
+    for (int b = 0; b < 3; ++b)
+        lam[7 + b] = fma_(C.dt, P[b], dv * V[b]);
+}
+
 DR_HD float clamp_grad(float x, float lo, float hi)
 {
     return (x >= lo && x <= hi) ? 1.0f : 0.0f;

@@ -895,69 +895,7 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
 constexpr int REF_BLOCK = 256;
 constexpr int REF_SEGS = 16; // 14 * REF_SEGS <= REF_BLOCK
 constexpr int REF_COMP = 14; // affine part + 13 state directions
-
-// lam as 13 floats: p[3], q[4], v[3], w[3]
-__device__ __forceinline__ void grad_to_flat(const QGrad& g, float f[13])
-{
-    f[0] = g.p[0]; f[1] = g.p[1]; f[2] = g.p[2];
-    f[3] = g.q[0]; f[4] = g.q[1]; f[5] = g.q[2]; f[6] = g.q[3];
-    f[7] = g.v[0]; f[8] = g.v[1]; f[9] = g.v[2];
-    f[10] = g.w[0]; f[11] = g.w[1]; f[12] = g.w[2];
-}
-
-__device__ __forceinline__ QGrad grad_from_flat(const float f[13])
-{
-    QGrad g;
-    g.p[0] = f[0]; g.p[1] = f[1]; g.p[2] = f[2];
-    g.q[0] = f[3]; g.q[1] = f[4]; g.q[2] = f[5]; g.q[3] = f[6];
-    g.v[0] = f[7]; g.v[1] = f[8]; g.v[2] = f[9];
-    g.w[0] = f[10]; g.w[1] = f[11]; g.w[2] = f[12];
-    return g;
-}
-
-// State x_t, control u_t and the running-cost gradient of x_t (row t - 1 of
-// the gradient table; row 0 at t = 0, where it is not used) of one step.
-struct RefRow
-{
-    float4 x[4], u, gp;
-    __device__ static RefRow load(const float4* xs, const float4* us, const float4* gps, int t)
-    {
-        RefRow r;
-        for (int k = 0; k < 4; ++k)
-            r.x[k] = xs[t * 4 + k];
-        r.u = us[t];
-        r.gp = gps[t > 0 ? t - 1 : 0];
-        return r;
-    }
-    __device__ QState state() const
-    {
-        QState s;
-        s.p[0] = x[0].x; s.p[1] = x[0].y; s.p[2] = x[0].z; s.q[0] = x[0].w;
-        s.q[1] = x[1].x; s.q[2] = x[1].y; s.q[3] = x[1].z; s.v[0] = x[1].w;
-        s.v[1] = x[2].x; s.v[2] = x[2].y; s.w[0] = x[2].z; s.w[1] = x[2].w;
-        s.w[2] = x[3].x; s.gate = (int)f2u(x[3].y); s.laps = (int)f2u(x[3].z);
-        return s;
-    }
-};
-
-// The dynamic part (p, q, v, w) of state slot t of the state table; the
-// fourth word's gate and lap fields are left for the gate pass.
-__device__ __forceinline__ void put_dyn(float4* xs, int t, const QState& s)
-{
-    xs[t * 4 + 0] = make_float4(s.p[0], s.p[1], s.p[2], s.q[0]);
-    xs[t * 4 + 1] = make_float4(s.q[1], s.q[2], s.q[3], s.v[0]);
-    xs[t * 4 + 2] = make_float4(s.v[1], s.v[2], s.w[0], s.w[1]);
-    xs[t * 4 + 3].x = s.w[2];
-}
-
-__device__ __forceinline__ void get_dyn(const float4* xs, int t, QState& s)
-{
-    const float4 a = xs[t * 4 + 0], b = xs[t * 4 + 1], c = xs[t * 4 + 2];
-    s.p[0] = a.x; s.p[1] = a.y; s.p[2] = a.z; s.q[0] = a.w;
-    s.q[1] = b.x; s.q[2] = b.y; s.q[3] = b.z; s.v[0] = b.w;
-    s.v[1] = c.x; s.v[2] = c.y; s.w[0] = c.z; s.w[1] = c.w;
-    s.w[2] = xs[t * 4 + 3].x;
-}
+constexpr int REF_JAC4 = (QJAC + 3) / 4; // float4 slots per step of the Jacobian table
 
 // Block-wide inclusive scans for the refinement's forward pass: lane t of
 // the REF_BLOCK-lane workgroup holds element t (identity past the last
@@ -1071,6 +1009,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     __shared__ float s_psi[REF_SEGS * REF_COMP * 16];
     __shared__ float s_lam[REF_SEGS * 16];     // lam_b of each reverse segment
     __shared__ float4 s_gu[DR_MAX_T * REF_COMP]; // per step t and chain c of its segment: dJ/du_t along chain c
+    __shared__ float4 s_jac[DR_MAX_T * REF_JAC4]; // per step: the Jacobian blocks of quad_step_jacobian
     __shared__ float4 s_op[MAX_OPP], s_ov[MAX_OPP];
     __shared__ float4 s_uc[DR_MAX_T + 1];  // the clamped controls the forward pass integrates (+1: read-ahead slot)
     __shared__ uint32_t s_cross[DR_MAX_T]; // bit g: the move x_t -> x_{t+1} crosses gate g
@@ -1145,7 +1084,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         }
     }
     const QState x0 = state_from_abi(drones[I.drone]);
-    float* const s_xf = reinterpret_cast<float*>(s_x); // the state table as floats: slot t at 16 t (put_dyn's layout)
+    float* const s_xf = reinterpret_cast<float*>(s_x); // the state table as floats: slot t at 16 t (p0 p1 p2 q0 | q1 q2 q3 v0 | v1 v2 w0 w1 | w2 gate laps -)
     __syncthreads();
 
     for (int it = 0;; ++it)
@@ -1272,13 +1211,30 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                 if (gate_crossed(s_track.gates[gi], margin, p0, p1))
                     atomicOr(&s_cross[t], 1u << gi);
             }
-            const int Tw = (T + WAVE - 1) & ~(WAVE - 1); // cost and gradient lanes in separate waves
-            for (int i = tid; i < 2 * Tw; i += REF_BLOCK)
+            const int Tw = (T + WAVE - 1) & ~(WAVE - 1); // cost, gradient and Jacobian lanes in separate waves
+            for (int i = tid; i < 3 * Tw; i += REF_BLOCK)
             {
-                const bool grad = i >= Tw;
-                const int t = grad ? i - Tw : i;
-                if (t >= T || (grad && it == n_steps))
+                const int kind = i / Tw, t = i - kind * Tw;
+                if (t >= T || (kind > 0 && it == n_steps))
                     continue;
+                if (kind == 2)
+                {
+                    // the Jacobian blocks of step t for the reverse pass
+                    QState x;
+                    for (int c = 0; c < 4; ++c)
+                        x.q[c] = s_xf[t * 16 + 3 + c];
+                    for (int c = 0; c < 3; ++c)
+                        x.w[c] = s_xf[t * 16 + 10 + c];
+                    const float4 u4 = s_uc[t];
+                    const float uc[4] = {u4.x, u4.y, u4.z, u4.w};
+                    float J[REF_JAC4 * 4];
+                    quad_step_jacobian(C, x, uc, J);
+                    J[REF_JAC4 * 4 - 1] = 0.0f;
+                    for (int k = 0; k < REF_JAC4; ++k)
+                        s_jac[t * REF_JAC4 + k] = make_float4(J[4 * k], J[4 * k + 1], J[4 * k + 2], J[4 * k + 3]);
+                    continue;
+                }
+                const bool grad = kind == 1;
                 QState x;
                 x.p[0] = s_xf[(t + 1) * 16];
                 x.p[1] = s_xf[(t + 1) * 16 + 1];
@@ -1426,28 +1382,44 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         {
             const int seg = tid / REF_COMP, comp = tid - seg * REF_COMP;
             const int a = seg * R, b = min(a + R, te + 1);
-            float f[13];
+            float lam[13];
 #pragma unroll
             for (int i = 0; i < 13; ++i)
-                f[i] = (i + 1 == comp) ? 1.0f : 0.0f;
-            QGrad lam = grad_from_flat(f);
-            RefRow nx = RefRow::load(s_x, s_u, s_gp, b - 1);
-            for (int t = b - 1; t >= a; --t)
-            {
-                const RefRow r = nx; // the previous step's row is loaded one step ahead, unconditionally
-                nx = RefRow::load(s_x, s_u, s_gp, t > a ? t - 1 : t);
-                const QState x = r.state();
-                const float uin[4] = {r.u.x, r.u.y, r.u.z, r.u.w};
-                float uc[4], gu[4];
-                clamp_control(C, uin, uc);
-                quad_step_vjp(C, x, uc, lam, gu);
+                lam[i] = (i + 1 == comp) ? 1.0f : 0.0f;
+            // the step's Jacobian is loaded one step ahead (it does not depend
+            // on lam); two steps per trip, alternating two register buffers
+            const auto stepj = [&](int t, const float4 (&jq)[REF_JAC4]) {
+                const float4 gp = s_gp[t > 0 ? t - 1 : 0];
+                float gu[4];
+                quad_step_jvp_t(C, reinterpret_cast<const float*>(jq), lam, gu);
                 s_gu[t * REF_COMP + comp] = make_float4(gu[0], gu[1], gu[2], gu[3]);
                 const bool add = comp == 0 && t >= 1;
-                lam.p[0] = add ? lam.p[0] + r.gp.x : lam.p[0];
-                lam.p[1] = add ? lam.p[1] + r.gp.y : lam.p[1];
-                lam.p[2] = add ? lam.p[2] + r.gp.z : lam.p[2];
+                lam[0] = add ? lam[0] + gp.x : lam[0];
+                lam[1] = add ? lam[1] + gp.y : lam[1];
+                lam[2] = add ? lam[2] + gp.z : lam[2];
+            };
+            float4 ja[REF_JAC4], jb[REF_JAC4];
+#pragma unroll
+            for (int k = 0; k < REF_JAC4; ++k)
+                ja[k] = s_jac[(b - 1) * REF_JAC4 + k];
+            for (int t = b - 1; t >= a; t -= 2)
+            {
+                const int t1 = t - 1 >= a ? t - 1 : a;
+#pragma unroll
+                for (int k = 0; k < REF_JAC4; ++k)
+                    jb[k] = s_jac[t1 * REF_JAC4 + k];
+                __builtin_amdgcn_sched_barrier(0);
+                stepj(t, ja);
+                if (t - 1 < a)
+                    break;
+                const int t2 = t - 2 >= a ? t - 2 : a;
+#pragma unroll
+                for (int k = 0; k < REF_JAC4; ++k)
+                    ja[k] = s_jac[t2 * REF_JAC4 + k];
+                __builtin_amdgcn_sched_barrier(0);
+                stepj(t - 1, jb);
             }
-            grad_to_flat(lam, f);
+            const float* f = lam;
             float* out = s_psi + (seg * REF_COMP + comp) * 16;
 #pragma unroll
             for (int i = 0; i < 13; ++i)
@@ -1459,15 +1431,22 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         {
             // lanes 13..63 compute copies of rows 0..15 (row 13..15: unused
             // padding) and are never read; every load is unconditional
+            // (each segment's map is loaded one segment ahead: the loads do
+            // not depend on lam)
             const int i = tid & 15;
             float lam = s_lam[(n_seg - 1) * 16 + i];
+            float pc[REF_COMP];
+#pragma unroll
+            for (int c = 0; c < REF_COMP; ++c)
+                pc[c] = s_psi[((n_seg - 1) * REF_COMP + c) * 16 + i];
             for (int sg = n_seg - 1; sg >= 1; --sg)
             {
-                const float* P = s_psi + sg * REF_COMP * 16;
-                float pc[REF_COMP];
+                float pn[REF_COMP];
+                const float* P = s_psi + (sg > 1 ? sg - 1 : sg) * REF_COMP * 16;
 #pragma unroll
                 for (int c = 0; c < REF_COMP; ++c)
-                    pc[c] = P[c * 16 + i];
+                    pn[c] = P[c * 16 + i];
+                __builtin_amdgcn_sched_barrier(0);
                 float acc = pc[0];
 #pragma unroll
                 for (int j = 0; j < 13; ++j)
@@ -1478,6 +1457,9 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                 lam = acc;
                 if (tid < 13)
                     s_lam[(sg - 1) * 16 + tid] = lam;
+#pragma unroll
+                for (int c = 0; c < REF_COMP; ++c)
+                    pc[c] = pn[c];
             }
         }
         __syncthreads();

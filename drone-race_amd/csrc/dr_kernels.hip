@@ -897,13 +897,36 @@ constexpr int REF_SEGS = 16; // 14 * REF_SEGS <= REF_BLOCK
 constexpr int REF_COMP = 14; // affine part + 13 state directions
 constexpr int REF_JAC4 = (QJAC + 3) / 4; // float4 slots per step of the Jacobian table
 
+// The running-cost weights DECAY^t as the forward passes form them (a running
+// product, one multiply per step), tabulated at compile time.
+struct DecayTable
+{
+    float w[DR_MAX_T];
+};
+
+constexpr DecayTable make_decay_table()
+{
+    DecayTable d{};
+    float w = 1.0f;
+    for (int t = 0; t < DR_MAX_T; ++t)
+    {
+        d.w[t] = w;
+        w = w * DECAY;
+    }
+    return d;
+}
+
+__constant__ DecayTable c_decay = make_decay_table();
+
 // Block-wide inclusive scans for the refinement's forward pass: lane t of
 // the REF_BLOCK-lane workgroup holds element t (identity past the last
 // element), and after the scan it holds e_0 o e_1 o ... o e_t for an
-// associative op(earlier, later).  Within a wave: row shifts by 1, 2, 4, 8
-// and the row broadcasts of lanes 15 and 31 (DPP, no LDS); across waves: each
-// wave's total through LDS.  A lane whose shifted source lies outside its row
-// receives the identity, and op(identity, x) = x.
+// associative op(earlier, later), and ex holds e_0 o ... o e_{t-1} (the
+// identity at t = 0).  Within a wave: row shifts by 1, 2, 4, 8 and the row
+// broadcasts of lanes 15 and 31 (DPP, no LDS), then a wave shift by one for
+// the exclusive prefix; across waves: each wave's total through LDS (one
+// barrier; each scan uses its own s_tot).  A lane whose shifted source lies
+// outside its row receives the identity, and op(identity, x) = x.
 template <int CTRL, int ROWS, int N>
 __device__ __forceinline__ void dpp_shift(const float (&x)[N], const float (&id)[N], float (&out)[N])
 {
@@ -913,7 +936,7 @@ __device__ __forceinline__ void dpp_shift(const float (&x)[N], const float (&id)
 }
 
 template <int N, class Op>
-__device__ __forceinline__ void block_scan(float (&x)[N], const float (&id)[N], Op op, float* s_tot)
+__device__ __forceinline__ void block_scan(float (&x)[N], float (&ex)[N], const float (&id)[N], Op op, float* s_tot)
 {
     float y[N];
     dpp_shift<0x111, 0xf>(x, id, y); // row_shr:1
@@ -928,6 +951,7 @@ __device__ __forceinline__ void block_scan(float (&x)[N], const float (&id)[N], 
     op(y, x);
     dpp_shift<0x143, 0xc>(x, id, y); // row_bcast:31 into rows 2 and 3
     op(y, x);
+    dpp_shift<0x138, 0xf>(x, id, ex); // wave_shr:1: the exclusive prefix within the wave
     const int wave = threadIdx.x / WAVE;
     if ((threadIdx.x & (WAVE - 1)) == WAVE - 1)
 #pragma unroll
@@ -952,8 +976,8 @@ __device__ __forceinline__ void block_scan(float (&x)[N], const float (&id)[N], 
                 pre[i] = t[i];
         }
         op(pre, x);
+        op(pre, ex);
     }
-    __syncthreads(); // s_tot free for the next scan
 }
 
 // op(e, l): l <- the affine map "l after e", maps x -> A x + B_i (one A for
@@ -1013,7 +1037,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     __shared__ float4 s_op[MAX_OPP], s_ov[MAX_OPP];
     __shared__ float4 s_uc[DR_MAX_T + 1];  // the clamped controls the forward pass integrates (+1: read-ahead slot)
     __shared__ uint32_t s_cross[DR_MAX_T]; // bit g: the move x_t -> x_{t+1} crosses gate g
-    __shared__ float s_tot[(REF_BLOCK / WAVE) * 4]; // per-wave totals of the forward pass's scans
+    __shared__ float s_tot[4][(REF_BLOCK / WAVE) * 4]; // per-wave totals of the forward pass's four scans
     __shared__ int s_xstep[DR_MAX_T], s_xgate[DR_MAX_T], s_xlaps[DR_MAX_T]; // gate crossings: step, gate, laps after
     __shared__ int s_nx;
     __shared__ float4 s_ubest[DR_MAX_T]; // the lowest-cost sequence evaluated so far
@@ -1039,16 +1063,53 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     }
     {
         // the MPPI update (k_update's mode 1): nominal + num/den over the
-        // chunks in fixed order, box clamp; nu and the temperature rule
-        const double dn = sum_in_order(den + I.den_off, I.n_chunks, 1);
-        float* su = reinterpret_cast<float*>(s_u);
-        for (int i = tid; i < T * 4; i += REF_BLOCK)
+        // chunks in fixed order, box clamp; nu and the temperature rule.  Each
+        // lane sums the denominator and its (up to 4) values together, eight
+        // chunks of loads in flight at a time.
+        constexpr int VPL = (DR_MAX_T * 4 + REF_BLOCK - 1) / REF_BLOCK;
+        const int nc = I.n_chunks;
+        const long long st = (long long)T * 4;
+        const double* pd = den + I.den_off;
+        const double* pp = part + I.part_off;
+        double dn = 0.0, num[VPL];
+        float nom[VPL];
+#pragma unroll
+        for (int k = 0; k < VPL; ++k)
         {
-            const double num = sum_in_order(part + I.part_off + i, I.n_chunks, (long long)T * 4);
-            float u = nominal[I.nom_off + i];
-            if (dn > 1e-30)
-                u = u + (float)(num / dn);
-            su[i] = u;
+            const int i = tid + k * REF_BLOCK;
+            num[k] = 0.0;
+            nom[k] = i < T * 4 ? nominal[I.nom_off + i] : 0.0f;
+        }
+        for (int c0 = 0; c0 < nc; c0 += 8)
+        {
+            double vd[8], vp[VPL][8];
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+            {
+                vd[b] = c0 + b < nc ? pd[c0 + b] : 0.0;
+#pragma unroll
+                for (int k = 0; k < VPL; ++k)
+                {
+                    const int i = tid + k * REF_BLOCK;
+                    vp[k][b] = (c0 + b < nc && i < T * 4) ? pp[(c0 + b) * st + i] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+            {
+                dn += vd[b];
+#pragma unroll
+                for (int k = 0; k < VPL; ++k)
+                    num[k] += vp[k][b];
+            }
+        }
+        float* su = reinterpret_cast<float*>(s_u);
+#pragma unroll
+        for (int k = 0; k < VPL; ++k)
+        {
+            const int i = tid + k * REF_BLOCK;
+            if (i < T * 4)
+                su[i] = dn > 1e-30 ? nom[k] + (float)(num[k] / dn) : nom[k];
         }
         __syncthreads();
         for (int t = tid; t < T; t += REF_BLOCK)
@@ -1073,16 +1134,8 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             }
         }
     }
-    if (tid == 0)
-    {
-        // the running-cost weights, the forward pass's decay products
-        float w = 1.0f;
-        for (int t = 0; t < T; ++t)
-        {
-            s_w[t] = w;
-            w = w * DECAY;
-        }
-    }
+    for (int t = tid; t < T; t += REF_BLOCK)
+        s_w[t] = c_decay.w[t]; // the running-cost weights
     const QState x0 = state_from_abi(drones[I.drone]);
     float* const s_xf = reinterpret_cast<float*>(s_x); // the state table as floats: slot t at 16 t (p0 p1 p2 q0 | q1 q2 q3 v0 | v1 v2 w0 w1 | w2 gate laps -)
     __syncthreads();
@@ -1110,90 +1163,108 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         // attitude).  So the states are quad_step's to rounding, and the pass is
         // four scans instead of a T-step serial chain.
         {
+            // lane t: the state at step t from each scan's exclusive prefix (in
+            // registers, for the next scan), the state at t + 1 from the
+            // inclusive one (into the table)
             const int t = tid;
             const bool in = t < T;
             float* xs = s_xf + (t + 1) * 16; // slot t + 1
+            const float4 u4 = in ? s_uc[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const float id4[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+            float wt[3], qt[4], vt[3];
             {
-                const float id[4] = {1.0f, 0.0f, 0.0f, 0.0f};
-                float x[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+                float x[4] = {1.0f, 0.0f, 0.0f, 0.0f}, ex[4];
                 if (in)
                 {
-                    const float4 u = s_uc[t];
                     x[0] = 1.0f - C.rate_k;
-                    x[1] = C.rate_k * u.y;
-                    x[2] = C.rate_k * u.z;
-                    x[3] = C.rate_k * u.w;
+                    x[1] = C.rate_k * u4.y;
+                    x[2] = C.rate_k * u4.z;
+                    x[3] = C.rate_k * u4.w;
                 }
-                block_scan(x, id, AffineOp(), s_tot);
+                block_scan(x, ex, id4, AffineOp(), s_tot[0]);
+                for (int i = 0; i < 3; ++i)
+                    wt[i] = fma_(ex[0], x0.w[i], ex[1 + i]);
                 if (in)
                     for (int i = 0; i < 3; ++i)
                         xs[10 + i] = fma_(x[0], x0.w[i], x[1 + i]);
-                if (t == 0)
-                    for (int i = 0; i < 3; ++i)
-                        s_xf[10 + i] = x0.w[i];
             }
-            __syncthreads();
             {
-                const float id[4] = {1.0f, 0.0f, 0.0f, 0.0f};
-                float x[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+                float x[4] = {1.0f, 0.0f, 0.0f, 0.0f}, ex[4];
                 if (in)
                 {
-                    const float wt[3] = {s_xf[t * 16 + 10], s_xf[t * 16 + 11], s_xf[t * 16 + 12]};
                     const float4 inc = attitude_increment(C, wt);
                     x[0] = inc.w;
                     x[1] = inc.w * inc.x;
                     x[2] = inc.w * inc.y;
                     x[3] = inc.w * inc.z;
                 }
-                block_scan(x, id, QuatOp(), s_tot);
+                block_scan(x, ex, id4, QuatOp(), s_tot[1]);
+                const float q0[4] = {x0.q[0], x0.q[1], x0.q[2], x0.q[3]};
+                QuatOp()(q0, ex);
+                for (int i = 0; i < 4; ++i)
+                    qt[i] = ex[i];
                 if (in)
                 {
-                    float q[4] = {x0.q[0], x0.q[1], x0.q[2], x0.q[3]};
-                    QuatOp()(q, x);
+                    QuatOp()(q0, x);
                     for (int i = 0; i < 4; ++i)
                         xs[3 + i] = x[i];
                 }
-                if (t == 0)
-                    for (int i = 0; i < 4; ++i)
-                        s_xf[3 + i] = x0.q[i];
             }
-            __syncthreads();
+            if (in && it < n_steps)
             {
-                const float id[4] = {1.0f, 0.0f, 0.0f, 0.0f};
-                float x[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+                // the Jacobian blocks of step t for the reverse pass
+                QState xq;
+                for (int c = 0; c < 4; ++c)
+                    xq.q[c] = qt[c];
+                for (int c = 0; c < 3; ++c)
+                    xq.w[c] = wt[c];
+                const float uc[4] = {u4.x, u4.y, u4.z, u4.w};
+                float J[REF_JAC4 * 4];
+                quad_step_jacobian(C, xq, uc, J);
+                J[REF_JAC4 * 4 - 1] = 0.0f;
+                for (int k = 0; k < REF_JAC4; ++k)
+                    s_jac[t * REF_JAC4 + k] = make_float4(J[4 * k], J[4 * k + 1], J[4 * k + 2], J[4 * k + 3]);
+            }
+            {
+                float x[4] = {1.0f, 0.0f, 0.0f, 0.0f}, ex[4];
                 if (in)
                 {
-                    const float qt[4] = {s_xf[t * 16 + 3], s_xf[t * 16 + 4], s_xf[t * 16 + 5], s_xf[t * 16 + 6]};
                     float b3[3];
                     thrust_dir(qt, b3);
-                    const float at = s_uc[t].x * C.inv_mass;
+                    const float at = u4.x * C.inv_mass;
                     x[0] = fma_(-C.dt, C.drag, 1.0f);
                     x[1] = C.dt * (at * b3[0]);
                     x[2] = C.dt * (at * b3[1]);
                     x[3] = C.dt * fma_(at, b3[2], -C.gravity);
                 }
-                block_scan(x, id, AffineOp(), s_tot);
+                block_scan(x, ex, id4, AffineOp(), s_tot[2]);
+                for (int i = 0; i < 3; ++i)
+                    vt[i] = fma_(ex[0], x0.v[i], ex[1 + i]);
                 if (in)
                     for (int i = 0; i < 3; ++i)
                         xs[7 + i] = fma_(x[0], x0.v[i], x[1 + i]);
-                if (t == 0)
-                    for (int i = 0; i < 3; ++i)
-                        s_xf[7 + i] = x0.v[i];
             }
-            __syncthreads();
             {
-                const float id[3] = {0.0f, 0.0f, 0.0f};
-                float x[3] = {0.0f, 0.0f, 0.0f};
+                const float id3[3] = {0.0f, 0.0f, 0.0f};
+                float x[3] = {0.0f, 0.0f, 0.0f}, ex[3];
                 if (in)
                     for (int i = 0; i < 3; ++i)
-                        x[i] = C.dt * s_xf[t * 16 + 7 + i];
-                block_scan(x, id, AddOp(), s_tot);
+                        x[i] = C.dt * vt[i];
+                block_scan(x, ex, id3, AddOp(), s_tot[3]);
                 if (in)
                     for (int i = 0; i < 3; ++i)
                         xs[i] = x0.p[i] + x[i];
-                if (t == 0)
-                    for (int i = 0; i < 3; ++i)
-                        s_xf[i] = x0.p[i];
+            }
+            if (t == 0)
+            {
+                for (int i = 0; i < 3; ++i)
+                {
+                    s_xf[i] = x0.p[i];
+                    s_xf[7 + i] = x0.v[i];
+                    s_xf[10 + i] = x0.w[i];
+                }
+                for (int i = 0; i < 4; ++i)
+                    s_xf[3 + i] = x0.q[i];
             }
             __syncthreads();
         }
@@ -1211,29 +1282,12 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                 if (gate_crossed(s_track.gates[gi], margin, p0, p1))
                     atomicOr(&s_cross[t], 1u << gi);
             }
-            const int Tw = (T + WAVE - 1) & ~(WAVE - 1); // cost, gradient and Jacobian lanes in separate waves
-            for (int i = tid; i < 3 * Tw; i += REF_BLOCK)
+            const int Tw = (T + WAVE - 1) & ~(WAVE - 1); // cost and gradient lanes in separate waves
+            for (int i = tid; i < 2 * Tw; i += REF_BLOCK)
             {
                 const int kind = i / Tw, t = i - kind * Tw;
                 if (t >= T || (kind > 0 && it == n_steps))
                     continue;
-                if (kind == 2)
-                {
-                    // the Jacobian blocks of step t for the reverse pass
-                    QState x;
-                    for (int c = 0; c < 4; ++c)
-                        x.q[c] = s_xf[t * 16 + 3 + c];
-                    for (int c = 0; c < 3; ++c)
-                        x.w[c] = s_xf[t * 16 + 10 + c];
-                    const float4 u4 = s_uc[t];
-                    const float uc[4] = {u4.x, u4.y, u4.z, u4.w};
-                    float J[REF_JAC4 * 4];
-                    quad_step_jacobian(C, x, uc, J);
-                    J[REF_JAC4 * 4 - 1] = 0.0f;
-                    for (int k = 0; k < REF_JAC4; ++k)
-                        s_jac[t * REF_JAC4 + k] = make_float4(J[4 * k], J[4 * k + 1], J[4 * k + 2], J[4 * k + 3]);
-                    continue;
-                }
                 const bool grad = kind == 1;
                 QState x;
                 x.p[0] = s_xf[(t + 1) * 16];

@@ -31,17 +31,6 @@ struct ModelOpp
     }
 };
 
-// The same prediction from registers (index constant after unrolling).
-struct RegOpp
-{
-    float4 p[MAX_OPP], v[MAX_OPP];
-    float tau;
-    __device__ float4 operator()(int j) const
-    {
-        return make_float4(fma_(v[j].x, tau, p[j].x), fma_(v[j].y, tau, p[j].y), fma_(v[j].z, tau, p[j].z), 0.0f);
-    }
-};
-
 __device__ __forceinline__ void stage_model_opp(float4 (*dst)[2], const dr_drone_state& o, const PrInstDev& I, int b, int j)
 {
     dst[j][0] = make_float4(o.p[0], o.p[1], o.p[2], 0.0f);
@@ -377,7 +366,7 @@ __global__ __launch_bounds__(PR_WAVG_BLOCK) void k_pr_wavg(const PrInstDev* __re
 // when any model finds the chosen plan unsafe), the three temperature
 // updates (prmppi_controller.cu:328-346), and the plans for the next step:
 // unshifted (plan) and shifted with shift_fill (work).
-constexpr int UPD_CH = 8; // steps per chunk of the candidate rollouts' two-wave pipeline
+constexpr int UPD_CH = 8; // steps per chunk of the candidate rollouts' three-wave pipeline
 
 __global__ __launch_bounds__(256) void k_pr_update(const PrInstDev* __restrict__ insts, PrDyn* __restrict__ dyn,
                                                    const double* __restrict__ part, const double* __restrict__ den,
@@ -389,6 +378,9 @@ __global__ __launch_bounds__(256) void k_pr_update(const PrInstDev* __restrict__
     __shared__ float4 s_pv[DR_MAX_MODELS][MAX_OPP][2];
     __shared__ dr_track s_track;
     __shared__ float4 s_ring[2][UPD_CH][2 * DR_MAX_MODELS]; // position, gate | laps << 8
+    __shared__ float4 s_rq[2][UPD_CH][2 * DR_MAX_MODELS];   // attitude at the start of the step
+    __shared__ float s_rf[2][UPD_CH][2 * DR_MAX_MODELS];    // clamped collective thrust
+    __shared__ float4 s_rc[UPD_CH][2 * DR_MAX_MODELS];      // running cost, safety, terminal code of one chunk
 
     const int inst = blockIdx.x;
     const PrInstDev& I = insts[inst];
@@ -412,49 +404,44 @@ __global__ __launch_bounds__(256) void k_pr_update(const PrInstDev* __restrict__
     pr_candidates(I, part, work, sh, tid);
     __syncthreads();
 
-    // Noise-free rollouts, one (candidate, model) per lane l < 2M, in two
-    // waves joined by a double-buffered LDS ring of UPD_CH steps: lane l of
-    // wave 0 integrates the dynamics and the gate counter (the latency-bound
-    // recurrence; the next control in a register) and writes each step's
-    // position, gate and laps (the track's gates in registers); lane l of
-    // wave 1 evaluates the running cost,
-    // safety and terminal test of the previous chunk with the model's
-    // opponents in registers.  One barrier per chunk.  The state integrates
-    // every step; the cost, the safety and the state the terminal cost reads
-    // are frozen at the first terminal step.
+    // Noise-free rollouts, one (candidate, model) per lane l < 2M, in three
+    // waves joined by double-buffered LDS rings of UPD_CH steps, one barrier
+    // per chunk: lane l of wave 0 clamps the controls and integrates the body
+    // rates and the attitude (the next control in a register), handing on each
+    // step's pre-step attitude and thrust; lane l of wave 2, one chunk behind,
+    // integrates velocity and position and runs the gate counter (the track's
+    // gates in registers), writing each step's position, gate and laps; wave
+    // 1, two chunks behind, evaluates the running cost, safety and terminal
+    // test of every (step, candidate) pair of the chunk in parallel lanes, then
+    // lane l sums its candidate's costs in step order.  The operations are
+    // quad_step's and pr_stage_cost's.  The state integrates every step; the
+    // cost, the safety and the state the terminal cost reads are frozen at the
+    // first terminal step.
     {
         const int n_lanes = 2 * M;
         const int n_ch = (T + UPD_CH - 1) / UPD_CH;
-        const bool dyn_lane = tid < n_lanes;
+        const bool att_lane = tid < n_lanes;
         const int cl = tid - WAVE;
         const bool cost_lane = cl >= 0 && cl < n_lanes;
+        const int xl = tid - 2 * WAVE;
+        const bool trans_lane = xl >= 0 && xl < n_lanes;
         QState s = state_from_abi(drones[I.drone]);
         GateRegs gr;
         const float4* U = nullptr;
         float4 un = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (dyn_lane)
+        if (att_lane)
         {
             U = reinterpret_cast<const float4*>(sh.c[tid / M]);
-            gr.init(s_track, s.gate);
             un = U[0];
         }
-        RegOpp ro;
-        if (cost_lane)
-        {
-            const int m = cl - (cl / M) * M;
-#pragma unroll
-            for (int j = 0; j < MAX_OPP; ++j)
-            {
-                ro.p[j] = j < n_opp ? s_pv[m][j][0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                ro.v[j] = j < n_opp ? s_pv[m][j][1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            }
-        }
+        if (trans_lane)
+            gr.init(s_track, s.gate);
         float acc = 0.0f, decay = 1.0f, smax = -__builtin_inff();
         QState fin = s;
         bool stop = false;
-        for (int k = 0; k <= n_ch; ++k)
+        for (int k = 0; k <= n_ch + 1; ++k)
         {
-            if (dyn_lane && k < n_ch)
+            if (att_lane && k < n_ch)
             {
                 const int t0 = k * UPD_CH, t1 = min(t0 + UPD_CH, T);
                 for (int t = t0; t < t1; ++t)
@@ -463,41 +450,61 @@ __global__ __launch_bounds__(256) void k_pr_update(const PrInstDev* __restrict__
                     un = U[t + 1 < T ? t + 1 : t]; // issued every step, so never waited for early
                     float uin[4] = {u4.x + 0.0f, u4.y + 0.0f, u4.z + 0.0f, u4.w + 0.0f}, uc[4];
                     clamp_control(C, uin, uc);
-                    const float p0[3] = {s.p[0], s.p[1], s.p[2]};
-                    quad_step(C, s, uc);
-                    gr.step(KC.gate_traversal_margin, p0, s);
-                    s_ring[k & 1][t - t0][tid] = make_float4(s.p[0], s.p[1], s.p[2], u2f((uint32_t)s.gate | ((uint32_t)s.laps << 8)));
+                    s_rq[k & 1][t - t0][tid] = make_float4(s.q[0], s.q[1], s.q[2], s.q[3]);
+                    s_rf[k & 1][t - t0][tid] = uc[0];
+                    quad_attitude_step(C, s.q, s.w, uc);
                 }
             }
-            else if (cost_lane && k >= 1)
+            else if (trans_lane && k >= 1 && k <= n_ch)
             {
                 const int t0 = (k - 1) * UPD_CH, t1 = min(t0 + UPD_CH, T);
                 for (int t = t0; t < t1; ++t)
                 {
-                    const float4 r = s_ring[(k - 1) & 1][t - t0][cl];
+                    const float4 q4 = s_rq[(k - 1) & 1][t - t0][xl];
+                    const float q[4] = {q4.x, q4.y, q4.z, q4.w};
+                    const float uu[4] = {s_rf[(k - 1) & 1][t - t0][xl], 0.0f, 0.0f, 0.0f}; // the translation reads the thrust only
+                    const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+                    quad_translation_step(C, q, s.p, s.v, uu);
+                    gr.step(KC.gate_traversal_margin, p0, s);
+                    s_ring[(k - 1) & 1][t - t0][xl] = make_float4(s.p[0], s.p[1], s.p[2], u2f((uint32_t)s.gate | ((uint32_t)s.laps << 8)));
+                }
+            }
+            else if (cl >= 0 && cl < WAVE && k >= 2)
+            {
+                const int t0 = (k - 2) * UPD_CH, nt = min(t0 + UPD_CH, T) - t0;
+                for (int i = cl; i < nt * n_lanes; i += WAVE)
+                {
+                    const int to = i / n_lanes, c = i - to * n_lanes;
+                    const float4 r = s_ring[(k - 2) & 1][to][c];
                     const uint32_t gl = f2u(r.w);
-                    s.p[0] = r.x;
-                    s.p[1] = r.y;
-                    s.p[2] = r.z;
-                    s.gate = (int)(gl & 0xffu);
-                    s.laps = (int)(gl >> 8);
-                    ro.tau = C.dt * (float)(t + 1);
+                    QState x;
+                    x.p[0] = r.x;
+                    x.p[1] = r.y;
+                    x.p[2] = r.z;
+                    x.gate = (int)(gl & 0xffu);
+                    x.laps = (int)(gl >> 8);
+                    const ModelOpp mo{s_pv[c - (c / M) * M], C.dt * (float)(t0 + to + 1)};
                     float sf;
                     int term;
-                    const float c = pr_stage_cost(C, Q, KC, s_track, s, n_opp, ro, &sf, &term);
-                    if (!stop)
-                    {
-                        acc = fma_(c, decay, acc);
-                        smax = fmaxf(smax, sf);
-                        decay = decay * DECAY;
-                        fin.p[0] = s.p[0];
-                        fin.p[1] = s.p[1];
-                        fin.p[2] = s.p[2];
-                        fin.gate = s.gate;
-                        fin.laps = s.laps;
-                        stop = term != TERM_NONE;
-                    }
+                    const float cst = pr_stage_cost(C, Q, KC, s_track, x, n_opp, mo, &sf, &term);
+                    s_rc[to][c] = make_float4(cst, sf, u2f((uint32_t)term), 0.0f);
                 }
+                if (cost_lane)
+                    for (int to = 0; to < nt && !stop; ++to)
+                    {
+                        const float4 rc = s_rc[to][cl];
+                        acc = fma_(rc.x, decay, acc);
+                        smax = fmaxf(smax, rc.y);
+                        decay = decay * DECAY;
+                        const float4 r = s_ring[(k - 2) & 1][to][cl];
+                        const uint32_t gl = f2u(r.w);
+                        fin.p[0] = r.x;
+                        fin.p[1] = r.y;
+                        fin.p[2] = r.z;
+                        fin.gate = (int)(gl & 0xffu);
+                        fin.laps = (int)(gl >> 8);
+                        stop = (int)f2u(rc.z) != TERM_NONE;
+                    }
             }
             __syncthreads();
         }

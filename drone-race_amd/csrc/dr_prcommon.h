@@ -43,15 +43,43 @@ __device__ __forceinline__ void pr_begin(const PrInstDev& I, const PrDyn& Y, uin
 __device__ __forceinline__ void pr_candidates(const PrInstDev& I, const double* __restrict__ part, const float* __restrict__ work,
                                               PrUpdShared& sh, int tid)
 {
-    const int T = I.T;
-    for (int i = tid; i < 3 * T * 4; i += blockDim.x)
+    // each lane sums 4 values at a time over the chunks in order, 8 chunks of
+    // loads in flight together
+    const int T = I.T, n = 3 * T * 4, nc = I.n_chunks;
+    const int stride = (int)blockDim.x;
+    const double* pp = part + I.part_off;
+    for (int i0 = tid; i0 < n; i0 += 4 * stride)
     {
-        const int j = i / (T * 4), r = i - j * T * 4;
-        const double num = sum_in_order(part + I.part_off + (long long)j * T * 4 + r, I.n_chunks, 3LL * T * 4);
-        float u = work[I.plan_off + (j == 0 ? 0 : T * 4) + r];
-        if (sh.dn[j] > 1e-30)
-            u = (float)(num / sh.dn[j]);
-        sh.c[j][r] = u;
+        double num[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int c0 = 0; c0 < nc; c0 += 8)
+        {
+            double v[4][8];
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                {
+                    const int i = i0 + k * stride;
+                    v[k][b] = (c0 + b < nc && i < n) ? pp[(long long)(c0 + b) * n + i] : 0.0;
+                }
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    num[k] += v[k][b];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            const int i = i0 + k * stride;
+            if (i >= n)
+                break;
+            const int j = i / (T * 4), r = i - j * T * 4;
+            float u = work[I.plan_off + (j == 0 ? 0 : T * 4) + r];
+            if (sh.dn[j] > 1e-30)
+                u = (float)(num[k] / sh.dn[j]);
+            sh.c[j][r] = u;
+        }
     }
 }
 

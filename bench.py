@@ -423,7 +423,17 @@ def c0_runs(pkg, n_steps=500):
                     "final_gate_count": g.gate + scn.tracks[0].n_gates * g.laps}}
 
 
-def pr_mppi(pkg, steps=100, warmup=10):
+def tune_launch(eng, autotune):
+    """Launch shape the engine's autotuner keeps (it restores the plans and state it steps), with the
+    ms per step it measured; None with autotuning off (the engine's default launch)."""
+    if not autotune:
+        return None
+    cfg, ms = eng.autotune(trials=3)
+    log(f"bench: autotuned launch {cfg} ({ms:.3f} ms/step)")
+    return {"launch": cfg, "ms_per_step": ms}
+
+
+def pr_mppi(pkg, steps=100, warmup=10, autotune=True):
     """PR-MPPI, 4 drones, B ~ U{3..6} opponent-velocity branches per controller,
     Dirichlet(1) beliefs, K = 8192 samples evaluated under every branch for
     both plans, T = 80, closed loop, graph replays."""
@@ -431,6 +441,7 @@ def pr_mppi(pkg, steps=100, warmup=10):
     scn = sc.c3_pr_four_drones()
     n = len(scn.instances)
     eng = pkg.PREngine(scn)
+    tuned = tune_launch(eng, autotune)
     eng.step(warmup, advance=True)
     t0 = time.perf_counter()
     eng.step(steps, advance=True)
@@ -443,7 +454,7 @@ def pr_mppi(pkg, steps=100, warmup=10):
     eng.close()
     rollouts = sum(2 * i.n_branches * i.n_samples for i in scn.instances)
     ss = sum(2 * i.n_branches * i.n_samples * i.n_timesteps for i in scn.instances)
-    return {"workload": scn.name, "branches": [i.n_branches for i in scn.instances],
+    return {"workload": scn.name, "autotuned": tuned, "branches": [i.n_branches for i in scn.instances],
             "control_steps_per_s": n * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
             "rollouts_per_step": rollouts, "rollout_sample_steps_per_s": ss * steps / elapsed,
             "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate": float(ms[1]), "weighted_avg": float(ms[2]),
@@ -458,13 +469,14 @@ def sr_roofline(key, rollout_steps, rollout_ms, kernel):
                            "FP32 vector (VALU) work, no MFMA; oracle-counted flops per rollout-step")
 
 
-def sr_race(pkg, steps=50, warmup=5, n_inst=1):
+def sr_race(pkg, steps=50, warmup=5, n_inst=1, autotune=True):
     """Plain MPPI on the reference's StratRace2D race at its dense setting
     (K = 2^16, T = 70, temperature 10, sigma 1.5, round track r = 6 m), closed
     loop with the environment's noise, graph replays; n_inst worlds."""
     sc = pkg.scenarios
     scn = sc.sr_race(n_inst=n_inst)
     eng = pkg.SREngine(scn)
+    tuned = tune_launch(eng, autotune)
     eng.step(warmup, advance=True)
     t0 = time.perf_counter()
     eng.step(steps, advance=True)
@@ -474,19 +486,20 @@ def sr_race(pkg, steps=50, warmup=5, n_inst=1):
     ms = eng.last_timing() / 5
     eng.close()
     ss = scn.sample_steps_per_step
-    return {"workload": scn.name, "control_steps_per_s": n_inst * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+    return {"workload": scn.name, "autotuned": tuned, "control_steps_per_s": n_inst * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
             "sample_steps_per_s": ss * steps / elapsed,
             "kernel_ms_per_step": {"rollout": float(ms[0]), "weighted_avg": float(ms[1]), "update_world": float(ms[2]),
                                    "step": float(ms[3])},
             "roofline": sr_roofline("sr_race", ss, float(ms[0]), "k_sr_rollout"), "steps": steps}
 
 
-def pr_sr_race(pkg, steps=30, warmup=3):
+def pr_sr_race(pkg, steps=30, warmup=3, autotune=True):
     """PR-MPPI on the reference's StratRace2D race at its PR setting (K = 2^16, T = 60,
     P = 9 models drawn from the belief, temperature 10, sigma 0.5), closed loop, one world."""
     sc = pkg.scenarios
     scn = sc.pr_sr_race(n_inst=1)
     eng = pkg.PRSREngine(scn)
+    tuned = tune_launch(eng, autotune)
     eng.step(warmup, advance=True)
     t0 = time.perf_counter()
     eng.step(steps, advance=True)
@@ -497,7 +510,7 @@ def pr_sr_race(pkg, steps=30, warmup=3):
     eng.close()
     ins = scn.instances[0]
     rollouts = 2 * ins.n_draws * ins.n_samples
-    return {"workload": scn.name, "control_steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+    return {"workload": scn.name, "autotuned": tuned, "control_steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
             "rollouts_per_step": rollouts, "rollout_env_steps_per_s": rollouts * ins.n_timesteps * steps / elapsed,
             "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate": float(ms[1]), "weighted_avg": float(ms[2]),
                                    "update_choice_world": float(ms[3]), "step": float(ms[4])},
@@ -521,7 +534,7 @@ def timed_branching_steps(eng, n=5):
     return ms / n, ran / n
 
 
-def bm_race(pkg, steps=30, warmup=3, n_verif=0, n_inst=1):
+def bm_race(pkg, steps=30, warmup=3, n_verif=0, n_inst=1, autotune=True):
     """Branching spline MPPI (the reference's default MPPI controller) on its StratRace2D race at
     the spline setting (K = 2^16, T = 80, 20 knots, two opponent models, uniform initial belief,
     temperature 10 adapted, sigma 0.5), closed loop with belief updates, n_inst worlds batched in
@@ -531,6 +544,7 @@ def bm_race(pkg, steps=30, warmup=3, n_verif=0, n_inst=1):
     sc = pkg.scenarios
     scn = sc.bm_race(n_inst=n_inst, n_verif=n_verif)
     eng = pkg.BMEngine(scn)
+    tuned = tune_launch(eng, autotune)
     eng.step(warmup, advance=True)
     t0 = time.perf_counter()
     eng.step(steps, advance=True)
@@ -540,7 +554,7 @@ def bm_race(pkg, steps=30, warmup=3, n_verif=0, n_inst=1):
     eng.close()
     ins = scn.instances[0]
     rollouts = n_inst * scn.env.n_models * ins.n_samples
-    return {"workload": scn.name, "control_steps_per_s": n_inst * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+    return {"workload": scn.name, "autotuned": tuned, "control_steps_per_s": n_inst * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
             "rollouts_per_step": rollouts, "rollout_env_steps_per_s": rollouts * ins.n_timesteps * steps / elapsed,
             "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate_min": float(ms[1]), "weighted_avg": float(ms[2]),
                                    "update_verify": float(ms[3]), "choice_world": float(ms[4]), "step": float(ms[5])},
@@ -550,13 +564,14 @@ def bm_race(pkg, steps=30, warmup=3, n_verif=0, n_inst=1):
             "verification_samples": n_verif, "steps": steps}
 
 
-def ho_race(pkg, steps=30, warmup=3):
+def ho_race(pkg, steps=30, warmup=3, autotune=True):
     """Branching spline MPPI on the reference's HiddenObs course at its spline setting
     (hiddenObsEnv, main.py:705-833: K = 2^16, T = 80, 20 knots, temperature 10 adapted,
     sigma 0.8, uniform belief over which hidden obstacle exists), closed loop, one world."""
     sc = pkg.scenarios
     scn = sc.ho_race(n_inst=1)
     eng = pkg.HoBMEngine(scn)
+    tuned = tune_launch(eng, autotune)
     eng.step(warmup, advance=True)
     t0 = time.perf_counter()
     eng.step(steps, advance=True)
@@ -569,7 +584,7 @@ def ho_race(pkg, steps=30, warmup=3):
     eng.close()
     ins = scn.instances[0]
     rollouts = 2 * ins.n_samples
-    return {"workload": scn.name, "control_steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+    return {"workload": scn.name, "autotuned": tuned, "control_steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
             "rollouts_per_step": rollouts, "rollout_env_steps_per_s": rollouts * ins.n_timesteps * steps / elapsed,
             "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate_min": float(ms[1]), "weighted_avg": float(ms[2]),
                                    "update_verify": float(ms[3]), "choice_world": float(ms[4]), "step": float(ms[5])},
@@ -581,7 +596,7 @@ def ho_race(pkg, steps=30, warmup=3):
             "steps": steps}
 
 
-def drace_race(pkg, steps=30, warmup=3):
+def drace_race(pkg, steps=30, warmup=3, autotune=True):
     """Branching spline MPPI on the reference's DroneRace course at its spline setting
     (tinyGateEnv, main.py:40-161: K = 2^16, T = 60, 10 knots, temperature 10 adapted, sigma 2,
     two PD opponent models on the upper and lower raceline, uniform belief), closed loop, one
@@ -589,6 +604,7 @@ def drace_race(pkg, steps=30, warmup=3):
     sc = pkg.scenarios
     scn = sc.drace_race(n_inst=1)
     eng = pkg.DraceBMEngine(scn)
+    tuned = tune_launch(eng, autotune)
     eng.step(warmup, advance=True)
     t0 = time.perf_counter()
     eng.step(steps, advance=True)
@@ -600,7 +616,7 @@ def drace_race(pkg, steps=30, warmup=3):
     eng.close()
     ins = scn.instances[0]
     rollouts = scn.env.n_models * ins.n_samples
-    return {"workload": scn.name, "control_steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+    return {"workload": scn.name, "autotuned": tuned, "control_steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
             "rollouts_per_step": rollouts, "rollout_env_steps_per_s": rollouts * ins.n_timesteps * steps / elapsed,
             "kernel_ms_per_step": {"rollout": float(ms[0]), "aggregate_min": float(ms[1]), "weighted_avg": float(ms[2]),
                                    "update_verify": float(ms[3]), "choice_world": float(ms[4]), "step": float(ms[5])},
@@ -756,23 +772,23 @@ def main():
         print(json.dumps({"session": session_race(pkg, steps=min(args.steps, 30))}), flush=True)
         return
     if args.ho_only:
-        print(json.dumps({"ho_race": ho_race(pkg, steps=min(args.steps, 30), warmup=args.warmup),
-                          "drace_race": drace_race(pkg, steps=min(args.steps, 30), warmup=args.warmup)}), flush=True)
+        print(json.dumps({"ho_race": ho_race(pkg, steps=min(args.steps, 30), warmup=args.warmup, autotune=not args.no_autotune),
+                          "drace_race": drace_race(pkg, steps=min(args.steps, 30), warmup=args.warmup, autotune=not args.no_autotune)}), flush=True)
         return
     if args.bm_only:
-        print(json.dumps({"bm_race": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup),
-                          "bm_race_verified": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup, n_verif=2 ** 17)}),
+        print(json.dumps({"bm_race": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup, autotune=not args.no_autotune),
+                          "bm_race_verified": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup, n_verif=2 ** 17, autotune=not args.no_autotune)}),
               flush=True)
         return
     if args.sr_only:
-        print(json.dumps({"sr_race": sr_race(pkg, steps=args.steps, warmup=args.warmup),
-                          "pr_sr_race": pr_sr_race(pkg, steps=min(args.steps, 30), warmup=args.warmup),
-                          "bm_race": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup),
-                          "bm_race_verified": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup, n_verif=2 ** 17)}),
+        print(json.dumps({"sr_race": sr_race(pkg, steps=args.steps, warmup=args.warmup, autotune=not args.no_autotune),
+                          "pr_sr_race": pr_sr_race(pkg, steps=min(args.steps, 30), warmup=args.warmup, autotune=not args.no_autotune),
+                          "bm_race": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup, autotune=not args.no_autotune),
+                          "bm_race_verified": bm_race(pkg, steps=min(args.steps, 30), warmup=args.warmup, n_verif=2 ** 17, autotune=not args.no_autotune)}),
               flush=True)
         return
     if args.pr_only:
-        print(json.dumps({"pr_mppi": pr_mppi(pkg, steps=args.steps, warmup=args.warmup)}), flush=True)
+        print(json.dumps({"pr_mppi": pr_mppi(pkg, steps=args.steps, warmup=args.warmup, autotune=not args.no_autotune)}), flush=True)
         return
     if args.plain_only:
         launch = json.loads(args.launch) if args.launch else None
@@ -836,23 +852,23 @@ def main():
         log(f"bench: batch64 speedup {out['batch64']['speedup']:.2f}x")
     if not args.no_pr:
         log("bench: PR-MPPI, 4 drones, K=8192 per branch, T=80 ...")
-        out["pr_mppi"] = pr_mppi(pkg)
+        out["pr_mppi"] = pr_mppi(pkg, autotune=not args.no_autotune)
         log(f"bench: pr_mppi {json.dumps(out['pr_mppi'])}")
     if not args.no_sr:
         log("bench: StratRace2D plain MPPI, K=65536, T=70 ...")
-        out["sr_race"] = sr_race(pkg)
+        out["sr_race"] = sr_race(pkg, autotune=not args.no_autotune)
         log("bench: PR-MPPI on StratRace2D, K=65536, T=60, P=9 ...")
-        out["pr_sr_race"] = pr_sr_race(pkg)
+        out["pr_sr_race"] = pr_sr_race(pkg, autotune=not args.no_autotune)
         log("bench: branching spline MPPI on StratRace2D, K=65536, T=80, 20 knots ...")
-        out["bm_race"] = bm_race(pkg)
+        out["bm_race"] = bm_race(pkg, autotune=not args.no_autotune)
         log("bench: branching spline MPPI, 8 worlds in one batch ...")
-        out["bm_race_8x"] = bm_race(pkg, n_inst=8)
+        out["bm_race_8x"] = bm_race(pkg, n_inst=8, autotune=not args.no_autotune)
         log("bench: branching spline MPPI with plan verification (2^17 samples, horizon 40) ...")
-        out["bm_race_verified"] = bm_race(pkg, n_verif=2 ** 17)
+        out["bm_race_verified"] = bm_race(pkg, n_verif=2 ** 17, autotune=not args.no_autotune)
         log("bench: branching spline MPPI on HiddenObs, K=65536, T=80, 20 knots ...")
-        out["ho_race"] = ho_race(pkg)
+        out["ho_race"] = ho_race(pkg, autotune=not args.no_autotune)
         log("bench: branching spline MPPI on DroneRace, K=65536, T=60, 10 knots ...")
-        out["drace_race"] = drace_race(pkg)
+        out["drace_race"] = drace_race(pkg, autotune=not args.no_autotune)
         log("bench: wire-protocol session (MPPI and PR-MPPI headers at the reference settings) ...")
         out["session"] = session_race(pkg)
     if not args.no_cpu:

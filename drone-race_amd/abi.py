@@ -260,6 +260,7 @@ SIGNATURES = {
     "dr_pr_synchronize": (_I, [_P]),
     "dr_pr_set_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
     "dr_pr_get_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
+    "dr_pr_autotune": (_I, [_P, _I, C.POINTER(LaunchCfg), _F]),
     "dr_pr_set_timing": (_I, [_P, _I]),
     "dr_pr_last_timing": (_I, [_P, _F, _I]),
     "dr_sr_create": (_I, [C.POINTER(SrEnv), C.POINTER(SrCost), C.POINTER(SrInstance), _I, C.POINTER(_P)]),
@@ -275,6 +276,7 @@ SIGNATURES = {
     "dr_sr_get_diag": (_I, [_P, C.POINTER(Diag), _I]),
     "dr_sr_set_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
     "dr_sr_get_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
+    "dr_sr_autotune": (_I, [_P, _I, C.POINTER(LaunchCfg), _F]),
     "dr_sr_set_timing": (_I, [_P, _I]),
     "dr_sr_last_timing": (_I, [_P, _F, _I]),
     "dr_prsr_create": (_I, [C.POINTER(SrEnv), C.POINTER(SrCost), C.POINTER(PrsrInstance), _I, C.POINTER(_P)]),
@@ -300,6 +302,9 @@ SIGNATURES = {
     "dr_bm_step": (_I, [_P, _I, _I, _F]),
     "dr_bm_get_events": (_I, [_P, C.POINTER(C.c_int32), _I]),
     "dr_bm_get_noise": (_I, [_P, _I, _F, C.c_size_t]),
+    "dr_bm_set_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
+    "dr_bm_get_launch": (_I, [_P, C.POINTER(LaunchCfg)]),
+    "dr_bm_autotune": (_I, [_P, _I, C.POINTER(LaunchCfg), _F]),
     "dr_bm_get_costs_true": (_I, [_P, _I, _F, _I]),
     "dr_bm_get_branches": (_I, [_P, _I, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _I]),
     "dr_bm_get_costs": (_I, [_P, _I, _F, _I]),

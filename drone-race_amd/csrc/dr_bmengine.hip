@@ -157,6 +157,7 @@ struct dr_bm_engine
     hipGraphExec_t graph_exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     bool timing = false;
     bool record_lengths = false; // rollout lengths into d_rlen (a separate kernel instantiation)
+    int rollout_grid = 0;        // rollout workgroups (0: one per tile; else a persistent grid over the tiles)
     std::vector<hipEvent_t> events;
     int timed_steps = 0;
     float last_ms[BM_NEV] = {0, 0, 0, 0, 0, 0};
@@ -186,7 +187,7 @@ int dr_bm_engine::enqueue_a(hipEvent_t* ev)
     const int n = (int)inst.size();
     if (ev)
         HIPCHK(hipEventRecord(ev[0], stream));
-    HIPCHK(launch_bm_rollout(env_kind, max_m, max_tm, 0, stream, d_inst.p, d_dyn.p, d_tiles.p, (int)tiles.size(), d_env.p, cost, d_states.p,
+    HIPCHK(launch_bm_rollout(env_kind, max_m, max_tm, rollout_grid, stream, d_inst.p, d_dyn.p, d_tiles.p, (int)tiles.size(), d_env.p, cost, d_states.p,
                              d_knots.p, d_B.p, d_noise.p, d_ctrue.p, d_used.p, d_btime.p, record_lengths ? d_rlen.p : nullptr));
     if (ev)
         HIPCHK(hipEventRecord(ev[1], stream));
@@ -969,6 +970,115 @@ int dr_bm_set_record_lengths(dr_bm_engine* e, int on)
         }
     }
     e->record_lengths = on != 0;
+    return DR_OK;
+}
+
+static void bm_drop_graphs(dr_bm_engine* e)
+{
+    for (auto& row : e->graph_exec)
+        for (auto& g : row)
+            if (g)
+            {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+}
+
+int dr_bm_set_launch(dr_bm_engine* e, const dr_launch_cfg* cfg)
+{
+    if (!e || !cfg)
+        return host_fail(DR_EINVAL, "NULL argument");
+    if (cfg->rollout_grid < 0)
+        return host_fail(DR_EINVAL, "rollout_grid must be >= 0");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (cfg->rollout_grid != e->rollout_grid)
+        bm_drop_graphs(e);
+    e->rollout_grid = cfg->rollout_grid;
+    return DR_OK;
+}
+
+int dr_bm_get_launch(dr_bm_engine* e, dr_launch_cfg* cfg)
+{
+    if (!e || !cfg)
+        return host_fail(DR_EINVAL, "NULL argument");
+    *cfg = dr_launch_cfg{};
+    cfg->rollout_block = BM_BLOCK;
+    cfg->rollout_grid = e->rollout_grid;
+    cfg->reduce_chunk = BM_CHUNK;
+    cfg->use_graph = 1;
+    cfg->steps_per_graph = 1;
+    return DR_OK;
+}
+
+// Times the rollout grid candidates (one workgroup per tile, or a persistent
+// grid of 1, 2, 4 or 8 workgroups per CU) on the engine's own workload,
+// trials steps each after one warm-up step, no world step, and keeps the
+// fastest; the knots, plans and per-instance state the timing steps change
+// are restored.
+int dr_bm_autotune(dr_bm_engine* e, int trials, dr_launch_cfg* best, float* best_ms)
+{
+    if (!e || trials < 1)
+        return host_fail(DR_EINVAL, "autotune: bad handle or trials");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    std::vector<float> knots(e->knot_total), dense(e->dense_total);
+    std::vector<BmDyn> dyn(e->inst.size());
+    HIPCHK(hipMemcpy(knots.data(), e->d_knots.p, knots.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dense.data(), e->d_dense.p, dense.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dyn.data(), e->d_dyn.p, dyn.size() * sizeof(BmDyn), hipMemcpyDeviceToHost));
+    int n_cu = 256;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, e->device) == hipSuccess)
+            n_cu = prop.multiProcessorCount;
+    }
+    const int saved = e->rollout_grid;
+    int best_g = saved;
+    float best_t = 1e30f;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    int rc = DR_OK;
+    for (int g : {0, n_cu, 2 * n_cu, 4 * n_cu, 8 * n_cu})
+    {
+        if (g >= (int)e->tiles.size())
+            continue;
+        dr_launch_cfg c{};
+        c.rollout_grid = g;
+        if ((rc = dr_bm_set_launch(e, &c)))
+            break;
+        if ((rc = e->run(1, false)))
+            break;
+        float ms = 0;
+        if (hipEventRecord(a, e->stream) != hipSuccess)
+            rc = host_fail(DR_EHIP, "autotune: hipEventRecord failed");
+        if (!rc)
+            rc = e->run(trials, false);
+        if (!rc && (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                    hipEventElapsedTime(&ms, a, b) != hipSuccess))
+            rc = host_fail(DR_EHIP, "autotune: event timing failed");
+        if (rc)
+            break;
+        ms /= (float)trials;
+        if (ms < best_t)
+        {
+            best_t = ms;
+            best_g = g;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    dr_launch_cfg c{};
+    c.rollout_grid = rc ? saved : best_g;
+    dr_bm_set_launch(e, &c);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemcpy(e->d_knots.p, knots.data(), knots.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_dense.p, dense.data(), dense.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_dyn.p, dyn.data(), dyn.size() * sizeof(BmDyn), hipMemcpyHostToDevice));
+    if (best)
+        dr_bm_get_launch(e, best);
+    if (best_ms)
+        *best_ms = best_t;
     return DR_OK;
 }
 

@@ -832,6 +832,90 @@ int dr_pr_set_launch(dr_prengine* e, const dr_launch_cfg* cfg)
     return e->build_layout();
 }
 
+// Times the candidate launch shapes on the engine's own workload (trials
+// steps each after one warm-up step, no closed-loop advance) and keeps the
+// fastest; the plans and per-instance state the timing steps change are
+// restored.  Candidates: rollout waves per workgroup (every rollout of a
+// sample group, or 2, 4, 8) x weighted-average chunk x timestep group.
+int dr_pr_autotune(dr_prengine* e, int trials, dr_launch_cfg* best, float* best_ms)
+{
+    if (!e || trials < 1)
+        return host_fail(DR_EINVAL, "autotune: bad handle or trials");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const size_t n = e->inst.size();  // one PrDyn per instance; PR-StratRace keeps placeholders in inst
+    std::vector<float> work(e->plan_total), plan(e->plan_total);
+    std::vector<PrDyn> dyn(n);
+    HIPCHK(hipMemcpy(work.data(), e->d_work.p, work.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(plan.data(), e->d_plan.p, plan.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dyn.data(), e->d_dyn.p, dyn.size() * sizeof(PrDyn), hipMemcpyDeviceToHost));
+    const dr_launch_cfg saved = e->launch;
+    dr_launch_cfg best_cfg = saved;
+    float best_t = 1e30f;
+    std::vector<dr_launch_cfg> cands;
+    for (int rb : {0, 128, 256, 512})
+        for (int ch : {512, 1024, 2048})
+            for (int tg : {16, 32})
+            {
+                dr_launch_cfg c = saved;
+                c.rollout_block = rb;
+                c.reduce_chunk = ch;
+                c.reduce_tgroup = tg;
+                c.use_graph = 1;
+                c.steps_per_graph = 1;
+                cands.push_back(c);
+            }
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    int rc = DR_OK;
+    for (const dr_launch_cfg& c : cands)
+    {
+        rc = dr_pr_set_launch(e, &c);
+        if (rc)
+            break;
+        e->drop_graphs();
+        rc = e->run(1, false);
+        if (rc)
+            break;
+        float ms = 0;
+        if (hipEventRecord(a, e->stream) != hipSuccess)
+            rc = host_fail(DR_EHIP, "autotune: hipEventRecord failed");
+        if (!rc)
+            rc = e->run(trials, false);
+        if (!rc && (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                    hipEventElapsedTime(&ms, a, b) != hipSuccess))
+            rc = host_fail(DR_EHIP, "autotune: event timing failed");
+        if (rc)
+            break;
+        ms /= (float)trials;
+        if (ms < best_t)
+        {
+            best_t = ms;
+            best_cfg = c;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc)
+    {
+        dr_pr_set_launch(e, &saved);
+        e->drop_graphs();
+        return rc;
+    }
+    best_cfg.steps_per_graph = saved.steps_per_graph;
+    if ((rc = dr_pr_set_launch(e, &best_cfg)))
+        return rc;
+    e->drop_graphs();
+    HIPCHK(hipMemcpy(e->d_work.p, work.data(), work.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_plan.p, plan.data(), plan.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_dyn.p, dyn.data(), dyn.size() * sizeof(PrDyn), hipMemcpyHostToDevice));
+    if (best)
+        *best = best_cfg;
+    if (best_ms)
+        *best_ms = best_t;
+    return DR_OK;
+}
+
 int dr_pr_get_launch(dr_prengine* e, dr_launch_cfg* cfg)
 {
     if (!e || !cfg)

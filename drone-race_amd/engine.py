@@ -16,6 +16,14 @@ def _fptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
+def _autotune(fn, h, trials):
+    """Runs one engine's launch-shape search; returns (chosen launch fields, ms per step)."""
+    cfg = abi.LaunchCfg()
+    ms = C.c_float()
+    check(fn(h, int(trials), C.byref(cfg), C.byref(ms)))
+    return {f: getattr(cfg, f) for f, _ in abi.LaunchCfg._fields_}, ms.value
+
+
 class Engine:
     def __init__(self, scenario):
         lib = abi.load()
@@ -116,10 +124,7 @@ class Engine:
         return cfg
 
     def autotune(self, trials=5):
-        cfg = abi.LaunchCfg()
-        ms = C.c_float()
-        check(self.lib.dr_engine_autotune(self.h, trials, C.byref(cfg), C.byref(ms)))
-        return {f: getattr(cfg, f) for f, _ in abi.LaunchCfg._fields_}, ms.value
+        return _autotune(self.lib.dr_engine_autotune, self.h, trials)
 
     def set_eval(self, on):
         check(self.lib.dr_engine_set_eval(self.h, 1 if on else 0))
@@ -341,6 +346,10 @@ class PREngine:
         check(self.lib.dr_pr_get_launch(self.h, C.byref(cfg)))
         return cfg
 
+    def autotune(self, trials=5):
+        """Times the rollout/reduction launch shapes on this workload and keeps the fastest; state is restored."""
+        return _autotune(self.lib.dr_pr_autotune, self.h, trials)
+
     def set_timing(self, on=True):
         check(self.lib.dr_pr_set_timing(self.h, 1 if on else 0))
 
@@ -431,6 +440,15 @@ class SREngine:
         for k, v in kw.items():
             setattr(cfg, k, int(v))
         check(self.lib.dr_sr_set_launch(self.h, C.byref(cfg)))
+
+    def get_launch(self):
+        cfg = abi.LaunchCfg()
+        check(self.lib.dr_sr_get_launch(self.h, C.byref(cfg)))
+        return cfg
+
+    def autotune(self, trials=5):
+        """Times the rollout/reduction launch shapes on this workload and keeps the fastest; state is restored."""
+        return _autotune(self.lib.dr_sr_autotune, self.h, trials)
 
     def set_timing(self, on=True):
         check(self.lib.dr_sr_set_timing(self.h, 1 if on else 0))
@@ -617,6 +635,21 @@ class BMEngine:
         """Record the rollout lengths of the following steps (a slower rollout instantiation)."""
         check(self.lib.dr_bm_set_record_lengths(self.h, 1 if on else 0))
         self.recording = bool(on)
+
+    def set_launch(self, rollout_grid=0):
+        """rollout_grid 0: one workgroup per rollout tile; > 0: that many persistent workgroups."""
+        cfg = self.get_launch()
+        cfg.rollout_grid = int(rollout_grid)
+        check(self.lib.dr_bm_set_launch(self.h, C.byref(cfg)))
+
+    def get_launch(self):
+        cfg = abi.LaunchCfg()
+        check(self.lib.dr_bm_get_launch(self.h, C.byref(cfg)))
+        return cfg
+
+    def autotune(self, trials=5):
+        """Times the rollout grid shapes on this workload and keeps the fastest; state is restored."""
+        return _autotune(self.lib.dr_bm_autotune, self.h, trials)
 
     def rollout_lengths(self, i):
         """Environment steps each rollout of the last step ran [n_models][K] (recording on)."""

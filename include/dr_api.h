@@ -343,6 +343,11 @@ int dr_pr_synchronize(dr_prengine* e);
  * rollout_split are unused. */
 int dr_pr_set_launch(dr_prengine* e, const dr_launch_cfg* cfg);
 int dr_pr_get_launch(dr_prengine* e, dr_launch_cfg* cfg);
+/* Times the launch shapes (rollout waves per workgroup x reduce_chunk x
+ * reduce_tgroup) on the engine's own workload, trials steps each, and keeps
+ * the fastest; the plans and state the timing steps change are restored.
+ * No reference counterpart (its block size is fixed, prmppi_controller.cu:185). */
+int dr_pr_autotune(dr_prengine* e, int trials, dr_launch_cfg* best, float* best_ms);
 int dr_pr_set_timing(dr_prengine* e, int enable);
 /* Device ms summed over the last timed run: rollout, aggregation, weighted
  * averages, update (+ closed-loop advance), whole step. */
@@ -470,6 +475,9 @@ int dr_sr_get_diag(dr_sr_engine* e, dr_diag* diag, int n);
  * steps_per_graph are used. */
 int dr_sr_set_launch(dr_sr_engine* e, const dr_launch_cfg* cfg);
 int dr_sr_get_launch(dr_sr_engine* e, dr_launch_cfg* cfg);
+/* As dr_pr_autotune over rollout_block x reduce_chunk x reduce_tgroup.  No
+ * reference counterpart (mppi_controller.cu:277 fixes the block). */
+int dr_sr_autotune(dr_sr_engine* e, int trials, dr_launch_cfg* best, float* best_ms);
 int dr_sr_set_timing(dr_sr_engine* e, int enable);
 /* Device ms summed over the last timed run: rollout, weighted average, update + world step, whole step. */
 int dr_sr_last_timing(dr_sr_engine* e, float* ms, int n);
@@ -539,6 +547,15 @@ int dr_bm_get_events(dr_bm_engine* e, int32_t* events, int n);
  * branching time per model [n_models][K]; aggregated costs [n_models + 1][K];
  * masked minima [n_models + 1][M]; nu [n_models + 1]. */
 int dr_bm_get_noise(dr_bm_engine* e, int instance, float* noise, size_t n);
+/* Launch shape of the branching rollout: rollout_grid = 0 runs one workgroup
+ * per rollout tile, > 0 a persistent grid of that many workgroups looping over
+ * the tiles; the other fields are ignored (get_launch reports the fixed
+ * ones).  dr_bm_autotune times rollout_grid in {0, 1, 2, 4, 8 per CU} on the
+ * engine's own workload, trials steps each, keeps the fastest and restores the
+ * knots, plans and state the timing steps change.  No reference counterpart. */
+int dr_bm_set_launch(dr_bm_engine* e, const dr_launch_cfg* cfg);
+int dr_bm_get_launch(dr_bm_engine* e, dr_launch_cfg* cfg);
+int dr_bm_autotune(dr_bm_engine* e, int trials, dr_launch_cfg* best, float* best_ms);
 int dr_bm_get_costs_true(dr_bm_engine* e, int instance, float* costs, int n);
 int dr_bm_get_branches(dr_bm_engine* e, int instance, int32_t* used, int32_t* time, int n);
 int dr_bm_get_costs(dr_bm_engine* e, int instance, float* costs, int n);

@@ -433,15 +433,19 @@ def tune_launch(eng, autotune):
     return {"launch": cfg, "ms_per_step": ms}
 
 
-def pr_mppi(pkg, steps=100, warmup=10, autotune=True):
+def pr_mppi(pkg, steps=100, warmup=10, autotune=True, launch=None):
     """PR-MPPI, 4 drones, B ~ U{3..6} opponent-velocity branches per controller,
     Dirichlet(1) beliefs, K = 8192 samples evaluated under every branch for
-    both plans, T = 80, closed loop, graph replays."""
+    both plans, T = 80, closed loop, graph replays; launch shape given, or autotuned."""
     sc = pkg.scenarios
     scn = sc.c3_pr_four_drones()
     n = len(scn.instances)
     eng = pkg.PREngine(scn)
-    tuned = tune_launch(eng, autotune)
+    if launch:
+        eng.set_launch(**launch)
+        tuned = {"launch": launch, "ms_per_step": None}
+    else:
+        tuned = tune_launch(eng, autotune)
     eng.step(warmup, advance=True)
     t0 = time.perf_counter()
     eng.step(steps, advance=True)
@@ -735,6 +739,7 @@ def main():
     ap.add_argument("--c4-legs", default="", help="comma-separated subset of the 8-drone legs to run with --adjoint-only")
     ap.add_argument("--c4-launch", default="", help="launch shape of the 8-drone legs as JSON (skips their autotuner)")
     ap.add_argument("--pr-only", action="store_true", help="run only the PR-MPPI section (for profiling)")
+    ap.add_argument("--pr-launch", default="", help="launch shape of the PR-MPPI section as JSON (skips its autotuner)")
     ap.add_argument("--sr-only", action="store_true", help="run only the StratRace2D section (for profiling)")
     ap.add_argument("--bm-only", action="store_true", help="run only the branching spline MPPI section (for profiling)")
     ap.add_argument("--session-only", action="store_true", help="run only the wire-protocol session section")
@@ -788,7 +793,9 @@ def main():
               flush=True)
         return
     if args.pr_only:
-        print(json.dumps({"pr_mppi": pr_mppi(pkg, steps=args.steps, warmup=args.warmup, autotune=not args.no_autotune)}), flush=True)
+        launch = json.loads(args.pr_launch) if args.pr_launch else None
+        print(json.dumps({"pr_mppi": pr_mppi(pkg, steps=args.steps, warmup=args.warmup, autotune=not args.no_autotune, launch=launch)}),
+              flush=True)
         return
     if args.plain_only:
         launch = json.loads(args.launch) if args.launch else None

@@ -134,10 +134,13 @@ def refine_oracle(oracle, scn, opp, U, n_steps, step, ck=10):
     return Us, np.array(Js)
 
 
-def check_refined(eng, act, Us, Js):
+def check_refined(eng, act, Us, Js, tol=1e-5):
     """The engine's shifted nominal and action against the iterate of lowest J
-    (within 1e-4 relative); iterates whose J is within rounding (1e-5
-    relative) of the lowest are all acceptable.  Returns (lowest, matched)."""
+    (within tol relative: the forward recurrences run as prefix scans, the
+    same arithmetic in another order, and each gradient step multiplies the
+    gradient's rounding by the step size); iterates whose J is within rounding
+    (1e-5 relative) of the lowest are all acceptable.  Returns (lowest,
+    matched)."""
     best = int(np.argmin(Js))
     tie = 1e-5 * max(1.0, abs(Js[best]))
     ok = [k for k in range(len(Js)) if Js[k] <= Js[best] + tie]
@@ -145,8 +148,8 @@ def check_refined(eng, act, Us, Js):
     errs = {k: np.max(np.abs(got - Us[k][1:]), initial=0.0) / max(1.0, np.abs(Us[k]).max()) for k in ok}  # T = 1: action only
     k = min(errs, key=errs.get)
     say(f"  J of the iterates {np.array2string(Js, precision=4)}; lowest {best}; engine kept {k} (max rel err {errs[k]:.2e})")
-    assert errs[k] < 1e-4
-    assert np.abs(act[0] - Us[k][0]).max() / max(1.0, np.abs(Us[k][0]).max()) < 1e-4
+    assert errs[k] < tol
+    assert np.abs(act[0] - Us[k][0]).max() / max(1.0, np.abs(Us[k][0]).max()) < tol
     return best, k
 
 
@@ -157,7 +160,9 @@ def test_refined_step_matches_oracle(pkg, oracle, step, kept):
     step lowers J; at step 5 all three overshoot and the plain update is kept;
     at step 50 the second iterate is the lowest (J of the iterates, FP32
     oracle: [-88.7, -103.0, -107.4, -109.0], [-88.7, 159.7, 135.4, 109.1],
-    [-88.7, 77.8, -92.4, 72.0])."""
+    [-88.7, 77.8, -92.4, 72.0]).  Plans within 1e-5 relative of the oracle's
+    iterate up to step 5; at step 50 the two gradient steps multiply the
+    gradient's rounding 50-fold, and the bar is 1e-4."""
     say(f"case: adjoint-refined control step, step {step}")
     sc = pkg.scenarios
     T, K = 60, 2048
@@ -172,7 +177,7 @@ def test_refined_step_matches_oracle(pkg, oracle, step, kept):
                                      np.array(ins.sigma[:], np.float32))
     upd = oracle.update(0, scn.params, costs, noise, nom0, ins.lambda_, np.array(ins.shift_fill[:], np.float32))
     Us, Js = refine_oracle(oracle, scn, opp, upd["updated"].astype(np.float32), 3, step)
-    best, _ = check_refined(eng, act, Us, Js)
+    best, _ = check_refined(eng, act, Us, Js, tol=1e-5 if step <= 5.0 else 1e-4)
     assert best == kept
     eng.close()
 

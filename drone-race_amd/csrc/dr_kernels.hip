@@ -431,17 +431,21 @@ __global__ __launch_bounds__(256) void k_rollout_p4(const InstDev* __restrict__ 
 // SIMD four instruction streams.  Stages and the chunk each works on at
 // chunk period i (stages joined by LDS rings of CH steps, one barrier per
 // period):
-//   P   Philox4x32-10 words, even steps / odd steps (two waves)   chunk i
+//   P   Philox4x32-10 words                                        chunk i
 //   B   Box-Muller, sigma, noise row store, even / odd (two)      chunk i - 1
 //   R   clamp(nominal + eps), body rates and attitude             chunk i - 2
 //   X   velocity, position from the pre-step attitude, gate       chunk i - 3
-//   C   running cost and terminal test, even / odd steps (two)    chunk i - 4
-//   S   (with the even cost wave) discounted sum in step order,
-//       stop at the first terminal state                          chunk i - 5
-// Wave w works as stage (w + blockIdx) mod 8 in the order P0 P1 B0 B1 C1 C0
-// X R: waves w and w + 4 share a SIMD, and the pairs (P, C), (B, X), (B, R)
-// carry about equal work whatever the rotation.  The arithmetic of every
-// sample is that of k_rollout.
+//   C   running cost and terminal test                            chunk i - 4
+//   S   discounted sum in step order, stop at the first terminal  chunk i - 5
+// A chunk period lasts as long as its busiest wave (one wave issues about
+// one VALU instruction per 8 cycles), so the work is dealt out by count:
+// waves 0-3 each make the Philox words and the running cost of the steps
+// u = w mod 4 of their chunks (the running cost with 7 opponents is the
+// largest stage), waves 4-5 the Box-Muller of even / odd steps, wave 6 the
+// attitude and the discounted sum, wave 7 the translation.  Wave w works as
+// (w + blockIdx) mod 8; waves w and w + 4 share a SIMD, so each SIMD pairs a
+// cost wave with a Box-Muller, attitude or translation wave whatever the
+// rotation.  The arithmetic of every sample is that of k_rollout.
 template <int CH>
 __global__ __launch_bounds__(512, 4) void k_rollout_p8(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
                                                        const RTile* __restrict__ tiles, int n_tiles, const dr_track* __restrict__ tracks,
@@ -462,11 +466,9 @@ __global__ __launch_bounds__(512, 4) void k_rollout_p8(const InstDev* __restrict
 
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
-    // stage of this wave: 0 P-even 1 P-odd 2 B-even 3 B-odd 4 C-odd 5 C-even(+S) 6 X 7 R
+    // 0-3 P + C of steps u = role mod 4, 4 B-even, 5 B-odd, 6 R + S, 7 X
     const int role = ((tid / WAVE) + (int)blockIdx.x) & 7;
-    constexpr int kStage[8] = {0, 0, 1, 1, 4, 4, 3, 2};
-    const int stage = kStage[role];
-    const int parity = (role == 1 || role == 3 || role == 4) ? 1 : 0;
+    const int parity = role & 1; // B waves
     for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
     {
         const RTile rt = tiles[tile];
@@ -507,124 +509,139 @@ __global__ __launch_bounds__(512, 4) void k_rollout_p8(const InstDev* __restrict
 
         for (int i = 0; i < n_chunks + 5; ++i)
         {
-            const int c = i - stage;
-            if (live && c >= 0 && c < n_chunks)
+            if (live)
             {
-                if (stage == 0)
+                if (role < 4)
                 {
-#pragma unroll
-                    for (int u = parity; u < CH; u += 2)
+                    if (i < n_chunks)
                     {
-                        U4 ctr;
-                        ctr.x = (uint32_t)k;
-                        ctr.y = (uint32_t)(c * CH + u);
-                        ctr.z = step;
-                        ctr.w = STREAM_CONTROL;
-                        const U4 w = philox4x32_10(ctr, key0, key1);
-                        r_w[c & 1][u][lane] = make_uint4(w.x, w.y, w.z, w.w);
-                    }
-                }
-                else if (stage == 1)
-                {
-                    float4* nrow = noise + I.noise_off + (long long)k * T;
 #pragma unroll
-                    for (int u = parity; u < CH; u += 2)
-                    {
-                        const int t = c * CH + u;
-                        const uint4 w = r_w[c & 1][u][lane];
-                        float z[4];
-                        box_muller(w.x, w.y, &z[0], &z[1]);
-                        box_muller(w.z, w.w, &z[2], &z[3]);
-                        const float4 e = make_float4(sg0 * z[0], sg1 * z[1], sg2 * z[2], sg3 * z[3]);
-                        r_e[c & 1][u][lane] = e;
-                        if (t < T)
-                            nrow[t] = e;
-                    }
-                }
-                else if (stage == 2)
-                {
-#pragma unroll
-                    for (int u = 0; u < CH; ++u)
-                    {
-                        const int t = c * CH + u;
-                        if (t < T)
+                        for (int u = role; u < CH; u += 4)
                         {
-                            const float4 e = r_e[c & 1][u][lane];
-                            float uin[4], uc[4];
-                            uin[0] = s_nom[t * 4 + 0] + e.x;
-                            uin[1] = s_nom[t * 4 + 1] + e.y;
-                            uin[2] = s_nom[t * 4 + 2] + e.z;
-                            uin[3] = s_nom[t * 4 + 3] + e.w;
-                            clamp_control(C, uin, uc);
-                            r_q[c & 1][u][lane] = make_float4(s.q[0], s.q[1], s.q[2], s.q[3]);
-                            r_f[c & 1][u][lane] = uc[0];
-                            quad_attitude_step(C, s.q, s.w, uc);
+                            U4 ctr;
+                            ctr.x = (uint32_t)k;
+                            ctr.y = (uint32_t)(i * CH + u);
+                            ctr.z = step;
+                            ctr.w = STREAM_CONTROL;
+                            const U4 w = philox4x32_10(ctr, key0, key1);
+                            r_w[i & 1][u][lane] = make_uint4(w.x, w.y, w.z, w.w);
+                        }
+                    }
+                    const int c = i - 4;
+                    if (c >= 0 && c < n_chunks)
+                    {
+#pragma unroll
+                        for (int u = role; u < CH; u += 4)
+                        {
+                            const int t = c * CH + u;
+                            if (t < T)
+                            {
+                                const float4 v = r_s[c % 3][u][lane];
+                                const uint32_t code = f2u(v.w);
+                                QState x;
+                                x.p[0] = v.x;
+                                x.p[1] = v.y;
+                                x.p[2] = v.z;
+                                x.gate = (int)(code & 0xffu);
+                                x.laps = (int)code >> 8;
+                                int term;
+                                const float cst = stage_cost(C, Q, KC, s_track, x, s_opp + t * opp_stride, n_opp, &term);
+                                r_c[c & 1][u][lane] = make_float2(cst, u2f((uint32_t)term));
+                            }
                         }
                     }
                 }
-                else if (stage == 3)
+                else if (role < 6)
                 {
-#pragma unroll
-                    for (int u = 0; u < CH; ++u)
+                    const int c = i - 1;
+                    if (c >= 0 && c < n_chunks)
                     {
-                        const int t = c * CH + u;
-                        if (t < T)
+                        float4* nrow = noise + I.noise_off + (long long)k * T;
+#pragma unroll
+                        for (int u = parity; u < CH; u += 2)
                         {
-                            const float4 q4 = r_q[c & 1][u][lane];
-                            const float q[4] = {q4.x, q4.y, q4.z, q4.w};
-                            const float uu[4] = {r_f[c & 1][u][lane], 0.0f, 0.0f, 0.0f}; // the translation reads the thrust only
-                            const float p0[3] = {s.p[0], s.p[1], s.p[2]};
-                            quad_translation_step(C, q, s.p, s.v, uu);
-                            update_gate(s_track.gates, s_track.n_gates, KC.gate_traversal_margin, p0, s);
-                            r_s[c % 3][u][lane] = make_float4(s.p[0], s.p[1], s.p[2], u2f((uint32_t)(s.gate | (s.laps << 8))));
+                            const int t = c * CH + u;
+                            const uint4 w = r_w[c & 1][u][lane];
+                            float z[4];
+                            box_muller(w.x, w.y, &z[0], &z[1]);
+                            box_muller(w.z, w.w, &z[2], &z[3]);
+                            const float4 e = make_float4(sg0 * z[0], sg1 * z[1], sg2 * z[2], sg3 * z[3]);
+                            r_e[c & 1][u][lane] = e;
+                            if (t < T)
+                                nrow[t] = e;
+                        }
+                    }
+                }
+                else if (role == 6)
+                {
+                    const int c = i - 2;
+                    if (c >= 0 && c < n_chunks)
+                    {
+#pragma unroll
+                        for (int u = 0; u < CH; ++u)
+                        {
+                            const int t = c * CH + u;
+                            if (t < T)
+                            {
+                                const float4 e = r_e[c & 1][u][lane];
+                                float uin[4], uc[4];
+                                uin[0] = s_nom[t * 4 + 0] + e.x;
+                                uin[1] = s_nom[t * 4 + 1] + e.y;
+                                uin[2] = s_nom[t * 4 + 2] + e.z;
+                                uin[3] = s_nom[t * 4 + 3] + e.w;
+                                clamp_control(C, uin, uc);
+                                r_q[c & 1][u][lane] = make_float4(s.q[0], s.q[1], s.q[2], s.q[3]);
+                                r_f[c & 1][u][lane] = uc[0];
+                                quad_attitude_step(C, s.q, s.w, uc);
+                            }
+                        }
+                    }
+                    // S: the chunk costed one period earlier, in step order
+                    const int cs = i - 5;
+                    if (cs >= 0 && cs < n_chunks)
+                    {
+#pragma unroll
+                        for (int u = 0; u < CH; ++u)
+                        {
+                            const int t = cs * CH + u;
+                            if (t < T && !stop)
+                            {
+                                const float2 ct = r_c[cs & 1][u][lane];
+                                last = r_s[cs % 3][u][lane];
+                                acc = fma_(ct.x, decay, acc);
+                                decay = decay * DECAY;
+                                stop = ((int)f2u(ct.y) != TERM_NONE);
+                            }
                         }
                     }
                 }
                 else
                 {
-#pragma unroll
-                    for (int u = parity; u < CH; u += 2)
+                    const int c = i - 3;
+                    if (c >= 0 && c < n_chunks)
                     {
-                        const int t = c * CH + u;
-                        if (t < T)
+#pragma unroll
+                        for (int u = 0; u < CH; ++u)
                         {
-                            const float4 v = r_s[c % 3][u][lane];
-                            const uint32_t code = f2u(v.w);
-                            QState x;
-                            x.p[0] = v.x;
-                            x.p[1] = v.y;
-                            x.p[2] = v.z;
-                            x.gate = (int)(code & 0xffu);
-                            x.laps = (int)code >> 8;
-                            int term;
-                            const float cst = stage_cost(C, Q, KC, s_track, x, s_opp + t * opp_stride, n_opp, &term);
-                            r_c[c & 1][u][lane] = make_float2(cst, u2f((uint32_t)term));
+                            const int t = c * CH + u;
+                            if (t < T)
+                            {
+                                const float4 q4 = r_q[c & 1][u][lane];
+                                const float q[4] = {q4.x, q4.y, q4.z, q4.w};
+                                const float uu[4] = {r_f[c & 1][u][lane], 0.0f, 0.0f, 0.0f}; // the translation reads the thrust only
+                                const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+                                quad_translation_step(C, q, s.p, s.v, uu);
+                                update_gate(s_track.gates, s_track.n_gates, KC.gate_traversal_margin, p0, s);
+                                r_s[c % 3][u][lane] = make_float4(s.p[0], s.p[1], s.p[2], u2f((uint32_t)(s.gate | (s.laps << 8))));
+                            }
                         }
-                    }
-                }
-            }
-            // S: the even cost wave sums the chunk costed one period earlier
-            const int cs = i - 5;
-            if (role == 5 && live && cs >= 0 && cs < n_chunks)
-            {
-#pragma unroll
-                for (int u = 0; u < CH; ++u)
-                {
-                    const int t = cs * CH + u;
-                    if (t < T && !stop)
-                    {
-                        const float2 ct = r_c[cs & 1][u][lane];
-                        last = r_s[cs % 3][u][lane];
-                        acc = fma_(ct.x, decay, acc);
-                        decay = decay * DECAY;
-                        stop = ((int)f2u(ct.y) != TERM_NONE);
                     }
                 }
             }
             __syncthreads();
         }
 
-        if (role == 5)
+        if (role == 6)
         {
             float cost = __builtin_inff();
             if (live)

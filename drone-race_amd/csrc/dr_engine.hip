@@ -36,13 +36,15 @@ hipError_t launch_wavg(int chunk, int n_wtiles, hipStream_t st, const InstDev* i
                        const float* costs, const float4* noise, const float* tmin, double* part, double* den);
 hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
                          float* nominal, float* actions, const QuadConst& C, int advance, dr_drone_state* drones, const int* drone_inst,
-                         int n_drones, const dr_track* tracks, float margin, int mode);
+                         int n_drones, const dr_track* tracks, float margin);
 hipError_t launch_costgrad(int lanes, hipStream_t st, const InstDev* insts, int inst, int T, int n_ck, int ck,
                            const dr_track* tracks, const dr_drone_state* drones, const float* U, int n_seq, float* J, float* G,
                            const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, int opp_stride);
 hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
                          const dr_track* tracks, const dr_drone_state* drones, float* nominal, int n_steps, float step,
-                         const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC);
+                         const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, float* actions, int advance,
+                         const int* drone_inst, int n_drones, dr_drone_state* drones_out, int noise_blocks, float4* noise);
+hipError_t launch_noise(int blocks, hipStream_t st, const InstDev* insts, const InstDyn* dyn, int n_inst, float4* noise);
 
 } // namespace dr
 
@@ -111,6 +113,11 @@ struct dr_engine
     int refine_steps = 0; // adjoint gradient steps per control step (0: plain MPPI)
     float refine_step = 0.05f;
     int refine_ck = 10;
+    // rollout_split 4 (with the refinement on): the noise rows the next
+    // rollout reads are made by the refinement's spare workgroups; true while
+    // the noise buffer holds the rows of every instance's current step
+    bool noise_ready = false;
+    int n_cu = 256;
     DevBuf<float> d_gU, d_gJ, d_gG; // scratch of dr_engine_cost_grad
     QuadConst C{};
     CostConst Q{};
@@ -138,7 +145,12 @@ struct dr_engine
     DevBuf<RTile> d_rtiles;
     DevBuf<WTile> d_wtiles;
     DevBuf<dr_track> d_tracks;
+    // Two copies of the world's drone states.  The plain step moves the drones
+    // in place; the refined step reads copy `cur` and writes the moved drones
+    // into the other copy (the refinement's workgroups read every drone while
+    // others finish), which then becomes current.
     DevBuf<dr_drone_state> d_drones;
+    int cur = 0;
     DevBuf<int> d_drone_inst;
     DevBuf<float> d_nominal;
     DevBuf<float> d_costs;
@@ -148,8 +160,11 @@ struct dr_engine
     DevBuf<double> d_den;
     DevBuf<float> d_actions;
 
-    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
-    int graph_steps[2] = {0, 0};
+    // graphs by (advance, parity of the drone copy they start from, size):
+    // size 0 holds launch.steps_per_graph steps, size 1 one step (the
+    // remainder of a run, and runs shorter than steps_per_graph)
+    hipGraphExec_t graph_exec[4][2] = {};
+    int graph_steps[4][2] = {};
 
     bool timing = false;
     std::vector<hipEvent_t> events;
@@ -173,17 +188,28 @@ struct dr_engine
 
     void drop_graphs()
     {
-        for (int i = 0; i < 2; ++i)
-        {
-            if (graph_exec[i])
-                (void)hipGraphExecDestroy(graph_exec[i]);
-            graph_exec[i] = nullptr;
-            graph_steps[i] = 0;
-        }
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 2; ++j)
+            {
+                if (graph_exec[i][j])
+                    (void)hipGraphExecDestroy(graph_exec[i][j]);
+                graph_exec[i][j] = nullptr;
+                graph_steps[i][j] = 0;
+            }
     }
 
+    dr_drone_state* drones_at(int p) const { return d_drones.p + (size_t)p * n_drones; }
+    // whether a step moves the drones into the other copy
+    bool flips(bool advance) const { return refine_steps > 0 && advance; }
+    // whether the rollout reads noise rows made ahead (rollout_split 4 needs
+    // the refinement, whose idle CUs make them; without it split 4 runs as 3)
+    bool noise_ahead() const { return launch.rollout_split == 4 && refine_steps > 0; }
+    int rollout_split() const { return launch.rollout_split == 4 && refine_steps == 0 ? 3 : launch.rollout_split; }
+
     int build_layout();
-    int enqueue_step(bool advance, int timed_index);
+    int enqueue_step(bool advance, int timed_index, int p);
+    hipGraphExec_t graph(bool advance, int p, int spg);
+    int build_graph(bool advance, int p, int spg);
     int run(int n_steps, bool advance);
 };
 
@@ -280,9 +306,12 @@ int dr_engine::build_layout()
     return DR_OK;
 }
 
-int dr_engine::enqueue_step(bool advance, int timed_index)
+// One control step of the batch starting from drone copy p.
+int dr_engine::enqueue_step(bool advance, int timed_index, int p)
 {
     const int n = (int)inst.size();
+    dr_drone_state* const drones = drones_at(p);
+    dr_drone_state* const moved = flips(advance) ? drones_at(p ^ 1) : drones;
     hipEvent_t* ev = (timed_index >= 0) ? &events[(size_t)timed_index * kStepEvents] : nullptr;
     if (ev)
         HIPCHK(hipEventRecord(ev[0], stream));
@@ -290,11 +319,11 @@ int dr_engine::enqueue_step(bool advance, int timed_index)
     {
         HIPCHK(hipEventRecord(ev_fork, stream));
         HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
-        HIPCHK(launch_eval(n, side, d_inst.p, d_dyn.p, d_tracks.p, d_drones.p, d_nominal.p, C, Q, cost, max_T, opp_stride));
+        HIPCHK(launch_eval(n, side, d_inst.p, d_dyn.p, d_tracks.p, drones, d_nominal.p, C, Q, cost, max_T, opp_stride));
         HIPCHK(hipEventRecord(ev_join, side));
     }
-    HIPCHK(launch_rollout(launch.rollout_block, launch.noise_batch, launch.rollout_split, launch.rollout_grid, stream, d_inst.p,
-                          d_dyn.p, d_rtiles.p, (int)rtiles.size(), d_tracks.p, d_drones.p, d_nominal.p, d_costs.p, d_noise.p, d_tmin.p, C, Q,
+    HIPCHK(launch_rollout(launch.rollout_block, launch.noise_batch, rollout_split(), launch.rollout_grid, stream, d_inst.p,
+                          d_dyn.p, d_rtiles.p, (int)rtiles.size(), d_tracks.p, drones, d_nominal.p, d_costs.p, d_noise.p, d_tmin.p, C, Q,
                           cost, max_T, opp_stride));
     if (ev)
         HIPCHK(hipEventRecord(ev[1], stream));
@@ -306,15 +335,18 @@ int dr_engine::enqueue_step(bool advance, int timed_index)
         HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
     if (refine_steps > 0)
     {
-        // the MPPI update runs inside the refinement kernel
+        // the MPPI update, the action, the shift and the closed-loop step run
+        // inside the refinement kernel
         if (ev)
             HIPCHK(hipEventRecord(ev[3], stream));
-        HIPCHK(launch_refine(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_tracks.p, d_drones.p, d_nominal.p, refine_steps,
-                             refine_step, C, Q, cost));
+        // with noise made ahead, the workgroups past the instances (one per
+        // CU the refinement leaves idle) make the next step's noise rows
+        const int noise_blocks = noise_ahead() ? std::max(n_cu - n, 64) : 0;
+        HIPCHK(launch_refine(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_tracks.p, drones, d_nominal.p, refine_steps,
+                             refine_step, C, Q, cost, d_actions.p, advance ? 1 : 0, d_drone_inst.p, n_drones, moved, noise_blocks,
+                             d_noise.p));
         if (ev)
             HIPCHK(hipEventRecord(ev[4], stream));
-        HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, advance ? 1 : 0, d_drones.p,
-                             d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin, 2));
     }
     else
     {
@@ -323,13 +355,56 @@ int dr_engine::enqueue_step(bool advance, int timed_index)
             HIPCHK(hipEventRecord(ev[3], stream));
             HIPCHK(hipEventRecord(ev[4], stream));
         }
-        HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, advance ? 1 : 0, d_drones.p,
-                             d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin, 0));
+        HIPCHK(launch_update(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_nominal.p, d_actions.p, C, advance ? 1 : 0, drones,
+                             d_drone_inst.p, n_drones, d_tracks.p, cost.gate_traversal_margin));
     }
     if (exec_cost && advance)
-        HIPCHK(launch_exec(n, stream, d_inst.p, d_dyn.p, d_tracks.p, d_drones.p, C, Q, cost));
+        HIPCHK(launch_exec(n, stream, d_inst.p, d_dyn.p, d_tracks.p, moved, C, Q, cost));
     if (ev)
         HIPCHK(hipEventRecord(ev[5], stream));
+    return DR_OK;
+}
+
+// The graph of spg control steps that starts from drone copy p (captured once
+// per launch shape).
+hipGraphExec_t dr_engine::graph(bool advance, int p, int spg)
+{
+    const int gi = (advance ? 2 : 0) + p, sz = spg == 1 ? 1 : 0;
+    return graph_steps[gi][sz] == spg ? graph_exec[gi][sz] : nullptr;
+}
+
+int dr_engine::build_graph(bool advance, int p, int spg)
+{
+    const int gi = (advance ? 2 : 0) + p, sz = spg == 1 ? 1 : 0;
+    hipGraphExec_t& ge = graph_exec[gi][sz];
+    if (ge && graph_steps[gi][sz] == spg)
+        return DR_OK;
+    if (ge)
+        (void)hipGraphExecDestroy(ge);
+    ge = nullptr;
+    graph_steps[gi][sz] = 0;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int rc = DR_OK;
+    for (int s = 0; s < spg && rc == DR_OK; ++s)
+    {
+        rc = enqueue_step(advance, -1, p);
+        p ^= flips(advance) ? 1 : 0;
+    }
+    hipError_t ce = hipStreamEndCapture(stream, &g);
+    if (rc)
+    {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ce != hipSuccess)
+        return fail(DR_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess)
+        return fail(DR_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    graph_steps[gi][sz] = spg;
     return DR_OK;
 }
 
@@ -338,6 +413,14 @@ int dr_engine::run(int n_steps, bool advance)
     if (n_steps < 0)
         return fail(DR_EINVAL, "n_steps < 0");
     timed_steps = 0;
+    if (!noise_ahead())
+        noise_ready = false; // the rollouts write the current step's rows themselves
+    else if (!noise_ready && n_steps > 0)
+    {
+        // the first step's rows; every step then makes the next step's
+        HIPCHK(launch_noise(n_cu * 4, stream, d_inst.p, d_dyn.p, (int)inst.size(), d_noise.p));
+        noise_ready = true;
+    }
     if (timing)
     {
         const size_t need = (size_t)n_steps * kStepEvents;
@@ -349,9 +432,10 @@ int dr_engine::run(int n_steps, bool advance)
         }
         for (int s = 0; s < n_steps; ++s)
         {
-            int rc = enqueue_step(advance, s);
+            int rc = enqueue_step(advance, s, cur);
             if (rc)
                 return rc;
+            cur ^= flips(advance) ? 1 : 0;
         }
         timed_steps = n_steps;
         return DR_OK;
@@ -360,50 +444,26 @@ int dr_engine::run(int n_steps, bool advance)
     {
         for (int s = 0; s < n_steps; ++s)
         {
-            int rc = enqueue_step(advance, -1);
+            int rc = enqueue_step(advance, -1, cur);
             if (rc)
                 return rc;
+            cur ^= flips(advance) ? 1 : 0;
         }
         return DR_OK;
     }
-    const int gi = advance ? 1 : 0;
+    // graphs of steps_per_graph steps, the remainder as one-step graphs; a
+    // graph of an odd number of flipping steps ends on the other drone copy,
+    // whose graph the next launch uses
     const int spg = std::max(1, launch.steps_per_graph);
-    if (!graph_exec[gi] || graph_steps[gi] != spg)
-    {
-        if (graph_exec[gi])
-            (void)hipGraphExecDestroy(graph_exec[gi]);
-        graph_exec[gi] = nullptr;
-        hipGraph_t g = nullptr;
-        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        int rc = DR_OK;
-        for (int s = 0; s < spg && rc == DR_OK; ++s)
-            rc = enqueue_step(advance, -1);
-        hipError_t ce = hipStreamEndCapture(stream, &g);
-        if (rc)
-        {
-            if (g)
-                (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (ce != hipSuccess)
-            return fail(DR_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-        hipError_t ie = hipGraphInstantiate(&graph_exec[gi], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ie != hipSuccess)
-            return fail(DR_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
-        graph_steps[gi] = spg;
-    }
     int done = 0;
-    while (done + spg <= n_steps)
+    while (done < n_steps)
     {
-        HIPCHK(hipGraphLaunch(graph_exec[gi], stream));
-        done += spg;
-    }
-    for (; done < n_steps; ++done)
-    {
-        int rc = enqueue_step(advance, -1);
-        if (rc)
+        const int sz = n_steps - done >= spg ? spg : 1;
+        if (int rc = build_graph(advance, cur, sz))
             return rc;
+        HIPCHK(hipGraphLaunch(graph(advance, cur, sz), stream));
+        done += sz;
+        cur ^= (flips(advance) && (sz & 1)) ? 1 : 0;
     }
     return DR_OK;
 }
@@ -476,6 +536,11 @@ int dr_engine_create(const dr_quad_params* params, const dr_cost_cfg* cost, cons
     };
     if (hipGetDevice(&e->device) != hipSuccess)
         return bail(fail(DR_EHIP, "no HIP device"));
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, e->device) == hipSuccess && prop.multiProcessorCount > 0)
+            e->n_cu = prop.multiProcessorCount;
+    }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -503,7 +568,7 @@ int dr_engine_create(const dr_quad_params* params, const dr_cost_cfg* cost, cons
         return true;
     };
     if (!chk(e->d_dyn.alloc(n_instances), "alloc dyn") || !chk(e->d_tracks.alloc(n_tracks), "alloc tracks") ||
-        !chk(e->d_drones.alloc(n_drones), "alloc drones") || !chk(e->d_drone_inst.alloc(n_drones), "alloc drone_inst") ||
+        !chk(e->d_drones.alloc((size_t)n_drones * 2), "alloc drones") || !chk(e->d_drone_inst.alloc(n_drones), "alloc drone_inst") ||
         !chk(e->d_nominal.alloc(e->nom_total), "alloc nominal") || !chk(e->d_costs.alloc(e->cost_total), "alloc costs") ||
         !chk(e->d_noise.alloc(e->noise_total), "alloc noise") || !chk(e->d_actions.alloc((size_t)n_instances * 4), "alloc actions"))
         return bail(DR_ENOMEM);
@@ -545,7 +610,7 @@ int dr_engine_set_drones(dr_engine* e, const dr_drone_state* states, int n_drone
         return fail(DR_EINVAL, "set_drones: bad handle, pointer or count");
     if (int rc = check_drones(states, n_drones))
         return rc;
-    HIPCHK(hipMemcpyAsync(e->d_drones.p, states, n_drones * sizeof(dr_drone_state), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->drones_at(e->cur), states, n_drones * sizeof(dr_drone_state), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return DR_OK;
 }
@@ -554,7 +619,7 @@ int dr_engine_get_drones(dr_engine* e, dr_drone_state* states, int n_drones)
 {
     if (!e || !states || n_drones != e->n_drones)
         return fail(DR_EINVAL, "get_drones: bad handle, pointer or count");
-    HIPCHK(hipMemcpyAsync(states, e->d_drones.p, n_drones * sizeof(dr_drone_state), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(states, e->drones_at(e->cur), n_drones * sizeof(dr_drone_state), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return DR_OK;
 }
@@ -602,6 +667,7 @@ int dr_engine_set_step(dr_engine* e, int instance, uint32_t step)
     y.step = step;
     HIPCHK(hipMemcpyAsync(e->d_dyn.p + instance, &y, sizeof(InstDyn), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    e->noise_ready = false;
     return DR_OK;
 }
 
@@ -726,14 +792,15 @@ int dr_engine_set_launch(dr_engine* e, const dr_launch_cfg* cfg)
         c.steps_per_graph = 1;
     if (!valid_block(c.rollout_block) || !valid_chunk(c.reduce_chunk) || !valid_nb(c.noise_batch) || c.rollout_grid < 0 ||
         c.steps_per_graph < 1 || c.steps_per_graph > 64 || c.reduce_tgroup < 1 || c.reduce_tgroup > DR_MAX_T ||
-        (c.rollout_split < 0 || c.rollout_split > 3) || (c.rollout_split == 3 && c.noise_batch > 2))
+        (c.rollout_split < 0 || c.rollout_split > 4) || (c.rollout_split >= 3 && c.noise_batch > 2))
         return fail(DR_EINVAL, "set_launch: rollout_block in {64,128,256}, reduce_chunk a multiple of 64 in [64,4096], "
-                               "noise_batch in {1,2,4} ({1,2} with rollout_split 3), steps_per_graph in [1,64], "
-                               "reduce_tgroup in [1,256], rollout_split in {0,1,2,3}");
+                               "noise_batch in {1,2,4} ({1,2} with rollout_split 3 or 4), steps_per_graph in [1,64], "
+                               "reduce_tgroup in [1,256], rollout_split in {0,1,2,3,4}");
     HIPCHK(hipStreamSynchronize(e->stream));
     const bool relayout = c.rollout_block != e->launch.rollout_block || c.reduce_chunk != e->launch.reduce_chunk ||
                           c.reduce_tgroup != e->launch.reduce_tgroup || c.rollout_split != e->launch.rollout_split;
     e->launch = c;
+    e->noise_ready = false;
     e->drop_graphs();
     if (relayout)
         return e->build_layout();
@@ -804,6 +871,7 @@ int dr_engine_set_refine(dr_engine* e, int n_grad_steps, float step, int ckpt_ev
     e->refine_steps = n_grad_steps;
     e->refine_step = step;
     e->refine_ck = ckpt_every;
+    e->noise_ready = false;
     e->drop_graphs();
     return DR_OK;
 }
@@ -828,7 +896,7 @@ int dr_engine_cost_grad_device(dr_engine* e, int instance, const float* U, int n
     if (table + lanes * per_lane > kAdjLdsBudget)
         return fail(DR_EINVAL, "cost_grad: horizon and checkpoint interval need too much LDS");
     hipStream_t st = stream ? (hipStream_t)stream : e->stream;
-    HIPCHK(launch_costgrad(lanes, st, e->d_inst.p, instance, I.T, n_ck, ck, e->d_tracks.p, e->d_drones.p, U, n_seq, J, dJdU, e->C, e->Q,
+    HIPCHK(launch_costgrad(lanes, st, e->d_inst.p, instance, I.T, n_ck, ck, e->d_tracks.p, e->drones_at(e->cur), U, n_seq, J, dJdU, e->C, e->Q,
                            e->cost, e->opp_stride));
     return DR_OK;
 }
@@ -952,23 +1020,18 @@ int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* bes
     const dr_launch_cfg saved = e->launch;
     dr_launch_cfg best_cfg = saved;
     float best_t = 1e30f;
-    int n_cu = 256;
-    {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, e->device) == hipSuccess)
-            n_cu = prop.multiProcessorCount;
-    }
-    // Candidate shapes: the four rollout structures (one lane per sample,
-    // noise wave + dynamics wave, four- and eight-stage pipelines) x noise
+    // Candidate shapes: the rollout structures (one lane per sample, noise
+    // wave + dynamics wave, four- and eight-stage pipelines, and with the
+    // refinement on the eight-wave pipeline over noise made ahead) x noise
     // batch x weighted-average chunk x timestep group.
     std::vector<dr_launch_cfg> cands;
-    for (int split : {0, 1, 2, 3})
+    for (int split : {0, 1, 2, 3, 4})
         for (int nb : {1, 2, 4})
             for (int ch : {512, 1024, 2048})
                 for (int tg : {16, 32})
                     for (int rb : {128, 256})
                     {
-                        if ((split != 0 && rb != 128) || (split == 3 && nb > 2))
+                        if ((split != 0 && rb != 128) || (split == 3 && nb > 2) || (split == 4 && (nb > 1 || e->refine_steps == 0)))
                             continue;
                         dr_launch_cfg c = saved;
                         c.rollout_split = split;
@@ -981,7 +1044,6 @@ int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* bes
                         c.steps_per_graph = 1;
                         cands.push_back(c);
                     }
-    (void)n_cu;
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
@@ -1011,6 +1073,40 @@ int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* bes
             best_cfg = c;
         }
     }
+    // Then the steps per graph replay for the best shape: consecutive replays
+    // of one graph leave a gap on the device that the steps inside a graph do
+    // not have (runs shorter than a graph use one-step graphs).  A longer graph
+    // is kept only if it is at least 2 % faster per step.
+    for (int spg : {2, 4, 8, 16})
+    {
+        if (rc)
+            break;
+        dr_launch_cfg c = best_cfg;
+        c.steps_per_graph = spg;
+        rc = dr_engine_set_launch(e, &c);
+        if (rc)
+            break;
+        rc = e->run(spg, false); // warm-up and graph build
+        if (rc)
+            break;
+        const int n_run = ((std::max(trials, 2 * spg) + spg - 1) / spg) * spg;
+        float ms = 0;
+        if (hipEventRecord(a, e->stream) != hipSuccess)
+            rc = fail(DR_EHIP, "autotune: hipEventRecord failed");
+        if (!rc)
+            rc = e->run(n_run, false);
+        if (!rc && (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                    hipEventElapsedTime(&ms, a, b) != hipSuccess))
+            rc = fail(DR_EHIP, "autotune: event timing failed");
+        if (rc)
+            break;
+        ms /= (float)n_run;
+        if (ms < 0.98f * best_t)
+        {
+            best_t = ms;
+            best_cfg = c;
+        }
+    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (rc)
@@ -1018,12 +1114,12 @@ int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* bes
         dr_engine_set_launch(e, &saved);
         return rc;
     }
-    best_cfg.steps_per_graph = saved.steps_per_graph;
     rc = dr_engine_set_launch(e, &best_cfg);
     if (rc)
         return rc;
     HIPCHK(hipMemcpy(e->d_nominal.p, nom.data(), nom.size() * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_dyn.p, dyn.data(), dyn.size() * sizeof(InstDyn), hipMemcpyHostToDevice));
+    e->noise_ready = false; // the step counters are back where they were
     if (best)
         *best = best_cfg;
     if (best_ms)

@@ -52,6 +52,7 @@ struct InstDyn
     float exec_cost;          // running cost of the executed closed-loop states, summed (k_exec)
     float exec_progress;      // race progress of the last executed state (k_exec)
     uint32_t exec_collisions; // executed states in collision (k_exec)
+    uint32_t step_ahead;      // step + 1, written by k_wavg: the step whose noise k_refine's spare workgroups make
 };
 
 struct RTile

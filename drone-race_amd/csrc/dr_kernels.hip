@@ -426,6 +426,61 @@ __global__ __launch_bounds__(256) void k_rollout_p4(const InstDev* __restrict__ 
     }
 }
 
+// One element of a sample's noise row: sigma (x) the four normals of Philox
+// counter (k, t, step, STREAM_CONTROL) under the instance key (Box-Muller on
+// words x, y and on z, w), as every rollout makes it.
+__device__ __forceinline__ float4 control_noise(uint32_t key0, uint32_t key1, uint32_t k, uint32_t t, uint32_t step, float sg0,
+                                                float sg1, float sg2, float sg3)
+{
+    const U4 w = philox4x32_10(U4{k, t, step, STREAM_CONTROL}, key0, key1);
+    float z[4];
+    box_muller(w.x, w.y, &z[0], &z[1]);
+    box_muller(w.z, w.w, &z[2], &z[3]);
+    return make_float4(sg0 * z[0], sg1 * z[1], sg2 * z[2], sg3 * z[3]);
+}
+
+// The noise rows of every instance, made ahead of the rollout that reads
+// them (rollout_split 4): worker `wk` of `n_wk` writes elements wk, wk + n_wk,
+// ... of each instance's [K][T] block (consecutive workers, consecutive
+// float4: coalesced), four elements per trip so that four Philox chains are
+// in flight.  ahead: the step after the current one (dyn.step_ahead, written
+// by k_wavg before the refinement); otherwise the current step.
+__device__ void make_noise_rows(const InstDev* __restrict__ insts, const InstDyn* __restrict__ dyn, int n_inst, float4* __restrict__ noise,
+                                int wk, int n_wk, bool ahead)
+{
+    for (int i = 0; i < n_inst; ++i)
+    {
+        const InstDev& I = insts[i];
+        const uint32_t step = ahead ? dyn[i].step_ahead : dyn[i].step;
+        const int T = I.T, n = I.K * T;
+        const float sg0 = I.sigma[0], sg1 = I.sigma[1], sg2 = I.sigma[2], sg3 = I.sigma[3];
+        float4* out = noise + I.noise_off;
+        for (int e0 = wk; e0 < n; e0 += 4 * n_wk)
+        {
+            float4 v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int e = min(e0 + j * n_wk, n - 1);
+                const int k = e / T, t = e - k * T;
+                v[j] = control_noise(I.key0, I.key1, (uint32_t)k, (uint32_t)t, step, sg0, sg1, sg2, sg3);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (e0 + j * n_wk < n)
+                    out[e0 + j * n_wk] = v[j];
+        }
+    }
+}
+
+// The current step's noise rows of every instance, before a rollout that
+// reads them when they were not made ahead (the first step of a run).
+__global__ __launch_bounds__(256) void k_noise(const InstDev* __restrict__ insts, const InstDyn* __restrict__ dyn, int n_inst,
+                                               float4* __restrict__ noise)
+{
+    make_noise_rows(insts, dyn, n_inst, noise, blockIdx.x * 256 + threadIdx.x, gridDim.x * 256, false);
+}
+
 // Eight-stage pipelined rollout for small sample counts: a workgroup of eight
 // waves covers 64 samples, so that a launch with few tiles still gives every
 // SIMD four instruction streams.  Stages and the chunk each works on at
@@ -635,6 +690,215 @@ __global__ __launch_bounds__(512, 4) void k_rollout_p8(const InstDev* __restrict
                                 r_s[c % 3][u][lane] = make_float4(s.p[0], s.p[1], s.p[2], u2f((uint32_t)(s.gate | (s.laps << 8))));
                             }
                         }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+        if (role == 6)
+        {
+            float cost = __builtin_inff();
+            if (live)
+            {
+                const uint32_t code = f2u(last.w);
+                QState x;
+                x.p[0] = last.x;
+                x.p[1] = last.y;
+                x.p[2] = last.z;
+                x.gate = (int)(code & 0xffu);
+                x.laps = (int)code >> 8;
+                acc = acc + final_cost(KC, s_track, x);
+                costs[I.cost_off + k] = acc;
+                cost = acc;
+            }
+            const float wm = wave_min(cost);
+            if (lane == 0)
+                tmin[tile] = wm;
+        }
+    }
+}
+
+// Eight-wave rollout over noise rows made ahead (rollout_split 4: by the
+// spare workgroups of the previous step's refinement, or by k_noise): no
+// Philox or Box-Muller in the rollout, the rows are read back.  Stages and the
+// chunk each works on at chunk period i (LDS rings of CH = 4 steps, one
+// barrier per period):
+//   R   noise rows (requested two chunks ahead), clamp(nominal + eps), body
+//       rates and attitude                                            chunk i
+//   X   velocity, position from the pre-step attitude, gate           chunk i - 1
+//   C   running cost and terminal test of step u of the chunk         chunk i - 2
+//   S   discounted sum in step order, stop at the first terminal      chunk i - 3
+// Roles (wave w works as (w + blockIdx) mod 8; waves w and w + 4 share a
+// SIMD): 0, 1, 4, 5 the running cost of steps u = 0, 1, 2, 3; 6 R and S; 7 X;
+// 2 and 3 idle, so the SIMD pairs are {C, C}, {C, C}, {-, R + S}, {-, X}.
+// The arithmetic of every sample is that of k_rollout.
+__global__ __launch_bounds__(512, 4) void k_rollout_p8nz(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
+                                                         const RTile* __restrict__ tiles, int n_tiles, const dr_track* __restrict__ tracks,
+                                                         const dr_drone_state* __restrict__ drones, const float* __restrict__ nominal,
+                                                         float* __restrict__ costs, const float4* __restrict__ noise,
+                                                         float* __restrict__ tmin, QuadConst C, CostConst Q, dr_cost_cfg KC, int opp_stride)
+{
+    constexpr int CH = 4;
+    extern __shared__ float4 s_opp[]; // [T][opp_stride]
+    __shared__ float4 r_q[2][CH][WAVE]; // R -> X: attitude at the start of the step
+    __shared__ float r_f[2][CH][WAVE];  // R -> X: clamped collective thrust
+    __shared__ float4 r_s[3][CH][WAVE]; // X -> C, S: position after the step, gate | laps << 8
+    __shared__ float2 r_c[2][CH][WAVE]; // C -> S: running cost, terminal code
+    __shared__ float s_nom[DR_MAX_T * 4];
+    __shared__ dr_track s_track;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int role = ((tid / WAVE) + (int)blockIdx.x) & 7;
+    // the step of each chunk whose running cost a cost wave makes
+    const int cu = role < 2 ? role : role - 2;
+    for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
+    {
+        const RTile rt = tiles[tile];
+        const InstDev& I = insts[rt.inst];
+        const int K = I.K, T = I.T;
+        const int n_opp = I.n_opp;
+
+        __syncthreads(); // LDS of the previous tile is no longer read
+        for (int i = tid; i < T * 4; i += 512)
+            s_nom[i] = nominal[I.nom_off + i];
+        {
+            const float* src = reinterpret_cast<const float*>(tracks + I.track);
+            float* dst = reinterpret_cast<float*>(&s_track);
+            for (int i = tid; i < (int)(sizeof(dr_track) / 4); i += 512)
+                dst[i] = src[i];
+        }
+        for (int i = tid; i < T * n_opp; i += 512)
+        {
+            const int t = i / n_opp, j = i - t * n_opp;
+            s_opp[t * opp_stride + j] = opp_predict(drones[I.opp[j]], C.dt * (float)(t + 1));
+        }
+        __syncthreads();
+
+        const int k = rt.k0 + lane;
+        const bool live = k < K;
+        const int n_chunks = (T + CH - 1) / CH;
+        const float4* nrow = noise + I.noise_off + (long long)(live ? k : 0) * T;
+
+        QState s; // R: attitude and rates; X: position, velocity, gate, laps
+        if (live)
+            s = state_from_abi(drones[I.drone]);
+        // S: discounted sum, the state the terminal cost reads (position and gate code)
+        float acc = 0.0f, decay = 1.0f;
+        float4 last = make_float4(s.p[0], s.p[1], s.p[2], u2f((uint32_t)(s.gate | (s.laps << 8))));
+        bool stop = false;
+        // R: the rows of the even chunks in ea, of the odd chunks in eb, each
+        // requested two periods before it is used
+        float4 ea[CH], eb[CH];
+        const auto request = [&](float4 (&e)[CH], int c) {
+#pragma unroll
+            for (int u = 0; u < CH; ++u)
+                e[u] = nrow[min(c * CH + u, T - 1)];
+        };
+        const auto attitude = [&](const float4 (&e)[CH], int c) {
+#pragma unroll
+            for (int u = 0; u < CH; ++u)
+            {
+                const int t = c * CH + u;
+                if (t < T)
+                {
+                    float uin[4], uc[4];
+                    uin[0] = s_nom[t * 4 + 0] + e[u].x;
+                    uin[1] = s_nom[t * 4 + 1] + e[u].y;
+                    uin[2] = s_nom[t * 4 + 2] + e[u].z;
+                    uin[3] = s_nom[t * 4 + 3] + e[u].w;
+                    clamp_control(C, uin, uc);
+                    r_q[c & 1][u][lane] = make_float4(s.q[0], s.q[1], s.q[2], s.q[3]);
+                    r_f[c & 1][u][lane] = uc[0];
+                    quad_attitude_step(C, s.q, s.w, uc);
+                }
+            }
+        };
+        if (live && role == 6)
+        {
+            request(ea, 0);
+            request(eb, 1);
+        }
+
+        for (int i = 0; i < n_chunks + 3; ++i)
+        {
+            if (live)
+            {
+                if (role == 6)
+                {
+                    const int c = i;
+                    if (c < n_chunks)
+                    {
+                        if ((c & 1) == 0)
+                        {
+                            attitude(ea, c);
+                            request(ea, c + 2);
+                        }
+                        else
+                        {
+                            attitude(eb, c);
+                            request(eb, c + 2);
+                        }
+                    }
+                    // S: the chunk costed one period earlier, in step order
+                    const int cs = i - 3;
+                    if (cs >= 0 && cs < n_chunks)
+                    {
+#pragma unroll
+                        for (int u = 0; u < CH; ++u)
+                        {
+                            const int t = cs * CH + u;
+                            if (t < T && !stop)
+                            {
+                                const float2 ct = r_c[cs & 1][u][lane];
+                                last = r_s[cs % 3][u][lane];
+                                acc = fma_(ct.x, decay, acc);
+                                decay = decay * DECAY;
+                                stop = ((int)f2u(ct.y) != TERM_NONE);
+                            }
+                        }
+                    }
+                }
+                else if (role == 7)
+                {
+                    const int c = i - 1;
+                    if (c >= 0 && c < n_chunks)
+                    {
+#pragma unroll
+                        for (int u = 0; u < CH; ++u)
+                        {
+                            const int t = c * CH + u;
+                            if (t < T)
+                            {
+                                const float4 q4 = r_q[c & 1][u][lane];
+                                const float q[4] = {q4.x, q4.y, q4.z, q4.w};
+                                const float uu[4] = {r_f[c & 1][u][lane], 0.0f, 0.0f, 0.0f}; // the translation reads the thrust only
+                                const float p0[3] = {s.p[0], s.p[1], s.p[2]};
+                                quad_translation_step(C, q, s.p, s.v, uu);
+                                update_gate(s_track.gates, s_track.n_gates, KC.gate_traversal_margin, p0, s);
+                                r_s[c % 3][u][lane] = make_float4(s.p[0], s.p[1], s.p[2], u2f((uint32_t)(s.gate | (s.laps << 8))));
+                            }
+                        }
+                    }
+                }
+                else if (role != 2 && role != 3)
+                {
+                    const int c = i - 2;
+                    const int t = c * CH + cu;
+                    if (c >= 0 && c < n_chunks && t < T)
+                    {
+                        const float4 v = r_s[c % 3][cu][lane];
+                        const uint32_t code = f2u(v.w);
+                        QState x;
+                        x.p[0] = v.x;
+                        x.p[1] = v.y;
+                        x.p[2] = v.z;
+                        x.gate = (int)(code & 0xffu);
+                        x.laps = (int)code >> 8;
+                        int term;
+                        const float cst = stage_cost(C, Q, KC, s_track, x, s_opp + t * opp_stride, n_opp, &term);
+                        r_c[c & 1][cu][lane] = make_float2(cst, u2f((uint32_t)term));
                     }
                 }
             }
@@ -1040,8 +1304,20 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                                                       const double* __restrict__ part, const double* __restrict__ den,
                                                       const dr_track* __restrict__ tracks, const dr_drone_state* __restrict__ drones,
                                                       float* __restrict__ nominal, int n_steps, float step, QuadConst C, CostConst Q,
-                                                      dr_cost_cfg KC)
+                                                      dr_cost_cfg KC, float* __restrict__ actions, int advance,
+                                                      const int* __restrict__ drone_inst, int n_drones,
+                                                      dr_drone_state* __restrict__ drones_out, int n_inst, float4* __restrict__ noise)
 {
+    // Workgroups past the instances make the next step's noise rows (for a
+    // rollout over noise made ahead) on the CUs the refinement leaves idle:
+    // the refinement's workgroups come first in the grid and hold their CUs
+    // alone (each takes most of a CU's LDS).
+    if ((int)blockIdx.x >= n_inst)
+    {
+        make_noise_rows(insts, dyn, n_inst, noise, ((int)blockIdx.x - n_inst) * REF_BLOCK + (int)threadIdx.x,
+                        ((int)gridDim.x - n_inst) * REF_BLOCK, true);
+        return;
+    }
     __shared__ dr_track s_track;
     __shared__ float4 s_x[(DR_MAX_T + 1) * 4]; // state table, put_state layout
     __shared__ float4 s_u[DR_MAX_T];           // the sequence being refined
@@ -1079,7 +1355,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         s_ov[tid] = make_float4(o.v[0], o.v[1], o.v[2], 0.0f);
     }
     {
-        // the MPPI update (k_update's mode 1): nominal + num/den over the
+        // the MPPI update (as k_update makes it): nominal + num/den over the
         // chunks in fixed order, box clamp; nu and the temperature rule.  Each
         // lane sums the denominator and its (up to 4) values together, eight
         // chunks of loads in flight at a time.
@@ -1573,14 +1849,49 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         __syncthreads();
     }
     __syncthreads();
+    // the kept sequence's first control is the action, and the sequence
+    // shifted left by one step (shift_fill at the end) the next nominal
+    // (as k_update makes them)
     for (int t = tid; t < T; t += REF_BLOCK)
     {
-        const float4 u4 = s_ubest[t];
+        const float4 u4 = t + 1 < T ? s_ubest[t + 1] : make_float4(I.shift_fill[0], I.shift_fill[1], I.shift_fill[2], I.shift_fill[3]);
         float* u = nominal + I.nom_off + t * 4;
         u[0] = u4.x;
         u[1] = u4.y;
         u[2] = u4.z;
         u[3] = u4.w;
+    }
+    if (tid < 4)
+    {
+        const float4 a = s_ubest[0];
+        actions[blockIdx.x * 4 + tid] = tid == 0 ? a.x : tid == 1 ? a.y : tid == 2 ? a.z : a.w;
+    }
+    if (tid == 0)
+    {
+        dyn[blockIdx.x].step = dyn[blockIdx.x].step + 1u;
+        // Closed loop (as k_update): the drone integrates the action, the
+        // drones without a controller (spread over the workgroups) keep their
+        // velocity.  Other workgroups may still read this step's drones, so
+        // the moved drones go to the other copy of the world.
+        if (advance)
+        {
+            QState s = x0; // the drone as this launch read it
+            const float4 a = s_ubest[0];
+            const float act[4] = {a.x, a.y, a.z, a.w};
+            advance_state(s, act, s_track, C, margin);
+            dr_drone_state st;
+            state_to_abi(s, st);
+            st._pad = 0;
+            drones_out[I.drone] = st;
+            for (int d = blockIdx.x; d < n_drones; d += n_inst) // the refinement's workgroups only
+                if (drone_inst[d] < 0)
+                {
+                    dr_drone_state o = drones[d];
+                    for (int c = 0; c < 3; ++c)
+                        o.p[c] = fma_(C.dt, o.v[c], o.p[c]);
+                    drones_out[d] = o;
+                }
+        }
     }
 }
 
@@ -1645,7 +1956,10 @@ __global__ __launch_bounds__(WAVG_BLOCK) void k_wavg(const InstDev* __restrict__
             d += s_den[w];
         den[I.den_off + wt.chunk] = d;
         if (wt.chunk == 0)
+        {
             dyn[wt.inst].min_cost = minC;
+            dyn[wt.inst].step_ahead = dyn[wt.inst].step + 1u;
+        }
     }
 
     const int TG = wt.t1 - wt.t0;
@@ -1690,69 +2004,38 @@ __global__ __launch_bounds__(WAVG_BLOCK) void k_wavg(const InstDev* __restrict__
 // advance != 0 the closed loop follows (SimulationEngine::dynStep,
 // engine.cpp:286-312): the instance's drone integrates its action with the
 // same dynamics and gate logic, noise-free, and drones without a controller
-// (distributed over the workgroups) keep their velocity.
-// mode 0: update, clamp, action, shift, advance; mode 1: update and clamp
-// only (the unshifted nominal is written back for the adjoint refinement);
-// mode 2: action, shift, advance from the (refined) unshifted nominal.
+// (distributed over the workgroups) keep their velocity.  With the adjoint
+// refinement on, k_refine does all of this itself.
 __global__ __launch_bounds__(512) void k_update(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
                                                 const double* __restrict__ part, const double* __restrict__ den,
                                                 float* __restrict__ nominal, float* __restrict__ actions, QuadConst C, int advance,
                                                 dr_drone_state* __restrict__ drones, const int* __restrict__ drone_inst, int n_drones,
-                                                const dr_track* __restrict__ tracks, float margin, int mode)
+                                                const dr_track* __restrict__ tracks, float margin)
 {
     __shared__ float s_u[DR_MAX_T * 4];
     const int inst = blockIdx.x;
     const InstDev& I = insts[inst];
     const int T = I.T, tid = threadIdx.x;
 
-    double dn = 0.0;
-    if (mode == 2)
+    const double dn = sum_in_order(den + I.den_off, I.n_chunks, 1);
+    for (int i = tid; i < T * 4; i += blockDim.x)
     {
-        for (int i = tid; i < T * 4; i += blockDim.x)
-            s_u[i] = nominal[I.nom_off + i];
-        __syncthreads();
+        const double num = sum_in_order(part + I.part_off + i, I.n_chunks, (long long)T * 4);
+        float u = nominal[I.nom_off + i];
+        if (dn > 1e-30)
+            u = u + (float)(num / dn);
+        s_u[i] = u;
     }
-    else
+    __syncthreads();
+    for (int t = tid; t < T; t += blockDim.x)
     {
-        dn = sum_in_order(den + I.den_off, I.n_chunks, 1);
-        for (int i = tid; i < T * 4; i += blockDim.x)
-        {
-            const double num = sum_in_order(part + I.part_off + i, I.n_chunks, (long long)T * 4);
-            float u = nominal[I.nom_off + i];
-            if (dn > 1e-30)
-                u = u + (float)(num / dn);
-            s_u[i] = u;
-        }
-        __syncthreads();
-        for (int t = tid; t < T; t += blockDim.x)
-        {
-            float in[4] = {s_u[t * 4 + 0], s_u[t * 4 + 1], s_u[t * 4 + 2], s_u[t * 4 + 3]};
-            float out[4];
-            clamp_control(C, in, out);
-            for (int d = 0; d < 4; ++d)
-                s_u[t * 4 + d] = out[d];
-        }
-        __syncthreads();
-        if (mode == 1)
-        {
-            for (int i = tid; i < T * 4; i += blockDim.x)
-                nominal[I.nom_off + i] = s_u[i];
-            if (tid == 0)
-            {
-                InstDyn& Y = dyn[inst];
-                const float nu = (float)dn;
-                Y.nu = nu;
-                if (I.adapt)
-                {
-                    if (nu > 0.001f * (float)I.K)
-                        Y.lambda = Y.lambda * 0.9f;
-                    else if (nu < 0.0005f * (float)I.K)
-                        Y.lambda = Y.lambda * 1.2f;
-                }
-            }
-            return;
-        }
+        float in[4] = {s_u[t * 4 + 0], s_u[t * 4 + 1], s_u[t * 4 + 2], s_u[t * 4 + 3]};
+        float out[4];
+        clamp_control(C, in, out);
+        for (int d = 0; d < 4; ++d)
+            s_u[t * 4 + d] = out[d];
     }
+    __syncthreads();
     if (tid < 4)
         actions[inst * 4 + tid] = s_u[tid];
     for (int i = tid; i < T * 4; i += blockDim.x)
@@ -1763,17 +2046,14 @@ __global__ __launch_bounds__(512) void k_update(const InstDev* __restrict__ inst
     if (tid == 0)
     {
         InstDyn& Y = dyn[inst];
-        if (mode == 0)
+        const float nu = (float)dn;
+        Y.nu = nu;
+        if (I.adapt)
         {
-            const float nu = (float)dn;
-            Y.nu = nu;
-            if (I.adapt)
-            {
-                if (nu > 0.001f * (float)I.K)
-                    Y.lambda = Y.lambda * 0.9f;
-                else if (nu < 0.0005f * (float)I.K)
-                    Y.lambda = Y.lambda * 1.2f;
-            }
+            if (nu > 0.001f * (float)I.K)
+                Y.lambda = Y.lambda * 0.9f;
+            else if (nu < 0.0005f * (float)I.K)
+                Y.lambda = Y.lambda * 1.2f;
         }
         Y.step = Y.step + 1u;
         if (advance)
@@ -1850,6 +2130,12 @@ hipError_t launch_rollout(int block, int nb, int split, int grid, hipStream_t st
         return hipErrorInvalidValue;
     const int g = grid > 0 ? (grid < n_tiles ? grid : n_tiles) : n_tiles;
     const size_t shm = (size_t)max_T * opp_stride * sizeof(float4);
+    if (split == 4)
+    {
+        // tiles of 64 samples, eight waves, over the noise rows made ahead
+        k_rollout_p8nz<<<g, 512, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC, opp_stride);
+        return hipGetLastError();
+    }
     if (split == 3)
     {
         // tiles of 64 samples, eight stage waves; chunk length = 2 * noise_batch steps
@@ -1952,10 +2238,15 @@ hipError_t launch_wavg(int chunk, int n_wtiles, hipStream_t st, const InstDev* i
 
 hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
                          float* nominal, float* actions, const QuadConst& C, int advance, dr_drone_state* drones, const int* drone_inst,
-                         int n_drones, const dr_track* tracks, float margin, int mode)
+                         int n_drones, const dr_track* tracks, float margin)
 {
-    k_update<<<n_inst, 512, 0, st>>>(insts, dyn, part, den, nominal, actions, C, advance, drones, drone_inst, n_drones, tracks, margin,
-                                     mode);
+    k_update<<<n_inst, 512, 0, st>>>(insts, dyn, part, den, nominal, actions, C, advance, drones, drone_inst, n_drones, tracks, margin);
+    return hipGetLastError();
+}
+
+hipError_t launch_noise(int blocks, hipStream_t st, const InstDev* insts, const InstDyn* dyn, int n_inst, float4* noise)
+{
+    k_noise<<<blocks, 256, 0, st>>>(insts, dyn, n_inst, noise);
     return hipGetLastError();
 }
 
@@ -1970,9 +2261,11 @@ hipError_t launch_costgrad(int lanes, hipStream_t st, const InstDev* insts, int 
 
 hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
                          const dr_track* tracks, const dr_drone_state* drones, float* nominal, int n_steps, float step,
-                         const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC)
+                         const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, float* actions, int advance,
+                         const int* drone_inst, int n_drones, dr_drone_state* drones_out, int noise_blocks, float4* noise)
 {
-    k_refine<<<n_inst, REF_BLOCK, 0, st>>>(insts, dyn, part, den, tracks, drones, nominal, n_steps, step, C, Q, KC);
+    k_refine<<<n_inst + noise_blocks, REF_BLOCK, 0, st>>>(insts, dyn, part, den, tracks, drones, nominal, n_steps, step, C, Q, KC,
+                                                          actions, advance, drone_inst, n_drones, drones_out, n_inst, noise);
     return hipGetLastError();
 }
 

@@ -555,14 +555,19 @@ DR_HD void state_to_abi(const QState& s, dr_drone_state& d)
 
 // Closed-loop step of a controlled drone: clamp its action, one Euler step,
 // gate update (SimulationEngine::dynStep, engine.cpp:286-312).
-DR_HD void advance_drone(dr_drone_state& st, const float* act, const dr_track& tr, const QuadConst& C, float margin)
+DR_HD void advance_state(QState& s, const float* act, const dr_track& tr, const QuadConst& C, float margin)
 {
-    QState s = state_from_abi(st);
     float u[4];
     clamp_control(C, act, u);
     const float p0[3] = {s.p[0], s.p[1], s.p[2]};
     quad_step(C, s, u);
     update_gate(tr.gates, tr.n_gates, margin, p0, s);
+}
+
+DR_HD void advance_drone(dr_drone_state& st, const float* act, const dr_track& tr, const QuadConst& C, float margin)
+{
+    QState s = state_from_abi(st);
+    advance_state(s, act, tr, C, margin);
     state_to_abi(s, st);
 }
 

@@ -163,7 +163,9 @@ typedef struct dr_launch_cfg
     int32_t use_graph;        /* 0: plain stream launches */
     int32_t reduce_tgroup;    /* timesteps per weighted-average workgroup: 1..256 */
     int32_t rollout_split;    /* 0: one lane per sample; 1: noise wave + dynamics wave; 2: four-stage wave pipeline;
-                                 3: eight-stage wave pipeline (noise_batch 1 or 2) */
+                                 3: eight-stage wave pipeline (noise_batch 1 or 2); 4: with the adjoint refinement on,
+                                 an eight-wave pipeline that reads noise rows made ahead by the previous step's
+                                 refinement launch on the CUs it leaves idle (as 3 without the refinement) */
 } dr_launch_cfg;
 
 typedef struct dr_engine dr_engine;
@@ -196,7 +198,9 @@ int dr_engine_synchronize(dr_engine* e);
 
 /* Results of the last control step, for tests and telemetry. */
 int dr_engine_get_costs(dr_engine* e, int instance, float* costs, int n);
-int dr_engine_get_noise(dr_engine* e, int instance, float* noise, size_t n); /* layout [k][t][4] */
+/* Noise rows, layout [k][t][4]: those of the last control step, or with
+ * rollout_split 4 and the refinement on those made ahead for the next one. */
+int dr_engine_get_noise(dr_engine* e, int instance, float* noise, size_t n);
 int dr_engine_get_actions(dr_engine* e, float* actions, int n_floats);
 int dr_engine_get_diag(dr_engine* e, dr_diag* diag, int n_instances);
 

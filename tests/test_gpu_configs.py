@@ -202,3 +202,92 @@ def test_eight_drone_refined_closed_loop(pkg, oracle):
     e_act, e_state = rel_err(g_acts, o_acts), rel_err(_state_rows(g_drones), _state_rows(drones))
     say(f"  after 3 steps: actions max rel {e_act:.2e}, drone states max rel {e_state:.2e}")
     assert e_act <= 1e-3 and e_state <= 1e-3
+
+
+def test_refined_closed_loop_launch_paths(pkg):
+    """The refined closed loop moves the drones into the other copy of the drone states each step.
+    Actions, nominals and drone states are the same bits whether every step is one graph replay
+    (the graphs of both copies in turn), several steps share one graph (odd and even counts), or
+    the kernels are launched one at a time; steps without advance move nothing; drones without a
+    controller keep their velocity."""
+    sc = pkg.scenarios
+    scn = sc.c4_eight_drones(512, 40)
+    scn.instances = scn.instances[:6]  # drones 6 and 7 fly without a controller
+    dt = scn.params.dt
+    v0 = _state_rows(scn.drones)[6:, 7:10]
+    p0 = _state_rows(scn.drones)[6:, 0:3]
+    runs = []
+    for launch in (dict(use_graph=0), dict(use_graph=1, steps_per_graph=1), dict(use_graph=1, steps_per_graph=3),
+                   dict(use_graph=1, steps_per_graph=2)):
+        say(f"case: {scn.name} with 6 controllers, refined closed loop, launch {launch}")
+        eng = pkg.Engine(scn)
+        eng.set_refine(3, 0.05, 10)
+        eng.set_launch(**launch)
+        acts = [eng.step(1, advance=True), eng.step(4, advance=True)]
+        before = _state_rows(eng.get_drones())
+        acts.append(eng.step(2, advance=False))
+        assert np.array_equal(_state_rows(eng.get_drones()), before)
+        acts.append(eng.step(3, advance=True))
+        rows = _state_rows(eng.get_drones())
+        noms = [eng.get_nominal(i) for i in range(len(scn.instances))]
+        eng.close()
+        runs.append((acts, rows, noms))
+        # 8 closed-loop steps at constant velocity (one FMA per step and axis)
+        p = p0.astype(np.float32)
+        for _ in range(8):
+            p = (p.astype(np.float64) + np.float64(np.float32(dt)) * v0).astype(np.float32)
+        assert np.allclose(rows[6:, 0:3], p, rtol=1e-6, atol=1e-6)
+        assert np.array_equal(rows[6:, 7:10], v0)
+    for acts, rows, noms in runs[1:]:
+        for a, b in zip(acts, runs[0][0]):
+            assert np.array_equal(a, b)
+        assert np.array_equal(rows, runs[0][1])
+        for a, b in zip(noms, runs[0][2]):
+            assert np.array_equal(a, b)
+
+
+def test_noise_made_ahead_matches_eight_stage_rollout(pkg):
+    """rollout_split 4 (the refinement launch's spare workgroups make the next step's noise rows,
+    the rollout reads them) against rollout_split 3 (every rollout makes its own): the same costs,
+    actions, nominals and drone states bit for bit over closed-loop steps, steps without advance,
+    one-step and three-step graphs, plain launches and a step counter set by hand; the rows in
+    the buffer after a split-4 step are the next step's."""
+    sc = pkg.scenarios
+    scn = sc.c4_eight_drones(512, 40)
+    scn.instances = scn.instances[:6]
+    n = len(scn.instances)
+    say(f"case: {scn.name} with 6 controllers, refined, rollout_split 4 against 3")
+    ref = pkg.Engine(scn)
+    nz = pkg.Engine(scn)
+    for e in (ref, nz):
+        e.set_refine(3, 0.05, 10)
+    ref.set_launch(rollout_split=3, noise_batch=2, use_graph=1, steps_per_graph=1)
+    nz.set_launch(rollout_split=4, noise_batch=1, use_graph=1, steps_per_graph=3)
+
+    def same(tag):
+        for i in range(n):
+            assert np.array_equal(ref.costs(i), nz.costs(i)), (tag, i)
+            assert np.array_equal(ref.get_nominal(i), nz.get_nominal(i)), (tag, i)
+        assert np.array_equal(_state_rows(ref.get_drones()), _state_rows(nz.get_drones())), tag
+
+    for n_steps, adv in ((1, True), (4, True), (2, False), (3, True)):
+        assert np.array_equal(ref.step(n_steps, advance=adv), nz.step(n_steps, advance=adv))
+        same(f"{n_steps} steps advance={adv}")
+    ahead = [nz.noise(i) for i in range(n)]
+    ref.step(1, advance=True)
+    for i in range(n):
+        assert np.array_equal(ref.noise(i), ahead[i])
+    nz.step(1, advance=True)
+    same("after the rows made ahead")
+    for i in range(n):
+        ref.set_step(i, 7 + i)
+        nz.set_step(i, 7 + i)
+    nz.set_launch(rollout_split=4, noise_batch=1, use_graph=0)
+    assert np.array_equal(ref.step(2, advance=True), nz.step(2, advance=True))
+    same("step counters set by hand, plain launches")
+    nz.set_refine(0, 0.05, 10)  # without the refinement split 4 runs as split 3
+    ref.set_refine(0, 0.05, 10)
+    assert np.array_equal(ref.step(2, advance=True), nz.step(2, advance=True))
+    same("refinement off")
+    ref.close()
+    nz.close()

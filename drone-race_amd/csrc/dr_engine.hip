@@ -43,7 +43,8 @@ hipError_t launch_costgrad(int lanes, hipStream_t st, const InstDev* insts, int 
 hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
                          const dr_track* tracks, const dr_drone_state* drones, float* nominal, int n_steps, float step,
                          const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, float* actions, int advance,
-                         const int* drone_inst, int n_drones, dr_drone_state* drones_out, int noise_blocks, float4* noise);
+                         const int* drone_inst, int n_drones, dr_drone_state* drones_out, int n_cu, size_t lds_per_cu, float4* noise,
+                         int tcap);
 hipError_t launch_noise(int blocks, hipStream_t st, const InstDev* insts, const InstDyn* dyn, int n_inst, float4* noise);
 
 } // namespace dr
@@ -118,6 +119,7 @@ struct dr_engine
     // the noise buffer holds the rows of every instance's current step
     bool noise_ready = false;
     int n_cu = 256;
+    size_t lds_per_cu = 160 * 1024;
     DevBuf<float> d_gU, d_gJ, d_gG; // scratch of dr_engine_cost_grad
     QuadConst C{};
     CostConst Q{};
@@ -339,12 +341,11 @@ int dr_engine::enqueue_step(bool advance, int timed_index, int p)
         // inside the refinement kernel
         if (ev)
             HIPCHK(hipEventRecord(ev[3], stream));
-        // with noise made ahead, the workgroups past the instances (one per
-        // CU the refinement leaves idle) make the next step's noise rows
-        const int noise_blocks = noise_ahead() ? std::max(n_cu - n, 64) : 0;
+        // with noise made ahead, the refinement launch also makes the next
+        // step's noise rows on the CUs it leaves idle
         HIPCHK(launch_refine(n, stream, d_inst.p, d_dyn.p, d_part.p, d_den.p, d_tracks.p, drones, d_nominal.p, refine_steps,
-                             refine_step, C, Q, cost, d_actions.p, advance ? 1 : 0, d_drone_inst.p, n_drones, moved, noise_blocks,
-                             d_noise.p));
+                             refine_step, C, Q, cost, d_actions.p, advance ? 1 : 0, d_drone_inst.p, n_drones, moved, n_cu, lds_per_cu,
+                             noise_ahead() ? d_noise.p : nullptr, max_T));
         if (ev)
             HIPCHK(hipEventRecord(ev[4], stream));
     }
@@ -539,7 +540,11 @@ int dr_engine_create(const dr_quad_params* params, const dr_cost_cfg* cost, cons
     {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, e->device) == hipSuccess && prop.multiProcessorCount > 0)
+        {
             e->n_cu = prop.multiProcessorCount;
+            if (prop.maxSharedMemoryPerMultiProcessor > 0)
+                e->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
+        }
     }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||

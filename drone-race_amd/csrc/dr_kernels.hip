@@ -1178,6 +1178,14 @@ constexpr int REF_SEGS = 16; // 14 * REF_SEGS <= REF_BLOCK
 constexpr int REF_COMP = 14; // affine part + 13 state directions
 constexpr int REF_JAC4 = (QJAC + 3) / 4; // float4 slots per step of the Jacobian table
 
+// Dynamic LDS of k_refine for horizons up to tcap: the float4 tables (state
+// slots, sequence, dJ/dp, chain gradients, Jacobian blocks, clamped controls,
+// best sequence) and seven 4-byte tables.
+__host__ __device__ constexpr size_t refine_lds_bytes(int tcap)
+{
+    return sizeof(float4) * ((size_t)(tcap + 1) * 4 + (size_t)tcap * (3 + REF_COMP + REF_JAC4) + tcap + 1) + 4 * 7 * (size_t)tcap;
+}
+
 // The running-cost weights DECAY^t as the forward passes form them (a running
 // product, one multiply per step), tabulated at compile time.
 struct DecayTable
@@ -1306,7 +1314,8 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                                                       float* __restrict__ nominal, int n_steps, float step, QuadConst C, CostConst Q,
                                                       dr_cost_cfg KC, float* __restrict__ actions, int advance,
                                                       const int* __restrict__ drone_inst, int n_drones,
-                                                      dr_drone_state* __restrict__ drones_out, int n_inst, float4* __restrict__ noise)
+                                                      dr_drone_state* __restrict__ drones_out, int n_inst, float4* __restrict__ noise,
+                                                      int tcap)
 {
     // Workgroups past the instances make the next step's noise rows (for a
     // rollout over noise made ahead) on the CUs the refinement leaves idle:
@@ -1319,25 +1328,30 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         return;
     }
     __shared__ dr_track s_track;
-    __shared__ float4 s_x[(DR_MAX_T + 1) * 4]; // state table, put_state layout
-    __shared__ float4 s_u[DR_MAX_T];           // the sequence being refined
-    __shared__ float4 s_gp[DR_MAX_T];          // dJ/dp of the running cost of x_{t+1}
-    __shared__ float s_w[DR_MAX_T];            // running-cost weight of step t
     __shared__ float s_psi[REF_SEGS * REF_COMP * 16];
-    __shared__ float s_lam[REF_SEGS * 16];     // lam_b of each reverse segment
-    __shared__ float4 s_gu[DR_MAX_T * REF_COMP]; // per step t and chain c of its segment: dJ/du_t along chain c
-    __shared__ float4 s_jac[DR_MAX_T * REF_JAC4]; // per step: the Jacobian blocks of quad_step_jacobian
+    __shared__ float s_lam[REF_SEGS * 16]; // lam_b of each reverse segment
     __shared__ float4 s_op[MAX_OPP], s_ov[MAX_OPP];
-    __shared__ float4 s_uc[DR_MAX_T + 1];  // the clamped controls the forward pass integrates (+1: read-ahead slot)
-    __shared__ uint32_t s_cross[DR_MAX_T]; // bit g: the move x_t -> x_{t+1} crosses gate g
     __shared__ float s_tot[4][(REF_BLOCK / WAVE) * 4]; // per-wave totals of the forward pass's four scans
-    __shared__ int s_xstep[DR_MAX_T], s_xgate[DR_MAX_T], s_xlaps[DR_MAX_T]; // gate crossings: step, gate, laps after
     __shared__ int s_nx;
-    __shared__ float4 s_ubest[DR_MAX_T]; // the lowest-cost sequence evaluated so far
     __shared__ float s_jbest;
     __shared__ int s_tend, s_better;
-    __shared__ float s_cb[DR_MAX_T]; // running cost of x_{t+1} without the winner test
-    __shared__ int s_tb[DR_MAX_T];   // its terminal code (collision or none)
+    // The per-step tables, in dynamic LDS sized by the batch's longest horizon
+    // tcap (refine_lds_bytes): at T = 100 two workgroups fit on one CU.
+    extern __shared__ float4 s_steps[];
+    float4* const s_x = s_steps;                 // (tcap + 1) * 4: state table, put_state layout
+    float4* const s_u = s_x + (tcap + 1) * 4;    // the sequence being refined
+    float4* const s_gp = s_u + tcap;             // dJ/dp of the running cost of x_{t+1}
+    float4* const s_gu = s_gp + tcap;            // tcap * REF_COMP: per step t and chain c of its segment: dJ/du_t along chain c
+    float4* const s_jac = s_gu + tcap * REF_COMP; // tcap * REF_JAC4: per step: the Jacobian blocks of quad_step_jacobian
+    float4* const s_uc = s_jac + tcap * REF_JAC4; // tcap + 1: the clamped controls the forward pass integrates (+1: read-ahead slot)
+    float4* const s_ubest = s_uc + tcap + 1;     // the lowest-cost sequence evaluated so far
+    float* const s_w = reinterpret_cast<float*>(s_ubest + tcap); // running-cost weight of step t
+    float* const s_cb = s_w + tcap;              // running cost of x_{t+1} without the winner test
+    int* const s_tb = reinterpret_cast<int*>(s_cb + tcap); // its terminal code (collision or none)
+    uint32_t* const s_cross = reinterpret_cast<uint32_t*>(s_tb + tcap); // bit g: the move x_t -> x_{t+1} crosses gate g
+    int* const s_xstep = reinterpret_cast<int*>(s_cross + tcap); // gate crossings: step, gate, laps after
+    int* const s_xgate = s_xstep + tcap;
+    int* const s_xlaps = s_xgate + tcap;
 
     const InstDev& I = insts[blockIdx.x];
     const int T = I.T, n_opp = I.n_opp, tid = threadIdx.x;
@@ -2262,10 +2276,37 @@ hipError_t launch_costgrad(int lanes, hipStream_t st, const InstDev* insts, int 
 hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, InstDyn* dyn, const double* part, const double* den,
                          const dr_track* tracks, const dr_drone_state* drones, float* nominal, int n_steps, float step,
                          const QuadConst& C, const CostConst& Q, const dr_cost_cfg& KC, float* actions, int advance,
-                         const int* drone_inst, int n_drones, dr_drone_state* drones_out, int noise_blocks, float4* noise)
+                         const int* drone_inst, int n_drones, dr_drone_state* drones_out, int n_cu, size_t lds_per_cu, float4* noise,
+                         int tcap)
 {
-    k_refine<<<n_inst + noise_blocks, REF_BLOCK, 0, st>>>(insts, dyn, part, den, tracks, drones, nominal, n_steps, step, C, Q, KC,
-                                                          actions, advance, drone_inst, n_drones, drones_out, n_inst, noise);
+    static const hipError_t attr = hipFuncSetAttribute((const void*)k_refine, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)refine_lds_bytes(DR_MAX_T));
+    if (attr != hipSuccess)
+        return attr;
+    if (tcap < 1 || tcap > DR_MAX_T)
+        return hipErrorInvalidValue;
+    // With noise made ahead (noise != nullptr), the workgroups past the
+    // instances make the next step's noise rows on the CUs the refinement
+    // leaves idle: as many as fit next to it (two per CU when the LDS of two
+    // workgroups fits in one CU's, one otherwise).
+    int noise_blocks = 0;
+    if (noise)
+    {
+        static size_t static_lds = 0;
+        if (!static_lds)
+        {
+            hipFuncAttributes fa;
+            if (hipFuncGetAttributes(&fa, (const void*)k_refine) != hipSuccess)
+                return hipErrorInvalidValue;
+            static_lds = fa.sharedSizeBytes;
+        }
+        const size_t per_wg = static_lds + refine_lds_bytes(tcap);
+        const int per_cu = 2 * per_wg <= lds_per_cu ? 2 : 1;
+        noise_blocks = per_cu * n_cu > n_inst + 64 ? per_cu * n_cu - n_inst : 64;
+    }
+    k_refine<<<n_inst + noise_blocks, REF_BLOCK, refine_lds_bytes(tcap), st>>>(insts, dyn, part, den, tracks, drones, nominal, n_steps,
+                                                                               step, C, Q, KC, actions, advance, drone_inst, n_drones,
+                                                                               drones_out, n_inst, noise, tcap);
     return hipGetLastError();
 }
 

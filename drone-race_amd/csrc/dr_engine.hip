@@ -1036,7 +1036,7 @@ int dr_engine_autotune(dr_engine* e, int trials, dr_launch_cfg* best, float* bes
                 for (int tg : {16, 32})
                     for (int rb : {128, 256})
                     {
-                        if ((split != 0 && rb != 128) || (split == 3 && nb > 2) || (split == 4 && (nb > 1 || e->refine_steps == 0)))
+                        if ((split != 0 && rb != 128) || (split == 3 && nb > 2) || (split == 4 && (nb > 2 || e->refine_steps == 0)))
                             continue;
                         dr_launch_cfg c = saved;
                         c.rollout_split = split;

@@ -722,7 +722,7 @@ __global__ __launch_bounds__(512, 4) void k_rollout_p8(const InstDev* __restrict
 // Eight-wave rollout over noise rows made ahead (rollout_split 4: by the
 // spare workgroups of the previous step's refinement, or by k_noise): no
 // Philox or Box-Muller in the rollout, the rows are read back.  Stages and the
-// chunk each works on at chunk period i (LDS rings of CH = 4 steps, one
+// chunk each works on at chunk period i (LDS rings of CH = 4 or 8 steps, one
 // barrier per period):
 //   R   noise rows (requested two chunks ahead), clamp(nominal + eps), body
 //       rates and attitude                                            chunk i
@@ -730,16 +730,18 @@ __global__ __launch_bounds__(512, 4) void k_rollout_p8(const InstDev* __restrict
 //   C   running cost and terminal test of step u of the chunk         chunk i - 2
 //   S   discounted sum in step order, stop at the first terminal      chunk i - 3
 // Roles (wave w works as (w + blockIdx) mod 8; waves w and w + 4 share a
-// SIMD): 0, 1, 4, 5 the running cost of steps u = 0, 1, 2, 3; 6 R and S; 7 X;
-// 2 and 3 idle, so the SIMD pairs are {C, C}, {C, C}, {-, R + S}, {-, X}.
-// The arithmetic of every sample is that of k_rollout.
+// SIMD): 0, 1, 4, 5 the running cost of steps u = 0, 1, 2, 3 (mod 4) of each
+// chunk; 6 R and S; 7 X; 2 and 3 idle, so the SIMD pairs are {C, C}, {C, C},
+// {-, R + S}, {-, X}.  A longer chunk halves the barriers and pipeline fill
+// and drain per step.  The arithmetic of every sample is that of k_rollout.
+template <int CH>
 __global__ __launch_bounds__(512, 4) void k_rollout_p8nz(const InstDev* __restrict__ insts, InstDyn* __restrict__ dyn,
                                                          const RTile* __restrict__ tiles, int n_tiles, const dr_track* __restrict__ tracks,
                                                          const dr_drone_state* __restrict__ drones, const float* __restrict__ nominal,
                                                          float* __restrict__ costs, const float4* __restrict__ noise,
                                                          float* __restrict__ tmin, QuadConst C, CostConst Q, dr_cost_cfg KC, int opp_stride)
 {
-    constexpr int CH = 4;
+    static_assert(CH % 4 == 0, "four cost waves share each chunk");
     extern __shared__ float4 s_opp[]; // [T][opp_stride]
     __shared__ float4 r_q[2][CH][WAVE]; // R -> X: attitude at the start of the step
     __shared__ float r_f[2][CH][WAVE];  // R -> X: clamped collective thrust
@@ -788,9 +790,10 @@ __global__ __launch_bounds__(512, 4) void k_rollout_p8nz(const InstDev* __restri
         float acc = 0.0f, decay = 1.0f;
         float4 last = make_float4(s.p[0], s.p[1], s.p[2], u2f((uint32_t)(s.gate | (s.laps << 8))));
         bool stop = false;
-        // R: the rows of the even chunks in ea, of the odd chunks in eb, each
-        // requested two periods before it is used
-        float4 ea[CH], eb[CH];
+        // R: with CH = 4 the rows of the even chunks in ea, of the odd chunks
+        // in eb, each requested two periods before it is used; with CH = 8
+        // (periods twice as long, registers scarce) ea alone, one period ahead
+        float4 ea[CH], eb[CH == 4 ? CH : 1];
         const auto request = [&](float4 (&e)[CH], int c) {
 #pragma unroll
             for (int u = 0; u < CH; ++u)
@@ -818,7 +821,8 @@ __global__ __launch_bounds__(512, 4) void k_rollout_p8nz(const InstDev* __restri
         if (live && role == 6)
         {
             request(ea, 0);
-            request(eb, 1);
+            if constexpr (CH == 4)
+                request(eb, 1);
         }
 
         for (int i = 0; i < n_chunks + 3; ++i)
@@ -830,15 +834,23 @@ __global__ __launch_bounds__(512, 4) void k_rollout_p8nz(const InstDev* __restri
                     const int c = i;
                     if (c < n_chunks)
                     {
-                        if ((c & 1) == 0)
+                        if constexpr (CH == 4)
                         {
-                            attitude(ea, c);
-                            request(ea, c + 2);
+                            if ((c & 1) == 0)
+                            {
+                                attitude(ea, c);
+                                request(ea, c + 2);
+                            }
+                            else
+                            {
+                                attitude(eb, c);
+                                request(eb, c + 2);
+                            }
                         }
                         else
                         {
-                            attitude(eb, c);
-                            request(eb, c + 2);
+                            attitude(ea, c);
+                            request(ea, c + 1);
                         }
                     }
                     // S: the chunk costed one period earlier, in step order
@@ -885,20 +897,27 @@ __global__ __launch_bounds__(512, 4) void k_rollout_p8nz(const InstDev* __restri
                 else if (role != 2 && role != 3)
                 {
                     const int c = i - 2;
-                    const int t = c * CH + cu;
-                    if (c >= 0 && c < n_chunks && t < T)
+                    if (c >= 0 && c < n_chunks)
                     {
-                        const float4 v = r_s[c % 3][cu][lane];
-                        const uint32_t code = f2u(v.w);
-                        QState x;
-                        x.p[0] = v.x;
-                        x.p[1] = v.y;
-                        x.p[2] = v.z;
-                        x.gate = (int)(code & 0xffu);
-                        x.laps = (int)code >> 8;
-                        int term;
-                        const float cst = stage_cost(C, Q, KC, s_track, x, s_opp + t * opp_stride, n_opp, &term);
-                        r_c[c & 1][cu][lane] = make_float2(cst, u2f((uint32_t)term));
+#pragma unroll
+                        for (int u = cu; u < CH; u += 4)
+                        {
+                            const int t = c * CH + u;
+                            if (t < T)
+                            {
+                                const float4 v = r_s[c % 3][u][lane];
+                                const uint32_t code = f2u(v.w);
+                                QState x;
+                                x.p[0] = v.x;
+                                x.p[1] = v.y;
+                                x.p[2] = v.z;
+                                x.gate = (int)(code & 0xffu);
+                                x.laps = (int)code >> 8;
+                                int term;
+                                const float cst = stage_cost(C, Q, KC, s_track, x, s_opp + t * opp_stride, n_opp, &term);
+                                r_c[c & 1][u][lane] = make_float2(cst, u2f((uint32_t)term));
+                            }
+                        }
                     }
                 }
             }
@@ -1353,9 +1372,13 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     int* const s_xgate = s_xstep + tcap;
     int* const s_xlaps = s_xgate + tcap;
 
+    // The refinement is a chain of dependent phases: its waves take the issue
+    // slots first where noise workgroups share the CU.
+    __builtin_amdgcn_s_setprio(3);
     const InstDev& I = insts[blockIdx.x];
     const int T = I.T, n_opp = I.n_opp, tid = threadIdx.x;
     const float margin = KC.gate_traversal_margin;
+    const uint32_t step0 = dyn[blockIdx.x].step; // read now, incremented at the end
     {
         const float* src = reinterpret_cast<const float*>(tracks + I.track);
         float* dst = reinterpret_cast<float*>(&s_track);
@@ -1881,8 +1904,9 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         actions[blockIdx.x * 4 + tid] = tid == 0 ? a.x : tid == 1 ? a.y : tid == 2 ? a.z : a.w;
     }
     if (tid == 0)
+        dyn[blockIdx.x].step = step0 + 1u;
+    if (tid == WAVE)
     {
-        dyn[blockIdx.x].step = dyn[blockIdx.x].step + 1u;
         // Closed loop (as k_update): the drone integrates the action, the
         // drones without a controller (spread over the workgroups) keep their
         // velocity.  Other workgroups may still read this step's drones, so
@@ -2146,8 +2170,16 @@ hipError_t launch_rollout(int block, int nb, int split, int grid, hipStream_t st
     const size_t shm = (size_t)max_T * opp_stride * sizeof(float4);
     if (split == 4)
     {
-        // tiles of 64 samples, eight waves, over the noise rows made ahead
-        k_rollout_p8nz<<<g, 512, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC, opp_stride);
+        // tiles of 64 samples, eight waves, over the noise rows made ahead;
+        // chunk length = 4 * noise_batch steps
+        if (nb == 1)
+            k_rollout_p8nz<4><<<g, 512, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                   opp_stride);
+        else if (nb == 2)
+            k_rollout_p8nz<8><<<g, 512, shm, st>>>(insts, dyn, tiles, n_tiles, tracks, drones, nominal, costs, noise, tmin, C, Q, KC,
+                                                   opp_stride);
+        else
+            return hipErrorInvalidValue;
         return hipGetLastError();
     }
     if (split == 3)

@@ -165,7 +165,8 @@ typedef struct dr_launch_cfg
     int32_t rollout_split;    /* 0: one lane per sample; 1: noise wave + dynamics wave; 2: four-stage wave pipeline;
                                  3: eight-stage wave pipeline (noise_batch 1 or 2); 4: with the adjoint refinement on,
                                  an eight-wave pipeline that reads noise rows made ahead by the previous step's
-                                 refinement launch on the CUs it leaves idle (as 3 without the refinement) */
+                                 refinement launch on the CUs it leaves idle, chunks of 4 * noise_batch steps (as 3
+                                 without the refinement) */
 } dr_launch_cfg;
 
 typedef struct dr_engine dr_engine;

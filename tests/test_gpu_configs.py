@@ -246,23 +246,24 @@ def test_refined_closed_loop_launch_paths(pkg):
             assert np.array_equal(a, b)
 
 
-def test_noise_made_ahead_matches_eight_stage_rollout(pkg):
+@pytest.mark.parametrize("nb", [1, 2])
+def test_noise_made_ahead_matches_eight_stage_rollout(pkg, nb):
     """rollout_split 4 (the refinement launch's spare workgroups make the next step's noise rows,
-    the rollout reads them) against rollout_split 3 (every rollout makes its own): the same costs,
-    actions, nominals and drone states bit for bit over closed-loop steps, steps without advance,
-    one-step and three-step graphs, plain launches and a step counter set by hand; the rows in
-    the buffer after a split-4 step are the next step's."""
+    the rollout reads them; chunks of 4 * noise_batch steps) against rollout_split 3 (every rollout
+    makes its own): the same costs, actions, nominals and drone states bit for bit over closed-loop
+    steps, steps without advance, one-step and three-step graphs, plain launches and a step
+    counter set by hand; the rows in the buffer after a split-4 step are the next step's."""
     sc = pkg.scenarios
-    scn = sc.c4_eight_drones(512, 40)
+    scn = sc.c4_eight_drones(512, 42)  # T = 42: the last chunk is partial for both chunk lengths
     scn.instances = scn.instances[:6]
     n = len(scn.instances)
-    say(f"case: {scn.name} with 6 controllers, refined, rollout_split 4 against 3")
+    say(f"case: {scn.name} with 6 controllers, refined, rollout_split 4 (noise_batch {nb}) against 3")
     ref = pkg.Engine(scn)
     nz = pkg.Engine(scn)
     for e in (ref, nz):
         e.set_refine(3, 0.05, 10)
     ref.set_launch(rollout_split=3, noise_batch=2, use_graph=1, steps_per_graph=1)
-    nz.set_launch(rollout_split=4, noise_batch=1, use_graph=1, steps_per_graph=3)
+    nz.set_launch(rollout_split=4, noise_batch=nb, use_graph=1, steps_per_graph=3)
 
     def same(tag):
         for i in range(n):
@@ -282,7 +283,7 @@ def test_noise_made_ahead_matches_eight_stage_rollout(pkg):
     for i in range(n):
         ref.set_step(i, 7 + i)
         nz.set_step(i, 7 + i)
-    nz.set_launch(rollout_split=4, noise_batch=1, use_graph=0)
+    nz.set_launch(rollout_split=4, noise_batch=nb, use_graph=0)
     assert np.array_equal(ref.step(2, advance=True), nz.step(2, advance=True))
     same("step counters set by hand, plain launches")
     nz.set_refine(0, 0.05, 10)  # without the refinement split 4 runs as split 3

@@ -440,11 +440,13 @@ __device__ __forceinline__ float4 control_noise(uint32_t key0, uint32_t key1, ui
 }
 
 // The noise rows of every instance, made ahead of the rollout that reads
-// them (rollout_split 4): worker `wk` of `n_wk` writes elements wk, wk + n_wk,
-// ... of each instance's [K][T] block (consecutive workers, consecutive
-// float4: coalesced), four elements per trip so that four Philox chains are
-// in flight.  ahead: the step after the current one (dyn.step_ahead, written
-// by k_wavg before the refinement); otherwise the current step.
+// them (rollout_split 4).  Worker `wk` of `n_wk` takes, instance by instance,
+// the units wk, wk + n_wk, ... of K x Q units: unit (k, q) is the span of
+// ceil(T / Q) steps q of sample k's row, Q = ceil(n_wk / K) so that every
+// worker has a unit.  Within a unit the Philox rounds that read only the
+// sample index are made once, and four steps are in flight per trip.  ahead:
+// the step after the current one (dyn.step_ahead, written by k_wavg before
+// the refinement); otherwise the current step.
 __device__ void make_noise_rows(const InstDev* __restrict__ insts, const InstDyn* __restrict__ dyn, int n_inst, float4* __restrict__ noise,
                                 int wk, int n_wk, bool ahead)
 {
@@ -452,23 +454,25 @@ __device__ void make_noise_rows(const InstDev* __restrict__ insts, const InstDyn
     {
         const InstDev& I = insts[i];
         const uint32_t step = ahead ? dyn[i].step_ahead : dyn[i].step;
-        const int T = I.T, n = I.K * T;
+        const int T = I.T, K = I.K;
+        const int Q = min(T, (n_wk + K - 1) / K), span = (T + Q - 1) / Q;
         const float sg0 = I.sigma[0], sg1 = I.sigma[1], sg2 = I.sigma[2], sg3 = I.sigma[3];
-        float4* out = noise + I.noise_off;
-        for (int e0 = wk; e0 < n; e0 += 4 * n_wk)
+        for (int u = wk; u < K * Q; u += n_wk)
         {
-            float4 v[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
+            const int k = u / Q, q = u - k * Q;
+            const int t0 = q * span, t1 = min(T, t0 + span);
+            float4* row = noise + I.noise_off + (long long)k * T;
+            for (int t = t0; t < t1; t += 4)
             {
-                const int e = min(e0 + j * n_wk, n - 1);
-                const int k = e / T, t = e - k * T;
-                v[j] = control_noise(I.key0, I.key1, (uint32_t)k, (uint32_t)t, step, sg0, sg1, sg2, sg3);
-            }
+                float4 v[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (e0 + j * n_wk < n)
-                    out[e0 + j * n_wk] = v[j];
+                for (int j = 0; j < 4; ++j)
+                    v[j] = control_noise(I.key0, I.key1, (uint32_t)k, (uint32_t)min(t + j, t1 - 1), step, sg0, sg1, sg2, sg3);
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (t + j < t1)
+                        row[t + j] = v[j];
+            }
         }
     }
 }

@@ -57,6 +57,20 @@ DR_HD float u2f(uint32_t u)
     return c.f;
 }
 
+// a ^ b ^ c.  On the device one three-input bitwise instruction
+// (v_bitop3_b32 with the truth table of a three-way XOR, 0x96) instead of two
+// XORs; on the host two XORs.  The same bits either way.
+DR_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 DR_HD U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
 {
 #pragma unroll
@@ -72,9 +86,11 @@ DR_HD U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
         const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
         const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
         U4 n;
-        n.x = hi1 ^ c.y ^ k0;
+        // the first round's words may be uniform across a wave (the counter
+        // words of a step): plain XORs there keep them in scalar registers
+        n.x = r == 0 ? hi1 ^ c.y ^ k0 : xor3(hi1, c.y, k0);
         n.y = lo1;
-        n.z = hi0 ^ c.w ^ k1;
+        n.z = r == 0 ? hi0 ^ c.w ^ k1 : xor3(hi0, c.w, k1);
         n.w = lo0;
         c = n;
     }

@@ -42,6 +42,10 @@ FLOPS_PER_SAMPLE_STEP = 292.07
 # count_refine); tests/test_bench_constants.py keeps both equal to the counter.
 C4_FLOPS_PER_SAMPLE_STEP = 356.12
 C4_REFINE_FLOPS = 2420640
+# The noise part of the per-sample-step figures (Box-Muller and sigma scaling; oracle/count_ops.py
+# NOISE_FLOPS_PER_STEP).  With the noise rows made ahead (rollout_split 4) the refinement launch
+# makes them for the next step and the rollout reads them back.
+NOISE_FLOPS_PER_SAMPLE_STEP = 96
 # HBM bytes per rollout sample-step the rollout kernel must move: the float4
 # noise row it writes (read back once by the weighted average).
 ROLLOUT_BYTES_PER_SAMPLE_STEP = 16.0
@@ -329,16 +333,29 @@ def vector_roofline(kernel, flops, ms, note):
 
 def c4_rooflines(pkg, adj):
     """Rooflines of the adjoint leg's kernels from its per-kernel HIP-event times; the dominant one
-    (largest device time per step) is the headline roofline."""
+    (largest device time per step) is the headline roofline.  With the noise rows made ahead
+    (rollout_split 4), the rollout's flops exclude the noise and read its rows back, and the
+    refinement launch (8 latency-bound refinement workgroups + the next step's noise rows on the
+    CUs they leave idle) counts the noise flops and writes the rows."""
     sc = pkg.scenarios
     scn = sc.c4_eight_drones(4096)
     ss = scn.sample_steps_per_step
     kt = adj["kernel_ms_per_step"]
-    r = {"k_rollout": vector_roofline("k_rollout", C4_FLOPS_PER_SAMPLE_STEP * ss, kt["rollout_ms"],
-                                      "oracle-counted flops per sample-step (8 drones: 7 opponents in the cost)"),
-         "k_refine": vector_roofline("k_refine", C4_REFINE_FLOPS, kt["refine_ms"],
+    ahead = adj.get("launch", {}).get("rollout_split") == 4
+    roll_flops = (C4_FLOPS_PER_SAMPLE_STEP - (NOISE_FLOPS_PER_SAMPLE_STEP if ahead else 0)) * ss
+    ref_flops = C4_REFINE_FLOPS + (NOISE_FLOPS_PER_SAMPLE_STEP * ss if ahead else 0)
+    r = {"k_rollout": vector_roofline("k_rollout", roll_flops, kt["rollout_ms"],
+                                      "oracle-counted flops per sample-step (8 drones: 7 opponents in the cost)" +
+                                      ("; the noise rows are read back (made by the previous refinement launch)" if ahead else "")),
+         "k_refine": vector_roofline("k_refine", ref_flops, kt["refine_ms"],
                                      "oracle-counted flops of the one-lane adjoint (3 iterates of J and dJ/dU + the last J) per "
-                                     "launch of 8 controllers; bound by its serial per-step dependency chain, not by issue")}
+                                     "launch of 8 controllers, bound by its serial per-step dependency chain, not by issue" +
+                                     ("; plus the next step's noise rows (96 flops per sample-step, Philox integer work "
+                                      "uncounted) made by the launch's other workgroups on the CUs the refinement leaves idle"
+                                      if ahead else ""))}
+    if ahead:
+        r["k_refine"]["parts"] = {"refine_flops": C4_REFINE_FLOPS, "noise_flops": NOISE_FLOPS_PER_SAMPLE_STEP * ss,
+                                  "noise_bytes_written": ROLLOUT_BYTES_PER_SAMPLE_STEP * ss}
     wavg_bytes = ROLLOUT_BYTES_PER_SAMPLE_STEP * ss
     ms = kt["reduce_ms"]
     ach = wavg_bytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
@@ -350,9 +367,10 @@ def c4_rooflines(pkg, adj):
     if os.path.exists(tfile):
         with open(tfile) as f:
             tr = json.load(f)
-        for k in r:
-            if k in tr:
-                r[k]["traffic"] = tr[k]
+        if tr.get("launch", {}).get("rollout_split") == adj.get("launch", {}).get("rollout_split"):
+            for k in r:
+                if k in tr:
+                    r[k]["traffic"] = tr[k]
     return dom, r
 
 

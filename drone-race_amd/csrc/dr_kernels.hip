@@ -443,7 +443,7 @@ __device__ __forceinline__ float4 control_noise(uint32_t key0, uint32_t key1, ui
 // them (rollout_split 4).  Worker `wk` of `n_wk` takes, instance by instance,
 // the units wk, wk + n_wk, ... of K x Q units: unit (k, q) is the span of
 // ceil(T / Q) steps q of sample k's row, Q = ceil(n_wk / K) so that every
-// worker has a unit.  Within a unit the Philox rounds that read only the
+// worker has a unit where the rows allow.  Within a unit the Philox rounds that read only the
 // sample index are made once, and four steps are in flight per trip.  ahead:
 // the step after the current one (dyn.step_ahead, written by k_wavg before
 // the refinement); otherwise the current step.
@@ -455,7 +455,7 @@ __device__ void make_noise_rows(const InstDev* __restrict__ insts, const InstDyn
         const InstDev& I = insts[i];
         const uint32_t step = ahead ? dyn[i].step_ahead : dyn[i].step;
         const int T = I.T, K = I.K;
-        const int Q = min(T, (n_wk + K - 1) / K), span = (T + Q - 1) / Q;
+        const int Q = min((T + 3) / 4, (n_wk + K - 1) / K), span = (T + Q - 1) / Q;
         const float sg0 = I.sigma[0], sg1 = I.sigma[1], sg2 = I.sigma[2], sg3 = I.sigma[3];
         for (int u = wk; u < K * Q; u += n_wk)
         {

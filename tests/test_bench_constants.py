@@ -49,3 +49,9 @@ def test_eight_drone_flops_match_counter(pkg):
     measured = count_ops.count(scn)
     assert abs(bench.C4_FLOPS_PER_SAMPLE_STEP - measured) < 0.01, measured
     assert bench.C4_REFINE_FLOPS == count_ops.count_refine(scn, n_grad=3, ck=10)
+
+
+def test_noise_flops_match_counter():
+    bench = _load(os.path.join(ROOT, "bench.py"), "bench_mod")
+    count_ops = _load(os.path.join(ROOT, "oracle", "count_ops.py"), "count_ops_mod")
+    assert bench.NOISE_FLOPS_PER_SAMPLE_STEP == count_ops.NOISE_FLOPS_PER_STEP

@@ -1202,11 +1202,11 @@ constexpr int REF_COMP = 14; // affine part + 13 state directions
 constexpr int REF_JAC4 = (QJAC + 3) / 4; // float4 slots per step of the Jacobian table
 
 // Dynamic LDS of k_refine for horizons up to tcap: the float4 tables (state
-// slots, sequence, dJ/dp, chain gradients, Jacobian blocks, clamped controls,
-// best sequence) and seven 4-byte tables.
+// slots, sequence, dJ/dp, chain gradients, Jacobian blocks, best sequence)
+// and seven 4-byte tables.
 __host__ __device__ constexpr size_t refine_lds_bytes(int tcap)
 {
-    return sizeof(float4) * ((size_t)(tcap + 1) * 4 + (size_t)tcap * (3 + REF_COMP + REF_JAC4) + tcap + 1) + 4 * 7 * (size_t)tcap;
+    return sizeof(float4) * ((size_t)(tcap + 1) * 4 + (size_t)tcap * (3 + REF_COMP + REF_JAC4)) + 4 * 7 * (size_t)tcap;
 }
 
 // The running-cost weights DECAY^t as the forward passes form them (a running
@@ -1356,7 +1356,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     __shared__ float4 s_op[MAX_OPP], s_ov[MAX_OPP];
     __shared__ float s_tot[4][(REF_BLOCK / WAVE) * 4]; // per-wave totals of the forward pass's four scans
     __shared__ int s_nx;
-    __shared__ float s_jbest;
+    __shared__ float s_jbest, s_fin;
     __shared__ int s_tend, s_better;
     // The per-step tables, in dynamic LDS sized by the batch's longest horizon
     // tcap (refine_lds_bytes): at T = 100 two workgroups fit on one CU.
@@ -1366,8 +1366,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     float4* const s_gp = s_u + tcap;             // dJ/dp of the running cost of x_{t+1}
     float4* const s_gu = s_gp + tcap;            // tcap * REF_COMP: per step t and chain c of its segment: dJ/du_t along chain c
     float4* const s_jac = s_gu + tcap * REF_COMP; // tcap * REF_JAC4: per step: the Jacobian blocks of quad_step_jacobian
-    float4* const s_uc = s_jac + tcap * REF_JAC4; // tcap + 1: the clamped controls the forward pass integrates (+1: read-ahead slot)
-    float4* const s_ubest = s_uc + tcap + 1;     // the lowest-cost sequence evaluated so far
+    float4* const s_ubest = s_jac + tcap * REF_JAC4; // the lowest-cost sequence evaluated so far
     float* const s_w = reinterpret_cast<float*>(s_ubest + tcap); // running-cost weight of step t
     float* const s_cb = s_w + tcap;              // running cost of x_{t+1} without the winner test
     int* const s_tb = reinterpret_cast<int*>(s_cb + tcap); // its terminal code (collision or none)
@@ -1476,17 +1475,9 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
 
     for (int it = 0;; ++it)
     {
-        // P: the clamped controls of this iterate; crossing bits cleared
-        for (int t = tid; t < T; t += REF_BLOCK)
-        {
-            const float4 u4 = s_u[t];
-            const float in[4] = {u4.x, u4.y, u4.z, u4.w};
-            float out[4];
-            clamp_control(C, in, out);
-            s_uc[t] = make_float4(out[0], out[1], out[2], out[3]);
-            s_cross[t] = 0u;
-        }
-        __syncthreads();
+        // The iterate's controls are already clamped (the MPPI update and every
+        // projected step clamp them, and the clamp is idempotent), so the
+        // forward pass integrates s_u as it is.
         // A: forward pass of the dynamics, every state into the state table, as
         // block-wide scans over the steps (lane t holds step t).  Each
         // recurrence is associative in exact arithmetic: the body rates and the
@@ -1503,7 +1494,9 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             const int t = tid;
             const bool in = t < T;
             float* xs = s_xf + (t + 1) * 16; // slot t + 1
-            const float4 u4 = in ? s_uc[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const float4 u4 = in ? s_u[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (in)
+                s_cross[t] = 0u; // crossing bits of this pass (set after the scans' barriers)
             const float id4[4] = {1.0f, 0.0f, 0.0f, 0.0f};
             float wt[3], qt[4], vt[3];
             {
@@ -1706,7 +1699,9 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         }
         __syncthreads();
         // C: the winner test on the running cost of every x_{t+1} (laps now
-        // known), the first terminal step, and the discounted cost for J
+        // known), the first terminal step, and the discounted cost for J; the
+        // last lane makes the terminal cost of x_T meanwhile (the trajectory
+        // that runs the whole horizon, the usual case)
         for (int t = tid; t < T; t += REF_BLOCK)
         {
             int term = s_tb[t];
@@ -1714,6 +1709,11 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             if (term != TERM_NONE)
                 atomicMin(&s_tend, t);
             s_gp[t].w = c * s_w[t]; // discounted running cost
+        }
+        if (tid == REF_BLOCK - 1)
+        {
+            float w;
+            s_fin = final_cost(KC, s_track, get_state(s_x, T, 1, &w));
         }
         __syncthreads();
         const int te = s_tend;
@@ -1731,8 +1731,8 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             if (tid == 0)
             {
                 float w;
-                const QState s = get_state(s_x, te + 1, 1, &w);
-                const float J = (float)(acc + (double)final_cost(KC, s_track, s));
+                const float fin = te == T - 1 ? s_fin : final_cost(KC, s_track, get_state(s_x, te + 1, 1, &w));
+                const float J = (float)(acc + (double)fin);
                 const bool better = it == 0 || J < s_jbest;
                 if (better)
                     s_jbest = J;

@@ -277,3 +277,40 @@ def test_cost_grad_rejects_mismatched_shapes(pkg):
     with pytest.raises(ValueError):
         rollout_cost(torch.zeros((2, T + 1, 4), device="cuda:0"), eng, 0, 10)
     eng.close()
+
+
+def test_refined_step_through_many_gates(pkg, oracle):
+    """Eight gates 0.5 m apart on a straight line, the drone flying through them at 5 m/s: within
+    the 60-step horizon (1.2 s) every iterate passes most of the gates (J below -10 x 5: the
+    terminal cost is -10 per gate of progress and the running costs are not negative), so the
+    refinement's gate counter runs through many crossings per pass.  The refined nominal and
+    action against the oracle's iterate of lowest J."""
+    say("case: refined step through eight gates 0.5 m apart")
+    sc = pkg.scenarios
+    T, K, n = 60, 1024, 8
+    centers = [[0.5 * i, 0.0, 4.0] for i in range(n)]
+    tr = sc._make_track(centers, [[1.0, 0.0, 0.0]] * n, 1.0)
+    drones = (pkg.abi.DroneState * 2)()
+    for j, y in enumerate((0.0, 6.0)):
+        d = drones[j]
+        d.p[0], d.p[1], d.p[2] = -0.6, y, 4.0
+        d.q[0] = 1.0
+        d.v[0] = 5.0
+        d.gate, d.laps = n - 1, 0
+    scn = sc.Scenario("gates", [tr], [sc.instance(K, T, 0, 0, 31)], [0, 0], drones, 1, "gates")
+    o = pkg.abi.Opponent()
+    for c in range(3):
+        o.p[c], o.v[c] = drones[1].p[c], drones[1].v[c]
+    opp = [o]
+    eng = pkg.Engine(scn)
+    eng.set_refine(3, 0.05, 10)
+    nom0 = eng.get_nominal(0).copy()
+    act = eng.step(1)
+    ins = scn.instances[0]
+    costs, noise, _ = oracle.rollout(0, scn.params, scn.cost, scn.tracks[0], scn.drones[0], opp, nom0, K, T, ins.seed, 0,
+                                     np.array(ins.sigma[:], np.float32))
+    upd = oracle.update(0, scn.params, costs, noise, nom0, ins.lambda_, np.array(ins.shift_fill[:], np.float32))
+    Us, Js = refine_oracle(oracle, scn, opp, upd["updated"].astype(np.float32), 3, 0.05)
+    assert Js.max() < -50.0
+    check_refined(eng, act, Us, Js)
+    eng.close()

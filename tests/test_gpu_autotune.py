@@ -176,3 +176,42 @@ def test_bm_grid_shapes_and_autotune(pkg, kind):
             np.testing.assert_array_equal(o[4], r[4])
             assert same_bits(o[5], r[5]) and same_bits(o[6], r[6])
         say(f"  grid {grid}: identical to one workgroup per tile")
+
+
+@pytest.mark.parametrize("refine", [0, 3])
+def test_engine_autotune_restores_and_agrees(pkg, refine):
+    """The batch engine's autotuner (rollout structure, noise batch, weighted-average chunk and
+    step group, then the steps per graph; with the refinement on also the rollout over noise made
+    ahead) on three uneven instances: the plans, step counters and drone states are as before it,
+    and the next closed-loop step agrees with an engine left at its default launch -- per-sample
+    costs bit-exact, plans and actions within 1e-6 relative (other reduce chunks can move the FP32
+    update by one rounding)."""
+    from test_gpu_parity import small_scenario
+    sc = pkg.scenarios
+    scn = small_scenario(sc, n_samples=512, n_timesteps=30, n_drones=4, controlled=3, seed=9, track="lemniscate")
+    scn.instances[1] = sc.instance(1000, 57, 0, 1, sc.seed_mix(9, 1))
+    scn.instances[2] = sc.instance(2048, 100, 0, 2, sc.seed_mix(9, 2))
+    n = len(scn.instances)
+    say(f"case: batch-engine autotune, 3 uneven instances, refinement {refine}")
+    tuned = pkg.Engine(scn)
+    ref = pkg.Engine(scn)
+    for e in (tuned, ref):
+        e.set_refine(refine, 0.05, 10)
+        e.step(2, advance=True)
+    noms = [tuned.get_nominal(i).copy() for i in range(n)]
+    steps = [d["step"] for d in tuned.diag()]
+    drones = [list(d.p) + list(d.v) for d in tuned.get_drones()]
+    cfg, ms = tuned.autotune(trials=2)
+    say(f"  tuned launch {cfg} ({ms:.3f} ms per step)")
+    for i in range(n):
+        assert same_bits(tuned.get_nominal(i), noms[i])
+    assert [d["step"] for d in tuned.diag()] == steps
+    assert [list(d.p) + list(d.v) for d in tuned.get_drones()] == drones
+    a = tuned.step(1, advance=True)
+    b = ref.step(1, advance=True)
+    assert rel_err(a, b) <= REL
+    for i in range(n):
+        np.testing.assert_array_equal(tuned.costs(i), ref.costs(i))
+        assert rel_err(tuned.get_nominal(i), ref.get_nominal(i)) <= REL
+    tuned.close()
+    ref.close()

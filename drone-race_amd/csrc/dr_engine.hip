@@ -46,6 +46,7 @@ hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, InstD
                          const int* drone_inst, int n_drones, dr_drone_state* drones_out, int n_cu, size_t lds_per_cu, float4* noise,
                          int tcap);
 hipError_t launch_noise(int blocks, hipStream_t st, const InstDev* insts, const InstDyn* dyn, int n_inst, float4* noise);
+hipError_t refine_allow_lds();
 
 } // namespace dr
 
@@ -546,6 +547,8 @@ int dr_engine_create(const dr_quad_params* params, const dr_cost_cfg* cost, cons
                 e->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
         }
     }
+    if (refine_allow_lds() != hipSuccess)
+        return bail(fail(DR_EHIP, "hipFuncSetAttribute (refinement LDS) failed"));
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||

@@ -2294,6 +2294,13 @@ hipError_t launch_update(int n_inst, hipStream_t st, const InstDev* insts, InstD
     return hipGetLastError();
 }
 
+// The refinement's dynamic LDS exceeds the default limit at long horizons:
+// allowed up to DR_MAX_T on the current device (once per engine).
+hipError_t refine_allow_lds()
+{
+    return hipFuncSetAttribute((const void*)k_refine, hipFuncAttributeMaxDynamicSharedMemorySize, (int)refine_lds_bytes(DR_MAX_T));
+}
+
 hipError_t launch_noise(int blocks, hipStream_t st, const InstDev* insts, const InstDyn* dyn, int n_inst, float4* noise)
 {
     k_noise<<<blocks, 256, 0, st>>>(insts, dyn, n_inst, noise);
@@ -2315,10 +2322,6 @@ hipError_t launch_refine(int n_inst, hipStream_t st, const InstDev* insts, InstD
                          const int* drone_inst, int n_drones, dr_drone_state* drones_out, int n_cu, size_t lds_per_cu, float4* noise,
                          int tcap)
 {
-    static const hipError_t attr = hipFuncSetAttribute((const void*)k_refine, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                       (int)refine_lds_bytes(DR_MAX_T));
-    if (attr != hipSuccess)
-        return attr;
     if (tcap < 1 || tcap > DR_MAX_T)
         return hipErrorInvalidValue;
     // With noise made ahead (noise != nullptr), the workgroups past the

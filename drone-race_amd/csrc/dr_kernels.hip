@@ -1835,16 +1835,21 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                 for (int c = 0; c < REF_COMP; ++c)
                     pn[c] = P[c * 16 + i];
                 __builtin_amdgcn_sched_barrier(0);
+                // every lane takes the whole of lam_b from the segment's slot,
+                // written by lanes 0..12 (four 16-byte LDS reads instead of
+                // thirteen lane reads in a row; the slot was written by this
+                // wave, so a wait on the write is all the ordering needed)
+                const float4* lb = reinterpret_cast<const float4*>(s_lam + sg * 16);
+                const float4 l0 = lb[0], l1 = lb[1], l2 = lb[2], l3 = lb[3];
+                const float lj[13] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w, l2.x, l2.y, l2.z, l2.w, l3.x};
                 float acc = pc[0];
 #pragma unroll
                 for (int j = 0; j < 13; ++j)
-                {
-                    const float lj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lam), j));
-                    acc = fma_(pc[1 + j], lj, acc);
-                }
+                    acc = fma_(pc[1 + j], lj[j], acc);
                 lam = acc;
                 if (tid < 13)
                     s_lam[(sg - 1) * 16 + tid] = lam;
+                __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0): the slot is written before the next segment reads it
 #pragma unroll
                 for (int c = 0; c < REF_COMP; ++c)
                     pc[c] = pn[c];

@@ -1356,7 +1356,7 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
     __shared__ float4 s_op[MAX_OPP], s_ov[MAX_OPP];
     __shared__ float s_tot[4][(REF_BLOCK / WAVE) * 4]; // per-wave totals of the forward pass's four scans
     __shared__ int s_nx;
-    __shared__ float s_jbest, s_fin;
+    __shared__ float s_jbest, s_fin, s_fgrad[3];
     __shared__ int s_tend, s_better;
     // The per-step tables, in dynamic LDS sized by the batch's longest horizon
     // tcap (refine_lds_bytes): at T = 100 two workgroups fit on one CU.
@@ -1700,8 +1700,8 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         __syncthreads();
         // C: the winner test on the running cost of every x_{t+1} (laps now
         // known), the first terminal step, and the discounted cost for J; the
-        // last lane makes the terminal cost of x_T meanwhile (the trajectory
-        // that runs the whole horizon, the usual case)
+        // last lane makes the terminal cost of x_T and its dJ/dp meanwhile (the
+        // trajectory that runs the whole horizon, the usual case)
         for (int t = tid; t < T; t += REF_BLOCK)
         {
             int term = s_tb[t];
@@ -1713,7 +1713,15 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         if (tid == REF_BLOCK - 1)
         {
             float w;
-            s_fin = final_cost(KC, s_track, get_state(s_x, T, 1, &w));
+            const QState s = get_state(s_x, T, 1, &w);
+            s_fin = final_cost(KC, s_track, s);
+            if (it < n_steps)
+            {
+                float g[3] = {0.0f, 0.0f, 0.0f};
+                final_cost_grad(KC, s_track, s, g);
+                for (int i = 0; i < 3; ++i)
+                    s_fgrad[i] = g[i];
+            }
         }
         __syncthreads();
         const int te = s_tend;
@@ -1747,21 +1755,23 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
             break;
         if (tid == REF_BLOCK - 1)
         {
-            // terminal adjoint: final-cost gradient, then the running cost of x_{te+1}
-            float w;
-            const QState s = get_state(s_x, te + 1, 1, &w);
-            const float tau = C.dt * (float)(te + 1);
-            const auto opp_at = [&](int j) {
-                const float4 p = s_op[j], v = s_ov[j];
-                return make_float4(fma_(v.x, tau, p.x), fma_(v.y, tau, p.y), fma_(v.z, tau, p.z), 0.0f);
-            };
+            // terminal adjoint: the final cost's dJ/dp at x_{te+1} (made in C
+            // when te = T - 1) plus the running cost's there (the gradient lane
+            // of step te made it from the same position, opponents and weight)
             float g[3] = {0.0f, 0.0f, 0.0f};
-            final_cost_grad(KC, s_track, s, g);
-            stage_cost_grad_at(C, Q, KC, s_track, s, n_opp, opp_at, s_w[te], g);
+            if (te == T - 1)
+                for (int i = 0; i < 3; ++i)
+                    g[i] = s_fgrad[i];
+            else
+            {
+                float w;
+                final_cost_grad(KC, s_track, get_state(s_x, te + 1, 1, &w), g);
+            }
+            const float4 gp = s_gp[te];
             float* lam = s_lam + (n_seg - 1) * 16;
-            lam[0] = g[0];
-            lam[1] = g[1];
-            lam[2] = g[2];
+            lam[0] = g[0] + gp.x;
+            lam[1] = g[1] + gp.y;
+            lam[2] = g[2] + gp.z;
             for (int i = 3; i < 13; ++i)
                 lam[i] = 0.0f;
         }

@@ -16,12 +16,37 @@ __device__ __forceinline__ float wave_min(float v)
     return v;
 }
 
+// v + the v of the lane that the DPP control CTRL selects (both 32-bit halves
+// moved); lanes of the rows that ROWS leaves out add +0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_add(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+    return v + __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the wave, the same value in every lane, of full waves only.  The
+// pairing tree of a butterfly over lane masks 1, 2, 4, ..., 32 (each addition
+// takes the same two operands, so the same bits), in DPP instead of LDS
+// permutes: pairs within quads, then the other half of each 8 and of each 16
+// lanes (mirrors: after the levels before, every lane of a half holds the
+// same value), then rows 0 + 1 and 2 + 3 in rows 1 and 3 (row broadcast of
+// lanes 15), then all four rows in row 3 (lane 31); lane 63's total is read
+// out for every lane.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1)
-        v += __shfl_xor(v, m, WAVE_LANES);
-    return v;
+    v = dpp_add<0xB1, 0xf>(v);  // quad_perm [1, 0, 3, 2]
+    v = dpp_add<0x4E, 0xf>(v);  // quad_perm [2, 3, 0, 1]
+    v = dpp_add<0x141, 0xf>(v); // row_half_mirror
+    v = dpp_add<0x140, 0xf>(v); // row_mirror
+    v = dpp_add<0x142, 0xa>(v); // row_bcast:15 into rows 1 and 3
+    v = dpp_add<0x143, 0xc>(v); // row_bcast:31 into rows 2 and 3
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, WAVE_LANES - 1);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), WAVE_LANES - 1);
+    return __longlong_as_double(((long long)hi << 32) | (long long)lo);
 }
 
 // Sum of n values p[0], p[stride], ... in index order; loads are issued in

@@ -8,12 +8,26 @@ namespace dr {
 
 constexpr int WAVE_LANES = 64;
 
+// fminf(v, the v of the lane that the DPP control CTRL selects); lanes of the
+// rows that ROWS leaves out take min(v, +inf) = v.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_min(float v)
+{
+    const int inf = __float_as_int(__builtin_inff());
+    return fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(inf, __float_as_int(v), CTRL, ROWS, 0xf, false)));
+}
+
+// Minimum over the wave, the same value in every lane, of full waves only:
+// the pairing tree of wave_sum below (the butterfly's), read out of lane 63.
 __device__ __forceinline__ float wave_min(float v)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1)
-        v = fminf(v, __shfl_xor(v, m, WAVE_LANES));
-    return v;
+    v = dpp_min<0xB1, 0xf>(v);  // quad_perm [1, 0, 3, 2]
+    v = dpp_min<0x4E, 0xf>(v);  // quad_perm [2, 3, 0, 1]
+    v = dpp_min<0x141, 0xf>(v); // row_half_mirror
+    v = dpp_min<0x140, 0xf>(v); // row_mirror
+    v = dpp_min<0x142, 0xa>(v); // row_bcast:15 into rows 1 and 3
+    v = dpp_min<0x143, 0xc>(v); // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), WAVE_LANES - 1));
 }
 
 // v + the v of the lane that the DPP control CTRL selects (both 32-bit halves

@@ -1175,9 +1175,9 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
 //     cost of x_{t+1} without the winner test and its dJ/dp (both read the
 //     position only).
 //  G  wave 0 runs the gate counter by searching the crossing bits for the
-//     next gate's crossing (one ballot per 64 steps) and every state slot
-//     takes the gate and laps after the crossings before it.
-//  C  one lane per step t: the winner test on x_{t+1} (laps now known), the
+//     next gate's crossing (one ballot per 64 steps).
+//  C  one lane per step t: the gate and laps of x_{t+1} (after the crossings
+//     before it) into its state slot, the winner test on x_{t+1}, the
 //     discounted running cost; the first terminal step t_end ends the
 //     trajectory (cost_grad_lane's break); wave 0 sums J.
 //  D  the reverse recurrence lam_t = A_t^T lam_{t+1} + g_t is linear in lam.
@@ -1686,34 +1686,44 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                 s_nx = n_x;
         }
         __syncthreads();
-        for (int k = tid; k <= T; k += REF_BLOCK)
-        {
-            int gate = x0.gate, laps = x0.laps;
+        // C: every state slot k takes the gate and laps after the crossings
+        // before it (slot t + 1 by the lane of step t, slot 0 by lane 0); the
+        // winner test on the running cost of every x_{t+1}, the first terminal
+        // step, and the discounted cost for J; the last lane makes the terminal
+        // cost of x_T and its dJ/dp meanwhile (the trajectory that runs the
+        // whole horizon, the usual case)
+        const auto gate_laps = [&](int k, int& gate, int& laps) {
+            gate = x0.gate;
+            laps = x0.laps;
             for (int i = 0; i < s_nx && s_xstep[i] < k; ++i)
             {
                 gate = s_xgate[i];
                 laps = s_xlaps[i];
             }
-            s_x[k * 4 + 3].y = u2f((uint32_t)gate);
-            s_x[k * 4 + 3].z = u2f((uint32_t)laps);
+        };
+        if (tid == 0)
+        {
+            s_x[3].y = u2f((uint32_t)x0.gate);
+            s_x[3].z = u2f((uint32_t)x0.laps);
         }
-        __syncthreads();
-        // C: the winner test on the running cost of every x_{t+1} (laps now
-        // known), the first terminal step, and the discounted cost for J; the
-        // last lane makes the terminal cost of x_T and its dJ/dp meanwhile (the
-        // trajectory that runs the whole horizon, the usual case)
         for (int t = tid; t < T; t += REF_BLOCK)
         {
+            int gate, laps;
+            gate_laps(t + 1, gate, laps);
+            s_x[(t + 1) * 4 + 3].y = u2f((uint32_t)gate);
+            s_x[(t + 1) * 4 + 3].z = u2f((uint32_t)laps);
             int term = s_tb[t];
-            const float c = stage_cost_win(KC, s_track, (int)f2u(s_x[(t + 1) * 4 + 3].z), s_cb[t], &term);
+            const float c = stage_cost_win(KC, s_track, laps, s_cb[t], &term);
             if (term != TERM_NONE)
                 atomicMin(&s_tend, t);
             s_gp[t].w = c * s_w[t]; // discounted running cost
         }
         if (tid == REF_BLOCK - 1)
         {
+            // x_T's gate and laps made here (lane T - 1 writes its slot in this phase)
             float w;
-            const QState s = get_state(s_x, T, 1, &w);
+            QState s = get_state(s_x, T, 1, &w);
+            gate_laps(T, s.gate, s.laps);
             s_fin = final_cost(KC, s_track, s);
             if (it < n_steps)
             {

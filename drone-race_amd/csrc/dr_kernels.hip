@@ -1179,7 +1179,8 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
 //  C  one lane per step t: the gate and laps of x_{t+1} (after the crossings
 //     before it) into its state slot, the winner test on x_{t+1}, the
 //     discounted running cost; the first terminal step t_end ends the
-//     trajectory (cost_grad_lane's break); wave 0 sums J.
+//     trajectory (cost_grad_lane's break); the last wave sums J while
+//     waves 0-2 make the segment maps of D.
 //  D  the reverse recurrence lam_t = A_t^T lam_{t+1} + g_t is linear in lam.
 //     The steps 0..t_end are cut into n_seg <= REF_SEGS reverse segments
 //     [a, b); per segment, 14 lanes carry lam_b = 0 with the g_t terms
@@ -1197,7 +1198,7 @@ __global__ void k_costgrad(const InstDev* __restrict__ insts, int inst, const dr
 // differently from the one-lane sweep, so J and the gradient agree with
 // cost_grad_lane to rounding, not bit for bit.
 constexpr int REF_BLOCK = 256;
-constexpr int REF_SEGS = 16; // 14 * REF_SEGS <= REF_BLOCK
+constexpr int REF_SEGS = 13; // 14 * REF_SEGS <= REF_BLOCK - WAVE: the last wave is free for J during D
 constexpr int REF_COMP = 14; // affine part + 13 state directions
 constexpr int REF_JAC4 = (QJAC + 3) / 4; // float4 slots per step of the Jacobian table
 
@@ -1737,16 +1738,18 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
         const int te = s_tend;
         const int R = (te + REF_SEGS) / REF_SEGS; // ceil((te + 1) / REF_SEGS)
         const int n_seg = (te + R) / R;           // ceil((te + 1) / R)
-        if (tid < 64)
+        // J of this sequence in the last wave (running costs up to te summed in
+        // FP64, then the terminal cost; the plain MPPI update, pass 0, is the
+        // default, a gradient step is kept only if its J is lower) and the
+        // terminal adjoint, while waves 0-2 make the segment maps (D)
+        if (tid >= REF_BLOCK - WAVE)
         {
-            // J of this sequence (running costs up to te summed in FP64, then
-            // the terminal cost); the plain MPPI update (pass 0) is the
-            // default, a gradient step is kept only if its J is lower
+            const int jl = tid - (REF_BLOCK - WAVE); // lane of the last wave
             double acc = 0.0;
-            for (int t = tid; t <= te; t += 64)
+            for (int t = jl; t <= te; t += 64)
                 acc += (double)s_gp[t].w;
             acc = wave_sum(acc);
-            if (tid == 0)
+            if (jl == 0)
             {
                 float w;
                 const float fin = te == T - 1 ? s_fin : final_cost(KC, s_track, get_state(s_x, te + 1, 1, &w));
@@ -1756,37 +1759,31 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                     s_jbest = J;
                 s_better = better;
             }
-        }
-        __syncthreads();
-        if (s_better)
-            for (int t = tid; t < T; t += REF_BLOCK)
-                s_ubest[t] = s_u[t];
-        if (it == n_steps)
-            break;
-        if (tid == REF_BLOCK - 1)
-        {
-            // terminal adjoint: the final cost's dJ/dp at x_{te+1} (made in C
-            // when te = T - 1) plus the running cost's there (the gradient lane
-            // of step te made it from the same position, opponents and weight)
-            float g[3] = {0.0f, 0.0f, 0.0f};
-            if (te == T - 1)
-                for (int i = 0; i < 3; ++i)
-                    g[i] = s_fgrad[i];
-            else
+            if (tid == REF_BLOCK - 1 && it < n_steps)
             {
-                float w;
-                final_cost_grad(KC, s_track, get_state(s_x, te + 1, 1, &w), g);
+                // terminal adjoint: the final cost's dJ/dp at x_{te+1} (made in C
+                // when te = T - 1) plus the running cost's there (the gradient lane
+                // of step te made it from the same position, opponents and weight)
+                float g[3] = {0.0f, 0.0f, 0.0f};
+                if (te == T - 1)
+                    for (int i = 0; i < 3; ++i)
+                        g[i] = s_fgrad[i];
+                else
+                {
+                    float w;
+                    final_cost_grad(KC, s_track, get_state(s_x, te + 1, 1, &w), g);
+                }
+                const float4 gp = s_gp[te];
+                float* lam = s_lam + (n_seg - 1) * 16;
+                lam[0] = g[0] + gp.x;
+                lam[1] = g[1] + gp.y;
+                lam[2] = g[2] + gp.z;
+                for (int i = 3; i < 13; ++i)
+                    lam[i] = 0.0f;
             }
-            const float4 gp = s_gp[te];
-            float* lam = s_lam + (n_seg - 1) * 16;
-            lam[0] = g[0] + gp.x;
-            lam[1] = g[1] + gp.y;
-            lam[2] = g[2] + gp.z;
-            for (int i = 3; i < 13; ++i)
-                lam[i] = 0.0f;
         }
         // D: per reverse segment, the affine map lam_a = Psi lam_b + psi
-        if (tid < REF_COMP * n_seg)
+        else if (it < n_steps && tid < REF_COMP * n_seg)
         {
             const int seg = tid / REF_COMP, comp = tid - seg * REF_COMP;
             const int a = seg * R, b = min(a + R, te + 1);
@@ -1834,6 +1831,11 @@ __global__ __launch_bounds__(REF_BLOCK) void k_refine(const InstDev* __restrict_
                 out[i] = f[i];
         }
         __syncthreads();
+        if (s_better)
+            for (int t = tid; t < T; t += REF_BLOCK)
+                s_ubest[t] = s_u[t];
+        if (it == n_steps)
+            break;
         // E: lam_b of every segment, last to first (wave 0; lane i = component i)
         if (tid < 64)
         {
